@@ -1,0 +1,391 @@
+/*
+ * veris_hip.h -- C ABI of the MI355X (gfx950) sea-ice step.
+ *
+ * Each entry point replaces one kernel of the reference plugin (cited per
+ * declaration as file:line of team-ocean/veris).  The reference kernels are
+ * pure functions of a Veros `state`; here the state is passed as plain
+ * structs of device pointers (vh_fields) and scalars (vh_settings) and the
+ * results are written into caller-owned device buffers.
+ *
+ * Conventions for every entry point:
+ *  - every field is a device array of shape (nx+2*olx, ny+2*oly) in C order
+ *    (axis 0 = x / i, axis 1 = y / j contiguous) with the element type
+ *    vh_domain.dtype (VH_F64 or VH_F32);
+ *  - neighbour access wraps over the padded array exactly as numpy.roll does in
+ *    the reference, so halo cells hold the same values the reference computes;
+ *  - outputs must not alias inputs; calls are stream-ordered and asynchronous
+ *    on `stream` (a hipStream_t, NULL = legacy default stream);
+ *  - no entry point allocates device memory: scratch comes from the caller's
+ *    `workspace` (size from the matching *_workspace_bytes call);
+ *  - the return value is VH_OK (0) or a negative VH_E* code, never an
+ *    exception; vh_last_error() gives the message of the calling thread's
+ *    last failure.
+ */
+#ifndef VERIS_HIP_H
+#define VERIS_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VH_ABI_VERSION 1
+
+enum {
+  VH_OK = 0,
+  VH_EINVAL = -1,      /* null pointer, aliasing or bad argument value */
+  VH_ESHAPE = -2,      /* unsupported nx/ny/olx/oly */
+  VH_EDTYPE = -3,      /* unsupported dtype for this entry point */
+  VH_EUNSUPPORTED = -4,/* setting combination not implemented */
+  VH_EHIP = -5,        /* HIP runtime error (launch, copy) */
+  VH_EWORKSPACE = -6   /* workspace missing or too small */
+};
+
+enum { VH_F64 = 0, VH_F32 = 1 };
+
+/* Domain geometry.  fill_mode 0 reproduces Veros' single-process
+ * enforce_boundaries (x-periodic copy when cyclic_x, nothing otherwise;
+ * veris/fill_overlap.py:10-11); fill_mode 1 is the plugin's own doubly
+ * periodic copy (veris/fill_overlap.py:12-17, setting veros_fill=False). */
+typedef struct vh_domain {
+  int32_t nx, ny;   /* interior cells */
+  int32_t olx, oly; /* overlap widths; must both be 2 */
+  int32_t cyclic_x; /* host grid is x-cyclic (Veros enable_cyclic_x) */
+  int32_t fill_mode;
+  int32_t dtype;    /* VH_F64 | VH_F32 */
+  int32_t tile;     /* EVP tile shape index (1..vh_evp_tile_count()), 0 = library default */
+} vh_domain;
+
+/* Every scalar setting of veris/settings.py:4-176, by name and in order, as
+ * double (ints and flags converted). */
+typedef struct vh_settings {
+  double deltatTherm;
+  double recip_deltatTherm;
+  double deltatDyn;
+  double recip_deltatDyn;
+  double olx;
+  double oly;
+  double gridcellWidth;
+  double nITC;
+  double recip_nITC;
+  double noSlip;
+  double useRelativeWind;
+  double secondOrderBC;
+  double extensiveFld;
+  double useRealFreshWaterFlux;
+  double useFreedrift;
+  double useEVP;
+  double evpAlpha;
+  double evpBeta;
+  double useAdaptiveEVP;
+  double aEVPalphaMin;
+  double aEvpCoeff;
+  double explicitDrag;
+  double nEVPsteps;
+  double computeEvpResidual;
+  double veros_fill;
+  double use_coastline;
+  double rhoIce;
+  double rhoFresh;
+  double rhoSea;
+  double rhoAir;
+  double rhoSnow;
+  double recip_rhoFresh;
+  double recip_rhoSea;
+  double rhoIce2rhoSnow;
+  double rhoIce2rhoFresh;
+  double rhoFresh2rhoSnow;
+  double dryIceAlb;
+  double dryIceAlb_south;
+  double wetIceAlb;
+  double wetIceAlb_south;
+  double drySnowAlb;
+  double drySnowAlb_south;
+  double wetSnowAlb;
+  double wetSnowAlb_south;
+  double wetAlbTemp;
+  double lhFusion;
+  double lhEvap;
+  double lhSublim;
+  double cpAir;
+  double cpWater;
+  double stefBoltz;
+  double iceEmiss;
+  double snowEmiss;
+  double iceConduct;
+  double snowConduct;
+  double hCut;
+  double shortwave;
+  double tempFrz;
+  double dtempFrz_dS;
+  double saltIce_ref;
+  double saltOcn_ref;
+  double minLWdown;
+  double maxTIce;
+  double minTIce;
+  double minTAir;
+  double dalton;
+  double Area_reg;
+  double hIce_reg;
+  double celsius2K;
+  double stantonNr;
+  double uStarBase;
+  double McPheeTaperFac;
+  double h0;
+  double recip_h0;
+  double h0_south;
+  double recip_h0_south;
+  double airTurnAngle;
+  double waterTurnAngle;
+  double sinWat;
+  double cosWat;
+  double wSpeedMin;
+  double hIce_min;
+  double Area_min;
+  double airIceDrag;
+  double airIceDrag_south;
+  double waterIceDrag;
+  double waterIceDrag_south;
+  double cDragMin;
+  double seaIceLoadFac;
+  double gravity;
+  double PlasDefCoeff;
+  double deltaMin;
+  double pressReplFac;
+  double pStar;
+  double cStar;
+  double basalDragU0;
+  double basalDragK1;
+  double basalDragK2;
+  double cBasalStar;
+  double tensileStrFac;
+  double CrMax;
+  double sideDragCoeff;
+  double sideDragU0;
+  double umin_o;
+  double umin_i;
+  double zref;
+  double ztref;
+  double bolzc;
+  double avogad;
+  double rgas;
+  double mwdair;
+  double mwwv;
+  double rdair;
+  double rwv;
+  double zvir;
+  double cpdair;
+  double cpwv;
+  double cpvir;
+  double karman;
+  double latvap;
+  double p0;
+  double cappa;
+  double zzsice;
+  double ch;
+  double ce;
+  double eps2;
+  double emissivity;
+  double ocean_emissivity;
+  double snow_emissivity;
+  double ice_emissivity;
+  double tf0kel;
+  double gamma_blk;
+  double ocean_albedo;
+  double ice_albedo;
+} vh_settings;
+
+/* One device pointer per variable of veris/variables.py:5-106, by name and in
+ * order.  Entry points read only the fields they need; a NULL k1AtC, k2AtC,
+ * k1AtZ or k2AtZ is read as an all-zero field. */
+typedef struct vh_fields {
+  const void* hIceMean;
+  const void* hSnowMean;
+  const void* Area;
+  const void* TSurf;
+  const void* SeaIceMassC;
+  const void* SeaIceMassU;
+  const void* SeaIceMassV;
+  const void* SeaIceStrength;
+  const void* os_hIceMean;
+  const void* os_hSnowMean;
+  const void* AreaW;
+  const void* AreaS;
+  const void* uIce;
+  const void* vIce;
+  const void* sigma1;
+  const void* sigma2;
+  const void* sigma12;
+  const void* WindForcingX;
+  const void* WindForcingY;
+  const void* recip_hIceMean;
+  const void* SeaIceLoad;
+  const void* IcePenetSW;
+  const void* uOcean;
+  const void* vOcean;
+  const void* theta;
+  const void* ocSalt;
+  const void* Qnet;
+  const void* OceanStressU;
+  const void* OceanStressV;
+  const void* saltflux;
+  const void* R_low;
+  const void* ssh_an;
+  const void* Qsw;
+  const void* uWind;
+  const void* vWind;
+  const void* wSpeed;
+  const void* surfPress;
+  const void* SWdown;
+  const void* LWdown;
+  const void* ATemp;
+  const void* aqh;
+  const void* precip;
+  const void* snowfall;
+  const void* evap;
+  const void* runoff;
+  const void* EmPmR;
+  const void* maskInC;
+  const void* maskInU;
+  const void* maskInV;
+  const void* iceMask;
+  const void* iceMaskU;
+  const void* iceMaskV;
+  const void* k1AtC;
+  const void* k2AtC;
+  const void* k1AtZ;
+  const void* k2AtZ;
+  const void* Fu;
+  const void* Fv;
+  const void* fCori;
+  const void* dxC;
+  const void* dyC;
+  const void* dxG;
+  const void* dyG;
+  const void* dxU;
+  const void* dyU;
+  const void* dxV;
+  const void* dyV;
+  const void* recip_dxC;
+  const void* recip_dyC;
+  const void* recip_dxG;
+  const void* recip_dyG;
+  const void* recip_dxU;
+  const void* recip_dyU;
+  const void* recip_dxV;
+  const void* recip_dyV;
+  const void* rA;
+  const void* rAu;
+  const void* rAv;
+  const void* rAz;
+  const void* recip_rA;
+  const void* recip_rAu;
+  const void* recip_rAv;
+  const void* recip_rAz;
+} vh_fields;
+
+/* Outputs of vh_growth, the 11-tuple of veris/growth.py:385-397. */
+typedef struct vh_growth_out {
+  void* hIceMean;
+  void* hSnowMean;
+  void* Area;
+  void* TSurf;
+  void* EmPmR;
+  void* forc_salt_surface;
+  void* Qsw;
+  void* Qnet;
+  void* SeaIceLoad;
+  void* IcePenetSW;
+  void* recip_hIceMean;
+} vh_growth_out;
+
+/* ABI version (VH_ABI_VERSION) of the loaded library. */
+int vh_version(void);
+
+/* Copies the calling thread's last error message into buf (NUL-terminated);
+ * returns its full length. */
+int vh_last_error(char* buf, size_t len);
+
+/* SeaIceMass (veris/area_mass.py:17-28): massC = rhoIce*h + rhoSnow*hs and its
+ * two-point averages to u- and v-points. */
+int vh_seaice_mass(const vh_domain* d, const vh_settings* s, const vh_fields* f,
+                   void* SeaIceMassC, void* SeaIceMassU, void* SeaIceMassV, void* stream);
+
+/* AreaWS (veris/area_mass.py:5-14). */
+int vh_area_ws(const vh_domain* d, const vh_settings* s, const vh_fields* f,
+               void* AreaW, void* AreaS, void* stream);
+
+/* SeaIceStrength (veris/dynamics_routines.py:5-18).  pStarField may be NULL
+ * (scalar setting pStar) or a per-cell P* field of the domain's dtype. */
+int vh_seaice_strength(const vh_domain* d, const vh_settings* s, const vh_fields* f,
+                       const void* pStarField, void* SeaIceStrength, void* stream);
+
+/* WindForcingXY with tauXY (veris/dynsolver.py:7-84). */
+int vh_wind_forcing_xy(const vh_domain* d, const vh_settings* s, const vh_fields* f,
+                       void* WindForcingX, void* WindForcingY, void* stream);
+
+/* Number of EVP tile shapes (valid vh_domain.tile values are 1..count). */
+int vh_evp_tile_count(void);
+
+/* Workgroup shape (ri x rj threads; each owns a (ri-4) x (rj-4) tile plus a
+ * two-cell ring) of EVP tile index `tile`. */
+int vh_evp_tile_shape(int32_t tile, int32_t* ri, int32_t* rj);
+
+/* Workspace bytes vh_evp_solver needs for this domain. */
+int vh_evp_workspace_bytes(const vh_domain* d, size_t* bytes);
+
+/* evp_solver / IceVelocities (veris/evp_solver.py:257-319, body :22-254,
+ * dispatch veris/dynsolver.py:87-102): n_substeps fused EVP substeps starting
+ * from f->uIce, f->vIce, f->sigma1, f->sigma2, f->sigma12 (f->uIce, f->vIce are
+ * also the previous-step velocities uIceNm1, vIceNm1); the final state is
+ * written to the five outputs.  Non-adaptive EVP with noSlip only. */
+int vh_evp_solver(const vh_domain* d, const vh_settings* s, const vh_fields* f,
+                  int32_t n_substeps, void* uIce, void* vIce, void* sigma1, void* sigma2,
+                  void* sigma12, void* workspace, size_t workspace_bytes, void* stream);
+
+/* OceanStressUV (veris/ocean_stress.py:8-40), overlap-filled. */
+int vh_ocean_stress_uv(const vh_domain* d, const vh_settings* s, const vh_fields* f,
+                       void* OceanStressU, void* OceanStressV, void* stream);
+
+/* Workspace bytes vh_advection needs for this domain. */
+int vh_advection_workspace_bytes(const vh_domain* d, size_t* bytes);
+
+/* Advection (veris/advection.py:15-91, fluxes :94-175) of hIceMean, hSnowMean
+ * and Area with the superbee-limited, dimensionally split scheme. */
+int vh_advection(const vh_domain* d, const vh_settings* s, const vh_fields* f,
+                 void* hIceMean, void* hSnowMean, void* Area, void* workspace,
+                 size_t workspace_bytes, void* stream);
+
+/* clean_up_advection (veris/clean_up.py:5-39). */
+int vh_clean_up_advection(const vh_domain* d, const vh_settings* s, const vh_fields* f,
+                          void* hIceMean, void* hSnowMean, void* Area, void* TSurf,
+                          void* os_hIceMean, void* os_hSnowMean, void* stream);
+
+/* ridging (veris/clean_up.py:42-47): Area = min(f->Area, 1). */
+int vh_ridging(const vh_domain* d, const vh_settings* s, const vh_fields* f, void* Area,
+               void* stream);
+
+/* Growth with solve4temp (veris/growth.py:8-397, veris/solve4temp.py:5-174). */
+int vh_growth(const vh_domain* d, const vh_settings* s, const vh_fields* f,
+              const vh_growth_out* out, void* stream);
+
+/* fill_overlap (veris/fill_overlap.py:6-18) of one field, in place. */
+int vh_fill_overlap(const vh_domain* d, void* field, void* stream);
+
+/* Workspace bytes vh_global_sum_f64 needs for this domain. */
+int vh_global_sum_workspace_bytes(const vh_domain* d, size_t* bytes);
+
+/* global_sum (veris/global_sum.py:1-4, a stub in the reference): FP64 sum over
+ * the interior cells of field (times weight if not NULL), accumulated in FP64
+ * whatever the domain dtype; the result is written to *result (device
+ * memory, one double). */
+int vh_global_sum_f64(const vh_domain* d, const void* field, const void* weight,
+                      double* result, void* workspace, size_t workspace_bytes, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* VERIS_HIP_H */

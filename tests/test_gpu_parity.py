@@ -1,0 +1,257 @@
+"""GPU parity: each HIP drop-in against the NumPy FP64 oracle on the same
+seeded inputs.
+
+Bar: the FP64 kernels that use only +, -, *, /, sqrt and comparisons (mass,
+area, wind forcing, the EVP subcycle, ocean stress, advection, clean-up,
+ridging, fill) are compared BIT FOR BIT on every padded cell, halos included.
+Kernels that call exp/pow/log (strength, basal drag, growth) are compared with
+the relative tolerance stated in each test (max|diff| / max|oracle|).
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+
+from gpu_helpers import assert_close, assert_exact, g2n, pre_evp
+from oracle import veris_np as O
+from veris_amd import kernels as K
+from veris_amd import synth
+from veris_amd.step import seaice_step
+
+pytestmark = pytest.mark.gpu
+
+EVP_NAMES = ("uIce", "vIce", "sigma1", "sigma2", "sigma12")
+
+
+def _evp_pair(st_np, n, dev, **kw):
+    want = O.evp_solver(st_np, n_substeps=n, **kw)
+    got = K.evp_solver(st_np.to(dev), n_substeps=n)
+    return [g2n(t) for t in got], want
+
+
+def _spun_up(config_state, dev, n_spin=25):
+    """State with non-zero velocities and stresses (oracle EVP spin-up)."""
+    st = pre_evp(config_state)
+    vs = st.variables
+    vs.uIce, vs.vIce, vs.sigma1, vs.sigma2, vs.sigma12 = O.evp_solver(st, n_substeps=n_spin)
+    return pre_evp(st)
+
+
+def test_pre_evp_kernels(cuda):
+    st = _spun_up(synth.config0(), cuda)
+    g = st.to(cuda)
+    for name, got, want in zip(("SeaIceMassC", "SeaIceMassU", "SeaIceMassV"), K.SeaIceMass(g),
+                               O.SeaIceMass(st)):
+        assert_exact(name, g2n(got), want)
+    for name, got, want in zip(("AreaW", "AreaS"), K.AreaWS(g), O.AreaWS(st)):
+        assert_exact(name, g2n(got), want)
+    for name, got, want in zip(("WindForcingX", "WindForcingY"), K.WindForcingXY(g),
+                               O.WindForcingXY(st)):
+        assert_exact(name, g2n(got), want)
+    # exp() of the device math library: a few ulp
+    assert_close("SeaIceStrength", g2n(K.SeaIceStrength(g)), O.SeaIceStrength(st), 1e-15)
+    pstar = np.random.default_rng(5).lognormal(np.log(27500.0), 0.3, st.shape)
+    assert_close("SeaIceStrength(P* field)",
+                 g2n(K.SeaIceStrength(g, torch.from_numpy(pstar).to(cuda))),
+                 O.SeaIceStrength(st, pStar=pstar), 1e-15)
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 40])
+def test_evp_substeps_bit_exact(cuda, n):
+    st = _spun_up(synth.config0(), cuda)
+    got, want = _evp_pair(st, n, cuda)
+    for name, a, b in zip(EVP_NAMES, got, want):
+        assert_exact(f"{name}[n={n}]", a, b)
+
+
+def test_evp_full_subcycle_config0(cuda):
+    """400 substeps from rest, the 4-degree workload."""
+    st = pre_evp(synth.config0())
+    got, want = _evp_pair(st, 400, cuda)
+    for name, a, b in zip(EVP_NAMES, got, want):
+        assert_exact(f"{name}[400]", a, b)
+
+
+def test_evp_config1_1deg(cuda):
+    st = _spun_up(synth.config1(), cuda, n_spin=3)
+    got, want = _evp_pair(st, 12, cuda)
+    for name, a, b in zip(EVP_NAMES, got, want):
+        assert_exact(f"{name}[1deg]", a, b)
+
+
+@pytest.mark.parametrize("variant", ["regional", "own_fill", "turning", "kterms", "tensile",
+                                     "no_repl_pressure", "small_domain"])
+def test_evp_setting_variants(cuda, variant):
+    if variant == "regional":
+        st = synth.regional(7, 0, 37, 29)
+    elif variant == "own_fill":
+        st = synth.tiny(20, 14, veros_fill=False)
+    elif variant == "turning":
+        st = synth.tiny(20, 14, sinWat=0.42261826174069944, cosWat=0.9063077870366499)
+    elif variant == "tensile":
+        st = synth.tiny(20, 14, tensileStrFac=0.05)
+    elif variant == "no_repl_pressure":
+        st = synth.tiny(20, 14, pressReplFac=0)
+    elif variant == "small_domain":
+        st = synth.tiny(4, 4)
+    else:
+        st = synth.tiny(20, 14)
+        r = np.random.default_rng(3)
+        for k in ("k1AtC", "k2AtC", "k1AtZ", "k2AtZ"):
+            st.variables[k] = r.normal(0, 1e-7, st.shape)
+    st = _spun_up(st, cuda, n_spin=10)
+    got, want = _evp_pair(st, 30, cuda)
+    for name, a, b in zip(EVP_NAMES, got, want):
+        assert_exact(f"{name}[{variant}]", a, b)
+
+
+def test_evp_basal_drag(cuda):
+    """basalDragK2 != 0 switches on the exp/log soft-max: tolerance 1e-12."""
+    st = synth.tiny(20, 14, basalDragK2=15.0)
+    st.variables.R_low = st.variables.R_low * 0.002  # shallow: keels ground
+    st = _spun_up(st, cuda, n_spin=10)
+    got, want = _evp_pair(st, 30, cuda)
+    for name, a, b in zip(EVP_NAMES, got, want):
+        assert_close(f"{name}[basal]", a, b, 1e-12)
+
+
+def test_evp_zero_substeps_and_land(cuda):
+    st = _spun_up(synth.tiny(12, 10), cuda, n_spin=5)
+    got, want = _evp_pair(st, 0, cuda)
+    for name, a, b in zip(EVP_NAMES, got, want):
+        assert_exact(f"{name}[n=0]", a, b)
+    # all land: masks zero everywhere, denominators hit the ==0 guard
+    vs = st.variables
+    for k in ("iceMask", "iceMaskU", "iceMaskV", "maskInC", "maskInU", "maskInV"):
+        vs[k] = np.zeros(st.shape)
+    st = pre_evp(st)
+    got, want = _evp_pair(st, 5, cuda)
+    for name, a, b in zip(EVP_NAMES, got, want):
+        assert_exact(f"{name}[land]", a, b)
+
+
+def test_ocean_stress(cuda):
+    for angle in (0.0, 25.0):
+        st = _spun_up(synth.tiny(24, 16, waterTurnAngle=angle), cuda, n_spin=20)
+        g = st.to(cuda)
+        want = O.OceanStressUV(st)
+        got = K.OceanStressUV(g)
+        for name, a, b in zip(("OceanStressU", "OceanStressV"), got, want):
+            if angle == 0.0:
+                assert_exact(f"{name}[{angle}]", g2n(a), b)
+            else:  # sin/cos of the host libm vs numpy: 1 ulp in the turning factors
+                assert_close(f"{name}[{angle}]", g2n(a), b, 1e-14)
+
+
+def _after_evp(st):
+    vs = st.variables
+    vs.uIce, vs.vIce, vs.sigma1, vs.sigma2, vs.sigma12 = O.evp_solver(st, n_substeps=60)
+    return st
+
+
+def test_advection(cuda):
+    for st in (synth.config0(), synth.regional(8, 1, 40, 33), synth.tiny(16, 12, veros_fill=False)):
+        st = _after_evp(pre_evp(st))
+        want = O.Advection(st)
+        got = K.Advection(st.to(cuda))
+        for name, a, b in zip(("hIceMean", "hSnowMean", "Area"), got, want):
+            assert_exact(f"adv {name}", g2n(a), b)
+
+
+def test_clean_up_and_ridging(cuda):
+    st = synth.config0()
+    vs = st.variables
+    r = np.random.default_rng(11)
+    vs.hIceMean = vs.hIceMean + r.normal(0, 0.05, st.shape)
+    vs.hSnowMean = vs.hSnowMean + r.normal(0, 0.05, st.shape)
+    vs.Area = vs.Area * 1.3 - 0.1
+    vs.hIceMean[5:9, 5:9] = 5e-6
+    g = st.to(cuda)
+    names = ("hIceMean", "hSnowMean", "Area", "TSurf", "os_hIceMean", "os_hSnowMean")
+    for name, a, b in zip(names, K.clean_up_advection(g), O.clean_up_advection(st)):
+        assert_exact(f"clean_up {name}", g2n(a), b)
+    assert_exact("ridging", g2n(K.ridging(g)), O.ridging(st))
+
+
+def test_growth(cuda):
+    """pow/exp of the device math library: tolerance 1e-12 relative to the
+    largest value of each output."""
+    st = synth.config0()
+    vs = st.variables
+    r = np.random.default_rng(2)
+    vs.TSurf = 250.0 + 25.0 * r.random(st.shape)
+    vs.snowfall = np.abs(r.normal(0, 1e-8, st.shape))
+    vs.precip = np.abs(r.normal(0, 1e-8, st.shape))
+    vs.evap = r.normal(0, 1e-8, st.shape)
+    vs.os_hIceMean = np.abs(r.normal(0, 1e-3, st.shape))
+    g = st.to(cuda)
+    want = O.Growth(st)
+    got = K.Growth(g)
+    names = ("hIceMean", "hSnowMean", "Area", "TSurf", "EmPmR", "forc_salt_surface", "Qsw",
+             "Qnet", "SeaIceLoad", "IcePenetSW", "recip_hIceMean")
+    for name, a, b in zip(names, got, want):
+        assert_close(f"growth {name}", g2n(a), b, 1e-12)
+
+
+def test_fill_and_global_sum(cuda):
+    st = synth.config0()
+    a = np.random.default_rng(4).normal(size=st.shape)
+    g = st.to(cuda)
+    for fill_mode in (True, False):
+        st.settings.veros_fill = fill_mode
+        g.settings.veros_fill = fill_mode
+        assert_exact(f"fill[{fill_mode}]", g2n(K.fill_overlap(g, torch.from_numpy(a).to(cuda))),
+                     O.fill_overlap(st, a))
+    w = st.variables.rA
+    s = K.global_sum(g, torch.from_numpy(a).to(cuda), g.variables.rA).item()
+    ref = O.global_sum(a[2:-2, 2:-2] * w[2:-2, 2:-2])
+    assert abs(s - ref) <= 1e-13 * O.global_sum(np.abs(a[2:-2, 2:-2] * w[2:-2, 2:-2]))
+
+
+@pytest.mark.parametrize("n_steps", [1, 2])
+def test_full_step_config0(cuda, n_steps):
+    """Whole step (pre-EVP, 400 EVP substeps, ocean stress, advection, clean-up,
+    ridging, growth, fills).  exp/pow in strength and growth make the GPU
+    differ from the oracle by ulps that the next step carries on; tolerance
+    1e-9 relative to each field's maximum."""
+    st = synth.config0()
+    q0 = st.variables.Qnet.copy()
+    sw = st.variables.SWdown.copy()
+    g = st.to(cuda)
+    qg = torch.from_numpy(q0).to(cuda)
+    swg = torch.from_numpy(sw).to(cuda)
+    for _ in range(n_steps):
+        O.seaice_step(st, q0, sw)
+        seaice_step(g, qg, swg)
+    for name in ("uIce", "vIce", "sigma1", "sigma2", "sigma12", "OceanStressU", "OceanStressV",
+                 "hIceMean", "hSnowMean", "Area", "TSurf", "Qnet", "Qsw", "EmPmR",
+                 "forc_salt_surface_ice", "SeaIceLoad", "IcePenetSW", "recip_hIceMean"):
+        assert_close(f"step{n_steps} {name}", g2n(g.variables[name]), st.variables[name], 1e-9)
+
+
+def test_errors_are_reported(cuda):
+    from veris_amd._native import VerisHipError
+
+    st = pre_evp(synth.tiny(12, 10)).to(cuda)
+    with pytest.raises(VerisHipError, match="useAdaptiveEVP"):
+        st.settings.useAdaptiveEVP = True
+        K.evp_solver(st, 3)
+    st.settings.useAdaptiveEVP = False
+    st.variables.uIce = st.variables.uIce[:, :-1].contiguous()
+    with pytest.raises(ValueError):
+        K.evp_solver(st, 3)
+
+
+@pytest.mark.parametrize("tile", [1, 2, 3, 4, 5, 6, 7])
+def test_evp_every_tile_shape_bit_exact(cuda, tile):
+    """Every workgroup shape gives the same bits (direct launches for n < 8,
+    graph replay for n >= 8)."""
+    st = _spun_up(synth.config0(), cuda, n_spin=5)
+    g = st.to(cuda)
+    for n in (3, 17):
+        want = O.evp_solver(st, n_substeps=n)
+        got = K.evp_solver(g, n_substeps=n, tile=tile)
+        for name, a, b in zip(EVP_NAMES, got, want):
+            assert_exact(f"{name}[tile={tile}, n={n}]", g2n(a), b)

@@ -1,0 +1,368 @@
+// Thermodynamic growth (veris/growth.py:8-397) with the surface energy
+// balance solve4temp (veris/solve4temp.py:5-174): pointwise, one thread per
+// padded cell, all nITC thickness categories and their Newton iterations in
+// registers.  Built with -ffp-contract=off; expressions keep the reference's
+// association order (exp/pow come from the device math library, so results
+// agree with an IEEE host evaluation to a few ulp, not bit for bit).
+#include <cmath>
+
+#include "vh_common.h"
+
+namespace vh {
+namespace {
+
+constexpr int BLK = 256;
+
+struct GrowthScalars {
+  double qi, qs;
+  double Area_reg, hIce_reg, recip_nITC, nITC_d;
+  int nITC;
+  double tempFrz, dtempFrz_dS, celsius2K;
+  double snowRate_fac;  // rhoFresh2rhoSnow
+  double dtT, rdtT;
+  double t0, t1, t2;    // mixed-layer constants 0.4, 7/0.4, stantonNr*uStarBase*rhoSea*cpWater
+  double McPheeTaperFac, recip_h0, recip_h0_south;
+  double rhoSnow, rhoIce, recip_rhoSea, rhoIce2rhoSnow, rhoIce2rhoFresh, rhoFresh,
+      rhoFresh2rhoSnow, saltIce_ref;
+  // solve4temp
+  double d1, d1i, Tmelt, Tmeltw, minLWdown, minTAir_K, wSpeedMin, snowEmiss, iceEmiss, stefBoltz;
+  double dryIceAlb, drySnowAlb, wetIceAlb, wetSnowAlb, dryIceAlb_south, drySnowAlb_south,
+      wetIceAlb_south, wetSnowAlb_south, hCut, shortwave, kIS /*iceConduct*snowConduct*/,
+      snowConduct, iceConduct, minTIce_K, lhSublim;
+  double aa1, aa2, bb1, bb1m /*1-bb1*/, ppa, cc1, cc2;
+};
+
+template <typename T>
+struct GrowthIO {
+  const T *h, *hs, *A, *Ts, *ocSalt, *theta, *snowfall, *precip, *Qnet, *Qsw, *evap, *runoff,
+      *osh, *oshs, *M, *fC, *LWdown, *ATemp, *wSpeed, *aqh, *SWdown;
+  T *oh, *ohs, *oA, *oTs, *oEmPmR, *ofss, *oQsw, *oQnet, *oload, *open, *orh;
+};
+
+template <typename T>
+struct Fl {
+  T Fc, Flh, Fia, dF;
+};
+
+template <typename T>
+__device__ inline Fl<T> fluxes(const GrowthScalars& k, T t1, bool ice, T kEff, T Tfrz, T ug,
+                               T aqh, T d3, T lw, T absSW, T ta) {
+  Fl<T> r;
+  const T t2 = t1 * t1;
+  const T t3 = t2 * t1;
+  const T t4 = t2 * t2;
+  const T svp = pow((T)10, (T)(-k.aa1) / t1 + (T)k.aa2);
+  const T qsurf = ice ? ((T)k.bb1 * svp) / ((T)k.ppa - (T)k.bb1m * svp) : (T)0;
+  const T c3 = pow((T)10, (T)k.aa1 / t1);
+  const T cd = (T)k.cc2 - c3 * (T)k.ppa;
+  const T dq = ice ? ((T)k.cc1 * c3) / ((cd * cd) * t2) : (T)0;
+  r.Fc = ice ? kEff * (Tfrz - t1) : (T)0;
+  r.Flh = ice ? ((T)k.d1i * ug) * (qsurf - aqh) : (T)0;
+  const T Flw = ice ? t4 * d3 : (T)0;
+  const T Fs = ice ? ((T)k.d1 * ug) * (t1 - ta) : (T)0;
+  r.Fia = ice ? ((((-lw) - absSW) + Flw) + Fs) + r.Flh : (T)0;
+  r.dF = ice ? (((T)4 * d3) * t3 + (T)k.d1 * ug) + ((T)k.d1i * ug) * dq : (T)0;
+  return r;
+}
+
+// returns T_out; writes F_io_net, F_ia_net, IcePenetSW of one category
+template <typename T>
+__device__ inline T solve4temp(const GrowthScalars& k, T hI, T hS, T Tin, T Tfrz, T fc, T LWd,
+                               T ATemp, T wSpeed, T aqh, T SWd, T* Fio, T* Fia, T* penSW) {
+  T Tl = Tin;
+  const T lwc = fmax((T)k.minLWdown, LWd);
+  const T ta = fmax((T)k.minTAir_K, ATemp);
+  const T ug = fmax((T)k.wSpeedMin, wSpeed);
+  const bool ice = hI > (T)0;
+  const bool snow = hS > (T)0;
+  const T emis = snow ? (T)k.snowEmiss : (T)k.iceEmiss;
+  const T d3 = emis * (T)k.stefBoltz;
+  const T lw = emis * lwc;
+  T aI = ice ? (T)k.dryIceAlb : (T)0;
+  T aS = ice ? (T)k.drySnowAlb : (T)0;
+  const bool wet = (hI > (T)0) && (Tl >= (T)k.Tmeltw);
+  if (wet) {
+    aI = (T)k.wetIceAlb;
+    aS = (T)k.wetSnowAlb;
+  }
+  const bool south = (hI > (T)0) && (fc < (T)0);
+  if (south) {
+    aI = (T)k.dryIceAlb_south;
+    aS = (T)k.drySnowAlb_south;
+  }
+  if (south && (Tl >= (T)k.Tmeltw)) {
+    aI = (T)k.wetIceAlb_south;
+    aS = (T)k.wetSnowAlb_south;
+  }
+  T alb = ice ? aI + (hS / (T)k.hCut) * (aS - aI) : (T)0;
+  if (hS > (T)k.hCut) alb = aS;
+  if (hS == (T)0) alb = aI;
+  T pen = ice ? (T)k.shortwave * exp((T)(-1.5) * hI) : (T)0;
+  if (snow) pen = (T)0;
+  *penSW = ice ? ((-((T)1 - alb)) * pen) * SWd : (T)0;
+  const T absSW = ice ? (((T)1 - alb) * ((T)1 - pen)) * SWd : (T)0;
+  const T kEff = ice ? (T)k.kIS / ((T)k.snowConduct * hI + (T)k.iceConduct * hS) : (T)0;
+  for (int it = 0; it < 6; ++it) {
+    const Fl<T> r = fluxes(k, Tl, ice, kEff, Tfrz, ug, aqh, d3, lw, absSW, ta);
+    Tl = ice ? Tl + (r.Fc - r.Fia) / (kEff + r.dF) : (T)0;
+    Tl = fmin(Tl, (T)k.Tmelt);
+    Tl = fmax(Tl, (T)k.minTIce_K);
+  }
+  const Fl<T> r = fluxes(k, Tl, ice, kEff, Tfrz, ug, aqh, d3, lw, absSW, ta);
+  const bool up = r.Fc > (T)0;
+  *Fio = up ? r.Fc : (T)0;
+  *Fia = up ? (T)0 : r.Fia;
+  return ice ? Tl : Tin;
+}
+
+template <typename T>
+__global__ __launch_bounds__(BLK) void growth_kernel(GrowthScalars k, GrowthIO<T> io, size_t n) {
+  const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  const T h0 = io.h[c], hs0 = io.hs[c], A0 = io.A[c];
+  const bool noIce0 = (h0 == (T)0) || (A0 == (T)0);
+  const T hP = h0 * (T)(!noIce0);
+  const T hsP = hs0 * (T)(!noIce0);
+  const T AP = A0 * (T)(!noIce0);
+  const bool isIce = hP > (T)0;
+  const T regA = sqrt(AP * AP + (T)k.Area_reg);
+  const T rregA = (T)1 / regA;
+  T hIA = isIce ? hP * rregA : (T)0;
+  const T rhIA = AP / sqrt(hP * hP + (T)k.hIce_reg);
+  const T hSA = isIce ? hsP * rregA : (T)0;
+  hIA = fmax(hIA, (T)0.05);
+  const T sal = io.ocSalt[c];
+  const T Tfrz = ((T)k.tempFrz + (T)k.dtempFrz_dS * sal) + (T)k.celsius2K;
+  const T Ts0 = io.Ts[c];
+  const T fc = io.fC[c];
+  const T LWd = io.LWdown[c], ATm = io.ATemp[c], wsp = io.wSpeed[c], aqh = io.aqh[c],
+          SWd = io.SWdown[c];
+  T Tsum = 0, Fio_s = 0, Fia_s = 0, pen_s = 0;
+  for (int l = 0; l < k.nITC; ++l) {
+    const T p = (T)(2 * (l + 1) - 1) * (T)k.recip_nITC;
+    T Fio, Fia, pen;
+    const T Tl = solve4temp(k, hIA * p, hSA * p, Ts0, Tfrz, fc, LWd, ATm, wsp, aqh, SWd, &Fio, &Fia,
+                            &pen);
+    if (l == 0) {
+      Tsum = Tl;
+      Fio_s = Fio;
+      Fia_s = Fia;
+      pen_s = pen;
+    } else {
+      Tsum = Tsum + Tl;
+      Fio_s = Fio_s + Fio;
+      Fia_s = Fia_s + Fia;
+      pen_s = pen_s + pen;
+    }
+  }
+  const T TSurf = Tsum * (T)k.recip_nITC;
+  const T Fio = (Fio_s * (T)k.recip_nITC) * AP;
+  T Fia = (Fia_s * (T)k.recip_nITC) * AP;
+  const T pen = (pen_s * (T)k.recip_nITC) * AP;
+  const bool cold = (AP > (T)0) && (Tsum / (T)k.nITC_d < (T)k.celsius2K);
+  const T snowf = io.snowfall[c], prec = io.precip[c];
+  const T snowRate = (cold ? snowf + prec : snowf) * (T)k.rhoFresh2rhoSnow;
+  const T rainToSea = cold ? (T)0 : prec;
+  const T snowAcc = (snowRate * AP) * (T)k.dtT;
+  T gUnder = Fio * (T)k.qi;
+  gUnder = AP == (T)0 ? (T)0 : gUnder;
+  const T potMeltRate = (-Fia) * (T)k.qs;
+  const T potMelt = potMeltRate * (T)k.dtT;
+  const bool allMelt = potMelt >= hsP;
+  const T melt = allMelt ? hsP : potMelt;
+  const T meltRate = allMelt ? melt * (T)k.rdtT : potMeltRate;
+  const T toMelt = allMelt ? ((-hsP) * (T)k.rdtT) / (T)k.qs : Fia;
+  Fia = Fia - toMelt;
+  const T gSurf = Fia * (T)k.qi;
+  const T gNet = (gUnder + gSurf) * rregA;
+  const T sth = fmax(io.theta[c], Tfrz);
+  const T mltf = (T)1 + ((T)k.McPheeTaperFac - (T)1) / ((T)1 + exp((AP - (T)k.t0) * (T)k.t1));
+  const T Foi = ((T)(-k.t2) * (sth - Tfrz)) * mltf;
+  const T gML = Foi * (T)k.qi;
+  const T qnet_in = io.Qnet[c];
+  const T gOW = (T)k.qi * qnet_in;
+  const T dh = (gNet * AP + gOW * ((T)1 - AP)) + gML;
+  const T dhs = (snowRate - meltRate) * AP;
+  const T half = (T)0.5 * rhIA;
+  T dAoa = (half * gOW) * ((T)1 - AP);
+  const bool grow = (gOW > (T)0) && ((AP > (T)0) || (dh > (T)0));
+  if (grow) dAoa = gOW * ((T)1 - AP);
+  dAoa = (grow && (fc < (T)0)) ? dAoa * (T)k.recip_h0_south : dAoa * (T)k.recip_h0;
+  const T dAoi = gML <= (T)0 ? half * gML : (T)0;
+  const T dAia = ((gNet <= (T)0) && (hP > (T)0)) ? (half * gNet) * AP : (T)0;
+  const T dA = (dAoa + dAoi) + dAia;
+  const T Mc = io.M[c];
+  T A = AP + (dA * Mc) * (T)k.dtT;
+  T h = hP + (dh * Mc) * (T)k.dtT;
+  T hs = hsP + (dhs * Mc) * (T)k.dtT;
+  A = fmin(fmax(A, (T)0), (T)1);
+  h = fmax(h, (T)0);
+  hs = fmax(hs, (T)0);
+  const bool noIce = (h == (T)0) || (A == (T)0);
+  A = A * (T)(!noIce);
+  hs = hs * (T)(!noIce);
+  h = h * (T)(!noIce);
+  const T hsub = (hs * (T)k.rhoSnow + h * (T)k.rhoIce) * (T)k.recip_rhoSea;
+  const T flood = fmax((T)0, hsub - h);
+  h = h + flood;
+  hs = hs - flood * (T)k.rhoIce2rhoSnow;
+  const T rh = (T)1 / sqrt(h * h + (T)k.hIce_reg);
+  const T Qsw = pen * AP + io.Qsw[c] * ((T)1 - AP);
+  const T dH = h - hP;
+  const T meltedSnow = (hsP + snowAcc) - hs;
+  const T eIce = dH / (T)k.qi;
+  const T eOut = (AP * ((Fia + Fio) + pen) + ((T)1 - AP) * qnet_in) * (T)k.dtT;
+  const T Qnet = (eOut - eIce) * (T)k.rdtT;
+  const T salFac = ((sal > (T)0) && (sal > (T)k.saltIce_ref)) ? (T)1 - (T)k.saltIce_ref / sal : (T)1;
+  const T fwIce = ((-dH) * (T)k.rhoIce2rhoFresh) * salFac;
+  const T t0s = fmin((T)k.saltIce_ref, sal);
+  const T osh = io.osh[c], oshs = io.oshs[c];
+  const T saltflux = ((((dH + osh) * t0s) * Mc) * (T)k.rhoIce) * (T)k.rdtT;
+  const T fwSnow = meltedSnow / (T)k.rhoFresh2rhoSnow;
+  const T emp = (Mc * (((((-io.evap[c]) - prec) * ((T)1 - AP) - rainToSea * AP) - io.runoff[c]) -
+                       (fwIce + fwSnow) / (T)k.dtT)) * (T)k.rhoFresh +
+                ((Mc * (osh * (T)k.rhoIce + oshs * (T)k.rhoSnow)) * (T)k.rdtT);
+  const T fss = (emp * sal) / (T)k.rhoFresh + saltflux;
+  io.oh[c] = h;
+  io.ohs[c] = hs;
+  io.oA[c] = A;
+  io.oTs[c] = TSurf;
+  io.oEmPmR[c] = emp;
+  io.ofss[c] = fss;
+  io.oQsw[c] = Qsw;
+  io.oQnet[c] = Qnet;
+  io.oload[c] = h * (T)k.rhoIce + hs * (T)k.rhoSnow;
+  io.open[c] = pen;
+  io.orh[c] = rh;
+}
+
+template <typename T>
+int growth(const Geom& g, const vh_settings* s, const vh_fields* f, const vh_growth_out* o,
+           hipStream_t st) {
+  VH_REQUIRE(o != nullptr, VH_EINVAL, "growth: null outputs");
+  GrowthIO<T> io;
+  io.h = (const T*)f->hIceMean;
+  io.hs = (const T*)f->hSnowMean;
+  io.A = (const T*)f->Area;
+  io.Ts = (const T*)f->TSurf;
+  io.ocSalt = (const T*)f->ocSalt;
+  io.theta = (const T*)f->theta;
+  io.snowfall = (const T*)f->snowfall;
+  io.precip = (const T*)f->precip;
+  io.Qnet = (const T*)f->Qnet;
+  io.Qsw = (const T*)f->Qsw;
+  io.evap = (const T*)f->evap;
+  io.runoff = (const T*)f->runoff;
+  io.osh = (const T*)f->os_hIceMean;
+  io.oshs = (const T*)f->os_hSnowMean;
+  io.M = (const T*)f->iceMask;
+  io.fC = (const T*)f->fCori;
+  io.LWdown = (const T*)f->LWdown;
+  io.ATemp = (const T*)f->ATemp;
+  io.wSpeed = (const T*)f->wSpeed;
+  io.aqh = (const T*)f->aqh;
+  io.SWdown = (const T*)f->SWdown;
+  io.oh = (T*)o->hIceMean;
+  io.ohs = (T*)o->hSnowMean;
+  io.oA = (T*)o->Area;
+  io.oTs = (T*)o->TSurf;
+  io.oEmPmR = (T*)o->EmPmR;
+  io.ofss = (T*)o->forc_salt_surface;
+  io.oQsw = (T*)o->Qsw;
+  io.oQnet = (T*)o->Qnet;
+  io.oload = (T*)o->SeaIceLoad;
+  io.open = (T*)o->IcePenetSW;
+  io.orh = (T*)o->recip_hIceMean;
+  const void* in[] = {io.h, io.hs, io.A, io.Ts, io.ocSalt, io.theta, io.snowfall, io.precip,
+                      io.Qnet, io.Qsw, io.evap, io.runoff, io.osh, io.oshs, io.M, io.fC,
+                      io.LWdown, io.ATemp, io.wSpeed, io.aqh, io.SWdown};
+  const void* out[] = {io.oh, io.ohs, io.oA, io.oTs, io.oEmPmR, io.ofss, io.oQsw, io.oQnet,
+                       io.oload, io.open, io.orh};
+  for (const void* p : in) VH_REQUIRE(p != nullptr, VH_EINVAL, "growth: null input field");
+  for (const void* p : out) {
+    VH_REQUIRE(p != nullptr, VH_EINVAL, "growth: null output");
+    for (const void* q : in) VH_REQUIRE(p != q, VH_EINVAL, "growth: output aliases an input");
+  }
+  const int nITC = (int)s->nITC;
+  VH_REQUIRE(nITC >= 1 && nITC <= 64, VH_EINVAL, "nITC=%d", nITC);
+  GrowthScalars k;
+  k.qi = 1 / (s->rhoIce * s->lhFusion);
+  k.qs = 1 / (s->rhoSnow * s->lhFusion);
+  k.Area_reg = s->Area_reg;
+  k.hIce_reg = s->hIce_reg;
+  k.recip_nITC = s->recip_nITC;
+  k.nITC = nITC;
+  k.nITC_d = (double)nITC;
+  k.tempFrz = s->tempFrz;
+  k.dtempFrz_dS = s->dtempFrz_dS;
+  k.celsius2K = s->celsius2K;
+  k.dtT = s->deltatTherm;
+  k.rdtT = s->recip_deltatTherm;
+  k.t0 = 0.4;
+  k.t1 = 7 / 0.4;
+  k.t2 = s->stantonNr * s->uStarBase * s->rhoSea * s->cpWater;
+  k.McPheeTaperFac = s->McPheeTaperFac;
+  k.recip_h0 = s->recip_h0;
+  k.recip_h0_south = s->recip_h0_south;
+  k.rhoSnow = s->rhoSnow;
+  k.rhoIce = s->rhoIce;
+  k.recip_rhoSea = s->recip_rhoSea;
+  k.rhoIce2rhoSnow = s->rhoIce2rhoSnow;
+  k.rhoIce2rhoFresh = s->rhoIce2rhoFresh;
+  k.rhoFresh = s->rhoFresh;
+  k.rhoFresh2rhoSnow = s->rhoFresh2rhoSnow;
+  k.snowRate_fac = s->rhoFresh2rhoSnow;
+  k.saltIce_ref = s->saltIce_ref;
+  k.d1 = s->dalton * s->cpAir * s->rhoAir;
+  k.d1i = s->dalton * s->lhSublim * s->rhoAir;
+  k.Tmelt = s->celsius2K;
+  k.Tmeltw = k.Tmelt + s->wetAlbTemp;
+  k.minLWdown = s->minLWdown;
+  k.minTAir_K = s->celsius2K + s->minTAir;
+  k.wSpeedMin = s->wSpeedMin;
+  k.snowEmiss = s->snowEmiss;
+  k.iceEmiss = s->iceEmiss;
+  k.stefBoltz = s->stefBoltz;
+  k.dryIceAlb = s->dryIceAlb;
+  k.drySnowAlb = s->drySnowAlb;
+  k.wetIceAlb = s->wetIceAlb;
+  k.wetSnowAlb = s->wetSnowAlb;
+  k.dryIceAlb_south = s->dryIceAlb_south;
+  k.drySnowAlb_south = s->drySnowAlb_south;
+  k.wetIceAlb_south = s->wetIceAlb_south;
+  k.wetSnowAlb_south = s->wetSnowAlb_south;
+  k.hCut = s->hCut;
+  k.shortwave = s->shortwave;
+  k.kIS = s->iceConduct * s->snowConduct;
+  k.snowConduct = s->snowConduct;
+  k.iceConduct = s->iceConduct;
+  k.minTIce_K = s->celsius2K + s->minTIce;
+  k.lhSublim = s->lhSublim;
+  k.aa1 = 2663.5;
+  k.aa2 = 12.537;
+  k.bb1 = 0.622;
+  k.bb1m = 1 - k.bb1;
+  k.ppa = 100000;
+  const double cc0 = std::pow(10.0, k.aa2);
+  k.cc1 = cc0 * k.aa1 * k.bb1 * k.ppa * std::log(10.0);
+  k.cc2 = cc0 * (1 - k.bb1);
+  const size_t n = field_elems(g);
+  hipLaunchKernelGGL(growth_kernel<T>, dim3((unsigned)((n + BLK - 1) / BLK)), dim3(BLK), 0, st, k,
+                     io, n);
+  VH_TRY_HIP(hipGetLastError());
+  return VH_OK;
+}
+
+}  // namespace
+}  // namespace vh
+
+extern "C" int vh_growth(const vh_domain* d, const vh_settings* s, const vh_fields* f,
+                         const vh_growth_out* out, void* stream) {
+  vh::Geom g;
+  int rc = vh::make_geom(d, &g);
+  if (rc) return rc;
+  VH_REQUIRE(s != nullptr && f != nullptr, VH_EINVAL, "null settings or fields");
+  hipStream_t st = (hipStream_t)stream;
+  if (d->dtype == VH_F64) return vh::growth<double>(g, s, f, out, st);
+  return vh::growth<float>(g, s, f, out, st);
+}

@@ -1,0 +1,482 @@
+// Once-per-step stencil and pointwise kernels of the sea-ice step: pre-EVP
+// forcing (veris/area_mass.py, veris/dynamics_routines.py:5-18,
+// veris/dynsolver.py:7-84), ocean stress (veris/ocean_stress.py:8-40),
+// clean-up and ridging (veris/clean_up.py:5-47), the overlap fill
+// (veris/fill_overlap.py:6-18) and an FP64 global sum (veris/global_sum.py).
+//
+// One thread per padded cell, neighbours wrapped like numpy.roll; these
+// kernels run once per model step and move < 1% of the step's bytes, so they
+// are plain coalesced sweeps.  Built with -ffp-contract=off: every expression
+// keeps the reference's association order.
+#include "vh_common.h"
+
+namespace vh {
+namespace {
+
+constexpr int BLK = 256;
+
+#define CELL_PROLOGUE                                                   \
+  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;     \
+  if (idx >= (size_t)g.NX * g.NY) return;                               \
+  const int i = (int)(idx / g.NY);                                      \
+  const int j = (int)(idx - (size_t)i * g.NY);                          \
+  const int im = (i == 0 ? g.NX - 1 : i - 1), ip = (i == g.NX - 1 ? 0 : i + 1); \
+  const int jm = (j == 0 ? g.NY - 1 : j - 1), jp = (j == g.NY - 1 ? 0 : j + 1); \
+  (void)im; (void)ip; (void)jm; (void)jp;
+#define AT(ii, jj) ((size_t)(ii) * g.NY + (jj))
+
+template <typename T>
+const T* C_(const void* p) { return static_cast<const T*>(p); }
+
+// ---------------------------------------------------------------- pre-EVP
+
+template <typename T>
+__global__ __launch_bounds__(BLK) void seaice_mass_kernel(Geom g, const T* h, const T* hs, T rhoI,
+                                                          T rhoS, T* mC, T* mU, T* mV) {
+  CELL_PROLOGUE
+  const T c = rhoI * h[idx] + rhoS * hs[idx];
+  const T cim = rhoI * h[AT(im, j)] + rhoS * hs[AT(im, j)];
+  const T cjm = rhoI * h[AT(i, jm)] + rhoS * hs[AT(i, jm)];
+  mC[idx] = c;
+  mU[idx] = (T)0.5 * (c + cim);
+  mV[idx] = (T)0.5 * (c + cjm);
+}
+
+template <typename T>
+__global__ __launch_bounds__(BLK) void area_ws_kernel(Geom g, const T* A, T* AW, T* AS) {
+  CELL_PROLOGUE
+  AW[idx] = (T)0.5 * (A[idx] + A[AT(im, j)]);
+  AS[idx] = (T)0.5 * (A[idx] + A[AT(i, jm)]);
+}
+
+template <typename T>
+__global__ __launch_bounds__(BLK) void strength_kernel(Geom g, const T* h, const T* A, const T* M,
+                                                       const T* psf, T ps, T cStar, T* P) {
+  CELL_PROLOGUE
+  const T pstar = psf ? psf[idx] : ps;
+  P[idx] = ((pstar * h[idx]) * exp((-cStar) * ((T)1 - A[idx]))) * M[idx];
+}
+
+template <typename T>
+struct WindArgs {
+  const T *u, *v, *uW, *vW, *fC, *MU, *MV, *AW, *AS, *mU, *mV, *rdxC, *rdyC, *ssh, *sp, *load;
+  T sinWin, cosWin, wmin, wmin2, caN, caS, rhoAir, grav, rrho, loadFac;
+  int relWind, realFW;
+};
+
+template <typename T>
+__device__ inline void tau_c(const Geom& g, const WindArgs<T>& w, int i, int j, T* tx, T* ty) {
+  const int ip = i == g.NX - 1 ? 0 : i + 1;
+  const int jp = j == g.NY - 1 ? 0 : j + 1;
+  const size_t c = AT(i, j);
+  T urel = w.uW[c], vrel = w.vW[c];
+  if (w.relWind) {
+    urel = urel - (T)0.5 * (w.u[c] + w.u[AT(ip, j)]);
+    vrel = vrel - (T)0.5 * (w.v[c] + w.v[AT(i, jp)]);
+  }
+  const T wsq = urel * urel + vrel * vrel;
+  const T ws = wsq < w.wmin2 ? w.wmin : sqrt(wsq);
+  const T f = w.fC[c];
+  const T cda = ((f < (T)0 ? w.caS : w.caN) * w.rhoAir) * ws;
+  const T sg = sgn(f);
+  *tx = cda * (w.cosWin * urel - (sg * w.sinWin) * vrel);
+  *ty = cda * (w.cosWin * vrel + (sg * w.sinWin) * urel);
+}
+
+template <typename T>
+__device__ inline T phi_at(const WindArgs<T>& w, size_t c) {
+  T phi = w.grav * w.ssh[c];
+  if (w.realFW) phi = phi + (w.sp[c] + (w.load[c] * w.grav) * w.loadFac) * w.rrho;
+  else phi = phi + w.sp[c] * w.rrho;
+  return phi;
+}
+
+template <typename T>
+__global__ __launch_bounds__(BLK) void wind_forcing_kernel(Geom g, WindArgs<T> w, T* WFX, T* WFY) {
+  CELL_PROLOGUE
+  T tx, ty, txim, tyim, txjm, tyjm;
+  tau_c(g, w, i, j, &tx, &ty);
+  tau_c(g, w, im, j, &txim, &tyim);
+  tau_c(g, w, i, jm, &txjm, &tyjm);
+  const T tauX = ((T)0.5 * (tx + txim)) * w.MU[idx];
+  const T tauY = ((T)0.5 * (ty + tyjm)) * w.MV[idx];
+  const T phi = phi_at(w, idx);
+  T fx = tauX * w.AW[idx];
+  T fy = tauY * w.AS[idx];
+  fx = fx - (w.mU[idx] * w.rdxC[idx]) * (phi - phi_at(w, AT(im, j)));
+  fy = fy - (w.mV[idx] * w.rdyC[idx]) * (phi - phi_at(w, AT(i, jm)));
+  WFX[idx] = fx;
+  WFY[idx] = fy;
+}
+
+// ---------------------------------------------------------------- ocean stress
+
+template <typename T>
+struct DragArgs {
+  const T *u, *v, *uOc, *vOc, *fC, *mInU, *mInV, *M;
+  T cwN, cwS, rhoSea, cmin, cmin2;
+};
+
+template <typename T>
+__device__ inline T cdrag_c(const Geom& g, const DragArgs<T>& a, int i, int j) {
+  const int ip = i == g.NX - 1 ? 0 : i + 1;
+  const int jp = j == g.NY - 1 ? 0 : j + 1;
+  const size_t c = AT(i, j), cip = AT(ip, j), cjp = AT(i, jp);
+  const T D = (a.fC[c] < (T)0 ? a.cwS : a.cwN) * a.rhoSea;
+  const T du = (a.u[c] - a.uOc[c]) * a.mInU[c];
+  const T duip = (a.u[cip] - a.uOc[cip]) * a.mInU[cip];
+  const T dv = (a.v[c] - a.vOc[c]) * a.mInV[c];
+  const T dvjp = (a.v[cjp] - a.vOc[cjp]) * a.mInV[cjp];
+  const T su = du + duip, sv = dv + dvjp;
+  const T w = (T)0.25 * (su * su + sv * sv);
+  const T cd = ((D * D) * w > a.cmin2) ? D * sqrt(w) : a.cmin;
+  return cd * a.M[c];
+}
+
+template <typename T>
+__global__ __launch_bounds__(BLK) void ocean_stress_kernel(Geom g, DragArgs<T> a, T sinW, T cosW,
+                                                           T* tu, T* tv) {
+  CELL_PROLOGUE
+  if (!is_computed(g, i, j)) return;
+  const T cd = cdrag_c(g, a, i, j);
+  const T cdim = cdrag_c(g, a, im, j);
+  const T cdjm = cdrag_c(g, a, i, jm);
+  const T du = a.u[idx] - a.uOc[idx];
+  const T dv = a.v[idx] - a.vOc[idx];
+  const T sg = sgn(a.fC[idx]);
+  // dvAtC at (i, j) and (i, j-1); duAtC at (i, j) and (i-1, j)
+  const T dvC = (T)0.5 * (dv + (a.v[AT(i, jp)] - a.vOc[AT(i, jp)]));
+  const T dvCjm = (T)0.5 * ((a.v[AT(i, jm)] - a.vOc[AT(i, jm)]) + dv);
+  const T duC = (T)0.5 * (du + (a.u[AT(ip, j)] - a.uOc[AT(ip, j)]));
+  const T duCim = (T)0.5 * ((a.u[AT(im, j)] - a.uOc[AT(im, j)]) + du);
+  const T su = (((T)0.5 * (cd + cdim)) * cosW) * du -
+               ((sg * sinW) * (T)0.5) * (cd * dvC + cdjm * dvCjm);
+  const T sv = (((T)0.5 * (cd + cdjm)) * cosW) * dv +
+               ((sg * sinW) * (T)0.5) * (cd * duC + cdim * duCim);
+  store_filled(g, tu, i, j, su);
+  store_filled(g, tv, i, j, sv);
+}
+
+// ---------------------------------------------------------------- clean-up, ridging
+
+template <typename T>
+__global__ __launch_bounds__(BLK) void clean_up_kernel(Geom g, const T* hin, const T* hsin,
+                                                       const T* Ain, const T* Tin, T hmin, T Amin,
+                                                       T c2k, T* h, T* hs, T* A, T* Ts, T* osh,
+                                                       T* oshs) {
+  CELL_PROLOGUE
+  const T h0 = hin[idx], hs0 = hsin[idx];
+  osh[idx] = fmax(-h0, (T)0);
+  oshs[idx] = fmax(-hs0, (T)0);
+  T hh = fmax(h0, (T)0), hhs = fmax(hs0, (T)0), AA = fmax(Ain[idx], (T)0);
+  const bool thin = hh <= hmin;
+  hh = hh * (T)(!thin);
+  hhs = hhs * (T)(!thin);
+  Ts[idx] = thin ? c2k : Tin[idx];
+  if (hh == (T)0 && hhs == (T)0) AA = (T)0;
+  if (hh > (T)0 || hhs > (T)0) AA = fmax(AA, Amin);
+  h[idx] = hh;
+  hs[idx] = hhs;
+  A[idx] = AA;
+}
+
+template <typename T>
+__global__ __launch_bounds__(BLK) void ridging_kernel(Geom g, const T* Ain, T* A) {
+  CELL_PROLOGUE
+  A[idx] = fmin(Ain[idx], (T)1);
+}
+
+// ---------------------------------------------------------------- fill, sum
+
+template <typename T>
+__global__ __launch_bounds__(BLK) void fill_kernel(Geom g, T* a) {
+  // one thread per overlap cell of the padded array
+  CELL_PROLOGUE
+  if (is_computed(g, i, j)) return;
+  int si = i, sj = j;
+  if (g.fill_x) {
+    if (i < g.ol) si = i + g.nx;
+    else if (i >= g.nx + g.ol) si = i - g.nx;
+  }
+  if (g.fill_y) {
+    if (j < g.ol) sj = j + g.ny;
+    else if (j >= g.ny + g.ol) sj = j - g.ny;
+  }
+  a[idx] = a[AT(si, sj)];
+}
+
+template <typename T>
+__global__ __launch_bounds__(BLK) void sum_partial_kernel(Geom g, const T* x, const T* w,
+                                                          double* part) {
+  // interior cells only; fixed-order tree per block, so results are reproducible
+  __shared__ double red[BLK / 64];
+  const size_t n = (size_t)g.nx * g.ny;
+  double acc = 0.0;
+  for (size_t k = (size_t)blockIdx.x * BLK + threadIdx.x; k < n; k += (size_t)gridDim.x * BLK) {
+    const int ii = (int)(k / g.ny) + g.ol, jj = (int)(k % g.ny) + g.ol;
+    const size_t c = AT(ii, jj);
+    double v = (double)x[c];
+    if (w) v = v * (double)w[c];
+    acc += v;
+  }
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s = 0.0;
+    for (int q = 0; q < BLK / 64; ++q) s += red[q];
+    part[blockIdx.x] = s;
+  }
+}
+
+__global__ __launch_bounds__(BLK) void sum_final_kernel(const double* part, int np, double* out) {
+  __shared__ double red[BLK / 64];
+  double acc = 0.0;
+  for (int k = threadIdx.x; k < np; k += BLK) acc += part[k];
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s = 0.0;
+    for (int q = 0; q < BLK / 64; ++q) s += red[q];
+    *out = s;
+  }
+}
+
+constexpr int SUM_BLOCKS = 512;
+
+// ---------------------------------------------------------------- host side
+
+template <typename T>
+int seaice_mass(const Geom& g, const vh_settings* s, const vh_fields* f, void* C, void* U, void* V,
+                hipStream_t st) {
+  VH_REQUIRE(f->hIceMean && f->hSnowMean && C && U && V, VH_EINVAL, "seaice_mass: null pointer");
+  hipLaunchKernelGGL(seaice_mass_kernel<T>, cell_grid(g, BLK), dim3(BLK), 0, st, g,
+                     C_<T>(f->hIceMean), C_<T>(f->hSnowMean), (T)s->rhoIce, (T)s->rhoSnow, (T*)C,
+                     (T*)U, (T*)V);
+  VH_TRY_HIP(hipGetLastError());
+  return VH_OK;
+}
+
+template <typename T>
+int area_ws(const Geom& g, const vh_fields* f, void* W, void* S, hipStream_t st) {
+  VH_REQUIRE(f->Area && W && S, VH_EINVAL, "area_ws: null pointer");
+  hipLaunchKernelGGL(area_ws_kernel<T>, cell_grid(g, BLK), dim3(BLK), 0, st, g, C_<T>(f->Area),
+                     (T*)W, (T*)S);
+  VH_TRY_HIP(hipGetLastError());
+  return VH_OK;
+}
+
+template <typename T>
+int strength(const Geom& g, const vh_settings* s, const vh_fields* f, const void* psf, void* P,
+             hipStream_t st) {
+  VH_REQUIRE(f->hIceMean && f->Area && f->iceMask && P, VH_EINVAL, "seaice_strength: null pointer");
+  hipLaunchKernelGGL(strength_kernel<T>, cell_grid(g, BLK), dim3(BLK), 0, st, g,
+                     C_<T>(f->hIceMean), C_<T>(f->Area), C_<T>(f->iceMask), C_<T>(psf),
+                     (T)s->pStar, (T)s->cStar, (T*)P);
+  VH_TRY_HIP(hipGetLastError());
+  return VH_OK;
+}
+
+template <typename T>
+int wind_forcing(const Geom& g, const vh_settings* s, const vh_fields* f, void* X, void* Y,
+                 hipStream_t st) {
+  WindArgs<T> w;
+  w.u = C_<T>(f->uIce);
+  w.v = C_<T>(f->vIce);
+  w.uW = C_<T>(f->uWind);
+  w.vW = C_<T>(f->vWind);
+  w.fC = C_<T>(f->fCori);
+  w.MU = C_<T>(f->iceMaskU);
+  w.MV = C_<T>(f->iceMaskV);
+  w.AW = C_<T>(f->AreaW);
+  w.AS = C_<T>(f->AreaS);
+  w.mU = C_<T>(f->SeaIceMassU);
+  w.mV = C_<T>(f->SeaIceMassV);
+  w.rdxC = C_<T>(f->recip_dxC);
+  w.rdyC = C_<T>(f->recip_dyC);
+  w.ssh = C_<T>(f->ssh_an);
+  w.sp = C_<T>(f->surfPress);
+  w.load = C_<T>(f->SeaIceLoad);
+  w.relWind = s->useRelativeWind != 0.0;
+  w.realFW = s->useRealFreshWaterFlux != 0.0;
+  const void* req[] = {w.uW, w.vW, w.fC, w.MU, w.MV, w.AW, w.AS, w.mU, w.mV, w.rdxC, w.rdyC, w.ssh,
+                       w.sp, X, Y};
+  for (const void* p : req) VH_REQUIRE(p != nullptr, VH_EINVAL, "wind_forcing_xy: null pointer");
+  if (w.relWind) VH_REQUIRE(w.u && w.v, VH_EINVAL, "wind_forcing_xy: null uIce/vIce");
+  if (w.realFW) VH_REQUIRE(w.load, VH_EINVAL, "wind_forcing_xy: null SeaIceLoad");
+  const double ang = s->airTurnAngle * (M_PI / 180.0);
+  w.sinWin = (T)sin(ang);
+  w.cosWin = (T)cos(ang);
+  w.wmin = (T)s->wSpeedMin;
+  w.wmin2 = (T)(s->wSpeedMin * s->wSpeedMin);
+  w.caN = (T)s->airIceDrag;
+  w.caS = (T)s->airIceDrag_south;
+  w.rhoAir = (T)s->rhoAir;
+  w.grav = (T)s->gravity;
+  w.rrho = (T)s->recip_rhoSea;
+  w.loadFac = (T)s->seaIceLoadFac;
+  hipLaunchKernelGGL(wind_forcing_kernel<T>, cell_grid(g, BLK), dim3(BLK), 0, st, g, w, (T*)X,
+                     (T*)Y);
+  VH_TRY_HIP(hipGetLastError());
+  return VH_OK;
+}
+
+template <typename T>
+int ocean_stress(const Geom& g, const vh_settings* s, const vh_fields* f, void* U, void* V,
+                 hipStream_t st) {
+  DragArgs<T> a;
+  a.u = C_<T>(f->uIce);
+  a.v = C_<T>(f->vIce);
+  a.uOc = C_<T>(f->uOcean);
+  a.vOc = C_<T>(f->vOcean);
+  a.fC = C_<T>(f->fCori);
+  a.mInU = C_<T>(f->maskInU);
+  a.mInV = C_<T>(f->maskInV);
+  a.M = C_<T>(f->iceMask);
+  const void* req[] = {a.u, a.v, a.uOc, a.vOc, a.fC, a.mInU, a.mInV, a.M, U, V};
+  for (const void* p : req) VH_REQUIRE(p != nullptr, VH_EINVAL, "ocean_stress_uv: null pointer");
+  a.cwN = (T)s->waterIceDrag;
+  a.cwS = (T)s->waterIceDrag_south;
+  a.rhoSea = (T)s->rhoSea;
+  a.cmin = (T)s->cDragMin;
+  a.cmin2 = (T)(s->cDragMin * s->cDragMin);
+  const double ang = s->waterTurnAngle * (M_PI / 180.0);
+  hipLaunchKernelGGL(ocean_stress_kernel<T>, cell_grid(g, BLK), dim3(BLK), 0, st, g, a,
+                     (T)sin(ang), (T)cos(ang), (T*)U, (T*)V);
+  VH_TRY_HIP(hipGetLastError());
+  return VH_OK;
+}
+
+template <typename T>
+int clean_up(const Geom& g, const vh_settings* s, const vh_fields* f, void* h, void* hs, void* A,
+             void* Ts, void* osh, void* oshs, hipStream_t st) {
+  const void* req[] = {f->hIceMean, f->hSnowMean, f->Area, f->TSurf, h, hs, A, Ts, osh, oshs};
+  for (const void* p : req) VH_REQUIRE(p != nullptr, VH_EINVAL, "clean_up_advection: null pointer");
+  hipLaunchKernelGGL(clean_up_kernel<T>, cell_grid(g, BLK), dim3(BLK), 0, st, g,
+                     C_<T>(f->hIceMean), C_<T>(f->hSnowMean), C_<T>(f->Area), C_<T>(f->TSurf),
+                     (T)s->hIce_min, (T)s->Area_min, (T)s->celsius2K, (T*)h, (T*)hs, (T*)A, (T*)Ts,
+                     (T*)osh, (T*)oshs);
+  VH_TRY_HIP(hipGetLastError());
+  return VH_OK;
+}
+
+template <typename T>
+int ridging(const Geom& g, const vh_fields* f, void* A, hipStream_t st) {
+  VH_REQUIRE(f->Area && A, VH_EINVAL, "ridging: null pointer");
+  hipLaunchKernelGGL(ridging_kernel<T>, cell_grid(g, BLK), dim3(BLK), 0, st, g, C_<T>(f->Area),
+                     (T*)A);
+  VH_TRY_HIP(hipGetLastError());
+  return VH_OK;
+}
+
+template <typename T>
+int fill(const Geom& g, void* a, hipStream_t st) {
+  VH_REQUIRE(a != nullptr, VH_EINVAL, "fill_overlap: null field");
+  if (!g.fill_x && !g.fill_y) return VH_OK;
+  hipLaunchKernelGGL(fill_kernel<T>, cell_grid(g, BLK), dim3(BLK), 0, st, g, (T*)a);
+  VH_TRY_HIP(hipGetLastError());
+  return VH_OK;
+}
+
+template <typename T>
+int gsum(const Geom& g, const void* x, const void* w, double* out, void* ws, size_t wsb,
+         hipStream_t st) {
+  VH_REQUIRE(x && out, VH_EINVAL, "global_sum: null pointer");
+  VH_REQUIRE(ws && wsb >= SUM_BLOCKS * sizeof(double), VH_EWORKSPACE,
+             "global_sum workspace: need %zu bytes", SUM_BLOCKS * sizeof(double));
+  hipLaunchKernelGGL(sum_partial_kernel<T>, dim3(SUM_BLOCKS), dim3(BLK), 0, st, g, C_<T>(x),
+                     C_<T>(w), (double*)ws);
+  hipLaunchKernelGGL(sum_final_kernel, dim3(1), dim3(BLK), 0, st, (const double*)ws, SUM_BLOCKS,
+                     out);
+  VH_TRY_HIP(hipGetLastError());
+  return VH_OK;
+}
+
+}  // namespace
+}  // namespace vh
+
+#define VH_DISPATCH(d, call_f64, call_f32) ((d)->dtype == VH_F64 ? (call_f64) : (call_f32))
+#define VH_PRE(d, s, f)                                                    \
+  vh::Geom g;                                                              \
+  {                                                                        \
+    int rc_ = vh::make_geom(d, &g);                                        \
+    if (rc_) return rc_;                                                   \
+  }                                                                        \
+  VH_REQUIRE((s) != nullptr && (f) != nullptr, VH_EINVAL, "null settings or fields"); \
+  hipStream_t st = (hipStream_t)stream;
+
+extern "C" int vh_seaice_mass(const vh_domain* d, const vh_settings* s, const vh_fields* f,
+                              void* C, void* U, void* V, void* stream) {
+  VH_PRE(d, s, f)
+  return VH_DISPATCH(d, vh::seaice_mass<double>(g, s, f, C, U, V, st),
+                     vh::seaice_mass<float>(g, s, f, C, U, V, st));
+}
+
+extern "C" int vh_area_ws(const vh_domain* d, const vh_settings* s, const vh_fields* f, void* W,
+                          void* S, void* stream) {
+  VH_PRE(d, s, f)
+  return VH_DISPATCH(d, vh::area_ws<double>(g, f, W, S, st), vh::area_ws<float>(g, f, W, S, st));
+}
+
+extern "C" int vh_seaice_strength(const vh_domain* d, const vh_settings* s, const vh_fields* f,
+                                  const void* psf, void* P, void* stream) {
+  VH_PRE(d, s, f)
+  return VH_DISPATCH(d, vh::strength<double>(g, s, f, psf, P, st),
+                     vh::strength<float>(g, s, f, psf, P, st));
+}
+
+extern "C" int vh_wind_forcing_xy(const vh_domain* d, const vh_settings* s, const vh_fields* f,
+                                  void* X, void* Y, void* stream) {
+  VH_PRE(d, s, f)
+  return VH_DISPATCH(d, vh::wind_forcing<double>(g, s, f, X, Y, st),
+                     vh::wind_forcing<float>(g, s, f, X, Y, st));
+}
+
+extern "C" int vh_ocean_stress_uv(const vh_domain* d, const vh_settings* s, const vh_fields* f,
+                                  void* U, void* V, void* stream) {
+  VH_PRE(d, s, f)
+  return VH_DISPATCH(d, vh::ocean_stress<double>(g, s, f, U, V, st),
+                     vh::ocean_stress<float>(g, s, f, U, V, st));
+}
+
+extern "C" int vh_clean_up_advection(const vh_domain* d, const vh_settings* s, const vh_fields* f,
+                                     void* h, void* hs, void* A, void* Ts, void* osh, void* oshs,
+                                     void* stream) {
+  VH_PRE(d, s, f)
+  return VH_DISPATCH(d, vh::clean_up<double>(g, s, f, h, hs, A, Ts, osh, oshs, st),
+                     vh::clean_up<float>(g, s, f, h, hs, A, Ts, osh, oshs, st));
+}
+
+extern "C" int vh_ridging(const vh_domain* d, const vh_settings* s, const vh_fields* f, void* A,
+                          void* stream) {
+  VH_PRE(d, s, f)
+  return VH_DISPATCH(d, vh::ridging<double>(g, f, A, st), vh::ridging<float>(g, f, A, st));
+}
+
+extern "C" int vh_fill_overlap(const vh_domain* d, void* a, void* stream) {
+  vh::Geom g;
+  int rc = vh::make_geom(d, &g);
+  if (rc) return rc;
+  hipStream_t st = (hipStream_t)stream;
+  return VH_DISPATCH(d, vh::fill<double>(g, a, st), vh::fill<float>(g, a, st));
+}
+
+extern "C" int vh_global_sum_workspace_bytes(const vh_domain* d, size_t* bytes) {
+  vh::Geom g;
+  int rc = vh::make_geom(d, &g);
+  if (rc) return rc;
+  VH_REQUIRE(bytes != nullptr, VH_EINVAL, "null bytes");
+  *bytes = vh::SUM_BLOCKS * sizeof(double);
+  return VH_OK;
+}
+
+extern "C" int vh_global_sum_f64(const vh_domain* d, const void* x, const void* w, double* out,
+                                 void* ws, size_t wsb, void* stream) {
+  vh::Geom g;
+  int rc = vh::make_geom(d, &g);
+  if (rc) return rc;
+  hipStream_t st = (hipStream_t)stream;
+  return VH_DISPATCH(d, vh::gsum<double>(g, x, w, out, ws, wsb, st),
+                     vh::gsum<float>(g, x, w, out, ws, wsb, st));
+}

@@ -1,0 +1,49 @@
+// Library-wide entry points (version, error text) and the domain check.
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+
+#include "vh_common.h"
+
+namespace vh {
+
+static thread_local char g_err[512] = "";
+
+void set_error(const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+}
+
+int make_geom(const vh_domain* d, Geom* g) {
+  VH_REQUIRE(d != nullptr && g != nullptr, VH_EINVAL, "null domain");
+  VH_REQUIRE(d->olx == 2 && d->oly == 2, VH_ESHAPE, "olx=%d oly=%d: only 2 is supported",
+             d->olx, d->oly);
+  VH_REQUIRE(d->nx >= 4 && d->ny >= 4, VH_ESHAPE, "nx=%d ny=%d: need at least 4 x 4 interior cells",
+             d->nx, d->ny);
+  VH_REQUIRE(d->dtype == VH_F64 || d->dtype == VH_F32, VH_EDTYPE, "dtype %d", d->dtype);
+  VH_REQUIRE(d->fill_mode == 0 || d->fill_mode == 1, VH_EINVAL, "fill_mode %d", d->fill_mode);
+  g->nx = d->nx;
+  g->ny = d->ny;
+  g->ol = 2;
+  g->NX = d->nx + 4;
+  g->NY = d->ny + 4;
+  g->fill_x = d->fill_mode == 1 ? 1 : (d->cyclic_x ? 1 : 0);
+  g->fill_y = d->fill_mode == 1 ? 1 : 0;
+  return VH_OK;
+}
+
+}  // namespace vh
+
+extern "C" int vh_version(void) { return VH_ABI_VERSION; }
+
+extern "C" int vh_last_error(char* buf, size_t len) {
+  size_t n = strlen(vh::g_err);
+  if (buf && len) {
+    size_t c = n < len - 1 ? n : len - 1;
+    memcpy(buf, vh::g_err, c);
+    buf[c] = '\0';
+  }
+  return (int)n;
+}

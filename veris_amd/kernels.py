@@ -1,0 +1,237 @@
+"""Drop-in replacements of the reference's sea-ice kernels on the GPU.
+
+Every function takes the Veros-style ``state`` (``state.variables`` holds
+device tensors in the reference layout, ``state.settings`` the scalars) and
+returns fresh tensors in the reference's return order, exactly like the
+``@veros_kernel`` functions it replaces; the caller assigns the results.  All
+arithmetic runs in the HIP kernels of ``libveris_hip.so`` on the tensors'
+device; there is no CPU path.
+"""
+
+from __future__ import annotations
+
+import ctypes
+
+import torch
+
+from . import _native as N
+from .variables import VARIABLE_NAMES
+
+_KTERMS = ("k1AtC", "k2AtC", "k1AtZ", "k2AtZ")
+_ws_cache: dict = {}
+
+
+def _dtype_code(t: torch.Tensor) -> int:
+    if t.dtype == torch.float64:
+        return N.VH_F64
+    if t.dtype == torch.float32:
+        return N.VH_F32
+    raise TypeError(f"unsupported dtype {t.dtype}; use float64 or float32")
+
+
+def _all_zero(t: torch.Tensor) -> bool:
+    """Whether ``t`` is all zeros, remembered on the tensor object for its
+    current version counter (an address-keyed cache would be reused by a new
+    tensor that the allocator places at the same address)."""
+    tag = getattr(t, "_vh_all_zero", None)
+    if tag is not None and tag[0] == t._version:
+        return tag[1]
+    res = bool(torch.count_nonzero(t).item() == 0)
+    t._vh_all_zero = (t._version, res)
+    return res
+
+
+class Ctx:
+    """Per-call ABI arguments derived from a state."""
+
+    def __init__(self, state, ref: torch.Tensor | None = None):
+        vs, sett = state.variables, state.settings
+        ref = vs.iceMask if ref is None else ref
+        if not isinstance(ref, torch.Tensor) or ref.device.type != "cuda":
+            raise TypeError("state variables must be torch tensors on a GPU device")
+        self.device = ref.device
+        self.dtype = ref.dtype
+        self.shape = (int(sett.nx) + 2 * int(sett.olx), int(sett.ny) + 2 * int(sett.oly))
+        self.dom = N.domain_struct(sett, _dtype_code(ref))
+        self.sett = N.settings_struct(sett)
+        self.fields = N.Fields()
+        for name in VARIABLE_NAMES:
+            t = vs.get(name)
+            if t is None:
+                continue
+            self._check(name, t)
+            setattr(self.fields, name, t.data_ptr())
+        if all(vs.get(k) is None or _all_zero(vs.get(k)) for k in _KTERMS):
+            for k in _KTERMS:
+                setattr(self.fields, k, None)
+        self.stream = torch.cuda.current_stream(self.device).cuda_stream
+
+    def _check(self, name, t):
+        if not isinstance(t, torch.Tensor):
+            raise TypeError(f"variable {name} is not a torch tensor")
+        if t.device != self.device or t.dtype != self.dtype:
+            raise TypeError(f"variable {name}: {t.dtype} on {t.device}, expected "
+                            f"{self.dtype} on {self.device}")
+        if tuple(t.shape) != self.shape or not t.is_contiguous():
+            raise ValueError(f"variable {name}: shape {tuple(t.shape)} (contiguous="
+                             f"{t.is_contiguous()}), expected contiguous {self.shape}")
+
+    def empty(self, n=1):
+        out = [torch.empty(self.shape, dtype=self.dtype, device=self.device) for _ in range(n)]
+        return out if n > 1 else out[0]
+
+    def workspace(self, query: str) -> torch.Tensor:
+        nbytes = N.workspace_bytes(query, self.dom)
+        key = (query, self.device, self.stream, self.shape, self.dtype)
+        ws = _ws_cache.get(key)
+        if ws is None or ws.numel() < nbytes:
+            ws = torch.empty(max(nbytes, 256), dtype=torch.uint8, device=self.device)
+            _ws_cache[key] = ws
+        return ws
+
+    def args(self):
+        return ctypes.byref(self.dom), ctypes.byref(self.sett), ctypes.byref(self.fields)
+
+
+def _p(t):
+    return None if t is None else t.data_ptr()
+
+
+# --------------------------------------------------------------------------
+
+
+def SeaIceMass(state):
+    """veris/area_mass.py:17-28 -> (SeaIceMassC, SeaIceMassU, SeaIceMassV)."""
+    c = Ctx(state)
+    out = c.empty(3)
+    N.call("vh_seaice_mass", *c.args(), *map(_p, out), c.stream)
+    return tuple(out)
+
+
+def AreaWS(state):
+    """veris/area_mass.py:5-14 -> (AreaW, AreaS)."""
+    c = Ctx(state)
+    out = c.empty(2)
+    N.call("vh_area_ws", *c.args(), *map(_p, out), c.stream)
+    return tuple(out)
+
+
+def SeaIceStrength(state, pStar: torch.Tensor | None = None):
+    """veris/dynamics_routines.py:5-18 -> SeaIceStrength.
+
+    ``pStar`` optionally replaces the scalar setting by a per-cell field."""
+    c = Ctx(state)
+    if pStar is not None:
+        c._check("pStar", pStar)
+    out = c.empty()
+    N.call("vh_seaice_strength", *c.args(), _p(pStar), _p(out), c.stream)
+    return out
+
+
+def WindForcingXY(state):
+    """veris/dynsolver.py:50-84 (with tauXY :7-47) -> (WindForcingX, WindForcingY)."""
+    c = Ctx(state)
+    out = c.empty(2)
+    N.call("vh_wind_forcing_xy", *c.args(), *map(_p, out), c.stream)
+    return tuple(out)
+
+
+# EVP tile shape index used when a call does not name one (0 = library default);
+# set by veris_amd.autotune.
+EVP_TILE = {}
+
+
+def evp_solver(state, n_substeps: int | None = None, tile: int | None = None):
+    """veris/evp_solver.py:257-319 -> (uIce, vIce, sigma1, sigma2, sigma12).
+
+    ``tile`` selects the workgroup shape (see :func:`veris_amd.autotune.tune_evp`)."""
+    c = Ctx(state)
+    if tile is None:
+        tile = EVP_TILE.get((c.shape, c.dtype), 0)
+    c.dom.tile = int(tile)
+    n = int(state.settings.nEVPsteps if n_substeps is None else n_substeps)
+    out = c.empty(5)
+    ws = c.workspace("vh_evp_workspace_bytes")
+    N.call("vh_evp_solver", *c.args(), n, *map(_p, out), _p(ws), ws.numel(), c.stream)
+    return tuple(out)
+
+
+def IceVelocities(state, n_substeps: int | None = None):
+    """veris/dynsolver.py:87-102: the EVP branch (free drift is not provided)."""
+    s = state.settings
+    if not s.useEVP:
+        raise NotImplementedError("only the EVP solver is provided (useEVP=True)")
+    return evp_solver(state, n_substeps)
+
+
+def OceanStressUV(state):
+    """veris/ocean_stress.py:8-40 -> (OceanStressU, OceanStressV)."""
+    c = Ctx(state)
+    out = c.empty(2)
+    N.call("vh_ocean_stress_uv", *c.args(), *map(_p, out), c.stream)
+    return tuple(out)
+
+
+def Advection(state):
+    """veris/advection.py:15-25 -> (hIceMean, hSnowMean, Area)."""
+    c = Ctx(state)
+    out = c.empty(3)
+    ws = c.workspace("vh_advection_workspace_bytes")
+    N.call("vh_advection", *c.args(), *map(_p, out), _p(ws), ws.numel(), c.stream)
+    return tuple(out)
+
+
+def clean_up_advection(state):
+    """veris/clean_up.py:5-39 ->
+    (hIceMean, hSnowMean, Area, TSurf, os_hIceMean, os_hSnowMean)."""
+    c = Ctx(state)
+    out = c.empty(6)
+    N.call("vh_clean_up_advection", *c.args(), *map(_p, out), c.stream)
+    return tuple(out)
+
+
+def ridging(state):
+    """veris/clean_up.py:42-47 -> Area."""
+    c = Ctx(state)
+    out = c.empty()
+    N.call("vh_ridging", *c.args(), _p(out), c.stream)
+    return out
+
+
+def Growth(state):
+    """veris/growth.py:8-397 -> (hIceMean, hSnowMean, Area, TSurf, EmPmR,
+    forc_salt_surface, Qsw, Qnet, SeaIceLoad, IcePenetSW, recip_hIceMean)."""
+    c = Ctx(state)
+    out = c.empty(len(N.GROWTH_OUT_NAMES))
+    go = N.GrowthOut(*[t.data_ptr() for t in out])
+    N.call("vh_growth", *c.args(), ctypes.byref(go), c.stream)
+    return tuple(out)
+
+
+def fill_overlap(state, a: torch.Tensor) -> torch.Tensor:
+    """veris/fill_overlap.py:6-18 -> filled copy of ``a``."""
+    c = Ctx(state, ref=a)
+    c._check("field", a)
+    out = a.clone()
+    N.call("vh_fill_overlap", ctypes.byref(c.dom), _p(out), c.stream)
+    return out
+
+
+def fill_overlap_uv(state, u, v):
+    """veris/fill_overlap.py:21-23."""
+    return fill_overlap(state, u), fill_overlap(state, v)
+
+
+def global_sum(state, a: torch.Tensor, weight: torch.Tensor | None = None) -> torch.Tensor:
+    """FP64 sum of ``a`` (times ``weight``) over the interior cells, as a 0-d
+    float64 device tensor.  The reference's ``global_sum`` (veris/global_sum.py:1-4)
+    is an identity stub; this is the reduction it stands for."""
+    c = Ctx(state, ref=a)
+    c._check("field", a)
+    if weight is not None:
+        c._check("weight", weight)
+    res = torch.empty((), dtype=torch.float64, device=c.device)
+    ws = c.workspace("vh_global_sum_workspace_bytes")
+    N.call("vh_global_sum_f64", ctypes.byref(c.dom), _p(a), _p(weight), _p(res), _p(ws),
+           ws.numel(), c.stream)
+    return res
