@@ -329,9 +329,12 @@ int vh_wind_forcing_xy(const vh_domain* d, const vh_settings* s, const vh_fields
 /* Number of EVP tile shapes (valid vh_domain.tile values are 1..count). */
 int vh_evp_tile_count(void);
 
-/* Workgroup shape (ri x rj threads; each owns a (ri-4) x (rj-4) tile plus a
- * two-cell ring) of EVP tile index `tile`. */
-int vh_evp_tile_shape(int32_t tile, int32_t* ri, int32_t* rj);
+/* Launch shape of EVP tile index `tile`: ri x rj threads, one per cell of a
+ * (ri-4k) x (rj-4k) tile plus a 2k-cell ring, running k substeps per launch.
+ * k = 1 reproduces the reference on every cell; k > 1 keeps the velocities on
+ * every cell and the stresses on every cell except the periodic overlap
+ * columns whose reference values are wrap-around artefacts (see DESIGN). */
+int vh_evp_tile_shape(int32_t tile, int32_t* ri, int32_t* rj, int32_t* k);
 
 /* Workspace bytes vh_evp_solver needs for this domain. */
 int vh_evp_workspace_bytes(const vh_domain* d, size_t* bytes);

@@ -1,0 +1,91 @@
+"""The oracle pins must fail on plausible restatement mistakes.
+
+Each case plants one mistake (a flipped roll direction, a swapped metric, a
+dropped factor, a wrong sign) into a temporary copy of the NumPy oracle, on the
+CPU, and runs the pin tests against that copy; the pins must report a failure
+for every planted mistake.
+"""
+
+from __future__ import annotations
+
+import concurrent.futures as cf
+import os
+import subprocess
+import sys
+from pathlib import Path
+
+import pytest
+
+ROOT = Path(__file__).resolve().parent.parent
+ORACLE = ROOT / "oracle" / "veris_np.py"
+
+MUTATIONS = {
+    "slope ratio upwind side": ("cr = np.where(trans > 0, rjm, rjp)", "cr = np.where(trans > 0, rjp, rjm)"),
+    "shear corner direction": ("e12c = 0.25 * vs.recip_rA * (q + jp1(q))", "e12c = 0.25 * vs.recip_rA * (q + jm1(q))"),
+    "stressdiv x shear term": ("dx = (a - im1(a) - b + jp1(b)) * vs.recip_rAu", "dx = (a - im1(a) - b + jm1(b)) * vs.recip_rAu"),
+    "stressdiv y shear term": ("dy = (c - jm1(c) - d + ip1(d)) * vs.recip_rAv", "dy = (c - jm1(c) - d + im1(d)) * vs.recip_rAv"),
+    "u denominator drag": ("0.5 * (cd + im1(cd)) * s.cosWat + 0.5 * (cb + im1(cb)))", "0.5 * (cd + ip1(cd)) * s.cosWat + 0.5 * (cb + im1(cb)))"),
+    "v denominator drag": ("0.5 * (cd + jm1(cd)) * s.cosWat + 0.5 * (cb + jm1(cb)))", "0.5 * (cd + jp1(cd)) * s.cosWat + 0.5 * (cb + jm1(cb)))"),
+    "no-slip v term": ("hv = vs.iceMaskV - im1(vs.iceMaskV)", "hv = vs.iceMaskV - ip1(vs.iceMaskV)"),
+    "z-mask corners": ("mz = mz * jm1(mz)", "mz = mz * jp1(mz)"),
+    "zeta factor": ("zeta = 0.5 * (P * (1 + s.tensileStrFac)) / dreg", "zeta = (P * (1 + s.tensileStrFac)) / dreg"),
+    "replacement pressure": ("+ 2.0 * zeta * dC * s.pressReplFac / (1 + s.tensileStrFac))", "+ zeta * dC * s.pressReplFac / (1 + s.tensileStrFac))"),
+    "alpha relaxation": ("s1 = (s1 * (alpha - rev) + divergence) * den1 * vs.iceMask", "s1 = (s1 * alpha + divergence) * den1 * vs.iceMask"),
+    "tension 1/e^2": ("s2 = (s2 * (alpha - rev) + tension * rrev) * den2 * vs.iceMask", "s2 = (s2 * (alpha - rev) + tension) * den2 * vs.iceMask"),
+    "shear 1/e^2": ("s12 = (s12 * (alpha - rev) + shear * rrev) * den2", "s12 = (s12 * (alpha - rev) + shear) * den2"),
+    "z-point average": ("z = c + im1(c)", "z = c + ip1(c)"),
+    "Coriolis sign": ("fx = fx + 0.5 * (fv + im1(fv))", "fx = fx - 0.5 * (fv + im1(fv))"),
+    "Coriolis u average": ("fu = vs.SeaIceMassC * vs.fCori * 0.5 * (u + ip1(u))", "fu = vs.SeaIceMassC * vs.fCori * 0.5 * (u + im1(u))"),
+    "Coriolis v-face average": ("fy = fy - 0.5 * (fu + jm1(fu))", "fy = fy - 0.5 * (fu + jp1(fu))"),
+    "stress divergence sign": ("+ fx + divx) / denU", "+ fx - divx) / denU"),
+    "beta + 1": ("bp1U = betaU + s.recip_deltatDyn", "bp1U = betaU"),
+    "drag v faces": ("w = 0.25 * ((du + ip1(du)) ** 2 + (dv + jp1(dv)) ** 2)", "w = 0.25 * ((du + ip1(du)) ** 2 + (dv + jm1(dv)) ** 2)"),
+    "metric term k1C": ("e22 = (dvdy + uave * vs.k1AtC) * vs.maskInC", "e22 = (dvdy + vave * vs.k1AtC) * vs.maskInC"),
+    "mass at u-points": ("mU = 0.5 * (mC + im1(mC))", "mU = 0.5 * (mC + ip1(mC))"),
+    "area at v-points": ("return 0.5 * (vs.Area + im1(vs.Area)), 0.5 * (vs.Area + jm1(vs.Area))", "return 0.5 * (vs.Area + im1(vs.Area)), 0.5 * (vs.Area + jp1(vs.Area))"),
+    "tilt direction": ("fx = fx - vs.SeaIceMassU * vs.recip_dxC * (phi - im1(phi))", "fx = fx - vs.SeaIceMassU * vs.recip_dxC * (phi - ip1(phi))"),
+    "wind stress v average": ("ty = 0.5 * (ty + jm1(ty)) * vs.iceMaskV", "ty = 0.5 * (ty + jp1(ty)) * vs.iceMaskV"),
+    "relative wind": ("vrel = vs.vWind - 0.5 * (vs.vIce + jp1(vs.vIce))", "vrel = vs.vWind - 0.5 * (vs.vIce + jm1(vs.vIce))"),
+    "ocean stress drag": ("tu = 0.5 * (cd + im1(cd)) * cosW * du", "tu = 0.5 * (cd + ip1(cd)) * cosW * du"),
+    "x CFL metric": ("cfl = np.abs(vs.uIce * s.deltatTherm * vs.recip_dxC)", "cfl = np.abs(vs.uIce * s.deltatTherm * vs.recip_dyC)"),
+    "y slope mask": ("rj = (f[:, 2:-1] - f[:, 1:-2]) * mS[:, 2:-1]", "rj = (f[:, 2:-1] - f[:, 1:-2]) * mS[:, 1:-2]"),
+    "x update faces": ("f = f - s.deltatTherm * vs.maskInC * vs.recip_rA * (ip1(F) - F)", "f = f - s.deltatTherm * vs.maskInC * vs.recip_rA * (F - im1(F))"),
+    "thin ice boundary": ("thin = h <= s.hIce_min", "thin = h < s.hIce_min"),
+    "Newton denominator": ("T = np.where(ice, T + (Fc - Fia) / (kEff + dF), 0)", "T = np.where(ice, T + (Fc - Fia) / (kEff - dF), 0)"),
+    "sensible heat sign": ("Fs = np.where(ice, d1 * ug * (t1 - ta), 0)", "Fs = np.where(ice, d1 * ug * (ta - t1), 0)"),
+    "latent heat sign": ("Flh = np.where(ice, d1i * ug * (qs - vs.aqh), 0)", "Flh = np.where(ice, d1i * ug * (vs.aqh - qs), 0)"),
+    "snow albedo": ("alb = np.where(hS > s.hCut, aS, alb)", "alb = np.where(hS > s.hCut, aI, alb)"),
+    "penetrating short wave": ("pen = np.where(ice, s.shortwave * np.exp(-1.5 * hI), 0)", "pen = np.where(ice, s.shortwave * np.exp(-1.5 * hS), 0)"),
+    "McPhee taper": ("mltf = 1 + (s.McPheeTaperFac - 1) / (1 + np.exp((AP - t0) * t1))", "mltf = 1 + (s.McPheeTaperFac - 1) / (1 + np.exp((AP + t0) * t1))"),
+    "snow melt energy": ("toMelt = np.where(allMelt, -hsP * s.recip_deltatTherm / qs, Fia)", "toMelt = np.where(allMelt, -hsP * s.recip_deltatTherm / qi, Fia)"),
+    "lead closing": ("dAoa = np.where(grow, gOW * (1 - AP), dAoa)", "dAoa = np.where(grow, gOW * AP, dAoa)"),
+    "flooding density ratio": ("hs = hs - flood * s.rhoIce2rhoSnow", "hs = hs - flood"),
+    "fresh water sign": ("- (fwIce + fwSnow) / s.deltatTherm) * s.rhoFresh", "+ (fwIce + fwSnow) / s.deltatTherm) * s.rhoFresh"),
+}
+
+
+def _run(name, tmp_path):
+    src = ORACLE.read_text()
+    a, b = MUTATIONS[name]
+    assert src.count(a) == 1, f"mutation anchor not unique or missing: {name}"
+    path = tmp_path / (name.replace(" ", "_").replace("/", "_") + ".py")
+    path.write_text(src.replace(a, b))
+    env = dict(os.environ, ORACLE_NP_PATH=str(path))
+    r = subprocess.run([sys.executable, "-m", "pytest", str(ROOT / "tests" / "test_oracle_pins.py"),
+                        "-x", "-q", "-p", "no:cacheprovider"], cwd=ROOT, env=env,
+                       capture_output=True, text=True)
+    return name, r.returncode
+
+
+def test_every_planted_mistake_is_caught(tmp_path):
+    with cf.ThreadPoolExecutor(max_workers=6) as ex:
+        results = list(ex.map(lambda n: _run(n, tmp_path), MUTATIONS))
+    missed = [n for n, rc in results if rc == 0]
+    assert not missed, f"pins did not catch: {missed}"
+    assert all(rc == 1 for _, rc in results), results
+
+
+def test_unmutated_oracle_passes_the_pins():
+    r = subprocess.run([sys.executable, "-m", "pytest", str(ROOT / "tests" / "test_oracle_pins.py"),
+                        "-q", "-p", "no:cacheprovider"], cwd=ROOT, capture_output=True, text=True)
+    assert r.returncode == 0, r.stdout[-2000:]

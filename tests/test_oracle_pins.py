@@ -1,0 +1,855 @@
+"""Pins of the NumPy oracle against what the mathematics fixes.
+
+The reference ships no tests, golden vectors or fixtures for this path, and it
+cannot run here (Veros and JAX are absent), so each oracle function is pinned
+by a closed form, an invariant or an independent computation chosen so that a
+plausible restatement mistake (a wrong roll direction, a swapped metric, a
+dropped term, a wrong relaxation factor) fails:
+
+* strain rates: quadratic velocity fields, whose staggered centred differences
+  are exact, plus the no-slip boundary term's position;
+* viscosities: Hibler's invariants for pure divergence and pure shear;
+* stress relaxation: exact geometric decay (1 - 1/alpha)^k at rest;
+* stress divergence: flux-form telescoping with non-uniform metrics and the
+  exact divergence of a linear stress;
+* water drag: closed form max(D |du|, cDragMin) in both hemispheres;
+* momentum: the implicit-Euler free-drift fixed point and its exact
+  contraction factor;
+* advection: exact one-cell shift at CFL 1, conservation, the textbook
+  superbee limiter, the discrete maximum principle;
+* solve4temp: scipy's root of an independently written surface energy balance;
+* Growth: open-water freezing closed form, energy closure, flooding invariants;
+* clean-up, ridging, fill, mass, strength, wind stress: closed forms.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+from scipy.optimize import brentq
+
+from oracle_fixtures import load_oracle, uniform_state, xy
+
+O = load_oracle()
+
+INNER = (slice(2, -2), slice(2, -2))
+
+
+def close(got, want, rtol):
+    """|got - want| <= rtol * max|want| (relative to the field's scale)."""
+    want = np.broadcast_to(want, np.shape(got))
+    np.testing.assert_allclose(got, want, rtol=0, atol=rtol * float(np.max(np.abs(want))))
+
+
+def test_strainrates_quadratic_fields_exact():
+    st = uniform_state(nx=20, ny=16, dx=1.2e4, dy=0.8e4)
+    x, y, dx, dy = xy(st)
+    a, b, c, d = 3e-12, -2e-12, 5e-12, 7e-12
+    xu, yu = x, y + 0.5 * dy          # u-points: west face
+    xv, yv = x + 0.5 * dx, y          # v-points: south face
+    u = a * xu**2 + b * yu**2
+    v = c * xv**2 + d * yv**2
+    e11, e22, e12 = O.strainrates(st, u, v)
+    xc, yc = x + 0.5 * dx, y + 0.5 * dy
+    close(e11[INNER], (2 * a * xc)[INNER], 1e-11)
+    close(e22[INNER], (2 * d * yc)[INNER], 1e-11)
+    close(e12[INNER], (b * y + c * x)[INNER], 1e-11)
+
+
+def test_strainrates_no_slip_term_position():
+    """A land row south of row j0 adds 2*u/dy at z-points of row j0 only."""
+    st = uniform_state(nx=8, ny=10)
+    j0 = 6
+    st.variables.iceMaskU[:, :j0] = 0.0
+    u0 = 0.2
+    u = np.full(st.shape, u0)
+    v = np.zeros(st.shape)
+    _, _, e12 = O.strainrates(st, u, v)
+    dy = st.variables.dyU[0, 0]
+    np.testing.assert_allclose(e12[2:-2, j0], 2 * u0 / dy, rtol=1e-14)
+    others = [jj for jj in range(2, st.shape[1] - 2) if jj != j0]
+    assert np.all(e12[2:-2, others] == 0)
+
+
+def _visc(st, e11, e22, e12, P):
+    z, _, p, d = O.viscosities(st, np.full(st.shape, e11), np.full(st.shape, e22),
+                               np.full(st.shape, e12), P=np.full(st.shape, P))
+    return z[INNER].mean(), p[INNER].mean(), d[INNER].mean()
+
+
+def test_viscosities_hibler_invariants():
+    st = uniform_state()
+    P, dmin = 2.0e4, st.settings.deltaMin
+    # pure divergence: Delta = |e11 + e22|
+    for eps in (3e-7, -2e-8):
+        z, p, d = _visc(st, eps, eps, 0.0, P)
+        assert d == pytest.approx(2 * abs(eps), rel=1e-14)
+        assert z == pytest.approx(0.5 * P / (2 * abs(eps) + dmin), rel=1e-14)
+        assert p == pytest.approx(P * d / (d + dmin), rel=1e-14)
+    # pure shear through the normal components: Delta = |e11 - e22| / e
+    z, p, d = _visc(st, 4e-7, -4e-7, 0.0, P)
+    assert d == pytest.approx(8e-7 / 2.0, rel=1e-14)
+    # shear strain at z-points: Delta = 2|e12_C|/e with e12_C^2 the area-weighted
+    # four-point average rAz*e12^2*4/(4 rA) = e12^2/2 for the plugin's rAz = rA/2
+    s = 6e-7
+    z, p, d = _visc(st, 0.0, 0.0, s, P)
+    assert d == pytest.approx(2 * np.sqrt(s**2 / 2) / 2.0, rel=1e-14)
+
+
+def test_viscosities_tensile_and_no_replacement_pressure():
+    st = uniform_state(tensileStrFac=0.1, pressReplFac=0)
+    P, eps = 3.0e4, 1e-7
+    z, p, d = _visc(st, eps, eps, 0.0, P)
+    assert z == pytest.approx(0.5 * P * 1.1 / (d + st.settings.deltaMin), rel=1e-14)
+    assert p == pytest.approx(P * (1 - 0.1), rel=1e-14)
+
+
+def _rest_state(**kw):
+    st = uniform_state(nx=12, ny=10, veros_fill=False, **kw)
+    vs = st.variables
+    vs.SeaIceMassC = np.full(st.shape, 900.0)
+    vs.SeaIceMassU = vs.SeaIceMassC.copy()
+    vs.SeaIceMassV = vs.SeaIceMassC.copy()
+    vs.AreaW = np.ones(st.shape)
+    vs.AreaS = np.ones(st.shape)
+    return st
+
+
+def test_stress_relaxation_geometric_decay_at_rest():
+    """u = v = 0 and uniform stresses stay at rest (zero divergence), strain
+    rates vanish, press = 0, so sigma_k = sigma_0 (1 - 1/alpha)^k exactly."""
+    st = _rest_state()
+    vs = st.variables
+    vs.SeaIceStrength = np.full(st.shape, 1e4)
+    s1, s2, s12 = 1500.0, -700.0, 300.0
+    vs.sigma1 = np.full(st.shape, s1)
+    vs.sigma2 = np.full(st.shape, s2)
+    vs.sigma12 = np.full(st.shape, s12)
+    k = 37
+    u, v, a1, a2, a12 = O.evp_solver(st, n_substeps=k)
+    assert np.all(u == 0) and np.all(v == 0)
+    fac = (1 - 1 / st.settings.evpAlpha) ** k
+    for got, want in ((a1, s1), (a2, s2), (a12, s12)):
+        np.testing.assert_allclose(got[INNER], want * fac, rtol=1e-12)
+
+
+def test_stressdiv_flux_form_telescopes():
+    st = uniform_state(nx=14, ny=11, veros_fill=False)
+    r = np.random.default_rng(1)
+    vs = st.variables
+    for k in ("dxU", "dyU", "dxV", "dyV", "rAu", "rAv"):
+        vs[k] = r.uniform(0.5, 2.0, st.shape) * vs[k]
+    vs.recip_rAu = 1 / vs.rAu
+    vs.recip_rAv = 1 / vs.rAv
+    s11, s22, s12 = (r.normal(size=st.shape) for _ in range(3))
+    dx_, dy_ = O.stressdiv(st, s11, s22, s12)
+    scale = np.abs(s11 * vs.dyV).sum() + np.abs(s12 * vs.dxV).sum()
+    assert abs((dx_ * vs.rAu).sum()) < 1e-13 * scale
+    assert abs((dy_ * vs.rAv).sum()) < 1e-13 * scale
+
+
+def test_stressdiv_linear_stress():
+    st = uniform_state(nx=10, ny=9, dx=2e4, dy=3e4)
+    x, y, dx, dy = xy(st)
+    s11 = 4.0 * (x + 0.5 * dx)   # at c-points
+    s22 = -2.0 * (y + 0.5 * dy)
+    s12 = 1.5 * x + 2.5 * y       # at z-points: d/dy -> 2.5, d/dx -> 1.5
+    divx, divy = O.stressdiv(st, s11, s22, s12)
+    np.testing.assert_allclose(divx[INNER], 4.0 + 2.5, rtol=1e-12)
+    np.testing.assert_allclose(divy[INNER], -2.0 + 1.5, rtol=1e-12)
+
+
+def test_ocean_drag_closed_form_both_hemispheres():
+    st = uniform_state(waterIceDrag=0.0055, waterIceDrag_south=0.004)
+    vs = st.variables
+    vs.fCori = np.where(np.arange(st.shape[1])[None, :] < st.shape[1] // 2, -1e-4, 1e-4) \
+        * np.ones(st.shape)
+    s = st.settings
+    for du, dv in ((0.3, -0.4), (0.001, 0.002)):
+        vs.uOcean = np.full(st.shape, 0.05)
+        vs.vOcean = np.full(st.shape, -0.02)
+        cd = O.ocean_drag_coeffs(st, vs.uOcean + du, vs.vOcean + dv)
+        D = np.where(vs.fCori < 0, s.waterIceDrag_south, s.waterIceDrag) * s.rhoSea
+        want = np.maximum(D * np.hypot(du, dv), s.cDragMin)
+        np.testing.assert_allclose(cd[INNER], want[INNER], rtol=1e-14)
+
+
+def test_momentum_free_drift_fixed_point_and_contraction():
+    """P = 0, f = 0, no turning, cDragMin regime: u_k = u* + (u0 - u*) rho^k with
+    u* = (m uNm1/dt + tau + C A uo)/(m/dt + C A), rho = (beta m/dt)/(m(beta+1)/dt + C A)."""
+    st = _rest_state()
+    s = st.settings
+    vs = st.variables
+    m, tau, uo, vo, A = 900.0, 1e-3, 0.01, -0.004, 1.0
+    vs.WindForcingX = np.full(st.shape, tau)
+    vs.WindForcingY = np.full(st.shape, -tau / 2)
+    vs.uOcean = np.full(st.shape, uo)
+    vs.vOcean = np.full(st.shape, vo)
+    u0, v0 = 0.02, 0.015
+    vs.uIce = np.full(st.shape, u0)
+    vs.vIce = np.full(st.shape, v0)
+    dt, beta, C = s.deltatDyn, s.evpBeta, s.cDragMin
+    den = m * (beta + 1) / dt + C * A
+    rho = (beta * m / dt) / den
+    k = 60
+    u, v, *_ = O.evp_solver(st, n_substeps=k)
+    for got, x0, forcing, oc in ((u, u0, tau, uo), (v, v0, -tau / 2, vo)):
+        star = (m * x0 / dt + forcing + C * A * oc) / (m / dt + C * A)
+        want = star + (x0 - star) * rho**k
+        np.testing.assert_allclose(got[INNER], want, rtol=1e-12)
+
+
+def test_superbee_textbook():
+    r = np.linspace(-3, 5, 8001)
+    want = np.select([r <= 0, r <= 0.5, r <= 1, r <= 2], [0.0, 2 * r, 1.0, r], 2.0)
+    np.testing.assert_array_equal(O.limiter(r), want)
+
+
+def _adv_state(u, v, nx=24, ny=18, dx=2e4, dy=3e4):
+    st = uniform_state(nx=nx, ny=ny, dx=dx, dy=dy, veros_fill=False)
+    st.variables.uIce = np.full(st.shape, u) if np.isscalar(u) else u
+    st.variables.vIce = np.full(st.shape, v) if np.isscalar(v) else v
+    return st
+
+
+def _periodic(st, a):
+    return O.fill_overlap(st, a)
+
+
+def test_advection_cfl_one_is_exact_shift():
+    """dx != dy: a CFL of exactly one in x (or y) shifts the field by one cell."""
+    dx, dy = 2e4, 3e4
+    rng = np.random.default_rng(2)
+    for sgn_ in (1, -1):
+        for axis in (0, 1):
+            vel = sgn_ * (dx if axis == 0 else dy) / 86400.0
+            st = _adv_state(vel if axis == 0 else 0.0, vel if axis == 1 else 0.0)
+            f = _periodic(st, rng.uniform(0, 3, st.shape))
+            out = O.calc_Advection(st, f)
+            np.testing.assert_allclose(out[INNER], np.roll(f, sgn_, axis)[INNER], rtol=1e-13,
+                                       atol=1e-14)
+
+
+def test_advection_conservation_and_constant_field():
+    r = np.random.default_rng(3)
+    st = _adv_state(0.0, 0.0)
+    vs = st.variables
+    vs.uIce = _periodic(st, r.normal(0, 0.05, st.shape))
+    vs.vIce = _periodic(st, r.normal(0, 0.05, st.shape))
+    vs.rA = _periodic(st, r.uniform(0.7, 1.3, st.shape) * vs.rA)
+    vs.recip_rA = 1 / vs.rA
+    f = _periodic(st, r.uniform(0, 2, st.shape))
+    out = O.calc_Advection(st, f)
+    m0 = (vs.rA * f)[INNER].sum()
+    m1 = (vs.rA * out)[INNER].sum()
+    assert abs(m1 - m0) < 1e-13 * m0
+    st2 = _adv_state(0.07, -0.03)
+    c = np.full(st2.shape, 1.7)
+    np.testing.assert_allclose(O.calc_Advection(st2, c)[INNER], 1.7, rtol=1e-15)
+
+
+def test_advection_maximum_principle_1d():
+    r = np.random.default_rng(4)
+    for u in (0.09, -0.17, 0.23):
+        st = _adv_state(u, 0.0)
+        f = _periodic(st, r.uniform(0.5, 2.5, st.shape))
+        out = O.calc_Advection(st, f)[INNER]
+        assert out.max() <= f.max() * (1 + 1e-14)
+        assert out.min() >= f.min() * (1 - 1e-14)
+
+
+def _seb_residual(T, h, hs, s, LWdown, ATemp, wSpeed, aqh, SWdown, Tfrz, south=False):
+    """Surface energy balance written from the physics: conductive flux through
+    ice and snow minus the net upward flux to the atmosphere."""
+    k_eff = s.iceConduct * s.snowConduct / (s.snowConduct * h + s.iceConduct * hs)
+    snow = hs > 0
+    emis = s.snowEmiss if snow else s.iceEmiss
+    if south:
+        alb_i, alb_s = s.dryIceAlb_south, s.drySnowAlb_south
+    else:
+        alb_i, alb_s = s.dryIceAlb, s.drySnowAlb
+    alb = alb_i if hs == 0 else (alb_s if hs > s.hCut else alb_i + hs / s.hCut * (alb_s - alb_i))
+    pen = 0.0 if snow else s.shortwave * np.exp(-1.5 * h)
+    absorbed = (1 - alb) * (1 - pen) * SWdown
+    es = 10 ** (12.537 - 2663.5 / T)                 # saturation vapour pressure [hPa-like]
+    qs = 0.622 * es / (1e5 - 0.378 * es)            # saturation specific humidity
+    u = max(wSpeed, s.wSpeedMin)
+    ta = max(ATemp, s.celsius2K + s.minTAir)
+    f_up = (emis * s.stefBoltz * T**4 - emis * max(LWdown, s.minLWdown) - absorbed
+            + s.rhoAir * s.cpAir * s.dalton * u * (T - ta)
+            + s.rhoAir * s.lhSublim * s.dalton * u * (qs - aqh))
+    return k_eff * (Tfrz - T) - f_up
+
+
+def test_solve4temp_matches_root_of_energy_balance():
+    from veris_amd.settings import make_settings
+
+    st = uniform_state(nx=6, ny=4)
+    s = st.settings
+    vs = st.variables
+    r = np.random.default_rng(6)
+    h = r.uniform(0.2, 3.0, st.shape)
+    hs = np.where(r.random(st.shape) < 0.5, 0.0, r.uniform(0.01, 0.4, st.shape))
+    vs.LWdown = r.uniform(150, 300, st.shape)
+    vs.ATemp = r.uniform(235, 270, st.shape)
+    vs.wSpeed = r.uniform(1, 12, st.shape)
+    vs.SWdown = r.uniform(0, 150, st.shape)
+    vs.aqh = r.uniform(1e-4, 2e-3, st.shape)
+    vs.fCori = np.full(st.shape, 1e-4)
+    Tfrz = np.full(st.shape, s.tempFrz + s.celsius2K)
+    Tin = np.full(st.shape, 260.0)
+    T, Fio, Fia, pen, fws = O.solve4temp(st, h, hs, Tin, Tfrz)
+    checked = 0
+    for idx in np.ndindex(st.shape):
+        args = (h[idx], hs[idx], s, vs.LWdown[idx], vs.ATemp[idx], vs.wSpeed[idx],
+                vs.aqh[idx], vs.SWdown[idx], Tfrz[idx])
+        lo, hi = s.celsius2K + s.minTIce, s.celsius2K
+        if _seb_residual(lo, *args) * _seb_residual(hi, *args) > 0:
+            continue  # root outside the clamps
+        root = brentq(_seb_residual, lo, hi, args=args, xtol=1e-12)
+        assert abs(T[idx] - root) < 1e-6, (idx, T[idx], root)
+        checked += 1
+    assert checked > 10
+
+
+def test_growth_open_water_freezing_closed_form():
+    """No ice, ocean at freezing, heat loss Qnet > 0: all heat goes into new ice,
+    h = qi Qnet dt, A = h/h0, and the residual ocean heat flux is zero."""
+    st = uniform_state(nx=6, ny=4)
+    s = st.settings
+    vs = st.variables
+    vs.fCori = np.full(st.shape, 1e-4)
+    Tfrz = s.tempFrz + s.celsius2K
+    vs.theta = np.full(st.shape, Tfrz)
+    q = 80.0
+    vs.Qnet = np.full(st.shape, q)
+    vs.Qsw = np.full(st.shape, -100.0)
+    out = dict(zip(("h", "hs", "A", "T", "EmPmR", "fss", "Qsw", "Qnet", "load", "pen", "rh"),
+                   O.Growth(st)))
+    qi = 1 / (s.rhoIce * s.lhFusion)
+    h = qi * q * s.deltatTherm
+    np.testing.assert_allclose(out["h"], h, rtol=1e-14)
+    np.testing.assert_allclose(out["A"], h / s.h0, rtol=1e-14)
+    np.testing.assert_allclose(out["Qnet"], 0.0, atol=1e-9 * q)
+    np.testing.assert_allclose(out["Qsw"], -100.0, rtol=1e-15)
+    # freezing removes fresh water: E-P-R = h rho_ice / dt (salt-free ice)
+    np.testing.assert_allclose(out["EmPmR"], h * s.rhoIce / s.deltatTherm, rtol=1e-12)
+    np.testing.assert_allclose(out["fss"], out["EmPmR"] * 34.7 / s.rhoFresh, rtol=1e-12)
+    vs.Qnet = np.full(st.shape, -q)  # heating open water: nothing forms, all heat stays
+    out2 = O.Growth(st)
+    assert np.all(out2[0] == 0) and np.all(out2[2] == 0)
+    np.testing.assert_allclose(out2[7], -q, rtol=1e-14)
+
+
+def test_growth_flooding_and_open_ocean_passthrough():
+    """Flooding: no cell ends with its snow below the waterline (draft
+    (rho_s hs + rho_i h)/rho_sea <= h).  Open ocean that stays ice free passes
+    its heat flux through unchanged."""
+    from veris_amd import synth
+
+    st = synth.config0()
+    vs = st.variables
+    s = st.settings
+    vs.hSnowMean = vs.hSnowMean * 6.0  # heavy snow: many cells flood
+    h0 = vs.hIceMean.copy()
+    h, hs, A, T, EmPmR, fss, Qsw, Qnet, load, pen, rh = O.Growth(st)
+    draft = (hs * s.rhoSnow + h * s.rhoIce) * s.recip_rhoSea
+    assert np.all(draft <= h * (1 + 1e-12) + 1e-15)
+    flooded = np.isclose(draft, h, rtol=1e-12) & (hs > 0)
+    assert flooded.sum() > 50
+    np.testing.assert_allclose(load, h * s.rhoIce + hs * s.rhoSnow, rtol=1e-15)
+    noice = (h0 == 0) | (vs.Area == 0)
+    open_ = noice & (h == 0)
+    np.testing.assert_allclose(Qnet[open_], vs.Qnet[open_], rtol=1e-12, atol=1e-9)
+
+
+def test_growth_mixed_layer_melt_is_linear_in_sst():
+    """theta enters only through the McPhee ice-ocean flux
+    F_oi = -rho_sea cp St u* (theta - Tfrz) (1 + (McPhee-1)/(1 + exp((A-0.4) 7/0.4))),
+    so two runs differing by d(theta) differ in h by -qi F_oi dt."""
+    st = uniform_state(nx=6, ny=4)
+    s = st.settings
+    vs = st.variables
+    r = np.random.default_rng(12)
+    vs.hIceMean = r.uniform(1.5, 3.0, st.shape)
+    vs.hSnowMean = r.uniform(0.0, 0.2, st.shape)
+    vs.Area = r.uniform(0.2, 1.0, st.shape)
+    vs.fCori = np.full(st.shape, 1e-4)
+    Tfrz = s.tempFrz + s.celsius2K
+    vs.theta = np.full(st.shape, Tfrz + 0.5)
+    h1 = O.Growth(st)[0]
+    d = 0.25
+    vs.theta = vs.theta + d
+    h2 = O.Growth(st)[0]
+    qi = 1 / (s.rhoIce * s.lhFusion)
+    mltf = 1 + (s.McPheeTaperFac - 1) / (1 + np.exp((vs.Area - 0.4) * (7 / 0.4)))
+    want = -qi * s.rhoSea * s.cpWater * s.stantonNr * s.uStarBase * d * mltf * s.deltatTherm
+    close((h2 - h1)[INNER], want[INNER], 1e-9)
+
+
+def test_viscosity_shear_uses_north_east_corners():
+    """e12 varying in x and y: the c-point shear uses the area-weighted mean of
+    e12^2 over the four corners (i..i+1, j..j+1) of the cell."""
+    st = uniform_state(nx=10, ny=8)
+    x, y, dx, dy = xy(st)
+    e12 = 1e-7 + 3e-13 * x - 2e-13 * y      # z-point values
+    z, _, p, d = O.viscosities(st, np.zeros(st.shape), np.zeros(st.shape), e12,
+                               P=np.full(st.shape, 1e4))
+    corners = e12**2 + np.roll(e12, -1, 0)**2 + np.roll(e12, -1, 1)**2 \
+        + np.roll(np.roll(e12, -1, 0), -1, 1)**2
+    e12c = 0.25 * (0.5 * dx * dy) / (dx * dy) * corners   # rAz = rA/2 (plugin grid)
+    close(d[INNER], np.sqrt(0.25 * 4 * e12c)[INNER], 1e-13)
+
+
+def test_no_slip_term_position_v():
+    """A land column west of column i0 adds 2*v/dx at z-points of column i0."""
+    st = uniform_state(nx=10, ny=8)
+    i0 = 6
+    st.variables.iceMaskV[:i0, :] = 0.0
+    v0 = -0.3
+    _, _, e12 = O.strainrates(st, np.zeros(st.shape), np.full(st.shape, v0))
+    dx = st.variables.dxV[0, 0]
+    np.testing.assert_allclose(e12[i0, 2:-2], 2 * v0 / dx, rtol=1e-14)
+    others = [ii for ii in range(2, st.shape[0] - 2) if ii != i0]
+    assert np.all(e12[others, 2:-2] == 0)
+
+
+def test_momentum_with_coriolis_complex_recurrence():
+    """Uniform fields, P = 0, cDragMin regime, f != 0: z = u + iv obeys
+    z_{k+1} = lam z_k + c, lam = (beta m/dt - i m f)/den, so
+    z_k = z* + (z0 - z*) lam^k with z* = c/(1 - lam)."""
+    st = _rest_state()
+    s = st.settings
+    vs = st.variables
+    f = 1.4e-4
+    vs.fCori = np.full(st.shape, f)
+    m, A = 900.0, 1.0
+    tau = (2e-3, -1e-3)
+    vs.WindForcingX = np.full(st.shape, tau[0])
+    vs.WindForcingY = np.full(st.shape, tau[1])
+    oc = (0.01, 0.005)
+    vs.uOcean = np.full(st.shape, oc[0])
+    vs.vOcean = np.full(st.shape, oc[1])
+    z0 = 0.012 - 0.009j
+    vs.uIce = np.full(st.shape, z0.real)
+    vs.vIce = np.full(st.shape, z0.imag)
+    dt, beta, C = s.deltatDyn, s.evpBeta, s.cDragMin
+    den = m * (beta + 1) / dt + C * A
+    lam = (beta * m / dt - 1j * m * f) / den
+    c = (m * z0 / dt + complex(*tau) + C * A * complex(*oc)) / den
+    zs = c / (1 - lam)
+    for k in (1, 7, 150):
+        u, v, *_ = O.evp_solver(st, n_substeps=k)
+        want = zs + (z0 - zs) * lam**k
+        np.testing.assert_allclose(u[INNER], want.real, rtol=1e-11)
+        np.testing.assert_allclose(v[INNER], want.imag, rtol=1e-11)
+
+
+def test_momentum_drag_fixed_point_with_varying_drag():
+    """P = 0, f = 0: at convergence each u-point satisfies the free-drift
+    balance (m/dt + A cU) u = m u0/dt + tau + A cU uo, with cU the mean of the
+    water drag max(D |u - uo|_C, cDragMin) of the two c-points west and east of
+    the face; the drag coefficient differs by hemisphere across x."""
+    st = _rest_state(waterIceDrag=0.0055, waterIceDrag_south=0.0025)
+    s = st.settings
+    vs = st.variables
+    NX, NY = st.shape
+    vs.fCori = np.where(np.arange(NX)[:, None] < NX // 2, -1e-30, 1e-30) * np.ones(st.shape)
+    r = np.random.default_rng(21)
+    tau = O.fill_overlap(st, r.uniform(0.05, 0.2, st.shape))
+    vs.WindForcingX = tau
+    vs.WindForcingY = np.zeros(st.shape)
+    vs.uOcean = np.zeros(st.shape)
+    vs.vOcean = np.zeros(st.shape)
+    m = 900.0
+    u, v, *_ = O.evp_solver(st, n_substeps=3000)
+    D = np.where(vs.fCori < 0, s.waterIceDrag_south, s.waterIceDrag) * s.rhoSea
+    du = u - vs.uOcean
+    dvc = v - vs.vOcean
+    spd = 0.5 * np.hypot(du + np.roll(du, -1, 0), dvc + np.roll(dvc, -1, 1))
+    cC = np.maximum(D * spd, s.cDragMin)
+    cU = 0.5 * (cC + np.roll(cC, 1, 0))
+    dt = s.deltatDyn
+    res = (m / dt + cU) * u - tau   # u0 = uOcean = 0
+    assert np.max(np.abs(u[INNER])) > 0.05
+    assert np.max(np.abs(res[INNER])) < 1e-9 * np.max(tau)
+
+
+def test_clean_up_cases():
+    st = uniform_state(nx=4, ny=4)
+    vs = st.variables
+    s = st.settings
+    vs.hIceMean = np.zeros(st.shape)
+    vs.hSnowMean = np.zeros(st.shape)
+    vs.Area = np.zeros(st.shape)
+    vs.TSurf = np.full(st.shape, 260.0)
+    cases = [  # (h, hs, A) -> (h, hs, A, T, os_h, os_hs)
+        ((-0.2, 0.1, 0.5), (0.0, 0.0, 0.0, s.celsius2K, 0.2, 0.0)),
+        ((0.5, -0.1, 0.5), (0.5, 0.0, 0.5, 260.0, 0.0, 0.1)),
+        ((5e-6, 0.3, 0.5), (0.0, 0.0, 0.0, s.celsius2K, 0.0, 0.0)),
+        ((0.4, 0.0, 0.0), (0.4, 0.0, s.Area_min, 260.0, 0.0, 0.0)),
+        ((0.4, 0.1, 1e-7), (0.4, 0.1, s.Area_min, 260.0, 0.0, 0.0)),
+        ((0.0, 0.0, 0.3), (0.0, 0.0, 0.0, s.celsius2K, 0.0, 0.0)),
+        ((1.0, 0.2, -0.1), (1.0, 0.2, s.Area_min, 260.0, 0.0, 0.0)),
+        ((1e-5, 0.1, 0.4), (0.0, 0.0, 0.0, s.celsius2K, 0.0, 0.0)),  # h == hIce_min is thin
+    ]
+    for q, (inp, _) in enumerate(cases):
+        vs.hIceMean[2 + q // 4, 2 + q % 4] = inp[0]
+        vs.hSnowMean[2 + q // 4, 2 + q % 4] = inp[1]
+        vs.Area[2 + q // 4, 2 + q % 4] = inp[2]
+    out = O.clean_up_advection(st)
+    for q, (_, want) in enumerate(cases):
+        got = tuple(float(o[2 + q // 4, 2 + q % 4]) for o in out)
+        np.testing.assert_allclose(got, want, rtol=1e-15, atol=1e-18, err_msg=str(q))
+    vs.Area = np.full(st.shape, 1.3)
+    assert np.all(O.ridging(st) == 1.0)
+
+
+def test_fill_overlap_index_identity():
+    st = uniform_state(nx=9, ny=7)
+    a = np.random.default_rng(8).normal(size=st.shape)
+    b = O.fill_overlap(st, a)
+    nx = st.settings.nx
+    np.testing.assert_array_equal(b[:2], a[nx: nx + 2])
+    np.testing.assert_array_equal(b[nx + 2:], a[2:4])
+    np.testing.assert_array_equal(b[2: nx + 2], a[2: nx + 2])
+    st.settings.enable_cyclic_x = False
+    np.testing.assert_array_equal(O.fill_overlap(st, a), a)
+    st.settings.veros_fill = False
+    c = O.fill_overlap(st, a)
+    ny = st.settings.ny
+    np.testing.assert_array_equal(c[:, :2], c[:, ny: ny + 2])
+    np.testing.assert_array_equal(c[:2, 2:-2], a[nx: nx + 2, 2:-2])
+
+
+def test_mass_strength_and_wind_closed_forms():
+    st = uniform_state(nx=6, ny=5)
+    vs = st.variables
+    s = st.settings
+    vs.hIceMean = np.full(st.shape, 1.5)
+    vs.hSnowMean = np.full(st.shape, 0.2)
+    vs.Area = np.full(st.shape, 1.0)
+    mC, mU, mV = O.SeaIceMass(st)
+    np.testing.assert_allclose(mC, 1.5 * s.rhoIce + 0.2 * s.rhoSnow, rtol=1e-15)
+    np.testing.assert_allclose(mU, mC, rtol=1e-15)
+    np.testing.assert_allclose(O.SeaIceStrength(st), s.pStar * 1.5, rtol=1e-15)
+    vs.Area = np.full(st.shape, 0.8)
+    np.testing.assert_allclose(O.SeaIceStrength(st), s.pStar * 1.5 * np.exp(-s.cStar * 0.2),
+                               rtol=1e-14)
+    # wind stress at rest: |tau| = rhoAir * Cd * |U|^2, along the wind
+    vs.uWind = np.full(st.shape, 6.0)
+    vs.vWind = np.full(st.shape, -8.0)
+    vs.fCori = np.full(st.shape, 1e-4)
+    tx, ty = O.tauXY(st)
+    mag = s.rhoAir * s.airIceDrag * 10.0
+    np.testing.assert_allclose(tx[INNER], mag * 6.0, rtol=1e-14)
+    np.testing.assert_allclose(ty[INNER], mag * -8.0, rtol=1e-14)
+    # surface tilt: -m g d(ssh)/dx
+    vs.uWind[:] = 0.0
+    vs.vWind[:] = 0.0
+    vs.AreaW = np.ones(st.shape)
+    vs.AreaS = np.ones(st.shape)
+    vs.SeaIceMassU, vs.SeaIceMassV = mU, mV
+    x, y, dx, dy = xy(st)
+    vs.ssh_an = 1e-6 * (x + 0.5 * dx) - 2e-6 * (y + 0.5 * dy)
+    fx, fy = O.WindForcingXY(st)
+    np.testing.assert_allclose(fx[INNER], -mU[INNER] * s.gravity * 1e-6, rtol=1e-12)
+    np.testing.assert_allclose(fy[INNER], mV[INNER] * s.gravity * 2e-6, rtol=1e-12)
+
+
+def test_staggered_averages_land_at_their_points():
+    """Two-point averages to u/v points and the four-point average to z-points
+    reproduce a linear field at the target point (face / corner)."""
+    st = uniform_state(nx=10, ny=9, dx=2e4, dy=3e4)
+    x, y, dx, dy = xy(st)
+    vs = st.variables
+    s = st.settings
+    lin = lambda X, Y: 1.0 + 2e-6 * X - 3e-6 * Y  # noqa: E731
+    xc, yc = x + 0.5 * dx, y + 0.5 * dy
+    vs.hIceMean = lin(xc, yc)
+    vs.hSnowMean = np.zeros(st.shape)
+    vs.Area = lin(xc, yc)
+    mC, mU, mV = O.SeaIceMass(st)
+    close(mU[INNER], (s.rhoIce * lin(x, yc))[INNER], 1e-14)     # west face
+    close(mV[INNER], (s.rhoIce * lin(xc, y))[INNER], 1e-14)     # south face
+    aw, as_ = O.AreaWS(st)
+    close(aw[INNER], lin(x, yc)[INNER], 1e-14)
+    close(as_[INNER], lin(xc, y)[INNER], 1e-14)
+    close(O.c_point_to_z_point(st, lin(xc, yc))[INNER], lin(x, y)[INNER], 1e-14)
+
+
+def test_strain_rate_z_mask_covers_land_cell_corners():
+    """One land c-cell zeroes e12 exactly at its four corners."""
+    st = uniform_state(nx=10, ny=9)
+    x, y, dx, dy = xy(st)
+    u = 3e-12 * (y + 0.5 * dy) ** 2
+    v = 5e-12 * (x + 0.5 * dx) ** 2
+    _, _, ref = O.strainrates(st, u, v)
+    i0, j0 = 6, 5
+    st.variables.iceMask[i0, j0] = 0.0
+    _, _, e12 = O.strainrates(st, u, v)
+    corners = {(i0, j0), (i0 + 1, j0), (i0, j0 + 1), (i0 + 1, j0 + 1)}
+    for ii in range(2, st.shape[0] - 2):
+        for jj in range(2, st.shape[1] - 2):
+            want = 0.0 if (ii, jj) in corners else ref[ii, jj]
+            assert e12[ii, jj] == want, (ii, jj)
+
+
+def test_water_drag_uses_cell_face_velocities():
+    """c-point drag speed from the mean of the two u-faces (west, east) and the
+    two v-faces (south, north) of the cell, for velocities varying in x and y."""
+    st = uniform_state(nx=10, ny=9, dx=2e4, dy=3e4)
+    x, y, dx, dy = xy(st)
+    s = st.settings
+    st.variables.fCori = np.full(st.shape, 1e-4)
+    u = 0.1 + 1e-6 * x + 2e-6 * (y + 0.5 * dy)
+    v = -0.2 + 3e-6 * (x + 0.5 * dx) - 1e-6 * y
+    cd = O.ocean_drag_coeffs(st, u, v)
+    xc, yc = x + 0.5 * dx, y + 0.5 * dy
+    uc = 0.1 + 1e-6 * xc + 2e-6 * yc
+    vc = -0.2 + 3e-6 * xc - 1e-6 * yc
+    want = np.maximum(s.waterIceDrag * s.rhoSea * np.hypot(uc, vc), s.cDragMin)
+    close(cd[INNER], want[INNER], 1e-13)
+
+
+def test_coriolis_uses_four_point_velocity_average():
+    """One substep from u varying in x and y (P = 0, cDragMin regime): the v
+    update carries -m f times the mean of the four u-points around the v-point
+    (west/east faces of the cells north and south of it), and vice versa."""
+    st = _rest_state()
+    s = st.settings
+    vs = st.variables
+    x, y, dx, dy = xy(st)
+    f, m = 1.3e-4, 900.0
+    vs.fCori = np.full(st.shape, f)
+    u = 0.01 + 2e-8 * x + 3e-8 * (y + 0.5 * dy)
+    v = -0.005 + 1e-8 * (x + 0.5 * dx) - 2e-8 * y
+    vs.uIce, vs.vIce = u, v
+    u1, v1, *_ = O.evp_solver(st, n_substeps=1)
+    dt, beta, C = s.deltatDyn, s.evpBeta, s.cDragMin
+    den = m * (beta + 1) / dt + C
+    u_at_v = 0.01 + 2e-8 * (x + 0.5 * dx) + 3e-8 * y      # linear: 4-point mean is exact
+    v_at_u = -0.005 + 1e-8 * x - 2e-8 * (y + 0.5 * dy)
+    want_v = (beta * m / dt * v + m / dt * v - m * f * u_at_v) / den
+    want_u = (beta * m / dt * u + m / dt * u + m * f * v_at_u) / den
+    close(v1[INNER], want_v[INNER], 1e-12)
+    close(u1[INNER], want_u[INNER], 1e-12)
+
+
+def test_stress_divergence_accelerates_down_gradient():
+    """From rest with P = 0 (zeta = 0), sigma1 = sigma2 = L(x) relaxes to
+    (1-1/alpha) L and the first velocity is u = (1-1/alpha) dL/dx / den."""
+    st = _rest_state()
+    s = st.settings
+    vs = st.variables
+    x, y, dx, dy = xy(st)
+    L = 400.0 + 0.02 * (x + 0.5 * dx)
+    vs.sigma1 = L.copy()
+    vs.sigma2 = L.copy()
+    u1, v1, a1, a2, _ = O.evp_solver(st, n_substeps=1)
+    m, dt, beta, C = 900.0, s.deltatDyn, s.evpBeta, s.cDragMin
+    den = m * (beta + 1) / dt + C
+    fac = 1 - 1 / s.evpAlpha
+    close(u1[INNER], np.full(u1[INNER].shape, fac * 0.02 / den), 1e-12)
+    assert np.max(np.abs(v1[INNER])) < 1e-12 * abs(fac * 0.02 / den)
+
+
+def test_stress_relaxation_targets_viscous_plastic_law():
+    """One substep relaxes each stress toward its viscous-plastic value with
+    weight 1/alpha: sigma1* = 2 zeta (e11+e22) - P_r, sigma2* = 2 eta (e11-e22),
+    sigma12* = 2 eta_z e12 with eta = zeta/e^2 (e = PlasDefCoeff)."""
+    from veris_amd import synth
+
+    st = synth.tiny(16, 12)
+    vs = st.variables
+    s = st.settings
+    r = np.random.default_rng(31)
+    from gpu_helpers import pre_evp
+
+    pre_evp(st)
+    vs.uIce = O.fill_overlap(st, r.normal(0, 0.05, st.shape) * vs.iceMaskU)
+    vs.vIce = O.fill_overlap(st, r.normal(0, 0.05, st.shape) * vs.iceMaskV)
+    for k in ("sigma1", "sigma2", "sigma12"):
+        vs[k] = r.normal(0, 500, st.shape)
+    tr = {}
+    _, _, a1, a2, a12 = O.evp_substep(st, vs.uIce, vs.vIce, vs.uIce, vs.vIce, vs.sigma1,
+                                      vs.sigma2, vs.sigma12, trace=tr)
+    al, e2 = s.evpAlpha, s.PlasDefCoeff**2
+    eta = tr["zeta"] / e2
+    etaz = O.c_point_to_z_point(st, tr["zeta"]) / e2
+    t1 = 2 * tr["zeta"] * (tr["e11"] + tr["e22"]) - tr["press"]
+    t2 = 2 * eta * (tr["e11"] - tr["e22"])
+    t12 = 2 * etaz * tr["e12"]
+    M = vs.iceMask
+    close(a1[INNER], ((vs.sigma1 * (1 - 1 / al) + t1 / al) * M)[INNER], 1e-12)
+    close(a2[INNER], ((vs.sigma2 * (1 - 1 / al) + t2 / al) * M)[INNER], 1e-12)
+    close(a12[INNER], (vs.sigma12 * (1 - 1 / al) + t12 / al)[INNER], 1e-12)
+
+
+def _transpose_state(st):
+    """x <-> y mirror of a square-grid state (u <-> v, masks U <-> V, dx <-> dy)."""
+    from veris_amd.state import VerisState
+    from veris_amd.settings import SettingsNamespace
+    from veris_amd.variables import VariablesNamespace
+
+    swap = {"uIce": "vIce", "uOcean": "vOcean", "iceMaskU": "iceMaskV", "maskInU": "maskInV",
+            "SeaIceMassU": "SeaIceMassV", "AreaW": "AreaS", "WindForcingX": "WindForcingY",
+            "dxC": "dyC", "dxG": "dyG", "dxU": "dyV", "dyU": "dxV", "rAu": "rAv",
+            "k1AtC": "k2AtC", "k1AtZ": "k2AtZ"}
+    for a, b in list(swap.items()):
+        swap[b] = a
+    for k in ("dxC", "dyC", "dxG", "dyG", "dxU", "dyU", "dxV", "dyV", "rAu", "rAv"):
+        swap["recip_" + k] = "recip_" + swap[k]
+    vals = {}
+    for k, a in st.variables.items():
+        vals[swap.get(k, k)] = np.ascontiguousarray(a.T)
+    sett = SettingsNamespace(st.settings.as_dict())
+    sett.nx, sett.ny = st.settings.ny, st.settings.nx
+    return VerisState(sett, VariablesNamespace(vals))
+
+
+def test_evp_and_advection_transposition_symmetry():
+    """f = 0, square uniform grid, closed domain: transposing the inputs
+    transposes the EVP velocities (u <-> v), sigma1, sigma12 and flips sigma2;
+    advection by v alone equals advection by u alone of the transposed field."""
+    st = uniform_state(nx=14, ny=14, dx=2e4, dy=2e4, cyclic=False)
+    vs = st.variables
+    r = np.random.default_rng(41)
+    land = r.random(st.shape) < 0.2
+    M = np.where(land, 0.0, 1.0)
+    M[:2] = M[-2:] = 0.0
+    M[:, :2] = M[:, -2:] = 0.0
+    vs.iceMask = M
+    vs.maskInC = M
+    vs.iceMaskU = M * np.roll(M, 1, 0)
+    vs.iceMaskV = M * np.roll(M, 1, 1)
+    vs.maskInU, vs.maskInV = vs.iceMaskU, vs.iceMaskV
+    vs.hIceMean = r.uniform(0.5, 2.5, st.shape) * M
+    vs.hSnowMean = r.uniform(0, 0.3, st.shape) * M
+    vs.Area = r.uniform(0.3, 1.0, st.shape) * M
+    vs.uWind = r.normal(0, 6, st.shape)
+    vs.vWind = r.normal(0, 6, st.shape)
+    vs.uOcean = r.normal(0, 0.1, st.shape) * vs.iceMaskU
+    vs.vOcean = r.normal(0, 0.1, st.shape) * vs.iceMaskV
+    from gpu_helpers import pre_evp
+
+    pre_evp(st)
+    tt = _transpose_state(st)
+    out = O.evp_solver(st, n_substeps=6)
+    outT = O.evp_solver(tt, n_substeps=6)
+    close(outT[1].T, out[0], 1e-11)
+    close(outT[0].T, out[1], 1e-11)
+    close(outT[2].T, out[2], 1e-11)
+    close(-outT[3].T, out[3], 1e-11)
+    close(outT[4].T, out[4], 1e-11)
+    # advection: v-only vs u-only of the transposed state
+    vs.uIce = np.zeros(st.shape)
+    vs.vIce = r.normal(0, 0.1, st.shape) * vs.iceMaskV
+    tt = _transpose_state(st)
+    f = vs.hIceMean
+    np.testing.assert_array_equal(O.calc_Advection(tt, f.T).T, O.calc_Advection(st, f))
+
+
+def test_growth_all_snow_melted_closed_form():
+    """Thick ice under snow, strong surface heating, no conductive flux to the
+    ocean: the snow melts completely and the heat left after melting it,
+    F_ia + hs/(qs dt), melts ice at rate qi per unit flux (scaled by the
+    regularised area factor A/sqrt(A^2 + Area_reg))."""
+    st = uniform_state(nx=4, ny=4)
+    s = st.settings
+    vs = st.variables
+    vs.fCori = np.full(st.shape, 1e-4)
+    A, h, hs = 1.0, 2.0, 0.01
+    vs.Area = np.full(st.shape, A)
+    vs.hIceMean = np.full(st.shape, h)
+    vs.hSnowMean = np.full(st.shape, hs)
+    vs.TSurf = np.full(st.shape, 273.0)
+    vs.ATemp = np.full(st.shape, 285.0)
+    vs.LWdown = np.full(st.shape, 340.0)
+    vs.SWdown = np.full(st.shape, 400.0)
+    vs.wSpeed = np.full(st.shape, 10.0)
+    vs.aqh = np.full(st.shape, 8e-3)
+    Tfrz = s.tempFrz + s.celsius2K
+    vs.theta = np.full(st.shape, Tfrz)
+    vs.ocSalt = np.full(st.shape, 34.7)
+    out = O.Growth(st)
+    # surface flux of the categories, from solve4temp (pinned by the root test)
+    regA = np.sqrt(A**2 + s.Area_reg)
+    fia = 0.0
+    for l in range(s.nITC):
+        p = (2 * l + 1) * s.recip_nITC
+        _, fio_l, fia_l, _, _ = O.solve4temp(st, np.full(st.shape, max(h / regA, 0.05) * p),
+                                             np.full(st.shape, hs / regA * p), vs.TSurf,
+                                             np.full(st.shape, Tfrz))
+        assert np.all(fio_l == 0)
+        fia = fia + fia_l / s.nITC
+    fia = fia * A
+    qi = 1 / (s.rhoIce * s.lhFusion)
+    qs = 1 / (s.rhoSnow * s.lhFusion)
+    dt = s.deltatTherm
+    assert np.all(-fia * qs * dt >= hs)                 # heating melts all snow
+    h_new = h + (fia + hs / (qs * dt)) * qi / regA * A * dt
+    assert np.all(out[1][INNER] == 0)   # no snow left
+    close(out[0][INNER], h_new[INNER], 1e-12)
+
+
+def test_strainrates_metric_terms():
+    """Uniform U, V with metric terms k1, k2: e11 = V k2C, e22 = U k1C,
+    e12 = -(k1Z V + k2Z U)/2."""
+    st = uniform_state(nx=8, ny=7)
+    vs = st.variables
+    U, V = 0.3, -0.2
+    vs.k1AtC = np.full(st.shape, 2e-7)
+    vs.k2AtC = np.full(st.shape, -5e-7)
+    vs.k1AtZ = np.full(st.shape, 3e-7)
+    vs.k2AtZ = np.full(st.shape, 7e-7)
+    e11, e22, e12 = O.strainrates(st, np.full(st.shape, U), np.full(st.shape, V))
+    np.testing.assert_allclose(e11[INNER], V * -5e-7, rtol=1e-14)
+    np.testing.assert_allclose(e22[INNER], U * 2e-7, rtol=1e-14)
+    np.testing.assert_allclose(e12[INNER], -0.5 * (3e-7 * V + 7e-7 * U), rtol=1e-14)
+
+
+def _linear_velocity_fields(st, a=(0.1, 2e-6, -1e-6), b=(-0.05, 1e-6, 3e-6)):
+    x, y, dx, dy = xy(st)
+    fu = lambda X, Y: a[0] + a[1] * X + a[2] * Y  # noqa: E731
+    fv = lambda X, Y: b[0] + b[1] * X + b[2] * Y  # noqa: E731
+    u = fu(x, y + 0.5 * dy)
+    v = fv(x + 0.5 * dx, y)
+    uc = fu(x + 0.5 * dx, y + 0.5 * dy)
+    vc = fv(x + 0.5 * dx, y + 0.5 * dy)
+    return u, v, uc, vc
+
+
+def test_wind_stress_relative_to_ice_face_average():
+    """Relative wind at c-points from the face-averaged ice velocity; stress
+    rhoAir Cd |U_rel| U_rel averaged to the u- and v-faces."""
+    st = uniform_state(nx=10, ny=9, dx=2e4, dy=3e4)
+    vs = st.variables
+    s = st.settings
+    vs.fCori = np.full(st.shape, 1e-4)
+    u, v, uc, vc = _linear_velocity_fields(st)
+    vs.uIce, vs.vIce = u, v
+    vs.uWind = np.full(st.shape, 7.0)
+    vs.vWind = np.full(st.shape, -3.0)
+    tx, ty = O.tauXY(st)
+    ur, vr = 7.0 - uc, -3.0 - vc
+    sp = np.hypot(ur, vr)
+    tcx = s.rhoAir * s.airIceDrag * sp * ur
+    tcy = s.rhoAir * s.airIceDrag * sp * vr
+    close(tx[INNER], (0.5 * (tcx + np.roll(tcx, 1, 0)))[INNER], 1e-13)
+    close(ty[INNER], (0.5 * (tcy + np.roll(tcy, 1, 1)))[INNER], 1e-13)
+
+
+def test_ocean_stress_face_average_of_drag():
+    """Ice-ocean stress at a u-face: mean of the two adjacent c-point drag
+    coefficients (west/east cells) times the face velocity difference."""
+    st = uniform_state(nx=10, ny=9, dx=2e4, dy=3e4)
+    vs = st.variables
+    s = st.settings
+    vs.fCori = np.full(st.shape, 1e-4)
+    u, v, uc, vc = _linear_velocity_fields(st, a=(0.4, 2e-6, -1e-6), b=(-0.3, 1e-6, 3e-6))
+    vs.uIce, vs.vIce = u, v
+    tu, tv = O.OceanStressUV(st)
+    cC = np.maximum(s.waterIceDrag * s.rhoSea * np.hypot(uc, vc), s.cDragMin)
+    close(tu[INNER], (0.5 * (cC + np.roll(cC, 1, 0)) * u)[INNER], 1e-13)
+    close(tv[INNER], (0.5 * (cC + np.roll(cC, 1, 1)) * v)[INNER], 1e-13)

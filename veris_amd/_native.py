@@ -72,7 +72,7 @@ SIGNATURES = {
     "vh_seaice_strength": [_DP, _SP, _FP, _P, _P, _P],
     "vh_wind_forcing_xy": [_DP, _SP, _FP, _P, _P, _P],
     "vh_evp_tile_count": [],
-    "vh_evp_tile_shape": [_I32, ctypes.POINTER(_I32), ctypes.POINTER(_I32)],
+    "vh_evp_tile_shape": [_I32, ctypes.POINTER(_I32), ctypes.POINTER(_I32), ctypes.POINTER(_I32)],
     "vh_evp_workspace_bytes": [_DP, ctypes.POINTER(_SZ)],
     "vh_evp_solver": [_DP, _SP, _FP, _I32, _P, _P, _P, _P, _P, _P, _SZ, _P],
     "vh_ocean_stress_uv": [_DP, _SP, _FP, _P, _P, _P],
@@ -134,12 +134,12 @@ def domain_struct(sett, dtype_code: int, tile: int = 0) -> Domain:
 
 
 def evp_tile_shapes() -> dict:
-    """EVP tile index -> (ri, rj) workgroup shape."""
+    """EVP tile index -> (ri, rj, k): workgroup shape and substeps per launch."""
     out = {}
-    for k in range(1, lib().vh_evp_tile_count() + 1):
-        ri, rj = _I32(0), _I32(0)
-        call("vh_evp_tile_shape", k, ctypes.byref(ri), ctypes.byref(rj))
-        out[k] = (ri.value, rj.value)
+    for t in range(1, lib().vh_evp_tile_count() + 1):
+        ri, rj, k = _I32(0), _I32(0), _I32(0)
+        call("vh_evp_tile_shape", t, ctypes.byref(ri), ctypes.byref(rj), ctypes.byref(k))
+        out[t] = (ri.value, rj.value, k.value)
     return out
 
 

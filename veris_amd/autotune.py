@@ -1,0 +1,57 @@
+"""Tile-shape autotuner of the EVP substep kernel.
+
+The best workgroup shape depends on the domain (1 degree is latency bound at
+about one wave per SIMD; ensembles of large members are bandwidth bound), so
+it is measured rather than guessed: each candidate shape runs a short
+subcycle on the caller's own state, timed with HIP events on the current
+stream, and the fastest one is remembered per (shape, dtype) in
+:data:`veris_amd.kernels.EVP_TILE`.
+
+Exact shapes (one substep per launch) reproduce the reference on every cell
+and are the default candidates; multi-substep shapes (``exact=False``) also
+reproduce every velocity but return periodic values instead of the
+reference's wrap-around artefacts in the stresses of the periodic overlap
+columns.
+"""
+
+from __future__ import annotations
+
+import torch
+
+from . import _native as N
+from . import kernels as K
+
+
+def candidate_tiles(exact: bool = True) -> list[int]:
+    shapes = N.evp_tile_shapes()
+    return [t for t, (_, _, k) in shapes.items() if (k == 1 or not exact)]
+
+
+def time_evp(state, tile: int, n_substeps: int = 100, repeats: int = 3) -> float:
+    """Seconds per substep of ``tile`` on ``state`` (best of ``repeats``)."""
+    stream = torch.cuda.current_stream()
+    best = float("inf")
+    K.evp_solver(state, n_substeps, tile=tile)  # warm-up (and graph capture)
+    for _ in range(repeats):
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        K.evp_solver(state, n_substeps, tile=tile)
+        b.record(stream)
+        b.synchronize()
+        best = min(best, a.elapsed_time(b) * 1e-3 / n_substeps)
+    return best
+
+
+def tune_evp(state, exact: bool = True, n_substeps: int = 100, verbose: bool = False) -> int:
+    """Measure the candidate shapes on ``state``, remember and return the best."""
+    times = {}
+    for t in candidate_tiles(exact):
+        times[t] = time_evp(state, t, n_substeps)
+        if verbose:
+            print(f"tile {t} {N.evp_tile_shapes()[t]}: {times[t] * 1e6:.2f} us/substep")
+    best = min(times, key=times.get)
+    ref = state.variables.iceMask
+    shape = tuple(ref.shape)
+    K.EVP_TILE[(shape, ref.dtype)] = best
+    return best

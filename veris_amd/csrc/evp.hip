@@ -20,43 +20,64 @@ namespace vh {
 namespace {
 
 struct TileCfg {
-  int ri, rj;
+  int ri, rj, k;  // workgroup shape and substeps per launch
 };
 // index 0 = library default (chosen from the domain size)
-constexpr TileCfg kTiles[] = {{0, 0}, {8, 64}, {12, 64}, {16, 64}, {8, 32}, {16, 32}, {32, 32}, {6, 64}};
+constexpr TileCfg kTiles[] = {{0, 0, 0},   {8, 64, 1},  {12, 64, 1}, {16, 64, 1}, {8, 32, 1},
+                              {16, 32, 1}, {32, 32, 1}, {6, 64, 1},  {16, 64, 2}, {32, 32, 2},
+                              {12, 64, 2}, {32, 32, 3}, {16, 64, 3}, {32, 32, 4}, {16, 32, 2},
+                              {24, 32, 2}};
 constexpr int kNumTiles = sizeof(kTiles) / sizeof(kTiles[0]);
 
 int pick_tile(const Geom& g, int req) {
   if (req > 0 && req < kNumTiles) return req;
   const long long cells = (long long)g.NX * g.NY;
-  return cells < 400000 ? 1 : 3;
+  return cells < 400000 ? 5 : 3;
 }
 
-template <typename T, typename TD, int RI, int RJ>
-void launch_one(const EvpArgBlock<T>* ab, int k, const Geom& g, hipStream_t st) {
-  constexpr int TI = RI - 4, TJ = RJ - 4;
+template <typename T, typename TD, int RI, int RJ, int K>
+void launch_one(const EvpArgBlock<T>* ab, int launch, const Geom& g, hipStream_t st) {
+  constexpr int TI = RI - 4 * K, TJ = RJ - 4 * K;
   const int tiles_i = (g.NX + TI - 1) / TI, tiles_j = (g.NY + TJ - 1) / TJ;
-  const int per_xcd = (tiles_i * tiles_j + 7) / 8;
-  hipLaunchKernelGGL((evp_substep_kernel<T, TD, RI, RJ, false>), dim3(per_xcd * 8), dim3(RI * RJ),
-                     0, st, ab, k, tiles_j, per_xcd);
+  const int n_tiles = tiles_i * tiles_j;
+  const int per_xcd = (n_tiles + 7) / 8;
+  hipLaunchKernelGGL((evp_substep_kernel<T, TD, RI, RJ, K, false>), dim3(per_xcd * 8),
+                     dim3(RI * RJ), 0, st, ab, launch, n_tiles, tiles_j, per_xcd);
 }
 
 template <typename T, typename TD>
-void launch_substep(int cfg, const EvpArgBlock<T>* ab, int k, const Geom& g, hipStream_t st) {
+void launch_cfg(int cfg, const EvpArgBlock<T>* ab, int launch, const Geom& g, hipStream_t st) {
   switch (cfg) {
-    case 1: launch_one<T, TD, 8, 64>(ab, k, g, st); break;
-    case 2: launch_one<T, TD, 12, 64>(ab, k, g, st); break;
-    case 3: launch_one<T, TD, 16, 64>(ab, k, g, st); break;
-    case 4: launch_one<T, TD, 8, 32>(ab, k, g, st); break;
-    case 5: launch_one<T, TD, 16, 32>(ab, k, g, st); break;
-    case 6: launch_one<T, TD, 32, 32>(ab, k, g, st); break;
-    default: launch_one<T, TD, 6, 64>(ab, k, g, st); break;
+    case 1: launch_one<T, TD, 8, 64, 1>(ab, launch, g, st); break;
+    case 2: launch_one<T, TD, 12, 64, 1>(ab, launch, g, st); break;
+    case 3: launch_one<T, TD, 16, 64, 1>(ab, launch, g, st); break;
+    case 4: launch_one<T, TD, 8, 32, 1>(ab, launch, g, st); break;
+    case 5: launch_one<T, TD, 16, 32, 1>(ab, launch, g, st); break;
+    case 6: launch_one<T, TD, 32, 32, 1>(ab, launch, g, st); break;
+    case 7: launch_one<T, TD, 6, 64, 1>(ab, launch, g, st); break;
+    case 8: launch_one<T, TD, 16, 64, 2>(ab, launch, g, st); break;
+    case 9: launch_one<T, TD, 32, 32, 2>(ab, launch, g, st); break;
+    case 10: launch_one<T, TD, 12, 64, 2>(ab, launch, g, st); break;
+    case 11: launch_one<T, TD, 32, 32, 3>(ab, launch, g, st); break;
+    case 12: launch_one<T, TD, 16, 64, 3>(ab, launch, g, st); break;
+    case 13: launch_one<T, TD, 32, 32, 4>(ab, launch, g, st); break;
+    case 14: launch_one<T, TD, 16, 32, 2>(ab, launch, g, st); break;
+    default: launch_one<T, TD, 24, 32, 2>(ab, launch, g, st); break;
   }
 }
 
-int n_tiles_of(int cfg, const Geom& g) {
-  const int TI = kTiles[cfg].ri - 4, TJ = kTiles[cfg].rj - 4;
-  return ((g.NX + TI - 1) / TI) * ((g.NY + TJ - 1) / TJ);
+// A subcycle of nsub substeps with configuration cfg is nsub/K launches of
+// cfg followed by nsub%K single-substep launches of a K=1 shape.
+constexpr int kRemainderCfg = 5;
+
+int n_launches(int cfg, int nsub) { return nsub / kTiles[cfg].k + nsub % kTiles[cfg].k; }
+
+template <typename T, typename TD>
+void launch_substeps(int cfg, const EvpArgBlock<T>* ab, int nsub, const Geom& g, hipStream_t st) {
+  const int K = kTiles[cfg].k;
+  int l = 0;
+  for (int q = 0; q < nsub / K; ++q) launch_cfg<T, TD>(cfg, ab, l++, g, st);
+  for (int q = 0; q < nsub % K; ++q) launch_cfg<T, TD>(kRemainderCfg, ab, l++, g, st);
 }
 
 // ---- graph cache
@@ -101,7 +122,7 @@ int run_graph(int cfg, const EvpArgBlock<T>* ab_dev, int nsub, const Geom& g, in
     }
     hipGraph_t graph = nullptr;
     VH_TRY_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeRelaxed));
-    for (int k = 0; k < nsub; ++k) launch_substep<T, TD>(cfg, ab_dev, k, g, cs);
+    launch_substeps<T, TD>(cfg, ab_dev, nsub, g, cs);
     VH_TRY_HIP(hipStreamEndCapture(cs, &graph));
     VH_TRY_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
     VH_TRY_HIP(hipGraphDestroy(graph));
@@ -242,8 +263,7 @@ int run_evp(const Geom& g, const vh_settings* s, const vh_fields* fl, int nsub, 
   }
   const int cfg = pick_tile(g, tile_req);
   ab.g = g;
-  ab.nsub = nsub;
-  ab.n_tiles = n_tiles_of(cfg, g);
+  ab.nlaunch = n_launches(cfg, nsub);
   ab.members = nullptr;
   ab.tiles = nullptr;
   EvpArgBlock<T>* ab_dev = (EvpArgBlock<T>*)((char*)ws + 5 * fb);
@@ -253,7 +273,7 @@ int run_evp(const Geom& g, const vh_settings* s, const vh_fields* fl, int nsub, 
     int rc = run_graph<T, double>(cfg, ab_dev, nsub, g, dtype, st);
     if (rc) return rc;
   } else {
-    for (int k = 0; k < nsub; ++k) launch_substep<T, double>(cfg, ab_dev, k, g, st);
+    launch_substeps<T, double>(cfg, ab_dev, nsub, g, st);
   }
   VH_TRY_HIP(hipGetLastError());
   return VH_OK;
@@ -275,10 +295,11 @@ extern "C" int vh_evp_workspace_bytes(const vh_domain* d, size_t* bytes) {
 
 extern "C" int vh_evp_tile_count(void) { return vh::kNumTiles - 1; }
 
-extern "C" int vh_evp_tile_shape(int32_t tile, int32_t* ri, int32_t* rj) {
-  VH_REQUIRE(tile >= 1 && tile < vh::kNumTiles && ri && rj, VH_EINVAL, "tile %d", tile);
+extern "C" int vh_evp_tile_shape(int32_t tile, int32_t* ri, int32_t* rj, int32_t* k) {
+  VH_REQUIRE(tile >= 1 && tile < vh::kNumTiles && ri && rj && k, VH_EINVAL, "tile %d", tile);
   *ri = vh::kTiles[tile].ri;
   *rj = vh::kTiles[tile].rj;
+  *k = vh::kTiles[tile].k;
   return VH_OK;
 }
 
