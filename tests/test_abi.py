@@ -1,0 +1,87 @@
+"""The C ABI library loads on the CPU host and matches its header.
+
+No compute call is made here (there is no GPU): only symbol presence, struct
+layouts against include/veris_hip.h, and host-side argument validation.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import re
+from pathlib import Path
+
+import pytest
+
+from veris_amd import _native as N
+from veris_amd.settings import SETTINGS
+from veris_amd.variables import VARIABLE_NAMES
+
+HEADER = Path(__file__).resolve().parent.parent / "include" / "veris_hip.h"
+
+
+def _header():
+    return HEADER.read_text()
+
+
+def _struct_members(text, name):
+    body = re.search(r"typedef struct %s \{(.*?)\} %s;" % (name, name), text, re.S).group(1)
+    body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
+    members = []
+    for decl in body.split(";"):
+        decl = decl.strip()
+        if not decl:
+            continue
+        names = decl.split(None, 1)[1] if not decl.startswith("const") else decl.split(None, 2)[2]
+        for n in names.split(","):
+            members.append(n.strip().lstrip("*").strip())
+    return members
+
+
+def test_every_declared_function_is_exported_and_bound():
+    lib = N.lib()
+    decls = set(re.findall(r"^int (vh_\w+)\(", _header(), re.M))
+    assert decls, "no declarations parsed"
+    for name in decls:
+        assert hasattr(lib, name), f"{name} not exported by {N.LIB_PATH}"
+    assert decls == set(N.SIGNATURES), decls ^ set(N.SIGNATURES)
+
+
+def test_struct_layouts_match_header():
+    text = _header()
+    assert _struct_members(text, "vh_settings") == list(SETTINGS)
+    assert _struct_members(text, "vh_fields") == list(VARIABLE_NAMES)
+    assert _struct_members(text, "vh_domain") == [f[0] for f in N.Domain._fields_]
+    assert _struct_members(text, "vh_growth_out") == [f[0] for f in N.GrowthOut._fields_]
+    assert ctypes.sizeof(N.Settings) == 8 * len(SETTINGS)
+    assert ctypes.sizeof(N.Fields) == 8 * len(VARIABLE_NAMES)
+    assert ctypes.sizeof(N.Domain) == 32
+
+
+def test_version_and_host_side_validation():
+    lib = N.lib()
+    assert lib.vh_version() == N.ABI_VERSION
+    dom = N.Domain(nx=90, ny=40, olx=2, oly=2, cyclic_x=1, fill_mode=0, dtype=N.VH_F64, tile=0)
+    assert N.workspace_bytes("vh_evp_workspace_bytes", dom) >= 5 * 94 * 44 * 8
+    assert N.workspace_bytes("vh_advection_workspace_bytes", dom) >= 6 * 94 * 44 * 8
+    bad = N.Domain(nx=90, ny=40, olx=3, oly=2, cyclic_x=1, fill_mode=0, dtype=N.VH_F64, tile=0)
+    with pytest.raises(N.VerisHipError, match="VH_ESHAPE.*olx=3"):
+        N.workspace_bytes("vh_evp_workspace_bytes", bad)
+    bad = N.Domain(nx=90, ny=40, olx=2, oly=2, cyclic_x=1, fill_mode=0, dtype=7, tile=0)
+    with pytest.raises(N.VerisHipError, match="VH_EDTYPE"):
+        N.workspace_bytes("vh_evp_workspace_bytes", bad)
+    shapes = N.evp_tile_shapes()
+    assert len(shapes) == lib.vh_evp_tile_count()
+    for ri, rj, k in shapes.values():
+        assert ri * rj <= 1024 and (ri * rj) % 64 == 0 and ri > 4 * k and rj > 4 * k
+
+
+def test_null_arguments_are_rejected_without_touching_the_gpu():
+    dom = N.Domain(nx=8, ny=6, olx=2, oly=2, cyclic_x=1, fill_mode=0, dtype=N.VH_F64, tile=0)
+    sett = N.Settings()
+    fields = N.Fields()
+    with pytest.raises(N.VerisHipError, match="VH_EINVAL|VH_EWORKSPACE"):
+        N.call("vh_evp_solver", ctypes.byref(dom), ctypes.byref(sett), ctypes.byref(fields), 5,
+               None, None, None, None, None, None, 0, None)
+    with pytest.raises(N.VerisHipError, match="VH_EINVAL"):
+        N.call("vh_seaice_mass", ctypes.byref(dom), ctypes.byref(sett), ctypes.byref(fields),
+               None, None, None, None)
