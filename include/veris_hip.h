@@ -348,6 +348,39 @@ int vh_evp_solver(const vh_domain* d, const vh_settings* s, const vh_fields* f,
                   int32_t n_substeps, void* uIce, void* vIce, void* sigma1, void* sigma2,
                   void* sigma12, void* workspace, size_t workspace_bytes, void* stream);
 
+/* One member of a batched ensemble: a padded (nx+4, ny+4) box starting at
+ * element `offset` of every packed field. */
+typedef struct vh_member {
+  int32_t nx, ny;     /* interior cells (>= 4 each) */
+  int32_t cyclic_x;   /* x-periodic overlap fill (Veros enable_cyclic_x) */
+  int32_t fill_mode;  /* as vh_domain.fill_mode */
+  int64_t offset;     /* element offset of the member's box; boxes must not overlap */
+} vh_member;
+
+/* A batched ensemble of members of uneven sizes packed into shared arrays. */
+typedef struct vh_batch {
+  int32_t n_members;
+  int32_t dtype;            /* VH_F64 | VH_F32 */
+  int32_t tile;             /* EVP tile index, 0 = library default */
+  int32_t reserved;
+  const vh_member* members; /* host array of n_members */
+  int64_t total_elems;      /* elements of every packed field */
+} vh_batch;
+
+/* Workspace bytes vh_evp_solver_batched needs for this batch (any tile). */
+int vh_evp_batched_workspace_bytes(const vh_batch* b, size_t* bytes);
+
+/* evp_solver (veris/evp_solver.py:257-319) of every member of a batch in one
+ * launch sequence: `f` holds packed fields (one pointer per variable, each
+ * member's box at its offset) and the five packed outputs receive the final
+ * state.  Members run the same substep kernel as vh_evp_solver; with
+ * dtype VH_F32 the bulk is FP32 and Delta, zeta and the replacement pressure
+ * are FP64.  No extension of the reference (which has no batching). */
+int vh_evp_solver_batched(const vh_batch* b, const vh_settings* s, const vh_fields* f,
+                          int32_t n_substeps, void* uIce, void* vIce, void* sigma1,
+                          void* sigma2, void* sigma12, void* workspace, size_t workspace_bytes,
+                          void* stream);
+
 /* OceanStressUV (veris/ocean_stress.py:8-40), overlap-filled. */
 int vh_ocean_stress_uv(const vh_domain* d, const vh_settings* s, const vh_fields* f,
                        void* OceanStressU, void* OceanStressV, void* stream);

@@ -56,6 +56,17 @@ class GrowthOut(ctypes.Structure):
     _fields_ = [(name, ctypes.c_void_p) for name in GROWTH_OUT_NAMES]
 
 
+class Member(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("cyclic_x", ctypes.c_int32),
+                ("fill_mode", ctypes.c_int32), ("offset", ctypes.c_int64)]
+
+
+class Batch(ctypes.Structure):
+    _fields_ = [("n_members", ctypes.c_int32), ("dtype", ctypes.c_int32), ("tile", ctypes.c_int32),
+                ("reserved", ctypes.c_int32), ("members", ctypes.POINTER(Member)),
+                ("total_elems", ctypes.c_int64)]
+
+
 _P = ctypes.c_void_p
 _DP = ctypes.POINTER(Domain)
 _SP = ctypes.POINTER(Settings)
@@ -75,6 +86,9 @@ SIGNATURES = {
     "vh_evp_tile_shape": [_I32, ctypes.POINTER(_I32), ctypes.POINTER(_I32), ctypes.POINTER(_I32)],
     "vh_evp_workspace_bytes": [_DP, ctypes.POINTER(_SZ)],
     "vh_evp_solver": [_DP, _SP, _FP, _I32, _P, _P, _P, _P, _P, _P, _SZ, _P],
+    "vh_evp_batched_workspace_bytes": [ctypes.POINTER(Batch), ctypes.POINTER(_SZ)],
+    "vh_evp_solver_batched": [ctypes.POINTER(Batch), _SP, _FP, _I32, _P, _P, _P, _P, _P, _P, _SZ,
+                              _P],
     "vh_ocean_stress_uv": [_DP, _SP, _FP, _P, _P, _P],
     "vh_advection_workspace_bytes": [_DP, ctypes.POINTER(_SZ)],
     "vh_advection": [_DP, _SP, _FP, _P, _P, _P, _P, _SZ, _P],

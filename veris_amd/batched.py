@@ -1,0 +1,148 @@
+"""Ensembles of sea-ice domains of uneven sizes in one launch sequence.
+
+Members (each a :class:`~veris_amd.state.VerisState` with its own grid, masks,
+ice and forcing) are packed back to back into one device array per variable;
+the batched EVP entry point of the C ABI runs every member's subcycle in the
+same launches (one member table, tiles of all members in one grid), replayed
+from a hipGraph.  With float32 the bulk is FP32 while Delta, zeta and the
+replacement pressure stay FP64; budgets are summed in FP64 on the device.
+
+The reference has no batching; each member follows ``evp_solver``
+(veris/evp_solver.py:257-319) exactly as the single-domain drop-in does.
+"""
+
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _native as N
+from . import kernels as K
+from .state import VerisState
+from .variables import VARIABLE_NAMES
+
+_KTERMS = ("k1AtC", "k2AtC", "k1AtZ", "k2AtZ")
+_PRE_EVP = ("SeaIceMassC", "SeaIceMassU", "SeaIceMassV", "AreaW", "AreaS", "WindForcingX",
+            "WindForcingY", "SeaIceStrength")
+EVP_OUT = ("uIce", "vIce", "sigma1", "sigma2", "sigma12")
+
+
+class Ensemble:
+    """Packed ensemble on one device.
+
+    ``states``: member states (NumPy or torch variables); their settings must
+    agree except for ``nx``, ``ny`` and ``enable_cyclic_x``.
+    """
+
+    def __init__(self, states, device="cuda", dtype=torch.float32):
+        if not states:
+            raise ValueError("empty ensemble")
+        self.device = torch.device(device)
+        self.dtype = dtype
+        self.settings = states[0].settings
+        skip = {"nx", "ny", "enable_cyclic_x"}
+        for st in states[1:]:
+            for k in self.settings.keys():
+                if k not in skip and st.settings[k] != self.settings[k]:
+                    raise ValueError(f"members disagree on setting {k}")
+        self.shapes = [tuple(st.shape) for st in states]
+        sizes = [a * b for a, b in self.shapes]
+        self.offsets = np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int64)
+        self.total = int(sum(sizes))
+        self.n = len(states)
+        self.cyclic = [bool(st.settings.enable_cyclic_x) for st in states]
+        self.fill_mode = 0 if self.settings.veros_fill else 1
+        self.fields = {}
+        for name in VARIABLE_NAMES:
+            buf = torch.empty(self.total, dtype=dtype, device=self.device)
+            for m, st in enumerate(states):
+                a = st.variables[name]
+                t = a if isinstance(a, torch.Tensor) else torch.from_numpy(np.ascontiguousarray(a))
+                self.view(buf, m).copy_(t.to(device=self.device, dtype=dtype))
+            self.fields[name] = buf
+        self._members = (N.Member * self.n)()
+        for m, (NX, NY) in enumerate(self.shapes):
+            self._members[m] = N.Member(nx=NX - 4, ny=NY - 4, cyclic_x=int(self.cyclic[m]),
+                                        fill_mode=self.fill_mode, offset=int(self.offsets[m]))
+        self._ws = None
+        self.member_settings = [st.settings for st in states]
+
+    # -- views --------------------------------------------------------------
+
+    def view(self, buf, m):
+        """Member ``m``'s padded box of a packed 1-D buffer, as (NX, NY)."""
+        o = int(self.offsets[m])
+        NX, NY = self.shapes[m]
+        return buf[o: o + NX * NY].view(NX, NY)
+
+    def member_state(self, m) -> VerisState:
+        """A state whose variables are views into the packed arrays."""
+        vals = {name: self.view(buf, m) for name, buf in self.fields.items()}
+        return VerisState(self.member_settings[m], vals)
+
+    def unpack(self, name_or_buf):
+        buf = self.fields[name_or_buf] if isinstance(name_or_buf, str) else name_or_buf
+        return [self.view(buf, m) for m in range(self.n)]
+
+    # -- kernels ------------------------------------------------------------
+
+    def pre_evp(self):
+        """SeaIceMass, AreaWS, WindForcingXY, SeaIceStrength of every member,
+        written into the packed arrays (single-domain kernels on the views)."""
+        for m in range(self.n):
+            st = self.member_state(m)
+            vs = st.variables
+            vs.SeaIceMassC, vs.SeaIceMassU, vs.SeaIceMassV = K.SeaIceMass(st)
+            vs.AreaW, vs.AreaS = K.AreaWS(st)
+            vs.WindForcingX, vs.WindForcingY = K.WindForcingXY(st)
+            vs.SeaIceStrength = K.SeaIceStrength(st)
+            for name in _PRE_EVP:
+                self.view(self.fields[name], m).copy_(vs[name])
+
+    def _batch(self, tile):
+        return N.Batch(n_members=self.n, dtype=N.VH_F64 if self.dtype == torch.float64 else N.VH_F32,
+                       tile=int(tile), reserved=0, members=self._members, total_elems=self.total)
+
+    def _fields_struct(self):
+        f = N.Fields()
+        for name in VARIABLE_NAMES:
+            setattr(f, name, self.fields[name].data_ptr())
+        if all(K._all_zero(self.fields[k]) for k in _KTERMS):
+            for k in _KTERMS:
+                setattr(f, k, None)
+        return f
+
+    def evp(self, n_substeps=None, tile=0, out=None):
+        """Batched EVP subcycle; returns the five packed outputs
+        (uIce, vIce, sigma1, sigma2, sigma12)."""
+        n = int(self.settings.nEVPsteps if n_substeps is None else n_substeps)
+        b = self._batch(tile)
+        nbytes = ctypes.c_size_t(0)
+        N.call("vh_evp_batched_workspace_bytes", ctypes.byref(b), ctypes.byref(nbytes))
+        if self._ws is None or self._ws.numel() < nbytes.value:
+            self._ws = torch.empty(nbytes.value, dtype=torch.uint8, device=self.device)
+        if out is None:
+            out = [torch.empty(self.total, dtype=self.dtype, device=self.device) for _ in range(5)]
+        sett = N.settings_struct(self.settings)
+        fields = self._fields_struct()
+        stream = torch.cuda.current_stream(self.device).cuda_stream
+        N.call("vh_evp_solver_batched", ctypes.byref(b), ctypes.byref(sett), ctypes.byref(fields),
+               n, *[t.data_ptr() for t in out], self._ws.data_ptr(), self._ws.numel(), stream)
+        return tuple(out)
+
+    def interior_cells(self) -> int:
+        return int(sum((a - 4) * (b - 4) for a, b in self.shapes))
+
+    def budgets(self, names=("hIceMean", "hSnowMean", "Area")):
+        """FP64 device sums of rA * field over each member's interior:
+        returns {name: float64 tensor of shape (n_members,)}."""
+        res = {}
+        for name in names:
+            vals = []
+            for m in range(self.n):
+                st = self.member_state(m)
+                vals.append(K.global_sum(st, st.variables[name], st.variables.rA))
+            res[name] = torch.stack(vals)
+        return res

@@ -1,0 +1,213 @@
+// Batched EVP subcycle: host side of vh_evp_solver_batched and its FP64
+// instantiation (built with -ffp-contract=off, like the single-domain path).
+// The FP32 instantiation lives in evp_batched_fast.hip (contraction allowed).
+//
+// Members of uneven sizes are packed into shared arrays.  A device member
+// table (offset, geometry, index of the member's first tile) lets one launch
+// cover every member: a workgroup finds its member by binary search and its
+// tile inside the member arithmetically, and a member's tiles form one
+// contiguous run of the XCD-ordered tile sequence.  The member table and the
+// argument block are written into the caller's workspace by small kernels
+// that receive them by value (so no host buffer has to outlive the call);
+// the substep launches are replayed from a hipGraph.
+#include <algorithm>
+#include <cstdio>
+#include <vector>
+
+#include "evp_kernel.h"
+
+namespace vh {
+
+void evp_batched_launch_f32(int cfg, const EvpArgBlock<float>* ab, int which, int launch,
+                            int n_tiles, hipStream_t st);
+
+namespace {
+
+int member_geom(const vh_member& m, Geom* g) {
+  vh_domain d{m.nx, m.ny, 2, 2, m.cyclic_x, m.fill_mode, VH_F64, 0};
+  return make_geom(&d, g);
+}
+
+int n_tiles_of(int cfg, const Geom& g) {
+  const int TI = evp_tile_extent_i(cfg), TJ = evp_tile_extent_j(cfg);
+  return ((g.NX + TI - 1) / TI) * ((g.NY + TJ - 1) / TJ);
+}
+
+struct Layout {
+  size_t fb, ab, mem, total;
+};
+
+template <typename T>
+Layout layout_of(const vh_batch* b) {
+  Layout L;
+  L.fb = align256((size_t)b->total_elems * sizeof(T));
+  L.ab = align256(sizeof(EvpArgBlock<T>));
+  L.mem = align256((size_t)b->n_members * sizeof(MemberDesc));
+  L.total = 5 * L.fb + L.ab + L.mem;
+  return L;
+}
+
+int check_batch(const vh_batch* b) {
+  VH_REQUIRE(b != nullptr && b->members != nullptr && b->n_members > 0, VH_EINVAL, "empty batch");
+  VH_REQUIRE(b->dtype == VH_F64 || b->dtype == VH_F32, VH_EDTYPE, "dtype %d", b->dtype);
+  VH_REQUIRE(b->tile >= 0 && b->tile < kEvpNumTiles, VH_EINVAL, "tile %d", b->tile);
+  std::vector<std::pair<long long, long long>> spans;
+  long long tiles = 0;
+  for (int m = 0; m < b->n_members; ++m) {
+    Geom g;
+    int rc = member_geom(b->members[m], &g);
+    if (rc) return rc;
+    const long long n = (long long)g.NX * g.NY;
+    VH_REQUIRE(b->members[m].offset >= 0 && b->members[m].offset + n <= b->total_elems,
+               VH_ESHAPE, "member %d box [%lld, %lld) outside the packed fields (%lld)", m,
+               (long long)b->members[m].offset, (long long)b->members[m].offset + n,
+               (long long)b->total_elems);
+    spans.push_back({b->members[m].offset, b->members[m].offset + n});
+    tiles += n_tiles_of(7, g);  // the shape with the most tiles
+  }
+  VH_REQUIRE(tiles < (1LL << 30), VH_ESHAPE, "batch too large (%lld tiles)", tiles);
+  std::sort(spans.begin(), spans.end());
+  for (size_t q = 1; q < spans.size(); ++q)
+    VH_REQUIRE(spans[q].first >= spans[q - 1].second, VH_EINVAL, "member boxes overlap");
+  return VH_OK;
+}
+
+template <typename T>
+void launch_batched(int cfg, const EvpArgBlock<T>* ab, int which, int launch, int n_tiles,
+                    hipStream_t st);
+
+template <>
+void launch_batched<double>(int cfg, const EvpArgBlock<double>* ab, int which, int launch,
+                            int n_tiles, hipStream_t st) {
+  evp_launch_cfg<double, double, true>(cfg, ab, which, launch, n_tiles, 0, st);
+}
+
+template <>
+void launch_batched<float>(int cfg, const EvpArgBlock<float>* ab, int which, int launch,
+                           int n_tiles, hipStream_t st) {
+  evp_batched_launch_f32(cfg, ab, which, launch, n_tiles, st);
+}
+
+template <typename T>
+int run_batched(const vh_batch* b, const vh_settings* s, const vh_fields* fl, int nsub, void* uo,
+                void* vo, void* s1o, void* s2o, void* s12o, void* ws, size_t wsb, hipStream_t st) {
+  const Layout L = layout_of<T>(b);
+  VH_REQUIRE(ws != nullptr && wsb >= L.total, VH_EWORKSPACE,
+             "batched evp workspace: need %zu bytes, got %zu", L.total, wsb);
+  const void* req[kEvpMaxRequired];
+  const int nreq = evp_required_fields(fl, req);
+  for (int q = 0; q < nreq; ++q)
+    VH_REQUIRE(req[q] != nullptr, VH_EINVAL, "evp_solver_batched: missing input field");
+  const void* outs[] = {uo, vo, s1o, s2o, s12o};
+  for (int q = 0; q < 5; ++q) {
+    VH_REQUIRE(outs[q] != nullptr, VH_EINVAL, "evp_solver_batched: null output");
+    for (int r = 0; r < nreq; ++r)
+      VH_REQUIRE(outs[q] != req[r], VH_EINVAL, "evp_solver_batched: output aliases an input");
+    for (int r = 0; r < q; ++r)
+      VH_REQUIRE(outs[q] != outs[r], VH_EINVAL, "evp_solver_batched: two outputs alias");
+  }
+  VH_REQUIRE(s->useAdaptiveEVP == 0.0 && s->noSlip != 0.0 && s->secondOrderBC == 0.0 &&
+                 s->useEVP != 0.0,
+             VH_EUNSUPPORTED, "evp_solver_batched: only non-adaptive noSlip EVP is supported");
+  const int kt = (fl->k1AtC || fl->k2AtC || fl->k1AtZ || fl->k2AtZ) ? 1 : 0;
+  VH_REQUIRE(!kt || (fl->k1AtC && fl->k2AtC && fl->k1AtZ && fl->k2AtZ), VH_EINVAL,
+             "k1AtC/k2AtC/k1AtZ/k2AtZ must be all set or all NULL");
+  const int basal = s->basalDragK2 != 0.0 ? 1 : 0;
+  if (basal)
+    VH_REQUIRE(fl->R_low && fl->Area && fl->hIceMean, VH_EINVAL,
+               "basal drag needs R_low, Area, hIceMean");
+
+  const T* inp[5] = {evp_ptr<T>(fl->uIce), evp_ptr<T>(fl->vIce), evp_ptr<T>(fl->sigma1),
+                     evp_ptr<T>(fl->sigma2), evp_ptr<T>(fl->sigma12)};
+  T* outp[5] = {(T*)uo, (T*)vo, (T*)s1o, (T*)s2o, (T*)s12o};
+  if (nsub == 0) {
+    for (int q = 0; q < 5; ++q)
+      VH_TRY_HIP(hipMemcpyAsync(outp[q], inp[q], (size_t)b->total_elems * sizeof(T),
+                                hipMemcpyDeviceToDevice, st));
+    return VH_OK;
+  }
+
+  const int cfg = (b->tile > 0) ? b->tile : 3;
+  char* base = (char*)ws;
+  EvpArgBlock<T>* ab_dev = (EvpArgBlock<T>*)(base + 5 * L.fb);
+  MemberDesc* mem_dev = (MemberDesc*)(base + 5 * L.fb + L.ab);
+
+  // member table, written by value in chunks
+  int n_main = 0, n_rem = 0;
+  for (int first = 0; first < b->n_members; first += kMembersPerWrite) {
+    MemberChunk ch{};
+    ch.first = first;
+    ch.count = std::min(kMembersPerWrite, b->n_members - first);
+    for (int q = 0; q < ch.count; ++q) {
+      MemberDesc& md = ch.m[q];
+      md.off = b->members[first + q].offset;
+      member_geom(b->members[first + q], &md.g);
+      md.tb[0] = n_main;
+      md.tb[1] = n_rem;
+      n_main += n_tiles_of(cfg, md.g);
+      n_rem += n_tiles_of(kEvpRemainderCfg, md.g);
+    }
+    hipLaunchKernelGGL(evp_write_members_kernel, dim3(1), dim3(kMembersPerWrite), 0, st, mem_dev,
+                       ch);
+  }
+  VH_TRY_HIP(hipGetLastError());
+
+  EvpArgBlock<T> ab{};
+  fill_evp_scalars(s, kt, basal, &ab.sc);
+  fill_evp_const(fl, &ab.c);
+  for (int q = 0; q < 5; ++q) {
+    ab.in[q] = inp[q];
+    ab.out[q] = outp[q];
+    ab.alt[q] = (T*)(base + q * L.fb);
+  }
+  ab.g = Geom{};
+  ab.nlaunch = evp_n_launches(cfg, nsub);
+  ab.members = mem_dev;
+  ab.n_members = b->n_members;
+  hipLaunchKernelGGL(evp_write_args_kernel<T>, dim3(1), dim3(64), 0, st, ab_dev, ab);
+  VH_TRY_HIP(hipGetLastError());
+  const int K = kEvpTiles[cfg].k;
+  auto record = [&](hipStream_t cs) {
+    int l = 0;
+    for (int q = 0; q < nsub / K; ++q) launch_batched<T>(cfg, ab_dev, 0, l++, n_main, cs);
+    for (int q = 0; q < nsub % K; ++q) launch_batched<T>(kEvpRemainderCfg, ab_dev, 1, l++, n_rem, cs);
+  };
+  if (nsub >= 8 && graphs_enabled()) {
+    char key[192];
+    snprintf(key, sizeof(key), "evpb/%d/%d/%d/%d/%d/%p", (int)sizeof(T), cfg, nsub, n_main, n_rem,
+             (const void*)ab_dev);
+    int rc = graph_run(key, record, st);
+    if (rc) return rc;
+  } else {
+    record(st);
+  }
+  VH_TRY_HIP(hipGetLastError());
+  return VH_OK;
+}
+
+}  // namespace
+}  // namespace vh
+
+extern "C" int vh_evp_batched_workspace_bytes(const vh_batch* b, size_t* bytes) {
+  int rc = vh::check_batch(b);
+  if (rc) return rc;
+  VH_REQUIRE(bytes != nullptr, VH_EINVAL, "null bytes");
+  *bytes = b->dtype == VH_F64 ? vh::layout_of<double>(b).total : vh::layout_of<float>(b).total;
+  return VH_OK;
+}
+
+extern "C" int vh_evp_solver_batched(const vh_batch* b, const vh_settings* s, const vh_fields* f,
+                                     int32_t n_substeps, void* uIce, void* vIce, void* sigma1,
+                                     void* sigma2, void* sigma12, void* workspace,
+                                     size_t workspace_bytes, void* stream) {
+  int rc = vh::check_batch(b);
+  if (rc) return rc;
+  VH_REQUIRE(s != nullptr && f != nullptr, VH_EINVAL, "null settings or fields");
+  VH_REQUIRE(n_substeps >= 0, VH_EINVAL, "n_substeps=%d", n_substeps);
+  hipStream_t st = (hipStream_t)stream;
+  if (b->dtype == VH_F64)
+    return vh::run_batched<double>(b, s, f, n_substeps, uIce, vIce, sigma1, sigma2, sigma12,
+                                   workspace, workspace_bytes, st);
+  return vh::run_batched<float>(b, s, f, n_substeps, uIce, vIce, sigma1, sigma2, sigma12,
+                                workspace, workspace_bytes, st);
+}

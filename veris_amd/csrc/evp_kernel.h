@@ -65,17 +65,27 @@ struct EvpScalars {
 };
 
 // Per-member geometry of a batched launch: element offset of the member's
-// padded box inside every packed field, and its extents.
+// padded box inside every packed field, its extents, and the index of its
+// first tile in the main-shape and remainder-shape launches (tiles of a
+// member are consecutive; tb[.] of member m+1 = tb[.] of m + its tile count).
 struct MemberDesc {
   long long off;
   Geom g;
-  int pad;
+  int tb[2];
 };
 
-// One tile of a batched launch.
-struct TileDesc {
-  int member, i0, j0, pad;
+// Up to kMembersPerWrite member descriptors, passed by value to the kernel
+// that stores them (kernel arguments are copied when the launch is enqueued).
+constexpr int kMembersPerWrite = 48;
+struct MemberChunk {
+  int first, count;
+  MemberDesc m[kMembersPerWrite];
 };
+
+static __global__ void evp_write_members_kernel(MemberDesc* dst, MemberChunk chunk) {
+  const int q = threadIdx.x;
+  if (q < chunk.count) dst[chunk.first + q] = chunk.m[q];
+}
 
 // Everything a launch reads, in device memory (written once per solver call,
 // so a captured graph of launches stays valid when the caller's buffers move).
@@ -90,13 +100,140 @@ struct EvpArgBlock {
   int nlaunch;     // launches in the subcycle
   int pad;
   const MemberDesc* members;  // batched only
-  const TileDesc* tiles;      // batched only
+  int n_members;              // batched only
+  int pad2;
 };
+
+
+template <typename T>
+inline const T* evp_ptr(const void* p) {
+  return static_cast<const T*>(p);
+}
+
+// Scalars of one subcycle from the reference settings (host side).
+inline void fill_evp_scalars(const vh_settings* s, int kterms, int basal, EvpScalars* out) {
+  EvpScalars& sc = *out;
+  sc.rpd = 1.0 / (s->PlasDefCoeff * s->PlasDefCoeff);
+  sc.deltaMin = s->deltaMin;
+  sc.tens1p = 1 + s->tensileStrFac;
+  sc.tens1m = 1 - s->tensileStrFac;
+  sc.pr = s->pressReplFac;
+  sc.pr1m = 1 - s->pressReplFac;
+  sc.alphaM1 = s->evpAlpha - 1;
+  sc.den1 = 1 / s->evpAlpha;
+  sc.den2 = sc.den1;
+  sc.cosWat = s->cosWat;
+  sc.sinWat = s->sinWat;
+  sc.cwN = s->waterIceDrag;
+  sc.cwS = s->waterIceDrag_south;
+  sc.rhoSea = s->rhoSea;
+  sc.cDragMin = s->cDragMin;
+  sc.cDragMin2 = s->cDragMin * s->cDragMin;
+  sc.rdt = s->recip_deltatDyn;
+  sc.betaU = s->evpBeta * s->recip_deltatDyn;
+  sc.betaP1 = sc.betaU + s->recip_deltatDyn;
+  sc.K2 = s->basalDragK2;
+  sc.U0sq = s->basalDragU0 * s->basalDragU0;
+  sc.K1 = s->basalDragK1;
+  sc.cBasalStar = s->cBasalStar;
+  sc.kterms = kterms;
+  sc.turn = s->sinWat != 0.0 ? 1 : 0;
+  sc.basal = basal;
+  sc.pad = 0;
+}
+
+// Step-constant field pointers of one subcycle (host side).
+template <typename T>
+inline void fill_evp_const(const vh_fields* fl, EvpConst<T>* out) {
+  EvpConst<T>& f = *out;
+  f.uNm1 = evp_ptr<T>(fl->uIce);
+  f.vNm1 = evp_ptr<T>(fl->vIce);
+  f.P = evp_ptr<T>(fl->SeaIceStrength);
+  f.massC = evp_ptr<T>(fl->SeaIceMassC);
+  f.massU = evp_ptr<T>(fl->SeaIceMassU);
+  f.massV = evp_ptr<T>(fl->SeaIceMassV);
+  f.AreaW = evp_ptr<T>(fl->AreaW);
+  f.AreaS = evp_ptr<T>(fl->AreaS);
+  f.WFX = evp_ptr<T>(fl->WindForcingX);
+  f.WFY = evp_ptr<T>(fl->WindForcingY);
+  f.uOc = evp_ptr<T>(fl->uOcean);
+  f.vOc = evp_ptr<T>(fl->vOcean);
+  f.fC = evp_ptr<T>(fl->fCori);
+  f.M = evp_ptr<T>(fl->iceMask);
+  f.MU = evp_ptr<T>(fl->iceMaskU);
+  f.MV = evp_ptr<T>(fl->iceMaskV);
+  f.mInC = evp_ptr<T>(fl->maskInC);
+  f.mInU = evp_ptr<T>(fl->maskInU);
+  f.mInV = evp_ptr<T>(fl->maskInV);
+  f.rdxU = evp_ptr<T>(fl->recip_dxU);
+  f.rdyV = evp_ptr<T>(fl->recip_dyV);
+  f.rdyU = evp_ptr<T>(fl->recip_dyU);
+  f.rdxV = evp_ptr<T>(fl->recip_dxV);
+  f.dxU = evp_ptr<T>(fl->dxU);
+  f.dyU = evp_ptr<T>(fl->dyU);
+  f.dxV = evp_ptr<T>(fl->dxV);
+  f.dyV = evp_ptr<T>(fl->dyV);
+  f.rrAu = evp_ptr<T>(fl->recip_rAu);
+  f.rrAv = evp_ptr<T>(fl->recip_rAv);
+  f.rAz = evp_ptr<T>(fl->rAz);
+  f.rrA = evp_ptr<T>(fl->recip_rA);
+  f.k1C = evp_ptr<T>(fl->k1AtC);
+  f.k2C = evp_ptr<T>(fl->k2AtC);
+  f.k1Z = evp_ptr<T>(fl->k1AtZ);
+  f.k2Z = evp_ptr<T>(fl->k2AtZ);
+  f.Rlow = evp_ptr<T>(fl->R_low);
+  f.Area = evp_ptr<T>(fl->Area);
+  f.hIce = evp_ptr<T>(fl->hIceMean);
+}
+
+// Fields a subcycle needs (all must be non-null).
+inline int evp_required_fields(const vh_fields* fl, const void** req) {
+  const void* r[] = {fl->uIce, fl->vIce, fl->sigma1, fl->sigma2, fl->sigma12,
+                     fl->SeaIceStrength, fl->SeaIceMassC, fl->SeaIceMassU, fl->SeaIceMassV,
+                     fl->AreaW, fl->AreaS, fl->WindForcingX, fl->WindForcingY, fl->uOcean,
+                     fl->vOcean, fl->fCori, fl->iceMask, fl->iceMaskU, fl->iceMaskV,
+                     fl->maskInC, fl->maskInU, fl->maskInV, fl->recip_dxU, fl->recip_dyV,
+                     fl->recip_dyU, fl->recip_dxV, fl->dxU, fl->dyU, fl->dxV, fl->dyV,
+                     fl->recip_rAu, fl->recip_rAv, fl->rAz, fl->recip_rA};
+  const int n = (int)(sizeof(r) / sizeof(r[0]));
+  for (int q = 0; q < n; ++q) req[q] = r[q];
+  return n;
+}
+
+constexpr int kEvpMaxRequired = 34;
 
 template <typename T>
 __global__ void evp_write_args_kernel(EvpArgBlock<T>* dst, EvpArgBlock<T> src) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *dst = src;
 }
+
+// Global-address-space views of field pointers: loads through them compile to
+// global loads with a uniform (SGPR) base and a 32-bit per-lane offset.
+template <typename T>
+using gconst = const T __attribute__((address_space(1)))*;
+template <typename T>
+using gmut = T __attribute__((address_space(1)))*;
+template <typename T>
+__device__ inline gconst<T> G(const T* p) {
+  return (gconst<T>)p;
+}
+template <typename T>
+__device__ inline gmut<T> GW(T* p) {
+  return (gmut<T>)p;
+}
+
+// Element `idx` of a field whose base is wavefront-uniform: the byte offset is
+// formed in 32 bits so the load uses the scalar base + 32-bit lane offset form.
+template <typename T>
+__device__ inline T ldg(const T* base, unsigned idx) {
+  return *(gconst<T>)((const char*)base + (size_t)(idx * (unsigned)sizeof(T)));
+}
+template <typename T>
+__device__ inline void stg(T* base, unsigned idx, T v) {
+  *(gmut<T>)((char*)base + (size_t)(idx * (unsigned)sizeof(T))) = v;
+}
+
+__device__ inline int uniform(int x) { return __builtin_amdgcn_readfirstlane(x); }
 
 // padded index of region coordinate p along an axis of padded length N,
 // interior length n, overlap 2: roll wrap, or periodic interior if filled
@@ -108,8 +245,8 @@ __device__ inline int map_axis(int p, int N, int n, int filled) {
 
 template <typename T, typename TD, int RI, int RJ, int K, bool BATCHED>
 __global__ __launch_bounds__(RI* RJ) void evp_substep_kernel(const EvpArgBlock<T>* __restrict__ ab,
-                                                             int launch, int n_tiles, int tiles_j,
-                                                             int per_xcd) {
+                                                             int which, int launch, int n_tiles,
+                                                             int tiles_j, int per_xcd) {
   constexpr int R = 2 * K;
   constexpr int TI = RI - 2 * R, TJ = RJ - 2 * R, NT = RI * RJ;
   static_assert((RJ & (RJ - 1)) == 0, "RJ must be a power of two");
@@ -126,12 +263,30 @@ __global__ __launch_bounds__(RI* RJ) void evp_substep_kernel(const EvpArgBlock<T
   long long off = 0;
   int i0, j0;
   if (BATCHED) {
-    const TileDesc td = ab->tiles[tile];
-    const MemberDesc md = ab->members[td.member];
-    g = md.g;
-    off = md.off;
-    i0 = td.i0;
-    j0 = td.j0;
+    // member owning this tile: last m with tb[which] <= tile (uniform search)
+    const MemberDesc* ms = ab->members;
+    int lo = 0, hi = uniform(ab->n_members) - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (uniform(ms[mid].tb[which]) <= tile) lo = mid;
+      else hi = mid - 1;
+    }
+    lo = uniform(lo);
+    const MemberDesc* md = ms + lo;
+    g.NX = uniform(md->g.NX);
+    g.NY = uniform(md->g.NY);
+    g.nx = uniform(md->g.nx);
+    g.ny = uniform(md->g.ny);
+    g.ol = 2;
+    g.fill_x = uniform(md->g.fill_x);
+    g.fill_y = uniform(md->g.fill_y);
+    const long long mo = md->off;
+    off = ((long long)(unsigned)uniform((int)(mo >> 32)) << 32) | (unsigned)uniform((int)mo);
+    const int local = tile - uniform(md->tb[which]);
+    const int tj_m = (g.NY + TJ - 1) / TJ;
+    const int ti = local / tj_m;
+    i0 = ti * TI;
+    j0 = (local - ti * tj_m) * TJ;
   } else {
     g = ab->g;
     const int ti = tile / tiles_j;
@@ -151,75 +306,38 @@ __global__ __launch_bounds__(RI* RJ) void evp_substep_kernel(const EvpArgBlock<T
   const int i = i0 + ar - R, j = j0 + br - R;  // padded position
   const int gi = map_axis<K>(i, g.NX, g.nx, g.fill_x);
   const int gj = map_axis<K>(j, g.NY, g.ny, g.fill_y);
-  const long long c = off + (long long)gi * g.NY + gj;
+  const int c = gi * g.NY + gj;  // member-local element index (every region cell is in bounds)
 
-  // load predicates: union over the substeps (substep 0 has the widest sets)
-  const int a0 = ar - 2, b0 = br - 2;  // coordinates relative to the substep-0 tile
-  const int TI0 = RI - 4, TJ0 = RJ - 4;
-  const bool zc0 = (a0 >= -1) && (b0 >= -1);
-  const bool cc0 = zc0 && (a0 <= TI0) && (b0 <= TJ0);
-  const bool cs0 = zc0 && (a0 < TI0) && (b0 < TJ0);
-  const bool zs0 = (a0 >= 0) && (a0 <= TI0) && (b0 >= 0) && (b0 <= TJ0);
-  const bool up0 = (a0 >= 0) && (a0 < TI0) && (b0 >= 0) && (b0 < TJ0);
   // final output cell of this thread
   const bool own = (ar >= R) && (ar < R + TI) && (br >= R) && (br < R + TJ) && (i < g.NX) &&
                    (j < g.NY);
+  const bool upd_any = (ar >= 2) && (ar < RI - 2) && (br >= 2) && (br < RJ - 2) &&
+                       (K > 1 || is_computed(g, i, j));
 
-  // ---- all global loads of this cell, issued together
-  T u = src[0][c], v = src[1][c];
-  const T Mc = f.M[c], MUc = f.MU[c], MVc = f.MV[c];
-  const T uoc = f.uOc[c], voc = f.vOc[c];
-  const T mInUc = f.mInU[c], mInVc = f.mInV[c];
-  T rdyU = 0, rdxV = 0, rAz = 0, k1Z = 0, k2Z = 0;
-  if (zc0) {
-    rdyU = f.rdyU[c];
-    rdxV = f.rdxV[c];
-    rAz = f.rAz[c];
-    if (sc.kterms) {
-      k1Z = f.k1Z[c];
-      k2Z = f.k2Z[c];
-    }
-  }
-  T rdxU = 0, rdyV = 0, mInC = 0, rrA = 0, P = 0, k1C = 0, k2C = 0;
-  if (cc0) {
-    rdxU = f.rdxU[c];
-    rdyV = f.rdyV[c];
-    mInC = f.mInC[c];
-    rrA = f.rrA[c];
-    P = f.P[c];
-    if (sc.kterms) {
-      k1C = f.k1C[c];
-      k2C = f.k2C[c];
-    }
-  }
-  T s1 = 0, s2 = 0, dyV = 0, dxU = 0, fc = 0, mC = 0;
-  if (cs0) {
-    s1 = src[2][c];
-    s2 = src[3][c];
-    dyV = f.dyV[c];
-    dxU = f.dxU[c];
-    fc = f.fC[c];
-    mC = f.massC[c];
-  }
-  T s12 = 0, dxV = 0, dyU = 0;
-  if (zs0) {
-    s12 = src[4][c];
-    dxV = f.dxV[c];
-    dyU = f.dyU[c];
-  }
-  const bool upd_any = up0 && (K > 1 || is_computed(g, i, j));
-  T rrAu = 0, rrAv = 0, mU = 0, mV = 0, AW = 0, AS = 0, wfx = 0, wfy = 0, uNm1 = 0, vNm1 = 0;
-  if (upd_any) {
-    rrAu = f.rrAu[c];
-    rrAv = f.rrAv[c];
-    mU = f.massU[c];
-    mV = f.massV[c];
-    AW = f.AreaW[c];
-    AS = f.AreaS[c];
-    wfx = f.WFX[c];
-    wfy = f.WFY[c];
-    uNm1 = f.uNm1[c];
-    vNm1 = f.vNm1[c];
+  // ---- all global loads of this cell, issued together and unconditionally
+  T u = ldg(src[0] + off, c), v = ldg(src[1] + off, c);
+  const T Mc = ldg(f.M + off, c), MUc = ldg(f.MU + off, c), MVc = ldg(f.MV + off, c);
+  const T uoc = ldg(f.uOc + off, c), voc = ldg(f.vOc + off, c);
+  const T mInUc = ldg(f.mInU + off, c), mInVc = ldg(f.mInV + off, c);
+  const T rdyU = ldg(f.rdyU + off, c), rdxV = ldg(f.rdxV + off, c), rAz = ldg(f.rAz + off, c);
+  const T rdxU = ldg(f.rdxU + off, c), rdyV = ldg(f.rdyV + off, c), mInC = ldg(f.mInC + off, c);
+  const T rrA = ldg(f.rrA + off, c), P = ldg(f.P + off, c);
+  T s1 = ldg(src[2] + off, c), s2 = ldg(src[3] + off, c);
+  const T dyV = ldg(f.dyV + off, c), dxU = ldg(f.dxU + off, c), fc = ldg(f.fC + off, c);
+  const T mC = ldg(f.massC + off, c);
+  T s12 = ldg(src[4] + off, c);
+  const T dxV = ldg(f.dxV + off, c), dyU = ldg(f.dyU + off, c);
+  const T rrAu = ldg(f.rrAu + off, c), rrAv = ldg(f.rrAv + off, c);
+  const T mU = ldg(f.massU + off, c), mV = ldg(f.massV + off, c);
+  const T AW = ldg(f.AreaW + off, c), AS = ldg(f.AreaS + off, c);
+  const T wfx = ldg(f.WFX + off, c), wfy = ldg(f.WFY + off, c);
+  const T uNm1 = ldg(f.uNm1 + off, c), vNm1 = ldg(f.vNm1 + off, c);
+  T k1Z = 0, k2Z = 0, k1C = 0, k2C = 0;
+  if (sc.kterms) {
+    k1Z = ldg(f.k1Z + off, c);
+    k2Z = ldg(f.k2Z + off, c);
+    k1C = ldg(f.k1C + off, c);
+    k2C = ldg(f.k2C + off, c);
   }
   T cbc = (T)0;  // basal drag coefficient inputs are step constants: load lazily
   const T sg = sgn(fc);
@@ -306,17 +424,15 @@ __global__ __launch_bounds__(RI* RJ) void evp_substep_kernel(const EvpArgBlock<T
         const T cd = ((D * D) * w > (T)sc.cDragMin2) ? D * sqrt(w) : (T)sc.cDragMin;
         sCD[t] = cd * Mc;
         if (sc.basal) {
-          const long long cip =
-              off + (long long)map_axis<K>(i + 1, g.NX, g.nx, g.fill_x) * g.NY + gj;
-          const long long cjp =
-              off + (long long)gi * g.NY + map_axis<K>(j + 1, g.NY, g.ny, g.fill_y);
-          const T um = u * mInUc, umip = uip * f.mInU[cip];
-          const T vm = v * mInVc, vmjp = vjp * f.mInV[cjp];
+          const int cip = map_axis<K>(i + 1, g.NX, g.nx, g.fill_x) * g.NY + gj;
+          const int cjp = gi * g.NY + map_axis<K>(j + 1, g.NY, g.ny, g.fill_y);
+          const T um = u * mInUc, umip = uip * ldg(f.mInU + off, cip);
+          const T vm = v * mInVc, vmjp = vjp * ldg(f.mInV + off, cjp);
           T tb = (T)0.25 * (((um * um + umip * umip) + vm * vm) + vmjp * vmjp);
           tb = (T)sc.K2 / sqrt(tb + (T)sc.U0sq);
-          const T Ar = f.Area[c];
-          const T hc = (fabs(f.Rlow[c]) * Ar) / (T)sc.K1;
-          const T val = ((tb * log(exp((T)10.0 * (f.hIce[c] - hc)) + (T)1.0)) * (T)0.1) *
+          const T Ar = ldg(f.Area + off, c);
+          const T hc = (fabs(ldg(f.Rlow + off, c)) * Ar) / (T)sc.K1;
+          const T val = ((tb * log(exp((T)10.0 * (ldg(f.hIce + off, c) - hc)) + (T)1.0)) * (T)0.1) *
                         exp((T)(-sc.cBasalStar) * ((T)1.0 - Ar));
           cbc = Ar > (T)0.01 ? val : (T)0;
         }
@@ -385,19 +501,73 @@ __global__ __launch_bounds__(RI* RJ) void evp_substep_kernel(const EvpArgBlock<T
 
   // ---- results of the launch, at the cell's own padded position
   if (own) {
-    const long long cp = off + (long long)i * g.NY + j;
+    const int cp = i * g.NY + j;
     if (K == 1) {
       if (upd_any) {
-        store_filled(g, dst[0] + off, i, j, u);
-        store_filled(g, dst[1] + off, i, j, v);
+        store_filled(g, GW(dst[0] + off), i, j, u);
+        store_filled(g, GW(dst[1] + off), i, j, v);
       }
     } else {
-      dst[0][cp] = u;
-      dst[1][cp] = v;
+      stg(dst[0] + off, cp, u);
+      stg(dst[1] + off, cp, v);
     }
-    dst[2][cp] = s1;
-    dst[3][cp] = s2;
-    dst[4][cp] = s12;
+    stg(dst[2] + off, cp, s1);
+    stg(dst[3] + off, cp, s2);
+    stg(dst[4] + off, cp, s12);
+  }
+}
+
+}  // namespace vh
+
+namespace vh {
+
+// Launch shapes: ri x rj threads, k substeps per launch (index 0 = default).
+struct TileCfg {
+  int ri, rj, k;
+};
+constexpr TileCfg kEvpTiles[] = {{0, 0, 0},   {8, 64, 1},  {12, 64, 1}, {16, 64, 1}, {8, 32, 1},
+                                 {16, 32, 1}, {32, 32, 1}, {6, 64, 1},  {16, 64, 2}, {32, 32, 2},
+                                 {12, 64, 2}, {32, 32, 3}, {16, 64, 3}, {32, 32, 4}, {16, 32, 2},
+                                 {24, 32, 2}};
+constexpr int kEvpNumTiles = sizeof(kEvpTiles) / sizeof(kEvpTiles[0]);
+// single-substep shape used for the n % k remainder of a subcycle
+constexpr int kEvpRemainderCfg = 5;
+
+inline int evp_tile_extent_i(int cfg) { return kEvpTiles[cfg].ri - 4 * kEvpTiles[cfg].k; }
+inline int evp_tile_extent_j(int cfg) { return kEvpTiles[cfg].rj - 4 * kEvpTiles[cfg].k; }
+inline int evp_n_launches(int cfg, int nsub) {
+  return nsub / kEvpTiles[cfg].k + nsub % kEvpTiles[cfg].k;
+}
+
+template <typename T, typename TD, int RI, int RJ, int K, bool BATCHED>
+inline void evp_launch_shape(const EvpArgBlock<T>* ab, int which, int launch, int n_tiles,
+                             int tiles_j, hipStream_t st) {
+  const int per_xcd = (n_tiles + 7) / 8;
+  hipLaunchKernelGGL((evp_substep_kernel<T, TD, RI, RJ, K, BATCHED>), dim3(per_xcd * 8),
+                     dim3(RI * RJ), 0, st, ab, which, launch, n_tiles, tiles_j, per_xcd);
+}
+
+// One launch of shape `cfg`: tiles_j is used by single-domain launches, `which`
+// (0 main shape, 1 remainder shape) by batched ones.
+template <typename T, typename TD, bool BATCHED>
+inline void evp_launch_cfg(int cfg, const EvpArgBlock<T>* ab, int which, int launch, int n_tiles,
+                           int tiles_j, hipStream_t st) {
+  switch (cfg) {
+    case 1: evp_launch_shape<T, TD, 8, 64, 1, BATCHED>(ab, which, launch, n_tiles, tiles_j, st); break;
+    case 2: evp_launch_shape<T, TD, 12, 64, 1, BATCHED>(ab, which, launch, n_tiles, tiles_j, st); break;
+    case 3: evp_launch_shape<T, TD, 16, 64, 1, BATCHED>(ab, which, launch, n_tiles, tiles_j, st); break;
+    case 4: evp_launch_shape<T, TD, 8, 32, 1, BATCHED>(ab, which, launch, n_tiles, tiles_j, st); break;
+    case 5: evp_launch_shape<T, TD, 16, 32, 1, BATCHED>(ab, which, launch, n_tiles, tiles_j, st); break;
+    case 6: evp_launch_shape<T, TD, 32, 32, 1, BATCHED>(ab, which, launch, n_tiles, tiles_j, st); break;
+    case 7: evp_launch_shape<T, TD, 6, 64, 1, BATCHED>(ab, which, launch, n_tiles, tiles_j, st); break;
+    case 8: evp_launch_shape<T, TD, 16, 64, 2, BATCHED>(ab, which, launch, n_tiles, tiles_j, st); break;
+    case 9: evp_launch_shape<T, TD, 32, 32, 2, BATCHED>(ab, which, launch, n_tiles, tiles_j, st); break;
+    case 10: evp_launch_shape<T, TD, 12, 64, 2, BATCHED>(ab, which, launch, n_tiles, tiles_j, st); break;
+    case 11: evp_launch_shape<T, TD, 32, 32, 3, BATCHED>(ab, which, launch, n_tiles, tiles_j, st); break;
+    case 12: evp_launch_shape<T, TD, 16, 64, 3, BATCHED>(ab, which, launch, n_tiles, tiles_j, st); break;
+    case 13: evp_launch_shape<T, TD, 32, 32, 4, BATCHED>(ab, which, launch, n_tiles, tiles_j, st); break;
+    case 14: evp_launch_shape<T, TD, 16, 32, 2, BATCHED>(ab, which, launch, n_tiles, tiles_j, st); break;
+    default: evp_launch_shape<T, TD, 24, 32, 2, BATCHED>(ab, which, launch, n_tiles, tiles_j, st); break;
   }
 }
 

@@ -7,6 +7,8 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <functional>
+#include <string>
 
 #include "veris_hip.h"
 
@@ -68,8 +70,8 @@ __device__ inline bool is_computed(const Geom& g, int i, int j) {
 
 // Stores the value of computed cell (i, j) at (i, j) and at every overlap cell
 // that the periodic fill copies from it (x first, then y, as the fill does).
-template <typename T>
-__device__ inline void store_filled(const Geom& g, T* dst, int i, int j, T val) {
+template <typename P, typename T>
+__device__ inline void store_filled(const Geom& g, P dst, int i, int j, T val) {
   int ix[2] = {i, -1};
   int jy[2] = {j, -1};
   if (g.fill_x) {
@@ -86,10 +88,20 @@ __device__ inline void store_filled(const Geom& g, T* dst, int i, int j, T val) 
 #pragma unroll
     for (int b = 0; b < 2; ++b) {
       if (jy[b] < 0) continue;
-      dst[(size_t)ix[a] * g.NY + jy[b]] = val;
+      dst[ix[a] * g.NY + jy[b]] = val;
     }
   }
 }
+
+// Replays the launch sequence that `record(stream)` issues as a hipGraph on
+// `stream`, capturing it once per (device, key) on a private stream.  The
+// key must name everything the recorded launches bake in (kernel arguments,
+// grid sizes).  Returns VH_OK or VH_EHIP.
+int graph_run(const std::string& key, const std::function<void(hipStream_t)>& record,
+              hipStream_t stream);
+
+// Whether graph replay is enabled (environment VERIS_EVP_GRAPH=0 disables it).
+int graphs_enabled();
 
 // 1-D launch helper: one thread per padded cell.
 inline dim3 cell_grid(const Geom& g, int block) {

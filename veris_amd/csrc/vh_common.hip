@@ -1,7 +1,10 @@
 // Library-wide entry points (version, error text) and the domain check.
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <map>
+#include <mutex>
 
 #include "vh_common.h"
 
@@ -31,6 +34,61 @@ int make_geom(const vh_domain* d, Geom* g) {
   g->NY = d->ny + 4;
   g->fill_x = d->fill_mode == 1 ? 1 : (d->cyclic_x ? 1 : 0);
   g->fill_y = d->fill_mode == 1 ? 1 : 0;
+  return VH_OK;
+}
+
+namespace {
+std::mutex g_graph_mu;
+std::map<std::pair<int, std::string>, hipGraphExec_t> g_graphs;
+std::map<int, hipStream_t> g_capture_streams;
+}  // namespace
+
+int graphs_enabled() {
+  static int en = -1;
+  if (en < 0) {
+    const char* e = getenv("VERIS_EVP_GRAPH");
+    en = (e && e[0] == '0') ? 0 : 1;
+  }
+  return en;
+}
+
+int graph_run(const std::string& key, const std::function<void(hipStream_t)>& record,
+              hipStream_t st) {
+  int dev = 0;
+  VH_TRY_HIP(hipGetDevice(&dev));
+  hipGraphExec_t exec = nullptr;
+  hipStream_t cs = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_graph_mu);
+    auto it = g_graphs.find({dev, key});
+    if (it != g_graphs.end()) exec = it->second;
+    auto ct = g_capture_streams.find(dev);
+    if (ct != g_capture_streams.end()) cs = ct->second;
+  }
+  if (!exec) {
+    if (!cs) {
+      VH_TRY_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+      std::lock_guard<std::mutex> lk(g_graph_mu);
+      auto ins = g_capture_streams.emplace(dev, cs);
+      if (!ins.second) {
+        hipStreamDestroy(cs);
+        cs = ins.first->second;
+      }
+    }
+    hipGraph_t graph = nullptr;
+    VH_TRY_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeRelaxed));
+    record(cs);
+    VH_TRY_HIP(hipStreamEndCapture(cs, &graph));
+    VH_TRY_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    VH_TRY_HIP(hipGraphDestroy(graph));
+    std::lock_guard<std::mutex> lk(g_graph_mu);
+    auto ins = g_graphs.emplace(std::make_pair(dev, key), exec);
+    if (!ins.second) {
+      hipGraphExecDestroy(exec);
+      exec = ins.first->second;
+    }
+  }
+  VH_TRY_HIP(hipGraphLaunch(exec, st));
   return VH_OK;
 }
 
