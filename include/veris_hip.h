@@ -381,6 +381,68 @@ int vh_evp_solver_batched(const vh_batch* b, const vh_settings* s, const vh_fiel
                           void* sigma2, void* sigma12, void* workspace, size_t workspace_bytes,
                           void* stream);
 
+/* Adjoints of the EVP subcycle's inputs (outputs of vh_evp_adjoint).  Every
+ * pointer is a caller-owned device field of the domain's shape and dtype and
+ * must be non-NULL.  adj_uNm1/adj_vNm1 are the adjoints of the previous-step
+ * velocities in the momentum equation (the fields' uIce/vIce), separate from
+ * adj_uIce/adj_vIce, the adjoints of the subcycle's initial state (the same
+ * arrays in the reference: add both for the total).  The water drag adjoint
+ * is per cell, split by hemisphere (north: fCori >= 0); its sum over the
+ * whole padded field is d(objective)/d(waterIceDrag) and d/d(waterIceDrag_south). */
+typedef struct vh_evp_adjoint_out {
+  void *adj_uIce, *adj_vIce, *adj_sigma1, *adj_sigma2, *adj_sigma12;
+  void *adj_uNm1, *adj_vNm1;
+  void *adj_SeaIceStrength, *adj_SeaIceMassC, *adj_SeaIceMassU, *adj_SeaIceMassV;
+  void *adj_AreaW, *adj_AreaS, *adj_WindForcingX, *adj_WindForcingY;
+  void *adj_waterDrag_north, *adj_waterDrag_south;
+} vh_evp_adjoint_out;
+
+/* Adjoints of the pre-EVP inputs (outputs of vh_pre_evp_adjoint): per-cell
+ * strength parameter P*, thickness, snow thickness, and the air drag per cell
+ * split by hemisphere (sums over the whole padded field give d/d(airIceDrag)
+ * and d/d(airIceDrag_south)). */
+typedef struct vh_pre_evp_adjoint_out {
+  void *adj_pStar, *adj_hIceMean, *adj_hSnowMean, *adj_airDrag_north, *adj_airDrag_south;
+} vh_pre_evp_adjoint_out;
+
+/* Workspace bytes of vh_evp_forward_checkpointed / vh_evp_adjoint for a
+ * subcycle of n_substeps with a checkpoint every ckpt_every substeps (the
+ * two calls share the workspace: the adjoint reads the checkpoints). */
+int vh_evp_adjoint_workspace_bytes(const vh_domain* d, int32_t n_substeps, int32_t ckpt_every,
+                                   size_t* bytes);
+
+/* evp_solver (veris/evp_solver.py:257-319) that also keeps the state every
+ * ckpt_every substeps in `workspace` for a later vh_evp_adjoint with the same
+ * inputs.  Results equal vh_evp_solver's.  Supported: non-adaptive noSlip EVP
+ * without turning angle (sinWat = 0), basal drag or metric terms. */
+int vh_evp_forward_checkpointed(const vh_domain* d, const vh_settings* s, const vh_fields* f,
+                                int32_t n_substeps, int32_t ckpt_every, void* uIce, void* vIce,
+                                void* sigma1, void* sigma2, void* sigma12, void* workspace,
+                                size_t workspace_bytes, void* stream);
+
+/* Reverse mode of the subcycle (the derivative jax.grad takes through
+ * veris/evp_solver.py:22-319 in the reference): given the adjoints of the
+ * final u, v, sigma1, sigma2, sigma12 (any may be NULL = zero), writes the
+ * adjoints of every subcycle input into `out`.  Recomputes each segment from
+ * the checkpoints of the preceding vh_evp_forward_checkpointed call (same d,
+ * s, f, n_substeps, ckpt_every, workspace).  d sqrt(x)/dx is taken as 0 at
+ * x = 0 and the denominator guard contributes no derivative. */
+int vh_evp_adjoint(const vh_domain* d, const vh_settings* s, const vh_fields* f,
+                   int32_t n_substeps, int32_t ckpt_every, const void* adj_uIce,
+                   const void* adj_vIce, const void* adj_sigma1, const void* adj_sigma2,
+                   const void* adj_sigma12, const vh_evp_adjoint_out* out, void* workspace,
+                   size_t workspace_bytes, void* stream);
+
+/* Reverse mode of SeaIceStrength (veris/dynamics_routines.py:5-18, with
+ * pStarField per cell or NULL for the pStar setting), SeaIceMass
+ * (veris/area_mass.py:17-28) and WindForcingXY (veris/dynsolver.py:7-84):
+ * maps the strength, mass and wind-forcing adjoints of `evp_adj` to P*,
+ * hIceMean, hSnowMean and the air drag.  useRealFreshWaterFlux is not
+ * supported. */
+int vh_pre_evp_adjoint(const vh_domain* d, const vh_settings* s, const vh_fields* f,
+                       const void* pStarField, const vh_evp_adjoint_out* evp_adj,
+                       const vh_pre_evp_adjoint_out* out, void* stream);
+
 /* OceanStressUV (veris/ocean_stress.py:8-40), overlap-filled. */
 int vh_ocean_stress_uv(const vh_domain* d, const vh_settings* s, const vh_fields* f,
                        void* OceanStressU, void* OceanStressV, void* stream);

@@ -1,0 +1,206 @@
+"""GPU checks of the hand-written EVP adjoint (vh_evp_forward_checkpointed,
+vh_evp_adjoint, vh_pre_evp_adjoint) through veris_amd.adjoint.evp_subcycle.
+
+* The checkpointed forward equals the drop-in path (pre-EVP kernels +
+  vh_evp_solver) bit for bit.
+* Gradients of a scalar objective of all five outputs w.r.t. P* (per cell),
+  hIceMean (per cell) and the four drag coefficients equal torch.autograd of
+  an independent torch FP64 restatement (tests/torch_ref.py, the stand-in for
+  the reference's jax.grad) to 1e-9 relative: both are FP64 and differ only
+  in rounding order.
+* Central finite differences of the GPU forward along 20 random directions
+  match the directional derivative to 1e-5 (the 20 directions perturb all
+  parameters at once, with a step small enough not to flip a branch).
+* The checkpoint interval changes nothing: gradients are bit-identical.
+* FP32 state (Delta and zeta in FP64 in the forward) against the FP64
+  adjoint, at the tolerance stated in the test.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+
+import torch_ref as TR
+from gpu_helpers import assert_exact, g2n, pre_evp, report
+from oracle import veris_np as O
+from veris_amd import adjoint as A
+from veris_amd import kernels as K
+from veris_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+NAMES = ("uIce", "vIce", "sigma1", "sigma2", "sigma12")
+
+
+def _state(kind):
+    if kind == "global":
+        st = synth.global_latlon(31, 36, 20, -75.0, 80.0)
+    elif kind == "doubly_periodic":
+        st = synth.global_latlon(31, 36, 20, -75.0, 80.0, veros_fill=False)
+    elif kind == "regional":
+        st = synth.regional(32, 0, 30, 22)
+    else:
+        raise ValueError(kind)
+    pre_evp(st)
+    vs = st.variables  # spin up: non-zero velocities and stresses
+    vs.uIce, vs.vIce, vs.sigma1, vs.sigma2, vs.sigma12 = O.evp_solver(st, n_substeps=6)
+    return pre_evp(st)
+
+
+def _weights(st, seed):
+    """Cotangent weights of the objective, one per output, scaled so that
+    every output contributes at the same order."""
+    rng = np.random.default_rng(seed)
+    out = O.evp_solver(st, n_substeps=1)
+    ws = []
+    for q in range(5):
+        w = rng.standard_normal(st.shape)
+        w[:2] = w[-2:] = 0
+        w[:, :2] = w[:, -2:] = 0
+        ws.append(w / max(float(np.abs(out[q]).max()), 1e-30))
+    return ws
+
+
+def _objective(outs, ws):
+    return sum((o * w).sum() for o, w in zip(outs, ws)) + 0.5 * sum(
+        ((o * w) ** 2).sum() for o, w in zip(outs[:2], ws[:2]))
+
+
+def _params(st, device, dtype=torch.float64, requires_grad=True):
+    s = st.settings
+    rng = np.random.default_rng(7)
+    pstar = s.pStar * (1 + 0.2 * rng.random(st.shape))
+    mk = lambda a: torch.tensor(a, dtype=dtype, device=device).requires_grad_(requires_grad)
+    return dict(pStar=mk(pstar), hIceMean=mk(np.asarray(st.variables.hIceMean)),
+                waterIceDrag=mk(s.waterIceDrag), waterIceDrag_south=mk(s.waterIceDrag_south * 1.1),
+                airIceDrag=mk(s.airIceDrag), airIceDrag_south=mk(s.airIceDrag_south * 0.9))
+
+
+def _gpu_grads(st, n, ckpt, dtype=torch.float64, seed=3):
+    dev = torch.device("cuda")
+    gst = st.to(dev, dtype)
+    p = _params(st, dev, dtype=torch.float64)
+    p["hIceMean"] = p["hIceMean"].detach().to(dtype).requires_grad_(True)
+    outs = A.evp_subcycle(gst, n_substeps=n, ckpt_every=ckpt, **p)
+    ws = [torch.tensor(w, dtype=dtype, device=dev) for w in _weights(st, seed)]
+    J = _objective(outs, ws)
+    J.backward()
+    return {k: g2n(v.grad) for k, v in p.items()}, [g2n(o) for o in outs], float(J.detach())
+
+
+def _ref_grads(st, n, seed=3):
+    vs = {k: torch.from_numpy(np.asarray(v, dtype=np.float64)) for k, v in st.variables.items()}
+    s = st.settings
+    p = _params(st, "cpu")
+    f = vs["fCori"]
+    ca = torch.where(f < 0, p["airIceDrag_south"], p["airIceDrag"])
+    cw = torch.where(f < 0, p["waterIceDrag_south"], p["waterIceDrag"])
+    pre = TR.pre_evp(vs, s, p["pStar"], p["hIceMean"], ca)
+    outs = TR.evp(vs, s, pre, n, cw)
+    ws = [torch.from_numpy(w) for w in _weights(st, seed)]
+    J = _objective(outs, ws)
+    J.backward()
+    return {k: v.grad.numpy() for k, v in p.items()}, [o.detach().numpy() for o in outs], float(J.detach())
+
+
+@pytest.mark.parametrize("kind", ["global", "doubly_periodic", "regional"])
+def test_checkpointed_forward_is_the_drop_in(cuda, kind):
+    st = _state(kind)
+    gst = st.to(cuda)
+    vs = gst.variables
+    vs.SeaIceMassC, vs.SeaIceMassU, vs.SeaIceMassV = K.SeaIceMass(gst)
+    vs.AreaW, vs.AreaS = K.AreaWS(gst)
+    vs.WindForcingX, vs.WindForcingY = K.WindForcingXY(gst)
+    vs.SeaIceStrength = K.SeaIceStrength(gst)
+    for name in ("SeaIceMassC", "SeaIceMassU", "SeaIceMassV", "AreaW", "AreaS", "WindForcingX",
+                 "WindForcingY", "SeaIceStrength"):
+        assert_exact(f"{kind} pre-EVP {name}", g2n(vs[name]), st.variables[name])
+    for n, ckpt in ((23, 10), (7, 3), (5, 10)):
+        outs = A.evp_subcycle(gst, n_substeps=n, ckpt_every=ckpt)
+        want = K.evp_solver(gst, n_substeps=n)
+        for q in range(5):
+            assert_exact(f"{kind} {NAMES[q]} n={n} ckpt={ckpt}", g2n(outs[q]), g2n(want[q]))
+
+
+@pytest.mark.parametrize("kind", ["global", "doubly_periodic", "regional"])
+def test_adjoint_matches_torch_autograd(cuda, kind):
+    st = _state(kind)
+    n = 25
+    got, outs, J = _gpu_grads(st, n, ckpt=10)
+    want, routs, Jr = _ref_grads(st, n)
+    for q in range(5):
+        report(f"{kind} forward {NAMES[q]}", outs[q], routs[q])
+    assert abs(J - Jr) <= 1e-11 * abs(Jr)
+    for k in want:
+        g, w = np.asarray(got[k]), np.asarray(want[k])
+        err = float(np.linalg.norm(g - w) / max(np.linalg.norm(w), 1e-300))
+        print(f"{kind} d/d{k}: rel L2 err {err:.3e} |grad| {np.linalg.norm(w):.3e}", flush=True)
+        if k.endswith("_south") and kind == "regional":  # no southern cells
+            assert np.linalg.norm(w) == 0 and np.linalg.norm(g) == 0
+            continue
+        assert np.linalg.norm(w) > 0, k
+        assert err <= 1e-9, f"{kind} d/d{k}: {err:.3e}"
+
+
+def test_checkpoint_interval_is_irrelevant(cuda):
+    st = _state("global")
+    g1, _, _ = _gpu_grads(st, 24, ckpt=1)
+    g2, _, _ = _gpu_grads(st, 24, ckpt=7)
+    g3, _, _ = _gpu_grads(st, 24, ckpt=24)
+    for k in g1:
+        assert np.array_equal(g1[k], g2[k]) and np.array_equal(g1[k], g3[k]), k
+
+
+def test_adjoint_finite_differences(cuda):
+    """20 random directions in (P*, h, Cw_N, Cw_S, Ca_N, Ca_S) space; central
+    differences of the GPU forward with relative step 1e-6."""
+    st = _state("global")
+    n = 40
+    got, _, _ = _gpu_grads(st, n, ckpt=10)
+    gst = st.to(cuda)
+    base = _params(st, cuda, requires_grad=False)
+    ws = [torch.tensor(w, device=cuda) for w in _weights(st, 3)]
+    rng = np.random.default_rng(11)
+    worst = 0.0
+    for d in range(20):
+        # relative perturbations: h stays 0 where it is 0 (no mass appears
+        # on an ice-free cell, which would flip the denominator guard)
+        dirs = {k: torch.tensor(rng.standard_normal(tuple(v.shape)), device=cuda,
+                                dtype=torch.float64) * v for k, v in base.items()}
+        eps = 1e-6
+
+        def J(sign):
+            p = {k: base[k] + sign * eps * dirs[k] for k in base}
+            with torch.no_grad():
+                return float(_objective(A.evp_subcycle(gst, n_substeps=n, ckpt_every=10, **p), ws))
+
+        fd = (J(1) - J(-1)) / (2 * eps)
+        ad = sum(float(np.sum(np.asarray(got[k]) * g2n(dirs[k]))) for k in base)
+        rel = abs(fd - ad) / max(abs(ad), 1e-300)
+        worst = max(worst, rel)
+        print(f"direction {d}: fd={fd:.10e} adjoint={ad:.10e} rel={rel:.2e}", flush=True)
+    assert worst <= 1e-5
+
+
+def test_fp32_adjoint_against_fp64(cuda):
+    """FP32 state and adjoint against the FP64 adjoint, 25 substeps.
+    Bound: 2e-2 relative L2 per gradient."""
+    st = _state("global")
+    g64, _, _ = _gpu_grads(st, 25, ckpt=10)
+    g32, _, _ = _gpu_grads(st, 25, ckpt=10, dtype=torch.float32)
+    for k in g64:
+        w, g = np.asarray(g64[k]), np.asarray(g32[k], dtype=np.float64)
+        err = float(np.linalg.norm(g - w) / max(np.linalg.norm(w), 1e-300))
+        print(f"fp32 d/d{k}: rel L2 err {err:.3e}", flush=True)
+        assert err <= 2e-2, k
+
+
+def test_adjoint_rejects_unsupported(cuda):
+    from veris_amd._native import VerisHipError
+
+    st = _state("global")
+    st.settings.basalDragK2 = 8.0
+    with pytest.raises(VerisHipError, match="VH_EUNSUPPORTED"):
+        A.evp_subcycle(st.to(cuda), n_substeps=3)

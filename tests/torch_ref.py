@@ -1,0 +1,174 @@
+"""Torch (CPU, FP64) restatement of the pre-EVP forcing and the EVP subcycle,
+differentiated by torch.autograd -- TEST INFRASTRUCTURE ONLY.
+
+It stands in for ``jax.grad`` of the reference (JAX is absent here) when the
+hand-written HIP adjoint is checked.  It follows the reference's expressions
+(veris/evp_solver.py:22-319, veris/dynamics_routines.py:5-302,
+veris/averaging.py:4-19, veris/area_mass.py:5-28, veris/dynsolver.py:7-84)
+at default settings without the turning term and basal drag, written with
+torch.roll, and defines the derivative of sqrt at 0 as 0 (the reference's
+Delta = sqrt(Delta^2) and drag sqrt are not differentiable there; the HIP
+adjoint uses the same convention).
+"""
+
+from __future__ import annotations
+
+import torch
+
+
+class _SafeSqrt(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, x):
+        y = torch.sqrt(x)
+        ctx.save_for_backward(y)
+        return y
+
+    @staticmethod
+    def backward(ctx, g):
+        (y,) = ctx.saved_tensors
+        return torch.where(y > 0, g * 0.5 / torch.where(y > 0, y, torch.ones_like(y)), torch.zeros_like(y))
+
+
+ssqrt = _SafeSqrt.apply
+
+
+def im1(a):
+    return torch.roll(a, 1, 0)
+
+
+def ip1(a):
+    return torch.roll(a, -1, 0)
+
+
+def jm1(a):
+    return torch.roll(a, 1, 1)
+
+
+def jp1(a):
+    return torch.roll(a, -1, 1)
+
+
+def fill(sett, a):
+    if sett.veros_fill:
+        if not sett.enable_cyclic_x:
+            return a
+        a = a.clone()
+        a[-2:] = a[2:4]
+        a[:2] = a[-4:-2]
+        return a
+    a = a.clone()
+    a[:2, :] = a[-4:-2, :]
+    a[-2:, :] = a[2:4, :]
+    a[:, :2] = a[:, -4:-2]
+    a[:, -2:] = a[:, 2:4]
+    return a
+
+
+def pre_evp(vs, sett, pstar, h, ca):
+    """SeaIceMass, AreaWS, WindForcingXY, SeaIceStrength with a per-cell P*,
+    thickness h and air drag ca (both hemispheres)."""
+    mC = sett.rhoIce * h + sett.rhoSnow * vs["hSnowMean"]
+    mU = 0.5 * (mC + im1(mC))
+    mV = 0.5 * (mC + jm1(mC))
+    A = vs["Area"]
+    AW = 0.5 * (A + im1(A))
+    AS = 0.5 * (A + jm1(A))
+    u, v = vs["uIce"], vs["vIce"]
+    urel = vs["uWind"] - 0.5 * (u + ip1(u))
+    vrel = vs["vWind"] - 0.5 * (v + jp1(v))
+    wsq = urel**2 + vrel**2
+    ws = torch.where(wsq < sett.wSpeedMin**2, torch.full_like(wsq, sett.wSpeedMin), ssqrt(wsq))
+    cda = ca * sett.rhoAir * ws
+    tx = cda * urel
+    ty = cda * vrel
+    tx = 0.5 * (tx + im1(tx)) * vs["iceMaskU"]
+    ty = 0.5 * (ty + jm1(ty)) * vs["iceMaskV"]
+    phi = sett.gravity * vs["ssh_an"] + vs["surfPress"] * sett.recip_rhoSea
+    fx = tx * AW - mU * vs["recip_dxC"] * (phi - im1(phi))
+    fy = ty * AS - mV * vs["recip_dyC"] * (phi - jm1(phi))
+    P = pstar * h * torch.exp(-sett.cStar * (1 - A)) * vs["iceMask"]
+    return dict(SeaIceMassC=mC, SeaIceMassU=mU, SeaIceMassV=mV, AreaW=AW, AreaS=AS,
+                WindForcingX=fx, WindForcingY=fy, SeaIceStrength=P)
+
+
+def evp_substep(vs, sett, p, u, v, u0, v0, s1, s2, s12, cw):
+    """One EVP substep; ``p`` holds the pre-EVP fields, ``cw`` the water drag."""
+    M, MU, MV = vs["iceMask"], vs["iceMaskU"], vs["iceMaskV"]
+    dudx = (ip1(u) - u) * vs["recip_dxU"]
+    dvdy = (jp1(v) - v) * vs["recip_dyV"]
+    e11 = dudx * vs["maskInC"]
+    e22 = dvdy * vs["maskInC"]
+    dudy = (u - jm1(u)) * vs["recip_dyU"]
+    uavz = (u + jm1(u)) * 0.5
+    dvdx = (v - im1(v)) * vs["recip_dxV"]
+    vavz = (v + im1(v)) * 0.5
+    mz = M * im1(M)
+    mz = mz * jm1(mz)
+    e12 = 0.5 * (dudy + dvdx) * mz
+    e12 = e12 + (2.0 * uavz * vs["recip_dyU"] * (MU - jm1(MU))
+                 + 2.0 * vavz * vs["recip_dxV"] * (MV - im1(MV)))
+    rpd = 1.0 / sett.PlasDefCoeff**2
+    q = vs["rAz"] * e12**2
+    q = q + ip1(q)
+    e12c = 0.25 * vs["recip_rA"] * (q + jp1(q))
+    dsq = (e11 + e22) ** 2 + rpd * ((e11 - e22) ** 2 + 4.0 * e12c)
+    dC = ssqrt(dsq)
+    P = p["SeaIceStrength"]
+    zeta = 0.5 * (P * (1 + sett.tensileStrFac)) / (dC + sett.deltaMin)
+    press = (P * (1 - sett.pressReplFac) + 2.0 * zeta * dC * sett.pressReplFac
+             / (1 + sett.tensileStrFac)) * (1 - sett.tensileStrFac)
+    alpha = sett.evpAlpha
+    ep, em = e11 + e22, e11 - e22
+    n = M + im1(M)
+    n = n + jm1(n)
+    nrm = torch.where(n > 0, 1.0 / torch.where(n > 0, n, torch.ones_like(n)), torch.zeros_like(n))
+    zz = zeta + im1(zeta)
+    zz = nrm * (zz + jm1(zz))
+    s1 = (s1 * (alpha - 1) + (2 * zeta * ep - press)) / alpha * M
+    s2 = (s2 * (alpha - 1) + 2 * zeta * em * 0.25) / alpha * M
+    s12 = (s12 * (alpha - 1) + 2 * zz * e12 * 0.25) / alpha
+    s11 = 0.5 * (s1 + s2)
+    s22 = 0.5 * (s1 - s2)
+    a = s11 * vs["dyV"]
+    b = s12 * vs["dxV"]
+    divx = (a - im1(a) - b + jp1(b)) * vs["recip_rAu"]
+    c = s22 * vs["dxU"]
+    d = s12 * vs["dyU"]
+    divy = (c - jm1(c) - d + ip1(d)) * vs["recip_rAv"]
+    D = cw * sett.rhoSea
+    du = (u - vs["uOcean"]) * vs["maskInU"]
+    dv = (v - vs["vOcean"]) * vs["maskInV"]
+    w = 0.25 * ((du + ip1(du)) ** 2 + (dv + jp1(dv)) ** 2)
+    cd = torch.where(D**2 * w > sett.cDragMin**2, D * ssqrt(w), torch.full_like(w, sett.cDragMin)) * M
+    f = vs["fCori"]
+    mC, mU, mV = p["SeaIceMassC"], p["SeaIceMassU"], p["SeaIceMassV"]
+    fv = mC * f * 0.5 * (v + jp1(v))
+    fu = mC * f * 0.5 * (u + ip1(u))
+    cdU = 0.5 * (cd + im1(cd))
+    cdV = 0.5 * (cd + jm1(cd))
+    fx = p["WindForcingX"] + cdU * sett.cosWat * vs["uOcean"] * p["AreaW"] + 0.5 * (fv + im1(fv))
+    fy = p["WindForcingY"] + cdV * sett.cosWat * vs["vOcean"] * p["AreaS"] - 0.5 * (fu + jm1(fu))
+    bU = sett.evpBeta * sett.recip_deltatDyn
+    bP1 = bU + sett.recip_deltatDyn
+    denU = mU * bP1 + p["AreaW"] * (cdU * sett.cosWat)
+    denV = mV * bP1 + p["AreaS"] * (cdV * sett.cosWat)
+    denU = torch.where(denU == 0, torch.ones_like(denU), denU)
+    denV = torch.where(denV == 0, torch.ones_like(denV), denV)
+    un = MU * (bU * mU * u + mU * sett.recip_deltatDyn * u0 + fx + divx) / denU
+    vn = MV * (bU * mV * v + mV * sett.recip_deltatDyn * v0 + fy + divy) / denV
+    return fill(sett, un), fill(sett, vn), s1, s2, s12
+
+
+def evp(vs, sett, p, n, cw):
+    u, v = vs["uIce"], vs["vIce"]
+    s1, s2, s12 = vs["sigma1"], vs["sigma2"], vs["sigma12"]
+    for _ in range(n):
+        u, v, s1, s2, s12 = evp_substep(vs, sett, p, u, v, vs["uIce"], vs["vIce"], s1, s2, s12, cw)
+    return u, v, s1, s2, s12
+
+
+def misfit(vs, sett, pstar, h, cw, ca, n, ut, vt):
+    """L2 misfit of the final velocities to (ut, vt) over the interior."""
+    p = pre_evp(vs, sett, pstar, h, ca)
+    u, v, *_ = evp(vs, sett, p, n, cw)
+    return ((u - ut)[2:-2, 2:-2] ** 2).sum() + ((v - vt)[2:-2, 2:-2] ** 2).sum()

@@ -56,6 +56,23 @@ class GrowthOut(ctypes.Structure):
     _fields_ = [(name, ctypes.c_void_p) for name in GROWTH_OUT_NAMES]
 
 
+EVP_ADJOINT_NAMES = ("uIce", "vIce", "sigma1", "sigma2", "sigma12", "uNm1", "vNm1",
+                     "SeaIceStrength", "SeaIceMassC", "SeaIceMassU", "SeaIceMassV", "AreaW",
+                     "AreaS", "WindForcingX", "WindForcingY", "waterDrag_north",
+                     "waterDrag_south")
+
+
+class EvpAdjointOut(ctypes.Structure):
+    _fields_ = [("adj_" + name, ctypes.c_void_p) for name in EVP_ADJOINT_NAMES]
+
+
+PRE_EVP_ADJOINT_NAMES = ("pStar", "hIceMean", "hSnowMean", "airDrag_north", "airDrag_south")
+
+
+class PreEvpAdjointOut(ctypes.Structure):
+    _fields_ = [("adj_" + name, ctypes.c_void_p) for name in PRE_EVP_ADJOINT_NAMES]
+
+
 class Member(ctypes.Structure):
     _fields_ = [("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("cyclic_x", ctypes.c_int32),
                 ("fill_mode", ctypes.c_int32), ("offset", ctypes.c_int64)]
@@ -89,6 +106,12 @@ SIGNATURES = {
     "vh_evp_batched_workspace_bytes": [ctypes.POINTER(Batch), ctypes.POINTER(_SZ)],
     "vh_evp_solver_batched": [ctypes.POINTER(Batch), _SP, _FP, _I32, _P, _P, _P, _P, _P, _P, _SZ,
                               _P],
+    "vh_evp_adjoint_workspace_bytes": [_DP, _I32, _I32, ctypes.POINTER(_SZ)],
+    "vh_evp_forward_checkpointed": [_DP, _SP, _FP, _I32, _I32, _P, _P, _P, _P, _P, _P, _SZ, _P],
+    "vh_evp_adjoint": [_DP, _SP, _FP, _I32, _I32, _P, _P, _P, _P, _P,
+                       ctypes.POINTER(EvpAdjointOut), _P, _SZ, _P],
+    "vh_pre_evp_adjoint": [_DP, _SP, _FP, _P, ctypes.POINTER(EvpAdjointOut),
+                           ctypes.POINTER(PreEvpAdjointOut), _P],
     "vh_ocean_stress_uv": [_DP, _SP, _FP, _P, _P, _P],
     "vh_advection_workspace_bytes": [_DP, ctypes.POINTER(_SZ)],
     "vh_advection": [_DP, _SP, _FP, _P, _P, _P, _P, _SZ, _P],

@@ -36,10 +36,16 @@ void launch_substeps(int cfg, const EvpArgBlock<T>* ab, int nsub, const Geom& g,
   }
 }
 
+}  // namespace
+
+// Subcycle of nsub substeps from the state `in` (u, v, sigma1, sigma2,
+// sigma12) to `out`; fl->uIce/vIce are the previous-step velocities of the
+// momentum equation.  Shared by vh_evp_solver and the checkpointed forward
+// and segment recompute of the adjoint.
 template <typename T>
-int run_evp(const Geom& g, const vh_settings* s, const vh_fields* fl, int nsub, void* uo, void* vo,
-            void* s1o, void* s2o, void* s12o, void* ws, size_t ws_bytes, int tile_req,
-            int dtype, hipStream_t st) {
+int evp_run_core(const Geom& g, const vh_settings* s, const vh_fields* fl, const T* const in[5],
+                 T* const out[5], int nsub, void* ws, size_t ws_bytes, int tile_req, int dtype,
+                 hipStream_t st) {
   const size_t fb = align256(field_elems(g) * sizeof(T));
   const size_t need = 5 * fb + align256(sizeof(EvpArgBlock<T>));
   VH_REQUIRE(ws != nullptr && ws_bytes >= need, VH_EWORKSPACE,
@@ -52,13 +58,15 @@ int run_evp(const Geom& g, const vh_settings* s, const vh_fields* fl, int nsub, 
                        fl->recip_dyU, fl->recip_dxV, fl->dxU, fl->dyU, fl->dxV, fl->dyV,
                        fl->recip_rAu, fl->recip_rAv, fl->rAz, fl->recip_rA};
   for (const void* p : req) VH_REQUIRE(p != nullptr, VH_EINVAL, "evp_solver: missing input field");
-  const void* outs[] = {uo, vo, s1o, s2o, s12o};
   for (int q = 0; q < 5; ++q) {
-    VH_REQUIRE(outs[q] != nullptr, VH_EINVAL, "evp_solver: null output");
+    VH_REQUIRE(in[q] != nullptr, VH_EINVAL, "evp_solver: null input state");
+    VH_REQUIRE(out[q] != nullptr, VH_EINVAL, "evp_solver: null output");
     for (const void* p : req)
-      VH_REQUIRE(outs[q] != p, VH_EINVAL, "evp_solver: output aliases an input field");
+      VH_REQUIRE(out[q] != p, VH_EINVAL, "evp_solver: output aliases an input field");
+    for (int r = 0; r < 5; ++r)
+      VH_REQUIRE(out[q] != in[r], VH_EINVAL, "evp_solver: output aliases the input state");
     for (int r = 0; r < q; ++r)
-      VH_REQUIRE(outs[q] != outs[r], VH_EINVAL, "evp_solver: two outputs alias");
+      VH_REQUIRE(out[q] != out[r], VH_EINVAL, "evp_solver: two outputs alias");
   }
   VH_REQUIRE(s->useAdaptiveEVP == 0.0, VH_EUNSUPPORTED, "useAdaptiveEVP is not supported");
   VH_REQUIRE(s->noSlip != 0.0, VH_EUNSUPPORTED, "noSlip=False (side drag) is not supported");
@@ -76,9 +84,8 @@ int run_evp(const Geom& g, const vh_settings* s, const vh_fields* fl, int nsub, 
   fill_evp_scalars(s, kt, basal, &ab.sc);
   fill_evp_const(fl, &ab.c);
 
-  const T* inp[5] = {evp_ptr<T>(fl->uIce), evp_ptr<T>(fl->vIce), evp_ptr<T>(fl->sigma1), evp_ptr<T>(fl->sigma2),
-                     evp_ptr<T>(fl->sigma12)};
-  T* outp[5] = {(T*)uo, (T*)vo, (T*)s1o, (T*)s2o, (T*)s12o};
+  const T* const* inp = in;
+  T* const* outp = out;
   const size_t nb = field_elems(g) * sizeof(T);
   if (nsub == 0) {
     for (int q = 0; q < 5; ++q)
@@ -109,6 +116,25 @@ int run_evp(const Geom& g, const vh_settings* s, const vh_fields* fl, int nsub, 
   }
   VH_TRY_HIP(hipGetLastError());
   return VH_OK;
+}
+
+template int evp_run_core<double>(const Geom&, const vh_settings*, const vh_fields*,
+                                  const double* const[5], double* const[5], int, void*, size_t,
+                                  int, int, hipStream_t);
+template int evp_run_core<float>(const Geom&, const vh_settings*, const vh_fields*,
+                                 const float* const[5], float* const[5], int, void*, size_t, int,
+                                 int, hipStream_t);
+
+namespace {
+
+template <typename T>
+int run_evp(const Geom& g, const vh_settings* s, const vh_fields* fl, int nsub, void* uo, void* vo,
+            void* s1o, void* s2o, void* s12o, void* ws, size_t ws_bytes, int tile_req,
+            int dtype, hipStream_t st) {
+  const T* in[5] = {evp_ptr<T>(fl->uIce), evp_ptr<T>(fl->vIce), evp_ptr<T>(fl->sigma1),
+                    evp_ptr<T>(fl->sigma2), evp_ptr<T>(fl->sigma12)};
+  T* const out[5] = {(T*)uo, (T*)vo, (T*)s1o, (T*)s2o, (T*)s12o};
+  return evp_run_core<T>(g, s, fl, in, out, nsub, ws, ws_bytes, tile_req, dtype, st);
 }
 
 }  // namespace
