@@ -23,7 +23,7 @@ import pytest
 import torch
 
 import torch_ref as TR
-from gpu_helpers import assert_exact, g2n, pre_evp, report
+from gpu_helpers import assert_close, assert_exact, g2n, pre_evp, report
 from oracle import veris_np as O
 from veris_amd import adjoint as A
 from veris_amd import kernels as K
@@ -115,8 +115,11 @@ def test_checkpointed_forward_is_the_drop_in(cuda, kind):
     vs.WindForcingX, vs.WindForcingY = K.WindForcingXY(gst)
     vs.SeaIceStrength = K.SeaIceStrength(gst)
     for name in ("SeaIceMassC", "SeaIceMassU", "SeaIceMassV", "AreaW", "AreaS", "WindForcingX",
-                 "WindForcingY", "SeaIceStrength"):
+                 "WindForcingY"):
         assert_exact(f"{kind} pre-EVP {name}", g2n(vs[name]), st.variables[name])
+    # exp() of the device math library differs from NumPy's by an ulp here and there
+    assert_close(f"{kind} pre-EVP SeaIceStrength", g2n(vs.SeaIceStrength),
+                 st.variables.SeaIceStrength, 1e-15)
     for n, ckpt in ((23, 10), (7, 3), (5, 10)):
         outs = A.evp_subcycle(gst, n_substeps=n, ckpt_every=ckpt)
         want = K.evp_solver(gst, n_substeps=n)

@@ -3,25 +3,26 @@
 // (veris/dynamics_routines.py:5-18, veris/area_mass.py:17-28,
 // veris/dynsolver.py:7-84).
 //
-// Forward: vh_evp_forward_checkpointed runs the fused substep kernel and keeps
-// the state every `ckpt_every` substeps in the workspace.  Reverse: for each
-// segment, last first, the states inside it are recomputed from its
-// checkpoint (one fused substep launch each), then every substep is reversed
-// by six per-cell kernels in gather form -- each cell sums what its
-// neighbours contribute, so no atomics are needed:
-//   I   forward intermediates of the substep from its input state
-//       (e12, zeta, Delta, e11+e22, e11-e22, water drag)
-//   R1  u/v-points: momentum update and fill (overlap copies fold back)
-//   R2  c-points: stress divergence, stress relaxation, drag, Coriolis;
-//       z-points: sigma12 relaxation
-//   R3  c-points: zeta and Delta
-//   R4  z-points: shear strain
-//   R5  u/v-points: adjoint of the input velocities
-// Parameter adjoints (strength, masses, areas, wind forcing, previous-step
-// velocities, water drag per hemisphere) accumulate cell by cell.
+// Forward (vh_evp_forward_checkpointed): n single-substep launches of the
+// fused substep kernel; the state after every `ckpt_every` substeps stays in
+// the workspace.  Reverse (vh_evp_adjoint): for each segment, last first, the
+// substeps are recomputed from the segment's checkpoint by the same kernel in
+// its STORE form, which also keeps each substep's intermediates (e12, zeta,
+// Delta, e11+e22, e11-e22, water drag); then each substep is reversed by
+// three per-cell kernels in gather form -- each cell sums what its neighbours
+// contribute, so no atomics are needed:
+//   S  momentum update and fill (evaluated at the three u-points and three
+//      v-points a c-point touches), c-point stress divergence, stress
+//      relaxation, drag and Coriolis, z-point sigma12 relaxation
+//   D  c-point zeta and Delta
+//   V  z-point shear strain (at the three z-points a u/v-point touches) and
+//      the adjoint of the substep's input velocities
+// Every launch reads its pointers from argument blocks in the workspace that
+// one small kernel writes per call, so the n forward launches and the whole
+// reverse sweep are each replayed from one hipGraph.
 //
 // Non-smooth points: d sqrt(x)/dx is taken as 0 at x = 0 (Delta = 0 on
-// motionless or masked cells, zero relative velocity), the denominator guard
+// motionless or masked cells, zero relative velocity); the denominator guard
 // and the drag regime switch contribute no derivative.  Supported settings:
 // non-adaptive EVP, noSlip, no turning angle (sinWat = 0), no basal drag, no
 // metric terms -- the defaults.
@@ -31,37 +32,51 @@
 #include "evp_kernel.h"
 
 namespace vh {
-
-template <typename T>
-int evp_run_core(const Geom& g, const vh_settings* s, const vh_fields* fl, const T* const in[5],
-                 T* const out[5], int nsub, void* ws, size_t ws_bytes, int tile_req, int dtype,
-                 hipStream_t st);
-
 namespace {
 
 constexpr int BLK = 256;
 
 template <typename T>
-struct AdjConst {
-  const T *M, *MU, *MV, *mInC, *mInU, *mInV, *rdxU, *rdyV, *rdyU, *rdxV, *dxU, *dyU, *dxV, *dyV;
-  const T *rrAu, *rrAv, *rAz, *rrA, *fC, *uOc, *vOc, *P, *mC, *mU, *mV, *AW, *AS, *uNm1, *vNm1;
-};
-
-template <typename T>
-struct AdjWork {
-  T *E12, *ZETA, *DELTA, *EP, *EM, *CD;                     // forward intermediates
-  T *X, *Y, *FX, *FY, *CDU, *CDV, *UDIR, *VDIR;             // R1
-  T *ZBP, *EPB, *EMB, *DBP, *AB, *BB, *FUB, *FVB, *ZZB, *E12P;  // R2
-  T *E11B, *E22B, *QB;                                      // R3
-  T *E12B;                                                  // R4
-};
-
-template <typename T>
-struct AdjAcc {  // parameter adjoints, accumulated over substeps
+struct AdjAcc {  // parameter adjoints, accumulated over the substeps
   T *P, *mC, *mU, *mV, *AW, *AS, *WFX, *WFY, *u0, *v0, *DN, *DS;
 };
+constexpr int kAdjAccFields = 12;
 
-#define CELL_SETUP                                                                  \
+template <typename T>
+struct AdjWork {  // adjoint intermediates of one substep (workspace)
+  T *UDIR, *VDIR, *ZBP, *EPB, *EMB, *DBP, *AB, *BB, *FUB, *FVB, *ZZB, *E12P, *E11B, *E22B, *QB;
+};
+constexpr int kAdjWorkFields = 15;
+
+// What every reverse launch reads besides workspace-relative pointers.
+template <typename T>
+struct AdjArgs {
+  EvpConst<T> c;
+  EvpScalars sc;
+  Geom g;
+  AdjAcc<T> acc;
+  const T* lam_final[5];  // adjoints of the final state (NULL = 0)
+  T* outs[5];             // adjoints of the initial state
+};
+
+template <typename T>
+struct LamSet {  // state adjoints: of the substep's output (in) and input (out)
+  const T* in[5];
+  T* out[5];
+};
+
+// Where the states of a subcycle live (all pointers into the workspace
+// except the initial state in0 and the final state outs).
+template <typename T>
+struct BlockPlan {
+  const T* in0[5];
+  T* outs[5];
+  char* ws;
+  size_t fb, off_ckpt, off_pp, off_seg, off_interm;
+  int n, ce;
+};
+
+#define CELL_SETUP_G                                                                \
   const int n_ = g.NX * g.NY;                                                       \
   const int c = blockIdx.x * blockDim.x + threadIdx.x;                              \
   if (c >= n_) return;                                                              \
@@ -69,253 +84,335 @@ struct AdjAcc {  // parameter adjoints, accumulated over substeps
   const int im = i == 0 ? g.NX - 1 : i - 1, ip = i == g.NX - 1 ? 0 : i + 1;         \
   const int jm = j == 0 ? g.NY - 1 : j - 1, jp = j == g.NY - 1 ? 0 : j + 1;         \
   (void)im; (void)ip; (void)jm; (void)jp;
+#define CELL_SETUP         \
+  const Geom g = a->g;     \
+  CELL_SETUP_G
 #define AT(ii, jj) ((ii) * g.NY + (jj))
 
+__device__ inline int wm(int i, int n) { return i == 0 ? n - 1 : i - 1; }
+__device__ inline int wp(int i, int n) { return i == n - 1 ? 0 : i + 1; }
+
+// ---- argument blocks
 template <typename T>
-__device__ inline T e12_at(const Geom& g, const AdjConst<T>& k, const T* u, const T* v, int ii,
-                           int jj) {
-  const int im = ii == 0 ? g.NX - 1 : ii - 1, jm = jj == 0 ? g.NY - 1 : jj - 1;
-  const int c = AT(ii, jj);
-  const T u0 = u[c], ujm = u[AT(ii, jm)], v0 = v[c], vim = v[AT(im, jj)];
-  const T rdyU = k.rdyU[c], rdxV = k.rdxV[c];
-  const T mz = (k.M[c] * k.M[AT(im, jj)]) * (k.M[AT(ii, jm)] * k.M[AT(im, jm)]);
-  const T hu = k.MU[c] - k.MU[AT(ii, jm)], hv = k.MV[c] - k.MV[AT(im, jj)];
-  return ((T)0.5 * ((u0 - ujm) * rdyU + (v0 - vim) * rdxV)) * mz +
-         ((((T)2 * ((u0 + ujm) * (T)0.5)) * rdyU) * hu +
-          (((T)2 * ((v0 + vim) * (T)0.5)) * rdxV) * hv);
+__device__ inline T* slot(const BlockPlan<T>& p, size_t off, int idx) {
+  return (T*)(p.ws + off + (size_t)idx * p.fb);
+}
+
+// forward location of field q of the state after k substeps
+template <typename T>
+__device__ inline T* fwd_loc(const BlockPlan<T>& p, int k, int q) {
+  if (k == 0) return (T*)p.in0[q];
+  if (k == p.n) return p.outs[q];
+  if (k % p.ce == 0) return slot(p, p.off_ckpt, 5 * (k / p.ce) + q);
+  return slot(p, p.off_pp, 5 * ((k - 1) % 2) + q);
+}
+
+// mode 0: forward blocks (substep k: state k -> state k+1);
+// mode 1: recompute blocks (substep k inside its segment: segment slots, with
+// the intermediates of segment position k % ce) and the reverse arguments.
+template <typename T>
+__global__ void adj_write_blocks_kernel(EvpArgBlock<T>* blocks, size_t stride, EvpArgBlock<T> proto,
+                                        BlockPlan<T> p, int mode, AdjArgs<T>* aa, AdjArgs<T> av) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (mode == 1 && k == 0) *aa = av;
+  if (k >= p.n) return;
+  EvpArgBlock<T> b = proto;
+  if (mode == 0) {
+    for (int q = 0; q < 5; ++q) {
+      b.in[q] = fwd_loc(p, k, q);
+      b.out[q] = fwd_loc(p, k + 1, q);
+    }
+    for (int r = 0; r < 6; ++r) b.interm[r] = nullptr;
+  } else {
+    const int jj = k % p.ce, sgm = k / p.ce;
+    for (int q = 0; q < 5; ++q) {
+      b.in[q] = jj == 0 ? (k == 0 ? p.in0[q] : slot(p, p.off_ckpt, 5 * sgm + q))
+                        : slot(p, p.off_seg, 5 * jj + q);
+      b.out[q] = slot(p, p.off_seg, 5 * (jj + 1) + q);
+    }
+    for (int r = 0; r < 6; ++r) b.interm[r] = slot(p, p.off_interm, 6 * jj + r);
+  }
+  *(EvpArgBlock<T>*)((char*)blocks + (size_t)k * stride) = b;
+}
+
+// ---- reverse sweep: set-up and result copy
+template <typename T>
+__global__ __launch_bounds__(BLK) void adj_init_kernel(const AdjArgs<T>* __restrict__ a,
+                                                       LamSet<T> L) {
+  CELL_SETUP
+  const AdjAcc<T>& acc = a->acc;
+  T* const accp[kAdjAccFields] = {acc.P, acc.mC, acc.mU, acc.mV, acc.AW, acc.AS,
+                                  acc.WFX, acc.WFY, acc.u0, acc.v0, acc.DN, acc.DS};
+#pragma unroll
+  for (int q = 0; q < kAdjAccFields; ++q) accp[q][c] = (T)0;
+#pragma unroll
+  for (int q = 0; q < 5; ++q) L.out[q][c] = a->lam_final[q] ? a->lam_final[q][c] : (T)0;
 }
 
 template <typename T>
-__device__ inline T znorm(const Geom& g, const AdjConst<T>& k, int ii, int jj) {
-  const int im = ii == 0 ? g.NX - 1 : ii - 1, jm = jj == 0 ? g.NY - 1 : jj - 1;
-  const T n = (k.M[AT(ii, jj)] + k.M[AT(im, jj)]) + (k.M[AT(ii, jm)] + k.M[AT(im, jm)]);
-  return n > (T)0 ? (T)1 / n : (T)0;
+__global__ __launch_bounds__(BLK) void adj_finish_kernel(const AdjArgs<T>* __restrict__ a,
+                                                         LamSet<T> L) {
+  CELL_SETUP
+#pragma unroll
+  for (int q = 0; q < 5; ++q) a->outs[q][c] = L.in[q][c];
 }
+
+// Loads and stores below go through the global address space with 32-bit
+// offsets (ldg/stg): every index is a valid cell of the padded field, and the
+// kernels issue their loads before any data-dependent selection.
 
 // sum of a u/v adjoint over a computed cell and the overlap cells the fill
-// copies it to (the adjoint of the periodic fill)
+// copies it to (the adjoint of the periodic fill); branch-free: absent
+// partners read the cell itself with weight 0
 template <typename T>
 __device__ inline T gather_filled(const Geom& g, const T* a, int i, int j) {
-  int ix[2] = {i, -1}, jy[2] = {j, -1};
+  int ix = i, jy = j;
+  bool hx = false, hy = false;
   if (g.fill_x) {
-    if (i < 2 * g.ol) ix[1] = i + g.nx;
-    else if (i >= g.nx) ix[1] = i - g.nx;
+    hx = (i < 2 * g.ol) || (i >= g.nx);
+    ix = i < 2 * g.ol ? i + g.nx : (i >= g.nx ? i - g.nx : i);
   }
   if (g.fill_y) {
-    if (j < 2 * g.ol) jy[1] = j + g.ny;
-    else if (j >= g.ny) jy[1] = j - g.ny;
+    hy = (j < 2 * g.ol) || (j >= g.ny);
+    jy = j < 2 * g.ol ? j + g.ny : (j >= g.ny ? j - g.ny : j);
   }
-  T s = 0;
-  for (int p = 0; p < 2; ++p)
-    for (int q = 0; q < 2; ++q)
-      if (ix[p] >= 0 && jy[q] >= 0) s += a[AT(ix[p], jy[q])];
-  return s;
+  const T a00 = ldg(a, AT(i, j)), a10 = ldg(a, AT(ix, j)), a01 = ldg(a, AT(i, jy)),
+          a11 = ldg(a, AT(ix, jy));
+  return ((a00 + (hx ? a10 : (T)0)) + (hy ? a01 : (T)0)) + ((hx && hy) ? a11 : (T)0);
 }
 
-// ---- I: forward intermediates from the substep's input state
 template <typename T>
-__global__ __launch_bounds__(BLK) void adj_interm_kernel(Geom g, AdjConst<T> k, EvpScalars sc,
-                                                         const T* u, const T* v, AdjWork<T> w) {
-  CELL_SETUP
-  const T e12 = e12_at(g, k, u, v, i, j);
-  w.E12[c] = e12;
-  auto qz = [&](int ii, int jj) {
-    const T e = e12_at(g, k, u, v, ii, jj);
-    return k.rAz[AT(ii, jj)] * (e * e);
-  };
-  const T e12c = ((T)0.25 * k.rrA[c]) * ((qz(i, j) + qz(ip, j)) + (qz(i, jp) + qz(ip, jp)));
-  const T e11 = ((u[AT(ip, j)] - u[c]) * k.rdxU[c]) * k.mInC[c];
-  const T e22 = ((v[AT(i, jp)] - v[c]) * k.rdyV[c]) * k.mInC[c];
-  const T ep = e11 + e22, em = e11 - e22;
-  const T dsq = ep * ep + (T)sc.rpd * (em * em + (T)4 * e12c);
-  const T dC = sqrt(dsq);
-  const T zeta = ((T)0.5 * (k.P[c] * (T)sc.tens1p)) / (dC + (T)sc.deltaMin);
-  w.EP[c] = ep;
-  w.EM[c] = em;
-  w.DELTA[c] = dC;
-  w.ZETA[c] = zeta;
-  const T fc = k.fC[c];
-  const T D = (T)(fc < (T)0 ? sc.cwS : sc.cwN) * (T)sc.rhoSea;
-  const T a = (u[c] - k.uOc[c]) * k.mInU[c] + (u[AT(ip, j)] - k.uOc[AT(ip, j)]) * k.mInU[AT(ip, j)];
-  const T b = (v[c] - k.vOc[c]) * k.mInV[c] + (v[AT(i, jp)] - k.vOc[AT(i, jp)]) * k.mInV[AT(i, jp)];
-  const T ww = (T)0.25 * (a * a + b * b);
-  const T cd = ((D * D) * ww > (T)sc.cDragMin2) ? D * sqrt(ww) : (T)sc.cDragMin;
-  w.CD[c] = cd * k.M[c];
+struct MomAdj {
+  T nb;   // adjoint of the numerator (lambda * mask / den)
+  T db;   // adjoint of the denominator
+  T cdx;  // adjoint of the averaged drag coefficient
+  T cdA;  // the averaged drag coefficient
+};
+
+// Adjoint of the u-point (Y = 0) or v-point (Y = 1) momentum update at (i, j)
+// (veris/evp_solver.py:152-187 with the fill of :190-191); zero on cells the
+// fill overwrites.
+template <typename T, int Y>
+__device__ inline MomAdj<T> mom_point(const Geom& g, const EvpConst<T>& k, const EvpScalars& sc,
+                                      const T* CD, const T* xn, const T* lam, int i, int j) {
+  const int c = AT(i, j);
+  const int cm = Y ? AT(i, wm(j, g.NY)) : AT(wm(i, g.NX), j);
+  const T l = gather_filled(g, lam, i, j);
+  const T cd0 = ldg(CD, c), cd1 = ldg(CD, cm);
+  const T m = ldg(Y ? k.massV : k.massU, c);
+  const T ar = ldg(Y ? k.AreaS : k.AreaW, c);
+  const T oc = ldg(Y ? k.vOc : k.uOc, c);
+  const T mk = ldg(Y ? k.MV : k.MU, c);
+  const T x = ldg(xn, c);
+  const T cw = (T)sc.cosWat;
+  const T cdA = (T)0.5 * (cd0 + cd1);
+  const T den = m * (T)sc.betaP1 + ar * (cdA * cw);
+  const bool guard = den == (T)0;
+  const T r = (T)1 / (guard ? (T)1 : den);
+  const T lc = is_computed(g, i, j) ? l : (T)0;
+  MomAdj<T> o;
+  o.nb = lc * mk * r;
+  o.db = guard ? (T)0 : -lc * x * r;
+  o.cdx = (o.nb * oc + o.db) * (ar * cw);
+  o.cdA = cdA;
+  return o;
 }
 
-// ---- R1: u/v-points (momentum update, fill)
+// ---- S: momentum, stresses (veris/evp_solver.py:79-103), drag (:108-110),
+//         Coriolis (:142-143), stress divergence (veris/dynamics_routines.py:284-302)
 template <typename T>
-__global__ __launch_bounds__(BLK) void adj_r1_kernel(Geom g, AdjConst<T> k, EvpScalars sc,
-                                                     const T* u, const T* v, const T* un,
-                                                     const T* vn, const T* lu_in, const T* lv_in,
-                                                     AdjWork<T> w, AdjAcc<T> acc) {
+__global__ __launch_bounds__(BLK) void adj_stress_kernel(const AdjArgs<T>* __restrict__ a,
+                                                         const EvpArgBlock<T>* __restrict__ blk,
+                                                         LamSet<T> L, AdjWork<T> w) {
   CELL_SETUP
-  if (!is_computed(g, i, j)) {
-    w.X[c] = w.Y[c] = w.FX[c] = w.FY[c] = w.CDU[c] = w.CDV[c] = w.UDIR[c] = w.VDIR[c] = (T)0;
-    return;
-  }
-  const T lu = gather_filled(g, lu_in, i, j), lv = gather_filled(g, lv_in, i, j);
+  const EvpConst<T> k = a->c;
+  const EvpScalars& sc = a->sc;
+  const AdjAcc<T> acc = a->acc;
+  const T* u = blk->in[0];
+  const T* v = blk->in[1];
+  const T* un = blk->out[0];
+  const T* vn = blk->out[1];
+  const T* E12 = blk->interm[0];
+  const T* Z = blk->interm[1];
+  const T* DL = blk->interm[2];
+  const T* EP = blk->interm[3];
+  const T* EM = blk->interm[4];
+  const T* CD = blk->interm[5];
+  const int cip = AT(ip, j), cjp = AT(i, jp), cim = AT(im, j), cjm = AT(i, jm), cmm = AT(im, jm);
+  // momentum adjoints at the u-points c, ip, jm and the v-points c, jp, im
+  const MomAdj<T> U = mom_point<T, 0>(g, k, sc, CD, un, L.in[0], i, j);
+  const MomAdj<T> Uip = mom_point<T, 0>(g, k, sc, CD, un, L.in[0], ip, j);
+  const MomAdj<T> Ujm = mom_point<T, 0>(g, k, sc, CD, un, L.in[0], i, jm);
+  const MomAdj<T> V = mom_point<T, 1>(g, k, sc, CD, vn, L.in[1], i, j);
+  const MomAdj<T> Vjp = mom_point<T, 1>(g, k, sc, CD, vn, L.in[1], i, jp);
+  const MomAdj<T> Vim = mom_point<T, 1>(g, k, sc, CD, vn, L.in[1], im, j);
+  // c-point inputs
+  const T uc = ldg(u, c), uip = ldg(u, cip), vc = ldg(v, c), vjp = ldg(v, cjp);
+  const T uoc = ldg(k.uOc, c), uoip = ldg(k.uOc, cip), voc = ldg(k.vOc, c), vojp = ldg(k.vOc, cjp);
+  const T miu = ldg(k.mInU, c), miuip = ldg(k.mInU, cip), miv = ldg(k.mInV, c),
+          mivjp = ldg(k.mInV, cjp);
+  const T Mc = ldg(k.M, c), Mim = ldg(k.M, cim), Mjm = ldg(k.M, cjm), Mmm = ldg(k.M, cmm);
+  const T fc = ldg(k.fC, c);
+  const T rrAu = ldg(k.rrAu, c), rrAuip = ldg(k.rrAu, cip), rrAujm = ldg(k.rrAu, cjm);
+  const T rrAv = ldg(k.rrAv, c), rrAvjp = ldg(k.rrAv, cjp), rrAvim = ldg(k.rrAv, cim);
+  const T dyV = ldg(k.dyV, c), dxU = ldg(k.dxU, c), dxV = ldg(k.dxV, c), dyU = ldg(k.dyU, c);
+  const T mUc = ldg(k.massU, c), mVc = ldg(k.massV, c);
+  const T uNm1 = ldg(k.uNm1, c), vNm1 = ldg(k.vNm1, c);
+  const T l1 = ldg(L.in[2], c), l2 = ldg(L.in[3], c), l12 = ldg(L.in[4], c);
+  const T zeta = ldg(Z, c), dC = ldg(DL, c), ep = ldg(EP, c), em = ldg(EM, c), e12 = ldg(E12, c);
+  const T zim = ldg(Z, cim), zjm = ldg(Z, cjm), zmm = ldg(Z, cmm);
+  const T aP = ldg(acc.P, c), amC = ldg(acc.mC, c), aDN = ldg(acc.DN, c), aDS = ldg(acc.DS, c);
+
   const T cw = (T)sc.cosWat, bU = (T)sc.betaU, bP1 = (T)sc.betaP1, rdt = (T)sc.rdt;
-  {  // u-point
-    const T cdU = (T)0.5 * (w.CD[c] + w.CD[AT(im, j)]);
-    const T mU = k.mU[c], AW = k.AW[c], uo = k.uOc[c];
-    const T den = mU * bP1 + AW * (cdU * cw);
-    const bool guard = den == (T)0;
-    const T de = guard ? (T)1 : den;
-    const T Nb = lu * k.MU[c] / de;
-    const T db = guard ? (T)0 : -lu * un[c] / de;
-    w.X[c] = Nb * k.rrAu[c];
-    w.FX[c] = Nb;
-    acc.mU[c] += Nb * (bU * u[c] + rdt * k.uNm1[c]) + db * bP1;
-    acc.u0[c] += Nb * mU * rdt;
-    acc.WFX[c] += Nb;
-    acc.AW[c] += Nb * (cdU * cw * uo) + db * (cdU * cw);
-    w.CDU[c] = Nb * cw * uo * AW + db * AW * cw;
-    w.UDIR[c] = Nb * bU * mU;
+  if (is_computed(g, i, j)) {
+    const T amU = ldg(acc.mU, c), au0 = ldg(acc.u0, c), aWX = ldg(acc.WFX, c), aAW = ldg(acc.AW, c);
+    const T amV = ldg(acc.mV, c), av0 = ldg(acc.v0, c), aWY = ldg(acc.WFY, c), aAS = ldg(acc.AS, c);
+    stg(acc.mU, c, amU + (U.nb * (bU * uc + rdt * uNm1) + U.db * bP1));
+    stg(acc.u0, c, au0 + U.nb * mUc * rdt);
+    stg(acc.WFX, c, aWX + U.nb);
+    stg(acc.AW, c, aAW + (U.nb * (U.cdA * cw * uoc) + U.db * (U.cdA * cw)));
+    stg(acc.mV, c, amV + (V.nb * (bU * vc + rdt * vNm1) + V.db * bP1));
+    stg(acc.v0, c, av0 + V.nb * mVc * rdt);
+    stg(acc.WFY, c, aWY + V.nb);
+    stg(acc.AS, c, aAS + (V.nb * (V.cdA * cw * voc) + V.db * (V.cdA * cw)));
   }
-  {  // v-point
-    const T cdV = (T)0.5 * (w.CD[c] + w.CD[AT(i, jm)]);
-    const T mV = k.mV[c], AS = k.AS[c], vo = k.vOc[c];
-    const T den = mV * bP1 + AS * (cdV * cw);
-    const bool guard = den == (T)0;
-    const T de = guard ? (T)1 : den;
-    const T Nb = lv * k.MV[c] / de;
-    const T db = guard ? (T)0 : -lv * vn[c] / de;
-    w.Y[c] = Nb * k.rrAv[c];
-    w.FY[c] = Nb;
-    acc.mV[c] += Nb * (bU * v[c] + rdt * k.vNm1[c]) + db * bP1;
-    acc.v0[c] += Nb * mV * rdt;
-    acc.WFY[c] += Nb;
-    acc.AS[c] += Nb * (cdV * cw * vo) + db * (cdV * cw);
-    w.CDV[c] = Nb * cw * vo * AS + db * AS * cw;
-    w.VDIR[c] = Nb * bU * mV;
-  }
-}
+  stg(w.UDIR, c, U.nb * bU * mUc);
+  stg(w.VDIR, c, V.nb * bU * mVc);
+  const T Xc = U.nb * rrAu, Xip = Uip.nb * rrAuip, Xjm = Ujm.nb * rrAujm;
+  const T Yc = V.nb * rrAv, Yjp = Vjp.nb * rrAvjp, Yim = Vim.nb * rrAvim;
 
-// ---- R2: c-points (stresses, drag, Coriolis) and z-points (sigma12)
-template <typename T>
-__global__ __launch_bounds__(BLK) void adj_r2_kernel(Geom g, AdjConst<T> k, EvpScalars sc,
-                                                     const T* u, const T* v, const T* ls1,
-                                                     const T* ls2, const T* ls12, T* ls1o,
-                                                     T* ls2o, T* ls12o, AdjWork<T> w,
-                                                     AdjAcc<T> acc) {
-  CELL_SETUP
-  const T Mc = k.M[c], fc = k.fC[c];
-  // drag and Coriolis
-  const T cdb = (T)0.5 * (w.CDU[c] + w.CDU[AT(ip, j)]) + (T)0.5 * (w.CDV[c] + w.CDV[AT(i, jp)]);
-  const T fvb = (T)0.5 * (w.FX[c] + w.FX[AT(ip, j)]);
-  const T fub = -(T)0.5 * (w.FY[c] + w.FY[AT(i, jp)]);
-  acc.mC[c] += fvb * fc * (T)0.5 * (v[c] + v[AT(i, jp)]) + fub * fc * (T)0.5 * (u[c] + u[AT(ip, j)]);
-  w.FVB[c] = fvb;
-  w.FUB[c] = fub;
+  // drag and Coriolis at c
+  const T cdb = (T)0.5 * (U.cdx + Uip.cdx) + (T)0.5 * (V.cdx + Vjp.cdx);
+  const T fvb = (T)0.5 * (U.nb + Uip.nb);
+  const T fub = -(T)0.5 * (V.nb + Vjp.nb);
+  stg(acc.mC, c, amC + (fvb * fc * (T)0.5 * (vc + vjp) + fub * fc * (T)0.5 * (uc + uip)));
+  stg(w.FVB, c, fvb);
+  stg(w.FUB, c, fub);
   const T cbe = cdb * Mc;
-  const T D = (T)(fc < (T)0 ? sc.cwS : sc.cwN) * (T)sc.rhoSea;
-  const T a = (u[c] - k.uOc[c]) * k.mInU[c] + (u[AT(ip, j)] - k.uOc[AT(ip, j)]) * k.mInU[AT(ip, j)];
-  const T b = (v[c] - k.vOc[c]) * k.mInV[c] + (v[AT(i, jp)] - k.vOc[AT(i, jp)]) * k.mInV[AT(i, jp)];
-  const T ww = (T)0.25 * (a * a + b * b);
-  T wb = 0, Db = 0;
-  if ((D * D) * ww > (T)sc.cDragMin2) {
-    const T sw = sqrt(ww);
-    wb = sw > (T)0 ? cbe * D * (T)0.5 / sw : (T)0;
-    Db = cbe * sw;
-  }
-  w.AB[c] = wb * (T)0.5 * a;
-  w.BB[c] = wb * (T)0.5 * b;
-  if (fc < (T)0) acc.DS[c] += Db * (T)sc.rhoSea;
-  else acc.DN[c] += Db * (T)sc.rhoSea;
-  // stresses at c
-  const T s11b = k.dyV[c] * (w.X[c] - w.X[AT(ip, j)]);
-  const T s22b = k.dxU[c] * (w.Y[c] - w.Y[AT(i, jp)]);
-  const T s1t = ls1[c] + (T)0.5 * (s11b + s22b);
-  const T s2t = ls2[c] + (T)0.5 * (s11b - s22b);
+  const bool south = fc < (T)0;
+  const T D = (T)(south ? sc.cwS : sc.cwN) * (T)sc.rhoSea;
+  const T ad = (uc - uoc) * miu + (uip - uoip) * miuip;
+  const T bd = (vc - voc) * miv + (vjp - vojp) * mivjp;
+  const T ww = (T)0.25 * (ad * ad + bd * bd);
+  const T sw = sqrt(ww);
+  const bool drag = (D * D) * ww > (T)sc.cDragMin2;
+  const T wb = (drag && sw > (T)0) ? cbe * D * (T)0.5 / sw : (T)0;
+  const T Db = drag ? cbe * sw * (T)sc.rhoSea : (T)0;
+  stg(w.AB, c, wb * (T)0.5 * ad);
+  stg(w.BB, c, wb * (T)0.5 * bd);
+  stg(acc.DS, c, aDS + (south ? Db : (T)0));
+  stg(acc.DN, c, aDN + (south ? (T)0 : Db));
+
+  // sigma1, sigma2 at c
+  const T s11b = dyV * (Xc - Xip);
+  const T s22b = dxU * (Yc - Yjp);
+  const T s1t = l1 + (T)0.5 * (s11b + s22b);
+  const T s2t = l2 + (T)0.5 * (s11b - s22b);
   const T H1 = s1t * (T)sc.den1 * Mc, H2 = s2t * (T)sc.den2 * Mc;
-  ls1o[c] = H1 * (T)sc.alphaM1;
-  ls2o[c] = H2 * (T)sc.alphaM1;
-  const T zeta = w.ZETA[c], dC = w.DELTA[c], ep = w.EP[c], em = w.EM[c];
-  T zb = H1 * (T)2 * ep + H2 * (T)0.5 * em;
+  stg(L.out[2], c, H1 * (T)sc.alphaM1);
+  stg(L.out[3], c, H2 * (T)sc.alphaM1);
   const T pb = -H1;
-  acc.P[c] += pb * (T)sc.tens1m * (T)sc.pr1m;
-  zb += pb * (T)sc.tens1m * (T)2 * dC * (T)sc.pr / (T)sc.tens1p;
-  w.ZBP[c] = zb;
-  w.EPB[c] = H1 * (T)2 * zeta;
-  w.EMB[c] = H2 * (T)0.5 * zeta;
-  w.DBP[c] = pb * (T)sc.tens1m * (T)2 * zeta * (T)sc.pr / (T)sc.tens1p;
-  // sigma12 at z
-  const T s12t = ls12[c] + k.dxV[c] * (w.X[AT(i, jm)] - w.X[c]) + k.dyU[c] * (w.Y[AT(im, j)] - w.Y[c]);
+  const T pfac = (T)sc.tens1m * (T)2 * (T)sc.pr / (T)sc.tens1p;
+  stg(acc.P, c, aP + pb * (T)sc.tens1m * (T)sc.pr1m);
+  stg(w.ZBP, c, H1 * (T)2 * ep + H2 * (T)0.5 * em + pb * pfac * dC);
+  stg(w.EPB, c, H1 * (T)2 * zeta);
+  stg(w.EMB, c, H2 * (T)0.5 * zeta);
+  stg(w.DBP, c, pb * pfac * zeta);
+
+  // sigma12 at z = c
+  const T s12t = l12 + dxV * (Xjm - Xc) + dyU * (Yim - Yc);
   const T G = s12t * (T)sc.den2;
-  ls12o[c] = G * (T)sc.alphaM1;
-  const T zz = znorm(g, k, i, j) * ((w.ZETA[c] + w.ZETA[AT(im, j)]) +
-                                    (w.ZETA[AT(i, jm)] + w.ZETA[AT(im, jm)]));
-  w.ZZB[c] = G * (T)0.5 * w.E12[c];
-  w.E12P[c] = G * (T)0.5 * zz;
+  stg(L.out[4], c, G * (T)sc.alphaM1);
+  const T nz = (Mc + Mim) + (Mjm + Mmm);
+  const T nrm = nz > (T)0 ? (T)1 / nz : (T)0;
+  const T zz = nrm * ((zeta + zim) + (zjm + zmm));
+  stg(w.ZZB, c, G * (T)0.5 * e12 * nrm);  // carries the z-average weight of z = c
+  stg(w.E12P, c, G * (T)0.5 * zz);
 }
 
-// ---- R3: c-points (zeta, Delta)
+// ---- D: c-point zeta and Delta (veris/dynamics_routines.py:220-268)
 template <typename T>
-__global__ __launch_bounds__(BLK) void adj_r3_kernel(Geom g, AdjConst<T> k, EvpScalars sc,
-                                                     AdjWork<T> w, AdjAcc<T> acc) {
+__global__ __launch_bounds__(BLK) void adj_visc_kernel(const AdjArgs<T>* __restrict__ a,
+                                                       const EvpArgBlock<T>* __restrict__ blk,
+                                                       AdjWork<T> w) {
   CELL_SETUP
-  const T zb = w.ZBP[c] + znorm(g, k, i, j) * w.ZZB[c] + znorm(g, k, ip, j) * w.ZZB[AT(ip, j)] +
-               znorm(g, k, i, jp) * w.ZZB[AT(i, jp)] + znorm(g, k, ip, jp) * w.ZZB[AT(ip, jp)];
-  const T dC = w.DELTA[c], zeta = w.ZETA[c];
-  const T dreg = dC + (T)sc.deltaMin;
-  acc.P[c] += zb * (T)0.5 * (T)sc.tens1p / dreg;
-  const T Db = w.DBP[c] - zb * zeta / dreg;
+  const EvpConst<T> k = a->c;
+  const EvpScalars& sc = a->sc;
+  T* const accP = a->acc.P;
+  const T* const* it = blk->interm;
+  const T z0 = ldg(w.ZZB, c), z1 = ldg(w.ZZB, AT(ip, j)), z2 = ldg(w.ZZB, AT(i, jp)),
+          z3 = ldg(w.ZZB, AT(ip, jp));
+  const T zbp = ldg(w.ZBP, c), dbp = ldg(w.DBP, c), epb0 = ldg(w.EPB, c), emb0 = ldg(w.EMB, c);
+  const T zeta = ldg(it[1], c), dC = ldg(it[2], c), ep = ldg(it[3], c), em = ldg(it[4], c);
+  const T mi = ldg(k.mInC, c), rdxU = ldg(k.rdxU, c), rdyV = ldg(k.rdyV, c), rrA = ldg(k.rrA, c);
+  const T aP = ldg(accP, c);
+  const T zb = zbp + ((z0 + z1) + (z2 + z3));
+  const T rreg = (T)1 / (dC + (T)sc.deltaMin);
+  stg(accP, c, aP + zb * (T)0.5 * (T)sc.tens1p * rreg);
+  const T Db = dbp - zb * zeta * rreg;
   const T dsqb = dC > (T)0 ? Db * (T)0.5 / dC : (T)0;
-  const T ep = w.EP[c], em = w.EM[c];
-  const T epb = w.EPB[c] + dsqb * (T)2 * ep;
-  const T emb = w.EMB[c] + dsqb * (T)sc.rpd * (T)2 * em;
-  const T e12cb = dsqb * (T)sc.rpd * (T)4;
-  w.E11B[c] = (epb + emb) * k.rdxU[c] * k.mInC[c];
-  w.E22B[c] = (epb - emb) * k.rdyV[c] * k.mInC[c];
-  w.QB[c] = e12cb * (T)0.25 * k.rrA[c];
+  const T epb = epb0 + dsqb * (T)2 * ep;
+  const T emb = emb0 + dsqb * (T)sc.rpd * (T)2 * em;
+  stg(w.E11B, c, (epb + emb) * rdxU * mi);
+  stg(w.E22B, c, (epb - emb) * rdyV * mi);
+  stg(w.QB, c, dsqb * (T)sc.rpd * rrA);
 }
 
-// ---- R4: z-points (shear strain)
+// Adjoint of e12 at z-point (ii, jj) times its coefficients in
+// e12(z) = A1 u + A2 u[j-1] + A3 v + A4 v[i-1] (veris/dynamics_routines.py:160-175):
+// returns the adjoint e and the coefficients.
 template <typename T>
-__global__ __launch_bounds__(BLK) void adj_r4_kernel(Geom g, AdjConst<T> k, AdjWork<T> w) {
+struct E12Adj {
+  T e, A1, A2, A3, A4;
+};
+
+template <typename T>
+__device__ inline E12Adj<T> e12_adj(const Geom& g, const EvpConst<T>& k, const T* E12,
+                                    const AdjWork<T>& w, int ii, int jj) {
+  const int im = wm(ii, g.NX), jm = wm(jj, g.NY);
+  const int c = AT(ii, jj), cim = AT(im, jj), cjm = AT(ii, jm), cmm = AT(im, jm);
+  const T q0 = ldg(w.QB, c), q1 = ldg(w.QB, cim), q2 = ldg(w.QB, cjm), q3 = ldg(w.QB, cmm);
+  const T e12p = ldg(w.E12P, c), rAz = ldg(k.rAz, c), e12 = ldg(E12, c);
+  const T M0 = ldg(k.M, c), M1 = ldg(k.M, cim), M2 = ldg(k.M, cjm), M3 = ldg(k.M, cmm);
+  const T MU0 = ldg(k.MU, c), MU2 = ldg(k.MU, cjm), MV0 = ldg(k.MV, c), MV1 = ldg(k.MV, cim);
+  const T ry = ldg(k.rdyU, c), rx = ldg(k.rdxV, c);
+  E12Adj<T> o;
+  o.e = e12p + ((q0 + q1) + (q2 + q3)) * (T)2 * rAz * e12;
+  const T mz = (M0 * M1) * (M2 * M3);
+  const T hu = MU0 - MU2, hv = MV0 - MV1;
+  o.A1 = (T)0.5 * mz * ry + hu * ry;
+  o.A2 = -(T)0.5 * mz * ry + hu * ry;
+  o.A3 = (T)0.5 * mz * rx + hv * rx;
+  o.A4 = -(T)0.5 * mz * rx + hv * rx;
+  return o;
+}
+
+// ---- V: shear strain and the adjoint of the substep's input velocities
+template <typename T>
+__global__ __launch_bounds__(BLK) void adj_vel_kernel(const AdjArgs<T>* __restrict__ a,
+                                                      const EvpArgBlock<T>* __restrict__ blk,
+                                                      LamSet<T> L, AdjWork<T> w) {
   CELL_SETUP
-  const T qb = (w.QB[c] + w.QB[AT(im, j)]) + (w.QB[AT(i, jm)] + w.QB[AT(im, jm)]);
-  w.E12B[c] = w.E12P[c] + qb * (T)2 * k.rAz[c] * w.E12[c];
-}
-
-// coefficients of e12(z) = A1 u + A2 u[j-1] + A3 v + A4 v[i-1] at z-point (ii, jj)
-template <typename T>
-__device__ inline void e12_coeffs(const Geom& g, const AdjConst<T>& k, int ii, int jj, T* A1,
-                                  T* A2, T* A3, T* A4) {
-  const int im = ii == 0 ? g.NX - 1 : ii - 1, jm = jj == 0 ? g.NY - 1 : jj - 1;
-  const int c = AT(ii, jj);
-  const T mz = (k.M[c] * k.M[AT(im, jj)]) * (k.M[AT(ii, jm)] * k.M[AT(im, jm)]);
-  const T hu = k.MU[c] - k.MU[AT(ii, jm)], hv = k.MV[c] - k.MV[AT(im, jj)];
-  const T ry = k.rdyU[c], rx = k.rdxV[c];
-  *A1 = (T)0.5 * mz * ry + hu * ry;
-  *A2 = -(T)0.5 * mz * ry + hu * ry;
-  *A3 = (T)0.5 * mz * rx + hv * rx;
-  *A4 = -(T)0.5 * mz * rx + hv * rx;
-}
-
-// ---- R5: adjoint of the substep's input velocities
-template <typename T>
-__global__ __launch_bounds__(BLK) void adj_r5_kernel(Geom g, AdjConst<T> k, AdjWork<T> w,
-                                                     T* luo, T* lvo) {
-  CELL_SETUP
-  T A1, A2, A3, A4, B1, B2, B3, B4, C1, C2, C3, C4;
-  e12_coeffs(g, k, i, j, &A1, &A2, &A3, &A4);
-  e12_coeffs(g, k, i, jp, &B1, &B2, &B3, &B4);
-  e12_coeffs(g, k, ip, j, &C1, &C2, &C3, &C4);
-  const T fmc = k.fC[c] * k.mC[c];
-  luo[c] = w.UDIR[c] + (w.AB[c] + w.AB[AT(im, j)]) * k.mInU[c] + (w.E11B[AT(im, j)] - w.E11B[c]) +
-           w.E12B[c] * A1 + w.E12B[AT(i, jp)] * B2 + (T)0.5 * fmc * w.FUB[c] +
-           (T)0.5 * k.fC[AT(im, j)] * k.mC[AT(im, j)] * w.FUB[AT(im, j)];
-  lvo[c] = w.VDIR[c] + (w.BB[c] + w.BB[AT(i, jm)]) * k.mInV[c] + (w.E22B[AT(i, jm)] - w.E22B[c]) +
-           w.E12B[c] * A3 + w.E12B[AT(ip, j)] * C4 + (T)0.5 * fmc * w.FVB[c] +
-           (T)0.5 * k.fC[AT(i, jm)] * k.mC[AT(i, jm)] * w.FVB[AT(i, jm)];
-}
-
-template <typename T>
-__global__ __launch_bounds__(BLK) void copy_kernel(int n, const T* a, T* b) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c < n) b[c] = a ? a[c] : (T)0;
+  const EvpConst<T> k = a->c;
+  const T* E12 = blk->interm[0];
+  const E12Adj<T> zc = e12_adj(g, k, E12, w, i, j);
+  const E12Adj<T> zjp = e12_adj(g, k, E12, w, i, jp);
+  const E12Adj<T> zip = e12_adj(g, k, E12, w, ip, j);
+  const int cim = AT(im, j), cjm = AT(i, jm);
+  const T udir = ldg(w.UDIR, c), vdir = ldg(w.VDIR, c);
+  const T ab0 = ldg(w.AB, c), ab1 = ldg(w.AB, cim), bb0 = ldg(w.BB, c), bb1 = ldg(w.BB, cjm);
+  const T e11a = ldg(w.E11B, cim), e11b = ldg(w.E11B, c), e22a = ldg(w.E22B, cjm),
+          e22b = ldg(w.E22B, c);
+  const T fub0 = ldg(w.FUB, c), fub1 = ldg(w.FUB, cim), fvb0 = ldg(w.FVB, c), fvb1 = ldg(w.FVB, cjm);
+  const T fc = ldg(k.fC, c), fim = ldg(k.fC, cim), fjm = ldg(k.fC, cjm);
+  const T mC = ldg(k.massC, c), mCim = ldg(k.massC, cim), mCjm = ldg(k.massC, cjm);
+  const T miu = ldg(k.mInU, c), miv = ldg(k.mInV, c);
+  stg(L.out[0], c,
+      udir + (ab0 + ab1) * miu + (e11a - e11b) + zc.e * zc.A1 + zjp.e * zjp.A2 +
+          (T)0.5 * fc * mC * fub0 + (T)0.5 * fim * mCim * fub1);
+  stg(L.out[1], c,
+      vdir + (bb0 + bb1) * miv + (e22a - e22b) + zc.e * zc.A3 + zip.e * zip.A4 +
+          (T)0.5 * fc * mC * fvb0 + (T)0.5 * fjm * mCjm * fvb1);
 }
 
 // ---- pre-EVP adjoint: strength, mass, wind forcing
@@ -327,14 +424,15 @@ struct PreAdjArgs {
   const T *aP, *amC, *amU, *amV, *aWFX, *aWFY;
   // outputs
   T *apstar, *ah, *ahs, *aCaN, *aCaS;
-  T pstar, cStar, rhoI, rhoS, rhoAir, caN, caS, wmin, wmin2, grav, rrho, sinWin, cosWin;
+  T pstar, cStar, rhoI, rhoS, rhoAir, wmin, wmin2, grav, rrho, sinWin, cosWin;
   int relWind;
 };
 
+// d tau_c / d Ca at c-point (i, j) (veris/dynsolver.py:7-47)
 template <typename T>
-__device__ inline void tau_c_and_dca(const Geom& g, const PreAdjArgs<T>& a, int i, int j, T* dtx,
-                                     T* dty) {
-  const int ip = i == g.NX - 1 ? 0 : i + 1, jp = j == g.NY - 1 ? 0 : j + 1;
+__device__ inline void tau_c_dca(const Geom& g, const PreAdjArgs<T>& a, int i, int j, T* dtx,
+                                 T* dty) {
+  const int ip = wp(i, g.NX), jp = wp(j, g.NY);
   const int c = AT(i, j);
   T urel = a.uW[c], vrel = a.vW[c];
   if (a.relWind) {
@@ -344,107 +442,85 @@ __device__ inline void tau_c_and_dca(const Geom& g, const PreAdjArgs<T>& a, int 
   const T wsq = urel * urel + vrel * vrel;
   const T ws = wsq < a.wmin2 ? a.wmin : sqrt(wsq);
   const T sg = sgn(a.fC[c]);
-  // d tau / d Ca (Ca of this cell's hemisphere)
   *dtx = (a.rhoAir * ws) * (a.cosWin * urel - (sg * a.sinWin) * vrel);
   *dty = (a.rhoAir * ws) * (a.cosWin * vrel + (sg * a.sinWin) * urel);
 }
 
 template <typename T>
 __global__ __launch_bounds__(BLK) void pre_evp_adjoint_kernel(Geom g, PreAdjArgs<T> a) {
-  CELL_SETUP
-  // strength: P = P* h exp(-C*(1-A)) M
+  CELL_SETUP_G
+  // strength: P = P* h exp(-C (1 - A)) M
   const T E = exp((-a.cStar) * ((T)1 - a.A[c]));
   const T ps = a.pstarF ? a.pstarF[c] : a.pstar;
   a.apstar[c] = a.aP[c] * a.h[c] * E * a.M[c];
   T hb = a.aP[c] * ps * E * a.M[c];
-  // wind forcing: tilt term -mU rdxC (phi - phi[i-1]) adds to the mass adjoints
-  const T phi = a.grav * a.ssh[c] + a.sp[c] * a.rrho;
-  auto phi_at = [&](int ii, int jj) { return a.grav * a.ssh[AT(ii, jj)] + a.sp[AT(ii, jj)] * a.rrho; };
-  const T amU_t = a.amU[c] - a.aWFX[c] * a.rdxC[c] * (phi - phi_at(im, j));
+  // wind forcing: the tilt term -massU rdxC (phi - phi[i-1]) adds to the mass adjoints
+  auto phi_at = [&](int ii, int jj) {
+    return a.grav * a.ssh[AT(ii, jj)] + a.sp[AT(ii, jj)] * a.rrho;
+  };
+  const T phi = phi_at(i, j);
+  const T amU_c = a.amU[c] - a.aWFX[c] * a.rdxC[c] * (phi - phi_at(im, j));
   const T amU_ip = a.amU[AT(ip, j)] - a.aWFX[AT(ip, j)] * a.rdxC[AT(ip, j)] * (phi_at(ip, j) - phi);
-  const T amV_t = a.amV[c] - a.aWFY[c] * a.rdyC[c] * (phi - phi_at(i, jm));
+  const T amV_c = a.amV[c] - a.aWFY[c] * a.rdyC[c] * (phi - phi_at(i, jm));
   const T amV_jp = a.amV[AT(i, jp)] - a.aWFY[AT(i, jp)] * a.rdyC[AT(i, jp)] * (phi_at(i, jp) - phi);
-  // mass: mC = rhoI h + rhoS hs, mU = (mC + mC[i-1])/2, mV = (mC + mC[j-1])/2
-  const T amC = a.amC[c] + (T)0.5 * (amU_t + amU_ip) + (T)0.5 * (amV_t + amV_jp);
-  hb += a.rhoI * amC;
-  a.ah[c] = hb;
+  // mass: massC = rhoI h + rhoS hs, massU = (massC + massC[i-1])/2, massV likewise in j
+  const T amC = a.amC[c] + (T)0.5 * (amU_c + amU_ip) + (T)0.5 * (amV_c + amV_jp);
+  a.ah[c] = hb + a.rhoI * amC;
   a.ahs[c] = a.rhoS * amC;
-  // air drag: WFX = 0.5 (tau_c + tau_c[i-1]) MU AW + tilt, tau_c proportional to Ca
+  // air drag: WindForcingX = 0.5 (tau_c + tau_c[i-1]) MU AreaW + tilt, tau_c linear in Ca
   T dtx, dty;
-  tau_c_and_dca(g, a, i, j, &dtx, &dty);
-  const T gx = (T)0.5 * (a.aWFX[c] * a.AW[c] * a.MU[c] + a.aWFX[AT(ip, j)] * a.AW[AT(ip, j)] * a.MU[AT(ip, j)]);
-  const T gy = (T)0.5 * (a.aWFY[c] * a.AS[c] * a.MV[c] + a.aWFY[AT(i, jp)] * a.AS[AT(i, jp)] * a.MV[AT(i, jp)]);
+  tau_c_dca(g, a, i, j, &dtx, &dty);
+  const T gx = (T)0.5 * (a.aWFX[c] * a.AW[c] * a.MU[c] +
+                         a.aWFX[AT(ip, j)] * a.AW[AT(ip, j)] * a.MU[AT(ip, j)]);
+  const T gy = (T)0.5 * (a.aWFY[c] * a.AS[c] * a.MV[c] +
+                         a.aWFY[AT(i, jp)] * a.AS[AT(i, jp)] * a.MV[AT(i, jp)]);
   const T contrib = gx * dtx + gy * dty;
   a.aCaN[c] = a.fC[c] < (T)0 ? (T)0 : contrib;
   a.aCaS[c] = a.fC[c] < (T)0 ? contrib : (T)0;
 }
 
 // ---- host side
-template <typename T>
 struct AdjLayout {
-  size_t fb, abb, total;
-  int nck, ce;
-  size_t off_ck, off_seg, off_work, off_lam, off_ab, off_fwd;
+  size_t fb, blk, total;
+  int nck, ce, n;
+  size_t off_args, off_fblk, off_rblk, off_ckpt, off_pp, off_seg, off_interm, off_work, off_lam;
 };
 
 template <typename T>
-AdjLayout<T> adj_layout(const Geom& g, int nsub, int ce) {
-  AdjLayout<T> L;
+AdjLayout adj_layout(const Geom& g, int n, int ce) {
+  AdjLayout L;
   L.fb = align256(field_elems(g) * sizeof(T));
-  L.abb = align256(sizeof(EvpArgBlock<T>));
+  L.blk = align256(sizeof(EvpArgBlock<T>));
   L.ce = ce;
-  L.nck = (nsub + ce - 1) / ce;
-  L.off_ck = 0;
-  L.off_seg = L.off_ck + (size_t)5 * L.nck * L.fb;
-  L.off_work = L.off_seg + (size_t)5 * (ce + 1) * L.fb;
-  L.off_lam = L.off_work + 28 * L.fb;
-  L.off_ab = L.off_lam + 10 * L.fb;
-  L.off_fwd = L.off_ab + L.abb;  // workspace of the fused forward launches
-  L.total = L.off_fwd + 5 * L.fb + L.abb;  // = the single-domain solver's workspace
+  L.n = n;
+  L.nck = (n + ce - 1) / ce;
+  L.off_args = 0;
+  L.off_fblk = align256(sizeof(AdjArgs<T>));
+  L.off_rblk = L.off_fblk + (size_t)n * L.blk;
+  L.off_ckpt = L.off_rblk + (size_t)n * L.blk;
+  L.off_pp = L.off_ckpt + (size_t)5 * L.nck * L.fb;  // checkpoint slot 0 is unused
+  L.off_seg = L.off_pp + (size_t)10 * L.fb;
+  L.off_interm = L.off_seg + (size_t)5 * (ce + 1) * L.fb;  // segment slot 0 is unused
+  L.off_work = L.off_interm + (size_t)6 * ce * L.fb;
+  L.off_lam = L.off_work + (size_t)kAdjWorkFields * L.fb;
+  L.total = L.off_lam + (size_t)10 * L.fb;
   return L;
 }
 
 template <typename T>
-T* fld(void* ws, size_t off, size_t fb, int q) {
+T* wsf(void* ws, size_t off, size_t fb, int q) {
   return (T*)((char*)ws + off + (size_t)q * fb);
 }
 
-template <typename T>
-AdjConst<T> adj_const(const vh_fields* f) {
-  AdjConst<T> k;
-  k.M = evp_ptr<T>(f->iceMask);
-  k.MU = evp_ptr<T>(f->iceMaskU);
-  k.MV = evp_ptr<T>(f->iceMaskV);
-  k.mInC = evp_ptr<T>(f->maskInC);
-  k.mInU = evp_ptr<T>(f->maskInU);
-  k.mInV = evp_ptr<T>(f->maskInV);
-  k.rdxU = evp_ptr<T>(f->recip_dxU);
-  k.rdyV = evp_ptr<T>(f->recip_dyV);
-  k.rdyU = evp_ptr<T>(f->recip_dyU);
-  k.rdxV = evp_ptr<T>(f->recip_dxV);
-  k.dxU = evp_ptr<T>(f->dxU);
-  k.dyU = evp_ptr<T>(f->dyU);
-  k.dxV = evp_ptr<T>(f->dxV);
-  k.dyV = evp_ptr<T>(f->dyV);
-  k.rrAu = evp_ptr<T>(f->recip_rAu);
-  k.rrAv = evp_ptr<T>(f->recip_rAv);
-  k.rAz = evp_ptr<T>(f->rAz);
-  k.rrA = evp_ptr<T>(f->recip_rA);
-  k.fC = evp_ptr<T>(f->fCori);
-  k.uOc = evp_ptr<T>(f->uOcean);
-  k.vOc = evp_ptr<T>(f->vOcean);
-  k.P = evp_ptr<T>(f->SeaIceStrength);
-  k.mC = evp_ptr<T>(f->SeaIceMassC);
-  k.mU = evp_ptr<T>(f->SeaIceMassU);
-  k.mV = evp_ptr<T>(f->SeaIceMassV);
-  k.AW = evp_ptr<T>(f->AreaW);
-  k.AS = evp_ptr<T>(f->AreaS);
-  k.uNm1 = evp_ptr<T>(f->uIce);
-  k.vNm1 = evp_ptr<T>(f->vIce);
-  return k;
+int pick_adj_tile(const Geom& g, int req) {
+  if (req > 0) return req;
+  return (long long)g.NX * g.NY < 400000 ? 5 : 3;
 }
 
-int check_adj_settings(const vh_domain* d, const vh_settings* s, const vh_fields* f) {
+int check_adj(const vh_domain* d, const vh_settings* s, const vh_fields* f, int n, int ce) {
+  VH_REQUIRE(d && s && f, VH_EINVAL, "null domain, settings or fields");
+  VH_REQUIRE(n >= 1 && ce >= 1, VH_EINVAL, "n_substeps=%d ckpt_every=%d", n, ce);
+  VH_REQUIRE(n <= (1 << 20), VH_EINVAL, "n_substeps=%d too large", n);
   VH_REQUIRE(d->tile >= 0 && d->tile < kEvpNumTiles && (d->tile == 0 || kEvpTiles[d->tile].k == 1),
              VH_EINVAL, "adjoint: tile %d (needs a single-substep tile shape or 0)", d->tile);
   VH_REQUIRE(s->useAdaptiveEVP == 0.0 && s->noSlip != 0.0 && s->secondOrderBC == 0.0 &&
@@ -454,113 +530,172 @@ int check_adj_settings(const vh_domain* d, const vh_settings* s, const vh_fields
   VH_REQUIRE(s->basalDragK2 == 0.0, VH_EUNSUPPORTED, "adjoint: basal drag not supported");
   VH_REQUIRE(!f->k1AtC && !f->k2AtC && !f->k1AtZ && !f->k2AtZ, VH_EUNSUPPORTED,
              "adjoint: metric terms k1/k2 not supported (pass NULL for all-zero fields)");
+  const void* req[kEvpMaxRequired];
+  const int nreq = evp_required_fields(f, req);
+  for (int q = 0; q < nreq; ++q)
+    VH_REQUIRE(req[q] != nullptr, VH_EINVAL, "adjoint: missing input field");
   return VH_OK;
 }
 
 template <typename T>
-int forward_ckpt(const Geom& g, const vh_settings* s, const vh_fields* f, int nsub, int ce,
-                 void* const outs[5], void* ws, size_t wsb, int tile, int dtype, hipStream_t st) {
-  const AdjLayout<T> L = adj_layout<T>(g, nsub, ce);
-  VH_REQUIRE(ws && wsb >= L.total, VH_EWORKSPACE, "adjoint workspace: need %zu bytes, got %zu",
-             L.total, wsb);
-  const T* in0[5] = {evp_ptr<T>(f->uIce), evp_ptr<T>(f->vIce), evp_ptr<T>(f->sigma1),
-                     evp_ptr<T>(f->sigma2), evp_ptr<T>(f->sigma12)};
-  const size_t nb = field_elems(g) * sizeof(T);
-  for (int q = 0; q < 5; ++q)
-    VH_TRY_HIP(hipMemcpyAsync(fld<T>(ws, L.off_ck, L.fb, q), in0[q], nb, hipMemcpyDeviceToDevice, st));
-  for (int sgm = 0; sgm < L.nck; ++sgm) {
-    const int k0 = sgm * ce, n = std::min(ce, nsub - k0);
-    const T* in[5];
-    T* out[5];
-    for (int q = 0; q < 5; ++q) {
-      in[q] = fld<T>(ws, L.off_ck, L.fb, 5 * sgm + q);
-      out[q] = (sgm + 1 < L.nck) ? fld<T>(ws, L.off_ck, L.fb, 5 * (sgm + 1) + q) : (T*)outs[q];
-    }
-    int rc = evp_run_core<T>(g, s, f, in, out, n, (char*)ws + L.off_fwd, L.total - L.off_fwd,
-                             tile, dtype, st);
-    if (rc) return rc;
-  }
-  return VH_OK;
+EvpArgBlock<T> proto_block(const Geom& g, const vh_settings* s, const vh_fields* f) {
+  EvpArgBlock<T> ab{};
+  fill_evp_scalars(s, 0, 0, &ab.sc);
+  fill_evp_const(f, &ab.c);
+  ab.g = g;
+  ab.nlaunch = 1;
+  ab.members = nullptr;
+  return ab;
 }
 
 template <typename T>
-int adjoint(const Geom& g, const vh_settings* s, const vh_fields* f, int nsub, int ce,
-            const void* const lam_final[5], const vh_evp_adjoint_out* o, void* ws, size_t wsb,
-            int tile, int dtype, hipStream_t st) {
-  const AdjLayout<T> L = adj_layout<T>(g, nsub, ce);
-  VH_REQUIRE(ws && wsb >= L.total, VH_EWORKSPACE, "adjoint workspace: need %zu bytes, got %zu",
-             L.total, wsb);
-  const int n = g.NX * g.NY;
-  const dim3 grid((n + BLK - 1) / BLK);
-  AdjConst<T> k = adj_const<T>(f);
-  EvpScalars sc;
-  fill_evp_scalars(s, 0, 0, &sc);
-  AdjWork<T> w;
-  T** wp[] = {&w.E12, &w.ZETA, &w.DELTA, &w.EP, &w.EM, &w.CD, &w.X, &w.Y, &w.FX, &w.FY,
-              &w.CDU, &w.CDV, &w.UDIR, &w.VDIR, &w.ZBP, &w.EPB, &w.EMB, &w.DBP, &w.AB, &w.BB,
-              &w.FUB, &w.FVB, &w.ZZB, &w.E12P, &w.E11B, &w.E22B, &w.QB, &w.E12B};
-  for (int q = 0; q < 28; ++q) *wp[q] = fld<T>(ws, L.off_work, L.fb, q);
-  AdjAcc<T> acc{(T*)o->adj_SeaIceStrength, (T*)o->adj_SeaIceMassC, (T*)o->adj_SeaIceMassU,
-                (T*)o->adj_SeaIceMassV, (T*)o->adj_AreaW, (T*)o->adj_AreaS,
-                (T*)o->adj_WindForcingX, (T*)o->adj_WindForcingY, (T*)o->adj_uNm1,
-                (T*)o->adj_vNm1, (T*)o->adj_waterDrag_north, (T*)o->adj_waterDrag_south};
-  T* accp[] = {acc.P, acc.mC, acc.mU, acc.mV, acc.AW, acc.AS, acc.WFX, acc.WFY, acc.u0, acc.v0,
-               acc.DN, acc.DS};
-  for (T* p : accp) {
-    VH_REQUIRE(p != nullptr, VH_EINVAL, "adjoint: null parameter-adjoint output");
-    hipLaunchKernelGGL(copy_kernel<T>, grid, dim3(BLK), 0, st, n, (const T*)nullptr, p);
-  }
-  // lambda ping-pong: set A holds the adjoint of the substep output
-  T* lamA[5];
-  T* lamB[5];
+BlockPlan<T> block_plan(const AdjLayout& L, const vh_fields* f, void* const outs[5], void* ws) {
+  BlockPlan<T> p{};
+  const void* in0[5] = {f->uIce, f->vIce, f->sigma1, f->sigma2, f->sigma12};
   for (int q = 0; q < 5; ++q) {
-    lamA[q] = fld<T>(ws, L.off_lam, L.fb, q);
-    lamB[q] = fld<T>(ws, L.off_lam, L.fb, 5 + q);
-    hipLaunchKernelGGL(copy_kernel<T>, grid, dim3(BLK), 0, st, n, (const T*)lam_final[q], lamA[q]);
+    p.in0[q] = (const T*)in0[q];
+    p.outs[q] = outs ? (T*)outs[q] : nullptr;
   }
-  for (int sgm = L.nck - 1; sgm >= 0; --sgm) {
-    const int k0 = sgm * ce, len = std::min(ce, nsub - k0);
-    // recompute the segment's states seg[0..len] from its checkpoint
-    for (int q = 0; q < 5; ++q)
-      VH_TRY_HIP(hipMemcpyAsync(fld<T>(ws, L.off_seg, L.fb, q), fld<T>(ws, L.off_ck, L.fb, 5 * sgm + q),
-                                field_elems(g) * sizeof(T), hipMemcpyDeviceToDevice, st));
-    for (int kk = 0; kk < len; ++kk) {
-      const T* in[5];
-      T* out[5];
-      for (int q = 0; q < 5; ++q) {
-        in[q] = fld<T>(ws, L.off_seg, L.fb, 5 * kk + q);
-        out[q] = fld<T>(ws, L.off_seg, L.fb, 5 * (kk + 1) + q);
-      }
-      int rc = evp_run_core<T>(g, s, f, in, out, 1, (char*)ws + L.off_fwd, L.total - L.off_fwd,
-                               tile, dtype, st);
-      if (rc) return rc;
-    }
-    // reverse the segment's substeps
-    for (int kk = len - 1; kk >= 0; --kk) {
-      const T* xu = fld<T>(ws, L.off_seg, L.fb, 5 * kk + 0);
-      const T* xv = fld<T>(ws, L.off_seg, L.fb, 5 * kk + 1);
-      const T* yu = fld<T>(ws, L.off_seg, L.fb, 5 * (kk + 1) + 0);
-      const T* yv = fld<T>(ws, L.off_seg, L.fb, 5 * (kk + 1) + 1);
-      hipLaunchKernelGGL(adj_interm_kernel<T>, grid, dim3(BLK), 0, st, g, k, sc, xu, xv, w);
-      hipLaunchKernelGGL(adj_r1_kernel<T>, grid, dim3(BLK), 0, st, g, k, sc, xu, xv, yu, yv,
-                         (const T*)lamA[0], (const T*)lamA[1], w, acc);
-      hipLaunchKernelGGL(adj_r2_kernel<T>, grid, dim3(BLK), 0, st, g, k, sc, xu, xv,
-                         (const T*)lamA[2], (const T*)lamA[3], (const T*)lamA[4], lamB[2], lamB[3],
-                         lamB[4], w, acc);
-      hipLaunchKernelGGL(adj_r3_kernel<T>, grid, dim3(BLK), 0, st, g, k, sc, w, acc);
-      hipLaunchKernelGGL(adj_r4_kernel<T>, grid, dim3(BLK), 0, st, g, k, w);
-      hipLaunchKernelGGL(adj_r5_kernel<T>, grid, dim3(BLK), 0, st, g, k, w, lamB[0], lamB[1]);
-      for (int q = 0; q < 5; ++q) std::swap(lamA[q], lamB[q]);
-    }
+  p.ws = (char*)ws;
+  p.fb = L.fb;
+  p.off_ckpt = L.off_ckpt;
+  p.off_pp = L.off_pp;
+  p.off_seg = L.off_seg;
+  p.off_interm = L.off_interm;
+  p.n = L.n;
+  p.ce = L.ce;
+  return p;
+}
+
+void tiles_of(int cfg, const Geom& g, int* n_tiles, int* tiles_j) {
+  const int TI = evp_tile_extent_i(cfg), TJ = evp_tile_extent_j(cfg);
+  *tiles_j = (g.NY + TJ - 1) / TJ;
+  *n_tiles = ((g.NX + TI - 1) / TI) * *tiles_j;
+}
+
+template <typename T>
+int run_recorded(const char* kind, const Geom& g, const AdjLayout& L, int cfg, void* ws,
+                 const std::function<void(hipStream_t)>& record, hipStream_t st) {
+  if (graphs_enabled()) {
+    char key[192];
+    snprintf(key, sizeof(key), "%s/%d/%d/%d/%d/%d/%d/%d/%d/%d/%p", kind, (int)sizeof(T), cfg, L.n,
+             L.ce, g.NX, g.NY, g.nx, g.fill_x, g.fill_y, ws);
+    return graph_run(key, record, st);
   }
-  T* outs[5] = {(T*)o->adj_uIce, (T*)o->adj_vIce, (T*)o->adj_sigma1, (T*)o->adj_sigma2,
-                (T*)o->adj_sigma12};
-  for (int q = 0; q < 5; ++q) {
-    VH_REQUIRE(outs[q] != nullptr, VH_EINVAL, "adjoint: null state-adjoint output");
-    hipLaunchKernelGGL(copy_kernel<T>, grid, dim3(BLK), 0, st, n, (const T*)lamA[q], outs[q]);
-  }
+  record(st);
   VH_TRY_HIP(hipGetLastError());
   return VH_OK;
+}
+
+template <typename T>
+int forward_ckpt(const Geom& g, const vh_domain* d, const vh_settings* s, const vh_fields* f,
+                 int n, int ce, void* const outs[5], void* ws, size_t wsb, hipStream_t st) {
+  const AdjLayout L = adj_layout<T>(g, n, ce);
+  VH_REQUIRE(ws && wsb >= L.total, VH_EWORKSPACE, "adjoint workspace: need %zu bytes, got %zu",
+             L.total, wsb);
+  const void* req[kEvpMaxRequired];
+  const int nreq = evp_required_fields(f, req);
+  for (int q = 0; q < 5; ++q) {
+    VH_REQUIRE(outs[q] != nullptr, VH_EINVAL, "null output");
+    for (int r = 0; r < nreq; ++r)
+      VH_REQUIRE(outs[q] != req[r], VH_EINVAL, "output aliases an input field");
+    for (int r = 0; r < q; ++r) VH_REQUIRE(outs[q] != outs[r], VH_EINVAL, "two outputs alias");
+  }
+  const int cfg = pick_adj_tile(g, d->tile);
+  int n_tiles, tiles_j;
+  tiles_of(cfg, g, &n_tiles, &tiles_j);
+  EvpArgBlock<T>* fblk = (EvpArgBlock<T>*)((char*)ws + L.off_fblk);
+  AdjArgs<T> unused{};
+  hipLaunchKernelGGL(adj_write_blocks_kernel<T>, dim3((n + 63) / 64), dim3(64), 0, st, fblk, L.blk,
+                     proto_block<T>(g, s, f), block_plan<T>(L, f, outs, ws), 0,
+                     (AdjArgs<T>*)nullptr, unused);
+  VH_TRY_HIP(hipGetLastError());
+  auto record = [&](hipStream_t cs) {
+    for (int kk = 0; kk < n; ++kk)
+      evp_launch_single<T, double, false>(
+          cfg, (const EvpArgBlock<T>*)((char*)fblk + (size_t)kk * L.blk), n_tiles, tiles_j, cs);
+  };
+  return run_recorded<T>("adjf", g, L, cfg, ws, record, st);
+}
+
+template <typename T>
+int adjoint(const Geom& g, const vh_domain* d, const vh_settings* s, const vh_fields* f, int n,
+            int ce, const void* const lam_final[5], const vh_evp_adjoint_out* o, void* ws,
+            size_t wsb, hipStream_t st) {
+  const AdjLayout L = adj_layout<T>(g, n, ce);
+  VH_REQUIRE(ws && wsb >= L.total, VH_EWORKSPACE, "adjoint workspace: need %zu bytes, got %zu",
+             L.total, wsb);
+  AdjArgs<T> av{};
+  fill_evp_const(f, &av.c);
+  fill_evp_scalars(s, 0, 0, &av.sc);
+  av.g = g;
+  av.acc = AdjAcc<T>{(T*)o->adj_SeaIceStrength, (T*)o->adj_SeaIceMassC, (T*)o->adj_SeaIceMassU,
+                     (T*)o->adj_SeaIceMassV, (T*)o->adj_AreaW, (T*)o->adj_AreaS,
+                     (T*)o->adj_WindForcingX, (T*)o->adj_WindForcingY, (T*)o->adj_uNm1,
+                     (T*)o->adj_vNm1, (T*)o->adj_waterDrag_north, (T*)o->adj_waterDrag_south};
+  const void* accp[kAdjAccFields] = {av.acc.P, av.acc.mC, av.acc.mU, av.acc.mV,
+                                     av.acc.AW, av.acc.AS, av.acc.WFX, av.acc.WFY,
+                                     av.acc.u0, av.acc.v0, av.acc.DN, av.acc.DS};
+  const void* outs[5] = {o->adj_uIce, o->adj_vIce, o->adj_sigma1, o->adj_sigma2, o->adj_sigma12};
+  for (const void* p : accp) VH_REQUIRE(p != nullptr, VH_EINVAL, "adjoint: null output");
+  for (const void* p : outs) VH_REQUIRE(p != nullptr, VH_EINVAL, "adjoint: null output");
+  for (int q = 0; q < 5; ++q) {
+    av.lam_final[q] = (const T*)lam_final[q];
+    av.outs[q] = (T*)outs[q];
+  }
+  AdjArgs<T>* a_dev = (AdjArgs<T>*)((char*)ws + L.off_args);
+  EvpArgBlock<T>* rblk = (EvpArgBlock<T>*)((char*)ws + L.off_rblk);
+  hipLaunchKernelGGL(adj_write_blocks_kernel<T>, dim3((n + 63) / 64), dim3(64), 0, st, rblk, L.blk,
+                     proto_block<T>(g, s, f), block_plan<T>(L, f, nullptr, ws), 1, a_dev, av);
+  VH_TRY_HIP(hipGetLastError());
+
+  const int cfg = pick_adj_tile(g, d->tile);
+  int n_tiles, tiles_j;
+  tiles_of(cfg, g, &n_tiles, &tiles_j);
+  AdjWork<T> w;
+  T** wp_[kAdjWorkFields] = {&w.UDIR, &w.VDIR, &w.ZBP, &w.EPB, &w.EMB, &w.DBP, &w.AB, &w.BB,
+                             &w.FUB, &w.FVB, &w.ZZB, &w.E12P, &w.E11B, &w.E22B, &w.QB};
+  for (int q = 0; q < kAdjWorkFields; ++q) *wp_[q] = wsf<T>(ws, L.off_work, L.fb, q);
+  T* lam[2][5];
+  for (int q = 0; q < 5; ++q) {
+    lam[0][q] = wsf<T>(ws, L.off_lam, L.fb, q);
+    lam[1][q] = wsf<T>(ws, L.off_lam, L.fb, 5 + q);
+  }
+  const dim3 grid((unsigned)((field_elems(g) + BLK - 1) / BLK));
+  auto sets = [&](int cur) {
+    LamSet<T> ls;
+    for (int q = 0; q < 5; ++q) {
+      ls.in[q] = lam[cur][q];
+      ls.out[q] = lam[1 - cur][q];
+    }
+    return ls;
+  };
+  auto record = [&](hipStream_t cs) {
+    int cur = 0;
+    LamSet<T> init;
+    for (int q = 0; q < 5; ++q) {
+      init.in[q] = nullptr;
+      init.out[q] = lam[cur][q];
+    }
+    hipLaunchKernelGGL(adj_init_kernel<T>, grid, dim3(BLK), 0, cs, a_dev, init);
+    for (int sgm = L.nck - 1; sgm >= 0; --sgm) {
+      const int k0 = sgm * ce, len = std::min(ce, n - k0);
+      for (int jj = 0; jj < len; ++jj)
+        evp_launch_single<T, double, true>(
+            cfg, (const EvpArgBlock<T>*)((char*)rblk + (size_t)(k0 + jj) * L.blk), n_tiles,
+            tiles_j, cs);
+      for (int jj = len - 1; jj >= 0; --jj) {
+        const EvpArgBlock<T>* b = (const EvpArgBlock<T>*)((char*)rblk + (size_t)(k0 + jj) * L.blk);
+        const LamSet<T> ls = sets(cur);
+        hipLaunchKernelGGL(adj_stress_kernel<T>, grid, dim3(BLK), 0, cs, a_dev, b, ls, w);
+        hipLaunchKernelGGL(adj_visc_kernel<T>, grid, dim3(BLK), 0, cs, a_dev, b, w);
+        hipLaunchKernelGGL(adj_vel_kernel<T>, grid, dim3(BLK), 0, cs, a_dev, b, ls, w);
+        cur = 1 - cur;
+      }
+    }
+    hipLaunchKernelGGL(adj_finish_kernel<T>, grid, dim3(BLK), 0, cs, a_dev, sets(cur));
+  };
+  return run_recorded<T>("adjr", g, L, cfg, ws, record, st);
 }
 
 template <typename T>
@@ -606,8 +741,6 @@ int pre_evp_adjoint(const Geom& g, const vh_settings* s, const vh_fields* f, con
   a.rhoI = (T)s->rhoIce;
   a.rhoS = (T)s->rhoSnow;
   a.rhoAir = (T)s->rhoAir;
-  a.caN = (T)s->airIceDrag;
-  a.caS = (T)s->airIceDrag_south;
   a.wmin = (T)s->wSpeedMin;
   a.wmin2 = (T)(s->wSpeedMin * s->wSpeedMin);
   a.grav = (T)s->gravity;
@@ -623,6 +756,7 @@ int pre_evp_adjoint(const Geom& g, const vh_settings* s, const vh_fields* f, con
 }
 
 #undef CELL_SETUP
+#undef CELL_SETUP_G
 #undef AT
 
 }  // namespace
@@ -647,17 +781,15 @@ extern "C" int vh_evp_forward_checkpointed(const vh_domain* d, const vh_settings
   vh::Geom g;
   int rc = vh::make_geom(d, &g);
   if (rc) return rc;
-  VH_REQUIRE(d && s && f && n_substeps >= 1 && ckpt_every >= 1, VH_EINVAL, "bad arguments");
-  rc = vh::check_adj_settings(d, s, f);
+  rc = vh::check_adj(d, s, f, n_substeps, ckpt_every);
   if (rc) return rc;
   void* outs[5] = {uIce, vIce, sigma1, sigma2, sigma12};
-  for (void* p : outs) VH_REQUIRE(p != nullptr, VH_EINVAL, "null output");
   hipStream_t st = (hipStream_t)stream;
   if (d->dtype == VH_F64)
-    return vh::forward_ckpt<double>(g, s, f, n_substeps, ckpt_every, outs, workspace,
-                                    workspace_bytes, d->tile, d->dtype, st);
-  return vh::forward_ckpt<float>(g, s, f, n_substeps, ckpt_every, outs, workspace,
-                                 workspace_bytes, d->tile, d->dtype, st);
+    return vh::forward_ckpt<double>(g, d, s, f, n_substeps, ckpt_every, outs, workspace,
+                                    workspace_bytes, st);
+  return vh::forward_ckpt<float>(g, d, s, f, n_substeps, ckpt_every, outs, workspace,
+                                 workspace_bytes, st);
 }
 
 extern "C" int vh_evp_adjoint(const vh_domain* d, const vh_settings* s, const vh_fields* f,
@@ -668,16 +800,16 @@ extern "C" int vh_evp_adjoint(const vh_domain* d, const vh_settings* s, const vh
   vh::Geom g;
   int rc = vh::make_geom(d, &g);
   if (rc) return rc;
-  VH_REQUIRE(s && f && out && n_substeps >= 1 && ckpt_every >= 1, VH_EINVAL, "bad arguments");
-  rc = vh::check_adj_settings(d, s, f);
+  rc = vh::check_adj(d, s, f, n_substeps, ckpt_every);
   if (rc) return rc;
+  VH_REQUIRE(out != nullptr, VH_EINVAL, "null output struct");
   const void* lam[5] = {adj_uIce, adj_vIce, adj_sigma1, adj_sigma2, adj_sigma12};
   hipStream_t st = (hipStream_t)stream;
   if (d->dtype == VH_F64)
-    return vh::adjoint<double>(g, s, f, n_substeps, ckpt_every, lam, out, workspace,
-                               workspace_bytes, d->tile, d->dtype, st);
-  return vh::adjoint<float>(g, s, f, n_substeps, ckpt_every, lam, out, workspace, workspace_bytes,
-                            d->tile, d->dtype, st);
+    return vh::adjoint<double>(g, d, s, f, n_substeps, ckpt_every, lam, out, workspace,
+                               workspace_bytes, st);
+  return vh::adjoint<float>(g, d, s, f, n_substeps, ckpt_every, lam, out, workspace,
+                            workspace_bytes, st);
 }
 
 extern "C" int vh_pre_evp_adjoint(const vh_domain* d, const vh_settings* s, const vh_fields* f,
