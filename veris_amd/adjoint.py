@@ -5,7 +5,7 @@ The reference obtains sensitivities by ``jax.grad`` through its JAX kernels
 veris/area_mass.py:17-28 and veris/dynsolver.py:7-84).  Here the derivative
 is the hand-written adjoint of ``libveris_hip.so``: the forward subcycle keeps
 the state every ``ckpt_every`` substeps, and the backward pass recomputes each
-segment from its checkpoint and runs the adjoint kernels backwards through it.
+segment from its checkpoint and runs the adjoint kernel backwards through it.
 
 :func:`evp_subcycle` is differentiable (through ``torch.autograd``) with
 respect to the ice-strength parameter P* (a scalar or a per-cell field), the
@@ -24,89 +24,98 @@ import torch
 
 from . import _native as N
 from . import kernels as K
-from .settings import SettingsNamespace
-from .state import VerisState
+
+_DRAGS = ("waterIceDrag", "waterIceDrag_south", "airIceDrag", "airIceDrag_south")
+_PRE_OUT = ("SeaIceMassC", "SeaIceMassU", "SeaIceMassV", "AreaW", "AreaS", "WindForcingX",
+            "WindForcingY", "SeaIceStrength")
 
 
-
-def _with(state, settings: dict | None = None, variables: dict | None = None) -> VerisState:
-    sv = state.settings.as_dict()
-    sv.update(settings or {})
-    vv = dict(state.variables.items())
-    vv.update(variables or {})
-    return VerisState(SettingsNamespace(sv), vv)
-
-
-def _pre_evp(state, pstar_field):
-    mC, mU, mV = K.SeaIceMass(state)
-    st = _with(state, variables=dict(SeaIceMassC=mC, SeaIceMassU=mU, SeaIceMassV=mV))
-    AW, AS = K.AreaWS(st)
-    st = _with(st, variables=dict(AreaW=AW, AreaS=AS))
-    WX, WY = K.WindForcingXY(st)
-    P = K.SeaIceStrength(st, pStar=pstar_field)
-    return _with(st, variables=dict(WindForcingX=WX, WindForcingY=WY, SeaIceStrength=P))
+def _scalars(values):
+    """Host floats of 0-d tensors / numbers, with at most one device sync."""
+    dev = [v for v in values if isinstance(v, torch.Tensor) and v.device.type != "cpu"]
+    host = iter(torch.stack([v.detach().double().reshape(()) for v in dev]).tolist()) if dev else None
+    out = []
+    for v in values:
+        if isinstance(v, torch.Tensor):
+            out.append(next(host) if v.device.type != "cpu" else float(v))
+        else:
+            out.append(float(v))
+    return out
 
 
 class _EvpSubcycle(torch.autograd.Function):
     @staticmethod
     def forward(ctx, state, n, ckpt_every, tile, pstar, h, cwN, cwS, caN, caS):
-        vs = state.variables
-        ref = vs.iceMask
-        dt = ref.dtype
-        pstar_field = pstar.detach().to(dt).expand(ref.shape).contiguous()
-        scal = dict(waterIceDrag=float(cwN), waterIceDrag_south=float(cwS),
-                    airIceDrag=float(caN), airIceDrag_south=float(caS))
-        st = _with(state, settings=scal, variables=dict(hIceMean=h.detach().to(dt).contiguous()))
-        st = _pre_evp(st, pstar_field)
-        c = K.Ctx(st)
+        c = K.Ctx(state)
         c.dom.tile = int(tile)
+        dt, shape = c.dtype, c.shape
+        for name, val in zip(_DRAGS, _scalars([cwN, cwS, caN, caS])):
+            setattr(c.sett, name, val)
+        h = h.detach().to(dt).contiguous()
+        c._check("hIceMean", h)
+        c.fields.hIceMean = h.data_ptr()
+        pstar_field = pstar.detach().to(device=c.device, dtype=dt).expand(shape).contiguous()
+        pre = torch.empty((len(_PRE_OUT),) + shape, dtype=dt, device=c.device)
+        mC, mU, mV, AW, AS, WX, WY, P = pre.unbind(0)
+        N.call("vh_seaice_mass", *c.args(), mC.data_ptr(), mU.data_ptr(), mV.data_ptr(), c.stream)
+        N.call("vh_area_ws", *c.args(), AW.data_ptr(), AS.data_ptr(), c.stream)
+        for name, t in (("SeaIceMassC", mC), ("SeaIceMassU", mU), ("SeaIceMassV", mV),
+                        ("AreaW", AW), ("AreaS", AS)):
+            setattr(c.fields, name, t.data_ptr())
+        N.call("vh_wind_forcing_xy", *c.args(), WX.data_ptr(), WY.data_ptr(), c.stream)
+        N.call("vh_seaice_strength", *c.args(), pstar_field.data_ptr(), P.data_ptr(), c.stream)
+        for name, t in (("WindForcingX", WX), ("WindForcingY", WY), ("SeaIceStrength", P)):
+            setattr(c.fields, name, t.data_ptr())
         nbytes = ctypes.c_size_t(0)
         N.call("vh_evp_adjoint_workspace_bytes", ctypes.byref(c.dom), n, ckpt_every,
                ctypes.byref(nbytes))
         ws = torch.empty(max(int(nbytes.value), 256), dtype=torch.uint8, device=c.device)
-        out = c.empty(5)
-        N.call("vh_evp_forward_checkpointed", *c.args(), n, ckpt_every, *map(K._p, out),
-               K._p(ws), ws.numel(), c.stream)
-        ctx.st, ctx.n, ctx.ckpt, ctx.tile = st, n, ckpt_every, tile
-        ctx.pstar_field, ctx.ws = pstar_field, ws
-        ctx.in_dtypes = [x.dtype for x in (pstar, h, cwN, cwS, caN, caS)]
-        ctx.pstar_shape = pstar.shape
+        out = torch.empty((5,) + shape, dtype=dt, device=c.device).unbind(0)
+        N.call("vh_evp_forward_checkpointed", *c.args(), n, ckpt_every,
+               *(t.data_ptr() for t in out), ws.data_ptr(), ws.numel(), c.stream)
+        # the ABI structs point into these tensors: keep them alive for backward
+        ctx.c, ctx.keep = c, (state, h, pre, pstar_field)
+        ctx.n, ctx.ckpt, ctx.ws = n, ckpt_every, ws
+        ctx.meta = [(x.dtype, x.device, x.shape) for x in (pstar, cwN, cwS, caN, caS)]
+        ctx.h_meta = (h.dtype, h.device)
         return tuple(out)
 
     @staticmethod
     def backward(ctx, gu, gv, gs1, gs2, gs12):
-        st = ctx.st
-        c = K.Ctx(st)
-        c.dom.tile = int(ctx.tile)
-        grads = [None if g is None else g.to(c.dtype).contiguous() for g in (gu, gv, gs1, gs2, gs12)]
-        evp_out = {name: c.empty() for name in N.EVP_ADJOINT_NAMES}
-        eo = N.EvpAdjointOut(**{"adj_" + k: K._p(v) for k, v in evp_out.items()})
-        N.call("vh_evp_adjoint", *c.args(), ctx.n, ctx.ckpt, *map(K._p, grads), ctypes.byref(eo),
-               K._p(ctx.ws), ctx.ws.numel(), c.stream)
-        pre_out = {name: c.empty() for name in N.PRE_EVP_ADJOINT_NAMES}
-        po = N.PreEvpAdjointOut(**{"adj_" + k: K._p(v) for k, v in pre_out.items()})
-        N.call("vh_pre_evp_adjoint", *c.args(), K._p(ctx.pstar_field), ctypes.byref(eo),
+        c = ctx.c
+        shape, dt = c.shape, c.dtype
+        grads = [None if g is None else g.to(dt).contiguous() for g in (gu, gv, gs1, gs2, gs12)]
+        ebuf = torch.empty((len(N.EVP_ADJOINT_NAMES),) + shape, dtype=dt, device=c.device)
+        evp = dict(zip(N.EVP_ADJOINT_NAMES, ebuf.unbind(0)))
+        eo = N.EvpAdjointOut(**{"adj_" + k: v.data_ptr() for k, v in evp.items()})
+        N.call("vh_evp_adjoint", *c.args(), ctx.n, ctx.ckpt,
+               *(None if g is None else g.data_ptr() for g in grads), ctypes.byref(eo),
+               ctx.ws.data_ptr(), ctx.ws.numel(), c.stream)
+        pbuf = torch.empty((len(N.PRE_EVP_ADJOINT_NAMES),) + shape, dtype=dt, device=c.device)
+        pre = dict(zip(N.PRE_EVP_ADJOINT_NAMES, pbuf.unbind(0)))
+        po = N.PreEvpAdjointOut(**{"adj_" + k: v.data_ptr() for k, v in pre.items()})
+        pstar_field = ctx.keep[3]
+        N.call("vh_pre_evp_adjoint", *c.args(), pstar_field.data_ptr(), ctypes.byref(eo),
                ctypes.byref(po), c.stream)
-        dps, dh, dcwN, dcwS, dcaN, dcaS = ctx.in_dtypes
-        g_pstar = pre_out["pStar"].double()
-        if len(ctx.pstar_shape) == 0 or g_pstar.shape != ctx.pstar_shape:
-            g_pstar = g_pstar.sum().reshape(ctx.pstar_shape)
-        out = (
-            g_pstar.to(dps),
-            pre_out["hIceMean"].to(dh),
-            evp_out["waterDrag_north"].double().sum().to(dcwN),
-            evp_out["waterDrag_south"].double().sum().to(dcwS),
-            pre_out["airDrag_north"].double().sum().to(dcaN),
-            pre_out["airDrag_south"].double().sum().to(dcaS),
-        )
+        need = ctx.needs_input_grad  # (state, n, ckpt, tile, pstar, h, cwN, cwS, caN, caS)
+        # sums of the per-cell parameter adjoints over the padded field, in FP64
+        sums = torch.stack([pre["pStar"], evp["waterDrag_north"], evp["waterDrag_south"],
+                            pre["airDrag_north"], pre["airDrag_south"]]).double().sum(dim=(1, 2))
+        res = []
+        for q, (x_dt, x_dev, x_shape) in enumerate(ctx.meta):
+            if not need[4 if q == 0 else q + 5]:
+                res.append(None)
+            elif q == 0 and len(x_shape) == 2:
+                res.append(pre["pStar"].to(device=x_dev, dtype=x_dt))
+            else:
+                res.append(sums[q].reshape(x_shape).to(device=x_dev, dtype=x_dt))
+        g_h = pre["hIceMean"].to(device=ctx.h_meta[1], dtype=ctx.h_meta[0]) if need[5] else None
         ctx.ws = None
-        return (None, None, None, None) + out
+        return (None, None, None, None, res[0], g_h, res[1], res[2], res[3], res[4])
 
 
-def _as_tensor(x, device):
-    if isinstance(x, torch.Tensor):
-        return x
-    return torch.tensor(float(x), dtype=torch.float64, device=device)
+def _as_tensor(x):
+    return x if isinstance(x, torch.Tensor) else torch.tensor(float(x), dtype=torch.float64)
 
 
 def evp_subcycle(state, *, pStar=None, hIceMean=None, waterIceDrag=None,
@@ -120,23 +129,24 @@ def evp_subcycle(state, *, pStar=None, hIceMean=None, waterIceDrag=None,
     then ``n_substeps`` EVP substeps (veris/evp_solver.py:257-319), with the
     same results as the drop-in kernels.  Parameters left as None take the
     state's value and are constants; tensors with ``requires_grad`` receive
-    gradients in ``backward``: ``pStar`` (0-d or per-cell, float64 or the
-    state dtype), ``hIceMean`` (field), the four drag coefficients (0-d).
-    Supported settings are those of the default configuration: non-adaptive
-    noSlip EVP, no turning angles, no basal drag, no metric terms, no
-    sea-ice load in the surface tilt.
+    gradients in ``backward``: ``pStar`` (0-d or per-cell), ``hIceMean``
+    (field), the four drag coefficients (0-d).  Gradients come back on the
+    parameter's device and dtype.  Supported settings are those of the
+    default configuration: non-adaptive noSlip EVP, no turning angles, no
+    basal drag, no metric terms, no sea-ice load in the surface tilt.
     """
     vs, s = state.variables, state.settings
-    dev = vs.iceMask.device
     n = int(s.nEVPsteps if n_substeps is None else n_substeps)
     if n < 1 or ckpt_every < 1:
         raise ValueError("n_substeps and ckpt_every must be >= 1")
     args = (
-        _as_tensor(s.pStar if pStar is None else pStar, dev),
+        _as_tensor(s.pStar if pStar is None else pStar),
         vs.hIceMean if hIceMean is None else hIceMean,
-        _as_tensor(s.waterIceDrag if waterIceDrag is None else waterIceDrag, dev),
-        _as_tensor(s.waterIceDrag_south if waterIceDrag_south is None else waterIceDrag_south, dev),
-        _as_tensor(s.airIceDrag if airIceDrag is None else airIceDrag, dev),
-        _as_tensor(s.airIceDrag_south if airIceDrag_south is None else airIceDrag_south, dev),
+        _as_tensor(s.waterIceDrag if waterIceDrag is None else waterIceDrag),
+        _as_tensor(s.waterIceDrag_south if waterIceDrag_south is None else waterIceDrag_south),
+        _as_tensor(s.airIceDrag if airIceDrag is None else airIceDrag),
+        _as_tensor(s.airIceDrag_south if airIceDrag_south is None else airIceDrag_south),
     )
+    if args[0].dim() not in (0, 2):
+        raise ValueError("pStar must be 0-d or a field")
     return _EvpSubcycle.apply(state, n, int(ckpt_every), int(tile), *args)

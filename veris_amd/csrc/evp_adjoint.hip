@@ -6,20 +6,13 @@
 // Forward (vh_evp_forward_checkpointed): n single-substep launches of the
 // fused substep kernel; the state after every `ckpt_every` substeps stays in
 // the workspace.  Reverse (vh_evp_adjoint): for each segment, last first, the
-// substeps are recomputed from the segment's checkpoint by the same kernel in
-// its STORE form, which also keeps each substep's intermediates (e12, zeta,
-// Delta, e11+e22, e11-e22, water drag); then each substep is reversed by
-// three per-cell kernels in gather form -- each cell sums what its neighbours
-// contribute, so no atomics are needed:
-//   S  momentum update and fill (evaluated at the three u-points and three
-//      v-points a c-point touches), c-point stress divergence, stress
-//      relaxation, drag and Coriolis, z-point sigma12 relaxation
-//   D  c-point zeta and Delta
-//   V  z-point shear strain (at the three z-points a u/v-point touches) and
-//      the adjoint of the substep's input velocities
-// Every launch reads its pointers from argument blocks in the workspace that
-// one small kernel writes per call, so the n forward launches and the whole
-// reverse sweep are each replayed from one hipGraph.
+// segment's states are recomputed from its checkpoint by the same kernel, then
+// each substep is reversed by one tiled kernel (adj_substep_kernel) that
+// recomputes the substep's intermediates in LDS and runs the adjoint stages
+// in gather form -- each cell sums what its neighbours contribute, so no
+// atomics are needed.  Every launch reads its pointers from argument blocks
+// in the workspace that one small kernel writes per call, so the n forward
+// launches and the whole reverse sweep are each replayed from one hipGraph.
 //
 // Non-smooth points: d sqrt(x)/dx is taken as 0 at x = 0 (Delta = 0 on
 // motionless or masked cells, zero relative velocity); the denominator guard
@@ -28,6 +21,9 @@
 // metric terms -- the defaults.
 #include <algorithm>
 #include <cstdio>
+#include <cstdlib>
+#include <map>
+#include <mutex>
 
 #include "evp_kernel.h"
 
@@ -41,12 +37,6 @@ struct AdjAcc {  // parameter adjoints, accumulated over the substeps
   T *P, *mC, *mU, *mV, *AW, *AS, *WFX, *WFY, *u0, *v0, *DN, *DS;
 };
 constexpr int kAdjAccFields = 12;
-
-template <typename T>
-struct AdjWork {  // adjoint intermediates of one substep (workspace)
-  T *UDIR, *VDIR, *ZBP, *EPB, *EMB, *DBP, *AB, *BB, *FUB, *FVB, *ZZB, *E12P, *E11B, *E22B, *QB;
-};
-constexpr int kAdjWorkFields = 15;
 
 // What every reverse launch reads besides workspace-relative pointers.
 template <typename T>
@@ -72,8 +62,8 @@ struct BlockPlan {
   const T* in0[5];
   T* outs[5];
   char* ws;
-  size_t fb, off_ckpt, off_pp, off_seg, off_interm;
-  int n, ce;
+  size_t fb, off_ckpt, off_pp, off_seg, seg_stride;
+  int n, ce, nck;
 };
 
 #define CELL_SETUP_G                                                                \
@@ -108,8 +98,11 @@ __device__ inline T* fwd_loc(const BlockPlan<T>& p, int k, int q) {
 }
 
 // mode 0: forward blocks (substep k: state k -> state k+1);
-// mode 1: recompute blocks (substep k inside its segment: segment slots, with
-// the intermediates of segment position k % ce) and the reverse arguments.
+// mode 1: recompute blocks and the reverse arguments.  Substep k of segment
+// sgm reads state k from the checkpoint (first substep) or a segment slot and
+// names state k+1 as its output: a segment slot, or for the segment's last
+// substep the next checkpoint (slot nck holds a copy of the final state),
+// which the reverse sweep reads and never recomputes.
 template <typename T>
 __global__ void adj_write_blocks_kernel(EvpArgBlock<T>* blocks, size_t stride, EvpArgBlock<T> proto,
                                         BlockPlan<T> p, int mode, AdjArgs<T>* aa, AdjArgs<T> av) {
@@ -122,17 +115,29 @@ __global__ void adj_write_blocks_kernel(EvpArgBlock<T>* blocks, size_t stride, E
       b.in[q] = fwd_loc(p, k, q);
       b.out[q] = fwd_loc(p, k + 1, q);
     }
-    for (int r = 0; r < 6; ++r) b.interm[r] = nullptr;
   } else {
     const int jj = k % p.ce, sgm = k / p.ce;
+    const int len = min(p.ce, p.n - sgm * p.ce);
+    const size_t seg = p.off_seg + (size_t)(sgm % 2) * p.seg_stride;  // two slot sets
     for (int q = 0; q < 5; ++q) {
       b.in[q] = jj == 0 ? (k == 0 ? p.in0[q] : slot(p, p.off_ckpt, 5 * sgm + q))
-                        : slot(p, p.off_seg, 5 * jj + q);
-      b.out[q] = slot(p, p.off_seg, 5 * (jj + 1) + q);
+                        : slot(p, seg, 5 * jj + q);
+      b.out[q] = jj == len - 1 ? slot(p, p.off_ckpt, 5 * (sgm + 1) + q)
+                               : slot(p, seg, 5 * (jj + 1) + q);
     }
-    for (int r = 0; r < 6; ++r) b.interm[r] = slot(p, p.off_interm, 6 * jj + r);
   }
   *(EvpArgBlock<T>*)((char*)blocks + (size_t)k * stride) = b;
+}
+
+// copy of the final state (the forward's last outputs) into checkpoint slot nck
+template <typename T>
+__global__ __launch_bounds__(BLK) void adj_keep_final_kernel(const EvpArgBlock<T>* __restrict__ last,
+                                                             Geom g, T* dst, size_t fb) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= g.NX * g.NY) return;
+#pragma unroll
+  for (int q = 0; q < 5; ++q)
+    stg((T*)((char*)dst + (size_t)q * fb), (unsigned)c, ldg((const T*)last->out[q], (unsigned)c));
 }
 
 // ---- reverse sweep: set-up and result copy
@@ -144,7 +149,8 @@ __global__ __launch_bounds__(BLK) void adj_init_kernel(const AdjArgs<T>* __restr
   T* const accp[kAdjAccFields] = {acc.P, acc.mC, acc.mU, acc.mV, acc.AW, acc.AS,
                                   acc.WFX, acc.WFY, acc.u0, acc.v0, acc.DN, acc.DS};
 #pragma unroll
-  for (int q = 0; q < kAdjAccFields; ++q) accp[q][c] = (T)0;
+  for (int q = 0; q < kAdjAccFields; ++q)
+    if (accp[q]) accp[q][c] = (T)0;
 #pragma unroll
   for (int q = 0; q < 5; ++q) L.out[q][c] = a->lam_final[q] ? a->lam_final[q][c] : (T)0;
 }
@@ -181,238 +187,303 @@ __device__ inline T gather_filled(const Geom& g, const T* a, int i, int j) {
   return ((a00 + (hx ? a10 : (T)0)) + (hy ? a01 : (T)0)) + ((hx && hy) ? a11 : (T)0);
 }
 
-template <typename T>
-struct MomAdj {
-  T nb;   // adjoint of the numerator (lambda * mask / den)
-  T db;   // adjoint of the denominator
-  T cdx;  // adjoint of the averaged drag coefficient
-  T cdA;  // the averaged drag coefficient
-};
+// ---- the reverse of one substep, one workgroup per tile.
+//
+// Threads cover a region of RI x RJ cells around a TI x TJ tile (ring of 4).
+// Stages, each on a shrinking part of the region (rows/columns counted from
+// the region's low edge, "hi" from its high edge):
+//   A  load the substep's input state and constants          all cells
+//   B  e12 at z-points (forward, veris/dynamics_routines.py:160-175)      >= 1
+//   C  zeta, Delta, e11 +- e22, water drag at c-points (forward)     >= 1, hi >= 1
+//   S1 adjoint of the u/v momentum update + fill (veris/evp_solver.py:152-191)
+//                                                                    >= 2, hi >= 1
+//   S2 adjoint of the stress divergence, sigma relaxation, drag and Coriolis
+//      (veris/dynamics_routines.py:284-302, veris/evp_solver.py:79-145)
+//                                                                    >= 3, hi >= 2
+//   S3 adjoint of zeta and Delta (veris/dynamics_routines.py:220-268) >= 3, hi >= 3
+//   S4 adjoint of e12 and of e11, e22, drag, Coriolis w.r.t. the input
+//      velocities                                                 tile cells
+// The forward intermediates B, C are recomputed from the input state exactly
+// as the forward kernel computes them; nothing but the two states and the
+// state adjoints is read from or written to memory per substep, plus the
+// parameter adjoints of the tile's own cells.
+template <typename T, typename TD, int RI, int RJ>
+__global__ __launch_bounds__(RI* RJ) void adj_substep_kernel(const AdjArgs<T>* __restrict__ a,
+                                                             const EvpArgBlock<T>* __restrict__ blk,
+                                                             LamSet<T> L, int n_tiles, int tiles_j,
+                                                             int per_xcd) {
+  constexpr int R = 4, TI = RI - 2 * R, TJ = RJ - 2 * R, NT = RI * RJ;
+  static_assert((RJ & (RJ - 1)) == 0, "RJ must be a power of two");
+  static_assert(TI >= 1 && TJ >= 1, "tile too small");
+  __shared__ T sU[NT], sV[NT], sM[NT], sMU[NT], sMV[NT], sA[NT], sB[NT], sE12[NT], sQ[NT], sZ[NT];
+  __shared__ T sCD[NT], sNV[NT], sCXU[NT], sCXV[NT], sAB[NT], sBB[NT], sFU[NT], sFV[NT], sEP[NT];
 
-// Adjoint of the u-point (Y = 0) or v-point (Y = 1) momentum update at (i, j)
-// (veris/evp_solver.py:152-187 with the fill of :190-191); zero on cells the
-// fill overwrites.
-template <typename T, int Y>
-__device__ inline MomAdj<T> mom_point(const Geom& g, const EvpConst<T>& k, const EvpScalars& sc,
-                                      const T* CD, const T* xn, const T* lam, int i, int j) {
-  const int c = AT(i, j);
-  const int cm = Y ? AT(i, wm(j, g.NY)) : AT(wm(i, g.NX), j);
-  const T l = gather_filled(g, lam, i, j);
-  const T cd0 = ldg(CD, c), cd1 = ldg(CD, cm);
-  const T m = ldg(Y ? k.massV : k.massU, c);
-  const T ar = ldg(Y ? k.AreaS : k.AreaW, c);
-  const T oc = ldg(Y ? k.vOc : k.uOc, c);
-  const T mk = ldg(Y ? k.MV : k.MU, c);
-  const T x = ldg(xn, c);
-  const T cw = (T)sc.cosWat;
-  const T cdA = (T)0.5 * (cd0 + cd1);
-  const T den = m * (T)sc.betaP1 + ar * (cdA * cw);
-  const bool guard = den == (T)0;
-  const T r = (T)1 / (guard ? (T)1 : den);
-  const T lc = is_computed(g, i, j) ? l : (T)0;
-  MomAdj<T> o;
-  o.nb = lc * mk * r;
-  o.db = guard ? (T)0 : -lc * x * r;
-  o.cdx = (o.nb * oc + o.db) * (ar * cw);
-  o.cdA = cdA;
-  return o;
-}
-
-// ---- S: momentum, stresses (veris/evp_solver.py:79-103), drag (:108-110),
-//         Coriolis (:142-143), stress divergence (veris/dynamics_routines.py:284-302)
-template <typename T>
-__global__ __launch_bounds__(BLK) void adj_stress_kernel(const AdjArgs<T>* __restrict__ a,
-                                                         const EvpArgBlock<T>* __restrict__ blk,
-                                                         LamSet<T> L, AdjWork<T> w) {
-  CELL_SETUP
-  const EvpConst<T> k = a->c;
+  const int bid = blockIdx.x;
+  const int tile = (bid & 7) * per_xcd + (bid >> 3);
+  if (tile >= n_tiles) return;
+  const Geom g = a->g;
+  const int ti = tile / tiles_j;
+  const int i0 = ti * TI, j0 = (tile - ti * tiles_j) * TJ;
+  const EvpConst<T>& k = a->c;
   const EvpScalars& sc = a->sc;
-  const AdjAcc<T> acc = a->acc;
-  const T* u = blk->in[0];
-  const T* v = blk->in[1];
-  const T* un = blk->out[0];
-  const T* vn = blk->out[1];
-  const T* E12 = blk->interm[0];
-  const T* Z = blk->interm[1];
-  const T* DL = blk->interm[2];
-  const T* EP = blk->interm[3];
-  const T* EM = blk->interm[4];
-  const T* CD = blk->interm[5];
-  const int cip = AT(ip, j), cjp = AT(i, jp), cim = AT(im, j), cjm = AT(i, jm), cmm = AT(im, jm);
-  // momentum adjoints at the u-points c, ip, jm and the v-points c, jp, im
-  const MomAdj<T> U = mom_point<T, 0>(g, k, sc, CD, un, L.in[0], i, j);
-  const MomAdj<T> Uip = mom_point<T, 0>(g, k, sc, CD, un, L.in[0], ip, j);
-  const MomAdj<T> Ujm = mom_point<T, 0>(g, k, sc, CD, un, L.in[0], i, jm);
-  const MomAdj<T> V = mom_point<T, 1>(g, k, sc, CD, vn, L.in[1], i, j);
-  const MomAdj<T> Vjp = mom_point<T, 1>(g, k, sc, CD, vn, L.in[1], i, jp);
-  const MomAdj<T> Vim = mom_point<T, 1>(g, k, sc, CD, vn, L.in[1], im, j);
-  // c-point inputs
-  const T uc = ldg(u, c), uip = ldg(u, cip), vc = ldg(v, c), vjp = ldg(v, cjp);
-  const T uoc = ldg(k.uOc, c), uoip = ldg(k.uOc, cip), voc = ldg(k.vOc, c), vojp = ldg(k.vOc, cjp);
-  const T miu = ldg(k.mInU, c), miuip = ldg(k.mInU, cip), miv = ldg(k.mInV, c),
-          mivjp = ldg(k.mInV, cjp);
-  const T Mc = ldg(k.M, c), Mim = ldg(k.M, cim), Mjm = ldg(k.M, cjm), Mmm = ldg(k.M, cmm);
-  const T fc = ldg(k.fC, c);
-  const T rrAu = ldg(k.rrAu, c), rrAuip = ldg(k.rrAu, cip), rrAujm = ldg(k.rrAu, cjm);
-  const T rrAv = ldg(k.rrAv, c), rrAvjp = ldg(k.rrAv, cjp), rrAvim = ldg(k.rrAv, cim);
+  const AdjAcc<T>& acc = a->acc;
+
+  const int t = threadIdx.x;
+  const int ar = t / RJ, br = t & (RJ - 1);
+  const int i = i0 + ar - R, j = j0 + br - R;
+  const int gi = wrap(i, g.NX), gj = wrap(j, g.NY);
+  const int gip = wp(gi, g.NX), gjp = wp(gj, g.NY), gim = wm(gi, g.NX), gjm = wm(gj, g.NY);
+  const unsigned c = gi * g.NY + gj;
+  const unsigned cip = gip * g.NY + gj, cjp = gi * g.NY + gjp;
+  const unsigned cim = gim * g.NY + gj, cjm = gi * g.NY + gjm;
+  const bool own = (ar >= R) && (ar < R + TI) && (br >= R) && (br < R + TJ) && (i < g.NX) &&
+                   (j < g.NY);
+  const bool comp = is_computed(g, gi, gj);
+
+  // ---- A: every load of this cell, issued together
+  const T* xu = blk->in[0];
+  const T* xv = blk->in[1];
+  const T u = ldg(xu, c), v = ldg(xv, c);
+  const T Mc = ldg(k.M, c), MUc = ldg(k.MU, c), MVc = ldg(k.MV, c);
+  const T uoc = ldg(k.uOc, c), voc = ldg(k.vOc, c);
+  const T mInUc = ldg(k.mInU, c), mInVc = ldg(k.mInV, c);
+  const T rdyU = ldg(k.rdyU, c), rdxV = ldg(k.rdxV, c), rAz = ldg(k.rAz, c);
+  const T rdxU = ldg(k.rdxU, c), rdyV = ldg(k.rdyV, c), mInC = ldg(k.mInC, c);
+  const T rrA = ldg(k.rrA, c), P = ldg(k.P, c), fc = ldg(k.fC, c);
+  const T un = ldg(blk->out[0], c), vn = ldg(blk->out[1], c);
+  const T mU = ldg(k.massU, c), mV = ldg(k.massV, c), AW = ldg(k.AreaW, c), AS = ldg(k.AreaS, c);
+  const T rrAu = ldg(k.rrAu, c), rrAv = ldg(k.rrAv, c), mC = ldg(k.massC, c);
   const T dyV = ldg(k.dyV, c), dxU = ldg(k.dxU, c), dxV = ldg(k.dxV, c), dyU = ldg(k.dyU, c);
-  const T mUc = ldg(k.massU, c), mVc = ldg(k.massV, c);
-  const T uNm1 = ldg(k.uNm1, c), vNm1 = ldg(k.vNm1, c);
   const T l1 = ldg(L.in[2], c), l2 = ldg(L.in[3], c), l12 = ldg(L.in[4], c);
-  const T zeta = ldg(Z, c), dC = ldg(DL, c), ep = ldg(EP, c), em = ldg(EM, c), e12 = ldg(E12, c);
-  const T zim = ldg(Z, cim), zjm = ldg(Z, cjm), zmm = ldg(Z, cmm);
-  const T aP = ldg(acc.P, c), amC = ldg(acc.mC, c), aDN = ldg(acc.DN, c), aDS = ldg(acc.DS, c);
+  const T lu = gather_filled(g, L.in[0], gi, gj), lv = gather_filled(g, L.in[1], gi, gj);
+  // water drag factor of either hemisphere, selected by value (a select of
+  // the two setting addresses would become a per-lane load)
+  const T Dn = (T)sc.cwN * (T)sc.rhoSea, Ds = (T)sc.cwS * (T)sc.rhoSea;
+  const T D = fc < (T)0 ? Ds : Dn;
 
-  const T cw = (T)sc.cosWat, bU = (T)sc.betaU, bP1 = (T)sc.betaP1, rdt = (T)sc.rdt;
-  if (is_computed(g, i, j)) {
-    const T amU = ldg(acc.mU, c), au0 = ldg(acc.u0, c), aWX = ldg(acc.WFX, c), aAW = ldg(acc.AW, c);
-    const T amV = ldg(acc.mV, c), av0 = ldg(acc.v0, c), aWY = ldg(acc.WFY, c), aAS = ldg(acc.AS, c);
-    stg(acc.mU, c, amU + (U.nb * (bU * uc + rdt * uNm1) + U.db * bP1));
-    stg(acc.u0, c, au0 + U.nb * mUc * rdt);
-    stg(acc.WFX, c, aWX + U.nb);
-    stg(acc.AW, c, aAW + (U.nb * (U.cdA * cw * uoc) + U.db * (U.cdA * cw)));
-    stg(acc.mV, c, amV + (V.nb * (bU * vc + rdt * vNm1) + V.db * bP1));
-    stg(acc.v0, c, av0 + V.nb * mVc * rdt);
-    stg(acc.WFY, c, aWY + V.nb);
-    stg(acc.AS, c, aAS + (V.nb * (V.cdA * cw * voc) + V.db * (V.cdA * cw)));
+  sU[t] = u;
+  sV[t] = v;
+  sM[t] = Mc;
+  sMU[t] = MUc;
+  sMV[t] = MVc;
+  sA[t] = (u - uoc) * mInUc;
+  sB[t] = (v - voc) * mInVc;
+  __syncthreads();
+
+  // ---- B: e12 at z-points (the forward kernel's expression order)
+  T e12r = 0;
+  if (ar >= 1 && br >= 1) {
+    const T ujm = sU[t - 1], vim = sV[t - RJ];
+    const T dudy = (u - ujm) * rdyU;
+    const T uavz = (u + ujm) * (T)0.5;
+    const T dvdx = (v - vim) * rdxV;
+    const T vavz = (v + vim) * (T)0.5;
+    const T mz = (Mc * sM[t - RJ]) * (sM[t - 1] * sM[t - RJ - 1]);
+    T e12 = ((T)0.5 * (dudy + dvdx)) * mz;
+    const T hu = MUc - sMU[t - 1];
+    const T hv = MVc - sMV[t - RJ];
+    e12 = e12 + ((((T)2.0 * uavz) * rdyU) * hu + (((T)2.0 * vavz) * rdxV) * hv);
+    sE12[t] = e12;
+    sQ[t] = rAz * (e12 * e12);
+    e12r = e12;
   }
-  stg(w.UDIR, c, U.nb * bU * mUc);
-  stg(w.VDIR, c, V.nb * bU * mVc);
-  const T Xc = U.nb * rrAu, Xip = Uip.nb * rrAuip, Xjm = Ujm.nb * rrAujm;
-  const T Yc = V.nb * rrAv, Yjp = Vjp.nb * rrAvjp, Yim = Vim.nb * rrAvim;
+  __syncthreads();
 
-  // drag and Coriolis at c
-  const T cdb = (T)0.5 * (U.cdx + Uip.cdx) + (T)0.5 * (V.cdx + Vjp.cdx);
-  const T fvb = (T)0.5 * (U.nb + Uip.nb);
-  const T fub = -(T)0.5 * (V.nb + Vjp.nb);
-  stg(acc.mC, c, amC + (fvb * fc * (T)0.5 * (vc + vjp) + fub * fc * (T)0.5 * (uc + uip)));
-  stg(w.FVB, c, fvb);
-  stg(w.FUB, c, fub);
-  const T cbe = cdb * Mc;
-  const bool south = fc < (T)0;
-  const T D = (T)(south ? sc.cwS : sc.cwN) * (T)sc.rhoSea;
-  const T ad = (uc - uoc) * miu + (uip - uoip) * miuip;
-  const T bd = (vc - voc) * miv + (vjp - vojp) * mivjp;
-  const T ww = (T)0.25 * (ad * ad + bd * bd);
-  const T sw = sqrt(ww);
-  const bool drag = (D * D) * ww > (T)sc.cDragMin2;
-  const T wb = (drag && sw > (T)0) ? cbe * D * (T)0.5 / sw : (T)0;
-  const T Db = drag ? cbe * sw * (T)sc.rhoSea : (T)0;
-  stg(w.AB, c, wb * (T)0.5 * ad);
-  stg(w.BB, c, wb * (T)0.5 * bd);
-  stg(acc.DS, c, aDS + (south ? Db : (T)0));
-  stg(acc.DN, c, aDN + (south ? (T)0 : Db));
+  // ---- C: zeta, Delta, strain invariants, water drag at c-points
+  T zeta = 0, dC = 0, ep = 0, em = 0, sa = 0, sb = 0, ww = 0, sw = 0;
+  if (ar >= 1 && br >= 1 && ar <= RI - 2 && br <= RJ - 2) {
+    const T uip = sU[t + RJ], vjp = sV[t + 1];
+    const T e11 = ((uip - u) * rdxU) * mInC;
+    const T e22 = ((vjp - v) * rdyV) * mInC;
+    const T qa = sQ[t] + sQ[t + RJ];
+    const T qb = sQ[t + 1] + sQ[t + RJ + 1];
+    const T e12c = ((T)0.25 * rrA) * (qa + qb);
+    ep = e11 + e22;
+    em = e11 - e22;
+    const TD dsq = (TD)ep * (TD)ep + (TD)sc.rpd * ((TD)em * (TD)em + (TD)4.0 * (TD)e12c);
+    const TD dCd = sqrt(dsq);
+    const TD zd = ((TD)0.5 * ((TD)P * (TD)sc.tens1p)) / (dCd + (TD)sc.deltaMin);
+    zeta = (T)zd;
+    dC = (T)dCd;
+    sZ[t] = zeta;
+    sa = sA[t] + sA[t + RJ];
+    sb = sB[t] + sB[t + 1];
+    ww = (T)0.25 * (sa * sa + sb * sb);
+    sw = sqrt(ww);
+    const T cd = ((D * D) * ww > (T)sc.cDragMin2) ? D * sw : (T)sc.cDragMin;
+    sCD[t] = cd * Mc;
+  }
+  __syncthreads();
 
-  // sigma1, sigma2 at c
-  const T s11b = dyV * (Xc - Xip);
-  const T s22b = dxU * (Yc - Yjp);
-  const T s1t = l1 + (T)0.5 * (s11b + s22b);
-  const T s2t = l2 + (T)0.5 * (s11b - s22b);
-  const T H1 = s1t * (T)sc.den1 * Mc, H2 = s2t * (T)sc.den2 * Mc;
-  stg(L.out[2], c, H1 * (T)sc.alphaM1);
-  stg(L.out[3], c, H2 * (T)sc.alphaM1);
-  const T pb = -H1;
-  const T pfac = (T)sc.tens1m * (T)2 * (T)sc.pr / (T)sc.tens1p;
-  stg(acc.P, c, aP + pb * (T)sc.tens1m * (T)sc.pr1m);
-  stg(w.ZBP, c, H1 * (T)2 * ep + H2 * (T)0.5 * em + pb * pfac * dC);
-  stg(w.EPB, c, H1 * (T)2 * zeta);
-  stg(w.EMB, c, H2 * (T)0.5 * zeta);
-  stg(w.DBP, c, pb * pfac * zeta);
+  // parameter adjoints of this cell, read-modify-written at the end: issued
+  // here, after the forward recompute has released its registers
+  const T aP = ldg(acc.P, c), amC = ldg(acc.mC, c), amU = ldg(acc.mU, c), amV = ldg(acc.mV, c);
+  const T aWX = ldg(acc.WFX, c), aWY = ldg(acc.WFY, c), aDN = ldg(acc.DN, c), aDS = ldg(acc.DS, c);
+  const T uNm1 = ldg(k.uNm1, c), vNm1 = ldg(k.vNm1, c);
+  // neighbours' constants used by S4
+  const T rAz_jp = ldg(k.rAz, cjp), rdyU_jp = ldg(k.rdyU, cjp);
+  const T rAz_ip = ldg(k.rAz, cip), rdxV_ip = ldg(k.rdxV, cip);
+  const T fmc_im = ldg(k.fC, cim) * ldg(k.massC, cim), fmc_jm = ldg(k.fC, cjm) * ldg(k.massC, cjm);
 
-  // sigma12 at z = c
-  const T s12t = l12 + dxV * (Xjm - Xc) + dyU * (Yim - Yc);
-  const T G = s12t * (T)sc.den2;
-  stg(L.out[4], c, G * (T)sc.alphaM1);
-  const T nz = (Mc + Mim) + (Mjm + Mmm);
-  const T nrm = nz > (T)0 ? (T)1 / nz : (T)0;
-  const T zz = nrm * ((zeta + zim) + (zjm + zmm));
-  stg(w.ZZB, c, G * (T)0.5 * e12 * nrm);  // carries the z-average weight of z = c
-  stg(w.E12P, c, G * (T)0.5 * zz);
+  // ---- S1: momentum update at the u- and v-point of this cell
+  const T cw = (T)sc.cosWat, bU = (T)sc.betaU, bP1 = (T)sc.betaP1, rdt = (T)sc.rdt;
+  T nbU = 0, dbU = 0, cdAU = 0, nbV = 0, dbV = 0, cdAV = 0;
+  if (ar >= 2 && br >= 2 && ar <= RI - 2 && br <= RJ - 2) {
+    cdAU = (T)0.5 * (sCD[t] + sCD[t - RJ]);
+    cdAV = (T)0.5 * (sCD[t] + sCD[t - 1]);
+    const T denU = mU * bP1 + AW * (cdAU * cw);
+    const T denV = mV * bP1 + AS * (cdAV * cw);
+    const bool gU = denU == (T)0, gV = denV == (T)0;
+    const T rU = (T)1 / (gU ? (T)1 : denU), rV = (T)1 / (gV ? (T)1 : denV);
+    const T lcu = comp ? lu : (T)0, lcv = comp ? lv : (T)0;
+    nbU = lcu * MUc * rU;
+    dbU = gU ? (T)0 : -lcu * un * rU;
+    nbV = lcv * MVc * rV;
+    dbV = gV ? (T)0 : -lcv * vn * rV;
+    sA[t] = nbU * rrAu;  // X: adjoint of the u-point stress divergence
+    sB[t] = nbV * rrAv;  // Y: of the v-point one
+    sQ[t] = nbU;
+    sNV[t] = nbV;
+    sCXU[t] = (nbU * uoc + dbU) * (AW * cw);
+    sCXV[t] = (nbV * voc + dbV) * (AS * cw);
+  }
+  __syncthreads();
+
+  // ---- S2: stresses, drag and Coriolis at c, sigma12 at z
+  T Padd = 0, mCadd = 0, Db = 0, zbp = 0, epb0 = 0, emb0 = 0, dbp = 0;
+  T lo1 = 0, lo2 = 0, lo12 = 0;
+  if (ar >= 3 && br >= 3 && ar <= RI - 3 && br <= RJ - 3) {
+    const T Xc = sA[t], Xip = sA[t + RJ], Xjm = sA[t - 1];
+    const T Yc = sB[t], Yjp = sB[t + 1], Yim = sB[t - RJ];
+    const T cdb = (T)0.5 * (sCXU[t] + sCXU[t + RJ]) + (T)0.5 * (sCXV[t] + sCXV[t + 1]);
+    const T fvb = (T)0.5 * (sQ[t] + sQ[t + RJ]);
+    const T fub = -(T)0.5 * (sNV[t] + sNV[t + 1]);
+    const T uip = sU[t + RJ], vjp = sV[t + 1];
+    mCadd = fvb * fc * (T)0.5 * (v + vjp) + fub * fc * (T)0.5 * (u + uip);
+    sFV[t] = fvb;
+    sFU[t] = fub;
+    const T cbe = cdb * Mc;
+    const bool drag = (D * D) * ww > (T)sc.cDragMin2;
+    const T wb = (drag && sw > (T)0) ? cbe * D * (T)0.5 / sw : (T)0;
+    Db = drag ? cbe * sw * (T)sc.rhoSea : (T)0;
+    sAB[t] = wb * (T)0.5 * sa;
+    sBB[t] = wb * (T)0.5 * sb;
+    const T s11b = dyV * (Xc - Xip);
+    const T s22b = dxU * (Yc - Yjp);
+    const T s1t = l1 + (T)0.5 * (s11b + s22b);
+    const T s2t = l2 + (T)0.5 * (s11b - s22b);
+    const T H1 = s1t * (T)sc.den1 * Mc, H2 = s2t * (T)sc.den2 * Mc;
+    lo1 = H1 * (T)sc.alphaM1;
+    lo2 = H2 * (T)sc.alphaM1;
+    const T pb = -H1;
+    const T pfac = (T)sc.tens1m * (T)2 * (T)sc.pr / (T)sc.tens1p;
+    Padd = pb * (T)sc.tens1m * (T)sc.pr1m;
+    zbp = H1 * (T)2 * ep + H2 * (T)0.5 * em + pb * pfac * dC;
+    epb0 = H1 * (T)2 * zeta;
+    emb0 = H2 * (T)0.5 * zeta;
+    dbp = pb * pfac * zeta;
+    const T s12t = l12 + dxV * (Xjm - Xc) + dyU * (Yim - Yc);
+    const T G = s12t * (T)sc.den2;
+    lo12 = G * (T)sc.alphaM1;
+    const T nz = (Mc + sM[t - RJ]) + (sM[t - 1] + sM[t - RJ - 1]);
+    // 1 / nz for the 0/1 mask sums 1..4 (0 -> 0), without a division
+    const T nrm = nz == (T)4 ? (T)0.25
+                  : nz == (T)3 ? (T)(1.0 / 3.0)
+                  : nz == (T)2 ? (T)0.5
+                  : nz == (T)1 ? (T)1 : (nz > (T)0 ? (T)1 / nz : (T)0);
+    const T zz = nrm * ((sZ[t] + sZ[t - RJ]) + (sZ[t - 1] + sZ[t - RJ - 1]));
+    sCD[t] = G * (T)0.5 * e12r * nrm;  // zeta adjoint share of z = this cell, weighted
+    sEP[t] = G * (T)0.5 * zz;          // e12 adjoint from sigma12
+  }
+  __syncthreads();
+
+  // ---- S3: zeta and Delta at c
+  if (ar >= 3 && br >= 3 && ar <= RI - 4 && br <= RJ - 4) {
+    const T zb = zbp + ((sCD[t] + sCD[t + RJ]) + (sCD[t + 1] + sCD[t + RJ + 1]));
+    const T rreg = (T)1 / (dC + (T)sc.deltaMin);
+    Padd += zb * (T)0.5 * (T)sc.tens1p * rreg;
+    const T Dbar = dbp - zb * zeta * rreg;
+    const T dsqb = dC > (T)0 ? Dbar * (T)0.5 / dC : (T)0;
+    const T epb = epb0 + dsqb * (T)2 * ep;
+    const T emb = emb0 + dsqb * (T)sc.rpd * (T)2 * em;
+    sA[t] = (epb + emb) * rdxU * mInC;  // e11 adjoint (times its stencil factor)
+    sB[t] = (epb - emb) * rdyV * mInC;  // e22 adjoint
+    sQ[t] = dsqb * (T)sc.rpd * rrA;     // adjoint of rAz e12^2 at the four z-corners
+  }
+  __syncthreads();
+
+  // ---- S4: input-velocity adjoints; results of the tile's own cells
+  if (!own) return;
+  auto eadj = [&](int z, T rz) {
+    const T qs = (sQ[z] + sQ[z - RJ]) + (sQ[z - 1] + sQ[z - RJ - 1]);
+    return sEP[z] + qs * (T)2 * rz * sE12[z];
+  };
+  auto mzf = [&](int z) { return (sM[z] * sM[z - RJ]) * (sM[z - 1] * sM[z - RJ - 1]); };
+  const T ec = eadj(t, rAz), ejp = eadj(t + 1, rAz_jp), eip = eadj(t + RJ, rAz_ip);
+  const T mzc = mzf(t), mzjp = mzf(t + 1), mzip = mzf(t + RJ);
+  const T huc = MUc - sMU[t - 1], hujp = sMU[t + 1] - MUc;
+  const T hvc = MVc - sMV[t - RJ], hvip = sMV[t + RJ] - MVc;
+  const T A1 = (T)0.5 * mzc * rdyU + huc * rdyU;
+  const T A2jp = -(T)0.5 * mzjp * rdyU_jp + hujp * rdyU_jp;
+  const T A3 = (T)0.5 * mzc * rdxV + hvc * rdxV;
+  const T A4ip = -(T)0.5 * mzip * rdxV_ip + hvip * rdxV_ip;
+  const T fmc = fc * mC;
+  const T lu_out = nbU * bU * mU + (sAB[t] + sAB[t - RJ]) * mInUc + (sA[t - RJ] - sA[t]) +
+                   ec * A1 + ejp * A2jp + (T)0.5 * fmc * sFU[t] + (T)0.5 * fmc_im * sFU[t - RJ];
+  const T lv_out = nbV * bU * mV + (sBB[t] + sBB[t - 1]) * mInVc + (sB[t - 1] - sB[t]) +
+                   ec * A3 + eip * A4ip + (T)0.5 * fmc * sFV[t] + (T)0.5 * fmc_jm * sFV[t - 1];
+  stg(L.out[0], c, lu_out);
+  stg(L.out[1], c, lv_out);
+  stg(L.out[2], c, lo1);
+  stg(L.out[3], c, lo2);
+  stg(L.out[4], c, lo12);
+  stg(acc.P, c, aP + Padd);
+  stg(acc.mC, c, amC + mCadd);
+  if (fc < (T)0) stg(acc.DS, c, aDS + Db);
+  else stg(acc.DN, c, aDN + Db);
+  if (comp) {  // nbU = nbV = 0 elsewhere
+    stg(acc.mU, c, amU + (nbU * (bU * u + rdt * uNm1) + dbU * bP1));
+    stg(acc.WFX, c, aWX + nbU);
+    stg(acc.mV, c, amV + (nbV * (bU * v + rdt * vNm1) + dbV * bP1));
+    stg(acc.WFY, c, aWY + nbV);
+    if (acc.u0) {  // optional outputs (uniform branch)
+      stg(acc.u0, c, ldg(acc.u0, c) + nbU * mU * rdt);
+      stg(acc.v0, c, ldg(acc.v0, c) + nbV * mV * rdt);
+    }
+    if (acc.AW) {
+      stg(acc.AW, c, ldg(acc.AW, c) + (nbU * (cdAU * cw * uoc) + dbU * (cdAU * cw)));
+      stg(acc.AS, c, ldg(acc.AS, c) + (nbV * (cdAV * cw * voc) + dbV * (cdAV * cw)));
+    }
+  }
 }
 
-// ---- D: c-point zeta and Delta (veris/dynamics_routines.py:220-268)
-template <typename T>
-__global__ __launch_bounds__(BLK) void adj_visc_kernel(const AdjArgs<T>* __restrict__ a,
-                                                       const EvpArgBlock<T>* __restrict__ blk,
-                                                       AdjWork<T> w) {
-  CELL_SETUP
-  const EvpConst<T> k = a->c;
-  const EvpScalars& sc = a->sc;
-  T* const accP = a->acc.P;
-  const T* const* it = blk->interm;
-  const T z0 = ldg(w.ZZB, c), z1 = ldg(w.ZZB, AT(ip, j)), z2 = ldg(w.ZZB, AT(i, jp)),
-          z3 = ldg(w.ZZB, AT(ip, jp));
-  const T zbp = ldg(w.ZBP, c), dbp = ldg(w.DBP, c), epb0 = ldg(w.EPB, c), emb0 = ldg(w.EMB, c);
-  const T zeta = ldg(it[1], c), dC = ldg(it[2], c), ep = ldg(it[3], c), em = ldg(it[4], c);
-  const T mi = ldg(k.mInC, c), rdxU = ldg(k.rdxU, c), rdyV = ldg(k.rdyV, c), rrA = ldg(k.rrA, c);
-  const T aP = ldg(accP, c);
-  const T zb = zbp + ((z0 + z1) + (z2 + z3));
-  const T rreg = (T)1 / (dC + (T)sc.deltaMin);
-  stg(accP, c, aP + zb * (T)0.5 * (T)sc.tens1p * rreg);
-  const T Db = dbp - zb * zeta * rreg;
-  const T dsqb = dC > (T)0 ? Db * (T)0.5 / dC : (T)0;
-  const T epb = epb0 + dsqb * (T)2 * ep;
-  const T emb = emb0 + dsqb * (T)sc.rpd * (T)2 * em;
-  stg(w.E11B, c, (epb + emb) * rdxU * mi);
-  stg(w.E22B, c, (epb - emb) * rdyV * mi);
-  stg(w.QB, c, dsqb * (T)sc.rpd * rrA);
-}
-
-// Adjoint of e12 at z-point (ii, jj) times its coefficients in
-// e12(z) = A1 u + A2 u[j-1] + A3 v + A4 v[i-1] (veris/dynamics_routines.py:160-175):
-// returns the adjoint e and the coefficients.
-template <typename T>
-struct E12Adj {
-  T e, A1, A2, A3, A4;
+struct AdjTileCfg {
+  int ri, rj;
 };
+// 512- and 768-thread shapes: the FP64 kernel needs ~156 VGPRs, more than a
+// 1024-thread workgroup can give each lane.
+constexpr AdjTileCfg kAdjTiles[] = {{16, 32}, {24, 32}, {12, 64}};
+constexpr int kAdjNumTiles = sizeof(kAdjTiles) / sizeof(kAdjTiles[0]);
 
-template <typename T>
-__device__ inline E12Adj<T> e12_adj(const Geom& g, const EvpConst<T>& k, const T* E12,
-                                    const AdjWork<T>& w, int ii, int jj) {
-  const int im = wm(ii, g.NX), jm = wm(jj, g.NY);
-  const int c = AT(ii, jj), cim = AT(im, jj), cjm = AT(ii, jm), cmm = AT(im, jm);
-  const T q0 = ldg(w.QB, c), q1 = ldg(w.QB, cim), q2 = ldg(w.QB, cjm), q3 = ldg(w.QB, cmm);
-  const T e12p = ldg(w.E12P, c), rAz = ldg(k.rAz, c), e12 = ldg(E12, c);
-  const T M0 = ldg(k.M, c), M1 = ldg(k.M, cim), M2 = ldg(k.M, cjm), M3 = ldg(k.M, cmm);
-  const T MU0 = ldg(k.MU, c), MU2 = ldg(k.MU, cjm), MV0 = ldg(k.MV, c), MV1 = ldg(k.MV, cim);
-  const T ry = ldg(k.rdyU, c), rx = ldg(k.rdxV, c);
-  E12Adj<T> o;
-  o.e = e12p + ((q0 + q1) + (q2 + q3)) * (T)2 * rAz * e12;
-  const T mz = (M0 * M1) * (M2 * M3);
-  const T hu = MU0 - MU2, hv = MV0 - MV1;
-  o.A1 = (T)0.5 * mz * ry + hu * ry;
-  o.A2 = -(T)0.5 * mz * ry + hu * ry;
-  o.A3 = (T)0.5 * mz * rx + hv * rx;
-  o.A4 = -(T)0.5 * mz * rx + hv * rx;
-  return o;
+template <typename T, int RI, int RJ>
+void adj_launch_shape(const AdjArgs<T>* a, const EvpArgBlock<T>* b, LamSet<T> L, const Geom& g,
+                      hipStream_t st) {
+  constexpr int TI = RI - 8, TJ = RJ - 8;
+  const int tiles_j = (g.NY + TJ - 1) / TJ;
+  const int n_tiles = ((g.NX + TI - 1) / TI) * tiles_j;
+  const int per_xcd = (n_tiles + 7) / 8;
+  hipLaunchKernelGGL((adj_substep_kernel<T, double, RI, RJ>), dim3(per_xcd * 8), dim3(RI * RJ), 0,
+                     st, a, b, L, n_tiles, tiles_j, per_xcd);
 }
 
-// ---- V: shear strain and the adjoint of the substep's input velocities
 template <typename T>
-__global__ __launch_bounds__(BLK) void adj_vel_kernel(const AdjArgs<T>* __restrict__ a,
-                                                      const EvpArgBlock<T>* __restrict__ blk,
-                                                      LamSet<T> L, AdjWork<T> w) {
-  CELL_SETUP
-  const EvpConst<T> k = a->c;
-  const T* E12 = blk->interm[0];
-  const E12Adj<T> zc = e12_adj(g, k, E12, w, i, j);
-  const E12Adj<T> zjp = e12_adj(g, k, E12, w, i, jp);
-  const E12Adj<T> zip = e12_adj(g, k, E12, w, ip, j);
-  const int cim = AT(im, j), cjm = AT(i, jm);
-  const T udir = ldg(w.UDIR, c), vdir = ldg(w.VDIR, c);
-  const T ab0 = ldg(w.AB, c), ab1 = ldg(w.AB, cim), bb0 = ldg(w.BB, c), bb1 = ldg(w.BB, cjm);
-  const T e11a = ldg(w.E11B, cim), e11b = ldg(w.E11B, c), e22a = ldg(w.E22B, cjm),
-          e22b = ldg(w.E22B, c);
-  const T fub0 = ldg(w.FUB, c), fub1 = ldg(w.FUB, cim), fvb0 = ldg(w.FVB, c), fvb1 = ldg(w.FVB, cjm);
-  const T fc = ldg(k.fC, c), fim = ldg(k.fC, cim), fjm = ldg(k.fC, cjm);
-  const T mC = ldg(k.massC, c), mCim = ldg(k.massC, cim), mCjm = ldg(k.massC, cjm);
-  const T miu = ldg(k.mInU, c), miv = ldg(k.mInV, c);
-  stg(L.out[0], c,
-      udir + (ab0 + ab1) * miu + (e11a - e11b) + zc.e * zc.A1 + zjp.e * zjp.A2 +
-          (T)0.5 * fc * mC * fub0 + (T)0.5 * fim * mCim * fub1);
-  stg(L.out[1], c,
-      vdir + (bb0 + bb1) * miv + (e22a - e22b) + zc.e * zc.A3 + zip.e * zip.A4 +
-          (T)0.5 * fc * mC * fvb0 + (T)0.5 * fjm * mCjm * fvb1);
+void adj_launch(int cfg, const AdjArgs<T>* a, const EvpArgBlock<T>* b, LamSet<T> L, const Geom& g,
+                hipStream_t st) {
+  switch (cfg) {
+    case 1: adj_launch_shape<T, 24, 32>(a, b, L, g, st); break;
+    case 2: adj_launch_shape<T, 12, 64>(a, b, L, g, st); break;
+    default: adj_launch_shape<T, 16, 32>(a, b, L, g, st); break;
+  }
 }
 
 // ---- pre-EVP adjoint: strength, mass, wind forcing
@@ -483,7 +554,7 @@ __global__ __launch_bounds__(BLK) void pre_evp_adjoint_kernel(Geom g, PreAdjArgs
 struct AdjLayout {
   size_t fb, blk, total;
   int nck, ce, n;
-  size_t off_args, off_fblk, off_rblk, off_ckpt, off_pp, off_seg, off_interm, off_work, off_lam;
+  size_t off_args, off_fblk, off_rblk, off_ckpt, off_pp, off_seg, off_lam;
 };
 
 template <typename T>
@@ -498,11 +569,12 @@ AdjLayout adj_layout(const Geom& g, int n, int ce) {
   L.off_fblk = align256(sizeof(AdjArgs<T>));
   L.off_rblk = L.off_fblk + (size_t)n * L.blk;
   L.off_ckpt = L.off_rblk + (size_t)n * L.blk;
-  L.off_pp = L.off_ckpt + (size_t)5 * L.nck * L.fb;  // checkpoint slot 0 is unused
+  // checkpoint slots 1 .. nck-1, slot nck = copy of the final state (slot 0 unused)
+  L.off_pp = L.off_ckpt + (size_t)5 * (L.nck + 1) * L.fb;
   L.off_seg = L.off_pp + (size_t)10 * L.fb;
-  L.off_interm = L.off_seg + (size_t)5 * (ce + 1) * L.fb;  // segment slot 0 is unused
-  L.off_work = L.off_interm + (size_t)6 * ce * L.fb;
-  L.off_lam = L.off_work + (size_t)kAdjWorkFields * L.fb;
+  // two sets of segment slots (slot 0 of each unused): the recompute of one
+  // segment overlaps the reverse sweep of the next
+  L.off_lam = L.off_seg + (size_t)2 * 5 * (ce + 1) * L.fb;
   L.total = L.off_lam + (size_t)10 * L.fb;
   return L;
 }
@@ -510,6 +582,43 @@ AdjLayout adj_layout(const Geom& g, int n, int ce) {
 template <typename T>
 T* wsf(void* ws, size_t off, size_t fb, int q) {
   return (T*)((char*)ws + off + (size_t)q * fb);
+}
+
+// Tile shape of the reverse kernel (index into kAdjTiles, default 24 x 32,
+// the fastest measured at 1 degree; environment VERIS_ADJ_TILE overrides it).
+int adj_tile_cfg() {
+  static int cfg = -1;
+  if (cfg < 0) {
+    const char* e = getenv("VERIS_ADJ_TILE");
+    const int v = e ? atoi(e) : 1;
+    cfg = (v >= 0 && v < kAdjNumTiles) ? v : 1;
+  }
+  return cfg;
+}
+
+// A second stream and two events per device, for the fork/join of the
+// reverse sweep (created once, default priority, kept for the process).
+struct SideStream {
+  hipStream_t stream;
+  hipEvent_t fork, join;
+};
+
+int side_stream(SideStream* out) {
+  static std::mutex mu;
+  static std::map<int, SideStream> per_dev;
+  int dev = 0;
+  VH_TRY_HIP(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lk(mu);
+  auto it = per_dev.find(dev);
+  if (it == per_dev.end()) {
+    SideStream ss{};
+    VH_TRY_HIP(hipStreamCreateWithFlags(&ss.stream, hipStreamNonBlocking));
+    VH_TRY_HIP(hipEventCreateWithFlags(&ss.fork, hipEventDisableTiming));
+    VH_TRY_HIP(hipEventCreateWithFlags(&ss.join, hipEventDisableTiming));
+    it = per_dev.emplace(dev, ss).first;
+  }
+  *out = it->second;
+  return VH_OK;
 }
 
 int pick_adj_tile(const Geom& g, int req) {
@@ -559,9 +668,10 @@ BlockPlan<T> block_plan(const AdjLayout& L, const vh_fields* f, void* const outs
   p.ws = (char*)ws;
   p.fb = L.fb;
   p.off_ckpt = L.off_ckpt;
+  p.nck = L.nck;
   p.off_pp = L.off_pp;
   p.off_seg = L.off_seg;
-  p.off_interm = L.off_interm;
+  p.seg_stride = (size_t)5 * (L.ce + 1) * L.fb;
   p.n = L.n;
   p.ce = L.ce;
   return p;
@@ -610,10 +720,15 @@ int forward_ckpt(const Geom& g, const vh_domain* d, const vh_settings* s, const 
                      proto_block<T>(g, s, f), block_plan<T>(L, f, outs, ws), 0,
                      (AdjArgs<T>*)nullptr, unused);
   VH_TRY_HIP(hipGetLastError());
+  T* final_slot = wsf<T>(ws, L.off_ckpt, L.fb, 5 * L.nck);
   auto record = [&](hipStream_t cs) {
     for (int kk = 0; kk < n; ++kk)
-      evp_launch_single<T, double, false>(
+      evp_launch_single<T, double>(
           cfg, (const EvpArgBlock<T>*)((char*)fblk + (size_t)kk * L.blk), n_tiles, tiles_j, cs);
+    hipLaunchKernelGGL(adj_keep_final_kernel<T>, dim3((unsigned)((field_elems(g) + BLK - 1) / BLK)),
+                       dim3(BLK), 0, cs,
+                       (const EvpArgBlock<T>*)((char*)fblk + (size_t)(n - 1) * L.blk), g,
+                       final_slot, L.fb);
   };
   return run_recorded<T>("adjf", g, L, cfg, ws, record, st);
 }
@@ -637,7 +752,12 @@ int adjoint(const Geom& g, const vh_domain* d, const vh_settings* s, const vh_fi
                                      av.acc.AW, av.acc.AS, av.acc.WFX, av.acc.WFY,
                                      av.acc.u0, av.acc.v0, av.acc.DN, av.acc.DS};
   const void* outs[5] = {o->adj_uIce, o->adj_vIce, o->adj_sigma1, o->adj_sigma2, o->adj_sigma12};
-  for (const void* p : accp) VH_REQUIRE(p != nullptr, VH_EINVAL, "adjoint: null output");
+  for (int q = 0; q < kAdjAccFields; ++q)
+    VH_REQUIRE(accp[q] != nullptr || q == 4 || q == 5 || q == 8 || q == 9, VH_EINVAL,
+               "adjoint: null output");
+  VH_REQUIRE((av.acc.AW == nullptr) == (av.acc.AS == nullptr) &&
+                 (av.acc.u0 == nullptr) == (av.acc.v0 == nullptr),
+             VH_EINVAL, "adjoint: adj_AreaW/adj_AreaS and adj_uNm1/adj_vNm1 go in pairs");
   for (const void* p : outs) VH_REQUIRE(p != nullptr, VH_EINVAL, "adjoint: null output");
   for (int q = 0; q < 5; ++q) {
     av.lam_final[q] = (const T*)lam_final[q];
@@ -652,10 +772,7 @@ int adjoint(const Geom& g, const vh_domain* d, const vh_settings* s, const vh_fi
   const int cfg = pick_adj_tile(g, d->tile);
   int n_tiles, tiles_j;
   tiles_of(cfg, g, &n_tiles, &tiles_j);
-  AdjWork<T> w;
-  T** wp_[kAdjWorkFields] = {&w.UDIR, &w.VDIR, &w.ZBP, &w.EPB, &w.EMB, &w.DBP, &w.AB, &w.BB,
-                             &w.FUB, &w.FVB, &w.ZZB, &w.E12P, &w.E11B, &w.E22B, &w.QB};
-  for (int q = 0; q < kAdjWorkFields; ++q) *wp_[q] = wsf<T>(ws, L.off_work, L.fb, q);
+  const int acfg = adj_tile_cfg();
   T* lam[2][5];
   for (int q = 0; q < 5; ++q) {
     lam[0][q] = wsf<T>(ws, L.off_lam, L.fb, q);
@@ -670,6 +787,19 @@ int adjoint(const Geom& g, const vh_domain* d, const vh_settings* s, const vh_fi
     }
     return ls;
   };
+  SideStream side;
+  int rc = side_stream(&side);
+  if (rc) return rc;
+  auto recompute = [&](int sgm, hipStream_t cs) {
+    const int k0 = sgm * ce, len = std::min(ce, n - k0);
+    for (int jj = 0; jj + 1 < len; ++jj)
+      evp_launch_single<T, double>(
+          cfg, (const EvpArgBlock<T>*)((char*)rblk + (size_t)(k0 + jj) * L.blk), n_tiles, tiles_j,
+          cs);
+  };
+  // Segment sgm - 1 is recomputed on a second stream while segment sgm is
+  // reversed (they use different slot sets); the fork follows the reverse of
+  // segment sgm + 1, the last reader of the set that the recompute overwrites.
   auto record = [&](hipStream_t cs) {
     int cur = 0;
     LamSet<T> init;
@@ -678,24 +808,25 @@ int adjoint(const Geom& g, const vh_domain* d, const vh_settings* s, const vh_fi
       init.out[q] = lam[cur][q];
     }
     hipLaunchKernelGGL(adj_init_kernel<T>, grid, dim3(BLK), 0, cs, a_dev, init);
+    recompute(L.nck - 1, cs);
     for (int sgm = L.nck - 1; sgm >= 0; --sgm) {
+      if (sgm >= 1) {
+        hipEventRecord(side.fork, cs);
+        hipStreamWaitEvent(side.stream, side.fork, 0);
+        recompute(sgm - 1, side.stream);
+        hipEventRecord(side.join, side.stream);
+      }
       const int k0 = sgm * ce, len = std::min(ce, n - k0);
-      for (int jj = 0; jj < len; ++jj)
-        evp_launch_single<T, double, true>(
-            cfg, (const EvpArgBlock<T>*)((char*)rblk + (size_t)(k0 + jj) * L.blk), n_tiles,
-            tiles_j, cs);
       for (int jj = len - 1; jj >= 0; --jj) {
         const EvpArgBlock<T>* b = (const EvpArgBlock<T>*)((char*)rblk + (size_t)(k0 + jj) * L.blk);
-        const LamSet<T> ls = sets(cur);
-        hipLaunchKernelGGL(adj_stress_kernel<T>, grid, dim3(BLK), 0, cs, a_dev, b, ls, w);
-        hipLaunchKernelGGL(adj_visc_kernel<T>, grid, dim3(BLK), 0, cs, a_dev, b, w);
-        hipLaunchKernelGGL(adj_vel_kernel<T>, grid, dim3(BLK), 0, cs, a_dev, b, ls, w);
+        adj_launch<T>(acfg, a_dev, b, sets(cur), g, cs);
         cur = 1 - cur;
       }
+      if (sgm >= 1) hipStreamWaitEvent(cs, side.join, 0);
     }
     hipLaunchKernelGGL(adj_finish_kernel<T>, grid, dim3(BLK), 0, cs, a_dev, sets(cur));
   };
-  return run_recorded<T>("adjr", g, L, cfg, ws, record, st);
+  return run_recorded<T>("adjr", g, L, cfg * 16 + acfg, ws, record, st);
 }
 
 template <typename T>

@@ -102,9 +102,6 @@ struct EvpArgBlock {
   const MemberDesc* members;  // batched only
   int n_members;              // batched only
   int pad2;
-  // single-substep launches with STORE: e12, zeta, Delta, e11+e22, e11-e22
-  // and the water drag coefficient of the substep, at every cell
-  T* interm[6];
 };
 
 
@@ -246,7 +243,7 @@ __device__ inline int map_axis(int p, int N, int n, int filled) {
   return wrap(p, N);
 }
 
-template <typename T, typename TD, int RI, int RJ, int K, bool BATCHED, bool STORE = false>
+template <typename T, typename TD, int RI, int RJ, int K, bool BATCHED>
 __global__ __launch_bounds__(RI* RJ) void evp_substep_kernel(const EvpArgBlock<T>* __restrict__ ab,
                                                              int which, int launch, int n_tiles,
                                                              int tiles_j, int per_xcd) {
@@ -254,7 +251,6 @@ __global__ __launch_bounds__(RI* RJ) void evp_substep_kernel(const EvpArgBlock<T
   constexpr int TI = RI - 2 * R, TJ = RJ - 2 * R, NT = RI * RJ;
   static_assert((RJ & (RJ - 1)) == 0, "RJ must be a power of two");
   static_assert(TI >= 1 && TJ >= 1, "tile too small for K");
-  static_assert(!STORE || (K == 1 && !BATCHED), "STORE needs single-domain, single-substep tiles");
   __shared__ T sU[NT], sV[NT], sM[NT], sMU[NT], sMV[NT], sDU[NT], sDV[NT], sE12[NT], sQ[NT], sZ[NT];
   __shared__ T sS11[NT], sS22[NT], sCD[NT], sCB[NT], sFV[NT], sFU[NT];
 
@@ -384,7 +380,6 @@ __global__ __launch_bounds__(RI* RJ) void evp_substep_kernel(const EvpArgBlock<T
       e12 = e12 + ((((T)2.0 * uavz) * rdyU) * hu + (((T)2.0 * vavz) * rdxV) * hv);
       sE12[t] = e12;
       sQ[t] = rAz * (e12 * e12);
-      if (STORE && own) stg(ab->interm[0], c, e12);
     }
     __syncthreads();
 
@@ -412,12 +407,6 @@ __global__ __launch_bounds__(RI* RJ) void evp_substep_kernel(const EvpArgBlock<T
       const TD dreg = dC + (TD)sc.deltaMin;
       const TD zeta = ((TD)0.5 * ((TD)P * (TD)sc.tens1p)) / dreg;
       sZ[t] = (T)zeta;
-      if (STORE && own) {
-        stg(ab->interm[1], c, (T)zeta);
-        stg(ab->interm[2], c, (T)dC);
-        stg(ab->interm[3], c, ep);
-        stg(ab->interm[4], c, em);
-      }
       if (cs) {
         const TD pressD =
             (((TD)P * (TD)sc.pr1m) + ((((TD)2.0 * zeta) * dC) * (TD)sc.pr) / (TD)sc.tens1p) *
@@ -434,7 +423,6 @@ __global__ __launch_bounds__(RI* RJ) void evp_substep_kernel(const EvpArgBlock<T
         const T w = (T)0.25 * (su * su + sv * sv);
         const T cd = ((D * D) * w > (T)sc.cDragMin2) ? D * sqrt(w) : (T)sc.cDragMin;
         sCD[t] = cd * Mc;
-        if (STORE && own) stg(ab->interm[5], c, cd * Mc);
         if (sc.basal) {
           const int cip = map_axis<K>(i + 1, g.NX, g.nx, g.fill_x) * g.NY + gj;
           const int cjp = gi * g.NY + map_axis<K>(j + 1, g.NY, g.ny, g.fill_y);
@@ -584,14 +572,13 @@ inline void evp_launch_cfg(int cfg, const EvpArgBlock<T>* ab, int which, int lau
 }
 
 // Single-domain, single-substep launch of shape `cfg` (a shape with k = 1)
-// reading its arguments from `ab` (nlaunch = 1: in -> out); with STORE it
-// also writes the substep's intermediates to ab->interm.
-template <typename T, typename TD, bool STORE>
+// reading its arguments from `ab` (nlaunch = 1: in -> out).
+template <typename T, typename TD>
 inline void evp_launch_single(int cfg, const EvpArgBlock<T>* ab, int n_tiles, int tiles_j,
                               hipStream_t st) {
   const int per_xcd = (n_tiles + 7) / 8;
 #define VH_SINGLE(RI, RJ)                                                                   \
-  hipLaunchKernelGGL((evp_substep_kernel<T, TD, RI, RJ, 1, false, STORE>), dim3(per_xcd * 8), \
+  hipLaunchKernelGGL((evp_substep_kernel<T, TD, RI, RJ, 1, false>), dim3(per_xcd * 8), \
                      dim3(RI * RJ), 0, st, ab, 0, 0, n_tiles, tiles_j, per_xcd)
   switch (cfg) {
     case 1: VH_SINGLE(8, 64); break;
