@@ -12,6 +12,26 @@ EVP substep: algorithmic bytes per launch (38 words per interior cell, 57,600
 cells) over its average launch time measured with HIP events on the stream the
 substeps are launched on.  ``cpu_baseline`` times the NumPy FP64 oracle (one
 thread) on one 1-degree step.
+
+Beside the headline line the same JSON object carries the other measured
+workloads (skip them with ``--no-extras``):
+
+* ``batched``: 48 regional members of sizes 64..512 x 64..384 in one batched
+  EVP launch sequence, FP32 bulk (Delta, zeta, press in FP64) and FP64, 400
+  substeps, autotuned tile shapes; rates, the ratio to the plain FP64 1-degree
+  drop-in, and the error of every EVP output of the FP32 run against the FP64
+  run (the FP64 GPU path equals the NumPy oracle bit for bit) over interior
+  ocean cells, plus the FP64 budget sums of the FP32 fields;
+* ``adjoint``: pre-EVP + 120-substep EVP subcycle at 1 degree with the
+  gradient of a velocity misfit w.r.t. a P* field, the drag coefficients and
+  hIceMean (checkpoint every 10 substeps), FP64 and FP32: wall time of
+  forward + backward over the plain drop-in forward and over the
+  checkpointed forward, and central finite differences along 20 random
+  directions (FP64);
+* ``hires``: the 0.25-degree 1440 x 720 grid, one full step with 800 EVP
+  substeps in FP32 and in FP64, the error of every output of the FP32 step
+  against the FP64 step, FP64 budget sums, and the batched adjoint of 8
+  members at 240 x 240.
 """
 
 from __future__ import annotations
@@ -53,6 +73,282 @@ def cpu_baseline(n_substeps: int) -> dict:
     }
 
 
+def _sync_time(fn, reps: int = 5) -> float:
+    """Median wall seconds of ``fn()`` between device synchronisations."""
+    import torch
+
+    ts = []
+    for _ in range(reps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        fn()
+        torch.cuda.synchronize()
+        ts.append(time.perf_counter() - t0)
+    ts.sort()
+    return ts[len(ts) // 2]
+
+
+def _rel_l2(pairs):
+    """sqrt(sum |got - ref|^2 / sum |ref|^2) over (got, ref, mask) triples of
+    padded fields, interior cells with mask > 0."""
+    import torch
+
+    num = torch.zeros((), dtype=torch.float64, device=pairs[0][1].device)
+    den = torch.zeros_like(num)
+    for got, ref, mask in pairs:
+        sel = mask[2:-2, 2:-2] > 0
+        r = ref[2:-2, 2:-2].double()[sel]
+        d = got[2:-2, 2:-2].double()[sel] - r
+        num = num + (d * d).sum()
+        den = den + (r * r).sum()
+    return float((num / den).sqrt()) if float(den) > 0 else 0.0
+
+
+def bench_batched(dev, nsub: int, fp64_evp_rate: float, fp64_step_rate: float) -> dict:
+    import torch
+
+    from veris_amd import synth
+    from veris_amd.autotune import tune_batched
+    from veris_amd.batched import EVP_OUT, Ensemble
+
+    t_gen = time.perf_counter()
+    states = [synth.config2_member(m) for m in range(len(synth.config2_sizes()))]
+    e32 = Ensemble(states, dev, torch.float32)
+    e64 = Ensemble(states, dev, torch.float64)
+    e32.pre_evp()
+    e64.pre_evp()
+    t_gen = time.perf_counter() - t_gen
+    cells = e32.interior_cells()
+    # members are closed regional domains: every tile shape is exact
+    tile32, times32 = tune_batched(e32, exact=False)
+    tile64, _ = tune_batched(e64, exact=False)
+    out32 = e32.evp(nsub, tile=tile32)
+    out64 = e64.evp(nsub, tile=tile64)
+    t32 = _sync_time(lambda: e32.evp(nsub, tile=tile32, out=out32))
+    t64 = _sync_time(lambda: e64.evp(nsub, tile=tile64, out=out64))
+    masks = e64.unpack("maskInC")
+    err = {}
+    for q, name in enumerate(EVP_OUT):
+        err[name] = _rel_l2(list(zip(e32.unpack(out32[q]), e64.unpack(out64[q]), masks)))
+    b32 = e32.budgets()
+    b64 = e64.budgets()
+    budget_err = {k: float(((b32[k] - b64[k]).abs() / b64[k].abs().clamp_min(1e-300)).max())
+                  for k in b64}
+    from veris_amd import _native
+
+    shapes = _native.evp_tile_shapes()
+    rate32 = cells * nsub / t32
+    return {
+        "workload": f"{e32.n} regional members, sizes 64..512 x 64..384, {cells} interior cells, "
+                    f"{nsub} EVP substeps, one batched launch sequence (hipGraph)",
+        "fp32_value": rate32,
+        "fp64_value": cells * nsub / t64,
+        "unit": "cell-substeps/s",
+        "fp32_ms_per_subcycle": t32 * 1e3,
+        "fp32_tile": {"index": tile32, "shape": shapes[tile32]},
+        "fp64_tile": {"index": tile64, "shape": shapes[tile64]},
+        "fp32_tile_us_per_substep": {str(t): round(v * 1e6, 2) for t, v in times32.items()},
+        "fp32_over_plain_fp64_evp_1deg": rate32 / fp64_evp_rate,
+        "fp32_over_plain_fp64_full_step_1deg": rate32 / fp64_step_rate,
+        "target_ratio": 30.0,
+        "fp32_roofline": {"bound": "hbm", "achieved": rate32 * B_ALG_WORDS * 4 / 1e9,
+                          "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                          "frac": rate32 * B_ALG_WORDS * 4 / 1e9 / HBM_PEAK_GBS},
+        "fp32_rel_l2_vs_fp64": err,
+        "velocity_error_target": 1e-4,
+        "fp64_budget_rel_diff_of_fp32_fields": budget_err,
+        "setup_s": t_gen,
+    }
+
+
+def _adjoint_run(state, params, target, n, ckpt):
+    from veris_amd import adjoint as A
+
+    outs = A.evp_subcycle(state, n_substeps=n, ckpt_every=ckpt, **params)
+    ut, vt = target
+    return ((outs[0] - ut)[2:-2, 2:-2] ** 2).sum() + ((outs[1] - vt)[2:-2, 2:-2] ** 2).sum()
+
+
+def bench_adjoint(dev, ckpt: int = 10) -> dict:
+    import numpy as np
+    import torch
+
+    from veris_amd import kernels as K
+    from veris_amd import synth
+
+    res = {"workload": "1-degree 360x160, pre-EVP + 120 EVP substeps, checkpoint every "
+                       f"{ckpt}; loss = L2 misfit of final uIce/vIce to a seeded target; "
+                       "gradients w.r.t. a lognormal P* field, the water and air drag "
+                       "coefficients (one scalar each, both hemispheres) and hIceMean"}
+    st0 = synth.config3()
+    n = int(st0.settings.nEVPsteps)
+    pstar_np, ut_np, vt_np = synth.config3_adjoint_inputs(st0)
+    for dtype, tag in ((torch.float64, "fp64"), (torch.float32, "fp32")):
+        st = st0.to(dev, dtype)
+        target = (torch.tensor(ut_np, dtype=dtype, device=dev),
+                  torch.tensor(vt_np, dtype=dtype, device=dev))
+
+        def leafs():
+            cw = torch.tensor(st0.settings.waterIceDrag, dtype=torch.float64, device=dev,
+                              requires_grad=True)
+            ca = torch.tensor(st0.settings.airIceDrag, dtype=torch.float64, device=dev,
+                              requires_grad=True)
+            return {"pStar": torch.tensor(pstar_np, device=dev).requires_grad_(True),
+                    "hIceMean": st.variables.hIceMean.detach().clone().requires_grad_(True),
+                    "waterIceDrag": cw, "waterIceDrag_south": cw,
+                    "airIceDrag": ca, "airIceDrag_south": ca}
+
+        p = leafs()
+        pst = torch.tensor(pstar_np, dtype=dtype, device=dev)
+
+        def plain_forward():
+            vs = st.variables
+            vs.SeaIceMassC, vs.SeaIceMassU, vs.SeaIceMassV = K.SeaIceMass(st)
+            vs.AreaW, vs.AreaS = K.AreaWS(st)
+            vs.WindForcingX, vs.WindForcingY = K.WindForcingXY(st)
+            vs.SeaIceStrength = K.SeaIceStrength(st, pStar=pst)
+            K.evp_solver(st, n)
+
+        def ckpt_forward():
+            with torch.no_grad():
+                _adjoint_run(st, p, target, n, ckpt)
+
+        def fwd_bwd():
+            for v in p.values():
+                v.grad = None
+            _adjoint_run(st, p, target, n, ckpt).backward()
+
+        for fn in (plain_forward, ckpt_forward, fwd_bwd):
+            fn()
+        t_plain = _sync_time(plain_forward, 7)
+        t_fwd = _sync_time(ckpt_forward, 7)
+        t_fb = _sync_time(fwd_bwd, 7)
+        r = {"plain_forward_ms": t_plain * 1e3, "checkpointed_forward_ms": t_fwd * 1e3,
+             "forward_backward_ms": t_fb * 1e3,
+             "adjoint_over_plain_forward": t_fb / t_plain,
+             "adjoint_over_checkpointed_forward": t_fb / t_fwd, "target": 4.0}
+        if tag == "fp64":
+            fwd_bwd()
+            grads = {k: v.grad.detach().clone() for k, v in p.items()
+                     if k not in ("waterIceDrag_south", "airIceDrag_south")}
+            base = {k: v.detach() for k, v in p.items()}
+            rng = np.random.default_rng(11)
+            worst = 0.0
+            eps = 1e-6
+            for _ in range(20):
+                dirs = {k: torch.tensor(rng.standard_normal(tuple(base[k].shape)), device=dev,
+                                        dtype=torch.float64) * base[k].double()
+                        for k in grads}
+
+                def J(sign):
+                    q = {k: (base[k].double() + sign * eps * dirs[k]).to(base[k].dtype)
+                         for k in grads}
+                    q["waterIceDrag_south"] = q["waterIceDrag"]
+                    q["airIceDrag_south"] = q["airIceDrag"]
+                    with torch.no_grad():
+                        return float(_adjoint_run(st, q, target, n, ckpt))
+
+                fd = (J(1) - J(-1)) / (2 * eps)
+                ad = sum(float((grads[k].double() * dirs[k]).sum()) for k in grads)
+                worst = max(worst, abs(fd - ad) / max(abs(ad), 1e-300))
+            r["fd_directions"] = 20
+            r["fd_max_rel_err"] = worst
+            r["fd_target"] = 1e-5
+        res[tag] = r
+    return res
+
+
+def bench_hires(dev) -> dict:
+    import torch
+
+    from veris_amd import adjoint as A
+    from veris_amd import kernels as K
+    from veris_amd import synth
+    from veris_amd.step import seaice_step
+
+    st = synth.config4()
+    nsub = int(st.settings.nEVPsteps)
+    nx, ny = st.settings.nx, st.settings.ny
+    res = {"workload": f"0.25-degree {nx}x{ny} + 2-cell halo, one full step with {nsub} EVP "
+                       "substeps, FP32 bulk (Delta/zeta/press and growth in FP64) against FP64"}
+    q0 = st.variables.Qnet.copy()
+    sw = st.variables.SWdown.copy()
+    out = {}
+    for dtype, tag in ((torch.float64, "fp64"), (torch.float32, "fp32")):
+        g = st.to(dev, dtype)
+        q = torch.from_numpy(q0).to(dev, dtype)
+        s_ = torch.from_numpy(sw).to(dev, dtype)
+        start = g.copy(lambda a: a.clone())
+        seaice_step(g, q, s_, n_substeps=nsub)  # first step (graph capture)
+        out[tag] = g
+        g2 = start.copy(lambda a: a.clone())
+        t = _sync_time(lambda: seaice_step(g2, q, s_, n_substeps=nsub), 3)
+        res[f"{tag}_ms_per_step"] = t * 1e3
+        res[f"{tag}_value"] = nx * ny * nsub / t
+    res["unit"] = "cell-substeps/s (full step)"
+    names = ("uIce", "vIce", "sigma1", "sigma2", "sigma12", "hIceMean", "hSnowMean", "Area",
+             "TSurf", "Qnet", "EmPmR", "forc_salt_surface", "OceanStressU", "OceanStressV")
+    m = out["fp64"].variables.maskInC
+    res["fp32_rel_l2_vs_fp64"] = {
+        k: _rel_l2([(out["fp32"].variables[k], out["fp64"].variables[k], m)]) for k in names
+        if k in out["fp64"].variables.keys()}
+    # the same by latitude band: with a one-day step the polar rows run at
+    # advective CFL numbers above 1 and the errors grow poleward
+    ny_pad = m.shape[1]
+    lat_rows = -90.0 + (torch.arange(ny_pad, device=dev, dtype=torch.float64) - 1.5) * 180.0 / ny
+    bands = {}
+    for lo, hi in ((0, 60), (60, 75), (75, 85), (85, 90)):
+        sel = ((lat_rows.abs() >= lo) & (lat_rows.abs() < hi)).to(m.dtype)[None, :] * m
+        bands[f"{lo}-{hi}"] = {k: _rel_l2([(out["fp32"].variables[k], out["fp64"].variables[k],
+                                            sel)]) for k in ("uIce", "vIce", "hIceMean", "Area")}
+    res["fp32_rel_l2_vs_fp64_by_abs_lat"] = bands
+    u = out["fp64"].variables.uIce
+    res["max_advective_cfl"] = float((u.abs() * float(st.settings.deltatDyn) /
+                                      out["fp64"].variables.dxU)[2:-2, 2:-2].max())
+    budgets = {}
+    for k in ("hIceMean", "hSnowMean", "Area"):
+        b = [float(K.global_sum(out[t], out[t].variables[k], out[t].variables.rA))
+             for t in ("fp32", "fp64")]
+        budgets[k] = abs(b[0] - b[1]) / max(abs(b[1]), 1e-300)
+    res["fp64_budget_rel_diff"] = budgets
+
+    # batched adjoint: 8 members at 240 x 240, FP32, concurrent on 4 streams
+    members = [synth.config4_adjoint_member(mm).to(dev, torch.float32) for mm in range(8)]
+    n_adj = int(members[0].settings.nEVPsteps)
+    streams = [torch.cuda.Stream(dev) for _ in range(4)]
+
+    def member_loss(mm, stt):
+        h = stt.variables.hIceMean.detach().clone().requires_grad_(True)
+        ps = torch.tensor(stt.settings.pStar, dtype=torch.float64, device=dev, requires_grad=True)
+        outs = A.evp_subcycle(stt, pStar=ps, hIceMean=h, n_substeps=n_adj, ckpt_every=10)
+        return ((outs[0][2:-2, 2:-2]) ** 2).sum() + ((outs[1][2:-2, 2:-2]) ** 2).sum()
+
+    def batched_adjoint():
+        cur = torch.cuda.current_stream(dev)
+        for mm, stt in enumerate(members):
+            s2 = streams[mm % len(streams)]
+            s2.wait_stream(cur)
+            with torch.cuda.stream(s2):
+                member_loss(mm, stt).backward()
+        for s2 in streams:
+            cur.wait_stream(s2)
+
+    def batched_forward():
+        for stt in members:
+            K.evp_solver(stt, n_adj)
+
+    batched_adjoint()
+    batched_forward()
+    t_adj = _sync_time(batched_adjoint, 3)
+    t_fwd = _sync_time(batched_forward, 3)
+    res["batched_adjoint"] = {
+        "workload": f"8 regional members 240x240, FP32, {n_adj} EVP substeps, checkpoint every "
+                    "10, gradient of a velocity norm w.r.t. P* and hIceMean; members on 4 streams",
+        "forward_backward_ms": t_adj * 1e3, "plain_forward_ms": t_fwd * 1e3,
+        "adjoint_over_plain_forward": t_adj / t_fwd}
+    return res
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -60,6 +356,8 @@ def main(argv=None) -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--substeps", type=int, default=400)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the batched, adjoint and 0.25-degree measurements")
     args = ap.parse_args(argv)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: every box in this setup has a single MI355X; "
@@ -156,6 +454,11 @@ def main(argv=None) -> int:
         },
         "finite": finite,
     }
+    if not args.no_extras:
+        evp_rate = cells / per_launch_s
+        line["batched"] = bench_batched(dev, nsub, evp_rate, value)
+        line["adjoint"] = bench_adjoint(dev)
+        line["hires"] = bench_hires(dev)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(nsub)
     print(json.dumps(line), flush=True)

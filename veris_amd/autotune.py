@@ -55,3 +55,33 @@ def tune_evp(state, exact: bool = True, n_substeps: int = 100, verbose: bool = F
     shape = tuple(ref.shape)
     K.EVP_TILE[(shape, ref.dtype)] = best
     return best
+
+
+def time_batched(ens, tile: int, n_substeps: int = 40, repeats: int = 3) -> float:
+    """Seconds per substep of ``tile`` on the ensemble ``ens`` (best of ``repeats``)."""
+    stream = torch.cuda.current_stream()
+    out = ens.evp(n_substeps, tile=tile)  # warm-up (and graph capture)
+    best = float("inf")
+    for _ in range(repeats):
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        ens.evp(n_substeps, tile=tile, out=out)
+        b.record(stream)
+        b.synchronize()
+        best = min(best, a.elapsed_time(b) * 1e-3 / n_substeps)
+    return best
+
+
+def tune_batched(ens, exact: bool = True, n_substeps: int = 40, verbose: bool = False):
+    """Measure the candidate shapes on an :class:`~veris_amd.batched.Ensemble`
+    and return (best tile, {tile: seconds per substep}).  Multi-substep shapes
+    are exact for members without a periodic fill, so an ensemble of closed
+    regional members may pass ``exact=False`` without changing any result."""
+    times = {}
+    for t in candidate_tiles(exact):
+        times[t] = time_batched(ens, t, n_substeps)
+        if verbose:
+            print(f"tile {t} {N.evp_tile_shapes()[t]}: {times[t] * 1e6:.2f} us/substep")
+    best = min(times, key=times.get)
+    return best, times

@@ -115,33 +115,36 @@ __device__ inline T solve4temp(const GrowthScalars& k, T hI, T hS, T Tin, T Tfrz
   return ice ? Tl : Tin;
 }
 
-template <typename T>
+// Storage type T, arithmetic in C = double for both storage types: the
+// solve4temp flux derivative (cc1 c3 / (cd^2 T^2)) exceeds the FP32 range at
+// the minTIce clamp, and the kernel is a negligible part of a step.
+template <typename T, typename C = double>
 __global__ __launch_bounds__(BLK) void growth_kernel(GrowthScalars k, GrowthIO<T> io, size_t n) {
   const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= n) return;
-  const T h0 = io.h[c], hs0 = io.hs[c], A0 = io.A[c];
-  const bool noIce0 = (h0 == (T)0) || (A0 == (T)0);
-  const T hP = h0 * (T)(!noIce0);
-  const T hsP = hs0 * (T)(!noIce0);
-  const T AP = A0 * (T)(!noIce0);
-  const bool isIce = hP > (T)0;
-  const T regA = sqrt(AP * AP + (T)k.Area_reg);
-  const T rregA = (T)1 / regA;
-  T hIA = isIce ? hP * rregA : (T)0;
-  const T rhIA = AP / sqrt(hP * hP + (T)k.hIce_reg);
-  const T hSA = isIce ? hsP * rregA : (T)0;
-  hIA = fmax(hIA, (T)0.05);
-  const T sal = io.ocSalt[c];
-  const T Tfrz = ((T)k.tempFrz + (T)k.dtempFrz_dS * sal) + (T)k.celsius2K;
-  const T Ts0 = io.Ts[c];
-  const T fc = io.fC[c];
-  const T LWd = io.LWdown[c], ATm = io.ATemp[c], wsp = io.wSpeed[c], aqh = io.aqh[c],
+  const C h0 = io.h[c], hs0 = io.hs[c], A0 = io.A[c];
+  const bool noIce0 = (h0 == (C)0) || (A0 == (C)0);
+  const C hP = h0 * (C)(!noIce0);
+  const C hsP = hs0 * (C)(!noIce0);
+  const C AP = A0 * (C)(!noIce0);
+  const bool isIce = hP > (C)0;
+  const C regA = sqrt(AP * AP + (C)k.Area_reg);
+  const C rregA = (C)1 / regA;
+  C hIA = isIce ? hP * rregA : (C)0;
+  const C rhIA = AP / sqrt(hP * hP + (C)k.hIce_reg);
+  const C hSA = isIce ? hsP * rregA : (C)0;
+  hIA = fmax(hIA, (C)0.05);
+  const C sal = io.ocSalt[c];
+  const C Tfrz = ((C)k.tempFrz + (C)k.dtempFrz_dS * sal) + (C)k.celsius2K;
+  const C Ts0 = io.Ts[c];
+  const C fc = io.fC[c];
+  const C LWd = io.LWdown[c], ATm = io.ATemp[c], wsp = io.wSpeed[c], aqh = io.aqh[c],
           SWd = io.SWdown[c];
-  T Tsum = 0, Fio_s = 0, Fia_s = 0, pen_s = 0;
+  C Tsum = 0, Fio_s = 0, Fia_s = 0, pen_s = 0;
   for (int l = 0; l < k.nITC; ++l) {
-    const T p = (T)(2 * (l + 1) - 1) * (T)k.recip_nITC;
-    T Fio, Fia, pen;
-    const T Tl = solve4temp(k, hIA * p, hSA * p, Ts0, Tfrz, fc, LWd, ATm, wsp, aqh, SWd, &Fio, &Fia,
+    const C p = (C)(2 * (l + 1) - 1) * (C)k.recip_nITC;
+    C Fio, Fia, pen;
+    const C Tl = solve4temp(k, hIA * p, hSA * p, Ts0, Tfrz, fc, LWd, ATm, wsp, aqh, SWd, &Fio, &Fia,
                             &pen);
     if (l == 0) {
       Tsum = Tl;
@@ -155,85 +158,85 @@ __global__ __launch_bounds__(BLK) void growth_kernel(GrowthScalars k, GrowthIO<T
       pen_s = pen_s + pen;
     }
   }
-  const T TSurf = Tsum * (T)k.recip_nITC;
-  const T Fio = (Fio_s * (T)k.recip_nITC) * AP;
-  T Fia = (Fia_s * (T)k.recip_nITC) * AP;
-  const T pen = (pen_s * (T)k.recip_nITC) * AP;
-  const bool cold = (AP > (T)0) && (Tsum / (T)k.nITC_d < (T)k.celsius2K);
-  const T snowf = io.snowfall[c], prec = io.precip[c];
-  const T snowRate = (cold ? snowf + prec : snowf) * (T)k.rhoFresh2rhoSnow;
-  const T rainToSea = cold ? (T)0 : prec;
-  const T snowAcc = (snowRate * AP) * (T)k.dtT;
-  T gUnder = Fio * (T)k.qi;
-  gUnder = AP == (T)0 ? (T)0 : gUnder;
-  const T potMeltRate = (-Fia) * (T)k.qs;
-  const T potMelt = potMeltRate * (T)k.dtT;
+  const C TSurf = Tsum * (C)k.recip_nITC;
+  const C Fio = (Fio_s * (C)k.recip_nITC) * AP;
+  C Fia = (Fia_s * (C)k.recip_nITC) * AP;
+  const C pen = (pen_s * (C)k.recip_nITC) * AP;
+  const bool cold = (AP > (C)0) && (Tsum / (C)k.nITC_d < (C)k.celsius2K);
+  const C snowf = io.snowfall[c], prec = io.precip[c];
+  const C snowRate = (cold ? snowf + prec : snowf) * (C)k.rhoFresh2rhoSnow;
+  const C rainToSea = cold ? (C)0 : prec;
+  const C snowAcc = (snowRate * AP) * (C)k.dtT;
+  C gUnder = Fio * (C)k.qi;
+  gUnder = AP == (C)0 ? (C)0 : gUnder;
+  const C potMeltRate = (-Fia) * (C)k.qs;
+  const C potMelt = potMeltRate * (C)k.dtT;
   const bool allMelt = potMelt >= hsP;
-  const T melt = allMelt ? hsP : potMelt;
-  const T meltRate = allMelt ? melt * (T)k.rdtT : potMeltRate;
-  const T toMelt = allMelt ? ((-hsP) * (T)k.rdtT) / (T)k.qs : Fia;
+  const C melt = allMelt ? hsP : potMelt;
+  const C meltRate = allMelt ? melt * (C)k.rdtT : potMeltRate;
+  const C toMelt = allMelt ? ((-hsP) * (C)k.rdtT) / (C)k.qs : Fia;
   Fia = Fia - toMelt;
-  const T gSurf = Fia * (T)k.qi;
-  const T gNet = (gUnder + gSurf) * rregA;
-  const T sth = fmax(io.theta[c], Tfrz);
-  const T mltf = (T)1 + ((T)k.McPheeTaperFac - (T)1) / ((T)1 + exp((AP - (T)k.t0) * (T)k.t1));
-  const T Foi = ((T)(-k.t2) * (sth - Tfrz)) * mltf;
-  const T gML = Foi * (T)k.qi;
-  const T qnet_in = io.Qnet[c];
-  const T gOW = (T)k.qi * qnet_in;
-  const T dh = (gNet * AP + gOW * ((T)1 - AP)) + gML;
-  const T dhs = (snowRate - meltRate) * AP;
-  const T half = (T)0.5 * rhIA;
-  T dAoa = (half * gOW) * ((T)1 - AP);
-  const bool grow = (gOW > (T)0) && ((AP > (T)0) || (dh > (T)0));
-  if (grow) dAoa = gOW * ((T)1 - AP);
-  dAoa = (grow && (fc < (T)0)) ? dAoa * (T)k.recip_h0_south : dAoa * (T)k.recip_h0;
-  const T dAoi = gML <= (T)0 ? half * gML : (T)0;
-  const T dAia = ((gNet <= (T)0) && (hP > (T)0)) ? (half * gNet) * AP : (T)0;
-  const T dA = (dAoa + dAoi) + dAia;
-  const T Mc = io.M[c];
-  T A = AP + (dA * Mc) * (T)k.dtT;
-  T h = hP + (dh * Mc) * (T)k.dtT;
-  T hs = hsP + (dhs * Mc) * (T)k.dtT;
-  A = fmin(fmax(A, (T)0), (T)1);
-  h = fmax(h, (T)0);
-  hs = fmax(hs, (T)0);
-  const bool noIce = (h == (T)0) || (A == (T)0);
-  A = A * (T)(!noIce);
-  hs = hs * (T)(!noIce);
-  h = h * (T)(!noIce);
-  const T hsub = (hs * (T)k.rhoSnow + h * (T)k.rhoIce) * (T)k.recip_rhoSea;
-  const T flood = fmax((T)0, hsub - h);
+  const C gSurf = Fia * (C)k.qi;
+  const C gNet = (gUnder + gSurf) * rregA;
+  const C sth = fmax(io.theta[c], Tfrz);
+  const C mltf = (C)1 + ((C)k.McPheeTaperFac - (C)1) / ((C)1 + exp((AP - (C)k.t0) * (C)k.t1));
+  const C Foi = ((C)(-k.t2) * (sth - Tfrz)) * mltf;
+  const C gML = Foi * (C)k.qi;
+  const C qnet_in = io.Qnet[c];
+  const C gOW = (C)k.qi * qnet_in;
+  const C dh = (gNet * AP + gOW * ((C)1 - AP)) + gML;
+  const C dhs = (snowRate - meltRate) * AP;
+  const C half = (C)0.5 * rhIA;
+  C dAoa = (half * gOW) * ((C)1 - AP);
+  const bool grow = (gOW > (C)0) && ((AP > (C)0) || (dh > (C)0));
+  if (grow) dAoa = gOW * ((C)1 - AP);
+  dAoa = (grow && (fc < (C)0)) ? dAoa * (C)k.recip_h0_south : dAoa * (C)k.recip_h0;
+  const C dAoi = gML <= (C)0 ? half * gML : (C)0;
+  const C dAia = ((gNet <= (C)0) && (hP > (C)0)) ? (half * gNet) * AP : (C)0;
+  const C dA = (dAoa + dAoi) + dAia;
+  const C Mc = io.M[c];
+  C A = AP + (dA * Mc) * (C)k.dtT;
+  C h = hP + (dh * Mc) * (C)k.dtT;
+  C hs = hsP + (dhs * Mc) * (C)k.dtT;
+  A = fmin(fmax(A, (C)0), (C)1);
+  h = fmax(h, (C)0);
+  hs = fmax(hs, (C)0);
+  const bool noIce = (h == (C)0) || (A == (C)0);
+  A = A * (C)(!noIce);
+  hs = hs * (C)(!noIce);
+  h = h * (C)(!noIce);
+  const C hsub = (hs * (C)k.rhoSnow + h * (C)k.rhoIce) * (C)k.recip_rhoSea;
+  const C flood = fmax((C)0, hsub - h);
   h = h + flood;
-  hs = hs - flood * (T)k.rhoIce2rhoSnow;
-  const T rh = (T)1 / sqrt(h * h + (T)k.hIce_reg);
-  const T Qsw = pen * AP + io.Qsw[c] * ((T)1 - AP);
-  const T dH = h - hP;
-  const T meltedSnow = (hsP + snowAcc) - hs;
-  const T eIce = dH / (T)k.qi;
-  const T eOut = (AP * ((Fia + Fio) + pen) + ((T)1 - AP) * qnet_in) * (T)k.dtT;
-  const T Qnet = (eOut - eIce) * (T)k.rdtT;
-  const T salFac = ((sal > (T)0) && (sal > (T)k.saltIce_ref)) ? (T)1 - (T)k.saltIce_ref / sal : (T)1;
-  const T fwIce = ((-dH) * (T)k.rhoIce2rhoFresh) * salFac;
-  const T t0s = fmin((T)k.saltIce_ref, sal);
-  const T osh = io.osh[c], oshs = io.oshs[c];
-  const T saltflux = ((((dH + osh) * t0s) * Mc) * (T)k.rhoIce) * (T)k.rdtT;
-  const T fwSnow = meltedSnow / (T)k.rhoFresh2rhoSnow;
-  const T emp = (Mc * (((((-io.evap[c]) - prec) * ((T)1 - AP) - rainToSea * AP) - io.runoff[c]) -
-                       (fwIce + fwSnow) / (T)k.dtT)) * (T)k.rhoFresh +
-                ((Mc * (osh * (T)k.rhoIce + oshs * (T)k.rhoSnow)) * (T)k.rdtT);
-  const T fss = (emp * sal) / (T)k.rhoFresh + saltflux;
-  io.oh[c] = h;
-  io.ohs[c] = hs;
-  io.oA[c] = A;
-  io.oTs[c] = TSurf;
-  io.oEmPmR[c] = emp;
-  io.ofss[c] = fss;
-  io.oQsw[c] = Qsw;
-  io.oQnet[c] = Qnet;
-  io.oload[c] = h * (T)k.rhoIce + hs * (T)k.rhoSnow;
-  io.open[c] = pen;
-  io.orh[c] = rh;
+  hs = hs - flood * (C)k.rhoIce2rhoSnow;
+  const C rh = (C)1 / sqrt(h * h + (C)k.hIce_reg);
+  const C Qsw = pen * AP + io.Qsw[c] * ((C)1 - AP);
+  const C dH = h - hP;
+  const C meltedSnow = (hsP + snowAcc) - hs;
+  const C eIce = dH / (C)k.qi;
+  const C eOut = (AP * ((Fia + Fio) + pen) + ((C)1 - AP) * qnet_in) * (C)k.dtT;
+  const C Qnet = (eOut - eIce) * (C)k.rdtT;
+  const C salFac = ((sal > (C)0) && (sal > (C)k.saltIce_ref)) ? (C)1 - (C)k.saltIce_ref / sal : (C)1;
+  const C fwIce = ((-dH) * (C)k.rhoIce2rhoFresh) * salFac;
+  const C t0s = fmin((C)k.saltIce_ref, sal);
+  const C osh = io.osh[c], oshs = io.oshs[c];
+  const C saltflux = ((((dH + osh) * t0s) * Mc) * (C)k.rhoIce) * (C)k.rdtT;
+  const C fwSnow = meltedSnow / (C)k.rhoFresh2rhoSnow;
+  const C emp = (Mc * (((((-io.evap[c]) - prec) * ((C)1 - AP) - rainToSea * AP) - io.runoff[c]) -
+                       (fwIce + fwSnow) / (C)k.dtT)) * (C)k.rhoFresh +
+                ((Mc * (osh * (C)k.rhoIce + oshs * (C)k.rhoSnow)) * (C)k.rdtT);
+  const C fss = (emp * sal) / (C)k.rhoFresh + saltflux;
+  io.oh[c] = (T)h;
+  io.ohs[c] = (T)hs;
+  io.oA[c] = (T)A;
+  io.oTs[c] = (T)TSurf;
+  io.oEmPmR[c] = (T)emp;
+  io.ofss[c] = (T)fss;
+  io.oQsw[c] = (T)Qsw;
+  io.oQnet[c] = (T)Qnet;
+  io.oload[c] = (T)(h * (C)k.rhoIce + hs * (C)k.rhoSnow);
+  io.open[c] = (T)pen;
+  io.orh[c] = (T)rh;
 }
 
 template <typename T>

@@ -235,6 +235,37 @@ def config4(**kw) -> VerisState:
     return global_latlon(4, 1440, 720, -90.0, 90.0, h_sigma=10.0, polar_land=88.0, **kw)
 
 
+def config3_adjoint_inputs(state: VerisState, member: int = 0):
+    """Seeded inputs of the adjoint workload on ``state``'s grid: a spatial P*
+    field (lognormal, median 27500 N/m^2, log-std 0.3, 5-cell correlation)
+    and a target velocity field (N(0, 0.1) m/s smoothed over 3 cells, zero on
+    land and in the overlap).  Returns (pStar, uTarget, vTarget)."""
+    rng = rng_for(3, 20_000 + member)
+    NX, NY = state.shape
+    nx, ny = NX - 2 * OL, NY - 2 * OL
+    cyc = bool(state.settings.enable_cyclic_x)
+
+    def smooth_unit(sigma):
+        z = _gauss_smooth(rng.standard_normal((nx, ny)), sigma, periodic_x=cyc)
+        return (z - z.mean()) / z.std()
+
+    pstar = np.pad(27500.0 * np.exp(0.3 * smooth_unit(5.0)), OL, mode="edge")
+    pstar = fill_x(pstar) if cyc else pstar
+    vs = state.variables
+    ut = np.zeros((NX, NY))
+    vt = np.zeros((NX, NY))
+    ut[OL:-OL, OL:-OL] = 0.1 * smooth_unit(3.0)
+    vt[OL:-OL, OL:-OL] = 0.1 * smooth_unit(3.0)
+    return pstar, ut * np.asarray(vs.maskInU), vt * np.asarray(vs.maskInV)
+
+
+def config4_adjoint_member(member: int, **kw) -> VerisState:
+    """Member of the batched adjoint at 0.25 degree: 240 x 240 closed regional
+    domain with 10-cell thickness noise."""
+    kw.setdefault("nEVPsteps", 120)
+    return regional(4, 100 + member, 240, 240, **kw)
+
+
 def config2_sizes(n_members: int = 48):
     """Member sizes (nx, ny) of the ragged ensemble, uniform in 64..512 x 64..384."""
     rng = rng_for(2, 10_000)
