@@ -130,6 +130,15 @@ def bench_batched(dev, nsub: int, fp64_evp_rate: float, fp64_step_rate: float) -
     err = {}
     for q, name in enumerate(EVP_OUT):
         err[name] = _rel_l2(list(zip(e32.unpack(out32[q]), e64.unpack(out64[q]), masks)))
+    # FP32 storage with FP64 arithmetic
+    e32a = Ensemble(states, dev, torch.float32, arith="fp64")
+    e32a.pre_evp()
+    tile32a, _ = tune_batched(e32a, exact=False)
+    out32a = e32a.evp(nsub, tile=tile32a)
+    t32a = _sync_time(lambda: e32a.evp(nsub, tile=tile32a, out=out32a))
+    err_a = {name: _rel_l2(list(zip(e32a.unpack(out32a[q]), e64.unpack(out64[q]), masks)))
+             for q, name in enumerate(EVP_OUT)}
+    del e32a
     b32 = e32.budgets()
     b64 = e64.budgets()
     budget_err = {k: float(((b32[k] - b64[k]).abs() / b64[k].abs().clamp_min(1e-300)).max())
@@ -156,6 +165,11 @@ def bench_batched(dev, nsub: int, fp64_evp_rate: float, fp64_step_rate: float) -
                           "frac": rate32 * B_ALG_WORDS * 4 / 1e9 / HBM_PEAK_GBS},
         "fp32_rel_l2_vs_fp64": err,
         "velocity_error_target": 1e-4,
+        "fp32_storage_fp64_arith": {
+            "value": cells * nsub / t32a,
+            "tile": {"index": tile32a, "shape": shapes[tile32a]},
+            "over_plain_fp64_evp_1deg": cells * nsub / t32a / fp64_evp_rate,
+            "rel_l2_vs_fp64": err_a},
         "fp64_budget_rel_diff_of_fp32_fields": budget_err,
         "setup_s": t_gen,
     }

@@ -94,3 +94,28 @@ def test_batched_fp32_against_fp64_oracle(cuda):
             print(f"member {m} {name}: rel L2 {rel:.2e}", flush=True)
     assert worst["uIce"] <= 3e-4 and worst["vIce"] <= 3e-4, worst
     assert max(worst["sigma1"], worst["sigma2"], worst["sigma12"]) <= 1e-2, worst
+
+
+def test_batched_fp32_storage_fp64_arith_against_fp64_oracle(cuda):
+    """FP32-stored fields with FP64 arithmetic in every substep (arith="fp64"),
+    400 substeps, against the FP64 oracle over interior ice-covered cells.
+    Only the FP32 rounding of the stored state between substeps remains; a
+    NumPy emulation of that rounding gives ~1e-5 in velocity.  Bounds:
+    velocities 1e-4, stresses 5e-3."""
+    sizes = [(64, 64), (96, 80), (70, 112)]
+    states = [pre_evp(synth.regional(2, m, nx, ny)) for m, (nx, ny) in enumerate(sizes)]
+    ens = Ensemble(states, device=cuda, dtype=torch.float32, arith="fp64")
+    for tile in (3, 9):
+        out = ens.evp(400, tile=tile)
+        worst = {}
+        for m, st in enumerate(states):
+            want = O.evp_solver(st, n_substeps=400)
+            ice = (st.variables.iceMask > 0)[2:-2, 2:-2]
+            for q, name in enumerate(EVP_NAMES):
+                got = g2n(ens.view(out[q], m))[2:-2, 2:-2][ice]
+                ref = want[q][2:-2, 2:-2][ice]
+                rel = float(np.linalg.norm(got - ref) / max(np.linalg.norm(ref), 1e-300))
+                worst[name] = max(worst.get(name, 0.0), rel)
+        print(f"tile {tile}: " + ", ".join(f"{k} {v:.2e}" for k, v in worst.items()), flush=True)
+        assert worst["uIce"] <= 1e-4 and worst["vIce"] <= 1e-4, worst
+        assert max(worst["sigma1"], worst["sigma2"], worst["sigma12"]) <= 5e-3, worst

@@ -80,7 +80,7 @@ class Member(ctypes.Structure):
 
 class Batch(ctypes.Structure):
     _fields_ = [("n_members", ctypes.c_int32), ("dtype", ctypes.c_int32), ("tile", ctypes.c_int32),
-                ("reserved", ctypes.c_int32), ("members", ctypes.POINTER(Member)),
+                ("arith", ctypes.c_int32), ("members", ctypes.POINTER(Member)),
                 ("total_elems", ctypes.c_int64)]
 
 

@@ -33,10 +33,17 @@ class Ensemble:
     """Packed ensemble on one device.
 
     ``states``: member states (NumPy or torch variables); their settings must
-    agree except for ``nx``, ``ny`` and ``enable_cyclic_x``.
+    agree except for ``nx``, ``ny`` and ``enable_cyclic_x``.  ``arith``
+    ("native" or "fp64") chooses, for float32 fields, FP32 arithmetic (with
+    FP64 Delta, zeta, press) or FP64 arithmetic on the FP32-stored fields.
     """
 
-    def __init__(self, states, device="cuda", dtype=torch.float32):
+    def __init__(self, states, device="cuda", dtype=torch.float32, arith="native"):
+        if arith not in ("native", "fp64"):
+            raise ValueError("arith must be 'native' or 'fp64'")
+        if arith == "fp64" and dtype != torch.float32:
+            raise ValueError("arith='fp64' applies to float32 ensembles")
+        self.arith = arith
         if not states:
             raise ValueError("empty ensemble")
         self.device = torch.device(device)
@@ -103,7 +110,8 @@ class Ensemble:
 
     def _batch(self, tile):
         return N.Batch(n_members=self.n, dtype=N.VH_F64 if self.dtype == torch.float64 else N.VH_F32,
-                       tile=int(tile), reserved=0, members=self._members, total_elems=self.total)
+                       tile=int(tile), arith=1 if self.arith == "fp64" else 0,
+                       members=self._members, total_elems=self.total)
 
     def _fields_struct(self):
         f = N.Fields()

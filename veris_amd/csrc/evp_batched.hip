@@ -20,6 +20,8 @@ namespace vh {
 
 void evp_batched_launch_f32(int cfg, const EvpArgBlock<float>* ab, int which, int launch,
                             int n_tiles, hipStream_t st);
+void evp_batched_launch_f32_f64(int cfg, const EvpArgBlock<float>* ab, int which, int launch,
+                                int n_tiles, hipStream_t st);
 
 namespace {
 
@@ -51,6 +53,8 @@ int check_batch(const vh_batch* b) {
   VH_REQUIRE(b != nullptr && b->members != nullptr && b->n_members > 0, VH_EINVAL, "empty batch");
   VH_REQUIRE(b->dtype == VH_F64 || b->dtype == VH_F32, VH_EDTYPE, "dtype %d", b->dtype);
   VH_REQUIRE(b->tile >= 0 && b->tile < kEvpNumTiles, VH_EINVAL, "tile %d", b->tile);
+  VH_REQUIRE(b->arith == 0 || (b->arith == 1 && b->dtype == VH_F32), VH_EINVAL,
+             "arith %d (1 = FP64 arithmetic, FP32 storage only)", b->arith);
   std::vector<std::pair<long long, long long>> spans;
   long long tiles = 0;
   for (int m = 0; m < b->n_members; ++m) {
@@ -73,19 +77,20 @@ int check_batch(const vh_batch* b) {
 }
 
 template <typename T>
-void launch_batched(int cfg, const EvpArgBlock<T>* ab, int which, int launch, int n_tiles,
-                    hipStream_t st);
+void launch_batched(int cfg, int arith, const EvpArgBlock<T>* ab, int which, int launch,
+                    int n_tiles, hipStream_t st);
 
 template <>
-void launch_batched<double>(int cfg, const EvpArgBlock<double>* ab, int which, int launch,
+void launch_batched<double>(int cfg, int, const EvpArgBlock<double>* ab, int which, int launch,
                             int n_tiles, hipStream_t st) {
   evp_launch_cfg<double, double, true>(cfg, ab, which, launch, n_tiles, 0, st);
 }
 
 template <>
-void launch_batched<float>(int cfg, const EvpArgBlock<float>* ab, int which, int launch,
+void launch_batched<float>(int cfg, int arith, const EvpArgBlock<float>* ab, int which, int launch,
                            int n_tiles, hipStream_t st) {
-  evp_batched_launch_f32(cfg, ab, which, launch, n_tiles, st);
+  if (arith == 1) evp_batched_launch_f32_f64(cfg, ab, which, launch, n_tiles, st);
+  else evp_batched_launch_f32(cfg, ab, which, launch, n_tiles, st);
 }
 
 template <typename T>
@@ -169,13 +174,14 @@ int run_batched(const vh_batch* b, const vh_settings* s, const vh_fields* fl, in
   const int K = kEvpTiles[cfg].k;
   auto record = [&](hipStream_t cs) {
     int l = 0;
-    for (int q = 0; q < nsub / K; ++q) launch_batched<T>(cfg, ab_dev, 0, l++, n_main, cs);
-    for (int q = 0; q < nsub % K; ++q) launch_batched<T>(kEvpRemainderCfg, ab_dev, 1, l++, n_rem, cs);
+    for (int q = 0; q < nsub / K; ++q) launch_batched<T>(cfg, b->arith, ab_dev, 0, l++, n_main, cs);
+    for (int q = 0; q < nsub % K; ++q)
+      launch_batched<T>(kEvpRemainderCfg, b->arith, ab_dev, 1, l++, n_rem, cs);
   };
   if (nsub >= 8 && graphs_enabled()) {
     char key[192];
-    snprintf(key, sizeof(key), "evpb/%d/%d/%d/%d/%d/%p", (int)sizeof(T), cfg, nsub, n_main, n_rem,
-             (const void*)ab_dev);
+    snprintf(key, sizeof(key), "evpb/%d/%d/%d/%d/%d/%d/%p", (int)sizeof(T), b->arith, cfg, nsub,
+             n_main, n_rem, (const void*)ab_dev);
     int rc = graph_run(key, record, st);
     if (rc) return rc;
   } else {

@@ -1,6 +1,9 @@
-// FP32 instantiation of the batched EVP substep kernel (evp_kernel.h): FP32
-// bulk with FP64 Delta, zeta and replacement pressure.  Built with
-// -ffp-contract=fast so multiply-adds fuse.
+// FP32-storage instantiations of the batched EVP substep kernel
+// (evp_kernel.h), built with -ffp-contract=fast so multiply-adds fuse:
+//  - evp_batched_launch_f32: FP32 arithmetic, FP64 Delta, zeta and
+//    replacement pressure;
+//  - evp_batched_launch_f32_f64: FP64 arithmetic on FP32-stored fields (the
+//    HBM traffic of FP32, the rounding of FP64 inside each substep).
 #include "evp_kernel.h"
 
 namespace vh {
@@ -8,6 +11,11 @@ namespace vh {
 void evp_batched_launch_f32(int cfg, const EvpArgBlock<float>* ab, int which, int launch,
                             int n_tiles, hipStream_t st) {
   evp_launch_cfg<float, double, true>(cfg, ab, which, launch, n_tiles, 0, st);
+}
+
+void evp_batched_launch_f32_f64(int cfg, const EvpArgBlock<float>* ab, int which, int launch,
+                                int n_tiles, hipStream_t st) {
+  evp_launch_cfg<float, double, true, double>(cfg, ab, which, launch, n_tiles, 0, st);
 }
 
 }  // namespace vh

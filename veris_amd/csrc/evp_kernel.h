@@ -243,7 +243,7 @@ __device__ inline int map_axis(int p, int N, int n, int filled) {
   return wrap(p, N);
 }
 
-template <typename T, typename TD, int RI, int RJ, int K, bool BATCHED>
+template <typename T, typename TD, int RI, int RJ, int K, bool BATCHED, typename A = T>
 __global__ __launch_bounds__(RI* RJ) void evp_substep_kernel(const EvpArgBlock<T>* __restrict__ ab,
                                                              int which, int launch, int n_tiles,
                                                              int tiles_j, int per_xcd) {
@@ -251,8 +251,8 @@ __global__ __launch_bounds__(RI* RJ) void evp_substep_kernel(const EvpArgBlock<T
   constexpr int TI = RI - 2 * R, TJ = RJ - 2 * R, NT = RI * RJ;
   static_assert((RJ & (RJ - 1)) == 0, "RJ must be a power of two");
   static_assert(TI >= 1 && TJ >= 1, "tile too small for K");
-  __shared__ T sU[NT], sV[NT], sM[NT], sMU[NT], sMV[NT], sDU[NT], sDV[NT], sE12[NT], sQ[NT], sZ[NT];
-  __shared__ T sS11[NT], sS22[NT], sCD[NT], sCB[NT], sFV[NT], sFU[NT];
+  __shared__ A sU[NT], sV[NT], sM[NT], sMU[NT], sMV[NT], sDU[NT], sDV[NT], sE12[NT], sQ[NT], sZ[NT];
+  __shared__ A sS11[NT], sS22[NT], sCD[NT], sCB[NT], sFV[NT], sFU[NT];
 
   // tile of this workgroup: workgroups are dealt round-robin to the 8 XCDs,
   // so give each XCD a contiguous run of tiles (neighbours share L2 lines)
@@ -315,32 +315,32 @@ __global__ __launch_bounds__(RI* RJ) void evp_substep_kernel(const EvpArgBlock<T
                        (K > 1 || is_computed(g, i, j));
 
   // ---- all global loads of this cell, issued together and unconditionally
-  T u = ldg(src[0] + off, c), v = ldg(src[1] + off, c);
-  const T Mc = ldg(f.M + off, c), MUc = ldg(f.MU + off, c), MVc = ldg(f.MV + off, c);
-  const T uoc = ldg(f.uOc + off, c), voc = ldg(f.vOc + off, c);
-  const T mInUc = ldg(f.mInU + off, c), mInVc = ldg(f.mInV + off, c);
-  const T rdyU = ldg(f.rdyU + off, c), rdxV = ldg(f.rdxV + off, c), rAz = ldg(f.rAz + off, c);
-  const T rdxU = ldg(f.rdxU + off, c), rdyV = ldg(f.rdyV + off, c), mInC = ldg(f.mInC + off, c);
-  const T rrA = ldg(f.rrA + off, c), P = ldg(f.P + off, c);
-  T s1 = ldg(src[2] + off, c), s2 = ldg(src[3] + off, c);
-  const T dyV = ldg(f.dyV + off, c), dxU = ldg(f.dxU + off, c), fc = ldg(f.fC + off, c);
-  const T mC = ldg(f.massC + off, c);
-  T s12 = ldg(src[4] + off, c);
-  const T dxV = ldg(f.dxV + off, c), dyU = ldg(f.dyU + off, c);
-  const T rrAu = ldg(f.rrAu + off, c), rrAv = ldg(f.rrAv + off, c);
-  const T mU = ldg(f.massU + off, c), mV = ldg(f.massV + off, c);
-  const T AW = ldg(f.AreaW + off, c), AS = ldg(f.AreaS + off, c);
-  const T wfx = ldg(f.WFX + off, c), wfy = ldg(f.WFY + off, c);
-  const T uNm1 = ldg(f.uNm1 + off, c), vNm1 = ldg(f.vNm1 + off, c);
-  T k1Z = 0, k2Z = 0, k1C = 0, k2C = 0;
+  A u = ldg(src[0] + off, c), v = ldg(src[1] + off, c);
+  const A Mc = ldg(f.M + off, c), MUc = ldg(f.MU + off, c), MVc = ldg(f.MV + off, c);
+  const A uoc = ldg(f.uOc + off, c), voc = ldg(f.vOc + off, c);
+  const A mInUc = ldg(f.mInU + off, c), mInVc = ldg(f.mInV + off, c);
+  const A rdyU = ldg(f.rdyU + off, c), rdxV = ldg(f.rdxV + off, c), rAz = ldg(f.rAz + off, c);
+  const A rdxU = ldg(f.rdxU + off, c), rdyV = ldg(f.rdyV + off, c), mInC = ldg(f.mInC + off, c);
+  const A rrA = ldg(f.rrA + off, c), P = ldg(f.P + off, c);
+  A s1 = ldg(src[2] + off, c), s2 = ldg(src[3] + off, c);
+  const A dyV = ldg(f.dyV + off, c), dxU = ldg(f.dxU + off, c), fc = ldg(f.fC + off, c);
+  const A mC = ldg(f.massC + off, c);
+  A s12 = ldg(src[4] + off, c);
+  const A dxV = ldg(f.dxV + off, c), dyU = ldg(f.dyU + off, c);
+  const A rrAu = ldg(f.rrAu + off, c), rrAv = ldg(f.rrAv + off, c);
+  const A mU = ldg(f.massU + off, c), mV = ldg(f.massV + off, c);
+  const A AW = ldg(f.AreaW + off, c), AS = ldg(f.AreaS + off, c);
+  const A wfx = ldg(f.WFX + off, c), wfy = ldg(f.WFY + off, c);
+  const A uNm1 = ldg(f.uNm1 + off, c), vNm1 = ldg(f.vNm1 + off, c);
+  A k1Z = 0, k2Z = 0, k1C = 0, k2C = 0;
   if (sc.kterms) {
     k1Z = ldg(f.k1Z + off, c);
     k2Z = ldg(f.k2Z + off, c);
     k1C = ldg(f.k1C + off, c);
     k2C = ldg(f.k2C + off, c);
   }
-  T cbc = (T)0;  // basal drag coefficient inputs are step constants: load lazily
-  const T sg = sgn(fc);
+  A cbc = (A)0;  // basal drag coefficient inputs are step constants: load lazily
+  const A sg = sgn(fc);
 
 #pragma unroll 1
   for (int s = 0; s < K; ++s) {
@@ -366,18 +366,18 @@ __global__ __launch_bounds__(RI* RJ) void evp_substep_kernel(const EvpArgBlock<T
 
     // ---- B: strain rate at z-points (veris/dynamics_routines.py:160-175)
     if (zc) {
-      const T ujm = sU[t - 1], vim = sV[t - RJ];
-      const T dudy = (u - ujm) * rdyU;
-      const T uavz = (u + ujm) * (T)0.5;
-      const T dvdx = (v - vim) * rdxV;
-      const T vavz = (v + vim) * (T)0.5;
-      const T mz = (Mc * sM[t - RJ]) * (sM[t - 1] * sM[t - RJ - 1]);
-      T inner = dudy + dvdx;
+      const A ujm = sU[t - 1], vim = sV[t - RJ];
+      const A dudy = (u - ujm) * rdyU;
+      const A uavz = (u + ujm) * (A)0.5;
+      const A dvdx = (v - vim) * rdxV;
+      const A vavz = (v + vim) * (A)0.5;
+      const A mz = (Mc * sM[t - RJ]) * (sM[t - 1] * sM[t - RJ - 1]);
+      A inner = dudy + dvdx;
       if (sc.kterms) inner = (inner - k1Z * vavz) - k2Z * uavz;
-      T e12 = ((T)0.5 * inner) * mz;
-      const T hu = MUc - sMU[t - 1];
-      const T hv = MVc - sMV[t - RJ];
-      e12 = e12 + ((((T)2.0 * uavz) * rdyU) * hu + (((T)2.0 * vavz) * rdxV) * hv);
+      A e12 = ((A)0.5 * inner) * mz;
+      const A hu = MUc - sMU[t - 1];
+      const A hv = MVc - sMV[t - RJ];
+      e12 = e12 + ((((A)2.0 * uavz) * rdyU) * hu + (((A)2.0 * vavz) * rdxV) * hv);
       sE12[t] = e12;
       sQ[t] = rAz * (e12 * e12);
     }
@@ -386,58 +386,58 @@ __global__ __launch_bounds__(RI* RJ) void evp_substep_kernel(const EvpArgBlock<T
     // ---- C: c-point rheology (veris/dynamics_routines.py:151-158, 220-268),
     //         stresses (veris/evp_solver.py:79-94), drags (:108-110), Coriolis (:142-143)
     if (cc) {
-      const T uip = sU[t + RJ], vjp = sV[t + 1];
-      const T dudx = (uip - u) * rdxU;
-      const T uave = (uip + u) * (T)0.5;
-      const T dvdy = (vjp - v) * rdyV;
-      const T vave = (vjp + v) * (T)0.5;
-      T e11 = dudx, e22 = dvdy;
+      const A uip = sU[t + RJ], vjp = sV[t + 1];
+      const A dudx = (uip - u) * rdxU;
+      const A uave = (uip + u) * (A)0.5;
+      const A dvdy = (vjp - v) * rdyV;
+      const A vave = (vjp + v) * (A)0.5;
+      A e11 = dudx, e22 = dvdy;
       if (sc.kterms) {
         e11 = dudx + vave * k2C;
         e22 = dvdy + uave * k1C;
       }
       e11 = e11 * mInC;
       e22 = e22 * mInC;
-      const T qa = sQ[t] + sQ[t + RJ];
-      const T qb = sQ[t + 1] + sQ[t + RJ + 1];
-      const T e12c = ((T)0.25 * rrA) * (qa + qb);
-      const T ep = e11 + e22, em = e11 - e22;
+      const A qa = sQ[t] + sQ[t + RJ];
+      const A qb = sQ[t + 1] + sQ[t + RJ + 1];
+      const A e12c = ((A)0.25 * rrA) * (qa + qb);
+      const A ep = e11 + e22, em = e11 - e22;
       const TD dsq = (TD)ep * (TD)ep + (TD)sc.rpd * ((TD)em * (TD)em + (TD)4.0 * (TD)e12c);
       const TD dC = sqrt(dsq);
       const TD dreg = dC + (TD)sc.deltaMin;
       const TD zeta = ((TD)0.5 * ((TD)P * (TD)sc.tens1p)) / dreg;
-      sZ[t] = (T)zeta;
+      sZ[t] = (A)zeta;
       if (cs) {
         const TD pressD =
             (((TD)P * (TD)sc.pr1m) + ((((TD)2.0 * zeta) * dC) * (TD)sc.pr) / (TD)sc.tens1p) *
             (TD)sc.tens1m;
-        const T zt = (T)zeta, press = (T)pressD;
-        const T divg = (((T)2 * zt) * ep) - press;
-        const T tens = ((T)2 * zt) * em;
-        s1 = (((s1 * (T)sc.alphaM1) + divg) * (T)sc.den1) * Mc;
-        s2 = (((s2 * (T)sc.alphaM1) + tens * (T)0.25) * (T)sc.den2) * Mc;
-        sS11[t] = ((T)0.5 * (s1 + s2)) * dyV;
-        sS22[t] = ((T)0.5 * (s1 - s2)) * dxU;
-        const T D = (T)(fc < (T)0 ? sc.cwS : sc.cwN) * (T)sc.rhoSea;
-        const T su = sDU[t] + sDU[t + RJ], sv = sDV[t] + sDV[t + 1];
-        const T w = (T)0.25 * (su * su + sv * sv);
-        const T cd = ((D * D) * w > (T)sc.cDragMin2) ? D * sqrt(w) : (T)sc.cDragMin;
+        const A zt = (A)zeta, press = (A)pressD;
+        const A divg = (((A)2 * zt) * ep) - press;
+        const A tens = ((A)2 * zt) * em;
+        s1 = (((s1 * (A)sc.alphaM1) + divg) * (A)sc.den1) * Mc;
+        s2 = (((s2 * (A)sc.alphaM1) + tens * (A)0.25) * (A)sc.den2) * Mc;
+        sS11[t] = ((A)0.5 * (s1 + s2)) * dyV;
+        sS22[t] = ((A)0.5 * (s1 - s2)) * dxU;
+        const A D = (A)(fc < (A)0 ? sc.cwS : sc.cwN) * (A)sc.rhoSea;
+        const A su = sDU[t] + sDU[t + RJ], sv = sDV[t] + sDV[t + 1];
+        const A w = (A)0.25 * (su * su + sv * sv);
+        const A cd = ((D * D) * w > (A)sc.cDragMin2) ? D * sqrt(w) : (A)sc.cDragMin;
         sCD[t] = cd * Mc;
         if (sc.basal) {
           const int cip = map_axis<K>(i + 1, g.NX, g.nx, g.fill_x) * g.NY + gj;
           const int cjp = gi * g.NY + map_axis<K>(j + 1, g.NY, g.ny, g.fill_y);
-          const T um = u * mInUc, umip = uip * ldg(f.mInU + off, cip);
-          const T vm = v * mInVc, vmjp = vjp * ldg(f.mInV + off, cjp);
-          T tb = (T)0.25 * (((um * um + umip * umip) + vm * vm) + vmjp * vmjp);
-          tb = (T)sc.K2 / sqrt(tb + (T)sc.U0sq);
-          const T Ar = ldg(f.Area + off, c);
-          const T hc = (fabs(ldg(f.Rlow + off, c)) * Ar) / (T)sc.K1;
-          const T val = ((tb * log(exp((T)10.0 * (ldg(f.hIce + off, c) - hc)) + (T)1.0)) * (T)0.1) *
-                        exp((T)(-sc.cBasalStar) * ((T)1.0 - Ar));
-          cbc = Ar > (T)0.01 ? val : (T)0;
+          const A um = u * mInUc, umip = uip * ldg(f.mInU + off, cip);
+          const A vm = v * mInVc, vmjp = vjp * ldg(f.mInV + off, cjp);
+          A tb = (A)0.25 * (((um * um + umip * umip) + vm * vm) + vmjp * vmjp);
+          tb = (A)sc.K2 / sqrt(tb + (A)sc.U0sq);
+          const A Ar = ldg(f.Area + off, c);
+          const A hc = (fabs(ldg(f.Rlow + off, c)) * Ar) / (A)sc.K1;
+          const A val = ((tb * log(exp((A)10.0 * (ldg(f.hIce + off, c) - hc)) + (A)1.0)) * (A)0.1) *
+                        exp((A)(-sc.cBasalStar) * ((A)1.0 - Ar));
+          cbc = Ar > (A)0.01 ? val : (A)0;
         }
         sCB[t] = cbc;
-        const T mf = (mC * fc) * (T)0.5;
+        const A mf = (mC * fc) * (A)0.5;
         sFV[t] = mf * (v + vjp);
         sFU[t] = mf * (u + uip);
       }
@@ -450,11 +450,11 @@ __global__ __launch_bounds__(RI* RJ) void evp_substep_kernel(const EvpArgBlock<T
 
     // ---- D: sigma12 on z-points (veris/averaging.py:4-19, veris/evp_solver.py:86,103)
     if (zs) {
-      const T n = (Mc + sM[t - RJ]) + (sM[t - 1] + sM[t - RJ - 1]);
-      const T nrm = n > (T)0 ? (T)1.0 / n : (T)0.0;
-      const T zz = nrm * ((sZ[t] + sZ[t - RJ]) + (sZ[t - 1] + sZ[t - RJ - 1]));
-      const T shear = ((T)2 * zz) * sE12[t];
-      s12 = ((s12 * (T)sc.alphaM1) + shear * (T)0.25) * (T)sc.den2;
+      const A n = (Mc + sM[t - RJ]) + (sM[t - 1] + sM[t - RJ - 1]);
+      const A nrm = n > (A)0 ? (A)1.0 / n : (A)0.0;
+      const A zz = nrm * ((sZ[t] + sZ[t - RJ]) + (sZ[t - 1] + sZ[t - RJ - 1]));
+      const A shear = ((A)2 * zz) * sE12[t];
+      s12 = ((s12 * (A)sc.alphaM1) + shear * (A)0.25) * (A)sc.den2;
       sQ[t] = s12 * dxV;   // sigma12*dxV (q is dead after C)
       sDU[t] = s12 * dyU;  // sigma12*dyU (du is dead after C)
     }
@@ -463,37 +463,37 @@ __global__ __launch_bounds__(RI* RJ) void evp_substep_kernel(const EvpArgBlock<T
     // ---- E: stress divergence (veris/dynamics_routines.py:284-302), forcing
     //         (veris/evp_solver.py:112-145) and momentum update (:152-187)
     if (upd) {
-      const T divx = (((sS11[t] - sS11[t - RJ]) - sQ[t]) + sQ[t + 1]) * rrAu;
-      const T divy = (((sS22[t] - sS22[t - 1]) - sDU[t]) + sDU[t + RJ]) * rrAv;
-      const T cdc = sCD[t];
-      const T cdU = (T)0.5 * (cdc + sCD[t - RJ]);
-      const T cdV = (T)0.5 * (cdc + sCD[t - 1]);
-      T inx = (cdU * (T)sc.cosWat) * uoc;
-      T iny = (cdV * (T)sc.cosWat) * voc;
+      const A divx = (((sS11[t] - sS11[t - RJ]) - sQ[t]) + sQ[t + 1]) * rrAu;
+      const A divy = (((sS22[t] - sS22[t - 1]) - sDU[t]) + sDU[t + RJ]) * rrAv;
+      const A cdc = sCD[t];
+      const A cdU = (A)0.5 * (cdc + sCD[t - RJ]);
+      const A cdV = (A)0.5 * (cdc + sCD[t - 1]);
+      A inx = (cdU * (A)sc.cosWat) * uoc;
+      A iny = (cdV * (A)sc.cosWat) * voc;
       if (sc.turn) {
-        const T lmU = mU != (T)0 ? (T)1 : mU;
-        const T lmV = mV != (T)0 ? (T)1 : mV;
-        const T X = (cdc * (T)0.5) * ((uoc - u) + (sMU[t + 1] - sU[t + 1])) +
-                    (sCD[t - RJ] * (T)0.5) *
+        const A lmU = mU != (A)0 ? (A)1 : mU;
+        const A lmV = mV != (A)0 ? (A)1 : mV;
+        const A X = (cdc * (A)0.5) * ((uoc - u) + (sMU[t + 1] - sU[t + 1])) +
+                    (sCD[t - RJ] * (A)0.5) *
                         ((sMU[t - RJ] - sU[t - RJ]) + (sMU[t - RJ + 1] - sU[t - RJ + 1]));
-        const T Y = (cdc * (T)0.5) * ((voc - v) + (sMV[t + RJ] - sV[t + RJ])) +
-                    (sCD[t - 1] * (T)0.5) *
+        const A Y = (cdc * (A)0.5) * ((voc - v) + (sMV[t + RJ] - sV[t + RJ])) +
+                    (sCD[t - 1] * (A)0.5) *
                         ((sMV[t - 1] - sV[t - 1]) + (sMV[t + RJ - 1] - sV[t + RJ - 1]));
-        inx = inx - (((sg * (T)sc.sinWat) * (T)0.5) * X) * lmU;
-        iny = iny + (((sg * (T)sc.sinWat) * (T)0.5) * Y) * lmV;
+        inx = inx - (((sg * (A)sc.sinWat) * (A)0.5) * X) * lmU;
+        iny = iny + (((sg * (A)sc.sinWat) * (A)0.5) * Y) * lmV;
       }
-      T fx = wfx + inx * AW;
-      T fy = wfy + iny * AS;
-      fx = fx + (T)0.5 * (sFV[t] + sFV[t - RJ]);
-      fy = fy - (T)0.5 * (sFU[t] + sFU[t - 1]);
-      T denU = mU * (T)sc.betaP1 + AW * ((cdU * (T)sc.cosWat) + (T)0.5 * (sCB[t] + sCB[t - RJ]));
-      T denV = mV * (T)sc.betaP1 + AS * ((cdV * (T)sc.cosWat) + (T)0.5 * (sCB[t] + sCB[t - 1]));
-      denU = denU == (T)0 ? (T)1 : denU;
-      denV = denV == (T)0 ? (T)1 : denV;
-      const T un =
-          (MUc * ((((T)sc.betaU * mU) * u + (mU * (T)sc.rdt) * uNm1 + fx) + divx)) / denU;
-      const T vn =
-          (MVc * ((((T)sc.betaU * mV) * v + (mV * (T)sc.rdt) * vNm1 + fy) + divy)) / denV;
+      A fx = wfx + inx * AW;
+      A fy = wfy + iny * AS;
+      fx = fx + (A)0.5 * (sFV[t] + sFV[t - RJ]);
+      fy = fy - (A)0.5 * (sFU[t] + sFU[t - 1]);
+      A denU = mU * (A)sc.betaP1 + AW * ((cdU * (A)sc.cosWat) + (A)0.5 * (sCB[t] + sCB[t - RJ]));
+      A denV = mV * (A)sc.betaP1 + AS * ((cdV * (A)sc.cosWat) + (A)0.5 * (sCB[t] + sCB[t - 1]));
+      denU = denU == (A)0 ? (A)1 : denU;
+      denV = denV == (A)0 ? (A)1 : denV;
+      const A un =
+          (MUc * ((((A)sc.betaU * mU) * u + (mU * (A)sc.rdt) * uNm1 + fx) + divx)) / denU;
+      const A vn =
+          (MVc * ((((A)sc.betaU * mV) * v + (mV * (A)sc.rdt) * vNm1 + fy) + divy)) / denV;
       u = un;  // other threads read sU/sV, never these registers
       v = vn;
     }
@@ -508,12 +508,12 @@ __global__ __launch_bounds__(RI* RJ) void evp_substep_kernel(const EvpArgBlock<T
         store_filled(g, GW(dst[1] + off), i, j, v);
       }
     } else {
-      stg(dst[0] + off, cp, u);
-      stg(dst[1] + off, cp, v);
+      stg(dst[0] + off, cp, (T)u);
+      stg(dst[1] + off, cp, (T)v);
     }
-    stg(dst[2] + off, cp, s1);
-    stg(dst[3] + off, cp, s2);
-    stg(dst[4] + off, cp, s12);
+    stg(dst[2] + off, cp, (T)s1);
+    stg(dst[3] + off, cp, (T)s2);
+    stg(dst[4] + off, cp, (T)s12);
   }
 }
 
@@ -539,35 +539,36 @@ inline int evp_n_launches(int cfg, int nsub) {
   return nsub / kEvpTiles[cfg].k + nsub % kEvpTiles[cfg].k;
 }
 
-template <typename T, typename TD, int RI, int RJ, int K, bool BATCHED>
+template <typename T, typename TD, int RI, int RJ, int K, bool BATCHED, typename A = T>
 inline void evp_launch_shape(const EvpArgBlock<T>* ab, int which, int launch, int n_tiles,
                              int tiles_j, hipStream_t st) {
   const int per_xcd = (n_tiles + 7) / 8;
-  hipLaunchKernelGGL((evp_substep_kernel<T, TD, RI, RJ, K, BATCHED>), dim3(per_xcd * 8),
+  hipLaunchKernelGGL((evp_substep_kernel<T, TD, RI, RJ, K, BATCHED, A>), dim3(per_xcd * 8),
                      dim3(RI * RJ), 0, st, ab, which, launch, n_tiles, tiles_j, per_xcd);
 }
 
 // One launch of shape `cfg`: tiles_j is used by single-domain launches, `which`
-// (0 main shape, 1 remainder shape) by batched ones.
-template <typename T, typename TD, bool BATCHED>
+// (0 main shape, 1 remainder shape) by batched ones.  T is the storage type
+// of the fields, A the arithmetic type (default T), TD that of Delta/zeta.
+template <typename T, typename TD, bool BATCHED, typename A = T>
 inline void evp_launch_cfg(int cfg, const EvpArgBlock<T>* ab, int which, int launch, int n_tiles,
                            int tiles_j, hipStream_t st) {
   switch (cfg) {
-    case 1: evp_launch_shape<T, TD, 8, 64, 1, BATCHED>(ab, which, launch, n_tiles, tiles_j, st); break;
-    case 2: evp_launch_shape<T, TD, 12, 64, 1, BATCHED>(ab, which, launch, n_tiles, tiles_j, st); break;
-    case 3: evp_launch_shape<T, TD, 16, 64, 1, BATCHED>(ab, which, launch, n_tiles, tiles_j, st); break;
-    case 4: evp_launch_shape<T, TD, 8, 32, 1, BATCHED>(ab, which, launch, n_tiles, tiles_j, st); break;
-    case 5: evp_launch_shape<T, TD, 16, 32, 1, BATCHED>(ab, which, launch, n_tiles, tiles_j, st); break;
-    case 6: evp_launch_shape<T, TD, 32, 32, 1, BATCHED>(ab, which, launch, n_tiles, tiles_j, st); break;
-    case 7: evp_launch_shape<T, TD, 6, 64, 1, BATCHED>(ab, which, launch, n_tiles, tiles_j, st); break;
-    case 8: evp_launch_shape<T, TD, 16, 64, 2, BATCHED>(ab, which, launch, n_tiles, tiles_j, st); break;
-    case 9: evp_launch_shape<T, TD, 32, 32, 2, BATCHED>(ab, which, launch, n_tiles, tiles_j, st); break;
-    case 10: evp_launch_shape<T, TD, 12, 64, 2, BATCHED>(ab, which, launch, n_tiles, tiles_j, st); break;
-    case 11: evp_launch_shape<T, TD, 32, 32, 3, BATCHED>(ab, which, launch, n_tiles, tiles_j, st); break;
-    case 12: evp_launch_shape<T, TD, 16, 64, 3, BATCHED>(ab, which, launch, n_tiles, tiles_j, st); break;
-    case 13: evp_launch_shape<T, TD, 32, 32, 4, BATCHED>(ab, which, launch, n_tiles, tiles_j, st); break;
-    case 14: evp_launch_shape<T, TD, 16, 32, 2, BATCHED>(ab, which, launch, n_tiles, tiles_j, st); break;
-    default: evp_launch_shape<T, TD, 24, 32, 2, BATCHED>(ab, which, launch, n_tiles, tiles_j, st); break;
+    case 1: evp_launch_shape<T, TD, 8, 64, 1, BATCHED, A>(ab, which, launch, n_tiles, tiles_j, st); break;
+    case 2: evp_launch_shape<T, TD, 12, 64, 1, BATCHED, A>(ab, which, launch, n_tiles, tiles_j, st); break;
+    case 3: evp_launch_shape<T, TD, 16, 64, 1, BATCHED, A>(ab, which, launch, n_tiles, tiles_j, st); break;
+    case 4: evp_launch_shape<T, TD, 8, 32, 1, BATCHED, A>(ab, which, launch, n_tiles, tiles_j, st); break;
+    case 5: evp_launch_shape<T, TD, 16, 32, 1, BATCHED, A>(ab, which, launch, n_tiles, tiles_j, st); break;
+    case 6: evp_launch_shape<T, TD, 32, 32, 1, BATCHED, A>(ab, which, launch, n_tiles, tiles_j, st); break;
+    case 7: evp_launch_shape<T, TD, 6, 64, 1, BATCHED, A>(ab, which, launch, n_tiles, tiles_j, st); break;
+    case 8: evp_launch_shape<T, TD, 16, 64, 2, BATCHED, A>(ab, which, launch, n_tiles, tiles_j, st); break;
+    case 9: evp_launch_shape<T, TD, 32, 32, 2, BATCHED, A>(ab, which, launch, n_tiles, tiles_j, st); break;
+    case 10: evp_launch_shape<T, TD, 12, 64, 2, BATCHED, A>(ab, which, launch, n_tiles, tiles_j, st); break;
+    case 11: evp_launch_shape<T, TD, 32, 32, 3, BATCHED, A>(ab, which, launch, n_tiles, tiles_j, st); break;
+    case 12: evp_launch_shape<T, TD, 16, 64, 3, BATCHED, A>(ab, which, launch, n_tiles, tiles_j, st); break;
+    case 13: evp_launch_shape<T, TD, 32, 32, 4, BATCHED, A>(ab, which, launch, n_tiles, tiles_j, st); break;
+    case 14: evp_launch_shape<T, TD, 16, 32, 2, BATCHED, A>(ab, which, launch, n_tiles, tiles_j, st); break;
+    default: evp_launch_shape<T, TD, 24, 32, 2, BATCHED, A>(ab, which, launch, n_tiles, tiles_j, st); break;
   }
 }
 
