@@ -50,6 +50,23 @@ HBM_PEAK_GBS = 8000.0
 B_ALG_WORDS = 38  # algorithmic words per EVP cell-substep (33 reads + 5 writes)
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "evp_substep_1deg_fp64_traffic.md")
+
+
+def measured_traffic(kernel: str):
+    """HBM bytes per launch of ``kernel`` from the rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary (copy-calibrated), or None if it was not measured."""
+    try:
+        with open(TRAFFIC_FILE) as fh:
+            for line in fh:
+                parts = line.split()
+                if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch" and parts[1] == kernel:
+                    return float(parts[2])
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(n_substeps: int) -> dict:
     """NumPy FP64 oracle on one full 1-degree step (single thread)."""
     os.environ.setdefault("OMP_NUM_THREADS", "1")
@@ -372,6 +389,9 @@ def main(argv=None) -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the batched, adjoint and 0.25-degree measurements")
+    ap.add_argument("--calibration-copy", action="store_true",
+                    help="first copy a 64 MiB buffer 5 times (a kernel of known HBM traffic, "
+                         "the reference for rocprofv3 FETCH_SIZE / WRITE_SIZE counts)")
     args = ap.parse_args(argv)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: every box in this setup has a single MI355X; "
@@ -385,6 +405,13 @@ def main(argv=None) -> int:
     from veris_amd.step import seaice_step
 
     dev = torch.device("cuda:0")
+    if args.calibration_copy:
+        src = torch.ones(8 << 20, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     st = synth.config1()
     nx, ny = st.settings.nx, st.settings.ny
     q0 = torch.from_numpy(st.variables.Qnet.copy()).to(dev)
@@ -438,6 +465,7 @@ def main(argv=None) -> int:
     bytes_per_launch = B_ALG_WORDS * 8 * cells
     achieved = bytes_per_launch / per_launch_s / 1e9
     finite = all(bool(torch.isfinite(g.variables[k]).all()) for k in ("uIce", "hIceMean", "Qnet"))
+    kernel = f"evp_substep_kernel<double,double,{ri},{rj},{kk},false>"
 
     line = {
         "metric": "EVP cell-substeps/s at 1 degree (plain FP64 GPU drop-in, full step)",
@@ -457,12 +485,13 @@ def main(argv=None) -> int:
                    "nx": nx, "ny": ny, "evp_substeps": nsub, "parallelism": "none (1 GPU)"},
         "roofline": {
             "bound": "hbm",
-            "kernel": f"evp_substep_kernel<double,double,{ri},{rj},{kk},false>",
+            "kernel": kernel,
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": measured_traffic(kernel),
+            "traffic_source": os.path.relpath(TRAFFIC_FILE, ROOT),
             "avg_launch_us": per_launch_s * 1e6,
             "algorithmic_bytes_per_launch": bytes_per_launch,
         },

@@ -30,14 +30,20 @@ def _dtype_code(t: torch.Tensor) -> int:
 
 
 def _all_zero(t: torch.Tensor) -> bool:
-    """Whether ``t`` is all zeros, remembered on the tensor object for its
-    current version counter (an address-keyed cache would be reused by a new
-    tensor that the allocator places at the same address)."""
-    tag = getattr(t, "_vh_all_zero", None)
-    if tag is not None and tag[0] == t._version:
-        return tag[1]
-    res = bool(torch.count_nonzero(t).item() == 0)
-    t._vh_all_zero = (t._version, res)
+    """Whether ``t`` is all zeros, remembered on the tensor object that owns
+    the storage (a view's base: views share its version counter) for the
+    current version and the view's extent (an address-keyed cache would be
+    reused by a new tensor that the allocator places at the same address)."""
+    owner = t._base if t._base is not None else t
+    key = (t.storage_offset(), tuple(t.shape), tuple(t.stride()))
+    tags = getattr(owner, "_vh_all_zero", None)
+    if tags is None or tags[0] != t._version:
+        tags = (t._version, {})
+        owner._vh_all_zero = tags
+    res = tags[1].get(key)
+    if res is None:
+        res = bool(torch.count_nonzero(t).item() == 0)
+        tags[1][key] = res
     return res
 
 
