@@ -85,3 +85,54 @@ def test_null_arguments_are_rejected_without_touching_the_gpu():
     with pytest.raises(N.VerisHipError, match="VH_EINVAL"):
         N.call("vh_seaice_mass", ctypes.byref(dom), ctypes.byref(sett), ctypes.byref(fields),
                None, None, None, None)
+
+
+def test_adjoint_and_batch_structs_match_header():
+    text = _header()
+    assert _struct_members(text, "vh_evp_adjoint_out") == [f[0] for f in N.EvpAdjointOut._fields_]
+    assert _struct_members(text, "vh_pre_evp_adjoint_out") == [
+        f[0] for f in N.PreEvpAdjointOut._fields_]
+    assert _struct_members(text, "vh_batch") == [f[0] for f in N.Batch._fields_]
+    assert _struct_members(text, "vh_member") == [f[0] for f in N.Member._fields_]
+
+
+def test_adjoint_host_side_validation():
+    from veris_amd.settings import make_settings
+
+    dom = N.Domain(nx=360, ny=160, olx=2, oly=2, cyclic_x=1, fill_mode=0, dtype=N.VH_F64, tile=0)
+    n = ctypes.c_size_t(0)
+    N.call("vh_evp_adjoint_workspace_bytes", ctypes.byref(dom), 120, 10, ctypes.byref(n))
+    fb = 364 * 164 * 8
+    # checkpoints, ping-pong, two segment sets and the state adjoints at least
+    assert n.value >= (13 + 2 + 22 + 10) * 5 * fb // 5
+    m = ctypes.c_size_t(0)
+    N.call("vh_evp_adjoint_workspace_bytes", ctypes.byref(dom), 120, 1, ctypes.byref(m))
+    assert m.value > n.value  # every state kept
+    with pytest.raises(N.VerisHipError, match="VH_EINVAL"):
+        N.call("vh_evp_adjoint_workspace_bytes", ctypes.byref(dom), 0, 10, ctypes.byref(n))
+    sett = N.settings_struct(make_settings(basalDragK2=8.0))
+    fields = N.Fields()
+    with pytest.raises(N.VerisHipError, match="VH_EUNSUPPORTED.*basal"):
+        N.call("vh_evp_forward_checkpointed", ctypes.byref(dom), ctypes.byref(sett),
+               ctypes.byref(fields), 120, 10, None, None, None, None, None, None, 0, None)
+    sett = N.settings_struct(make_settings())
+    with pytest.raises(N.VerisHipError, match="VH_EINVAL.*missing input"):
+        N.call("vh_evp_forward_checkpointed", ctypes.byref(dom), ctypes.byref(sett),
+               ctypes.byref(fields), 120, 10, None, None, None, None, None, None, 0, None)
+    multi = N.Domain(nx=360, ny=160, olx=2, oly=2, cyclic_x=1, fill_mode=0, dtype=N.VH_F64, tile=8)
+    with pytest.raises(N.VerisHipError, match="VH_EINVAL.*single-substep"):
+        N.call("vh_evp_adjoint", ctypes.byref(multi), ctypes.byref(sett), ctypes.byref(fields),
+               120, 10, None, None, None, None, None, ctypes.byref(N.EvpAdjointOut()), None, 0,
+               None)
+
+
+def test_batch_arith_validation():
+    members = (N.Member * 1)(N.Member(nx=64, ny=64, cyclic_x=0, fill_mode=0, offset=0))
+    n = ctypes.c_size_t(0)
+    ok = N.Batch(n_members=1, dtype=N.VH_F32, tile=0, arith=1, members=members,
+                 total_elems=68 * 68)
+    N.call("vh_evp_batched_workspace_bytes", ctypes.byref(ok), ctypes.byref(n))
+    bad = N.Batch(n_members=1, dtype=N.VH_F64, tile=0, arith=1, members=members,
+                  total_elems=68 * 68)
+    with pytest.raises(N.VerisHipError, match="VH_EINVAL.*arith"):
+        N.call("vh_evp_batched_workspace_bytes", ctypes.byref(bad), ctypes.byref(n))
