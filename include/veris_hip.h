@@ -445,6 +445,51 @@ int vh_pre_evp_adjoint(const vh_domain* d, const vh_settings* s, const vh_fields
                        const void* pStarField, const vh_evp_adjoint_out* evp_adj,
                        const vh_pre_evp_adjoint_out* out, void* stream);
 
+/* Outputs of vh_pre_evp (caller-owned device fields). */
+typedef struct vh_pre_evp_out {
+  void *SeaIceMassC, *SeaIceMassU, *SeaIceMassV, *AreaW, *AreaS, *WindForcingX, *WindForcingY,
+      *SeaIceStrength;
+} vh_pre_evp_out;
+
+/* SeaIceMass (veris/area_mass.py:17-28), AreaWS (:5-14), WindForcingXY
+ * (veris/dynsolver.py:7-84) and SeaIceStrength (veris/dynamics_routines.py:
+ * 5-18, pStarField per cell or NULL for the pStar setting) in one sweep, with
+ * the same results as the four separate entry points. */
+int vh_pre_evp(const vh_domain* d, const vh_settings* s, const vh_fields* f,
+               const void* pStarField, const vh_pre_evp_out* out, void* stream);
+
+/* Outputs of vh_ocean_to_ice. */
+typedef struct vh_ocean_to_ice_out {
+  void *uOcean, *vOcean, *theta, *ocSalt, *Qnet, *Qsw, *wSpeed;
+} vh_ocean_to_ice_out;
+
+/* The host ocean's surface fields handed to the sea ice before a step, as the
+ * reference setup does inline (veris/setup/seaice_global_4deg/
+ * seaice_global_4deg.py:847,868-876): ocean surface velocities and salinity
+ * copied, theta = surface temperature + celsius2K, Qnet = -forc_temp_surface
+ * * cpWater * rhoSea, Qsw = -SWdown (f->SWdown), wSpeed = |(uWind, vWind)|.
+ * The surface fields are contiguous padded 2-D fields (the caller takes the
+ * top level and current time slice). */
+int vh_ocean_to_ice(const vh_domain* d, const vh_settings* s, const vh_fields* f,
+                    const void* u_surface, const void* v_surface, const void* temp_surface,
+                    const void* salt_surface, const void* forc_temp_surface,
+                    const vh_ocean_to_ice_out* out, void* stream);
+
+/* Outputs of vh_ice_to_ocean. */
+typedef struct vh_ice_to_ocean_out {
+  void *surface_taux, *surface_tauy, *forc_temp_surface, *forc_salt_surface;
+} vh_ice_to_ocean_out;
+
+/* The fluxes handed back to the ocean after a step
+ * (veris/setup/seaice_global_4deg/seaice_global_4deg.py:933-940): wind stress
+ * blended with f->OceanStressU/V by f->AreaW/AreaS, forc_temp_surface =
+ * -f->Qnet / (cpWater * rhoSea), forc_salt_surface = ice salt flux + the
+ * restoring term. */
+int vh_ice_to_ocean(const vh_domain* d, const vh_settings* s, const vh_fields* f,
+                    const void* surface_taux, const void* surface_tauy,
+                    const void* forc_salt_surface_ice, const void* forc_salt_surface_res,
+                    const vh_ice_to_ocean_out* out, void* stream);
+
 /* OceanStressUV (veris/ocean_stress.py:8-40), overlap-filled. */
 int vh_ocean_stress_uv(const vh_domain* d, const vh_settings* s, const vh_fields* f,
                        void* OceanStressU, void* OceanStressV, void* stream);

@@ -706,3 +706,31 @@ def seaice_step(state, qnet_forcing=None, swdown=None, n_substeps=None):
     for name in STEP_FILLS:
         vs[name] = fill_overlap(state, vs[name])
     return state
+
+
+# --------------------------------------------------------------------------
+# exchange with the host ocean (the setup's inline updates)
+# --------------------------------------------------------------------------
+
+
+def ocean_to_ice(state, u_surface, v_surface, temp_surface, salt_surface, forc_temp_surface):
+    """``seaice_global_4deg.py:847, 868-876`` -> (uOcean, vOcean, theta,
+    ocSalt, Qnet, Qsw, wSpeed); wSpeed from the wind before its overlap fill."""
+    vs, s = _vs(state)
+    theta = temp_surface + s.celsius2K
+    Qnet = -forc_temp_surface * s.cpWater * s.rhoSea
+    Qsw = -vs.SWdown
+    wSpeed = np.sqrt(vs.uWind**2 + vs.vWind**2)
+    return (np.array(u_surface, dtype=np.float64), np.array(v_surface, dtype=np.float64), theta,
+            np.array(salt_surface, dtype=np.float64), Qnet, Qsw, wSpeed)
+
+
+def ice_to_ocean(state, surface_taux, surface_tauy, forc_salt_surface_ice, forc_salt_surface_res):
+    """``seaice_global_4deg.py:935-940`` -> (surface_taux, surface_tauy,
+    forc_temp_surface, forc_salt_surface)."""
+    vs, s = _vs(state)
+    taux = surface_taux * (1 - vs.AreaW) + vs.OceanStressU * vs.AreaW
+    tauy = surface_tauy * (1 - vs.AreaS) + vs.OceanStressV * vs.AreaS
+    forc_temp = -vs.Qnet / (s.cpWater * s.rhoSea)
+    forc_salt = forc_salt_surface_ice + forc_salt_surface_res
+    return taux, tauy, forc_temp, forc_salt

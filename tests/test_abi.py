@@ -94,6 +94,9 @@ def test_adjoint_and_batch_structs_match_header():
         f[0] for f in N.PreEvpAdjointOut._fields_]
     assert _struct_members(text, "vh_batch") == [f[0] for f in N.Batch._fields_]
     assert _struct_members(text, "vh_member") == [f[0] for f in N.Member._fields_]
+    assert _struct_members(text, "vh_pre_evp_out") == [f[0] for f in N.PreEvpOut._fields_]
+    assert _struct_members(text, "vh_ocean_to_ice_out") == [f[0] for f in N.OceanToIceOut._fields_]
+    assert _struct_members(text, "vh_ice_to_ocean_out") == [f[0] for f in N.IceToOceanOut._fields_]
 
 
 def test_adjoint_host_side_validation():

@@ -58,6 +58,50 @@ def test_pre_evp_kernels(cuda):
                  O.SeaIceStrength(st, pStar=pstar), 1e-15)
 
 
+def test_fused_pre_evp_equals_separate_kernels(cuda):
+    """vh_pre_evp: every output equals the separate drop-in bit for bit (the
+    same device exp in the strength), with and without a P* field."""
+    st = _spun_up(synth.config0(), cuda)
+    g = st.to(cuda)
+    sep = (*K.SeaIceMass(g), *K.AreaWS(g), *K.WindForcingXY(g), K.SeaIceStrength(g))
+    names = ("SeaIceMassC", "SeaIceMassU", "SeaIceMassV", "AreaW", "AreaS", "WindForcingX",
+             "WindForcingY", "SeaIceStrength")
+    for name, got, want in zip(names, K.pre_evp(g), sep):
+        assert_exact(f"fused {name}", g2n(got), g2n(want))
+    pstar = torch.from_numpy(np.random.default_rng(5).lognormal(np.log(27500.0), 0.3,
+                                                                st.shape)).to(cuda)
+    assert_exact("fused SeaIceStrength(P* field)", g2n(K.pre_evp(g, pstar)[-1]),
+                 g2n(K.SeaIceStrength(g, pstar)))
+
+
+def test_ocean_coupling(cuda):
+    """vh_ocean_to_ice / vh_ice_to_ocean against the oracle, bit for bit."""
+    from veris_amd import coupling
+
+    st = _spun_up(synth.config0(), cuda)
+    vs = st.variables
+    rng = np.random.default_rng(9)
+    shp = st.shape
+    u, v = rng.normal(0, 0.1, shp), rng.normal(0, 0.1, shp)
+    t, sal = rng.normal(5, 8, shp), rng.normal(34.5, 1, shp)
+    ft = rng.normal(0, 2e-5, shp)
+    vs.OceanStressU, vs.OceanStressV = O.OceanStressUV(st)
+    g = st.to(cuda)
+    dev = lambda a: torch.from_numpy(a).to(cuda)
+    got = coupling.ocean_to_ice(g, dev(u), dev(v), dev(t), dev(sal), dev(ft))
+    want = O.ocean_to_ice(st, u, v, t, sal, ft)
+    for name, a, b in zip(("uOcean", "vOcean", "theta", "ocSalt", "Qnet", "Qsw", "wSpeed"),
+                          got, want):
+        assert_exact(name, g2n(a), b)
+    tx, ty = rng.normal(0, 0.1, shp), rng.normal(0, 0.1, shp)
+    fi, fr = rng.normal(0, 1e-6, shp), rng.normal(0, 1e-6, shp)
+    got = coupling.ice_to_ocean(g, dev(tx), dev(ty), dev(fi), dev(fr))
+    want = O.ice_to_ocean(st, tx, ty, fi, fr)
+    for name, a, b in zip(("surface_taux", "surface_tauy", "forc_temp_surface",
+                           "forc_salt_surface"), got, want):
+        assert_exact(name, g2n(a), b)
+
+
 @pytest.mark.parametrize("n", [1, 2, 3, 40])
 def test_evp_substeps_bit_exact(cuda, n):
     st = _spun_up(synth.config0(), cuda)

@@ -853,3 +853,41 @@ def test_ocean_stress_face_average_of_drag():
     cC = np.maximum(s.waterIceDrag * s.rhoSea * np.hypot(uc, vc), s.cDragMin)
     close(tu[INNER], (0.5 * (cC + np.roll(cC, 1, 0)) * u)[INNER], 1e-13)
     close(tv[INNER], (0.5 * (cC + np.roll(cC, 1, 1)) * v)[INNER], 1e-13)
+
+
+def test_ocean_coupling_round_trip_and_blending():
+    """Heat forcing -> Qnet -> heat forcing is the identity (to rounding);
+    the stress blend is the wind stress where AreaW = 0, the ice-ocean
+    stress where AreaW = 1 and their mean at 1/2; theta is in kelvin;
+    wSpeed is the wind's magnitude."""
+    st = uniform_state(nx=8, ny=6)
+    vs, s = st.variables, st.settings
+    rng = np.random.default_rng(3)
+    shape = st.shape
+    ft = rng.normal(0, 1e-5, shape)
+    u, v, t, sal = (rng.normal(0, 0.1, shape), rng.normal(0, 0.1, shape),
+                    rng.normal(5, 5, shape), rng.normal(34, 1, shape))
+    vs.uWind, vs.vWind = np.full(shape, 3.0), np.full(shape, -4.0)
+    vs.SWdown = rng.uniform(0, 300, shape)
+    uo, vo, theta, salt, Qnet, Qsw, ws = O.ocean_to_ice(st, u, v, t, sal, ft)
+    assert np.array_equal(uo, u) and np.array_equal(vo, v) and np.array_equal(salt, sal)
+    close(theta, t + 273.15, 1e-15)
+    close(Qsw, -vs.SWdown, 0)
+    close(ws, np.full(shape, 5.0), 1e-15)
+    vs.Qnet = Qnet
+    vs.AreaW = np.zeros(shape)
+    vs.AreaW[:, ::2] = 1.0
+    vs.AreaW[0] = 0.5
+    vs.AreaS = 1 - vs.AreaW
+    vs.OceanStressU = rng.normal(0, 0.1, shape)
+    vs.OceanStressV = rng.normal(0, 0.1, shape)
+    tx, ty = rng.normal(0, 0.1, shape), rng.normal(0, 0.1, shape)
+    fi, fr = rng.normal(0, 1e-6, shape), rng.normal(0, 1e-6, shape)
+    txo, tyo, fto, fso = O.ice_to_ocean(st, tx, ty, fi, fr)
+    close(fto, ft, 1e-15)  # round trip of the heat forcing
+    w = vs.AreaW
+    close(txo[w == 0], tx[w == 0], 0)
+    close(txo[w == 1], vs.OceanStressU[w == 1], 0)
+    close(txo[w == 0.5], 0.5 * (tx + vs.OceanStressU)[w == 0.5], 1e-15)
+    close(tyo[vs.AreaS == 1], vs.OceanStressV[vs.AreaS == 1], 0)
+    close(fso, fi + fr, 1e-15)

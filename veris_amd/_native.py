@@ -73,6 +73,28 @@ class PreEvpAdjointOut(ctypes.Structure):
     _fields_ = [("adj_" + name, ctypes.c_void_p) for name in PRE_EVP_ADJOINT_NAMES]
 
 
+PRE_EVP_NAMES = ("SeaIceMassC", "SeaIceMassU", "SeaIceMassV", "AreaW", "AreaS", "WindForcingX",
+                 "WindForcingY", "SeaIceStrength")
+
+
+class PreEvpOut(ctypes.Structure):
+    _fields_ = [(name, ctypes.c_void_p) for name in PRE_EVP_NAMES]
+
+
+OCEAN_TO_ICE_NAMES = ("uOcean", "vOcean", "theta", "ocSalt", "Qnet", "Qsw", "wSpeed")
+
+
+class OceanToIceOut(ctypes.Structure):
+    _fields_ = [(name, ctypes.c_void_p) for name in OCEAN_TO_ICE_NAMES]
+
+
+ICE_TO_OCEAN_NAMES = ("surface_taux", "surface_tauy", "forc_temp_surface", "forc_salt_surface")
+
+
+class IceToOceanOut(ctypes.Structure):
+    _fields_ = [(name, ctypes.c_void_p) for name in ICE_TO_OCEAN_NAMES]
+
+
 class Member(ctypes.Structure):
     _fields_ = [("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("cyclic_x", ctypes.c_int32),
                 ("fill_mode", ctypes.c_int32), ("offset", ctypes.c_int64)]
@@ -112,6 +134,9 @@ SIGNATURES = {
                        ctypes.POINTER(EvpAdjointOut), _P, _SZ, _P],
     "vh_pre_evp_adjoint": [_DP, _SP, _FP, _P, ctypes.POINTER(EvpAdjointOut),
                            ctypes.POINTER(PreEvpAdjointOut), _P],
+    "vh_pre_evp": [_DP, _SP, _FP, _P, ctypes.POINTER(PreEvpOut), _P],
+    "vh_ocean_to_ice": [_DP, _SP, _FP, _P, _P, _P, _P, _P, ctypes.POINTER(OceanToIceOut), _P],
+    "vh_ice_to_ocean": [_DP, _SP, _FP, _P, _P, _P, _P, ctypes.POINTER(IceToOceanOut), _P],
     "vh_ocean_stress_uv": [_DP, _SP, _FP, _P, _P, _P],
     "vh_advection_workspace_bytes": [_DP, ctypes.POINTER(_SZ)],
     "vh_advection": [_DP, _SP, _FP, _P, _P, _P, _P, _SZ, _P],

@@ -393,6 +393,164 @@ int gsum(const Geom& g, const void* x, const void* w, double* out, void* ws, siz
   return VH_OK;
 }
 
+// ---------------------------------------------------------------- fused pre-EVP
+
+// SeaIceMass, AreaWS, WindForcingXY and SeaIceStrength in one sweep; every
+// output is the same expression, in the same order, as the separate kernels
+// (masses and areas of the neighbours are recomputed instead of re-read).
+template <typename T>
+struct PreEvpIO {
+  const T *h, *hs, *A, *M, *psf;
+  T ps, cStar, rhoI, rhoS;
+  T *mC, *mU, *mV, *AW, *AS, *WX, *WY, *P;
+};
+
+template <typename T>
+__global__ __launch_bounds__(BLK) void pre_evp_kernel(Geom g, WindArgs<T> w, PreEvpIO<T> o) {
+  CELL_PROLOGUE
+  const T c = o.rhoI * o.h[idx] + o.rhoS * o.hs[idx];
+  const T cim = o.rhoI * o.h[AT(im, j)] + o.rhoS * o.hs[AT(im, j)];
+  const T cjm = o.rhoI * o.h[AT(i, jm)] + o.rhoS * o.hs[AT(i, jm)];
+  const T mU = (T)0.5 * (c + cim);
+  const T mV = (T)0.5 * (c + cjm);
+  const T a = o.A[idx];
+  const T AW = (T)0.5 * (a + o.A[AT(im, j)]);
+  const T AS = (T)0.5 * (a + o.A[AT(i, jm)]);
+  T tx, ty, txim, tyim, txjm, tyjm;
+  tau_c(g, w, i, j, &tx, &ty);
+  tau_c(g, w, im, j, &txim, &tyim);
+  tau_c(g, w, i, jm, &txjm, &tyjm);
+  const T tauX = ((T)0.5 * (tx + txim)) * w.MU[idx];
+  const T tauY = ((T)0.5 * (ty + tyjm)) * w.MV[idx];
+  const T phi = phi_at(w, idx);
+  T fx = tauX * AW;
+  T fy = tauY * AS;
+  fx = fx - (mU * w.rdxC[idx]) * (phi - phi_at(w, AT(im, j)));
+  fy = fy - (mV * w.rdyC[idx]) * (phi - phi_at(w, AT(i, jm)));
+  const T pstar = o.psf ? o.psf[idx] : o.ps;
+  o.mC[idx] = c;
+  o.mU[idx] = mU;
+  o.mV[idx] = mV;
+  o.AW[idx] = AW;
+  o.AS[idx] = AS;
+  o.WX[idx] = fx;
+  o.WY[idx] = fy;
+  o.P[idx] = ((pstar * o.h[idx]) * exp((-o.cStar) * ((T)1 - a))) * o.M[idx];
+}
+
+template <typename T>
+int pre_evp(const Geom& g, const vh_settings* s, const vh_fields* f, const void* psf,
+            const vh_pre_evp_out* out, hipStream_t st) {
+  VH_REQUIRE(out != nullptr, VH_EINVAL, "pre_evp: null output struct");
+  WindArgs<T> w;
+  w.u = C_<T>(f->uIce);
+  w.v = C_<T>(f->vIce);
+  w.uW = C_<T>(f->uWind);
+  w.vW = C_<T>(f->vWind);
+  w.fC = C_<T>(f->fCori);
+  w.MU = C_<T>(f->iceMaskU);
+  w.MV = C_<T>(f->iceMaskV);
+  w.AW = nullptr;
+  w.AS = nullptr;
+  w.mU = nullptr;
+  w.mV = nullptr;
+  w.rdxC = C_<T>(f->recip_dxC);
+  w.rdyC = C_<T>(f->recip_dyC);
+  w.ssh = C_<T>(f->ssh_an);
+  w.sp = C_<T>(f->surfPress);
+  w.load = C_<T>(f->SeaIceLoad);
+  w.relWind = s->useRelativeWind != 0.0;
+  w.realFW = s->useRealFreshWaterFlux != 0.0;
+  PreEvpIO<T> o;
+  o.h = C_<T>(f->hIceMean);
+  o.hs = C_<T>(f->hSnowMean);
+  o.A = C_<T>(f->Area);
+  o.M = C_<T>(f->iceMask);
+  o.psf = C_<T>(psf);
+  o.mC = (T*)out->SeaIceMassC;
+  o.mU = (T*)out->SeaIceMassU;
+  o.mV = (T*)out->SeaIceMassV;
+  o.AW = (T*)out->AreaW;
+  o.AS = (T*)out->AreaS;
+  o.WX = (T*)out->WindForcingX;
+  o.WY = (T*)out->WindForcingY;
+  o.P = (T*)out->SeaIceStrength;
+  const void* req[] = {w.uW, w.vW, w.fC, w.MU, w.MV, w.rdxC, w.rdyC, w.ssh, w.sp, o.h, o.hs, o.A,
+                       o.M, o.mC, o.mU, o.mV, o.AW, o.AS, o.WX, o.WY, o.P};
+  for (const void* p : req) VH_REQUIRE(p != nullptr, VH_EINVAL, "pre_evp: null pointer");
+  if (w.relWind) VH_REQUIRE(w.u && w.v, VH_EINVAL, "pre_evp: null uIce/vIce");
+  if (w.realFW) VH_REQUIRE(w.load, VH_EINVAL, "pre_evp: null SeaIceLoad");
+  T* outs[] = {o.mC, o.mU, o.mV, o.AW, o.AS, o.WX, o.WY, o.P};
+  const void* ins[] = {w.u, w.v, w.uW, w.vW, w.fC, w.MU, w.MV, w.rdxC, w.rdyC, w.ssh, w.sp, w.load,
+                       o.h, o.hs, o.A, o.M, o.psf};
+  for (int q = 0; q < 8; ++q) {
+    for (int r = 0; r < q; ++r) VH_REQUIRE(outs[q] != outs[r], VH_EINVAL, "pre_evp: outputs alias");
+    for (const void* p : ins) VH_REQUIRE(outs[q] != p, VH_EINVAL, "pre_evp: output aliases an input");
+  }
+  const double ang = s->airTurnAngle * (M_PI / 180.0);
+  w.sinWin = (T)sin(ang);
+  w.cosWin = (T)cos(ang);
+  w.wmin = (T)s->wSpeedMin;
+  w.wmin2 = (T)(s->wSpeedMin * s->wSpeedMin);
+  w.caN = (T)s->airIceDrag;
+  w.caS = (T)s->airIceDrag_south;
+  w.rhoAir = (T)s->rhoAir;
+  w.grav = (T)s->gravity;
+  w.rrho = (T)s->recip_rhoSea;
+  w.loadFac = (T)s->seaIceLoadFac;
+  o.ps = (T)s->pStar;
+  o.cStar = (T)s->cStar;
+  o.rhoI = (T)s->rhoIce;
+  o.rhoS = (T)s->rhoSnow;
+  hipLaunchKernelGGL(pre_evp_kernel<T>, cell_grid(g, BLK), dim3(BLK), 0, st, g, w, o);
+  VH_TRY_HIP(hipGetLastError());
+  return VH_OK;
+}
+
+// ---------------------------------------------------------------- ocean coupling
+
+// Fields the host ocean hands to the sea ice before a step
+// (veris/setup/seaice_global_4deg/seaice_global_4deg.py:847, 868-876); wSpeed is formed
+// from uWind/vWind before their overlap fill (:865), as there.
+template <typename T>
+struct OceanToIce {
+  const T *uS, *vS, *tS, *sS, *ftemp, *SWdown, *uW, *vW;
+  T *uO, *vO, *theta, *salt, *Qnet, *Qsw, *wSpeed;
+  T c2K, cp, rho;
+};
+
+template <typename T>
+__global__ __launch_bounds__(BLK) void ocean_to_ice_kernel(Geom g, OceanToIce<T> a) {
+  CELL_PROLOGUE
+  a.uO[idx] = a.uS[idx];
+  a.vO[idx] = a.vS[idx];
+  a.theta[idx] = a.tS[idx] + a.c2K;
+  a.salt[idx] = a.sS[idx];
+  a.Qnet[idx] = ((-a.ftemp[idx]) * a.cp) * a.rho;
+  a.Qsw[idx] = -a.SWdown[idx];
+  const T u = a.uW[idx], v = a.vW[idx];
+  a.wSpeed[idx] = sqrt(u * u + v * v);
+}
+
+// Fluxes the sea ice hands back to the ocean after a step
+// (veris/setup/seaice_global_4deg/seaice_global_4deg.py:933-940).
+template <typename T>
+struct IceToOcean {
+  const T *tx, *ty, *AW, *AS, *OSU, *OSV, *Qnet, *fssi, *fssr;
+  T *txo, *tyo, *ftemp, *fsalt;
+  T cpRho;
+};
+
+template <typename T>
+__global__ __launch_bounds__(BLK) void ice_to_ocean_kernel(Geom g, IceToOcean<T> a) {
+  CELL_PROLOGUE
+  const T aw = a.AW[idx], as = a.AS[idx];
+  a.txo[idx] = a.tx[idx] * ((T)1 - aw) + a.OSU[idx] * aw;
+  a.tyo[idx] = a.ty[idx] * ((T)1 - as) + a.OSV[idx] * as;
+  a.ftemp[idx] = (-a.Qnet[idx]) / a.cpRho;
+  a.fsalt[idx] = a.fssi[idx] + a.fssr[idx];
+}
+
 }  // namespace
 }  // namespace vh
 
@@ -479,4 +637,101 @@ extern "C" int vh_global_sum_f64(const vh_domain* d, const void* x, const void* 
   hipStream_t st = (hipStream_t)stream;
   return VH_DISPATCH(d, vh::gsum<double>(g, x, w, out, ws, wsb, st),
                      vh::gsum<float>(g, x, w, out, ws, wsb, st));
+}
+
+extern "C" int vh_pre_evp(const vh_domain* d, const vh_settings* s, const vh_fields* f,
+                          const void* pStarField, const vh_pre_evp_out* out, void* stream) {
+  VH_PRE(d, s, f)
+  return VH_DISPATCH(d, vh::pre_evp<double>(g, s, f, pStarField, out, st),
+                     vh::pre_evp<float>(g, s, f, pStarField, out, st));
+}
+
+namespace {
+
+template <typename T>
+int ocean_to_ice_t(const vh::Geom& g, const vh_settings* s, const vh_fields* f, const void* u,
+                   const void* v, const void* temp, const void* salt, const void* ftemp,
+                   const vh_ocean_to_ice_out* o, hipStream_t st) {
+  vh::OceanToIce<T> a;
+  a.uS = (const T*)u;
+  a.vS = (const T*)v;
+  a.tS = (const T*)temp;
+  a.sS = (const T*)salt;
+  a.ftemp = (const T*)ftemp;
+  a.SWdown = (const T*)f->SWdown;
+  a.uW = (const T*)f->uWind;
+  a.vW = (const T*)f->vWind;
+  a.uO = (T*)o->uOcean;
+  a.vO = (T*)o->vOcean;
+  a.theta = (T*)o->theta;
+  a.salt = (T*)o->ocSalt;
+  a.Qnet = (T*)o->Qnet;
+  a.Qsw = (T*)o->Qsw;
+  a.wSpeed = (T*)o->wSpeed;
+  const void* req[] = {a.uS, a.vS, a.tS, a.sS, a.ftemp, a.SWdown, a.uW, a.vW, a.uO, a.vO,
+                       a.theta, a.salt, a.Qnet, a.Qsw, a.wSpeed};
+  for (const void* p : req) VH_REQUIRE(p != nullptr, VH_EINVAL, "ocean_to_ice: null pointer");
+  a.c2K = (T)s->celsius2K;
+  a.cp = (T)s->cpWater;
+  a.rho = (T)s->rhoSea;
+  hipLaunchKernelGGL(vh::ocean_to_ice_kernel<T>, vh::cell_grid(g, 256), dim3(256), 0, st, g, a);
+  VH_TRY_HIP(hipGetLastError());
+  return VH_OK;
+}
+
+template <typename T>
+int ice_to_ocean_t(const vh::Geom& g, const vh_settings* s, const vh_fields* f, const void* taux,
+                   const void* tauy, const void* fsi, const void* fsr, const vh_ice_to_ocean_out* o,
+                   hipStream_t st) {
+  vh::IceToOcean<T> a;
+  a.tx = (const T*)taux;
+  a.ty = (const T*)tauy;
+  a.AW = (const T*)f->AreaW;
+  a.AS = (const T*)f->AreaS;
+  a.OSU = (const T*)f->OceanStressU;
+  a.OSV = (const T*)f->OceanStressV;
+  a.Qnet = (const T*)f->Qnet;
+  a.fssi = (const T*)fsi;
+  a.fssr = (const T*)fsr;
+  a.txo = (T*)o->surface_taux;
+  a.tyo = (T*)o->surface_tauy;
+  a.ftemp = (T*)o->forc_temp_surface;
+  a.fsalt = (T*)o->forc_salt_surface;
+  const void* req[] = {a.tx, a.ty, a.AW, a.AS, a.OSU, a.OSV, a.Qnet, a.fssi, a.fssr, a.txo, a.tyo,
+                       a.ftemp, a.fsalt};
+  for (const void* p : req) VH_REQUIRE(p != nullptr, VH_EINVAL, "ice_to_ocean: null pointer");
+  a.cpRho = (T)(s->cpWater * s->rhoSea);
+  hipLaunchKernelGGL(vh::ice_to_ocean_kernel<T>, vh::cell_grid(g, 256), dim3(256), 0, st, g, a);
+  VH_TRY_HIP(hipGetLastError());
+  return VH_OK;
+}
+
+}  // namespace
+
+extern "C" int vh_ocean_to_ice(const vh_domain* d, const vh_settings* s, const vh_fields* f,
+                               const void* u_surface, const void* v_surface,
+                               const void* temp_surface, const void* salt_surface,
+                               const void* forc_temp_surface, const vh_ocean_to_ice_out* out,
+                               void* stream) {
+  VH_PRE(d, s, f)
+  VH_REQUIRE(out != nullptr, VH_EINVAL, "ocean_to_ice: null output struct");
+  return VH_DISPATCH(d,
+                     ocean_to_ice_t<double>(g, s, f, u_surface, v_surface, temp_surface,
+                                            salt_surface, forc_temp_surface, out, st),
+                     ocean_to_ice_t<float>(g, s, f, u_surface, v_surface, temp_surface,
+                                           salt_surface, forc_temp_surface, out, st));
+}
+
+extern "C" int vh_ice_to_ocean(const vh_domain* d, const vh_settings* s, const vh_fields* f,
+                               const void* surface_taux, const void* surface_tauy,
+                               const void* forc_salt_surface_ice,
+                               const void* forc_salt_surface_res, const vh_ice_to_ocean_out* out,
+                               void* stream) {
+  VH_PRE(d, s, f)
+  VH_REQUIRE(out != nullptr, VH_EINVAL, "ice_to_ocean: null output struct");
+  return VH_DISPATCH(d,
+                     ice_to_ocean_t<double>(g, s, f, surface_taux, surface_tauy,
+                                            forc_salt_surface_ice, forc_salt_surface_res, out, st),
+                     ice_to_ocean_t<float>(g, s, f, surface_taux, surface_tauy,
+                                           forc_salt_surface_ice, forc_salt_surface_res, out, st));
 }

@@ -142,6 +142,22 @@ def WindForcingXY(state):
     return tuple(out)
 
 
+def pre_evp(state, pStar: torch.Tensor | None = None):
+    """SeaIceMass, AreaWS, WindForcingXY and SeaIceStrength in one fused launch
+    (veris/area_mass.py:5-28, veris/dynsolver.py:7-84,
+    veris/dynamics_routines.py:5-18) -> (SeaIceMassC, SeaIceMassU, SeaIceMassV,
+    AreaW, AreaS, WindForcingX, WindForcingY, SeaIceStrength); results equal
+    the four separate drop-ins, in that order of the reference setup."""
+    c = Ctx(state)
+    if pStar is not None:
+        c._check("pStar", pStar)
+    buf = torch.empty((len(N.PRE_EVP_NAMES),) + c.shape, dtype=c.dtype, device=c.device)
+    out = buf.unbind(0)
+    o = N.PreEvpOut(*[t.data_ptr() for t in out])
+    N.call("vh_pre_evp", *c.args(), _p(pStar), ctypes.byref(o), c.stream)
+    return tuple(out)
+
+
 # EVP tile shape index used when a call does not name one (0 = library default);
 # set by veris_amd.autotune.
 EVP_TILE = {}

@@ -24,10 +24,9 @@ def seaice_step(state, qnet_forcing=None, swdown=None, n_substeps=None):
         vs.Qnet = qnet_forcing.clone()
     if swdown is not None:
         vs.Qsw = -swdown
-    vs.SeaIceMassC, vs.SeaIceMassU, vs.SeaIceMassV = K.SeaIceMass(state)
-    vs.AreaW, vs.AreaS = K.AreaWS(state)
-    vs.WindForcingX, vs.WindForcingY = K.WindForcingXY(state)
-    vs.SeaIceStrength = K.SeaIceStrength(state)
+    # SeaIceMass, AreaWS, WindForcingXY, SeaIceStrength in one fused launch
+    (vs.SeaIceMassC, vs.SeaIceMassU, vs.SeaIceMassV, vs.AreaW, vs.AreaS, vs.WindForcingX,
+     vs.WindForcingY, vs.SeaIceStrength) = K.pre_evp(state)
     vs.uIce, vs.vIce, vs.sigma1, vs.sigma2, vs.sigma12 = K.IceVelocities(state, n_substeps)
     vs.OceanStressU, vs.OceanStressV = K.OceanStressUV(state)
     vs.hIceMean, vs.hSnowMean, vs.Area = K.Advection(state)
