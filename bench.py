@@ -389,6 +389,9 @@ def main(argv=None) -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the batched, adjoint and 0.25-degree measurements")
+    ap.add_argument("--only", choices=("batched", "adjoint", "hires"),
+                    help="run one of the extra workloads alone and print its JSON object "
+                         "(for profiling)")
     ap.add_argument("--calibration-copy", action="store_true",
                     help="first copy a 64 MiB buffer 5 times (a kernel of known HBM traffic, "
                          "the reference for rocprofv3 FETCH_SIZE / WRITE_SIZE counts)")
@@ -405,6 +408,15 @@ def main(argv=None) -> int:
     from veris_amd.step import seaice_step
 
     dev = torch.device("cuda:0")
+    if args.only:
+        if args.only == "batched":
+            res = bench_batched(dev, args.substeps, float("nan"), float("nan"))
+        elif args.only == "adjoint":
+            res = bench_adjoint(dev)
+        else:
+            res = bench_hires(dev)
+        print(json.dumps({args.only: res}), flush=True)
+        return 0
     if args.calibration_copy:
         src = torch.ones(8 << 20, dtype=torch.float64, device=dev)
         dst = torch.empty_like(src)
