@@ -266,23 +266,38 @@ def stressdiv(state, s11, s22, s12):
 # --------------------------------------------------------------------------
 
 
+def evp_cfac(state):
+    """``veris/evp_solver.py:266-271``: factor of the adaptive relaxation parameter."""
+    _, s = _vs(state)
+    if not s.useAdaptiveEVP:
+        return 0
+    return s.deltatDyn * 4 * (np.pi * s.aEvpCoeff) ** 2
+
+
 def evp_substep(state, u, v, uNm1, vNm1, s1, s2, s12, P=None, drag=None, trace=None):
-    """One pass of ``veris/evp_solver.py:22-254`` at non-adaptive settings.
+    """One pass of ``veris/evp_solver.py:22-254``.
 
     Returns the new (u, v, sigma1, sigma2, sigma12).  ``drag`` overrides the
     water drag scalars; ``trace`` (a dict) receives intermediates for tests.
+    With useAdaptiveEVP the relaxation parameters follow ``:65-73`` (c-points),
+    ``:97-100`` (z-points) and ``:148-150`` (u/v points).
     """
     vs, s = _vs(state)
-    if s.useAdaptiveEVP:
-        raise NotImplementedError("useAdaptiveEVP")
     if not s.noSlip:
         raise NotImplementedError("side drag (noSlip=False)")
     drag = drag or {}
     e11, e22, e12 = strainrates(state, u, v)
     zeta, _eta, press, dC = viscosities(state, e11, e22, e12, P=P)
-    alpha = s.evpAlpha
-    den1 = 1 / s.evpAlpha
-    den2 = den1
+    if s.useAdaptiveEVP:
+        alphaC = np.sqrt(zeta * evp_cfac(state) / np.maximum(vs.SeaIceMassC, 1e-4)
+                         * vs.recip_rA) * vs.iceMask
+        alphaC = np.maximum(alphaC, s.aEVPalphaMin)
+        den1 = 1.0 / alphaC
+        den2 = den1
+    else:
+        alphaC = s.evpAlpha
+        den1 = 1 / s.evpAlpha
+        den2 = den1
     rev = 1
     rrev = 0.25
     ep = e11 + e22
@@ -290,11 +305,17 @@ def evp_substep(state, u, v, uNm1, vNm1, s1, s2, s12, P=None, drag=None, trace=N
     divergence = 2 * zeta * ep - press
     tension = 2 * zeta * em
     shear = 2 * c_point_to_z_point(state, zeta) * e12
-    s1 = (s1 * (alpha - rev) + divergence) * den1 * vs.iceMask
-    s2 = (s2 * (alpha - rev) + tension * rrev) * den2 * vs.iceMask
+    s1 = (s1 * (alphaC - rev) + divergence) * den1 * vs.iceMask
+    s2 = (s2 * (alphaC - rev) + tension * rrev) * den2 * vs.iceMask
     s11 = 0.5 * (s1 + s2)
     s22 = 0.5 * (s1 - s2)
-    s12 = (s12 * (alpha - rev) + shear * rrev) * den2
+    if s.useAdaptiveEVP:
+        alphaZ = 0.5 * (alphaC + jm1(alphaC))
+        alphaZ = 0.5 * (alphaZ + im1(alphaZ))
+        den2 = 1.0 / alphaZ
+    else:
+        alphaZ = s.evpAlpha
+    s12 = (s12 * (alphaZ - rev) + shear * rrev) * den2
     divx, divy = stressdiv(state, s11, s22, s12)
     cd = ocean_drag_coeffs(state, u, v, **drag)
     cb = basal_drag_coeffs(state, u, v)
@@ -319,10 +340,14 @@ def evp_substep(state, u, v, uNm1, vNm1, s1, s2, s12, P=None, drag=None, trace=N
     fu = vs.SeaIceMassC * vs.fCori * 0.5 * (u + ip1(u))
     fx = fx + 0.5 * (fv + im1(fv))
     fy = fy - 0.5 * (fu + jm1(fu))
-    betaU = s.evpBeta * s.recip_deltatDyn
-    betaV = betaU
+    if s.useAdaptiveEVP:
+        betaU = 0.5 * (alphaC + im1(alphaC)) * s.recip_deltatDyn
+        betaV = 0.5 * (alphaC + jm1(alphaC)) * s.recip_deltatDyn
+    else:
+        betaU = s.evpBeta * s.recip_deltatDyn
+        betaV = betaU
     bp1U = betaU + s.recip_deltatDyn
-    bp1V = bp1U
+    bp1V = betaV + s.recip_deltatDyn
     denU = vs.SeaIceMassU * bp1U + vs.AreaW * (
         0.5 * (cd + im1(cd)) * s.cosWat + 0.5 * (cb + im1(cb)))
     denV = vs.SeaIceMassV * bp1V + vs.AreaS * (
@@ -336,7 +361,8 @@ def evp_substep(state, u, v, uNm1, vNm1, s1, s2, s12, P=None, drag=None, trace=N
     un, vn = fill_overlap_uv(state, un, vn)
     if trace is not None:
         trace.update(e11=e11, e22=e22, e12=e12, zeta=zeta, press=press, deltaC=dC,
-                     cDrag=cd, cBot=cb, divX=divx, divY=divy, ForcingX=fx, ForcingY=fy)
+                     cDrag=cd, cBot=cb, divX=divx, divY=divy, ForcingX=fx, ForcingY=fy,
+                     alphaC=alphaC, alphaZ=alphaZ, betaU=betaU, betaV=betaV)
     return un, vn, s1, s2, s12
 
 

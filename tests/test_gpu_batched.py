@@ -119,3 +119,23 @@ def test_batched_fp32_storage_fp64_arith_against_fp64_oracle(cuda):
         print(f"tile {tile}: " + ", ".join(f"{k} {v:.2e}" for k, v in worst.items()), flush=True)
         assert worst["uIce"] <= 1e-4 and worst["vIce"] <= 1e-4, worst
         assert max(worst["sigma1"], worst["sigma2"], worst["sigma12"]) <= 5e-3, worst
+
+
+@pytest.mark.parametrize("tile", [3, 9])
+def test_batched_adaptive_fp64_matches_oracle(cuda, tile):
+    """Adaptive EVP (per-cell alpha/beta) in the batched launch: every member
+    bit for bit against the oracle (closed regional members)."""
+    sizes = [(37, 29), (64, 23), (30, 61)]
+    states = [pre_evp(synth.regional(21, m, nx, ny, useAdaptiveEVP=True))
+              for m, (nx, ny) in enumerate(sizes)]
+    for st in states:
+        vs = st.variables
+        vs.uIce, vs.vIce, vs.sigma1, vs.sigma2, vs.sigma12 = O.evp_solver(st, n_substeps=5)
+        pre_evp(st)
+    ens = Ensemble(states, device=cuda, dtype=torch.float64)
+    out = ens.evp(17, tile=tile)
+    for m, st in enumerate(states):
+        want = O.evp_solver(st, n_substeps=17)
+        for q, name in enumerate(EVP_NAMES):
+            assert_exact(f"adaptive member {m} {name}[tile={tile}]", g2n(ens.view(out[q], m)),
+                         want[q])

@@ -126,10 +126,15 @@ def test_evp_config1_1deg(cuda):
 
 
 @pytest.mark.parametrize("variant", ["regional", "own_fill", "turning", "kterms", "tensile",
-                                     "no_repl_pressure", "small_domain"])
+                                     "no_repl_pressure", "small_domain", "adaptive",
+                                     "adaptive_regional"])
 def test_evp_setting_variants(cuda, variant):
     if variant == "regional":
         st = synth.regional(7, 0, 37, 29)
+    elif variant == "adaptive":
+        st = synth.tiny(20, 14, useAdaptiveEVP=True)
+    elif variant == "adaptive_regional":
+        st = synth.regional(7, 1, 37, 29, useAdaptiveEVP=True, aEvpCoeff=0.2, aEVPalphaMin=20.0)
     elif variant == "own_fill":
         st = synth.tiny(20, 14, veros_fill=False)
     elif variant == "turning":
@@ -279,10 +284,10 @@ def test_errors_are_reported(cuda):
     from veris_amd._native import VerisHipError
 
     st = pre_evp(synth.tiny(12, 10)).to(cuda)
-    with pytest.raises(VerisHipError, match="useAdaptiveEVP"):
-        st.settings.useAdaptiveEVP = True
+    with pytest.raises(VerisHipError, match="VH_EUNSUPPORTED.*noSlip"):
+        st.settings.noSlip = False
         K.evp_solver(st, 3)
-    st.settings.useAdaptiveEVP = False
+    st.settings.noSlip = True
     st.variables.uIce = st.variables.uIce[:, :-1].contiguous()
     with pytest.raises(ValueError):
         K.evp_solver(st, 3)
@@ -298,6 +303,27 @@ def _periodic_artefact_cols(st):
         return []
     NX = st.shape[0]
     return [0, 1, NX - 2, NX - 1]
+
+
+@pytest.mark.parametrize("tile", [1, 5, 8, 9, 13])
+def test_adaptive_evp_tile_shapes(cuda, tile):
+    """Adaptive EVP (per-cell alpha, beta) on single- and multi-substep
+    shapes: bit-exact against the oracle (stresses outside the periodic
+    artefact columns for k > 1 on the x-cyclic grid)."""
+    from veris_amd._native import evp_tile_shapes
+
+    k = evp_tile_shapes()[tile][2]
+    st = _spun_up(synth.config0(useAdaptiveEVP=True), cuda, n_spin=5)
+    g = st.to(cuda)
+    skip = _periodic_artefact_cols(st) if k > 1 else []
+    keep = np.setdiff1d(np.arange(st.shape[0]), skip)
+    want = O.evp_solver(st, n_substeps=21)
+    got = K.evp_solver(g, n_substeps=21, tile=tile)
+    for q, (name, a, b) in enumerate(zip(EVP_NAMES, got, want)):
+        a = g2n(a)
+        if q >= 2:
+            a, b = a[keep], b[keep]
+        assert_exact(f"adaptive {name}[tile={tile}]", a, b)
 
 
 @pytest.mark.parametrize("tile", ALL_TILES)

@@ -30,9 +30,15 @@ MUTATIONS = {
     "z-mask corners": ("mz = mz * jm1(mz)", "mz = mz * jp1(mz)"),
     "zeta factor": ("zeta = 0.5 * (P * (1 + s.tensileStrFac)) / dreg", "zeta = (P * (1 + s.tensileStrFac)) / dreg"),
     "replacement pressure": ("+ 2.0 * zeta * dC * s.pressReplFac / (1 + s.tensileStrFac))", "+ zeta * dC * s.pressReplFac / (1 + s.tensileStrFac))"),
-    "alpha relaxation": ("s1 = (s1 * (alpha - rev) + divergence) * den1 * vs.iceMask", "s1 = (s1 * alpha + divergence) * den1 * vs.iceMask"),
-    "tension 1/e^2": ("s2 = (s2 * (alpha - rev) + tension * rrev) * den2 * vs.iceMask", "s2 = (s2 * (alpha - rev) + tension) * den2 * vs.iceMask"),
-    "shear 1/e^2": ("s12 = (s12 * (alpha - rev) + shear * rrev) * den2", "s12 = (s12 * (alpha - rev) + shear) * den2"),
+    "alpha relaxation": ("s1 = (s1 * (alphaC - rev) + divergence) * den1 * vs.iceMask", "s1 = (s1 * alphaC + divergence) * den1 * vs.iceMask"),
+    "tension 1/e^2": ("s2 = (s2 * (alphaC - rev) + tension * rrev) * den2 * vs.iceMask", "s2 = (s2 * (alphaC - rev) + tension) * den2 * vs.iceMask"),
+    "shear 1/e^2": ("s12 = (s12 * (alphaZ - rev) + shear * rrev) * den2", "s12 = (s12 * (alphaZ - rev) + shear) * den2"),
+    "adaptive alpha z average": ("alphaZ = 0.5 * (alphaZ + im1(alphaZ))", "alphaZ = 0.5 * (alphaZ + ip1(alphaZ))"),
+    "adaptive z denominator": ("        den2 = 1.0 / alphaZ", "        den2 = 1.0 / alphaC"),
+    "adaptive beta u": ("betaU = 0.5 * (alphaC + im1(alphaC)) * s.recip_deltatDyn", "betaU = 0.5 * (alphaC + jm1(alphaC)) * s.recip_deltatDyn"),
+    "adaptive alpha mass": ("/ np.maximum(vs.SeaIceMassC, 1e-4)", "/ np.maximum(vs.SeaIceMassU, 1e-4)"),
+    "adaptive alpha floor": ("alphaC = np.maximum(alphaC, s.aEVPalphaMin)", "alphaC = np.maximum(alphaC, 1.0)"),
+    "adaptive c-factor": ("return s.deltatDyn * 4 * (np.pi * s.aEvpCoeff) ** 2", "return s.deltatDyn * 4 * (np.pi * s.aEvpCoeff)"),
     "z-point average": ("z = c + im1(c)", "z = c + ip1(c)"),
     "Coriolis sign": ("fx = fx + 0.5 * (fv + im1(fv))", "fx = fx - 0.5 * (fv + im1(fv))"),
     "Coriolis u average": ("fu = vs.SeaIceMassC * vs.fCori * 0.5 * (u + ip1(u))", "fu = vs.SeaIceMassC * vs.fCori * 0.5 * (u + im1(u))"),

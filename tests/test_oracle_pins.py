@@ -891,3 +891,83 @@ def test_ocean_coupling_round_trip_and_blending():
     close(txo[w == 0.5], 0.5 * (tx + vs.OceanStressU)[w == 0.5], 1e-15)
     close(tyo[vs.AreaS == 1], vs.OceanStressV[vs.AreaS == 1], 0)
     close(fso, fi + fr, 1e-15)
+
+
+def _random_closed_state(seed, **settings):
+    """Square closed domain with random land, ice and forcing (pre-EVP done)."""
+    st = uniform_state(nx=14, ny=14, dx=2e4, dy=2e4, cyclic=False, **settings)
+    vs = st.variables
+    r = np.random.default_rng(seed)
+    M = np.where(r.random(st.shape) < 0.2, 0.0, 1.0)
+    M[:2] = M[-2:] = 0.0
+    M[:, :2] = M[:, -2:] = 0.0
+    vs.iceMask = M
+    vs.maskInC = M
+    vs.iceMaskU = M * np.roll(M, 1, 0)
+    vs.iceMaskV = M * np.roll(M, 1, 1)
+    vs.maskInU, vs.maskInV = vs.iceMaskU, vs.iceMaskV
+    vs.hIceMean = r.uniform(0.5, 2.5, st.shape) * M
+    vs.hSnowMean = r.uniform(0, 0.3, st.shape) * M
+    vs.Area = r.uniform(0.3, 1.0, st.shape) * M
+    vs.uWind = r.normal(0, 6, st.shape)
+    vs.vWind = r.normal(0, 6, st.shape)
+    vs.uOcean = r.normal(0, 0.1, st.shape) * vs.iceMaskU
+    vs.vOcean = r.normal(0, 0.1, st.shape) * vs.iceMaskV
+    from gpu_helpers import pre_evp
+
+    return pre_evp(st)
+
+
+def test_adaptive_evp_at_its_floor_is_the_fixed_scheme():
+    """A negligible aEvpCoeff puts every adaptive alpha at aEVPalphaMin, so the
+    adaptive subcycle equals the fixed one with evpAlpha = evpBeta =
+    aEVPalphaMin bit for bit (the z- and u/v-point averages of a constant are
+    that constant)."""
+    a = _random_closed_state(17, useAdaptiveEVP=True, aEvpCoeff=1e-9, aEVPalphaMin=7.0)
+    f = _random_closed_state(17, evpAlpha=7.0, evpBeta=7.0)
+    for x, y in zip(O.evp_solver(a, n_substeps=12), O.evp_solver(f, n_substeps=12)):
+        np.testing.assert_array_equal(x, y)
+
+
+def test_adaptive_evp_relaxation_at_rest():
+    """At rest (u = v = 0: no strain, press = 0 with pressReplFac = 1) one
+    adaptive substep scales sigma1, sigma2 by 1 - 1/alphaC at c-points and
+    sigma12 by 1 - 1/alphaZ at z-points, where alphaC = max(sqrt(zeta c /
+    max(massC, 1e-4) / rA) M, alphaMin), zeta = P (1 + t) / (2 deltaMin), and
+    alphaZ is the mean of the four surrounding alphaC."""
+    st = _rest_state(useAdaptiveEVP=True, aEvpCoeff=0.5, aEVPalphaMin=5.0)
+    vs, s = st.variables, st.settings
+    r = np.random.default_rng(23)
+    P = r.uniform(2e3, 4e4, st.shape)
+    P[6:9, 2:5] = 0.0  # no strength: alpha at its floor
+    vs.SeaIceStrength = P
+    vs.SeaIceMassC = r.uniform(100.0, 3000.0, st.shape)
+    vs.SeaIceMassC[3, 4] = 0.0  # the 1e-4 floor of the mass
+    s1, s2, s12 = (r.normal(0, 1e3, st.shape) for _ in range(3))
+    vs.sigma1, vs.sigma2, vs.sigma12 = s1, s2, s12
+    u1, v1, a1, a2, a12 = O.evp_solver(st, n_substeps=1)
+    cfac = s.deltatDyn * 4 * (np.pi * s.aEvpCoeff) ** 2
+    zeta = P * (1 + s.tensileStrFac) / (2 * s.deltaMin)
+    aC = np.maximum(np.sqrt(zeta * cfac / np.maximum(vs.SeaIceMassC, 1e-4) / vs.rA), 5.0)
+    aZ = 0.25 * (aC + np.roll(aC, 1, 0) + np.roll(aC, 1, 1) + np.roll(np.roll(aC, 1, 0), 1, 1))
+    close(a1[INNER], (s1 * (1 - 1 / aC))[INNER], 1e-12)
+    close(a2[INNER], (s2 * (1 - 1 / aC))[INNER], 1e-12)
+    close(a12[INNER], (s12 * (1 - 1 / aZ))[INNER], 1e-12)
+    assert aC[3, 4] > 1e3 and aC.min() == 5.0  # both the mass floor and alpha floor are hit
+
+
+def test_adaptive_evp_transposition_symmetry():
+    """Transposing a square closed state transposes the adaptive subcycle
+    (u <-> v, sigma2 -> -sigma2), which fixes the directions of the alphaZ
+    and u/v beta averages."""
+    st = _random_closed_state(41, useAdaptiveEVP=True, aEvpCoeff=0.7, aEVPalphaMin=3.0)
+    vs = st.variables
+    vs.fCori = np.zeros(st.shape)
+    tt = _transpose_state(st)
+    out = O.evp_solver(st, n_substeps=6)
+    outT = O.evp_solver(tt, n_substeps=6)
+    close(outT[1].T, out[0], 1e-11)
+    close(outT[0].T, out[1], 1e-11)
+    close(outT[2].T, out[2], 1e-11)
+    close(-outT[3].T, out[3], 1e-11)
+    close(outT[4].T, out[4], 1e-11)

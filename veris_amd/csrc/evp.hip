@@ -68,7 +68,6 @@ int evp_run_core(const Geom& g, const vh_settings* s, const vh_fields* fl, const
     for (int r = 0; r < q; ++r)
       VH_REQUIRE(out[q] != out[r], VH_EINVAL, "evp_solver: two outputs alias");
   }
-  VH_REQUIRE(s->useAdaptiveEVP == 0.0, VH_EUNSUPPORTED, "useAdaptiveEVP is not supported");
   VH_REQUIRE(s->noSlip != 0.0, VH_EUNSUPPORTED, "noSlip=False (side drag) is not supported");
   VH_REQUIRE(s->secondOrderBC == 0.0, VH_EUNSUPPORTED, "secondOrderBC is not supported");
   VH_REQUIRE(s->useEVP != 0.0, VH_EUNSUPPORTED, "useEVP=False (free drift) is not supported");

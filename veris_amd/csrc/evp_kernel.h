@@ -61,7 +61,8 @@ struct EvpScalars {
   double cwN, cwS, rhoSea, cDragMin, cDragMin2;
   double betaU, betaP1, rdt;  // evpBeta/deltatDyn, betaU + 1/deltatDyn, 1/deltatDyn
   double K2, U0sq, K1, cBasalStar;
-  int kterms, turn, basal, pad;
+  double evpcfac, alphaMin;  // adaptive EVP: deltatDyn * 4 * (pi * aEvpCoeff)^2, aEVPalphaMin
+  int kterms, turn, basal, adaptive;
 };
 
 // Per-member geometry of a batched launch: element offset of the member's
@@ -139,7 +140,10 @@ inline void fill_evp_scalars(const vh_settings* s, int kterms, int basal, EvpSca
   sc.kterms = kterms;
   sc.turn = s->sinWat != 0.0 ? 1 : 0;
   sc.basal = basal;
-  sc.pad = 0;
+  sc.adaptive = s->useAdaptiveEVP != 0.0 ? 1 : 0;
+  // veris/evp_solver.py:266-271 (same operation order)
+  sc.evpcfac = sc.adaptive ? s->deltatDyn * 4 * ((M_PI * s->aEvpCoeff) * (M_PI * s->aEvpCoeff)) : 0.0;
+  sc.alphaMin = s->aEVPalphaMin;
 }
 
 // Step-constant field pointers of one subcycle (host side).
@@ -252,7 +256,7 @@ __global__ __launch_bounds__(RI* RJ) void evp_substep_kernel(const EvpArgBlock<T
   static_assert((RJ & (RJ - 1)) == 0, "RJ must be a power of two");
   static_assert(TI >= 1 && TJ >= 1, "tile too small for K");
   __shared__ A sU[NT], sV[NT], sM[NT], sMU[NT], sMV[NT], sDU[NT], sDV[NT], sE12[NT], sQ[NT], sZ[NT];
-  __shared__ A sS11[NT], sS22[NT], sCD[NT], sCB[NT], sFV[NT], sFU[NT];
+  __shared__ A sS11[NT], sS22[NT], sCD[NT], sCB[NT], sFV[NT], sFU[NT], sAl[NT];
 
   // tile of this workgroup: workgroups are dealt round-robin to the 8 XCDs,
   // so give each XCD a contiguous run of tiles (neighbours share L2 lines)
@@ -407,6 +411,15 @@ __global__ __launch_bounds__(RI* RJ) void evp_substep_kernel(const EvpArgBlock<T
       const TD dreg = dC + (TD)sc.deltaMin;
       const TD zeta = ((TD)0.5 * ((TD)P * (TD)sc.tens1p)) / dreg;
       sZ[t] = (A)zeta;
+      // relaxation parameter at c (veris/evp_solver.py:65-73 when adaptive)
+      A am1 = (A)sc.alphaM1, den1 = (A)sc.den1;
+      if (sc.adaptive) {
+        A aC = sqrt((((A)zeta * (A)sc.evpcfac) / fmax(mC, (A)1e-4)) * rrA) * Mc;
+        aC = fmax(aC, (A)sc.alphaMin);
+        sAl[t] = aC;
+        am1 = aC - (A)1;
+        den1 = (A)1.0 / aC;
+      }
       if (cs) {
         const TD pressD =
             (((TD)P * (TD)sc.pr1m) + ((((TD)2.0 * zeta) * dC) * (TD)sc.pr) / (TD)sc.tens1p) *
@@ -414,8 +427,8 @@ __global__ __launch_bounds__(RI* RJ) void evp_substep_kernel(const EvpArgBlock<T
         const A zt = (A)zeta, press = (A)pressD;
         const A divg = (((A)2 * zt) * ep) - press;
         const A tens = ((A)2 * zt) * em;
-        s1 = (((s1 * (A)sc.alphaM1) + divg) * (A)sc.den1) * Mc;
-        s2 = (((s2 * (A)sc.alphaM1) + tens * (A)0.25) * (A)sc.den2) * Mc;
+        s1 = (((s1 * am1) + divg) * den1) * Mc;
+        s2 = (((s2 * am1) + tens * (A)0.25) * den1) * Mc;
         sS11[t] = ((A)0.5 * (s1 + s2)) * dyV;
         sS22[t] = ((A)0.5 * (s1 - s2)) * dxU;
         const A D = (A)(fc < (A)0 ? sc.cwS : sc.cwN) * (A)sc.rhoSea;
@@ -454,7 +467,14 @@ __global__ __launch_bounds__(RI* RJ) void evp_substep_kernel(const EvpArgBlock<T
       const A nrm = n > (A)0 ? (A)1.0 / n : (A)0.0;
       const A zz = nrm * ((sZ[t] + sZ[t - RJ]) + (sZ[t - 1] + sZ[t - RJ - 1]));
       const A shear = ((A)2 * zz) * sE12[t];
-      s12 = ((s12 * (A)sc.alphaM1) + shear * (A)0.25) * (A)sc.den2;
+      A am1z = (A)sc.alphaM1, den2 = (A)sc.den2;
+      if (sc.adaptive) {  // veris/evp_solver.py:97-100
+        A aZ = (A)0.5 * (sAl[t] + sAl[t - 1]);
+        aZ = (A)0.5 * (aZ + (A)0.5 * (sAl[t - RJ] + sAl[t - RJ - 1]));
+        am1z = aZ - (A)1;
+        den2 = (A)1.0 / aZ;
+      }
+      s12 = ((s12 * am1z) + shear * (A)0.25) * den2;
       sQ[t] = s12 * dxV;   // sigma12*dxV (q is dead after C)
       sDU[t] = s12 * dyU;  // sigma12*dyU (du is dead after C)
     }
@@ -486,14 +506,21 @@ __global__ __launch_bounds__(RI* RJ) void evp_substep_kernel(const EvpArgBlock<T
       A fy = wfy + iny * AS;
       fx = fx + (A)0.5 * (sFV[t] + sFV[t - RJ]);
       fy = fy - (A)0.5 * (sFU[t] + sFU[t - 1]);
-      A denU = mU * (A)sc.betaP1 + AW * ((cdU * (A)sc.cosWat) + (A)0.5 * (sCB[t] + sCB[t - RJ]));
-      A denV = mV * (A)sc.betaP1 + AS * ((cdV * (A)sc.cosWat) + (A)0.5 * (sCB[t] + sCB[t - 1]));
+      A bU = (A)sc.betaU, bV = (A)sc.betaU, bP1U = (A)sc.betaP1, bP1V = (A)sc.betaP1;
+      if (sc.adaptive) {  // veris/evp_solver.py:148-155
+        bU = ((A)0.5 * (sAl[t] + sAl[t - RJ])) * (A)sc.rdt;
+        bV = ((A)0.5 * (sAl[t] + sAl[t - 1])) * (A)sc.rdt;
+        bP1U = bU + (A)sc.rdt;
+        bP1V = bV + (A)sc.rdt;
+      }
+      A denU = mU * bP1U + AW * ((cdU * (A)sc.cosWat) + (A)0.5 * (sCB[t] + sCB[t - RJ]));
+      A denV = mV * bP1V + AS * ((cdV * (A)sc.cosWat) + (A)0.5 * (sCB[t] + sCB[t - 1]));
       denU = denU == (A)0 ? (A)1 : denU;
       denV = denV == (A)0 ? (A)1 : denV;
       const A un =
-          (MUc * ((((A)sc.betaU * mU) * u + (mU * (A)sc.rdt) * uNm1 + fx) + divx)) / denU;
+          (MUc * (((bU * mU) * u + (mU * (A)sc.rdt) * uNm1 + fx) + divx)) / denU;
       const A vn =
-          (MVc * ((((A)sc.betaU * mV) * v + (mV * (A)sc.rdt) * vNm1 + fy) + divy)) / denV;
+          (MVc * (((bV * mV) * v + (mV * (A)sc.rdt) * vNm1 + fy) + divy)) / denV;
       u = un;  // other threads read sU/sV, never these registers
       v = vn;
     }
