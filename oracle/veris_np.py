@@ -14,7 +14,8 @@ Conventions (identical to the reference):
   evaluation that follows the same order gives the same bits.
 
 Pins (see ``tests/test_oracle_*.py``): closed forms for strain rates,
-viscosities, the stress relaxation, drag and the free-drift fixed point;
+viscosities, the stress relaxation, drag (water and lateral) and the
+free-drift fixed point;
 the telescoping invariant of ``stressdiv``; exact one-cell upwind shift,
 conservation and the TVD bound of the advection; the scipy root of the
 surface energy balance for ``solve4temp``; budget identities of ``Growth``;
@@ -219,6 +220,24 @@ def strainrates(state, u, v):
     return e11, e22, e12
 
 
+def side_drag(state, u, v):
+    """``veris/dynamics_routines.py:98-140``: lateral drag coefficients at
+    u- and v-points (land neighbours along the face normal, or the coastline
+    form factors Fu/Fv)."""
+    vs, s = _vs(state)
+    speed = 0.5 * np.sqrt((u + ip1(u)) ** 2 + (v + jp1(v)) ** 2)
+    speedU = np.where(vs.AreaW > 0, 0.5 * (speed + im1(speed)), 0)
+    speedV = np.where(vs.AreaS > 0, 0.5 * (speed + jm1(speed)), 0)
+    maskU = (2 - im1(vs.iceMaskU) - ip1(vs.iceMaskU)) * vs.iceMaskU
+    maskV = (2 - jm1(vs.iceMaskV) - jp1(vs.iceMaskV)) * vs.iceMaskV
+    if s.use_coastline:
+        maskU = vs.Fu
+        maskV = vs.Fv
+    sdU = vs.SeaIceMassU * s.sideDragCoeff * maskU / (speedU + s.sideDragU0)
+    sdV = vs.SeaIceMassV * s.sideDragCoeff * maskV / (speedV + s.sideDragU0)
+    return sdU, sdV
+
+
 def viscosities(state, e11, e22, e12, P=None):
     """``veris/dynamics_routines.py:220-268``; returns (zeta, eta, press, deltaC)."""
     vs, s = _vs(state)
@@ -283,8 +302,6 @@ def evp_substep(state, u, v, uNm1, vNm1, s1, s2, s12, P=None, drag=None, trace=N
     ``:97-100`` (z-points) and ``:148-150`` (u/v points).
     """
     vs, s = _vs(state)
-    if not s.noSlip:
-        raise NotImplementedError("side drag (noSlip=False)")
     drag = drag or {}
     e11, e22, e12 = strainrates(state, u, v)
     zeta, _eta, press, dC = viscosities(state, e11, e22, e12, P=P)
@@ -354,6 +371,10 @@ def evp_substep(state, u, v, uNm1, vNm1, s1, s2, s12, P=None, drag=None, trace=N
         0.5 * (cd + jm1(cd)) * s.cosWat + 0.5 * (cb + jm1(cb)))
     denU = np.where(denU == 0, 1, denU)
     denV = np.where(denV == 0, 1, denV)
+    if not s.noSlip:  # veris/evp_solver.py:172-179
+        sdU, sdV = side_drag(state, u, v)
+        denU = denU + sdU
+        denV = denV + sdV
     un = vs.iceMaskU * (betaU * vs.SeaIceMassU * u + vs.SeaIceMassU * s.recip_deltatDyn * uNm1
                         + fx + divx) / denU
     vn = vs.iceMaskV * (betaV * vs.SeaIceMassV * v + vs.SeaIceMassV * s.recip_deltatDyn * vNm1

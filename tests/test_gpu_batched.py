@@ -121,12 +121,17 @@ def test_batched_fp32_storage_fp64_arith_against_fp64_oracle(cuda):
         assert max(worst["sigma1"], worst["sigma2"], worst["sigma12"]) <= 5e-3, worst
 
 
+BRANCHES = {"adaptive": {"useAdaptiveEVP": True}, "side_drag": {"noSlip": False}}
+
+
 @pytest.mark.parametrize("tile", [3, 9])
-def test_batched_adaptive_fp64_matches_oracle(cuda, tile):
-    """Adaptive EVP (per-cell alpha/beta) in the batched launch: every member
-    bit for bit against the oracle (closed regional members)."""
+@pytest.mark.parametrize("branch", sorted(BRANCHES))
+def test_batched_branches_fp64_match_oracle(cuda, branch, tile):
+    """Adaptive EVP (per-cell alpha/beta) and free slip with lateral drag in
+    the batched launch: every member bit for bit against the oracle (closed
+    regional members)."""
     sizes = [(37, 29), (64, 23), (30, 61)]
-    states = [pre_evp(synth.regional(21, m, nx, ny, useAdaptiveEVP=True))
+    states = [pre_evp(synth.regional(21, m, nx, ny, **BRANCHES[branch]))
               for m, (nx, ny) in enumerate(sizes)]
     for st in states:
         vs = st.variables
@@ -137,5 +142,5 @@ def test_batched_adaptive_fp64_matches_oracle(cuda, tile):
     for m, st in enumerate(states):
         want = O.evp_solver(st, n_substeps=17)
         for q, name in enumerate(EVP_NAMES):
-            assert_exact(f"adaptive member {m} {name}[tile={tile}]", g2n(ens.view(out[q], m)),
+            assert_exact(f"{branch} member {m} {name}[tile={tile}]", g2n(ens.view(out[q], m)),
                          want[q])

@@ -127,10 +127,22 @@ def test_evp_config1_1deg(cuda):
 
 @pytest.mark.parametrize("variant", ["regional", "own_fill", "turning", "kterms", "tensile",
                                      "no_repl_pressure", "small_domain", "adaptive",
-                                     "adaptive_regional"])
+                                     "adaptive_regional", "side_drag", "side_drag_regional",
+                                     "coastline", "side_drag_adaptive"])
 def test_evp_setting_variants(cuda, variant):
     if variant == "regional":
         st = synth.regional(7, 0, 37, 29)
+    elif variant == "side_drag":
+        st = synth.tiny(20, 14, noSlip=False)
+    elif variant == "side_drag_regional":
+        st = synth.regional(7, 1, 37, 29, noSlip=False, sideDragCoeff=0.004, sideDragU0=0.02)
+    elif variant == "coastline":
+        st = synth.tiny(20, 14, noSlip=False, use_coastline=True)
+        r = np.random.default_rng(8)
+        st.variables.Fu = r.uniform(0.0, 2.0, st.shape) * st.variables.iceMaskU
+        st.variables.Fv = r.uniform(0.0, 2.0, st.shape) * st.variables.iceMaskV
+    elif variant == "side_drag_adaptive":
+        st = synth.tiny(20, 14, noSlip=False, useAdaptiveEVP=True)
     elif variant == "adaptive":
         st = synth.tiny(20, 14, useAdaptiveEVP=True)
     elif variant == "adaptive_regional":
@@ -284,10 +296,10 @@ def test_errors_are_reported(cuda):
     from veris_amd._native import VerisHipError
 
     st = pre_evp(synth.tiny(12, 10)).to(cuda)
-    with pytest.raises(VerisHipError, match="VH_EUNSUPPORTED.*noSlip"):
-        st.settings.noSlip = False
+    with pytest.raises(VerisHipError, match="VH_EUNSUPPORTED.*secondOrderBC"):
+        st.settings.secondOrderBC = True
         K.evp_solver(st, 3)
-    st.settings.noSlip = True
+    st.settings.secondOrderBC = False
     st.variables.uIce = st.variables.uIce[:, :-1].contiguous()
     with pytest.raises(ValueError):
         K.evp_solver(st, 3)
@@ -305,15 +317,19 @@ def _periodic_artefact_cols(st):
     return [0, 1, NX - 2, NX - 1]
 
 
+BRANCHES = {"adaptive": {"useAdaptiveEVP": True}, "side_drag": {"noSlip": False}}
+
+
 @pytest.mark.parametrize("tile", [1, 5, 8, 9, 13])
-def test_adaptive_evp_tile_shapes(cuda, tile):
-    """Adaptive EVP (per-cell alpha, beta) on single- and multi-substep
-    shapes: bit-exact against the oracle (stresses outside the periodic
-    artefact columns for k > 1 on the x-cyclic grid)."""
+@pytest.mark.parametrize("branch", sorted(BRANCHES))
+def test_branch_tile_shapes(cuda, branch, tile):
+    """Adaptive EVP (per-cell alpha, beta) and free slip with lateral drag on
+    single- and multi-substep shapes: bit-exact against the oracle (stresses
+    outside the periodic artefact columns for k > 1 on the x-cyclic grid)."""
     from veris_amd._native import evp_tile_shapes
 
     k = evp_tile_shapes()[tile][2]
-    st = _spun_up(synth.config0(useAdaptiveEVP=True), cuda, n_spin=5)
+    st = _spun_up(synth.config0(**BRANCHES[branch]), cuda, n_spin=5)
     g = st.to(cuda)
     skip = _periodic_artefact_cols(st) if k > 1 else []
     keep = np.setdiff1d(np.arange(st.shape[0]), skip)
@@ -323,7 +339,7 @@ def test_adaptive_evp_tile_shapes(cuda, tile):
         a = g2n(a)
         if q >= 2:
             a, b = a[keep], b[keep]
-        assert_exact(f"adaptive {name}[tile={tile}]", a, b)
+        assert_exact(f"{branch} {name}[tile={tile}]", a, b)
 
 
 @pytest.mark.parametrize("tile", ALL_TILES)

@@ -39,6 +39,14 @@ MUTATIONS = {
     "adaptive alpha mass": ("/ np.maximum(vs.SeaIceMassC, 1e-4)", "/ np.maximum(vs.SeaIceMassU, 1e-4)"),
     "adaptive alpha floor": ("alphaC = np.maximum(alphaC, s.aEVPalphaMin)", "alphaC = np.maximum(alphaC, 1.0)"),
     "adaptive c-factor": ("return s.deltatDyn * 4 * (np.pi * s.aEvpCoeff) ** 2", "return s.deltatDyn * 4 * (np.pi * s.aEvpCoeff)"),
+    "side drag speed v faces": ("(u + ip1(u)) ** 2 + (v + jp1(v)) ** 2)", "(u + ip1(u)) ** 2 + (v + jm1(v)) ** 2)"),
+    "side drag u-point speed": ("0.5 * (speed + im1(speed)), 0)", "0.5 * (speed + ip1(speed)), 0)"),
+    "side drag v area gate": ("speedV = np.where(vs.AreaS > 0", "speedV = np.where(vs.AreaW > 0"),
+    "side drag land count": ("maskU = (2 - im1(vs.iceMaskU) - ip1(vs.iceMaskU))", "maskU = (2 - im1(vs.iceMaskU) - jp1(vs.iceMaskU))"),
+    "side drag mass": ("sdU = vs.SeaIceMassU * s.sideDragCoeff", "sdU = vs.SeaIceMassC * s.sideDragCoeff"),
+    "coastline form factor": ("        maskU = vs.Fu", "        maskU = vs.Fv"),
+    "side drag u denominator": ("denU = denU + sdU", "denU = denU + sdV"),
+    "free-slip boundary term": ("    if s.noSlip:\n", "    if True:\n"),
     "z-point average": ("z = c + im1(c)", "z = c + ip1(c)"),
     "Coriolis sign": ("fx = fx + 0.5 * (fv + im1(fv))", "fx = fx - 0.5 * (fv + im1(fv))"),
     "Coriolis u average": ("fu = vs.SeaIceMassC * vs.fCori * 0.5 * (u + ip1(u))", "fu = vs.SeaIceMassC * vs.fCori * 0.5 * (u + im1(u))"),

@@ -971,3 +971,127 @@ def test_adaptive_evp_transposition_symmetry():
     close(outT[2].T, out[2], 1e-11)
     close(-outT[3].T, out[3], 1e-11)
     close(outT[4].T, out[4], 1e-11)
+
+
+def _coast_state(**settings):
+    """Uniform grid with a land pattern along x: u-point columns i = 2..4, 8
+    and 10 are land, so u-columns 5 and 11 have a land neighbour to the west,
+    columns 1 and 7 one to the east and column 9 one on each side."""
+    st = _rest_state(noSlip=False, **settings)
+    vs = st.variables
+    r = np.random.default_rng(29)
+    wetU = np.ones(st.shape[0])
+    wetU[[2, 3, 4, 8, 10]] = 0.0
+    vs.iceMaskU = wetU[:, None] * np.ones(st.shape)
+    vs.SeaIceMassC = r.uniform(500.0, 1500.0, st.shape)
+    vs.SeaIceMassU = r.uniform(500.0, 1500.0, st.shape)
+    vs.SeaIceMassV = r.uniform(500.0, 1500.0, st.shape)
+    return st
+
+
+def test_side_drag_closed_form():
+    """Lateral drag (Liu et al. 2022) at hand-picked points: the land count
+    along the face normal (0, 1 or 2), the ice speed averaged from the two
+    c-points either side of the u-point, and zero speed where AreaW = 0."""
+    st = _coast_state()
+    vs, s = st.variables, st.settings
+    U, U2, V = 0.3, -0.1, 0.4
+    u = np.full(st.shape, U)
+    u[6] = U2
+    v = np.zeros(st.shape)
+    v[:, 5] = V
+    vs.AreaW[9, 7] = 0.0
+    vs.AreaS[10, 7] = 0.0
+    sdU, sdV = O.side_drag(st, u, v)
+    c, U0 = s.sideDragCoeff, s.sideDragU0
+    mU = vs.SeaIceMassU
+    # speed at c-point (i, j): 0.5 |(u_i + u_i+1, v_j + v_j+1)|
+    plain = abs(U)                                   # rows away from j = 4, 5
+    sp = 0.5 * np.hypot(2 * U, V)                    # rows j = 4, 5 (v at j = 5)
+    # u-column 5: west land, neighbour c-points 4 and 5 (u_6 = U2 enters c-point 5)
+    spc5 = 0.5 * abs(U + U2)
+    for j, base in ((3, plain), (4, sp), (5, sp), (6, plain)):
+        c5 = spc5 if j not in (4, 5) else 0.5 * np.hypot(U + U2, V)
+        np.testing.assert_allclose(sdU[5, j], mU[5, j] * c * 1 / (0.5 * (c5 + base) + U0),
+                                   rtol=1e-14)
+    # u-column 7: east land; c-points 6 (u_6 + u_7) and 7 are both ordinary
+    spc6 = 0.5 * abs(U2 + U)
+    np.testing.assert_allclose(sdU[7, 3], mU[7, 3] * c / (0.5 * (plain + spc6) + U0), rtol=1e-14)
+    # u-column 9: land on both sides (mask 2); AreaW = 0 at (9, 7) -> zero speed
+    np.testing.assert_allclose(sdU[9, 3], mU[9, 3] * c * 2 / (plain + U0), rtol=1e-14)
+    np.testing.assert_allclose(sdU[9, 7], mU[9, 7] * c * 2 / U0, rtol=1e-14)
+    # wet interior and land u-points carry none
+    assert np.all(sdU[[0, 6, 12, 13, 14, 15]] == 0) and np.all(sdU[[2, 3, 4, 8, 10]] == 0)
+    # v-points: no land along y; AreaS gate only matters through a non-zero mask
+    assert np.all(sdV == 0)
+
+
+def test_side_drag_v_points_and_area_gate():
+    """The same land pattern laid along y acts on v-points: mask from the
+    j-neighbours of iceMaskV, speed averaged over c-points j-1 and j, and
+    AreaS = 0 zeroing the speed."""
+    st = _rest_state(noSlip=False)
+    vs, s = st.variables, st.settings
+    wetV = np.ones(st.shape[1])
+    wetV[[2, 3, 6]] = 0.0
+    vs.iceMaskV = np.ones(st.shape) * wetV[None, :]
+    vs.SeaIceMassV = np.random.default_rng(3).uniform(500.0, 1500.0, st.shape)
+    V, V2 = 0.25, 0.05
+    v = np.full(st.shape, V)
+    v[:, 5] = V2
+    u = np.zeros(st.shape)
+    vs.AreaS[4, 7] = 0.0
+    sdU, sdV = O.side_drag(st, u, v)
+    c, U0, mV = s.sideDragCoeff, s.sideDragU0, vs.SeaIceMassV
+    # v-row 4: land at 3 (south); c-points 3 (v_3 + v_4 = 2V) and 4 (v_4 + v_5 = V + V2)
+    np.testing.assert_allclose(sdV[2, 4], mV[2, 4] * c / (0.5 * (V + 0.5 * (V + V2)) + U0),
+                               rtol=1e-14)
+    # v-row 7: land at 6 (south) only; mask 1
+    np.testing.assert_allclose(sdV[2, 7], mV[2, 7] * c / (V + U0), rtol=1e-14)
+    np.testing.assert_allclose(sdV[4, 7], mV[4, 7] * c / U0, rtol=1e-14)
+    # v-row 5: land at 6 (north); c-points 4 (V + V2) and 5 (V2 + V)
+    np.testing.assert_allclose(sdV[2, 5], mV[2, 5] * c / (0.5 * (V + V2) + U0), rtol=1e-14)
+    assert np.all(sdU == 0) and np.all(sdV[:, [0, 8, 9, 10, 11, 12, 13]] == 0)
+
+
+def test_free_slip_has_no_boundary_term():
+    """noSlip = False drops the 2 u / dy boundary term: the uniform flow along
+    the coast of the no-slip position test then has no shear at all."""
+    st = uniform_state(nx=8, ny=10, noSlip=False)
+    st.variables.iceMaskU[:, :6] = 0.0
+    _, _, e12 = O.strainrates(st, np.full(st.shape, 0.2), np.zeros(st.shape))
+    assert np.all(e12 == 0)
+
+
+@pytest.mark.parametrize("coastline", [False, True])
+def test_side_drag_enters_the_momentum_denominator(coastline):
+    """One substep with P = 0, f = 0 and no turning: the u-update is
+    (beta m u0/dt + m uNm1/dt + tau + C A uo) / (m (beta+1)/dt + C A + D_side),
+    with D_side = m c F / (speed + U0) and F the land count, or the coastline
+    form factor Fu (Fv) when use_coastline is set."""
+    st = _coast_state(use_coastline=coastline)
+    vs, s = st.variables, st.settings
+    r = np.random.default_rng(31)
+    vs.Fu = r.uniform(0.0, 3.0, st.shape)
+    vs.Fv = r.uniform(0.0, 3.0, st.shape)
+    tau, uo, u0, v0 = 1e-3, 0.01, 0.02, 0.015
+    vs.WindForcingX = np.full(st.shape, tau)
+    vs.uOcean = np.full(st.shape, uo)
+    vs.uIce = np.full(st.shape, u0)
+    vs.vIce = np.full(st.shape, v0)
+    u, v, *_ = O.evp_solver(st, n_substeps=1)
+    dt, beta, C, A = s.deltatDyn, s.evpBeta, s.cDragMin, 1.0
+    speed = 0.5 * np.hypot(2 * u0, 2 * v0)
+    if coastline:
+        FU, FV = vs.Fu, vs.Fv
+    else:
+        FU = np.zeros(st.shape)
+        FU[[1, 5, 7, 11]] = 1.0
+        FU[9] = 2.0
+        FV = np.zeros(st.shape)
+    for got, m, x0, forcing, oc, F, mask in (
+            (u, vs.SeaIceMassU, u0, tau, uo, FU, vs.iceMaskU),
+            (v, vs.SeaIceMassV, v0, 0.0, 0.0, FV, vs.iceMaskV)):
+        den = m * (beta + 1) / dt + C * A + m * s.sideDragCoeff * F / (speed + s.sideDragU0)
+        want = mask * (beta * m * x0 / dt + m * x0 / dt + forcing + C * A * oc) / den
+        close(got[INNER], want[INNER], 1e-13)

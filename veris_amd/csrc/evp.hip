@@ -68,8 +68,10 @@ int evp_run_core(const Geom& g, const vh_settings* s, const vh_fields* fl, const
     for (int r = 0; r < q; ++r)
       VH_REQUIRE(out[q] != out[r], VH_EINVAL, "evp_solver: two outputs alias");
   }
-  VH_REQUIRE(s->noSlip != 0.0, VH_EUNSUPPORTED, "noSlip=False (side drag) is not supported");
-  VH_REQUIRE(s->secondOrderBC == 0.0, VH_EUNSUPPORTED, "secondOrderBC is not supported");
+  VH_REQUIRE(s->noSlip == 0.0 || s->secondOrderBC == 0.0, VH_EUNSUPPORTED,
+             "secondOrderBC is not supported");
+  if (s->noSlip == 0.0 && s->use_coastline != 0.0)
+    VH_REQUIRE(fl->Fu && fl->Fv, VH_EINVAL, "lateral drag with use_coastline needs Fu, Fv");
   VH_REQUIRE(s->useEVP != 0.0, VH_EUNSUPPORTED, "useEVP=False (free drift) is not supported");
   const int kt = (fl->k1AtC || fl->k2AtC || fl->k1AtZ || fl->k2AtZ) ? 1 : 0;
   VH_REQUIRE(!kt || (fl->k1AtC && fl->k2AtC && fl->k1AtZ && fl->k2AtZ), VH_EINVAL,

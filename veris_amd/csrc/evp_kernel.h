@@ -1,5 +1,6 @@
 // Fused EVP substep kernel template (gfx950): K substeps of
-// veris/evp_solver.py:22-254 (non-adaptive, noSlip) per launch, shared by the
+// veris/evp_solver.py:22-254 per launch (fixed or adaptive relaxation, no-slip
+// or free-slip with lateral drag; not secondOrderBC), shared by the
 // single-domain drop-in (evp.hip, FP64/FP32 in the reference's operation
 // order) and the batched ensemble path (evp_batched_fast.hip).
 //
@@ -48,6 +49,7 @@ struct EvpConst {
   const T *rdxU, *rdyV, *rdyU, *rdxV, *dxU, *dyU, *dxV, *dyV, *rrAu, *rrAv, *rAz, *rrA;
   const T *k1C, *k2C, *k1Z, *k2Z;  // metric terms (read when kterms)
   const T *Rlow, *Area, *hIce;     // basal drag inputs (read when basal)
+  const T *Fu, *Fv;                // coastline form factors (read when sidedrag and coast)
 };
 
 struct EvpScalars {
@@ -62,7 +64,10 @@ struct EvpScalars {
   double betaU, betaP1, rdt;  // evpBeta/deltatDyn, betaU + 1/deltatDyn, 1/deltatDyn
   double K2, U0sq, K1, cBasalStar;
   double evpcfac, alphaMin;  // adaptive EVP: deltatDyn * 4 * (pi * aEvpCoeff)^2, aEVPalphaMin
+  double sdCoeff, sdU0;      // sideDragCoeff, sideDragU0
   int kterms, turn, basal, adaptive;
+  int sidedrag;  // noSlip = False: free-slip e12 and lateral drag in the denominators
+  int coast;     // use_coastline: lateral drag form factors from Fu/Fv
 };
 
 // Per-member geometry of a batched launch: element offset of the member's
@@ -144,6 +149,10 @@ inline void fill_evp_scalars(const vh_settings* s, int kterms, int basal, EvpSca
   // veris/evp_solver.py:266-271 (same operation order)
   sc.evpcfac = sc.adaptive ? s->deltatDyn * 4 * ((M_PI * s->aEvpCoeff) * (M_PI * s->aEvpCoeff)) : 0.0;
   sc.alphaMin = s->aEVPalphaMin;
+  sc.sidedrag = s->noSlip == 0.0 ? 1 : 0;
+  sc.coast = s->use_coastline != 0.0 ? 1 : 0;
+  sc.sdCoeff = s->sideDragCoeff;
+  sc.sdU0 = s->sideDragU0;
 }
 
 // Step-constant field pointers of one subcycle (host side).
@@ -188,6 +197,8 @@ inline void fill_evp_const(const vh_fields* fl, EvpConst<T>* out) {
   f.Rlow = evp_ptr<T>(fl->R_low);
   f.Area = evp_ptr<T>(fl->Area);
   f.hIce = evp_ptr<T>(fl->hIceMean);
+  f.Fu = evp_ptr<T>(fl->Fu);
+  f.Fv = evp_ptr<T>(fl->Fv);
 }
 
 // Fields a subcycle needs (all must be non-null).
@@ -345,6 +356,12 @@ __global__ __launch_bounds__(RI* RJ) void evp_substep_kernel(const EvpArgBlock<T
   }
   A cbc = (A)0;  // basal drag coefficient inputs are step constants: load lazily
   const A sg = sgn(fc);
+  // lateral drag form factors at this u- and v-point (veris/dynamics_routines.py:119-130)
+  A sdmU = (A)0, sdmV = (A)0;
+  if (sc.sidedrag && sc.coast) {
+    sdmU = ldg(f.Fu + off, c);
+    sdmV = ldg(f.Fv + off, c);
+  }
 
 #pragma unroll 1
   for (int s = 0; s < K; ++s) {
@@ -356,6 +373,7 @@ __global__ __launch_bounds__(RI* RJ) void evp_substep_kernel(const EvpArgBlock<T
     const bool cs = (a >= -1) && (b >= -1) && (a < TIs) && (b < TJs);
     const bool zs = (a >= 0) && (b >= 0) && (a <= TIs) && (b <= TJs);
     const bool upd = (a >= 0) && (b >= 0) && (a < TIs) && (b < TJs) && upd_any;
+    A spd = (A)0;  // ice speed at c (lateral drag only)
 
     // ---- A
     if (s > 0) __syncthreads();  // E of the previous substep still reads LDS
@@ -379,11 +397,17 @@ __global__ __launch_bounds__(RI* RJ) void evp_substep_kernel(const EvpArgBlock<T
       A inner = dudy + dvdx;
       if (sc.kterms) inner = (inner - k1Z * vavz) - k2Z * uavz;
       A e12 = ((A)0.5 * inner) * mz;
-      const A hu = MUc - sMU[t - 1];
-      const A hv = MVc - sMV[t - RJ];
-      e12 = e12 + ((((A)2.0 * uavz) * rdyU) * hu + (((A)2.0 * vavz) * rdxV) * hv);
+      if (!sc.sidedrag) {  // no-slip boundary term (veris/dynamics_routines.py:170-175)
+        const A hu = MUc - sMU[t - 1];
+        const A hv = MVc - sMV[t - RJ];
+        e12 = e12 + ((((A)2.0 * uavz) * rdyU) * hu + (((A)2.0 * vavz) * rdxV) * hv);
+      }
       sE12[t] = e12;
       sQ[t] = rAz * (e12 * e12);
+    }
+    if (sc.sidedrag && !sc.coast && upd) {  // land neighbours along the face normal
+      sdmU = (((A)2 - sMU[t - RJ]) - sMU[t + RJ]) * MUc;
+      sdmV = (((A)2 - sMV[t - 1]) - sMV[t + 1]) * MVc;
     }
     __syncthreads();
 
@@ -405,6 +429,10 @@ __global__ __launch_bounds__(RI* RJ) void evp_substep_kernel(const EvpArgBlock<T
       const A qa = sQ[t] + sQ[t + RJ];
       const A qb = sQ[t + 1] + sQ[t + RJ + 1];
       const A e12c = ((A)0.25 * rrA) * (qa + qb);
+      if (sc.sidedrag) {  // ice speed at c (veris/dynamics_routines.py:109-111)
+        const A su = u + uip, sv = v + vjp;
+        spd = (A)0.5 * sqrt(su * su + sv * sv);
+      }
       const A ep = e11 + e22, em = e11 - e22;
       const TD dsq = (TD)ep * (TD)ep + (TD)sc.rpd * ((TD)em * (TD)em + (TD)4.0 * (TD)e12c);
       const TD dC = sqrt(dsq);
@@ -462,6 +490,7 @@ __global__ __launch_bounds__(RI* RJ) void evp_substep_kernel(const EvpArgBlock<T
     __syncthreads();
 
     // ---- D: sigma12 on z-points (veris/averaging.py:4-19, veris/evp_solver.py:86,103)
+    if (sc.sidedrag && cc) sDV[t] = spd;  // dv is dead after C
     if (zs) {
       const A n = (Mc + sM[t - RJ]) + (sM[t - 1] + sM[t - RJ - 1]);
       const A nrm = n > (A)0 ? (A)1.0 / n : (A)0.0;
@@ -517,6 +546,12 @@ __global__ __launch_bounds__(RI* RJ) void evp_substep_kernel(const EvpArgBlock<T
       A denV = mV * bP1V + AS * ((cdV * (A)sc.cosWat) + (A)0.5 * (sCB[t] + sCB[t - 1]));
       denU = denU == (A)0 ? (A)1 : denU;
       denV = denV == (A)0 ? (A)1 : denV;
+      if (sc.sidedrag) {  // lateral drag (veris/dynamics_routines.py:113-138, evp_solver.py:169-175)
+        const A spU = AW > (A)0 ? (A)0.5 * (sDV[t] + sDV[t - RJ]) : (A)0;
+        const A spV = AS > (A)0 ? (A)0.5 * (sDV[t] + sDV[t - 1]) : (A)0;
+        denU = denU + ((mU * (A)sc.sdCoeff) * sdmU) / (spU + (A)sc.sdU0);
+        denV = denV + ((mV * (A)sc.sdCoeff) * sdmV) / (spV + (A)sc.sdU0);
+      }
       const A un =
           (MUc * (((bU * mU) * u + (mU * (A)sc.rdt) * uNm1 + fx) + divx)) / denU;
       const A vn =
