@@ -40,6 +40,14 @@ def ip1(a):  # a[i+1, j]
     return np.roll(a, -1, 0)
 
 
+def im2(a):  # a[i-2, j]
+    return np.roll(a, 2, 0)
+
+
+def jm2(a):  # a[i, j-2]
+    return np.roll(a, 2, 1)
+
+
 def jm1(a):  # a[i, j-1]
     return np.roll(a, 1, 1)
 
@@ -196,10 +204,12 @@ def basal_drag_coeffs(state, u, v):
 
 
 def strainrates(state, u, v):
-    """``veris/dynamics_routines.py:143-217`` (no ``secondOrderBC``)."""
+    """``veris/dynamics_routines.py:143-217``.
+
+    The second-order boundary branch (``:177-215``) is restated as the
+    reference writes it: its one-sided differences shift u along x and v
+    along y (axis 0 for u, axis 1 for v) and scale by recip_dyU/recip_dxV."""
     vs, s = _vs(state)
-    if s.noSlip and s.secondOrderBC:
-        raise NotImplementedError("secondOrderBC")
     dudx = (ip1(u) - u) * vs.recip_dxU
     uave = (ip1(u) + u) * 0.5
     dvdy = (jp1(v) - v) * vs.recip_dyV
@@ -217,6 +227,15 @@ def strainrates(state, u, v):
         hu = vs.iceMaskU - jm1(vs.iceMaskU)
         hv = vs.iceMaskV - im1(vs.iceMaskV)
         e12 = e12 + (2.0 * uavz * vs.recip_dyU * hu + 2.0 * vavz * vs.recip_dxV * hv)
+    if s.noSlip and s.secondOrderBC:
+        MU, MV = vs.iceMaskU, vs.iceMaskV
+        hu2 = (MU - im1(MU)) / 3.0
+        hv2 = (MV - jm1(MV)) / 3.0
+        hu2 = hu2 * (im2(MU) * im1(MU) + ip1(MU) * MU)
+        hv2 = hv2 * (jm2(MV) * jm1(MV) + jp1(MV) * MV)
+        e12 = e12 + 0.5 * (
+            vs.recip_dyU * (6.0 * uavz - im2(u) * im1(MU) - ip1(u) * MU) * hu2
+            + vs.recip_dxV * (6.0 * vavz - jm2(v) * jm1(MV) - jp1(v) * MV) * hv2)
     return e11, e22, e12
 
 

@@ -121,15 +121,17 @@ def test_batched_fp32_storage_fp64_arith_against_fp64_oracle(cuda):
         assert max(worst["sigma1"], worst["sigma2"], worst["sigma12"]) <= 5e-3, worst
 
 
-BRANCHES = {"adaptive": {"useAdaptiveEVP": True}, "side_drag": {"noSlip": False}}
+BRANCHES = {"adaptive": {"useAdaptiveEVP": True}, "side_drag": {"noSlip": False},
+            "second_order": {"secondOrderBC": True}}
 
 
 @pytest.mark.parametrize("tile", [3, 9])
 @pytest.mark.parametrize("branch", sorted(BRANCHES))
 def test_batched_branches_fp64_match_oracle(cuda, branch, tile):
-    """Adaptive EVP (per-cell alpha/beta) and free slip with lateral drag in
-    the batched launch: every member bit for bit against the oracle (closed
-    regional members)."""
+    """Adaptive EVP (per-cell alpha/beta), free slip with lateral drag and
+    the second-order no-slip boundary term in the batched launch: every
+    member bit for bit against the oracle (closed regional members); the
+    second-order term refuses multi-substep shapes."""
     sizes = [(37, 29), (64, 23), (30, 61)]
     states = [pre_evp(synth.regional(21, m, nx, ny, **BRANCHES[branch]))
               for m, (nx, ny) in enumerate(sizes)]
@@ -138,6 +140,12 @@ def test_batched_branches_fp64_match_oracle(cuda, branch, tile):
         vs.uIce, vs.vIce, vs.sigma1, vs.sigma2, vs.sigma12 = O.evp_solver(st, n_substeps=5)
         pre_evp(st)
     ens = Ensemble(states, device=cuda, dtype=torch.float64)
+    if branch == "second_order" and tile == 9:
+        from veris_amd._native import VerisHipError
+
+        with pytest.raises(VerisHipError, match="VH_EUNSUPPORTED.*secondOrderBC"):
+            ens.evp(17, tile=tile)
+        return
     out = ens.evp(17, tile=tile)
     for m, st in enumerate(states):
         want = O.evp_solver(st, n_substeps=17)

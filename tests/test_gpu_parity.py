@@ -128,10 +128,21 @@ def test_evp_config1_1deg(cuda):
 @pytest.mark.parametrize("variant", ["regional", "own_fill", "turning", "kterms", "tensile",
                                      "no_repl_pressure", "small_domain", "adaptive",
                                      "adaptive_regional", "side_drag", "side_drag_regional",
-                                     "coastline", "side_drag_adaptive"])
+                                     "coastline", "side_drag_adaptive", "second_order",
+                                     "second_order_regional", "second_order_kterms"])
 def test_evp_setting_variants(cuda, variant):
     if variant == "regional":
         st = synth.regional(7, 0, 37, 29)
+    elif variant == "second_order":
+        st = synth.tiny(20, 14, secondOrderBC=True)
+    elif variant == "second_order_regional":
+        st = synth.regional(7, 2, 37, 29, secondOrderBC=True, useAdaptiveEVP=True)
+    elif variant == "second_order_kterms":
+        st = synth.tiny(20, 14, secondOrderBC=True, sinWat=0.42261826174069944,
+                        cosWat=0.9063077870366499)
+        r = np.random.default_rng(4)
+        for k in ("k1AtC", "k2AtC", "k1AtZ", "k2AtZ"):
+            st.variables[k] = r.normal(0, 1e-7, st.shape)
     elif variant == "side_drag":
         st = synth.tiny(20, 14, noSlip=False)
     elif variant == "side_drag_regional":
@@ -296,10 +307,10 @@ def test_errors_are_reported(cuda):
     from veris_amd._native import VerisHipError
 
     st = pre_evp(synth.tiny(12, 10)).to(cuda)
-    with pytest.raises(VerisHipError, match="VH_EUNSUPPORTED.*secondOrderBC"):
-        st.settings.secondOrderBC = True
+    with pytest.raises(VerisHipError, match="VH_EUNSUPPORTED.*useEVP"):
+        st.settings.useEVP = False
         K.evp_solver(st, 3)
-    st.settings.secondOrderBC = False
+    st.settings.useEVP = True
     st.variables.uIce = st.variables.uIce[:, :-1].contiguous()
     with pytest.raises(ValueError):
         K.evp_solver(st, 3)
@@ -317,20 +328,27 @@ def _periodic_artefact_cols(st):
     return [0, 1, NX - 2, NX - 1]
 
 
-BRANCHES = {"adaptive": {"useAdaptiveEVP": True}, "side_drag": {"noSlip": False}}
+BRANCHES = {"adaptive": {"useAdaptiveEVP": True}, "side_drag": {"noSlip": False},
+            "second_order": {"secondOrderBC": True}}
 
 
 @pytest.mark.parametrize("tile", [1, 5, 8, 9, 13])
 @pytest.mark.parametrize("branch", sorted(BRANCHES))
 def test_branch_tile_shapes(cuda, branch, tile):
-    """Adaptive EVP (per-cell alpha, beta) and free slip with lateral drag on
-    single- and multi-substep shapes: bit-exact against the oracle (stresses
-    outside the periodic artefact columns for k > 1 on the x-cyclic grid)."""
-    from veris_amd._native import evp_tile_shapes
+    """Adaptive EVP (per-cell alpha, beta), free slip with lateral drag and
+    the second-order no-slip boundary term on single- and multi-substep
+    shapes: bit-exact against the oracle (stresses outside the periodic
+    artefact columns for k > 1 on the x-cyclic grid).  The second-order term
+    reads two cells beyond the face, so multi-substep shapes refuse it."""
+    from veris_amd._native import VerisHipError, evp_tile_shapes
 
     k = evp_tile_shapes()[tile][2]
     st = _spun_up(synth.config0(**BRANCHES[branch]), cuda, n_spin=5)
     g = st.to(cuda)
+    if branch == "second_order" and k > 1:
+        with pytest.raises(VerisHipError, match="VH_EUNSUPPORTED.*secondOrderBC"):
+            K.evp_solver(g, n_substeps=3, tile=tile)
+        return
     skip = _periodic_artefact_cols(st) if k > 1 else []
     keep = np.setdiff1d(np.arange(st.shape[0]), skip)
     want = O.evp_solver(st, n_substeps=21)

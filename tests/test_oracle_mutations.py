@@ -47,6 +47,11 @@ MUTATIONS = {
     "coastline form factor": ("        maskU = vs.Fu", "        maskU = vs.Fv"),
     "side drag u denominator": ("denU = denU + sdU", "denU = denU + sdV"),
     "free-slip boundary term": ("    if s.noSlip:\n", "    if True:\n"),
+    "second-order u two back": ("- im2(u) * im1(MU)", "- im1(u) * im1(MU)"),
+    "second-order v scale": ("hv2 = (MV - jm1(MV)) / 3.0", "hv2 = (MV - jm1(MV)) / 2.0"),
+    "second-order v axis": ("hv2 = (MV - jm1(MV)) / 3.0", "hv2 = (MV - im1(MV)) / 3.0"),
+    "second-order v metric": ("+ vs.recip_dxV * (6.0 * vavz", "+ vs.recip_dyV * (6.0 * vavz"),
+    "second-order six": ("vs.recip_dyU * (6.0 * uavz", "vs.recip_dyU * (4.0 * uavz"),
     "z-point average": ("z = c + im1(c)", "z = c + ip1(c)"),
     "Coriolis sign": ("fx = fx + 0.5 * (fv + im1(fv))", "fx = fx - 0.5 * (fv + im1(fv))"),
     "Coriolis u average": ("fu = vs.SeaIceMassC * vs.fCori * 0.5 * (u + ip1(u))", "fu = vs.SeaIceMassC * vs.fCori * 0.5 * (u + im1(u))"),

@@ -1095,3 +1095,46 @@ def test_side_drag_enters_the_momentum_denominator(coastline):
         den = m * (beta + 1) / dt + C * A + m * s.sideDragCoeff * F / (speed + s.sideDragU0)
         want = mask * (beta * m * x0 / dt + m * x0 / dt + forcing + C * A * oc) / den
         close(got[INNER], want[INNER], 1e-13)
+
+
+def test_second_order_boundary_term_closed_form():
+    """secondOrderBC with land u-columns 3..5 and a velocity linear in x
+    (u_i = a + b i, independent of y, v = 0): only the two coast columns get a
+    shear, 0.5/dy * (6 u_6 - u_7)/3 east of the land (its first wet column
+    i = 6) and -0.5/dy * (6 u_3 - u_1)/3 at its first land column i = 3 --
+    the branch's one-sided differences as the reference writes them, along
+    x with 1/dy.  The mirror case (land v-rows, v linear in y) is the same
+    along y with 1/dx."""
+    st = uniform_state(nx=8, ny=10, secondOrderBC=True)
+    dx, dy = st.variables.dxV[0, 0], st.variables.dyU[0, 0]
+    a, b = 0.05, 0.013
+    wet = np.ones(st.shape[0])
+    wet[[3, 4, 5]] = 0.0
+    st.variables.iceMaskU = wet[:, None] * np.ones(st.shape)
+    uu = a + b * np.arange(st.shape[0])
+    u = uu[:, None] * np.ones(st.shape)
+    _, _, e12 = O.strainrates(st, u, np.zeros(st.shape))
+    want = np.zeros(st.shape)
+    want[6] = 0.5 / dy * (6 * uu[6] - uu[7]) / 3
+    want[3] = -0.5 / dy * (6 * uu[3] - uu[1]) / 3
+    close(e12, want, 1e-14)
+
+    st = uniform_state(nx=10, ny=8, secondOrderBC=True)
+    wet = np.ones(st.shape[1])
+    wet[[3, 4, 5]] = 0.0
+    st.variables.iceMaskV = np.ones(st.shape) * wet[None, :]
+    vv = a + b * np.arange(st.shape[1])
+    v = np.ones(st.shape) * vv[None, :]
+    _, _, e12 = O.strainrates(st, np.zeros(st.shape), v)
+    want = np.zeros(st.shape)
+    want[:, 6] = 0.5 / dx * (6 * vv[6] - vv[7]) / 3
+    want[:, 3] = -0.5 / dx * (6 * vv[3] - vv[1]) / 3
+    close(e12, want, 1e-14)
+
+
+def test_second_order_boundary_term_needs_no_slip():
+    """secondOrderBC only acts under noSlip: free slip ignores it."""
+    a = _random_closed_state(5, noSlip=False, secondOrderBC=True)
+    f = _random_closed_state(5, noSlip=False)
+    for x, y in zip(O.evp_solver(a, n_substeps=4), O.evp_solver(f, n_substeps=4)):
+        np.testing.assert_array_equal(x, y)

@@ -22,9 +22,12 @@ from . import _native as N
 from . import kernels as K
 
 
-def candidate_tiles(exact: bool = True) -> list[int]:
+def candidate_tiles(exact: bool = True, settings=None) -> list[int]:
+    """Tile indices to measure: single-substep shapes, plus the multi-substep
+    ones when ``exact`` is False and ``settings`` allow them."""
     shapes = N.evp_tile_shapes()
-    return [t for t, (_, _, k) in shapes.items() if (k == 1 or not exact)]
+    multi = not exact and (settings is None or not (settings.noSlip and settings.secondOrderBC))
+    return [t for t, (_, _, k) in shapes.items() if (k == 1 or multi)]
 
 
 def time_evp(state, tile: int, n_substeps: int = 100, repeats: int = 3) -> float:
@@ -46,7 +49,7 @@ def time_evp(state, tile: int, n_substeps: int = 100, repeats: int = 3) -> float
 def tune_evp(state, exact: bool = True, n_substeps: int = 100, verbose: bool = False) -> int:
     """Measure the candidate shapes on ``state``, remember and return the best."""
     times = {}
-    for t in candidate_tiles(exact):
+    for t in candidate_tiles(exact, state.settings):
         times[t] = time_evp(state, t, n_substeps)
         if verbose:
             print(f"tile {t} {N.evp_tile_shapes()[t]}: {times[t] * 1e6:.2f} us/substep")
@@ -79,7 +82,7 @@ def tune_batched(ens, exact: bool = True, n_substeps: int = 40, verbose: bool = 
     are exact for members without a periodic fill, so an ensemble of closed
     regional members may pass ``exact=False`` without changing any result."""
     times = {}
-    for t in candidate_tiles(exact):
+    for t in candidate_tiles(exact, ens.settings):
         times[t] = time_batched(ens, t, n_substeps)
         if verbose:
             print(f"tile {t} {N.evp_tile_shapes()[t]}: {times[t] * 1e6:.2f} us/substep")

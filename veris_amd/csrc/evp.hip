@@ -24,7 +24,8 @@ int pick_tile(const Geom& g, int req) {
 }
 
 template <typename T, typename TD>
-void launch_substeps(int cfg, const EvpArgBlock<T>* ab, int nsub, const Geom& g, hipStream_t st) {
+void launch_substeps(int cfg, const EvpArgBlock<T>* ab, int nsub, const Geom& g, bool so,
+                     hipStream_t st) {
   const int K = kEvpTiles[cfg].k;
   int l = 0;
   for (int q = 0; q < nsub / K + nsub % K; ++q) {
@@ -32,7 +33,7 @@ void launch_substeps(int cfg, const EvpArgBlock<T>* ab, int nsub, const Geom& g,
     const int TI = evp_tile_extent_i(c), TJ = evp_tile_extent_j(c);
     const int tiles_j = (g.NY + TJ - 1) / TJ;
     const int n_tiles = ((g.NX + TI - 1) / TI) * tiles_j;
-    evp_launch_cfg<T, TD, false>(c, ab, 0, l++, n_tiles, tiles_j, st);
+    evp_launch_cfg<T, TD, false>(c, ab, 0, l++, n_tiles, tiles_j, st, so);
   }
 }
 
@@ -68,8 +69,6 @@ int evp_run_core(const Geom& g, const vh_settings* s, const vh_fields* fl, const
     for (int r = 0; r < q; ++r)
       VH_REQUIRE(out[q] != out[r], VH_EINVAL, "evp_solver: two outputs alias");
   }
-  VH_REQUIRE(s->noSlip == 0.0 || s->secondOrderBC == 0.0, VH_EUNSUPPORTED,
-             "secondOrderBC is not supported");
   if (s->noSlip == 0.0 && s->use_coastline != 0.0)
     VH_REQUIRE(fl->Fu && fl->Fv, VH_EINVAL, "lateral drag with use_coastline needs Fu, Fv");
   VH_REQUIRE(s->useEVP != 0.0, VH_EUNSUPPORTED, "useEVP=False (free drift) is not supported");
@@ -99,6 +98,10 @@ int evp_run_core(const Geom& g, const vh_settings* s, const vh_fields* fl, const
     ab.alt[q] = (T*)((char*)ws + q * fb);
   }
   const int cfg = pick_tile(g, tile_req);
+  VH_REQUIRE(!ab.sc.sobc || kEvpTiles[cfg].k == 1, VH_EUNSUPPORTED,
+             "secondOrderBC needs a single-substep tile shape (tile %d has k=%d)", cfg,
+             kEvpTiles[cfg].k);
+  const bool so = ab.sc.sobc != 0;
   ab.g = g;
   ab.nlaunch = evp_n_launches(cfg, nsub);
   ab.members = nullptr;
@@ -107,13 +110,13 @@ int evp_run_core(const Geom& g, const vh_settings* s, const vh_fields* fl, const
   VH_TRY_HIP(hipGetLastError());
   if (nsub >= 8 && graphs_enabled()) {
     char key[160];
-    snprintf(key, sizeof(key), "evp1/%d/%d/%d/%d/%d/%d/%d/%d/%p", dtype, cfg, nsub, g.NX, g.NY,
-             g.nx, g.fill_x, g.fill_y, (const void*)ab_dev);
-    int rc = graph_run(key, [&](hipStream_t cs) { launch_substeps<T, double>(cfg, ab_dev, nsub, g, cs); },
+    snprintf(key, sizeof(key), "evp1/%d/%d/%d/%d/%d/%d/%d/%d/%d/%p", dtype, cfg, (int)so, nsub,
+             g.NX, g.NY, g.nx, g.fill_x, g.fill_y, (const void*)ab_dev);
+    int rc = graph_run(key, [&](hipStream_t cs) { launch_substeps<T, double>(cfg, ab_dev, nsub, g, so, cs); },
                        st);
     if (rc) return rc;
   } else {
-    launch_substeps<T, double>(cfg, ab_dev, nsub, g, st);
+    launch_substeps<T, double>(cfg, ab_dev, nsub, g, so, st);
   }
   VH_TRY_HIP(hipGetLastError());
   return VH_OK;

@@ -19,9 +19,9 @@
 namespace vh {
 
 void evp_batched_launch_f32(int cfg, const EvpArgBlock<float>* ab, int which, int launch,
-                            int n_tiles, hipStream_t st);
+                            int n_tiles, bool so, hipStream_t st);
 void evp_batched_launch_f32_f64(int cfg, const EvpArgBlock<float>* ab, int which, int launch,
-                                int n_tiles, hipStream_t st);
+                                int n_tiles, bool so, hipStream_t st);
 
 namespace {
 
@@ -77,20 +77,20 @@ int check_batch(const vh_batch* b) {
 }
 
 template <typename T>
-void launch_batched(int cfg, int arith, const EvpArgBlock<T>* ab, int which, int launch,
+void launch_batched(int cfg, int arith, bool so, const EvpArgBlock<T>* ab, int which, int launch,
                     int n_tiles, hipStream_t st);
 
 template <>
-void launch_batched<double>(int cfg, int, const EvpArgBlock<double>* ab, int which, int launch,
-                            int n_tiles, hipStream_t st) {
-  evp_launch_cfg<double, double, true>(cfg, ab, which, launch, n_tiles, 0, st);
+void launch_batched<double>(int cfg, int, bool so, const EvpArgBlock<double>* ab, int which,
+                            int launch, int n_tiles, hipStream_t st) {
+  evp_launch_cfg<double, double, true>(cfg, ab, which, launch, n_tiles, 0, st, so);
 }
 
 template <>
-void launch_batched<float>(int cfg, int arith, const EvpArgBlock<float>* ab, int which, int launch,
-                           int n_tiles, hipStream_t st) {
-  if (arith == 1) evp_batched_launch_f32_f64(cfg, ab, which, launch, n_tiles, st);
-  else evp_batched_launch_f32(cfg, ab, which, launch, n_tiles, st);
+void launch_batched<float>(int cfg, int arith, bool so, const EvpArgBlock<float>* ab, int which,
+                           int launch, int n_tiles, hipStream_t st) {
+  if (arith == 1) evp_batched_launch_f32_f64(cfg, ab, which, launch, n_tiles, so, st);
+  else evp_batched_launch_f32(cfg, ab, which, launch, n_tiles, so, st);
 }
 
 template <typename T>
@@ -111,8 +111,8 @@ int run_batched(const vh_batch* b, const vh_settings* s, const vh_fields* fl, in
     for (int r = 0; r < q; ++r)
       VH_REQUIRE(outs[q] != outs[r], VH_EINVAL, "evp_solver_batched: two outputs alias");
   }
-  VH_REQUIRE((s->noSlip == 0.0 || s->secondOrderBC == 0.0) && s->useEVP != 0.0, VH_EUNSUPPORTED,
-             "evp_solver_batched: secondOrderBC and useEVP=False are not supported");
+  VH_REQUIRE(s->useEVP != 0.0, VH_EUNSUPPORTED,
+             "evp_solver_batched: useEVP=False is not supported");
   if (s->noSlip == 0.0 && s->use_coastline != 0.0)
     VH_REQUIRE(fl->Fu && fl->Fv, VH_EINVAL, "lateral drag with use_coastline needs Fu, Fv");
   const int kt = (fl->k1AtC || fl->k2AtC || fl->k1AtZ || fl->k2AtZ) ? 1 : 0;
@@ -134,6 +134,9 @@ int run_batched(const vh_batch* b, const vh_settings* s, const vh_fields* fl, in
   }
 
   const int cfg = (b->tile > 0) ? b->tile : 3;
+  VH_REQUIRE(s->noSlip == 0.0 || s->secondOrderBC == 0.0 || kEvpTiles[cfg].k == 1,
+             VH_EUNSUPPORTED, "secondOrderBC needs a single-substep tile shape (tile %d has k=%d)",
+             cfg, kEvpTiles[cfg].k);
   char* base = (char*)ws;
   EvpArgBlock<T>* ab_dev = (EvpArgBlock<T>*)(base + 5 * L.fb);
   MemberDesc* mem_dev = (MemberDesc*)(base + 5 * L.fb + L.ab);
@@ -173,16 +176,18 @@ int run_batched(const vh_batch* b, const vh_settings* s, const vh_fields* fl, in
   hipLaunchKernelGGL(evp_write_args_kernel<T>, dim3(1), dim3(64), 0, st, ab_dev, ab);
   VH_TRY_HIP(hipGetLastError());
   const int K = kEvpTiles[cfg].k;
+  const bool so = ab.sc.sobc != 0;
   auto record = [&](hipStream_t cs) {
     int l = 0;
-    for (int q = 0; q < nsub / K; ++q) launch_batched<T>(cfg, b->arith, ab_dev, 0, l++, n_main, cs);
+    for (int q = 0; q < nsub / K; ++q)
+      launch_batched<T>(cfg, b->arith, so, ab_dev, 0, l++, n_main, cs);
     for (int q = 0; q < nsub % K; ++q)
-      launch_batched<T>(kEvpRemainderCfg, b->arith, ab_dev, 1, l++, n_rem, cs);
+      launch_batched<T>(kEvpRemainderCfg, b->arith, so, ab_dev, 1, l++, n_rem, cs);
   };
   if (nsub >= 8 && graphs_enabled()) {
     char key[192];
-    snprintf(key, sizeof(key), "evpb/%d/%d/%d/%d/%d/%d/%p", (int)sizeof(T), b->arith, cfg, nsub,
-             n_main, n_rem, (const void*)ab_dev);
+    snprintf(key, sizeof(key), "evpb/%d/%d/%d/%d/%d/%d/%d/%p", (int)sizeof(T), b->arith, (int)so,
+             cfg, nsub, n_main, n_rem, (const void*)ab_dev);
     int rc = graph_run(key, record, st);
     if (rc) return rc;
   } else {

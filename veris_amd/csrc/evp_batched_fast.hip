@@ -9,13 +9,13 @@
 namespace vh {
 
 void evp_batched_launch_f32(int cfg, const EvpArgBlock<float>* ab, int which, int launch,
-                            int n_tiles, hipStream_t st) {
-  evp_launch_cfg<float, double, true>(cfg, ab, which, launch, n_tiles, 0, st);
+                            int n_tiles, bool so, hipStream_t st) {
+  evp_launch_cfg<float, double, true>(cfg, ab, which, launch, n_tiles, 0, st, so);
 }
 
 void evp_batched_launch_f32_f64(int cfg, const EvpArgBlock<float>* ab, int which, int launch,
-                                int n_tiles, hipStream_t st) {
-  evp_launch_cfg<float, double, true, double>(cfg, ab, which, launch, n_tiles, 0, st);
+                                int n_tiles, bool so, hipStream_t st) {
+  evp_launch_cfg<float, double, true, double>(cfg, ab, which, launch, n_tiles, 0, st, so);
 }
 
 }  // namespace vh

@@ -1,6 +1,7 @@
 // Fused EVP substep kernel template (gfx950): K substeps of
 // veris/evp_solver.py:22-254 per launch (fixed or adaptive relaxation, no-slip
-// or free-slip with lateral drag; not secondOrderBC), shared by the
+// with or without the second-order boundary term, or free-slip with lateral
+// drag), shared by the
 // single-domain drop-in (evp.hip, FP64/FP32 in the reference's operation
 // order) and the batched ensemble path (evp_batched_fast.hip).
 //
@@ -68,6 +69,8 @@ struct EvpScalars {
   int kterms, turn, basal, adaptive;
   int sidedrag;  // noSlip = False: free-slip e12 and lateral drag in the denominators
   int coast;     // use_coastline: lateral drag form factors from Fu/Fv
+  int sobc;      // noSlip and secondOrderBC: second-order boundary shear (K = 1 shapes only)
+  int pad;
 };
 
 // Per-member geometry of a batched launch: element offset of the member's
@@ -151,6 +154,7 @@ inline void fill_evp_scalars(const vh_settings* s, int kterms, int basal, EvpSca
   sc.alphaMin = s->aEVPalphaMin;
   sc.sidedrag = s->noSlip == 0.0 ? 1 : 0;
   sc.coast = s->use_coastline != 0.0 ? 1 : 0;
+  sc.sobc = (s->noSlip != 0.0 && s->secondOrderBC != 0.0) ? 1 : 0;
   sc.sdCoeff = s->sideDragCoeff;
   sc.sdU0 = s->sideDragU0;
 }
@@ -258,7 +262,8 @@ __device__ inline int map_axis(int p, int N, int n, int filled) {
   return wrap(p, N);
 }
 
-template <typename T, typename TD, int RI, int RJ, int K, bool BATCHED, typename A = T>
+template <typename T, typename TD, int RI, int RJ, int K, bool BATCHED, typename A = T,
+          bool SO = false>
 __global__ __launch_bounds__(RI* RJ) void evp_substep_kernel(const EvpArgBlock<T>* __restrict__ ab,
                                                              int which, int launch, int n_tiles,
                                                              int tiles_j, int per_xcd) {
@@ -401,6 +406,30 @@ __global__ __launch_bounds__(RI* RJ) void evp_substep_kernel(const EvpArgBlock<T
         const A hu = MUc - sMU[t - 1];
         const A hv = MVc - sMV[t - RJ];
         e12 = e12 + ((((A)2.0 * uavz) * rdyU) * hu + (((A)2.0 * vavz) * rdxV) * hv);
+      }
+      if constexpr (SO && K == 1) {
+        // second-order boundary shear (veris/dynamics_routines.py:177-215): it
+        // reads u, v and the masks two cells away along the face normal, beyond
+        // the LDS region at its edge; with one substep per launch the launch
+        // input is the current state, so they are read from memory.  A separate
+        // instantiation keeps these registers out of the other variants.
+        const int cim2 = wrap(i - 2, g.NX) * g.NY + gj, cim1 = wrap(i - 1, g.NX) * g.NY + gj;
+        const int cip1 = wrap(i + 1, g.NX) * g.NY + gj;
+        const int cjm2 = gi * g.NY + wrap(j - 2, g.NY), cjm1 = gi * g.NY + wrap(j - 1, g.NY);
+        const int cjp1 = gi * g.NY + wrap(j + 1, g.NY);
+        const A uim2 = ldg(src[0] + off, cim2), uip1 = ldg(src[0] + off, cip1);
+        const A MUim2 = ldg(f.MU + off, cim2), MUim1 = ldg(f.MU + off, cim1);
+        const A MUip1 = ldg(f.MU + off, cip1);
+        const A vjm2 = ldg(src[1] + off, cjm2), vjp1 = ldg(src[1] + off, cjp1);
+        const A MVjm2 = ldg(f.MV + off, cjm2), MVjm1 = ldg(f.MV + off, cjm1);
+        const A MVjp1 = ldg(f.MV + off, cjp1);
+        A hu2 = (MUc - MUim1) / (A)3.0;
+        A hv2 = (MVc - MVjm1) / (A)3.0;
+        hu2 = hu2 * (MUim2 * MUim1 + MUip1 * MUc);
+        hv2 = hv2 * (MVjm2 * MVjm1 + MVjp1 * MVc);
+        const A tu = (rdyU * ((((A)6.0 * uavz) - uim2 * MUim1) - uip1 * MUc)) * hu2;
+        const A tv = (rdxV * ((((A)6.0 * vavz) - vjm2 * MVjm1) - vjp1 * MVc)) * hv2;
+        e12 = e12 + (A)0.5 * (tu + tv);
       }
       sE12[t] = e12;
       sQ[t] = rAz * (e12 * e12);
@@ -601,20 +630,42 @@ inline int evp_n_launches(int cfg, int nsub) {
   return nsub / kEvpTiles[cfg].k + nsub % kEvpTiles[cfg].k;
 }
 
-template <typename T, typename TD, int RI, int RJ, int K, bool BATCHED, typename A = T>
+template <typename T, typename TD, int RI, int RJ, int K, bool BATCHED, typename A = T,
+          bool SO = false>
 inline void evp_launch_shape(const EvpArgBlock<T>* ab, int which, int launch, int n_tiles,
                              int tiles_j, hipStream_t st) {
   const int per_xcd = (n_tiles + 7) / 8;
-  hipLaunchKernelGGL((evp_substep_kernel<T, TD, RI, RJ, K, BATCHED, A>), dim3(per_xcd * 8),
+  hipLaunchKernelGGL((evp_substep_kernel<T, TD, RI, RJ, K, BATCHED, A, SO>), dim3(per_xcd * 8),
                      dim3(RI * RJ), 0, st, ab, which, launch, n_tiles, tiles_j, per_xcd);
+}
+
+// Single-substep launch with the second-order no-slip boundary term (cfg must
+// be a k = 1 shape; the host checks).
+template <typename T, typename TD, bool BATCHED, typename A = T>
+inline void evp_launch_so(int cfg, const EvpArgBlock<T>* ab, int which, int launch, int n_tiles,
+                          int tiles_j, hipStream_t st) {
+  switch (cfg) {
+    case 1: evp_launch_shape<T, TD, 8, 64, 1, BATCHED, A, true>(ab, which, launch, n_tiles, tiles_j, st); break;
+    case 2: evp_launch_shape<T, TD, 12, 64, 1, BATCHED, A, true>(ab, which, launch, n_tiles, tiles_j, st); break;
+    case 3: evp_launch_shape<T, TD, 16, 64, 1, BATCHED, A, true>(ab, which, launch, n_tiles, tiles_j, st); break;
+    case 4: evp_launch_shape<T, TD, 8, 32, 1, BATCHED, A, true>(ab, which, launch, n_tiles, tiles_j, st); break;
+    case 6: evp_launch_shape<T, TD, 32, 32, 1, BATCHED, A, true>(ab, which, launch, n_tiles, tiles_j, st); break;
+    case 7: evp_launch_shape<T, TD, 6, 64, 1, BATCHED, A, true>(ab, which, launch, n_tiles, tiles_j, st); break;
+    default: evp_launch_shape<T, TD, 16, 32, 1, BATCHED, A, true>(ab, which, launch, n_tiles, tiles_j, st); break;
+  }
 }
 
 // One launch of shape `cfg`: tiles_j is used by single-domain launches, `which`
 // (0 main shape, 1 remainder shape) by batched ones.  T is the storage type
-// of the fields, A the arithmetic type (default T), TD that of Delta/zeta.
+// of the fields, A the arithmetic type (default T), TD that of Delta/zeta;
+// `so` selects the second-order boundary variant (k = 1 shapes only).
 template <typename T, typename TD, bool BATCHED, typename A = T>
 inline void evp_launch_cfg(int cfg, const EvpArgBlock<T>* ab, int which, int launch, int n_tiles,
-                           int tiles_j, hipStream_t st) {
+                           int tiles_j, hipStream_t st, bool so = false) {
+  if (so) {
+    evp_launch_so<T, TD, BATCHED, A>(cfg, ab, which, launch, n_tiles, tiles_j, st);
+    return;
+  }
   switch (cfg) {
     case 1: evp_launch_shape<T, TD, 8, 64, 1, BATCHED, A>(ab, which, launch, n_tiles, tiles_j, st); break;
     case 2: evp_launch_shape<T, TD, 12, 64, 1, BATCHED, A>(ab, which, launch, n_tiles, tiles_j, st); break;

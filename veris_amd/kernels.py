@@ -163,6 +163,15 @@ def pre_evp(state, pStar: torch.Tensor | None = None):
 EVP_TILE = {}
 
 
+def single_substep_ok(settings, tile: int) -> bool:
+    """False when ``settings`` need single-substep shapes (the second-order
+    no-slip boundary term reads two cells beyond the face) and ``tile`` is a
+    multi-substep shape."""
+    if not (settings.noSlip and settings.secondOrderBC):
+        return True
+    return N.evp_tile_shapes()[int(tile)][2] == 1
+
+
 def evp_solver(state, n_substeps: int | None = None, tile: int | None = None):
     """veris/evp_solver.py:257-319 -> (uIce, vIce, sigma1, sigma2, sigma12).
 
@@ -170,6 +179,8 @@ def evp_solver(state, n_substeps: int | None = None, tile: int | None = None):
     c = Ctx(state)
     if tile is None:
         tile = EVP_TILE.get((c.shape, c.dtype), 0)
+        if tile and not single_substep_ok(state.settings, tile):
+            tile = 0  # a tuned multi-substep shape cannot run the second-order boundary term
     c.dom.tile = int(tile)
     n = int(state.settings.nEVPsteps if n_substeps is None else n_substeps)
     out = c.empty(5)
