@@ -490,6 +490,13 @@ int vh_ice_to_ocean(const vh_domain* d, const vh_settings* s, const vh_fields* f
                     const void* forc_salt_surface_ice, const void* forc_salt_surface_res,
                     const vh_ice_to_ocean_out* out, void* stream);
 
+/* freedrift_solver (veris/freedrift_solver.py:5-62): steady free-drift
+ * velocities from WindForcingX/Y, hIceMean, fCori, uOcean/vOcean and the
+ * u/v masks, on every padded cell (the reference does not fill overlaps
+ * here). Device atan2/sin/cos: equal to the reference to rounding. */
+int vh_freedrift_solver(const vh_domain* d, const vh_settings* s, const vh_fields* f,
+                        void* uIce, void* vIce, void* stream);
+
 /* OceanStressUV (veris/ocean_stress.py:8-40), overlap-filled. */
 int vh_ocean_stress_uv(const vh_domain* d, const vh_settings* s, const vh_fields* f,
                        void* OceanStressU, void* OceanStressV, void* stream);

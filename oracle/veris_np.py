@@ -418,12 +418,50 @@ def evp_solver(state, n_substeps=None, P=None, drag=None):
     return u, v, s1, s2, s12
 
 
+def freedrift_solver(state):
+    """``veris/freedrift_solver.py:5-62``: steady free-drift velocities (no
+    internal stress) solved in closed form at c-points, averaged to u/v-points
+    and masked.  Returns (uIce, vIce)."""
+    vs, s = _vs(state)
+    tx = 0.5 * (vs.WindForcingX + ip1(vs.WindForcingX))
+    ty = 0.5 * (vs.WindForcingY + jp1(vs.WindForcingY))
+    mf = s.rhoIce * vs.hIceMean * vs.fCori
+    uoc = 0.5 * (vs.uOcean + ip1(vs.uOcean))
+    voc = 0.5 * (vs.vOcean + jp1(vs.vOcean))
+    rx = -tx - mf * voc
+    ry = -ty + mf * uoc
+    t1 = rx**2 + ry**2
+    nz = t1 > 0
+    rN = np.where(nz, np.sqrt(rx**2 + ry**2), 0)
+    rA = np.where(nz, np.arctan2(ry, rx), 0)
+    t1 = 1 / (np.where(vs.fCori < 0, s.waterIceDrag_south, s.waterIceDrag) * s.rhoSea)
+    t2 = t1**2 * mf**2
+    t3 = t1**2 * rN**2
+    t4 = t2**2 + 4 * t3
+    sN = np.where(t3 > 0, np.sqrt(0.5 * (np.sqrt(t4) - t2)), 0)
+    t1 = 1 / t1
+    t2 = t1 * sN**2
+    t3 = mf * sN
+    t4 = t2**2 + t3**2
+    ang = np.where(t4 > 0, rA - np.arctan2(t3, t2), 0)
+    uc = uoc - sN * np.cos(ang)
+    vc = voc - sN * np.sin(ang)
+    u = 0.5 * (im1(uc) + uc) * vs.iceMaskU
+    v = 0.5 * (jm1(vc) + vc) * vs.iceMaskV
+    return u, v
+
+
 def IceVelocities(state, **kw):
-    """``veris/dynsolver.py:87-102`` (EVP branch; free drift is out of scope)."""
-    s = state.settings
-    if s.useFreedrift and not s.useEVP:
-        raise NotImplementedError("free drift")
-    return evp_solver(state, **kw)
+    """``veris/dynsolver.py:87-102``.  The reference's free-drift branch reads
+    the stresses from an undefined name; here it keeps the state's stresses,
+    and EVP, when also on, overrides it as in the reference."""
+    vs, s = _vs(state)
+    if s.useEVP:
+        return evp_solver(state, **kw)
+    if s.useFreedrift:
+        u, v = freedrift_solver(state)
+        return u, v, vs.sigma1, vs.sigma2, vs.sigma12
+    raise ValueError("IceVelocities: neither useEVP nor useFreedrift is set")
 
 
 # --------------------------------------------------------------------------

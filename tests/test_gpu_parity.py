@@ -179,6 +179,27 @@ def test_evp_setting_variants(cuda, variant):
         assert_exact(f"{name}[{variant}]", a, b)
 
 
+@pytest.mark.parametrize("kind", ["global", "regional"])
+def test_freedrift_solver(cuda, kind):
+    """Free drift (closed-form c-point solution with atan2, cos, sin): the
+    device's transcendental functions differ from the host's by an ulp, so
+    the bar is 1e-13 of each field's scale; IceVelocities dispatches to it
+    when only useFreedrift is set and keeps the stresses."""
+    if kind == "global":
+        st = synth.config0(useFreedrift=True, useEVP=False, waterIceDrag_south=0.0042)
+    else:
+        st = synth.regional(7, 3, 37, 29, useFreedrift=True, useEVP=False)
+    st = _spun_up(st, cuda, n_spin=3)
+    want = O.freedrift_solver(st)
+    g = st.to(cuda)
+    got = K.freedrift_solver(g)
+    for name, a, b in zip(("uIce", "vIce"), got, want):
+        assert_close(f"freedrift {name}", g2n(a), b, 1e-13)
+    u, v, s1, s2, s12 = K.IceVelocities(g)
+    assert_close("IceVelocities uIce", g2n(u), want[0], 1e-13)
+    assert s1 is g.variables.sigma1 and s12 is g.variables.sigma12
+
+
 def test_evp_basal_drag(cuda):
     """basalDragK2 != 0 switches on the exp/log soft-max: tolerance 1e-12."""
     st = synth.tiny(20, 14, basalDragK2=15.0)
@@ -282,13 +303,14 @@ def test_fill_and_global_sum(cuda):
     assert abs(s - ref) <= 1e-13 * O.global_sum(np.abs(a[2:-2, 2:-2] * w[2:-2, 2:-2]))
 
 
-@pytest.mark.parametrize("n_steps", [1, 2])
-def test_full_step_config0(cuda, n_steps):
-    """Whole step (pre-EVP, 400 EVP substeps, ocean stress, advection, clean-up,
-    ridging, growth, fills).  exp/pow in strength and growth make the GPU
-    differ from the oracle by ulps that the next step carries on; tolerance
-    1e-9 relative to each field's maximum."""
-    st = synth.config0()
+@pytest.mark.parametrize("n_steps,solver", [(1, "evp"), (2, "evp"), (2, "freedrift")])
+def test_full_step_config0(cuda, n_steps, solver):
+    """Whole step (pre-EVP, 400 EVP substeps or free drift, ocean stress,
+    advection, clean-up, ridging, growth, fills).  exp/pow in strength and
+    growth make the GPU differ from the oracle by ulps that the next step
+    carries on; tolerance 1e-9 relative to each field's maximum."""
+    kw = {"useFreedrift": True, "useEVP": False} if solver == "freedrift" else {}
+    st = synth.config0(**kw)
     q0 = st.variables.Qnet.copy()
     sw = st.variables.SWdown.copy()
     g = st.to(cuda)
@@ -300,17 +322,18 @@ def test_full_step_config0(cuda, n_steps):
     for name in ("uIce", "vIce", "sigma1", "sigma2", "sigma12", "OceanStressU", "OceanStressV",
                  "hIceMean", "hSnowMean", "Area", "TSurf", "Qnet", "Qsw", "EmPmR",
                  "forc_salt_surface_ice", "SeaIceLoad", "IcePenetSW", "recip_hIceMean"):
-        assert_close(f"step{n_steps} {name}", g2n(g.variables[name]), st.variables[name], 1e-9)
+        assert_close(f"step{n_steps} {solver} {name}", g2n(g.variables[name]), st.variables[name],
+                     1e-9)
 
 
 def test_errors_are_reported(cuda):
     from veris_amd._native import VerisHipError
 
     st = pre_evp(synth.tiny(12, 10)).to(cuda)
-    with pytest.raises(VerisHipError, match="VH_EUNSUPPORTED.*useEVP"):
-        st.settings.useEVP = False
-        K.evp_solver(st, 3)
-    st.settings.useEVP = True
+    with pytest.raises(VerisHipError, match="VH_EUNSUPPORTED.*secondOrderBC"):
+        st.settings.secondOrderBC = True
+        K.evp_solver(st, 3, tile=8)
+    st.settings.secondOrderBC = False
     st.variables.uIce = st.variables.uIce[:, :-1].contiguous()
     with pytest.raises(ValueError):
         K.evp_solver(st, 3)

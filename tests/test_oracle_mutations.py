@@ -52,6 +52,13 @@ MUTATIONS = {
     "second-order v axis": ("hv2 = (MV - jm1(MV)) / 3.0", "hv2 = (MV - im1(MV)) / 3.0"),
     "second-order v metric": ("+ vs.recip_dxV * (6.0 * vavz", "+ vs.recip_dyV * (6.0 * vavz"),
     "second-order six": ("vs.recip_dyU * (6.0 * uavz", "vs.recip_dyU * (4.0 * uavz"),
+    "free drift wind centre": ("tx = 0.5 * (vs.WindForcingX + ip1(vs.WindForcingX))", "tx = 0.5 * (vs.WindForcingX + im1(vs.WindForcingX))"),
+    "free drift Coriolis sign": ("rx = -tx - mf * voc", "rx = -tx + mf * voc"),
+    "free drift turning": ("ang = np.where(t4 > 0, rA - np.arctan2(t3, t2), 0)", "ang = np.where(t4 > 0, rA + np.arctan2(t3, t2), 0)"),
+    "free drift quadratic": ("t4 = t2**2 + 4 * t3", "t4 = t2**2 + 2 * t3"),
+    "free drift hemisphere": ("np.where(vs.fCori < 0, s.waterIceDrag_south, s.waterIceDrag)", "np.where(vs.fCori > 0, s.waterIceDrag_south, s.waterIceDrag)"),
+    "free drift u average": ("u = 0.5 * (im1(uc) + uc) * vs.iceMaskU", "u = 0.5 * (ip1(uc) + uc) * vs.iceMaskU"),
+    "free drift v ocean centre": ("voc = 0.5 * (vs.vOcean + jp1(vs.vOcean))", "voc = 0.5 * (vs.vOcean + jm1(vs.vOcean))"),
     "z-point average": ("z = c + im1(c)", "z = c + ip1(c)"),
     "Coriolis sign": ("fx = fx + 0.5 * (fv + im1(fv))", "fx = fx - 0.5 * (fv + im1(fv))"),
     "Coriolis u average": ("fu = vs.SeaIceMassC * vs.fCori * 0.5 * (u + ip1(u))", "fu = vs.SeaIceMassC * vs.fCori * 0.5 * (u + im1(u))"),

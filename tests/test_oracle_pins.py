@@ -1138,3 +1138,106 @@ def test_second_order_boundary_term_needs_no_slip():
     f = _random_closed_state(5, noSlip=False)
     for x, y in zip(O.evp_solver(a, n_substeps=4), O.evp_solver(f, n_substeps=4)):
         np.testing.assert_array_equal(x, y)
+
+
+def _free_drift_residual(U, tau, Uoc, D, mf):
+    """Steady free drift in complex form: tau + D |Uoc - U| (Uoc - U) - i m f U."""
+    X = Uoc - U
+    return tau + D * abs(X) * X - 1j * mf * U
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_free_drift_solves_the_steady_balance(seed):
+    """On spatially uniform fields every u/v-point holds the c-point solution
+    of tau + D |Uoc - U| (Uoc - U) - i rhoIce h f U = 0 (air stress, water
+    drag, Coriolis), with D = waterIceDrag(_south) rhoSea by hemisphere; the
+    f = 0 case reduces to |X| X = tau / D."""
+    r = np.random.default_rng(100 + seed)
+    st = uniform_state(nx=6, ny=5, useFreedrift=True, useEVP=False, waterIceDrag_south=0.0042)
+    vs, s = st.variables, st.settings
+    tau = complex(*r.normal(0, 0.3, 2))
+    Uoc = complex(*r.normal(0, 0.2, 2))
+    h = r.uniform(0.2, 4.0)
+    f = (0.0, 1.4e-4, -1.3e-4, 1e-5, -1e-4, 1.2e-4)[seed]
+    vs.WindForcingX = np.full(st.shape, tau.real)
+    vs.WindForcingY = np.full(st.shape, tau.imag)
+    vs.uOcean = np.full(st.shape, Uoc.real)
+    vs.vOcean = np.full(st.shape, Uoc.imag)
+    vs.hIceMean = np.full(st.shape, h)
+    vs.fCori = np.full(st.shape, f)
+    u, v = O.freedrift_solver(st)
+    U = complex(u[5, 4], v[5, 4])
+    np.testing.assert_allclose(u, U.real, rtol=1e-13, atol=1e-16)
+    np.testing.assert_allclose(v, U.imag, rtol=1e-13, atol=1e-16)
+    D = (s.waterIceDrag_south if f < 0 else s.waterIceDrag) * s.rhoSea
+    res = _free_drift_residual(U, tau, Uoc, D, s.rhoIce * h * f)
+    assert abs(res) <= 1e-12 * abs(tau), (res, tau)
+    # the other hemisphere's drag coefficient would not balance
+    if f != 0:
+        D2 = (s.waterIceDrag if f < 0 else s.waterIceDrag_south) * s.rhoSea
+        assert abs(_free_drift_residual(U, tau, Uoc, D2, s.rhoIce * h * f)) > 1e-6 * abs(tau)
+
+
+def test_free_drift_staggering_against_a_numerical_root():
+    """Random fields: each c-point (between u-points i, i+1 and v-points j,
+    j+1) is solved independently with scipy's fsolve on the steady balance,
+    then averaged to the u-point (c-points i-1, i) and v-point (j-1, j) and
+    masked; with no wind stress and f = 0 the ice moves with the c-point
+    ocean velocity."""
+    from scipy.optimize import fsolve
+
+    st = uniform_state(nx=5, ny=4, useFreedrift=True, useEVP=False, waterIceDrag_south=0.0042)
+    vs, s = st.variables, st.settings
+    r = np.random.default_rng(12)
+    vs.WindForcingX = r.normal(0, 0.3, st.shape)
+    vs.WindForcingY = r.normal(0, 0.3, st.shape)
+    vs.uOcean = r.normal(0, 0.2, st.shape)
+    vs.vOcean = r.normal(0, 0.2, st.shape)
+    vs.hIceMean = r.uniform(0.2, 4.0, st.shape)
+    vs.fCori = r.choice([-1.3e-4, 1.4e-4], st.shape)
+    vs.iceMaskU = (r.random(st.shape) > 0.2).astype(float)
+    vs.iceMaskV = (r.random(st.shape) > 0.2).astype(float)
+    u, v = O.freedrift_solver(st)
+    NX, NY = st.shape
+    Uc = np.zeros(st.shape, complex)
+    for i in range(NX):
+        for j in range(NY):
+            ip, jp = (i + 1) % NX, (j + 1) % NY
+            tau = complex(0.5 * (vs.WindForcingX[i, j] + vs.WindForcingX[ip, j]),
+                          0.5 * (vs.WindForcingY[i, j] + vs.WindForcingY[i, jp]))
+            Uoc = complex(0.5 * (vs.uOcean[i, j] + vs.uOcean[ip, j]),
+                          0.5 * (vs.vOcean[i, j] + vs.vOcean[i, jp]))
+            D = (s.waterIceDrag_south if vs.fCori[i, j] < 0 else s.waterIceDrag) * s.rhoSea
+            mf = s.rhoIce * vs.hIceMean[i, j] * vs.fCori[i, j]
+
+            def eq(x):
+                z = _free_drift_residual(complex(*x), tau, Uoc, D, mf)
+                return [z.real, z.imag]
+
+            x = fsolve(eq, [Uoc.real, Uoc.imag], xtol=1e-12)
+            assert np.hypot(*eq(x)) <= 1e-12 * abs(tau)
+            Uc[i, j] = complex(*x)
+    want_u = 0.5 * (Uc.real + np.roll(Uc.real, 1, 0)) * vs.iceMaskU
+    want_v = 0.5 * (Uc.imag + np.roll(Uc.imag, 1, 1)) * vs.iceMaskV
+    close(u, want_u, 1e-9)
+    close(v, want_v, 1e-9)
+    vs.WindForcingX[:] = 0.0
+    vs.WindForcingY[:] = 0.0
+    vs.fCori[:] = 0.0
+    u, v = O.freedrift_solver(st)
+    uoc = vs.uOcean
+    close(u, 0.25 * (np.roll(uoc, 1, 0) + 2 * uoc + np.roll(uoc, -1, 0)) * vs.iceMaskU, 1e-14)
+
+
+def test_ice_velocities_dispatch():
+    """EVP wins when both solvers are on; free drift alone keeps the stresses."""
+    st = _random_closed_state(3, useFreedrift=True)
+    a = O.IceVelocities(st, n_substeps=3)
+    b = O.evp_solver(st, n_substeps=3)
+    for x, y in zip(a, b):
+        np.testing.assert_array_equal(x, y)
+    st.settings.useEVP = False
+    u, v, s1, s2, s12 = O.IceVelocities(st)
+    fu, fv = O.freedrift_solver(st)
+    np.testing.assert_array_equal(u, fu)
+    assert s1 is st.variables.sigma1 and s12 is st.variables.sigma12

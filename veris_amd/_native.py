@@ -138,6 +138,7 @@ SIGNATURES = {
     "vh_ocean_to_ice": [_DP, _SP, _FP, _P, _P, _P, _P, _P, ctypes.POINTER(OceanToIceOut), _P],
     "vh_ice_to_ocean": [_DP, _SP, _FP, _P, _P, _P, _P, ctypes.POINTER(IceToOceanOut), _P],
     "vh_ocean_stress_uv": [_DP, _SP, _FP, _P, _P, _P],
+    "vh_freedrift_solver": [_DP, _SP, _FP, _P, _P, _P],
     "vh_advection_workspace_bytes": [_DP, ctypes.POINTER(_SZ)],
     "vh_advection": [_DP, _SP, _FP, _P, _P, _P, _P, _SZ, _P],
     "vh_clean_up_advection": [_DP, _SP, _FP, _P, _P, _P, _P, _P, _P, _P],

@@ -71,7 +71,6 @@ int evp_run_core(const Geom& g, const vh_settings* s, const vh_fields* fl, const
   }
   if (s->noSlip == 0.0 && s->use_coastline != 0.0)
     VH_REQUIRE(fl->Fu && fl->Fv, VH_EINVAL, "lateral drag with use_coastline needs Fu, Fv");
-  VH_REQUIRE(s->useEVP != 0.0, VH_EUNSUPPORTED, "useEVP=False (free drift) is not supported");
   const int kt = (fl->k1AtC || fl->k2AtC || fl->k1AtZ || fl->k2AtZ) ? 1 : 0;
   VH_REQUIRE(!kt || (fl->k1AtC && fl->k2AtC && fl->k1AtZ && fl->k2AtZ), VH_EINVAL,
              "k1AtC/k2AtC/k1AtZ/k2AtZ must be all set or all NULL");

@@ -111,8 +111,6 @@ int run_batched(const vh_batch* b, const vh_settings* s, const vh_fields* fl, in
     for (int r = 0; r < q; ++r)
       VH_REQUIRE(outs[q] != outs[r], VH_EINVAL, "evp_solver_batched: two outputs alias");
   }
-  VH_REQUIRE(s->useEVP != 0.0, VH_EUNSUPPORTED,
-             "evp_solver_batched: useEVP=False is not supported");
   if (s->noSlip == 0.0 && s->use_coastline != 0.0)
     VH_REQUIRE(fl->Fu && fl->Fv, VH_EINVAL, "lateral drag with use_coastline needs Fu, Fv");
   const int kt = (fl->k1AtC || fl->k2AtC || fl->k1AtZ || fl->k2AtZ) ? 1 : 0;

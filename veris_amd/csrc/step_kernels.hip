@@ -1,6 +1,7 @@
 // Once-per-step stencil and pointwise kernels of the sea-ice step: pre-EVP
 // forcing (veris/area_mass.py, veris/dynamics_routines.py:5-18,
-// veris/dynsolver.py:7-84), ocean stress (veris/ocean_stress.py:8-40),
+// veris/dynsolver.py:7-84), the free-drift solver
+// (veris/freedrift_solver.py:5-62), ocean stress (veris/ocean_stress.py:8-40),
 // clean-up and ridging (veris/clean_up.py:5-47), the overlap fill
 // (veris/fill_overlap.py:6-18) and an FP64 global sum (veris/global_sum.py).
 //
@@ -155,6 +156,57 @@ __global__ __launch_bounds__(BLK) void ocean_stress_kernel(Geom g, DragArgs<T> a
                ((sg * sinW) * (T)0.5) * (cd * duC + cdim * duCim);
   store_filled(g, tu, i, j, su);
   store_filled(g, tv, i, j, sv);
+}
+
+// ---------------------------------------------------------------- free drift
+
+template <typename T>
+struct FreeDriftArgs {
+  const T *WFX, *WFY, *h, *fC, *uOc, *vOc, *MU, *MV;
+  T rhoIce, cwN, cwS, rhoSea;
+};
+
+// Steady free-drift velocity at c-point (i, j) (veris/freedrift_solver.py:12-52)
+template <typename T>
+__device__ inline void free_drift_c(const Geom& g, const FreeDriftArgs<T>& a, int i, int j, T* uc,
+                                    T* vc) {
+  const int ip = i == g.NX - 1 ? 0 : i + 1;
+  const int jp = j == g.NY - 1 ? 0 : j + 1;
+  const size_t c = AT(i, j), cip = AT(ip, j), cjp = AT(i, jp);
+  const T tx = (T)0.5 * (a.WFX[c] + a.WFX[cip]);
+  const T ty = (T)0.5 * (a.WFY[c] + a.WFY[cjp]);
+  const T f = a.fC[c];
+  const T mf = (a.rhoIce * a.h[c]) * f;
+  const T uoc = (T)0.5 * (a.uOc[c] + a.uOc[cip]);
+  const T voc = (T)0.5 * (a.vOc[c] + a.vOc[cjp]);
+  const T rx = (-tx) - mf * voc;
+  const T ry = (-ty) + mf * uoc;
+  const T r2 = rx * rx + ry * ry;
+  const T rN = r2 > (T)0 ? sqrt(rx * rx + ry * ry) : (T)0;
+  const T rA = r2 > (T)0 ? atan2(ry, rx) : (T)0;
+  const T t1 = (T)1 / ((f < (T)0 ? a.cwS : a.cwN) * a.rhoSea);
+  const T t2 = (t1 * t1) * (mf * mf);
+  const T t3 = (t1 * t1) * (rN * rN);
+  const T t4 = t2 * t2 + (T)4 * t3;
+  const T sN = t3 > (T)0 ? sqrt((T)0.5 * (sqrt(t4) - t2)) : (T)0;
+  const T q1 = (T)1 / t1;
+  const T q2 = q1 * (sN * sN);
+  const T q3 = mf * sN;
+  const T q4 = q2 * q2 + q3 * q3;
+  const T ang = q4 > (T)0 ? rA - atan2(q3, q2) : (T)0;
+  *uc = uoc - sN * cos(ang);
+  *vc = voc - sN * sin(ang);
+}
+
+template <typename T>
+__global__ __launch_bounds__(BLK) void free_drift_kernel(Geom g, FreeDriftArgs<T> a, T* u, T* v) {
+  CELL_PROLOGUE
+  T uc, vc, ucim, vcim, ucjm, vcjm;
+  free_drift_c(g, a, i, j, &uc, &vc);
+  free_drift_c(g, a, im, j, &ucim, &vcim);
+  free_drift_c(g, a, i, jm, &ucjm, &vcjm);
+  u[idx] = ((T)0.5 * (ucim + uc)) * a.MU[idx];
+  v[idx] = ((T)0.5 * (vcjm + vc)) * a.MV[idx];
 }
 
 // ---------------------------------------------------------------- clean-up, ridging
@@ -357,6 +409,33 @@ int clean_up(const Geom& g, const vh_settings* s, const vh_fields* f, void* h, v
                      C_<T>(f->hIceMean), C_<T>(f->hSnowMean), C_<T>(f->Area), C_<T>(f->TSurf),
                      (T)s->hIce_min, (T)s->Area_min, (T)s->celsius2K, (T*)h, (T*)hs, (T*)A, (T*)Ts,
                      (T*)osh, (T*)oshs);
+  VH_TRY_HIP(hipGetLastError());
+  return VH_OK;
+}
+
+template <typename T>
+int free_drift(const Geom& g, const vh_settings* s, const vh_fields* f, void* U, void* V,
+               hipStream_t st) {
+  FreeDriftArgs<T> a;
+  a.WFX = C_<T>(f->WindForcingX);
+  a.WFY = C_<T>(f->WindForcingY);
+  a.h = C_<T>(f->hIceMean);
+  a.fC = C_<T>(f->fCori);
+  a.uOc = C_<T>(f->uOcean);
+  a.vOc = C_<T>(f->vOcean);
+  a.MU = C_<T>(f->iceMaskU);
+  a.MV = C_<T>(f->iceMaskV);
+  const void* req[] = {a.WFX, a.WFY, a.h, a.fC, a.uOc, a.vOc, a.MU, a.MV};
+  for (const void* p : req) VH_REQUIRE(p != nullptr, VH_EINVAL, "freedrift_solver: null pointer");
+  VH_REQUIRE(U && V && U != V, VH_EINVAL, "freedrift_solver: null or aliased output");
+  for (const void* p : req)
+    VH_REQUIRE(U != p && V != p, VH_EINVAL, "freedrift_solver: output aliases an input");
+  a.rhoIce = (T)s->rhoIce;
+  a.cwN = (T)s->waterIceDrag;
+  a.cwS = (T)s->waterIceDrag_south;
+  a.rhoSea = (T)s->rhoSea;
+  hipLaunchKernelGGL(free_drift_kernel<T>, cell_grid(g, BLK), dim3(BLK), 0, st, g, a, (T*)U,
+                     (T*)V);
   VH_TRY_HIP(hipGetLastError());
   return VH_OK;
 }
@@ -596,6 +675,13 @@ extern "C" int vh_ocean_stress_uv(const vh_domain* d, const vh_settings* s, cons
   VH_PRE(d, s, f)
   return VH_DISPATCH(d, vh::ocean_stress<double>(g, s, f, U, V, st),
                      vh::ocean_stress<float>(g, s, f, U, V, st));
+}
+
+extern "C" int vh_freedrift_solver(const vh_domain* d, const vh_settings* s, const vh_fields* f,
+                                   void* uIce, void* vIce, void* stream) {
+  VH_PRE(d, s, f)
+  return VH_DISPATCH(d, vh::free_drift<double>(g, s, f, uIce, vIce, st),
+                     vh::free_drift<float>(g, s, f, uIce, vIce, st));
 }
 
 extern "C" int vh_clean_up_advection(const vh_domain* d, const vh_settings* s, const vh_fields* f,

@@ -189,12 +189,28 @@ def evp_solver(state, n_substeps: int | None = None, tile: int | None = None):
     return tuple(out)
 
 
+def freedrift_solver(state):
+    """veris/freedrift_solver.py:5-62 -> (uIce, vIce): steady free drift."""
+    c = Ctx(state)
+    out = c.empty(2)
+    N.call("vh_freedrift_solver", *c.args(), *map(_p, out), c.stream)
+    return tuple(out)
+
+
 def IceVelocities(state, n_substeps: int | None = None):
-    """veris/dynsolver.py:87-102: the EVP branch (free drift is not provided)."""
+    """veris/dynsolver.py:87-102 -> (uIce, vIce, sigma1, sigma2, sigma12).
+
+    EVP when useEVP is set (it overrides free drift, as in the reference);
+    otherwise free drift with the state's stresses unchanged (the reference's
+    free-drift branch reads them from an undefined name)."""
     s = state.settings
-    if not s.useEVP:
-        raise NotImplementedError("only the EVP solver is provided (useEVP=True)")
-    return evp_solver(state, n_substeps)
+    if s.useEVP:
+        return evp_solver(state, n_substeps)
+    if s.useFreedrift:
+        vs = state.variables
+        u, v = freedrift_solver(state)
+        return u, v, vs.sigma1, vs.sigma2, vs.sigma12
+    raise ValueError("IceVelocities: neither useEVP nor useFreedrift is set")
 
 
 def OceanStressUV(state):
