@@ -343,10 +343,28 @@ int vh_evp_workspace_bytes(const vh_domain* d, size_t* bytes);
  * dispatch veris/dynsolver.py:87-102): n_substeps fused EVP substeps starting
  * from f->uIce, f->vIce, f->sigma1, f->sigma2, f->sigma12 (f->uIce, f->vIce are
  * also the previous-step velocities uIceNm1, vIceNm1); the final state is
- * written to the five outputs.  Non-adaptive EVP with noSlip only. */
+ * written to the five outputs.  Fixed or adaptive relaxation (useAdaptiveEVP),
+ * no-slip with or without secondOrderBC (single-substep tile shapes), or free
+ * slip with lateral drag (noSlip = 0; f->Fu, f->Fv when use_coastline). */
 int vh_evp_solver(const vh_domain* d, const vh_settings* s, const vh_fields* f,
                   int32_t n_substeps, void* uIce, void* vIce, void* sigma1, void* sigma2,
                   void* sigma12, void* workspace, size_t workspace_bytes, void* stream);
+
+/* Workspace bytes vh_evp_solver_residuals needs for this domain and count. */
+int vh_evp_residuals_workspace_bytes(const vh_domain* d, int32_t n_substeps, size_t* bytes);
+
+/* The subcycle of vh_evp_solver (same outputs, bit for bit) with the residual
+ * norms of computeEvpResidual (veris/evp_solver.py:192-230): per substep k,
+ * resSig[k] = sum over the interior of (alphaC d sig11)^2 + (alphaC d sig22)^2
+ * + (alphaZ d sigma12)^2 and resU[k] = sum of (betaU du)^2 + (betaV dv)^2, where
+ * d is the change over the substep, sig11/sig22 = (sigma1 +- sigma2)/2 (the
+ * reference reads the previous ones from undefined names) and alpha/beta are
+ * the fixed or the substep's adaptive relaxation parameters.  resU and resSig
+ * are device arrays of n_substeps FP64 values; the sums are FP64. */
+int vh_evp_solver_residuals(const vh_domain* d, const vh_settings* s, const vh_fields* f,
+                            int32_t n_substeps, void* uIce, void* vIce, void* sigma1,
+                            void* sigma2, void* sigma12, double* resU, double* resSig,
+                            void* workspace, size_t workspace_bytes, void* stream);
 
 /* One member of a batched ensemble: a padded (nx+4, ny+4) box starting at
  * element `offset` of every packed field. */

@@ -377,9 +377,12 @@ def evp_substep(state, u, v, uNm1, vNm1, s1, s2, s12, P=None, drag=None, trace=N
     fx = fx + 0.5 * (fv + im1(fv))
     fy = fy - 0.5 * (fu + jm1(fu))
     if s.useAdaptiveEVP:
-        betaU = 0.5 * (alphaC + im1(alphaC)) * s.recip_deltatDyn
-        betaV = 0.5 * (alphaC + jm1(alphaC)) * s.recip_deltatDyn
+        evpBetaU = 0.5 * (alphaC + im1(alphaC))
+        evpBetaV = 0.5 * (alphaC + jm1(alphaC))
+        betaU = evpBetaU * s.recip_deltatDyn
+        betaV = evpBetaV * s.recip_deltatDyn
     else:
+        evpBetaU = evpBetaV = s.evpBeta
         betaU = s.evpBeta * s.recip_deltatDyn
         betaV = betaU
     bp1U = betaU + s.recip_deltatDyn
@@ -402,8 +405,47 @@ def evp_substep(state, u, v, uNm1, vNm1, s1, s2, s12, P=None, drag=None, trace=N
     if trace is not None:
         trace.update(e11=e11, e22=e22, e12=e12, zeta=zeta, press=press, deltaC=dC,
                      cDrag=cd, cBot=cb, divX=divx, divY=divy, ForcingX=fx, ForcingY=fy,
-                     alphaC=alphaC, alphaZ=alphaZ, betaU=betaU, betaV=betaV)
+                     alphaC=alphaC, alphaZ=alphaZ, betaU=betaU, betaV=betaV,
+                     evpBetaU=evpBetaU, evpBetaV=evpBetaV)
     return un, vn, s1, s2, s12
+
+
+def evp_residuals(state, old, new, trace):
+    """Residual norms of one substep, ``veris/evp_solver.py:192-230``: the
+    relaxation-weighted changes of sig11 = (sigma1 + sigma2)/2, sig22 =
+    (sigma1 - sigma2)/2, sigma12 and of u, v, squared and summed over the
+    interior.  The reference reads the previous sig11/sig22 from names it
+    never defines; they are taken here as the previous iterate's
+    (sigma1 +- sigma2)/2, the quantities its stress divergence uses
+    (``:93-94``).  Returns (resSig, resU)."""
+    vs, s = _vs(state)
+    uo, vo, s1o, s2o, s12o = old
+    un, vn, s1n, s2n, s12n = new
+    aC, aZ = trace["alphaC"], trace["alphaZ"]
+    r11 = (0.5 * (s1n + s2n) - 0.5 * (s1o + s2o)) * aC * vs.iceMask
+    r22 = (0.5 * (s1n - s2n) - 0.5 * (s1o - s2o)) * aC * vs.iceMask
+    r12 = (s12n - s12o) * aZ
+    ru = vs.iceMaskU * (un - uo) * trace["evpBetaU"]
+    rv = vs.iceMaskV * (vn - vo) * trace["evpBetaV"]
+    inner = (slice(s.olx, -s.olx), slice(s.oly, -s.oly))
+    return float((r11**2 + r22**2 + r12**2)[inner].sum()), float((ru**2 + rv**2)[inner].sum())
+
+
+def evp_solver_residuals(state, n_substeps=None):
+    """``veris/evp_solver.py:257-319`` with computeEvpResidual: returns
+    (uIce, vIce, sigma1, sigma2, sigma12, resU, resSig), resU and resSig one
+    value per substep."""
+    vs, s = _vs(state)
+    n = s.nEVPsteps if n_substeps is None else n_substeps
+    cur = (vs.uIce, vs.vIce, vs.sigma1, vs.sigma2, vs.sigma12)
+    resU = np.zeros(n)
+    resSig = np.zeros(n)
+    for k in range(n):
+        tr = {}
+        new = evp_substep(state, *cur[:2], vs.uIce, vs.vIce, *cur[2:], trace=tr)
+        resSig[k], resU[k] = evp_residuals(state, cur, new, tr)
+        cur = new
+    return (*cur, resU, resSig)
 
 
 def evp_solver(state, n_substeps=None, P=None, drag=None):

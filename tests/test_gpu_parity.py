@@ -200,6 +200,29 @@ def test_freedrift_solver(cuda, kind):
     assert s1 is g.variables.sigma1 and s12 is g.variables.sigma12
 
 
+@pytest.mark.parametrize("variant", ["default", "adaptive", "side_drag", "second_order"])
+def test_evp_residuals(cuda, variant):
+    """computeEvpResidual: per-substep stress and velocity residual norms
+    against the oracle (FP64 sums over the interior in a different order:
+    1e-12 relative per substep); the subcycle itself equals vh_evp_solver's
+    bit for bit."""
+    kw = {"default": {}, "adaptive": {"useAdaptiveEVP": True},
+          "side_drag": {"noSlip": False}, "second_order": {"secondOrderBC": True}}[variant]
+    st = _spun_up(synth.config0(computeEvpResidual=True, **kw), cuda, n_spin=5)
+    g = st.to(cuda)
+    n = 23
+    want = O.evp_solver_residuals(st, n_substeps=n)
+    got = K.evp_solver_residuals(g, n_substeps=n)
+    plain = K.evp_solver(g, n_substeps=n)
+    for q, name in enumerate(EVP_NAMES):
+        assert_exact(f"residual run {name}", g2n(got[q]), want[q])
+        assert_exact(f"residual run vs plain {name}", g2n(got[q]), g2n(plain[q]))
+    for name, a, b in (("resU", got[5], want[5]), ("resSig", got[6], want[6])):
+        a = g2n(a)
+        assert a.shape == (n,) and np.all(b > 0), name
+        np.testing.assert_allclose(a, b, rtol=1e-12, atol=0, err_msg=name)
+
+
 def test_evp_basal_drag(cuda):
     """basalDragK2 != 0 switches on the exp/log soft-max: tolerance 1e-12."""
     st = synth.tiny(20, 14, basalDragK2=15.0)

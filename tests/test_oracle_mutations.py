@@ -35,7 +35,7 @@ MUTATIONS = {
     "shear 1/e^2": ("s12 = (s12 * (alphaZ - rev) + shear * rrev) * den2", "s12 = (s12 * (alphaZ - rev) + shear) * den2"),
     "adaptive alpha z average": ("alphaZ = 0.5 * (alphaZ + im1(alphaZ))", "alphaZ = 0.5 * (alphaZ + ip1(alphaZ))"),
     "adaptive z denominator": ("        den2 = 1.0 / alphaZ", "        den2 = 1.0 / alphaC"),
-    "adaptive beta u": ("betaU = 0.5 * (alphaC + im1(alphaC)) * s.recip_deltatDyn", "betaU = 0.5 * (alphaC + jm1(alphaC)) * s.recip_deltatDyn"),
+    "adaptive beta u": ("evpBetaU = 0.5 * (alphaC + im1(alphaC))", "evpBetaU = 0.5 * (alphaC + jm1(alphaC))"),
     "adaptive alpha mass": ("/ np.maximum(vs.SeaIceMassC, 1e-4)", "/ np.maximum(vs.SeaIceMassU, 1e-4)"),
     "adaptive alpha floor": ("alphaC = np.maximum(alphaC, s.aEVPalphaMin)", "alphaC = np.maximum(alphaC, 1.0)"),
     "adaptive c-factor": ("return s.deltatDyn * 4 * (np.pi * s.aEvpCoeff) ** 2", "return s.deltatDyn * 4 * (np.pi * s.aEvpCoeff)"),
@@ -59,6 +59,10 @@ MUTATIONS = {
     "free drift hemisphere": ("np.where(vs.fCori < 0, s.waterIceDrag_south, s.waterIceDrag)", "np.where(vs.fCori > 0, s.waterIceDrag_south, s.waterIceDrag)"),
     "free drift u average": ("u = 0.5 * (im1(uc) + uc) * vs.iceMaskU", "u = 0.5 * (ip1(uc) + uc) * vs.iceMaskU"),
     "free drift v ocean centre": ("voc = 0.5 * (vs.vOcean + jp1(vs.vOcean))", "voc = 0.5 * (vs.vOcean + jm1(vs.vOcean))"),
+    "residual sig22": ("r22 = (0.5 * (s1n - s2n) - 0.5 * (s1o - s2o))", "r22 = (0.5 * (s1n + s2n) - 0.5 * (s1o + s2o))"),
+    "residual z alpha": ("r12 = (s12n - s12o) * aZ", "r12 = (s12n - s12o) * aC"),
+    "residual beta factor": ('ru = vs.iceMaskU * (un - uo) * trace["evpBetaU"]', 'ru = vs.iceMaskU * (un - uo) * trace["betaU"]'),
+    "residual interior": ("inner = (slice(s.olx, -s.olx), slice(s.oly, -s.oly))", "inner = (slice(None), slice(None))"),
     "z-point average": ("z = c + im1(c)", "z = c + ip1(c)"),
     "Coriolis sign": ("fx = fx + 0.5 * (fv + im1(fv))", "fx = fx - 0.5 * (fv + im1(fv))"),
     "Coriolis u average": ("fu = vs.SeaIceMassC * vs.fCori * 0.5 * (u + ip1(u))", "fu = vs.SeaIceMassC * vs.fCori * 0.5 * (u + im1(u))"),

@@ -1241,3 +1241,75 @@ def test_ice_velocities_dispatch():
     fu, fv = O.freedrift_solver(st)
     np.testing.assert_array_equal(u, fu)
     assert s1 is st.variables.sigma1 and s12 is st.variables.sigma12
+
+
+def test_evp_residuals_of_geometric_stress_decay():
+    """At rest the stresses decay as sigma_k = sigma_0 r^k, r = 1 - 1/alpha, so
+    alpha (sigma_k - sigma_k-1) = -sigma_0 r^(k-1): the stress residual is
+    N r^(2(k-1)) ((s1^2 + s2^2)/2 + s12^2) over the N interior cells, and the
+    velocity residual is zero."""
+    st = _rest_state()
+    vs = st.variables
+    vs.SeaIceStrength = np.full(st.shape, 1e4)
+    s1, s2, s12 = 1500.0, -700.0, 300.0
+    vs.sigma1 = np.full(st.shape, s1)
+    vs.sigma2 = np.full(st.shape, s2)
+    vs.sigma12 = np.full(st.shape, s12)
+    k = 9
+    out = O.evp_solver_residuals(st, n_substeps=k)
+    resU, resSig = out[5], out[6]
+    r = 1 - 1 / st.settings.evpAlpha
+    N = st.settings.nx * st.settings.ny
+    want = N * r ** (2 * np.arange(k)) * (0.5 * (s1**2 + s2**2) + s12**2)
+    np.testing.assert_allclose(resSig, want, rtol=1e-12)
+    assert np.all(resU == 0)
+    for a, b in zip(out[:5], O.evp_solver(st, n_substeps=k)):
+        np.testing.assert_array_equal(a, b)
+
+
+def test_evp_residuals_of_free_drift_contraction():
+    """The free-drift recurrence u_k = u* + (u0 - u*) rho^k (no stress) gives
+    a velocity residual beta^2 ((u0 - u*)^2 Nu + (v0 - v*)^2 Nv) rho^(2(k-1))
+    (rho - 1)^2 with beta = evpBeta (not beta/dt), and no stress residual."""
+    st = _rest_state()
+    s = st.settings
+    vs = st.variables
+    m, tau, uo, vo, A = 900.0, 1e-3, 0.01, -0.004, 1.0
+    vs.WindForcingX = np.full(st.shape, tau)
+    vs.WindForcingY = np.full(st.shape, -tau / 2)
+    vs.uOcean = np.full(st.shape, uo)
+    vs.vOcean = np.full(st.shape, vo)
+    u0, v0 = 0.02, 0.015
+    vs.uIce = np.full(st.shape, u0)
+    vs.vIce = np.full(st.shape, v0)
+    dt, beta, C = s.deltatDyn, s.evpBeta, s.cDragMin
+    rho = (beta * m / dt) / (m * (beta + 1) / dt + C * A)
+    k = 7
+    out = O.evp_solver_residuals(st, n_substeps=k)
+    N = s.nx * s.ny
+    amp = 0.0
+    for x0, forcing, oc in ((u0, tau, uo), (v0, -tau / 2, vo)):
+        star = (m * x0 / dt + forcing + C * A * oc) / (m / dt + C * A)
+        amp += (x0 - star) ** 2
+    want = N * beta**2 * amp * rho ** (2 * np.arange(k)) * (rho - 1) ** 2
+    np.testing.assert_allclose(out[5], want, rtol=1e-9)
+    assert np.all(out[6] == 0)
+
+
+def test_adaptive_evp_residuals_use_the_substep_alphas():
+    """Adaptive relaxation at rest (one substep): sigma_1 = sigma_0 (1 - 1/alpha)
+    with the per-cell alphaC (c-points) and alphaZ (z-points), so weighing the
+    change by the same alphas gives alpha (sigma_1 - sigma_0) = -sigma_0 in
+    every cell, whatever the alphas are; a residual weighed by evpAlpha or by
+    the wrong point's alpha would not."""
+    st = _rest_state(useAdaptiveEVP=True, aEvpCoeff=0.5, aEVPalphaMin=5.0)
+    vs, s = st.variables, st.settings
+    r = np.random.default_rng(23)
+    P = r.uniform(2e3, 4e4, st.shape)
+    vs.SeaIceStrength = P
+    vs.SeaIceMassC = r.uniform(100.0, 3000.0, st.shape)
+    s1, s2, s12 = (r.normal(0, 1e3, st.shape) for _ in range(3))
+    vs.sigma1, vs.sigma2, vs.sigma12 = s1, s2, s12
+    out = O.evp_solver_residuals(st, n_substeps=1)
+    want = (0.25 * (s1 + s2) ** 2 + 0.25 * (s1 - s2) ** 2 + s12**2)[INNER].sum()
+    np.testing.assert_allclose(out[6][0], want, rtol=1e-11)

@@ -139,6 +139,8 @@ SIGNATURES = {
     "vh_ice_to_ocean": [_DP, _SP, _FP, _P, _P, _P, _P, ctypes.POINTER(IceToOceanOut), _P],
     "vh_ocean_stress_uv": [_DP, _SP, _FP, _P, _P, _P],
     "vh_freedrift_solver": [_DP, _SP, _FP, _P, _P, _P],
+    "vh_evp_residuals_workspace_bytes": [_DP, _I32, ctypes.POINTER(_SZ)],
+    "vh_evp_solver_residuals": [_DP, _SP, _FP, _I32] + [_P] * 7 + [_P, _SZ, _P],
     "vh_advection_workspace_bytes": [_DP, ctypes.POINTER(_SZ)],
     "vh_advection": [_DP, _SP, _FP, _P, _P, _P, _P, _SZ, _P],
     "vh_clean_up_advection": [_DP, _SP, _FP, _P, _P, _P, _P, _P, _P, _P],

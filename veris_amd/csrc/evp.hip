@@ -25,7 +25,17 @@ int pick_tile(const Geom& g, int req) {
 
 template <typename T, typename TD>
 void launch_substeps(int cfg, const EvpArgBlock<T>* ab, int nsub, const Geom& g, bool so,
-                     hipStream_t st) {
+                     bool res, hipStream_t st) {
+  if (res) {  // one residual-writing launch per substep
+    const int TI = evp_tile_extent_i(kEvpResidualCfg), TJ = evp_tile_extent_j(kEvpResidualCfg);
+    const int tiles_j = (g.NY + TJ - 1) / TJ;
+    const int n_tiles = ((g.NX + TI - 1) / TI) * tiles_j;
+    for (int l = 0; l < nsub; ++l) {
+      if (so) evp_launch_res<T, true>(ab, l, n_tiles, tiles_j, st);
+      else evp_launch_res<T, false>(ab, l, n_tiles, tiles_j, st);
+    }
+    return;
+  }
   const int K = kEvpTiles[cfg].k;
   int l = 0;
   for (int q = 0; q < nsub / K + nsub % K; ++q) {
@@ -37,7 +47,40 @@ void launch_substeps(int cfg, const EvpArgBlock<T>* ab, int nsub, const Geom& g,
   }
 }
 
+// Residual norms per substep from the per-workgroup partials [substep][tile][2]
+// (stress, velocity): one workgroup per substep, fixed summation order.
+__global__ __launch_bounds__(256) void evp_residual_reduce_kernel(const double* part, int n_tiles,
+                                                                  double* resU, double* resSig) {
+  __shared__ double s0[256], s1[256];
+  const int k = blockIdx.x, t = threadIdx.x;
+  const double* p = part + (size_t)k * n_tiles * 2;
+  double a0 = 0.0, a1 = 0.0;
+  for (int q = t; q < n_tiles; q += 256) {
+    a0 += p[2 * q];
+    a1 += p[2 * q + 1];
+  }
+  s0[t] = a0;
+  s1[t] = a1;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (t < w) {
+      s0[t] += s0[t + w];
+      s1[t] += s1[t + w];
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    resSig[k] = s0[0];
+    resU[k] = s1[0];
+  }
+}
+
 }  // namespace
+
+int evp_residual_tiles(const Geom& g) {
+  const int TI = evp_tile_extent_i(kEvpResidualCfg), TJ = evp_tile_extent_j(kEvpResidualCfg);
+  return ((g.NX + TI - 1) / TI) * ((g.NY + TJ - 1) / TJ);
+}
 
 // Subcycle of nsub substeps from the state `in` (u, v, sigma1, sigma2,
 // sigma12) to `out`; fl->uIce/vIce are the previous-step velocities of the
@@ -46,7 +89,7 @@ void launch_substeps(int cfg, const EvpArgBlock<T>* ab, int nsub, const Geom& g,
 template <typename T>
 int evp_run_core(const Geom& g, const vh_settings* s, const vh_fields* fl, const T* const in[5],
                  T* const out[5], int nsub, void* ws, size_t ws_bytes, int tile_req, int dtype,
-                 hipStream_t st) {
+                 hipStream_t st, double* res = nullptr) {
   const size_t fb = align256(field_elems(g) * sizeof(T));
   const size_t need = 5 * fb + align256(sizeof(EvpArgBlock<T>));
   VH_REQUIRE(ws != nullptr && ws_bytes >= need, VH_EWORKSPACE,
@@ -96,7 +139,8 @@ int evp_run_core(const Geom& g, const vh_settings* s, const vh_fields* fl, const
     ab.out[q] = outp[q];
     ab.alt[q] = (T*)((char*)ws + q * fb);
   }
-  const int cfg = pick_tile(g, tile_req);
+  const int cfg = res ? kEvpResidualCfg : pick_tile(g, tile_req);
+  ab.res = res;
   VH_REQUIRE(!ab.sc.sobc || kEvpTiles[cfg].k == 1, VH_EUNSUPPORTED,
              "secondOrderBC needs a single-substep tile shape (tile %d has k=%d)", cfg,
              kEvpTiles[cfg].k);
@@ -109,13 +153,14 @@ int evp_run_core(const Geom& g, const vh_settings* s, const vh_fields* fl, const
   VH_TRY_HIP(hipGetLastError());
   if (nsub >= 8 && graphs_enabled()) {
     char key[160];
-    snprintf(key, sizeof(key), "evp1/%d/%d/%d/%d/%d/%d/%d/%d/%d/%p", dtype, cfg, (int)so, nsub,
-             g.NX, g.NY, g.nx, g.fill_x, g.fill_y, (const void*)ab_dev);
-    int rc = graph_run(key, [&](hipStream_t cs) { launch_substeps<T, double>(cfg, ab_dev, nsub, g, so, cs); },
-                       st);
+    snprintf(key, sizeof(key), "evp1/%d/%d/%d/%d/%d/%d/%d/%d/%d/%d/%p", dtype, cfg, (int)so,
+             res ? 1 : 0, nsub, g.NX, g.NY, g.nx, g.fill_x, g.fill_y, (const void*)ab_dev);
+    int rc = graph_run(
+        key, [&](hipStream_t cs) { launch_substeps<T, double>(cfg, ab_dev, nsub, g, so, res, cs); },
+        st);
     if (rc) return rc;
   } else {
-    launch_substeps<T, double>(cfg, ab_dev, nsub, g, so, st);
+    launch_substeps<T, double>(cfg, ab_dev, nsub, g, so, res, st);
   }
   VH_TRY_HIP(hipGetLastError());
   return VH_OK;
@@ -123,10 +168,10 @@ int evp_run_core(const Geom& g, const vh_settings* s, const vh_fields* fl, const
 
 template int evp_run_core<double>(const Geom&, const vh_settings*, const vh_fields*,
                                   const double* const[5], double* const[5], int, void*, size_t,
-                                  int, int, hipStream_t);
+                                  int, int, hipStream_t, double*);
 template int evp_run_core<float>(const Geom&, const vh_settings*, const vh_fields*,
                                  const float* const[5], float* const[5], int, void*, size_t, int,
-                                 int, hipStream_t);
+                                 int, hipStream_t, double*);
 
 namespace {
 
@@ -138,6 +183,30 @@ int run_evp(const Geom& g, const vh_settings* s, const vh_fields* fl, int nsub, 
                     evp_ptr<T>(fl->sigma2), evp_ptr<T>(fl->sigma12)};
   T* const out[5] = {(T*)uo, (T*)vo, (T*)s1o, (T*)s2o, (T*)s12o};
   return evp_run_core<T>(g, s, fl, in, out, nsub, ws, ws_bytes, tile_req, dtype, st);
+}
+
+// Subcycle with residual norms: the workspace holds the plain subcycle's
+// workspace followed by the residual partials.
+template <typename T>
+int run_evp_res(const Geom& g, const vh_settings* s, const vh_fields* fl, int nsub, void* const o[5],
+                double* resU, double* resSig, void* ws, size_t ws_bytes, int dtype, hipStream_t st) {
+  VH_REQUIRE(nsub >= 1, VH_EINVAL, "residuals need n_substeps >= 1");
+  VH_REQUIRE(resU && resSig && resU != resSig, VH_EINVAL, "null or aliased residual outputs");
+  const size_t base = 5 * align256(field_elems(g) * sizeof(T)) + align256(sizeof(EvpArgBlock<T>));
+  const int nt = evp_residual_tiles(g);
+  const size_t need = base + (size_t)nsub * nt * 2 * sizeof(double);
+  VH_REQUIRE(ws != nullptr && ws_bytes >= need, VH_EWORKSPACE,
+             "evp residual workspace: need %zu bytes, got %zu", need, ws_bytes);
+  double* part = (double*)((char*)ws + base);
+  const T* in[5] = {evp_ptr<T>(fl->uIce), evp_ptr<T>(fl->vIce), evp_ptr<T>(fl->sigma1),
+                    evp_ptr<T>(fl->sigma2), evp_ptr<T>(fl->sigma12)};
+  T* const out[5] = {(T*)o[0], (T*)o[1], (T*)o[2], (T*)o[3], (T*)o[4]};
+  int rc = evp_run_core<T>(g, s, fl, in, out, nsub, ws, base, 0, dtype, st, part);
+  if (rc) return rc;
+  hipLaunchKernelGGL(evp_residual_reduce_kernel, dim3(nsub), dim3(256), 0, st, part, nt, resU,
+                     resSig);
+  VH_TRY_HIP(hipGetLastError());
+  return VH_OK;
 }
 
 }  // namespace
@@ -179,4 +248,33 @@ extern "C" int vh_evp_solver(const vh_domain* d, const vh_settings* s, const vh_
                                workspace_bytes, d->tile, d->dtype, st);
   return vh::run_evp<float>(g, s, f, n_substeps, uIce, vIce, sigma1, sigma2, sigma12, workspace,
                             workspace_bytes, d->tile, d->dtype, st);
+}
+
+extern "C" int vh_evp_residuals_workspace_bytes(const vh_domain* d, int32_t n_substeps,
+                                                size_t* bytes) {
+  int rc = vh_evp_workspace_bytes(d, bytes);
+  if (rc) return rc;
+  VH_REQUIRE(n_substeps >= 1, VH_EINVAL, "n_substeps=%d", n_substeps);
+  vh::Geom g;
+  vh::make_geom(d, &g);
+  *bytes += (size_t)n_substeps * vh::evp_residual_tiles(g) * 2 * sizeof(double);
+  return VH_OK;
+}
+
+extern "C" int vh_evp_solver_residuals(const vh_domain* d, const vh_settings* s,
+                                       const vh_fields* f, int32_t n_substeps, void* uIce,
+                                       void* vIce, void* sigma1, void* sigma2, void* sigma12,
+                                       double* resU, double* resSig, void* workspace,
+                                       size_t workspace_bytes, void* stream) {
+  vh::Geom g;
+  int rc = vh::make_geom(d, &g);
+  if (rc) return rc;
+  VH_REQUIRE(s != nullptr && f != nullptr, VH_EINVAL, "null settings or fields");
+  hipStream_t st = (hipStream_t)stream;
+  void* const o[5] = {uIce, vIce, sigma1, sigma2, sigma12};
+  if (d->dtype == VH_F64)
+    return vh::run_evp_res<double>(g, s, f, n_substeps, o, resU, resSig, workspace,
+                                   workspace_bytes, d->dtype, st);
+  return vh::run_evp_res<float>(g, s, f, n_substeps, o, resU, resSig, workspace, workspace_bytes,
+                                d->dtype, st);
 }

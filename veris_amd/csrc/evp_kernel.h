@@ -66,6 +66,7 @@ struct EvpScalars {
   double K2, U0sq, K1, cBasalStar;
   double evpcfac, alphaMin;  // adaptive EVP: deltatDyn * 4 * (pi * aEvpCoeff)^2, aEVPalphaMin
   double sdCoeff, sdU0;      // sideDragCoeff, sideDragU0
+  double alpha, beta;        // evpAlpha, evpBeta (residual weights of the fixed scheme)
   int kterms, turn, basal, adaptive;
   int sidedrag;  // noSlip = False: free-slip e12 and lateral drag in the denominators
   int coast;     // use_coastline: lateral drag form factors from Fu/Fv
@@ -111,6 +112,7 @@ struct EvpArgBlock {
   const MemberDesc* members;  // batched only
   int n_members;              // batched only
   int pad2;
+  double* res;  // residual partial sums [launch][tile][2] (residual launches only)
 };
 
 
@@ -155,6 +157,8 @@ inline void fill_evp_scalars(const vh_settings* s, int kterms, int basal, EvpSca
   sc.sidedrag = s->noSlip == 0.0 ? 1 : 0;
   sc.coast = s->use_coastline != 0.0 ? 1 : 0;
   sc.sobc = (s->noSlip != 0.0 && s->secondOrderBC != 0.0) ? 1 : 0;
+  sc.alpha = s->evpAlpha;
+  sc.beta = s->evpBeta;
   sc.sdCoeff = s->sideDragCoeff;
   sc.sdU0 = s->sideDragU0;
 }
@@ -263,7 +267,7 @@ __device__ inline int map_axis(int p, int N, int n, int filled) {
 }
 
 template <typename T, typename TD, int RI, int RJ, int K, bool BATCHED, typename A = T,
-          bool SO = false>
+          bool SO = false, bool RES = false>
 __global__ __launch_bounds__(RI* RJ) void evp_substep_kernel(const EvpArgBlock<T>* __restrict__ ab,
                                                              int which, int launch, int n_tiles,
                                                              int tiles_j, int per_xcd) {
@@ -361,6 +365,10 @@ __global__ __launch_bounds__(RI* RJ) void evp_substep_kernel(const EvpArgBlock<T
   }
   A cbc = (A)0;  // basal drag coefficient inputs are step constants: load lazily
   const A sg = sgn(fc);
+  // residual inputs (RES, one substep per launch): the iterate at the start of
+  // the substep and the relaxation weights (fixed, or this substep's adaptive ones)
+  const A r_s1o = s1, r_s2o = s2, r_s12o = s12, r_uo = u, r_vo = v;
+  A r_aC = (A)sc.alpha, r_aZ = (A)sc.alpha, r_bU = (A)sc.beta, r_bV = (A)sc.beta;
   // lateral drag form factors at this u- and v-point (veris/dynamics_routines.py:119-130)
   A sdmU = (A)0, sdmV = (A)0;
   if (sc.sidedrag && sc.coast) {
@@ -476,6 +484,7 @@ __global__ __launch_bounds__(RI* RJ) void evp_substep_kernel(const EvpArgBlock<T
         sAl[t] = aC;
         am1 = aC - (A)1;
         den1 = (A)1.0 / aC;
+        if constexpr (RES) r_aC = aC;
       }
       if (cs) {
         const TD pressD =
@@ -531,6 +540,7 @@ __global__ __launch_bounds__(RI* RJ) void evp_substep_kernel(const EvpArgBlock<T
         aZ = (A)0.5 * (aZ + (A)0.5 * (sAl[t - RJ] + sAl[t - RJ - 1]));
         am1z = aZ - (A)1;
         den2 = (A)1.0 / aZ;
+        if constexpr (RES) r_aZ = aZ;
       }
       s12 = ((s12 * am1z) + shear * (A)0.25) * den2;
       sQ[t] = s12 * dxV;   // sigma12*dxV (q is dead after C)
@@ -570,6 +580,10 @@ __global__ __launch_bounds__(RI* RJ) void evp_substep_kernel(const EvpArgBlock<T
         bV = ((A)0.5 * (sAl[t] + sAl[t - 1])) * (A)sc.rdt;
         bP1U = bU + (A)sc.rdt;
         bP1V = bV + (A)sc.rdt;
+        if constexpr (RES) {
+          r_bU = (A)0.5 * (sAl[t] + sAl[t - RJ]);
+          r_bV = (A)0.5 * (sAl[t] + sAl[t - 1]);
+        }
       }
       A denU = mU * bP1U + AW * ((cdU * (A)sc.cosWat) + (A)0.5 * (sCB[t] + sCB[t - RJ]));
       A denV = mV * bP1V + AS * ((cdV * (A)sc.cosWat) + (A)0.5 * (sCB[t] + sCB[t - 1]));
@@ -587,6 +601,42 @@ __global__ __launch_bounds__(RI* RJ) void evp_substep_kernel(const EvpArgBlock<T
           (MVc * (((bV * mV) * v + (mV * (A)sc.rdt) * vNm1 + fy) + divy)) / denV;
       u = un;  // other threads read sU/sV, never these registers
       v = vn;
+    }
+  }
+
+  // ---- residual norms of the substep (veris/evp_solver.py:192-230) over the
+  //      interior: per-workgroup partial sums, reduced per substep afterwards
+  if constexpr (RES && K == 1) {
+    double rsig = 0.0, rvel = 0.0;
+    if (own && i >= 2 && i < g.NX - 2 && j >= 2 && j < g.NY - 2) {
+      const A r11 = ((((A)0.5 * (s1 + s2)) - ((A)0.5 * (r_s1o + r_s2o))) * r_aC) * Mc;
+      const A r22 = ((((A)0.5 * (s1 - s2)) - ((A)0.5 * (r_s1o - r_s2o))) * r_aC) * Mc;
+      const A r12 = (s12 - r_s12o) * r_aZ;
+      const A ru = (MUc * (u - r_uo)) * r_bU;
+      const A rv = (MVc * (v - r_vo)) * r_bV;
+      rsig = ((double)r11 * (double)r11 + (double)r22 * (double)r22) + (double)r12 * (double)r12;
+      rvel = (double)ru * (double)ru + (double)rv * (double)rv;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+      rsig += __shfl_xor(rsig, o);
+      rvel += __shfl_xor(rvel, o);
+    }
+    constexpr int NW = NT / 64;
+    __shared__ double sRed[2 * NW];
+    if ((t & 63) == 0) {
+      sRed[2 * (t >> 6)] = rsig;
+      sRed[2 * (t >> 6) + 1] = rvel;
+    }
+    __syncthreads();
+    if (t == 0) {
+      double a0 = 0.0, a1 = 0.0;
+      for (int q = 0; q < NW; ++q) {
+        a0 += sRed[2 * q];
+        a1 += sRed[2 * q + 1];
+      }
+      double* dp = ab->res + ((size_t)launch * n_tiles + tile) * 2;
+      dp[0] = a0;
+      dp[1] = a1;
     }
   }
 
@@ -654,6 +704,18 @@ inline void evp_launch_so(int cfg, const EvpArgBlock<T>* ab, int which, int laun
     default: evp_launch_shape<T, TD, 16, 32, 1, BATCHED, A, true>(ab, which, launch, n_tiles, tiles_j, st); break;
   }
 }
+
+// Single-domain single-substep launch that also writes the substep's residual
+// partial sums (shape 16 x 32, the library's default single-substep shape).
+template <typename T, bool SO>
+inline void evp_launch_res(const EvpArgBlock<T>* ab, int launch, int n_tiles, int tiles_j,
+                           hipStream_t st) {
+  const int per_xcd = (n_tiles + 7) / 8;
+  hipLaunchKernelGGL((evp_substep_kernel<T, double, 16, 32, 1, false, T, SO, true>),
+                     dim3(per_xcd * 8), dim3(16 * 32), 0, st, ab, 0, launch, n_tiles, tiles_j,
+                     per_xcd);
+}
+constexpr int kEvpResidualCfg = 5;
 
 // One launch of shape `cfg`: tiles_j is used by single-domain launches, `which`
 // (0 main shape, 1 remainder shape) by batched ones.  T is the storage type

@@ -189,6 +189,23 @@ def evp_solver(state, n_substeps: int | None = None, tile: int | None = None):
     return tuple(out)
 
 
+def evp_solver_residuals(state, n_substeps: int | None = None):
+    """veris/evp_solver.py:257-319 with computeEvpResidual ->
+    (uIce, vIce, sigma1, sigma2, sigma12, resU, resSig): the same subcycle as
+    :func:`evp_solver` (bit for bit) plus the per-substep residual norms as
+    FP64 device tensors of length n_substeps."""
+    c = Ctx(state)
+    n = int(state.settings.nEVPsteps if n_substeps is None else n_substeps)
+    out = c.empty(5)
+    res = torch.empty((2, n), dtype=torch.float64, device=c.device)
+    nbytes = ctypes.c_size_t(0)
+    N.call("vh_evp_residuals_workspace_bytes", ctypes.byref(c.dom), n, ctypes.byref(nbytes))
+    ws = torch.empty(max(nbytes.value, 256), dtype=torch.uint8, device=c.device)
+    N.call("vh_evp_solver_residuals", *c.args(), n, *map(_p, out), res[0].data_ptr(),
+           res[1].data_ptr(), _p(ws), ws.numel(), c.stream)
+    return (*out, res[0], res[1])
+
+
 def freedrift_solver(state):
     """veris/freedrift_solver.py:5-62 -> (uIce, vIce): steady free drift."""
     c = Ctx(state)
