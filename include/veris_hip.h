@@ -508,6 +508,23 @@ int vh_ice_to_ocean(const vh_domain* d, const vh_settings* s, const vh_fields* f
                     const void* forc_salt_surface_ice, const void* forc_salt_surface_res,
                     const vh_ice_to_ocean_out* out, void* stream);
 
+/* Outputs of vh_bulkf_formula_lanl, in the reference's return order. */
+typedef struct vh_bulk_flux_out {
+  void *flwupa, *flha, *fsha, *df0dT, *ust, *vst, *evp, *ssq, *dEvdT;
+} vh_bulk_flux_out;
+
+/* bulkf_formula_lanl (veris/heat_flux_MITgcm.py:5-123): open-ocean bulk
+ * fluxes (upward long wave, latent, sensible, their surface-temperature
+ * derivative, wind stress at c-points, evaporation, surface humidity and its
+ * derivative) from the wind uw/vw, air temperature ta [K], humidity qa and
+ * surface temperature tsf [K], with transfer coefficients from five
+ * Monin-Obukhov iterations.  `grav` is the host model's gravity (Veros'
+ * settings.grav).  Pointwise over the padded arrays; ocn_mask is not read
+ * (the reference uses it only for an array shape). */
+int vh_bulkf_formula_lanl(const vh_domain* d, const vh_settings* s, double grav, const void* uw,
+                          const void* vw, const void* ta, const void* qa, const void* tsf,
+                          const void* ocn_mask, const vh_bulk_flux_out* out, void* stream);
+
 /* freedrift_solver (veris/freedrift_solver.py:5-62): steady free-drift
  * velocities from WindForcingX/Y, hIceMean, fCori, uOcean/vOcean and the
  * u/v masks, on every padded cell (the reference does not fill overlaps

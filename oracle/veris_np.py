@@ -880,3 +880,63 @@ def ice_to_ocean(state, surface_taux, surface_tauy, forc_salt_surface_ice, forc_
     forc_temp = -vs.Qnet / (s.cpWater * s.rhoSea)
     forc_salt = forc_salt_surface_ice + forc_salt_surface_res
     return taux, tauy, forc_temp, forc_salt
+
+
+# --------------------------------------------------------------------------
+# forcing producer: open-ocean bulk fluxes
+# --------------------------------------------------------------------------
+
+
+def bulkf_formula_lanl(state, uw, vw, ta, qa, tsf, ocn_mask, grav=9.81):
+    """``veris/heat_flux_MITgcm.py:5-123``: open-ocean bulk fluxes with
+    stability-dependent transfer coefficients from five Monin-Obukhov
+    iterations.  ``grav`` is the host model's gravity (Veros' ``grav``).
+    Returns (flwupa, flha, fsha, df0dT, ust, vst, evp, ssq, dEvdT)."""
+    s = state.settings
+    ht, zref, zice = 2.0, 10.0, 0.0005
+    aln = np.log(ht / zref)
+    czol = zref * s.karman * grav
+    lath = np.ones_like(ocn_mask) * s.latvap
+    us = np.sqrt(uw * uw + vw * vw)
+    usm = np.maximum(us, 1.0)
+    t0 = ta * (1.0 + s.zvir * qa)
+    ssq = 3.797915 * np.exp(lath * (7.93252e-6 - 2.166847e-3 / tsf)) / 1013.0
+    deltap = ta - tsf + s.gamma_blk * ht
+    delq = qa - ssq
+    rdn = s.karman / np.log(zref / zice)
+    rhn = rdn
+    ren = rdn
+    ustar = rdn * usm
+    tstar = rhn * deltap
+    qstar = ren * delq
+    for _ in range(5):
+        huol = czol / ustar**2 * (tstar / t0 + qstar / (1.0 / s.zvir + qa))
+        huol = np.minimum(np.abs(huol), 10.0) * np.sign(huol)
+        stable = 0.5 + 0.5 * np.sign(huol)
+        xsq = np.maximum(np.sqrt(np.abs(1.0 - 16.0 * huol)), 1.0)
+        x = np.sqrt(xsq)
+        psimh = -5.0 * huol * stable + (1.0 - stable) * (
+            2.0 * np.log(0.5 * (1.0 + x)) + 2.0 * np.log(0.5 * (1.0 + xsq))
+            - 2.0 * np.arctan(x) + np.pi * 0.5)
+        psixh = -5.0 * huol * stable + (1.0 - stable) * (2.0 * np.log(0.5 * (1.0 + xsq)))
+        rd = rdn / (1.0 + rdn * (aln - psimh) / s.karman)
+        rh = rhn / (1.0 + rhn * (aln - psixh) / s.karman)
+        re = rh
+        ustar = rd * usm
+        qstar = re * delq
+        tstar = rh * deltap
+    csha = s.rhoAir * s.cpdair * us * rh * rd
+    clha = s.rhoAir * lath * us * re * rd
+    fsha = csha * deltap
+    flha = clha * delq
+    evp = -flha / lath
+    flwupa = s.ocean_emissivity * s.stefBoltz * tsf**4
+    dflwupdt = 4.0 * s.ocean_emissivity * s.stefBoltz * tsf**3
+    devdt = clha * ssq * 2.166847e-3 / (tsf * tsf)
+    dflhdt = -lath * devdt
+    dfshdt = -csha
+    df0dt = -dflwupdt + dfshdt + dflhdt
+    cdn = 2.7e-3 / usm + 0.142e-3 + 0.0764e-3 * usm
+    ust = s.rhoAir * cdn * us * uw
+    vst = s.rhoAir * cdn * us * vw
+    return flwupa, flha, fsha, df0dt, ust, vst, evp, ssq, devdt

@@ -63,6 +63,15 @@ MUTATIONS = {
     "residual z alpha": ("r12 = (s12n - s12o) * aZ", "r12 = (s12n - s12o) * aC"),
     "residual beta factor": ('ru = vs.iceMaskU * (un - uo) * trace["evpBetaU"]', 'ru = vs.iceMaskU * (un - uo) * trace["betaU"]'),
     "residual interior": ("inner = (slice(s.olx, -s.olx), slice(s.oly, -s.oly))", "inner = (slice(None), slice(None))"),
+    "bulk stable psi": ("psimh = -5.0 * huol * stable", "psimh = -6.0 * huol * stable"),
+    "bulk stability clip": ("huol = np.minimum(np.abs(huol), 10.0)", "huol = np.minimum(np.abs(huol), 5.0)"),
+    "bulk log-law sign": ("rd = rdn / (1.0 + rdn * (aln - psimh) / s.karman)", "rd = rdn / (1.0 + rdn * (aln + psimh) / s.karman)"),
+    "bulk heat psi": ("(1.0 - stable) * (2.0 * np.log(0.5 * (1.0 + xsq)))", "(1.0 - stable) * (np.log(0.5 * (1.0 + xsq)))"),
+    "bulk humidity buoyancy": ("qstar / (1.0 / s.zvir + qa)", "qstar / (s.zvir + qa)"),
+    "bulk lapse rate": ("deltap = ta - tsf + s.gamma_blk * ht", "deltap = ta - tsf - s.gamma_blk * ht"),
+    "bulk evaporation derivative": ("devdt = clha * ssq * 2.166847e-3 / (tsf * tsf)", "devdt = clha * ssq * 2.166847e-3 / tsf"),
+    "bulk long-wave derivative": ("dflwupdt = 4.0 * s.ocean_emissivity", "dflwupdt = 3.0 * s.ocean_emissivity"),
+    "bulk unstable pi/2": ("- 2.0 * np.arctan(x) + np.pi * 0.5)", "- 2.0 * np.arctan(x) + np.pi * 0.25)"),
     "z-point average": ("z = c + im1(c)", "z = c + ip1(c)"),
     "Coriolis sign": ("fx = fx + 0.5 * (fv + im1(fv))", "fx = fx - 0.5 * (fv + im1(fv))"),
     "Coriolis u average": ("fu = vs.SeaIceMassC * vs.fCori * 0.5 * (u + ip1(u))", "fu = vs.SeaIceMassC * vs.fCori * 0.5 * (u + im1(u))"),

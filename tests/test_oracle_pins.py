@@ -1313,3 +1313,118 @@ def test_adaptive_evp_residuals_use_the_substep_alphas():
     out = O.evp_solver_residuals(st, n_substeps=1)
     want = (0.25 * (s1 + s2) ** 2 + 0.25 * (s1 - s2) ** 2 + s12**2)[INNER].sum()
     np.testing.assert_allclose(out[6][0], want, rtol=1e-11)
+
+
+def _bulk(**kw):
+    st = uniform_state(nx=4, ny=4)
+    args = {k: np.atleast_1d(np.asarray(v, dtype=np.float64)) for k, v in kw.items()}
+    n = max(a.size for a in args.values())
+    args = {k: np.broadcast_to(a, (n,)).copy() for k, a in args.items()}
+    out = O.bulkf_formula_lanl(st, args["uw"], args["vw"], args["ta"], args["qa"], args["tsf"],
+                               np.ones(n), grav=9.81)
+    return st.settings, args, dict(zip(("flwupa", "flha", "fsha", "df0dT", "ust", "vst", "evp",
+                                        "ssq", "dEvdT"), out))
+
+
+def test_bulk_fluxes_neutral_limit():
+    """Air temperature chosen so that the buoyancy flux vanishes (the virtual
+    potential temperature difference cancels the humidity term, within 1e-9
+    on either side): the stability functions vanish and the transfer
+    coefficients are the neutral log-law value kappa / ln(ht / z0) with
+    ht = 2 m, z0 = 0.5 mm, so fsha = rhoAir cp U rd^2 dtheta and flha =
+    rhoAir L U rd^2 dq."""
+    st = uniform_state(nx=4, ny=4)
+    s = st.settings
+    tsf = np.array([271.0, 285.0, 300.0, 300.0])
+    qa = np.array([2e-3, 6e-3, 1.5e-2, 1.8e-2])
+    uw = np.array([3.0, -7.0, 12.0, 0.6])
+    vw = np.array([4.0, 2.0, -5.0, 0.3])
+    ssq = 3.797915 * np.exp(s.latvap * (7.93252e-6 - 2.166847e-3 / tsf)) / 1013.0
+    D = (qa - ssq) / (1 / s.zvir + qa)
+    ta0 = (tsf - 2.0 * s.gamma_blk) / (1 + D * (1 + s.zvir * qa))
+    rd = s.karman / np.log(2.0 / 0.0005)
+    for side in (+1e-9, -1e-9):
+        _, a, o = _bulk(uw=uw, vw=vw, ta=ta0 * (1 + side), qa=qa, tsf=tsf)
+        U = np.hypot(uw, vw)
+        dth = a["ta"] - tsf + 2.0 * s.gamma_blk
+        np.testing.assert_allclose(o["fsha"], s.rhoAir * s.cpdair * U * rd**2 * dth, rtol=1e-5)
+        np.testing.assert_allclose(o["flha"], s.rhoAir * s.latvap * U * rd**2 * (qa - ssq),
+                                   rtol=1e-5)
+        np.testing.assert_allclose(o["evp"], -o["flha"] / s.latvap, rtol=1e-15)
+        np.testing.assert_allclose(o["ssq"], ssq, rtol=1e-15)
+
+
+@pytest.mark.parametrize("regime", ["stable", "unstable"])
+def test_bulk_fluxes_at_the_stability_clip(regime):
+    """A weak wind under a strong inversion (stable) or over much warmer water
+    (unstable) saturates |zeta| = 10 from the first iteration on, so the
+    transfer coefficients are fixed: psi_m = psi_h = -5 zeta when stable; with
+    x^2 = sqrt(1 + 160), psi_m = 2 ln((1+x)/2) + 2 ln((1+x^2)/2) - 2 atan x +
+    pi/2 and psi_h = 2 ln((1+x^2)/2) when unstable (the forms the reference
+    uses); rd = rdn / (1 + rdn (ln(ht/zref) - psi_m)/kappa), rh alike."""
+    st = uniform_state(nx=4, ny=4)
+    s = st.settings
+    tsf = np.array([275.0, 290.0])
+    ta = tsf + (25.0 if regime == "stable" else -45.0)
+    qa = np.array([3e-3, 8e-3])
+    uw, vw = np.array([0.3, -0.5]), np.array([0.4, 0.2])
+    _, a, o = _bulk(uw=uw, vw=vw, ta=ta, qa=qa, tsf=tsf)
+    rdn = s.karman / np.log(10.0 / 0.0005)
+    aln = np.log(2.0 / 10.0)
+    if regime == "stable":
+        psim = psih = -50.0
+    else:
+        xsq = np.sqrt(161.0)
+        x = np.sqrt(xsq)
+        psih = 2 * np.log(0.5 * (1 + xsq))
+        psim = 2 * np.log(0.5 * (1 + x)) + psih - 2 * np.arctan(x) + np.pi / 2
+    rd = rdn / (1 + rdn * (aln - psim) / s.karman)
+    rh = rdn / (1 + rdn * (aln - psih) / s.karman)
+    U = np.hypot(uw, vw)
+    dth = ta - tsf + 2.0 * s.gamma_blk
+    np.testing.assert_allclose(o["fsha"], s.rhoAir * s.cpdair * U * rh * rd * dth, rtol=1e-13)
+    np.testing.assert_allclose(o["flha"], s.rhoAir * s.latvap * U * rh * rd * (qa - o["ssq"]),
+                               rtol=1e-13)
+
+
+def test_bulk_flux_derivatives_against_finite_differences():
+    """dEvdT = clha dq_s/dT / L with clha = flha / (qa - q_s), and
+    df0dT = -d(flwupa)/dT - fsha/dtheta - L dEvdT, the surface-temperature
+    derivatives at fixed transfer coefficients, checked with central
+    differences of q_s(T) and of the upward long-wave flux."""
+    st = uniform_state(nx=4, ny=4)
+    s = st.settings
+    tsf = np.array([272.0, 281.0, 296.0])
+    base = dict(uw=np.array([5.0, -2.0, 9.0]), vw=np.array([1.0, 6.0, -3.0]),
+                ta=np.array([268.0, 283.0, 293.0]), qa=np.array([2e-3, 7e-3, 1.3e-2]))
+    _, _, o = _bulk(tsf=tsf, **base)
+    h = 1e-3
+    _, _, op = _bulk(tsf=tsf + h, **base)
+    _, _, om = _bulk(tsf=tsf - h, **base)
+    dqs = (op["ssq"] - om["ssq"]) / (2 * h)
+    dlw = (op["flwupa"] - om["flwupa"]) / (2 * h)
+    clha = o["flha"] / (base["qa"] - o["ssq"])
+    np.testing.assert_allclose(o["dEvdT"], clha * dqs / s.latvap, rtol=1e-7)
+    dth = base["ta"] - tsf + 2.0 * s.gamma_blk
+    np.testing.assert_allclose(o["df0dT"], -dlw - o["fsha"] / dth - s.latvap * o["dEvdT"],
+                               rtol=1e-7)
+
+
+def test_bulk_fluxes_against_published_values():
+    """Saturation specific humidity at 1013 hPa within 1% of 0.622 e_s/p at
+    0 C (e_s = 6.112 hPa) and 3% at 30 C (42.47 hPa); upward long-wave flux
+    scaling as T^4 (a doubled temperature gives 16x); the neutral drag
+    coefficient of Large & Pond / Large & Yeager, 2.7e-3/U + 1.42e-4 +
+    7.64e-5 U, giving 1.18e-3 at 10 m/s."""
+    st = uniform_state(nx=4, ny=4)
+    s = st.settings
+    _, _, o = _bulk(uw=[10.0, 0.0], vw=[0.0, 10.0], ta=[273.15, 303.15], qa=[3e-3, 2e-2],
+                    tsf=[273.15, 303.15])
+    np.testing.assert_allclose(o["ssq"][0], 0.622 * 6.112 / 1013.25, rtol=1e-2)
+    np.testing.assert_allclose(o["ssq"][1], 0.622 * 42.47 / 1013.25, rtol=3e-2)
+    _, _, o2 = _bulk(uw=[10.0, 0.0], vw=[0.0, 10.0], ta=[273.15, 303.15], qa=[3e-3, 2e-2],
+                     tsf=[546.3, 606.3])
+    np.testing.assert_allclose(o2["flwupa"] / o["flwupa"], 16.0, rtol=1e-12)
+    np.testing.assert_allclose(o["ust"][0], s.rhoAir * 1.176e-3 * 100.0, rtol=1e-12)
+    np.testing.assert_allclose(o["vst"][1], s.rhoAir * 1.176e-3 * 100.0, rtol=1e-12)
+    assert o["ust"][1] == 0 and o["vst"][0] == 0

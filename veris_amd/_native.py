@@ -95,6 +95,13 @@ class IceToOceanOut(ctypes.Structure):
     _fields_ = [(name, ctypes.c_void_p) for name in ICE_TO_OCEAN_NAMES]
 
 
+BULK_FLUX_NAMES = ("flwupa", "flha", "fsha", "df0dT", "ust", "vst", "evp", "ssq", "dEvdT")
+
+
+class BulkFluxOut(ctypes.Structure):
+    _fields_ = [(name, ctypes.c_void_p) for name in BULK_FLUX_NAMES]
+
+
 class Member(ctypes.Structure):
     _fields_ = [("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("cyclic_x", ctypes.c_int32),
                 ("fill_mode", ctypes.c_int32), ("offset", ctypes.c_int64)]
@@ -139,6 +146,8 @@ SIGNATURES = {
     "vh_ice_to_ocean": [_DP, _SP, _FP, _P, _P, _P, _P, ctypes.POINTER(IceToOceanOut), _P],
     "vh_ocean_stress_uv": [_DP, _SP, _FP, _P, _P, _P],
     "vh_freedrift_solver": [_DP, _SP, _FP, _P, _P, _P],
+    "vh_bulkf_formula_lanl": [_DP, _SP, ctypes.c_double] + [_P] * 6 +
+                             [ctypes.POINTER(BulkFluxOut), _P],
     "vh_evp_residuals_workspace_bytes": [_DP, _I32, ctypes.POINTER(_SZ)],
     "vh_evp_solver_residuals": [_DP, _SP, _FP, _I32] + [_P] * 7 + [_P, _SZ, _P],
     "vh_advection_workspace_bytes": [_DP, ctypes.POINTER(_SZ)],
