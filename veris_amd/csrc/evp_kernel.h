@@ -39,6 +39,8 @@
 // FP32 path keeps it in FP64.
 #pragma once
 
+#include <atomic>
+
 #include "vh_common.h"
 
 namespace vh {
@@ -278,389 +280,406 @@ __global__ __launch_bounds__(RI* RJ) void evp_substep_kernel(const EvpArgBlock<T
   __shared__ A sU[NT], sV[NT], sM[NT], sMU[NT], sMV[NT], sDU[NT], sDV[NT], sE12[NT], sQ[NT], sZ[NT];
   __shared__ A sS11[NT], sS22[NT], sCD[NT], sCB[NT], sFV[NT], sFU[NT], sAl[NT];
 
-  // tile of this workgroup: workgroups are dealt round-robin to the 8 XCDs,
-  // so give each XCD a contiguous run of tiles (neighbours share L2 lines)
+  // tiles of this workgroup: workgroups are dealt round-robin to the 8 XCDs,
+  // so each XCD owns a contiguous run of per_xcd tiles (neighbours share L2
+  // lines); the grid may hold fewer workgroups than tiles, each then loops
+  // over its XCD's run with a stride of the XCD's workgroup count
   const int bid = blockIdx.x;
-  const int tile = (bid & 7) * per_xcd + (bid >> 3);
-  if (tile >= n_tiles) return;
-  Geom g;
-  long long off = 0;
-  int i0, j0;
-  if (BATCHED) {
-    // member owning this tile: last m with tb[which] <= tile (uniform search)
-    const MemberDesc* ms = ab->members;
-    int lo = 0, hi = uniform(ab->n_members) - 1;
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (uniform(ms[mid].tb[which]) <= tile) lo = mid;
-      else hi = mid - 1;
-    }
-    lo = uniform(lo);
-    const MemberDesc* md = ms + lo;
-    g.NX = uniform(md->g.NX);
-    g.NY = uniform(md->g.NY);
-    g.nx = uniform(md->g.nx);
-    g.ny = uniform(md->g.ny);
-    g.ol = 2;
-    g.fill_x = uniform(md->g.fill_x);
-    g.fill_y = uniform(md->g.fill_y);
-    const long long mo = md->off;
-    off = ((long long)(unsigned)uniform((int)(mo >> 32)) << 32) | (unsigned)uniform((int)mo);
-    const int local = tile - uniform(md->tb[which]);
-    const int tj_m = (g.NY + TJ - 1) / TJ;
-    const int ti = local / tj_m;
-    i0 = ti * TI;
-    j0 = (local - ti * tj_m) * TJ;
-  } else {
-    g = ab->g;
-    const int ti = tile / tiles_j;
-    i0 = ti * TI;
-    j0 = (tile - ti * tiles_j) * TJ;
-  }
-  const int nl = ab->nlaunch;
-  const T* const* src = (launch == 0) ? ab->in
-                                      : (((nl - launch) % 2 == 0) ? (const T* const*)ab->out
-                                                                  : (const T* const*)ab->alt);
-  T* const* dst = ((nl - 1 - launch) % 2 == 0) ? ab->out : ab->alt;
-  const EvpConst<T>& f = ab->c;
-  const EvpScalars& sc = ab->sc;
-
-  const int t = threadIdx.x;
-  const int ar = t / RJ, br = t & (RJ - 1);  // region coordinates
-  const int i = i0 + ar - R, j = j0 + br - R;  // padded position
-  const int gi = map_axis<K>(i, g.NX, g.nx, g.fill_x);
-  const int gj = map_axis<K>(j, g.NY, g.ny, g.fill_y);
-  const int c = gi * g.NY + gj;  // member-local element index (every region cell is in bounds)
-
-  // final output cell of this thread
-  const bool own = (ar >= R) && (ar < R + TI) && (br >= R) && (br < R + TJ) && (i < g.NX) &&
-                   (j < g.NY);
-  const bool upd_any = (ar >= 2) && (ar < RI - 2) && (br >= 2) && (br < RJ - 2) &&
-                       (K > 1 || is_computed(g, i, j));
-
-  // ---- all global loads of this cell, issued together and unconditionally
-  A u = ldg(src[0] + off, c), v = ldg(src[1] + off, c);
-  const A Mc = ldg(f.M + off, c), MUc = ldg(f.MU + off, c), MVc = ldg(f.MV + off, c);
-  const A uoc = ldg(f.uOc + off, c), voc = ldg(f.vOc + off, c);
-  const A mInUc = ldg(f.mInU + off, c), mInVc = ldg(f.mInV + off, c);
-  const A rdyU = ldg(f.rdyU + off, c), rdxV = ldg(f.rdxV + off, c), rAz = ldg(f.rAz + off, c);
-  const A rdxU = ldg(f.rdxU + off, c), rdyV = ldg(f.rdyV + off, c), mInC = ldg(f.mInC + off, c);
-  const A rrA = ldg(f.rrA + off, c), P = ldg(f.P + off, c);
-  A s1 = ldg(src[2] + off, c), s2 = ldg(src[3] + off, c);
-  const A dyV = ldg(f.dyV + off, c), dxU = ldg(f.dxU + off, c), fc = ldg(f.fC + off, c);
-  const A mC = ldg(f.massC + off, c);
-  A s12 = ldg(src[4] + off, c);
-  const A dxV = ldg(f.dxV + off, c), dyU = ldg(f.dyU + off, c);
-  const A rrAu = ldg(f.rrAu + off, c), rrAv = ldg(f.rrAv + off, c);
-  const A mU = ldg(f.massU + off, c), mV = ldg(f.massV + off, c);
-  const A AW = ldg(f.AreaW + off, c), AS = ldg(f.AreaS + off, c);
-  const A wfx = ldg(f.WFX + off, c), wfy = ldg(f.WFY + off, c);
-  const A uNm1 = ldg(f.uNm1 + off, c), vNm1 = ldg(f.vNm1 + off, c);
-  A k1Z = 0, k2Z = 0, k1C = 0, k2C = 0;
-  if (sc.kterms) {
-    k1Z = ldg(f.k1Z + off, c);
-    k2Z = ldg(f.k2Z + off, c);
-    k1C = ldg(f.k1C + off, c);
-    k2C = ldg(f.k2C + off, c);
-  }
-  A cbc = (A)0;  // basal drag coefficient inputs are step constants: load lazily
-  const A sg = sgn(fc);
-  // residual inputs (RES, one substep per launch): the iterate at the start of
-  // the substep and the relaxation weights (fixed, or this substep's adaptive ones)
-  const A r_s1o = s1, r_s2o = s2, r_s12o = s12, r_uo = u, r_vo = v;
-  A r_aC = (A)sc.alpha, r_aZ = (A)sc.alpha, r_bU = (A)sc.beta, r_bV = (A)sc.beta;
-  // lateral drag form factors at this u- and v-point (veris/dynamics_routines.py:119-130)
-  A sdmU = (A)0, sdmV = (A)0;
-  if (sc.sidedrag && sc.coast) {
-    sdmU = ldg(f.Fu + off, c);
-    sdmV = ldg(f.Fv + off, c);
-  }
-
-#pragma unroll 1
-  for (int s = 0; s < K; ++s) {
-    // coordinates relative to the substep-s tile, and its extent
-    const int a = ar - 2 - 2 * s, b = br - 2 - 2 * s;
-    const int TIs = RI - 4 - 4 * s, TJs = RJ - 4 - 4 * s;
-    const bool zc = (a >= -1) && (b >= -1) && (a < TIs + 2) && (b < TJs + 2);
-    const bool cc = (a >= -1) && (b >= -1) && (a <= TIs) && (b <= TJs);
-    const bool cs = (a >= -1) && (b >= -1) && (a < TIs) && (b < TJs);
-    const bool zs = (a >= 0) && (b >= 0) && (a <= TIs) && (b <= TJs);
-    const bool upd = (a >= 0) && (b >= 0) && (a < TIs) && (b < TJs) && upd_any;
-    A spd = (A)0;  // ice speed at c (lateral drag only)
-
-    // ---- A
-    if (s > 0) __syncthreads();  // E of the previous substep still reads LDS
-    sU[t] = u;
-    sV[t] = v;
-    sM[t] = Mc;
-    sMU[t] = MUc;
-    sMV[t] = MVc;
-    sDU[t] = (u - uoc) * mInUc;
-    sDV[t] = (v - voc) * mInVc;
-    __syncthreads();
-
-    // ---- B: strain rate at z-points (veris/dynamics_routines.py:160-175)
-    if (zc) {
-      const A ujm = sU[t - 1], vim = sV[t - RJ];
-      const A dudy = (u - ujm) * rdyU;
-      const A uavz = (u + ujm) * (A)0.5;
-      const A dvdx = (v - vim) * rdxV;
-      const A vavz = (v + vim) * (A)0.5;
-      const A mz = (Mc * sM[t - RJ]) * (sM[t - 1] * sM[t - RJ - 1]);
-      A inner = dudy + dvdx;
-      if (sc.kterms) inner = (inner - k1Z * vavz) - k2Z * uavz;
-      A e12 = ((A)0.5 * inner) * mz;
-      if (!sc.sidedrag) {  // no-slip boundary term (veris/dynamics_routines.py:170-175)
-        const A hu = MUc - sMU[t - 1];
-        const A hv = MVc - sMV[t - RJ];
-        e12 = e12 + ((((A)2.0 * uavz) * rdyU) * hu + (((A)2.0 * vavz) * rdxV) * hv);
+  const int xcd = bid & 7, k0 = bid >> 3;
+  auto run_tile = [&](const int tile) {
+    Geom g;
+    long long off = 0;
+    int i0, j0;
+    if (BATCHED) {
+      // member owning this tile: last m with tb[which] <= tile (uniform search)
+      const MemberDesc* ms = ab->members;
+      int lo = 0, hi = uniform(ab->n_members) - 1;
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (uniform(ms[mid].tb[which]) <= tile) lo = mid;
+        else hi = mid - 1;
       }
-      if constexpr (SO && K == 1) {
-        // second-order boundary shear (veris/dynamics_routines.py:177-215): it
-        // reads u, v and the masks two cells away along the face normal, beyond
-        // the LDS region at its edge; with one substep per launch the launch
-        // input is the current state, so they are read from memory.  A separate
-        // instantiation keeps these registers out of the other variants.
-        const int cim2 = wrap(i - 2, g.NX) * g.NY + gj, cim1 = wrap(i - 1, g.NX) * g.NY + gj;
-        const int cip1 = wrap(i + 1, g.NX) * g.NY + gj;
-        const int cjm2 = gi * g.NY + wrap(j - 2, g.NY), cjm1 = gi * g.NY + wrap(j - 1, g.NY);
-        const int cjp1 = gi * g.NY + wrap(j + 1, g.NY);
-        const A uim2 = ldg(src[0] + off, cim2), uip1 = ldg(src[0] + off, cip1);
-        const A MUim2 = ldg(f.MU + off, cim2), MUim1 = ldg(f.MU + off, cim1);
-        const A MUip1 = ldg(f.MU + off, cip1);
-        const A vjm2 = ldg(src[1] + off, cjm2), vjp1 = ldg(src[1] + off, cjp1);
-        const A MVjm2 = ldg(f.MV + off, cjm2), MVjm1 = ldg(f.MV + off, cjm1);
-        const A MVjp1 = ldg(f.MV + off, cjp1);
-        A hu2 = (MUc - MUim1) / (A)3.0;
-        A hv2 = (MVc - MVjm1) / (A)3.0;
-        hu2 = hu2 * (MUim2 * MUim1 + MUip1 * MUc);
-        hv2 = hv2 * (MVjm2 * MVjm1 + MVjp1 * MVc);
-        const A tu = (rdyU * ((((A)6.0 * uavz) - uim2 * MUim1) - uip1 * MUc)) * hu2;
-        const A tv = (rdxV * ((((A)6.0 * vavz) - vjm2 * MVjm1) - vjp1 * MVc)) * hv2;
-        e12 = e12 + (A)0.5 * (tu + tv);
-      }
-      sE12[t] = e12;
-      sQ[t] = rAz * (e12 * e12);
-    }
-    if (sc.sidedrag && !sc.coast && upd) {  // land neighbours along the face normal
-      sdmU = (((A)2 - sMU[t - RJ]) - sMU[t + RJ]) * MUc;
-      sdmV = (((A)2 - sMV[t - 1]) - sMV[t + 1]) * MVc;
-    }
-    __syncthreads();
-
-    // ---- C: c-point rheology (veris/dynamics_routines.py:151-158, 220-268),
-    //         stresses (veris/evp_solver.py:79-94), drags (:108-110), Coriolis (:142-143)
-    if (cc) {
-      const A uip = sU[t + RJ], vjp = sV[t + 1];
-      const A dudx = (uip - u) * rdxU;
-      const A uave = (uip + u) * (A)0.5;
-      const A dvdy = (vjp - v) * rdyV;
-      const A vave = (vjp + v) * (A)0.5;
-      A e11 = dudx, e22 = dvdy;
-      if (sc.kterms) {
-        e11 = dudx + vave * k2C;
-        e22 = dvdy + uave * k1C;
-      }
-      e11 = e11 * mInC;
-      e22 = e22 * mInC;
-      const A qa = sQ[t] + sQ[t + RJ];
-      const A qb = sQ[t + 1] + sQ[t + RJ + 1];
-      const A e12c = ((A)0.25 * rrA) * (qa + qb);
-      if (sc.sidedrag) {  // ice speed at c (veris/dynamics_routines.py:109-111)
-        const A su = u + uip, sv = v + vjp;
-        spd = (A)0.5 * sqrt(su * su + sv * sv);
-      }
-      const A ep = e11 + e22, em = e11 - e22;
-      const TD dsq = (TD)ep * (TD)ep + (TD)sc.rpd * ((TD)em * (TD)em + (TD)4.0 * (TD)e12c);
-      const TD dC = sqrt(dsq);
-      const TD dreg = dC + (TD)sc.deltaMin;
-      const TD zeta = ((TD)0.5 * ((TD)P * (TD)sc.tens1p)) / dreg;
-      sZ[t] = (A)zeta;
-      // relaxation parameter at c (veris/evp_solver.py:65-73 when adaptive)
-      A am1 = (A)sc.alphaM1, den1 = (A)sc.den1;
-      if (sc.adaptive) {
-        A aC = sqrt((((A)zeta * (A)sc.evpcfac) / fmax(mC, (A)1e-4)) * rrA) * Mc;
-        aC = fmax(aC, (A)sc.alphaMin);
-        sAl[t] = aC;
-        am1 = aC - (A)1;
-        den1 = (A)1.0 / aC;
-        if constexpr (RES) r_aC = aC;
-      }
-      if (cs) {
-        const TD pressD =
-            (((TD)P * (TD)sc.pr1m) + ((((TD)2.0 * zeta) * dC) * (TD)sc.pr) / (TD)sc.tens1p) *
-            (TD)sc.tens1m;
-        const A zt = (A)zeta, press = (A)pressD;
-        const A divg = (((A)2 * zt) * ep) - press;
-        const A tens = ((A)2 * zt) * em;
-        s1 = (((s1 * am1) + divg) * den1) * Mc;
-        s2 = (((s2 * am1) + tens * (A)0.25) * den1) * Mc;
-        sS11[t] = ((A)0.5 * (s1 + s2)) * dyV;
-        sS22[t] = ((A)0.5 * (s1 - s2)) * dxU;
-        const A D = (A)(fc < (A)0 ? sc.cwS : sc.cwN) * (A)sc.rhoSea;
-        const A su = sDU[t] + sDU[t + RJ], sv = sDV[t] + sDV[t + 1];
-        const A w = (A)0.25 * (su * su + sv * sv);
-        const A cd = ((D * D) * w > (A)sc.cDragMin2) ? D * sqrt(w) : (A)sc.cDragMin;
-        sCD[t] = cd * Mc;
-        if (sc.basal) {
-          const int cip = map_axis<K>(i + 1, g.NX, g.nx, g.fill_x) * g.NY + gj;
-          const int cjp = gi * g.NY + map_axis<K>(j + 1, g.NY, g.ny, g.fill_y);
-          const A um = u * mInUc, umip = uip * ldg(f.mInU + off, cip);
-          const A vm = v * mInVc, vmjp = vjp * ldg(f.mInV + off, cjp);
-          A tb = (A)0.25 * (((um * um + umip * umip) + vm * vm) + vmjp * vmjp);
-          tb = (A)sc.K2 / sqrt(tb + (A)sc.U0sq);
-          const A Ar = ldg(f.Area + off, c);
-          const A hc = (fabs(ldg(f.Rlow + off, c)) * Ar) / (A)sc.K1;
-          const A val = ((tb * log(exp((A)10.0 * (ldg(f.hIce + off, c) - hc)) + (A)1.0)) * (A)0.1) *
-                        exp((A)(-sc.cBasalStar) * ((A)1.0 - Ar));
-          cbc = Ar > (A)0.01 ? val : (A)0;
-        }
-        sCB[t] = cbc;
-        const A mf = (mC * fc) * (A)0.5;
-        sFV[t] = mf * (v + vjp);
-        sFU[t] = mf * (u + uip);
-      }
-    }
-    if (sc.turn) {  // ocean velocity for the turning term (sMU/sMV are free after B)
-      sMU[t] = uoc;
-      sMV[t] = voc;
-    }
-    __syncthreads();
-
-    // ---- D: sigma12 on z-points (veris/averaging.py:4-19, veris/evp_solver.py:86,103)
-    if (sc.sidedrag && cc) sDV[t] = spd;  // dv is dead after C
-    if (zs) {
-      const A n = (Mc + sM[t - RJ]) + (sM[t - 1] + sM[t - RJ - 1]);
-      const A nrm = n > (A)0 ? (A)1.0 / n : (A)0.0;
-      const A zz = nrm * ((sZ[t] + sZ[t - RJ]) + (sZ[t - 1] + sZ[t - RJ - 1]));
-      const A shear = ((A)2 * zz) * sE12[t];
-      A am1z = (A)sc.alphaM1, den2 = (A)sc.den2;
-      if (sc.adaptive) {  // veris/evp_solver.py:97-100
-        A aZ = (A)0.5 * (sAl[t] + sAl[t - 1]);
-        aZ = (A)0.5 * (aZ + (A)0.5 * (sAl[t - RJ] + sAl[t - RJ - 1]));
-        am1z = aZ - (A)1;
-        den2 = (A)1.0 / aZ;
-        if constexpr (RES) r_aZ = aZ;
-      }
-      s12 = ((s12 * am1z) + shear * (A)0.25) * den2;
-      sQ[t] = s12 * dxV;   // sigma12*dxV (q is dead after C)
-      sDU[t] = s12 * dyU;  // sigma12*dyU (du is dead after C)
-    }
-    __syncthreads();
-
-    // ---- E: stress divergence (veris/dynamics_routines.py:284-302), forcing
-    //         (veris/evp_solver.py:112-145) and momentum update (:152-187)
-    if (upd) {
-      const A divx = (((sS11[t] - sS11[t - RJ]) - sQ[t]) + sQ[t + 1]) * rrAu;
-      const A divy = (((sS22[t] - sS22[t - 1]) - sDU[t]) + sDU[t + RJ]) * rrAv;
-      const A cdc = sCD[t];
-      const A cdU = (A)0.5 * (cdc + sCD[t - RJ]);
-      const A cdV = (A)0.5 * (cdc + sCD[t - 1]);
-      A inx = (cdU * (A)sc.cosWat) * uoc;
-      A iny = (cdV * (A)sc.cosWat) * voc;
-      if (sc.turn) {
-        const A lmU = mU != (A)0 ? (A)1 : mU;
-        const A lmV = mV != (A)0 ? (A)1 : mV;
-        const A X = (cdc * (A)0.5) * ((uoc - u) + (sMU[t + 1] - sU[t + 1])) +
-                    (sCD[t - RJ] * (A)0.5) *
-                        ((sMU[t - RJ] - sU[t - RJ]) + (sMU[t - RJ + 1] - sU[t - RJ + 1]));
-        const A Y = (cdc * (A)0.5) * ((voc - v) + (sMV[t + RJ] - sV[t + RJ])) +
-                    (sCD[t - 1] * (A)0.5) *
-                        ((sMV[t - 1] - sV[t - 1]) + (sMV[t + RJ - 1] - sV[t + RJ - 1]));
-        inx = inx - (((sg * (A)sc.sinWat) * (A)0.5) * X) * lmU;
-        iny = iny + (((sg * (A)sc.sinWat) * (A)0.5) * Y) * lmV;
-      }
-      A fx = wfx + inx * AW;
-      A fy = wfy + iny * AS;
-      fx = fx + (A)0.5 * (sFV[t] + sFV[t - RJ]);
-      fy = fy - (A)0.5 * (sFU[t] + sFU[t - 1]);
-      A bU = (A)sc.betaU, bV = (A)sc.betaU, bP1U = (A)sc.betaP1, bP1V = (A)sc.betaP1;
-      if (sc.adaptive) {  // veris/evp_solver.py:148-155
-        bU = ((A)0.5 * (sAl[t] + sAl[t - RJ])) * (A)sc.rdt;
-        bV = ((A)0.5 * (sAl[t] + sAl[t - 1])) * (A)sc.rdt;
-        bP1U = bU + (A)sc.rdt;
-        bP1V = bV + (A)sc.rdt;
-        if constexpr (RES) {
-          r_bU = (A)0.5 * (sAl[t] + sAl[t - RJ]);
-          r_bV = (A)0.5 * (sAl[t] + sAl[t - 1]);
-        }
-      }
-      A denU = mU * bP1U + AW * ((cdU * (A)sc.cosWat) + (A)0.5 * (sCB[t] + sCB[t - RJ]));
-      A denV = mV * bP1V + AS * ((cdV * (A)sc.cosWat) + (A)0.5 * (sCB[t] + sCB[t - 1]));
-      denU = denU == (A)0 ? (A)1 : denU;
-      denV = denV == (A)0 ? (A)1 : denV;
-      if (sc.sidedrag) {  // lateral drag (veris/dynamics_routines.py:113-138, evp_solver.py:169-175)
-        const A spU = AW > (A)0 ? (A)0.5 * (sDV[t] + sDV[t - RJ]) : (A)0;
-        const A spV = AS > (A)0 ? (A)0.5 * (sDV[t] + sDV[t - 1]) : (A)0;
-        denU = denU + ((mU * (A)sc.sdCoeff) * sdmU) / (spU + (A)sc.sdU0);
-        denV = denV + ((mV * (A)sc.sdCoeff) * sdmV) / (spV + (A)sc.sdU0);
-      }
-      const A un =
-          (MUc * (((bU * mU) * u + (mU * (A)sc.rdt) * uNm1 + fx) + divx)) / denU;
-      const A vn =
-          (MVc * (((bV * mV) * v + (mV * (A)sc.rdt) * vNm1 + fy) + divy)) / denV;
-      u = un;  // other threads read sU/sV, never these registers
-      v = vn;
-    }
-  }
-
-  // ---- residual norms of the substep (veris/evp_solver.py:192-230) over the
-  //      interior: per-workgroup partial sums, reduced per substep afterwards
-  if constexpr (RES && K == 1) {
-    double rsig = 0.0, rvel = 0.0;
-    if (own && i >= 2 && i < g.NX - 2 && j >= 2 && j < g.NY - 2) {
-      const A r11 = ((((A)0.5 * (s1 + s2)) - ((A)0.5 * (r_s1o + r_s2o))) * r_aC) * Mc;
-      const A r22 = ((((A)0.5 * (s1 - s2)) - ((A)0.5 * (r_s1o - r_s2o))) * r_aC) * Mc;
-      const A r12 = (s12 - r_s12o) * r_aZ;
-      const A ru = (MUc * (u - r_uo)) * r_bU;
-      const A rv = (MVc * (v - r_vo)) * r_bV;
-      rsig = ((double)r11 * (double)r11 + (double)r22 * (double)r22) + (double)r12 * (double)r12;
-      rvel = (double)ru * (double)ru + (double)rv * (double)rv;
-    }
-    for (int o = 32; o > 0; o >>= 1) {
-      rsig += __shfl_xor(rsig, o);
-      rvel += __shfl_xor(rvel, o);
-    }
-    constexpr int NW = NT / 64;
-    __shared__ double sRed[2 * NW];
-    if ((t & 63) == 0) {
-      sRed[2 * (t >> 6)] = rsig;
-      sRed[2 * (t >> 6) + 1] = rvel;
-    }
-    __syncthreads();
-    if (t == 0) {
-      double a0 = 0.0, a1 = 0.0;
-      for (int q = 0; q < NW; ++q) {
-        a0 += sRed[2 * q];
-        a1 += sRed[2 * q + 1];
-      }
-      double* dp = ab->res + ((size_t)launch * n_tiles + tile) * 2;
-      dp[0] = a0;
-      dp[1] = a1;
-    }
-  }
-
-  // ---- results of the launch, at the cell's own padded position
-  if (own) {
-    const int cp = i * g.NY + j;
-    if (K == 1) {
-      if (upd_any) {
-        store_filled(g, GW(dst[0] + off), i, j, u);
-        store_filled(g, GW(dst[1] + off), i, j, v);
-      }
+      lo = uniform(lo);
+      const MemberDesc* md = ms + lo;
+      g.NX = uniform(md->g.NX);
+      g.NY = uniform(md->g.NY);
+      g.nx = uniform(md->g.nx);
+      g.ny = uniform(md->g.ny);
+      g.ol = 2;
+      g.fill_x = uniform(md->g.fill_x);
+      g.fill_y = uniform(md->g.fill_y);
+      const long long mo = md->off;
+      off = ((long long)(unsigned)uniform((int)(mo >> 32)) << 32) | (unsigned)uniform((int)mo);
+      const int local = tile - uniform(md->tb[which]);
+      const int tj_m = (g.NY + TJ - 1) / TJ;
+      const int ti = local / tj_m;
+      i0 = ti * TI;
+      j0 = (local - ti * tj_m) * TJ;
     } else {
-      stg(dst[0] + off, cp, (T)u);
-      stg(dst[1] + off, cp, (T)v);
+      g = ab->g;
+      const int ti = tile / tiles_j;
+      i0 = ti * TI;
+      j0 = (tile - ti * tiles_j) * TJ;
     }
-    stg(dst[2] + off, cp, (T)s1);
-    stg(dst[3] + off, cp, (T)s2);
-    stg(dst[4] + off, cp, (T)s12);
+    const int nl = ab->nlaunch;
+    const T* const* src = (launch == 0) ? ab->in
+                                        : (((nl - launch) % 2 == 0) ? (const T* const*)ab->out
+                                                                    : (const T* const*)ab->alt);
+    T* const* dst = ((nl - 1 - launch) % 2 == 0) ? ab->out : ab->alt;
+    const EvpConst<T>& f = ab->c;
+    const EvpScalars& sc = ab->sc;
+
+    const int t = threadIdx.x;
+    const int ar = t / RJ, br = t & (RJ - 1);  // region coordinates
+    const int i = i0 + ar - R, j = j0 + br - R;  // padded position
+    const int gi = map_axis<K>(i, g.NX, g.nx, g.fill_x);
+    const int gj = map_axis<K>(j, g.NY, g.ny, g.fill_y);
+    const int c = gi * g.NY + gj;  // member-local element index (every region cell is in bounds)
+
+    // final output cell of this thread
+    const bool own = (ar >= R) && (ar < R + TI) && (br >= R) && (br < R + TJ) && (i < g.NX) &&
+                     (j < g.NY);
+    const bool upd_any = (ar >= 2) && (ar < RI - 2) && (br >= 2) && (br < RJ - 2) &&
+                         (K > 1 || is_computed(g, i, j));
+
+    // ---- all global loads of this cell, issued together and unconditionally
+    A u = ldg(src[0] + off, c), v = ldg(src[1] + off, c);
+    const A Mc = ldg(f.M + off, c), MUc = ldg(f.MU + off, c), MVc = ldg(f.MV + off, c);
+    const A uoc = ldg(f.uOc + off, c), voc = ldg(f.vOc + off, c);
+    const A mInUc = ldg(f.mInU + off, c), mInVc = ldg(f.mInV + off, c);
+    const A rdyU = ldg(f.rdyU + off, c), rdxV = ldg(f.rdxV + off, c), rAz = ldg(f.rAz + off, c);
+    const A rdxU = ldg(f.rdxU + off, c), rdyV = ldg(f.rdyV + off, c), mInC = ldg(f.mInC + off, c);
+    const A rrA = ldg(f.rrA + off, c), P = ldg(f.P + off, c);
+    A s1 = ldg(src[2] + off, c), s2 = ldg(src[3] + off, c);
+    const A dyV = ldg(f.dyV + off, c), dxU = ldg(f.dxU + off, c), fc = ldg(f.fC + off, c);
+    const A mC = ldg(f.massC + off, c);
+    A s12 = ldg(src[4] + off, c);
+    const A dxV = ldg(f.dxV + off, c), dyU = ldg(f.dyU + off, c);
+    const A rrAu = ldg(f.rrAu + off, c), rrAv = ldg(f.rrAv + off, c);
+    const A mU = ldg(f.massU + off, c), mV = ldg(f.massV + off, c);
+    const A AW = ldg(f.AreaW + off, c), AS = ldg(f.AreaS + off, c);
+    const A wfx = ldg(f.WFX + off, c), wfy = ldg(f.WFY + off, c);
+    const A uNm1 = ldg(f.uNm1 + off, c), vNm1 = ldg(f.vNm1 + off, c);
+    A k1Z = 0, k2Z = 0, k1C = 0, k2C = 0;
+    if (sc.kterms) {
+      k1Z = ldg(f.k1Z + off, c);
+      k2Z = ldg(f.k2Z + off, c);
+      k1C = ldg(f.k1C + off, c);
+      k2C = ldg(f.k2C + off, c);
+    }
+    A cbc = (A)0;  // basal drag coefficient inputs are step constants: load lazily
+    const A sg = sgn(fc);
+    // residual inputs (RES, one substep per launch): the iterate at the start of
+    // the substep and the relaxation weights (fixed, or this substep's adaptive ones)
+    const A r_s1o = s1, r_s2o = s2, r_s12o = s12, r_uo = u, r_vo = v;
+    A r_aC = (A)sc.alpha, r_aZ = (A)sc.alpha, r_bU = (A)sc.beta, r_bV = (A)sc.beta;
+    // lateral drag form factors at this u- and v-point (veris/dynamics_routines.py:119-130)
+    A sdmU = (A)0, sdmV = (A)0;
+    if (sc.sidedrag && sc.coast) {
+      sdmU = ldg(f.Fu + off, c);
+      sdmV = ldg(f.Fv + off, c);
+    }
+
+  #pragma unroll 1
+    for (int s = 0; s < K; ++s) {
+      // coordinates relative to the substep-s tile, and its extent
+      const int a = ar - 2 - 2 * s, b = br - 2 - 2 * s;
+      const int TIs = RI - 4 - 4 * s, TJs = RJ - 4 - 4 * s;
+      const bool zc = (a >= -1) && (b >= -1) && (a < TIs + 2) && (b < TJs + 2);
+      const bool cc = (a >= -1) && (b >= -1) && (a <= TIs) && (b <= TJs);
+      const bool cs = (a >= -1) && (b >= -1) && (a < TIs) && (b < TJs);
+      const bool zs = (a >= 0) && (b >= 0) && (a <= TIs) && (b <= TJs);
+      const bool upd = (a >= 0) && (b >= 0) && (a < TIs) && (b < TJs) && upd_any;
+      A spd = (A)0;  // ice speed at c (lateral drag only)
+
+      // ---- A
+      if (s > 0) __syncthreads();  // E of the previous substep still reads LDS
+      sU[t] = u;
+      sV[t] = v;
+      sM[t] = Mc;
+      sMU[t] = MUc;
+      sMV[t] = MVc;
+      sDU[t] = (u - uoc) * mInUc;
+      sDV[t] = (v - voc) * mInVc;
+      __syncthreads();
+
+      // ---- B: strain rate at z-points (veris/dynamics_routines.py:160-175)
+      if (zc) {
+        const A ujm = sU[t - 1], vim = sV[t - RJ];
+        const A dudy = (u - ujm) * rdyU;
+        const A uavz = (u + ujm) * (A)0.5;
+        const A dvdx = (v - vim) * rdxV;
+        const A vavz = (v + vim) * (A)0.5;
+        const A mz = (Mc * sM[t - RJ]) * (sM[t - 1] * sM[t - RJ - 1]);
+        A inner = dudy + dvdx;
+        if (sc.kterms) inner = (inner - k1Z * vavz) - k2Z * uavz;
+        A e12 = ((A)0.5 * inner) * mz;
+        if (!sc.sidedrag) {  // no-slip boundary term (veris/dynamics_routines.py:170-175)
+          const A hu = MUc - sMU[t - 1];
+          const A hv = MVc - sMV[t - RJ];
+          e12 = e12 + ((((A)2.0 * uavz) * rdyU) * hu + (((A)2.0 * vavz) * rdxV) * hv);
+        }
+        if constexpr (SO && K == 1) {
+          // second-order boundary shear (veris/dynamics_routines.py:177-215): it
+          // reads u, v and the masks two cells away along the face normal, beyond
+          // the LDS region at its edge; with one substep per launch the launch
+          // input is the current state, so they are read from memory.  A separate
+          // instantiation keeps these registers out of the other variants.
+          const int cim2 = wrap(i - 2, g.NX) * g.NY + gj, cim1 = wrap(i - 1, g.NX) * g.NY + gj;
+          const int cip1 = wrap(i + 1, g.NX) * g.NY + gj;
+          const int cjm2 = gi * g.NY + wrap(j - 2, g.NY), cjm1 = gi * g.NY + wrap(j - 1, g.NY);
+          const int cjp1 = gi * g.NY + wrap(j + 1, g.NY);
+          const A uim2 = ldg(src[0] + off, cim2), uip1 = ldg(src[0] + off, cip1);
+          const A MUim2 = ldg(f.MU + off, cim2), MUim1 = ldg(f.MU + off, cim1);
+          const A MUip1 = ldg(f.MU + off, cip1);
+          const A vjm2 = ldg(src[1] + off, cjm2), vjp1 = ldg(src[1] + off, cjp1);
+          const A MVjm2 = ldg(f.MV + off, cjm2), MVjm1 = ldg(f.MV + off, cjm1);
+          const A MVjp1 = ldg(f.MV + off, cjp1);
+          A hu2 = (MUc - MUim1) / (A)3.0;
+          A hv2 = (MVc - MVjm1) / (A)3.0;
+          hu2 = hu2 * (MUim2 * MUim1 + MUip1 * MUc);
+          hv2 = hv2 * (MVjm2 * MVjm1 + MVjp1 * MVc);
+          const A tu = (rdyU * ((((A)6.0 * uavz) - uim2 * MUim1) - uip1 * MUc)) * hu2;
+          const A tv = (rdxV * ((((A)6.0 * vavz) - vjm2 * MVjm1) - vjp1 * MVc)) * hv2;
+          e12 = e12 + (A)0.5 * (tu + tv);
+        }
+        sE12[t] = e12;
+        sQ[t] = rAz * (e12 * e12);
+      }
+      if (sc.sidedrag && !sc.coast && upd) {  // land neighbours along the face normal
+        sdmU = (((A)2 - sMU[t - RJ]) - sMU[t + RJ]) * MUc;
+        sdmV = (((A)2 - sMV[t - 1]) - sMV[t + 1]) * MVc;
+      }
+      __syncthreads();
+
+      // ---- C: c-point rheology (veris/dynamics_routines.py:151-158, 220-268),
+      //         stresses (veris/evp_solver.py:79-94), drags (:108-110), Coriolis (:142-143)
+      if (cc) {
+        const A uip = sU[t + RJ], vjp = sV[t + 1];
+        const A dudx = (uip - u) * rdxU;
+        const A uave = (uip + u) * (A)0.5;
+        const A dvdy = (vjp - v) * rdyV;
+        const A vave = (vjp + v) * (A)0.5;
+        A e11 = dudx, e22 = dvdy;
+        if (sc.kterms) {
+          e11 = dudx + vave * k2C;
+          e22 = dvdy + uave * k1C;
+        }
+        e11 = e11 * mInC;
+        e22 = e22 * mInC;
+        const A qa = sQ[t] + sQ[t + RJ];
+        const A qb = sQ[t + 1] + sQ[t + RJ + 1];
+        const A e12c = ((A)0.25 * rrA) * (qa + qb);
+        if (sc.sidedrag) {  // ice speed at c (veris/dynamics_routines.py:109-111)
+          const A su = u + uip, sv = v + vjp;
+          spd = (A)0.5 * sqrt(su * su + sv * sv);
+        }
+        const A ep = e11 + e22, em = e11 - e22;
+        const TD dsq = (TD)ep * (TD)ep + (TD)sc.rpd * ((TD)em * (TD)em + (TD)4.0 * (TD)e12c);
+        const TD dC = sqrt(dsq);
+        const TD dreg = dC + (TD)sc.deltaMin;
+        const TD zeta = ((TD)0.5 * ((TD)P * (TD)sc.tens1p)) / dreg;
+        sZ[t] = (A)zeta;
+        // relaxation parameter at c (veris/evp_solver.py:65-73 when adaptive)
+        A am1 = (A)sc.alphaM1, den1 = (A)sc.den1;
+        if (sc.adaptive) {
+          A aC = sqrt((((A)zeta * (A)sc.evpcfac) / fmax(mC, (A)1e-4)) * rrA) * Mc;
+          aC = fmax(aC, (A)sc.alphaMin);
+          sAl[t] = aC;
+          am1 = aC - (A)1;
+          den1 = (A)1.0 / aC;
+          if constexpr (RES) r_aC = aC;
+        }
+        if (cs) {
+          const TD pressD =
+              (((TD)P * (TD)sc.pr1m) + ((((TD)2.0 * zeta) * dC) * (TD)sc.pr) / (TD)sc.tens1p) *
+              (TD)sc.tens1m;
+          const A zt = (A)zeta, press = (A)pressD;
+          const A divg = (((A)2 * zt) * ep) - press;
+          const A tens = ((A)2 * zt) * em;
+          s1 = (((s1 * am1) + divg) * den1) * Mc;
+          s2 = (((s2 * am1) + tens * (A)0.25) * den1) * Mc;
+          sS11[t] = ((A)0.5 * (s1 + s2)) * dyV;
+          sS22[t] = ((A)0.5 * (s1 - s2)) * dxU;
+          const A D = (A)(fc < (A)0 ? sc.cwS : sc.cwN) * (A)sc.rhoSea;
+          const A su = sDU[t] + sDU[t + RJ], sv = sDV[t] + sDV[t + 1];
+          const A w = (A)0.25 * (su * su + sv * sv);
+          const A cd = ((D * D) * w > (A)sc.cDragMin2) ? D * sqrt(w) : (A)sc.cDragMin;
+          sCD[t] = cd * Mc;
+          if (sc.basal) {
+            const int cip = map_axis<K>(i + 1, g.NX, g.nx, g.fill_x) * g.NY + gj;
+            const int cjp = gi * g.NY + map_axis<K>(j + 1, g.NY, g.ny, g.fill_y);
+            const A um = u * mInUc, umip = uip * ldg(f.mInU + off, cip);
+            const A vm = v * mInVc, vmjp = vjp * ldg(f.mInV + off, cjp);
+            A tb = (A)0.25 * (((um * um + umip * umip) + vm * vm) + vmjp * vmjp);
+            tb = (A)sc.K2 / sqrt(tb + (A)sc.U0sq);
+            const A Ar = ldg(f.Area + off, c);
+            const A hc = (fabs(ldg(f.Rlow + off, c)) * Ar) / (A)sc.K1;
+            const A val = ((tb * log(exp((A)10.0 * (ldg(f.hIce + off, c) - hc)) + (A)1.0)) * (A)0.1) *
+                          exp((A)(-sc.cBasalStar) * ((A)1.0 - Ar));
+            cbc = Ar > (A)0.01 ? val : (A)0;
+          }
+          sCB[t] = cbc;
+          const A mf = (mC * fc) * (A)0.5;
+          sFV[t] = mf * (v + vjp);
+          sFU[t] = mf * (u + uip);
+        }
+      }
+      if (sc.turn) {  // ocean velocity for the turning term (sMU/sMV are free after B)
+        sMU[t] = uoc;
+        sMV[t] = voc;
+      }
+      __syncthreads();
+
+      // ---- D: sigma12 on z-points (veris/averaging.py:4-19, veris/evp_solver.py:86,103)
+      if (sc.sidedrag && cc) sDV[t] = spd;  // dv is dead after C
+      if (zs) {
+        const A n = (Mc + sM[t - RJ]) + (sM[t - 1] + sM[t - RJ - 1]);
+        const A nrm = n > (A)0 ? (A)1.0 / n : (A)0.0;
+        const A zz = nrm * ((sZ[t] + sZ[t - RJ]) + (sZ[t - 1] + sZ[t - RJ - 1]));
+        const A shear = ((A)2 * zz) * sE12[t];
+        A am1z = (A)sc.alphaM1, den2 = (A)sc.den2;
+        if (sc.adaptive) {  // veris/evp_solver.py:97-100
+          A aZ = (A)0.5 * (sAl[t] + sAl[t - 1]);
+          aZ = (A)0.5 * (aZ + (A)0.5 * (sAl[t - RJ] + sAl[t - RJ - 1]));
+          am1z = aZ - (A)1;
+          den2 = (A)1.0 / aZ;
+          if constexpr (RES) r_aZ = aZ;
+        }
+        s12 = ((s12 * am1z) + shear * (A)0.25) * den2;
+        sQ[t] = s12 * dxV;   // sigma12*dxV (q is dead after C)
+        sDU[t] = s12 * dyU;  // sigma12*dyU (du is dead after C)
+      }
+      __syncthreads();
+
+      // ---- E: stress divergence (veris/dynamics_routines.py:284-302), forcing
+      //         (veris/evp_solver.py:112-145) and momentum update (:152-187)
+      if (upd) {
+        const A divx = (((sS11[t] - sS11[t - RJ]) - sQ[t]) + sQ[t + 1]) * rrAu;
+        const A divy = (((sS22[t] - sS22[t - 1]) - sDU[t]) + sDU[t + RJ]) * rrAv;
+        const A cdc = sCD[t];
+        const A cdU = (A)0.5 * (cdc + sCD[t - RJ]);
+        const A cdV = (A)0.5 * (cdc + sCD[t - 1]);
+        A inx = (cdU * (A)sc.cosWat) * uoc;
+        A iny = (cdV * (A)sc.cosWat) * voc;
+        if (sc.turn) {
+          const A lmU = mU != (A)0 ? (A)1 : mU;
+          const A lmV = mV != (A)0 ? (A)1 : mV;
+          const A X = (cdc * (A)0.5) * ((uoc - u) + (sMU[t + 1] - sU[t + 1])) +
+                      (sCD[t - RJ] * (A)0.5) *
+                          ((sMU[t - RJ] - sU[t - RJ]) + (sMU[t - RJ + 1] - sU[t - RJ + 1]));
+          const A Y = (cdc * (A)0.5) * ((voc - v) + (sMV[t + RJ] - sV[t + RJ])) +
+                      (sCD[t - 1] * (A)0.5) *
+                          ((sMV[t - 1] - sV[t - 1]) + (sMV[t + RJ - 1] - sV[t + RJ - 1]));
+          inx = inx - (((sg * (A)sc.sinWat) * (A)0.5) * X) * lmU;
+          iny = iny + (((sg * (A)sc.sinWat) * (A)0.5) * Y) * lmV;
+        }
+        A fx = wfx + inx * AW;
+        A fy = wfy + iny * AS;
+        fx = fx + (A)0.5 * (sFV[t] + sFV[t - RJ]);
+        fy = fy - (A)0.5 * (sFU[t] + sFU[t - 1]);
+        A bU = (A)sc.betaU, bV = (A)sc.betaU, bP1U = (A)sc.betaP1, bP1V = (A)sc.betaP1;
+        if (sc.adaptive) {  // veris/evp_solver.py:148-155
+          bU = ((A)0.5 * (sAl[t] + sAl[t - RJ])) * (A)sc.rdt;
+          bV = ((A)0.5 * (sAl[t] + sAl[t - 1])) * (A)sc.rdt;
+          bP1U = bU + (A)sc.rdt;
+          bP1V = bV + (A)sc.rdt;
+          if constexpr (RES) {
+            r_bU = (A)0.5 * (sAl[t] + sAl[t - RJ]);
+            r_bV = (A)0.5 * (sAl[t] + sAl[t - 1]);
+          }
+        }
+        A denU = mU * bP1U + AW * ((cdU * (A)sc.cosWat) + (A)0.5 * (sCB[t] + sCB[t - RJ]));
+        A denV = mV * bP1V + AS * ((cdV * (A)sc.cosWat) + (A)0.5 * (sCB[t] + sCB[t - 1]));
+        denU = denU == (A)0 ? (A)1 : denU;
+        denV = denV == (A)0 ? (A)1 : denV;
+        if (sc.sidedrag) {  // lateral drag (veris/dynamics_routines.py:113-138, evp_solver.py:169-175)
+          const A spU = AW > (A)0 ? (A)0.5 * (sDV[t] + sDV[t - RJ]) : (A)0;
+          const A spV = AS > (A)0 ? (A)0.5 * (sDV[t] + sDV[t - 1]) : (A)0;
+          denU = denU + ((mU * (A)sc.sdCoeff) * sdmU) / (spU + (A)sc.sdU0);
+          denV = denV + ((mV * (A)sc.sdCoeff) * sdmV) / (spV + (A)sc.sdU0);
+        }
+        const A un =
+            (MUc * (((bU * mU) * u + (mU * (A)sc.rdt) * uNm1 + fx) + divx)) / denU;
+        const A vn =
+            (MVc * (((bV * mV) * v + (mV * (A)sc.rdt) * vNm1 + fy) + divy)) / denV;
+        u = un;  // other threads read sU/sV, never these registers
+        v = vn;
+      }
+    }
+
+    // ---- residual norms of the substep (veris/evp_solver.py:192-230) over the
+    //      interior: per-workgroup partial sums, reduced per substep afterwards
+    if constexpr (RES && K == 1) {
+      double rsig = 0.0, rvel = 0.0;
+      if (own && i >= 2 && i < g.NX - 2 && j >= 2 && j < g.NY - 2) {
+        const A r11 = ((((A)0.5 * (s1 + s2)) - ((A)0.5 * (r_s1o + r_s2o))) * r_aC) * Mc;
+        const A r22 = ((((A)0.5 * (s1 - s2)) - ((A)0.5 * (r_s1o - r_s2o))) * r_aC) * Mc;
+        const A r12 = (s12 - r_s12o) * r_aZ;
+        const A ru = (MUc * (u - r_uo)) * r_bU;
+        const A rv = (MVc * (v - r_vo)) * r_bV;
+        rsig = ((double)r11 * (double)r11 + (double)r22 * (double)r22) + (double)r12 * (double)r12;
+        rvel = (double)ru * (double)ru + (double)rv * (double)rv;
+      }
+      for (int o = 32; o > 0; o >>= 1) {
+        rsig += __shfl_xor(rsig, o);
+        rvel += __shfl_xor(rvel, o);
+      }
+      constexpr int NW = NT / 64;
+      __shared__ double sRed[2 * NW];
+      if ((t & 63) == 0) {
+        sRed[2 * (t >> 6)] = rsig;
+        sRed[2 * (t >> 6) + 1] = rvel;
+      }
+      __syncthreads();
+      if (t == 0) {
+        double a0 = 0.0, a1 = 0.0;
+        for (int q = 0; q < NW; ++q) {
+          a0 += sRed[2 * q];
+          a1 += sRed[2 * q + 1];
+        }
+        double* dp = ab->res + ((size_t)launch * n_tiles + tile) * 2;
+        dp[0] = a0;
+        dp[1] = a1;
+      }
+    }
+
+    // ---- results of the launch, at the cell's own padded position
+    if (own) {
+      const int cp = i * g.NY + j;
+      if (K == 1) {
+        if (upd_any) {
+          store_filled(g, GW(dst[0] + off), i, j, u);
+          store_filled(g, GW(dst[1] + off), i, j, v);
+        }
+      } else {
+        stg(dst[0] + off, cp, (T)u);
+        stg(dst[1] + off, cp, (T)v);
+      }
+      stg(dst[2] + off, cp, (T)s1);
+      stg(dst[3] + off, cp, (T)s2);
+      stg(dst[4] + off, cp, (T)s12);
+    }
+  };
+  if constexpr (BATCHED) {
+    const int nwx = (int)(gridDim.x >> 3);
+    for (int k = k0; k < per_xcd; k += nwx) {
+      const int tile = xcd * per_xcd + k;
+      if (tile >= n_tiles) break;
+      if (k != k0) __syncthreads();  // the previous tile's last stage still reads LDS
+      run_tile(tile);
+    }
+  } else {
+    const int tile = xcd * per_xcd + k0;
+    if (tile < n_tiles) run_tile(tile);
   }
 }
 
 }  // namespace vh
 
 namespace vh {
+
+constexpr int kMaxDevices = 64;
 
 // Launch shapes: ri x rj threads, k substeps per launch (index 0 = default).
 struct TileCfg {
@@ -680,13 +699,39 @@ inline int evp_n_launches(int cfg, int nsub) {
   return nsub / kEvpTiles[cfg].k + nsub % kEvpTiles[cfg].k;
 }
 
+// Workgroups of a launch: one per tile, capped at what the device holds
+// resident at once (compute units x resident workgroups of `kernel`, cached
+// per device in `cache`), so a large batch runs as a grid that stays resident
+// and loops over its tiles instead of waiting on workgroup dispatch.
+template <typename F>
+inline int evp_grid(F kernel, int threads, int n_tiles, std::atomic<int>* cache) {
+  const int per_xcd = (n_tiles + 7) / 8;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return 8 * per_xcd;
+  int cap = cache[dev].load(std::memory_order_relaxed);
+  if (cap == 0) {
+    int ncu = 0, per_cu = 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, 0) != hipSuccess ||
+        ncu <= 0 || per_cu <= 0)
+      return 8 * per_xcd;
+    cap = ncu * per_cu;
+    cache[dev].store(cap, std::memory_order_relaxed);
+  }
+  const int cap_x = (cap + 7) / 8;
+  return 8 * (per_xcd < cap_x ? per_xcd : cap_x);
+}
+
 template <typename T, typename TD, int RI, int RJ, int K, bool BATCHED, typename A = T,
           bool SO = false>
 inline void evp_launch_shape(const EvpArgBlock<T>* ab, int which, int launch, int n_tiles,
                              int tiles_j, hipStream_t st) {
+  static std::atomic<int> cache[kMaxDevices];
   const int per_xcd = (n_tiles + 7) / 8;
-  hipLaunchKernelGGL((evp_substep_kernel<T, TD, RI, RJ, K, BATCHED, A, SO>), dim3(per_xcd * 8),
-                     dim3(RI * RJ), 0, st, ab, which, launch, n_tiles, tiles_j, per_xcd);
+  auto kernel = evp_substep_kernel<T, TD, RI, RJ, K, BATCHED, A, SO>;
+  const int grid = BATCHED ? evp_grid(kernel, RI * RJ, n_tiles, cache) : 8 * per_xcd;
+  hipLaunchKernelGGL(kernel, dim3(grid), dim3(RI * RJ), 0, st, ab, which, launch, n_tiles,
+                     tiles_j, per_xcd);
 }
 
 // Single-substep launch with the second-order no-slip boundary term (cfg must
