@@ -347,7 +347,16 @@ __global__ __launch_bounds__(RI* RJ) void evp_substep_kernel(const EvpArgBlock<T
     const A Mc = ldg(f.M + off, c), MUc = ldg(f.MU + off, c), MVc = ldg(f.MV + off, c);
     const A uoc = ldg(f.uOc + off, c), voc = ldg(f.vOc + off, c);
     const A mInUc = ldg(f.mInU + off, c), mInVc = ldg(f.mInV + off, c);
+    // issue order = order of use: stage A waits only for the loads above, the
+    // rest stream in while A and B run
+    __builtin_amdgcn_sched_barrier(0);
     const A rdyU = ldg(f.rdyU + off, c), rdxV = ldg(f.rdxV + off, c), rAz = ldg(f.rAz + off, c);
+    A k1Z = 0, k2Z = 0, k1C = 0, k2C = 0;  // metric terms
+    if (sc.kterms) {
+      k1Z = ldg(f.k1Z + off, c);
+      k2Z = ldg(f.k2Z + off, c);
+    }
+    __builtin_amdgcn_sched_barrier(0);
     const A rdxU = ldg(f.rdxU + off, c), rdyV = ldg(f.rdyV + off, c), mInC = ldg(f.mInC + off, c);
     const A rrA = ldg(f.rrA + off, c), P = ldg(f.P + off, c);
     A s1 = ldg(src[2] + off, c), s2 = ldg(src[3] + off, c);
@@ -360,10 +369,7 @@ __global__ __launch_bounds__(RI* RJ) void evp_substep_kernel(const EvpArgBlock<T
     const A AW = ldg(f.AreaW + off, c), AS = ldg(f.AreaS + off, c);
     const A wfx = ldg(f.WFX + off, c), wfy = ldg(f.WFY + off, c);
     const A uNm1 = ldg(f.uNm1 + off, c), vNm1 = ldg(f.vNm1 + off, c);
-    A k1Z = 0, k2Z = 0, k1C = 0, k2C = 0;
     if (sc.kterms) {
-      k1Z = ldg(f.k1Z + off, c);
-      k2Z = ldg(f.k2Z + off, c);
       k1C = ldg(f.k1C + off, c);
       k2C = ldg(f.k2C + off, c);
     }
