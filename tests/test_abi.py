@@ -139,3 +139,28 @@ def test_batch_arith_validation():
                   total_elems=68 * 68)
     with pytest.raises(N.VerisHipError, match="VH_EINVAL.*arith"):
         N.call("vh_evp_batched_workspace_bytes", ctypes.byref(bad), ctypes.byref(n))
+
+
+def test_forcing_residual_and_free_drift_validation():
+    assert _struct_members(_header(), "vh_bulk_flux_out") == [f[0] for f in N.BulkFluxOut._fields_]
+    dom = N.Domain(nx=90, ny=40, olx=2, oly=2, cyclic_x=1, fill_mode=0, dtype=N.VH_F64, tile=0)
+    sett = N.Settings()
+    fields = N.Fields()
+    with pytest.raises(N.VerisHipError, match="VH_EINVAL.*output struct"):
+        N.call("vh_bulkf_formula_lanl", ctypes.byref(dom), ctypes.byref(sett), 9.81,
+               None, None, None, None, None, None, None, None)
+    with pytest.raises(N.VerisHipError, match="VH_EINVAL.*null input"):
+        N.call("vh_bulkf_formula_lanl", ctypes.byref(dom), ctypes.byref(sett), 9.81,
+               None, None, None, None, None, None, ctypes.byref(N.BulkFluxOut()), None)
+    with pytest.raises(N.VerisHipError, match="VH_EINVAL.*freedrift"):
+        N.call("vh_freedrift_solver", ctypes.byref(dom), ctypes.byref(sett), ctypes.byref(fields),
+               None, None, None)
+    base = N.workspace_bytes("vh_evp_workspace_bytes", dom)
+    n = ctypes.c_size_t(0)
+    N.call("vh_evp_residuals_workspace_bytes", ctypes.byref(dom), 400, ctypes.byref(n))
+    assert n.value > base
+    with pytest.raises(N.VerisHipError, match="VH_EINVAL"):
+        N.call("vh_evp_residuals_workspace_bytes", ctypes.byref(dom), 0, ctypes.byref(n))
+    with pytest.raises(N.VerisHipError, match="VH_EINVAL.*residual outputs"):
+        N.call("vh_evp_solver_residuals", ctypes.byref(dom), ctypes.byref(sett),
+               ctypes.byref(fields), 10, None, None, None, None, None, None, None, None, 0, None)
