@@ -247,6 +247,34 @@ def test_bulk_fluxes(cuda):
         assert_close(f"bulk {name}", g2n(a), b, 1e-12)
 
 
+def test_forcing_and_free_drift_fp32(cuda):
+    """FP32 storage of the free-drift solver and the bulk fluxes against the
+    FP64 oracle on the same (FP32-representable) inputs: 2e-5 of each field's
+    scale (FP32 rounding through the Monin-Obukhov iterations and atan2)."""
+    from veris_amd._native import BULK_FLUX_NAMES
+    from veris_amd.heat_flux_MITgcm import bulkf_formula_lanl
+
+    st = _spun_up(synth.config0(useFreedrift=True, useEVP=False), cuda, n_spin=3)
+    for k in st.variables.keys():
+        st.variables[k] = np.asarray(st.variables[k], dtype=np.float32).astype(np.float64)
+    want = O.freedrift_solver(st)
+    g32 = st.to(cuda, dtype=torch.float32)
+    got = K.freedrift_solver(g32)
+    for name, a, b in zip(("uIce", "vIce"), got, want):
+        assert a.dtype == torch.float32
+        assert_close(f"freedrift fp32 {name}", g2n(a).astype(np.float64), b, 2e-5)
+    r = np.random.default_rng(78)
+    shp = st.shape
+    tsf = r.uniform(271.0, 303.0, shp)
+    ins = {"uw": r.normal(0, 8, shp), "vw": r.normal(0, 8, shp), "ta": tsf + r.normal(0, 6, shp),
+           "qa": r.uniform(1e-3, 2e-2, shp), "tsf": tsf, "ocn_mask": st.variables.iceMask.copy()}
+    ins = {k: v.astype(np.float32).astype(np.float64) for k, v in ins.items()}
+    want = O.bulkf_formula_lanl(st, *ins.values(), grav=st.settings.gravity)
+    got = bulkf_formula_lanl(g32, *[torch.from_numpy(v).float().to(cuda) for v in ins.values()])
+    for name, a, b in zip(BULK_FLUX_NAMES, got, want):
+        assert_close(f"bulk fp32 {name}", g2n(a).astype(np.float64), b, 2e-5)
+
+
 def test_evp_basal_drag(cuda):
     """basalDragK2 != 0 switches on the exp/log soft-max: tolerance 1e-12."""
     st = synth.tiny(20, 14, basalDragK2=15.0)
@@ -350,13 +378,20 @@ def test_fill_and_global_sum(cuda):
     assert abs(s - ref) <= 1e-13 * O.global_sum(np.abs(a[2:-2, 2:-2] * w[2:-2, 2:-2]))
 
 
-@pytest.mark.parametrize("n_steps,solver", [(1, "evp"), (2, "evp"), (2, "freedrift")])
+STEP_SOLVERS = {"evp": {}, "freedrift": {"useFreedrift": True, "useEVP": False},
+                "adaptive_side_drag": {"useAdaptiveEVP": True, "noSlip": False},
+                "second_order": {"secondOrderBC": True}}
+
+
+@pytest.mark.parametrize("n_steps,solver", [(1, "evp"), (2, "evp"), (2, "freedrift"),
+                                            (2, "adaptive_side_drag"), (2, "second_order")])
 def test_full_step_config0(cuda, n_steps, solver):
     """Whole step (pre-EVP, 400 EVP substeps or free drift, ocean stress,
-    advection, clean-up, ridging, growth, fills).  exp/pow in strength and
-    growth make the GPU differ from the oracle by ulps that the next step
-    carries on; tolerance 1e-9 relative to each field's maximum."""
-    kw = {"useFreedrift": True, "useEVP": False} if solver == "freedrift" else {}
+    advection, clean-up, ridging, growth, fills), default and alternative
+    dynamics settings.  exp/pow in strength and growth make the GPU differ
+    from the oracle by ulps that the next step carries on; tolerance 1e-9
+    relative to each field's maximum."""
+    kw = STEP_SOLVERS[solver]
     st = synth.config0(**kw)
     q0 = st.variables.Qnet.copy()
     sw = st.variables.SWdown.copy()
