@@ -19,8 +19,10 @@ free-drift fixed point;
 the telescoping invariant of ``stressdiv``; exact one-cell upwind shift,
 conservation and the TVD bound of the advection; the scipy root of the
 surface energy balance for ``solve4temp``; budget identities of ``Growth``;
-and an independent C scalar-loop restatement (``oracle/c/``) that must agree
-with this module bit for bit.
+and, for the EVP subcycle and free drift, an independent C scalar-loop
+restatement with explicit wrapped indices (``oracle/c/``, loaded by
+``oracle/c_ref.py``) that must agree with this module bit for bit wherever
+only +, -, *, /, sqrt and comparisons enter.
 """
 
 from __future__ import annotations

@@ -1,0 +1,187 @@
+"""The NumPy oracle against the scalar-loop C restatement (``oracle/c/``).
+
+Two restatements of ``veris/evp_solver.py:22-319`` and
+``veris/freedrift_solver.py:5-62`` written in different idioms -- whole-array
+``numpy.roll`` shifts in the oracle, explicit wrapped (i, j) indices cell by
+cell in C -- must give the same bits on every padded cell, halos included,
+for every solver branch.  A mistake in a shift direction, a metric, a mask or
+an association order in either one breaks the equality.  Where NumPy's
+transcendental functions may come from a different implementation than the C
+library's (exp/log of basal drag, atan2/cos/sin of free drift), the bar is a
+relative 1e-12 of each field's scale instead.
+
+The second test plants each EVP and free-drift mistake of
+``test_oracle_mutations`` into a copy of the oracle and requires the C
+comparison alone to see it.
+"""
+
+from __future__ import annotations
+
+import importlib.util
+import inspect
+
+import numpy as np
+import pytest
+
+from gpu_helpers import pre_evp
+from oracle import c_ref
+from oracle import veris_np as O
+from test_oracle_mutations import MUTATIONS, ORACLE
+from veris_amd import synth
+
+N_SUB = 12
+
+
+def _kterms(st, seed):
+    r = np.random.default_rng(seed)
+    for k in ("k1AtC", "k2AtC", "k1AtZ", "k2AtZ"):
+        st.variables[k] = r.normal(0, 1e-7, st.shape)
+    return st
+
+
+def _coast(st):
+    r = np.random.default_rng(8)
+    st.variables.Fu = r.uniform(0.0, 2.0, st.shape) * st.variables.iceMaskU
+    st.variables.Fv = r.uniform(0.0, 2.0, st.shape) * st.variables.iceMaskV
+    return st
+
+
+def _ice_edge(st):
+    """Open water in scattered cells: ice edges in both directions, so some
+    faces have AreaW > 0 = AreaS and others AreaS > 0 = AreaW."""
+    water = np.random.default_rng(11).random(st.shape) < 0.35
+    for k in ("Area", "hIceMean", "hSnowMean"):
+        st.variables[k] = np.where(water, 0.0, st.variables[k])
+    return st
+
+
+TURN = {"sinWat": 0.42261826174069944, "cosWat": 0.9063077870366499}
+VARIANTS = {
+    "default": lambda: synth.tiny(20, 14),
+    "regional": lambda: synth.regional(7, 0, 37, 29),
+    "own_fill": lambda: synth.tiny(20, 14, veros_fill=False),
+    "turning": lambda: synth.tiny(20, 14, **TURN),
+    "kterms": lambda: _kterms(synth.tiny(20, 14), 3),
+    "tensile": lambda: synth.tiny(20, 14, tensileStrFac=0.05),
+    "no_repl_pressure": lambda: synth.tiny(20, 14, pressReplFac=0),
+    "adaptive": lambda: synth.tiny(20, 14, useAdaptiveEVP=True),
+    "adaptive_regional": lambda: synth.regional(7, 1, 37, 29, useAdaptiveEVP=True, aEvpCoeff=0.2,
+                                                aEVPalphaMin=20.0),
+    "side_drag": lambda: synth.tiny(20, 14, noSlip=False),
+    "side_drag_adaptive": lambda: synth.regional(7, 1, 37, 29, noSlip=False, useAdaptiveEVP=True,
+                                                 sideDragCoeff=0.004, sideDragU0=0.02),
+    "side_drag_ice_edge": lambda: _ice_edge(synth.tiny(20, 14, noSlip=False)),
+    "coastline": lambda: _coast(synth.tiny(20, 14, noSlip=False, use_coastline=True)),
+    "second_order": lambda: synth.tiny(20, 14, secondOrderBC=True),
+    "second_order_regional": lambda: synth.regional(7, 2, 37, 29, secondOrderBC=True,
+                                                    useAdaptiveEVP=True),
+    "second_order_kterms": lambda: _kterms(synth.tiny(20, 14, secondOrderBC=True, **TURN), 4),
+    "basal": lambda: synth.tiny(20, 14, basalDragK2=15.0),
+}
+INEXACT = {"basal"}
+_STATES = {}
+
+
+def _state(name):
+    """Spun-up state of a variant (non-zero velocities and stresses)."""
+    if name not in _STATES:
+        st = pre_evp(VARIANTS[name]())
+        vs = st.variables
+        vs.uIce, vs.vIce, vs.sigma1, vs.sigma2, vs.sigma12 = O.evp_solver(st, n_substeps=10)
+        _STATES[name] = pre_evp(st)
+    return _STATES[name]
+
+
+def _freedrift_state(kind):
+    key = "freedrift_" + kind
+    if key not in _STATES:
+        if kind == "global":
+            st = synth.config0(useFreedrift=True, useEVP=False, waterIceDrag_south=0.0042)
+        else:
+            st = synth.regional(7, 3, 37, 29, useFreedrift=True, useEVP=False)
+        st = pre_evp(st)
+        vs = st.variables
+        vs.uIce, vs.vIce, vs.sigma1, vs.sigma2, vs.sigma12 = O.evp_solver(st, n_substeps=3)
+        _STATES[key] = pre_evp(st)
+    return _STATES[key]
+
+
+def _rel(a, b):
+    return float(np.max(np.abs(a - b))) / max(float(np.max(np.abs(b))), 1e-300)
+
+
+def _differs(oracle, name):
+    st = _state(name)
+    want = c_ref.evp_solver(st, N_SUB)
+    got = oracle.evp_solver(st, n_substeps=N_SUB)
+    if name in INEXACT:
+        return any(not _rel(a, b) <= 1e-12 for a, b in zip(got, want))
+    return any(not np.array_equal(a, b) for a, b in zip(got, want))
+
+
+@pytest.mark.parametrize("variant", sorted(VARIANTS))
+def test_evp_matches_c_restatement(variant):
+    st = _state(variant)
+    want = c_ref.evp_solver(st, N_SUB)
+    got = O.evp_solver(st, n_substeps=N_SUB)
+    for name, a, b in zip(("uIce", "vIce", "sigma1", "sigma2", "sigma12"), got, want):
+        assert np.all(np.isfinite(b)), name
+        assert float(np.max(np.abs(b))) > 0, f"{name}[{variant}] is all zero"
+        if variant in INEXACT:
+            assert _rel(a, b) <= 1e-12, f"{name}[{variant}] rel {_rel(a, b):.3e}"
+        else:
+            nbad = int(np.count_nonzero(a != b))
+            assert nbad == 0, f"{name}[{variant}]: {nbad} cells differ, rel {_rel(a, b):.3e}"
+
+
+@pytest.mark.parametrize("kind", ["global", "regional"])
+def test_freedrift_matches_c_restatement(kind):
+    st = _freedrift_state(kind)
+    want = c_ref.freedrift_solver(st)
+    got = O.freedrift_solver(st)
+    for name, a, b in zip(("uIce", "vIce"), got, want):
+        assert float(np.max(np.abs(b))) > 0
+        assert _rel(a, b) <= 1e-12, f"freedrift {name}[{kind}] rel {_rel(a, b):.3e}"
+
+
+_EVP_FUNCS = (O.strainrates, O.side_drag, O.viscosities, O.c_point_to_z_point, O.stressdiv,
+              O.evp_cfac, O.evp_substep, O.ocean_drag_coeffs, O.basal_drag_coeffs,
+              O.fill_overlap)
+_EVP_SRC = [inspect.getsource(f) for f in _EVP_FUNCS]
+_FD_SRC = inspect.getsource(O.freedrift_solver)
+EVP_MUTATIONS = sorted(n for n, (a, _) in MUTATIONS.items() if any(a in s for s in _EVP_SRC))
+FD_MUTATIONS = sorted(n for n, (a, _) in MUTATIONS.items() if a in _FD_SRC)
+
+
+def _mutant(name, tmp_path):
+    a, b = MUTATIONS[name]
+    src = ORACLE.read_text()
+    assert src.count(a) == 1, name
+    path = tmp_path / "mutant.py"
+    path.write_text(src.replace(a, b))
+    spec = importlib.util.spec_from_file_location("veris_np_mutant", path)
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def test_mutation_lists_are_not_empty():
+    assert len(EVP_MUTATIONS) >= 25 and len(FD_MUTATIONS) >= 5
+
+
+@pytest.mark.parametrize("mutation", EVP_MUTATIONS)
+def test_c_comparison_catches_evp_mistake(mutation, tmp_path):
+    mod = _mutant(mutation, tmp_path)
+    assert any(_differs(mod, v) for v in VARIANTS), f"C comparison missed: {mutation}"
+
+
+@pytest.mark.parametrize("mutation", FD_MUTATIONS)
+def test_c_comparison_catches_free_drift_mistake(mutation, tmp_path):
+    mod = _mutant(mutation, tmp_path)
+    caught = False
+    for kind in ("global", "regional"):
+        st = _freedrift_state(kind)
+        want = c_ref.freedrift_solver(st)
+        got = mod.freedrift_solver(st)
+        caught = caught or any(not _rel(a, b) <= 1e-12 for a, b in zip(got, want))
+    assert caught, f"C comparison missed: {mutation}"
