@@ -34,7 +34,7 @@ def _members():
     return states
 
 
-@pytest.mark.parametrize("tile", [3, 5, 1, 8, 13, 15])
+@pytest.mark.parametrize("tile", [3, 5, 1, 8, 13, 15, 16])
 def test_batched_fp64_matches_oracle(cuda, tile):
     from veris_amd._native import evp_tile_shapes
 
@@ -52,6 +52,28 @@ def test_batched_fp64_matches_oracle(cuda, tile):
                     keep = np.setdiff1d(np.arange(st.shape[0]), [0, 1, st.shape[0] - 2, st.shape[0] - 1])
                     got, ref = got[keep], ref[keep]
                 assert_exact(f"member {m} {name}[tile={tile}, n={n}]", got, ref)
+
+
+@pytest.mark.parametrize("case", ["half_mask", "curvilinear"])
+def test_batched_packed_constants_fallback(cuda, case):
+    """One member whose grid does not allow the compact step constants (a mask
+    other than 0/1, metrics varying along i) sends the whole batch to the full
+    fields; every member still equals the oracle bit for bit."""
+    states = _members()
+    vs = states[1].variables
+    if case == "half_mask":
+        vs.maskInV = vs.maskInV.copy()
+        vs.maskInV[10, 10] = 0.5
+    else:
+        wig = 1.0 + 1e-3 * np.cos(np.arange(states[1].shape[0]))[:, None]
+        vs.dyU = vs.dyU * wig
+        vs.rAz = vs.rAz * wig
+    ens = Ensemble(states, device=cuda, dtype=torch.float64)
+    out = ens.evp(9, tile=5)
+    for m, st in enumerate(states):
+        want = O.evp_solver(st, n_substeps=9)
+        for q, name in enumerate(EVP_NAMES):
+            assert_exact(f"member {m} {name}[{case}]", g2n(ens.view(out[q], m)), want[q])
 
 
 def test_batched_pre_evp_and_budgets(cuda):

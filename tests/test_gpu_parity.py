@@ -179,6 +179,69 @@ def test_evp_setting_variants(cuda, variant):
         assert_exact(f"{name}[{variant}]", a, b)
 
 
+def _assert_bits(name, got, want):
+    """Equal bit patterns (also tells +0 from -0)."""
+    got = np.ascontiguousarray(got, dtype=np.float64)
+    want = np.ascontiguousarray(want, dtype=np.float64)
+    nbad = int(np.count_nonzero(got.view(np.int64) != want.view(np.int64)))
+    assert nbad == 0, f"{name}: {nbad} cells differ in their bits from the oracle"
+
+
+def _packed_case(case):
+    st = synth.tiny(20, 14)
+    vs = st.variables
+    NX, NY = st.shape
+    if case == "half_mask":  # a mask value other than 0 or 1
+        vs.maskInC = vs.maskInC.copy()
+        wet = np.argwhere(vs.maskInC[2:-2, 2:-2] == 1.0)[7] + 2
+        vs.maskInC[tuple(wet)] = 0.5
+    elif case == "negative_zero_mask":  # -0.0 must not become +0.0
+        vs.iceMask = vs.iceMask.copy()
+        vs.iceMask[vs.iceMask == 0.0] = -0.0
+    elif case == "curvilinear":  # metrics that vary along i as well as j
+        wig = 1.0 + 1e-3 * np.cos(np.arange(NX))[:, None]
+        for k in ("dxU", "dyV", "recip_dxU", "recip_rAu"):
+            vs[k] = vs[k] * wig
+    elif case == "one_cell_metric":  # one cell off its row by one ulp
+        vs.recip_rAv = vs.recip_rAv.copy()
+        vs.recip_rAv[9, 6] = np.nextafter(vs.recip_rAv[9, 6], np.inf)
+    return st
+
+
+@pytest.mark.parametrize("case", ["half_mask", "negative_zero_mask", "curvilinear",
+                                  "one_cell_metric", "packed", "min_workspace"])
+def test_evp_packed_constants(cuda, case):
+    """The subcycle reads its masks as bits and its metrics as rows along j
+    only when every mask is exactly +0 or 1 and every metric equals its row
+    value in every bit; any other grid, or a workspace of the minimal size
+    (without room for the compact form), reads the full fields.  Every case
+    equals the oracle bit for bit, halos included."""
+    st = _packed_case(case if case not in ("packed", "min_workspace") else "none")
+    st = _spun_up(st, cuda, n_spin=5)
+    want = O.evp_solver(st, n_substeps=30)
+    g = st.to(cuda)
+    if case == "min_workspace":
+        from veris_amd import _native as N
+
+        c = K.Ctx(g)
+        NX, NY = c.shape
+        full = N.workspace_bytes("vh_evp_workspace_bytes", c.dom)
+        align = lambda b: (b + 255) // 256 * 256  # noqa: E731
+        minimal = full - align(NX * NY) - align(12 * NY * 8)
+        ws = torch.empty(minimal, dtype=torch.uint8, device=cuda)
+        out = c.empty(5)
+        N.call("vh_evp_solver", *c.args(), 30, *[t.data_ptr() for t in out], ws.data_ptr(),
+               minimal, c.stream)
+        with pytest.raises(RuntimeError):
+            N.call("vh_evp_solver", *c.args(), 30, *[t.data_ptr() for t in out],
+                   ws.data_ptr(), minimal - 256, c.stream)
+        got = [g2n(t) for t in out]
+    else:
+        got = [g2n(t) for t in K.evp_solver(g, n_substeps=30)]
+    for name, a, b in zip(EVP_NAMES, got, want):
+        _assert_bits(f"{name}[{case}]", a, b)
+
+
 @pytest.mark.parametrize("kind", ["global", "regional"])
 def test_freedrift_solver(cuda, kind):
     """Free drift (closed-form c-point solution with atan2, cos, sin): the

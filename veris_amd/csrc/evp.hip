@@ -77,6 +77,14 @@ __global__ __launch_bounds__(256) void evp_residual_reduce_kernel(const double* 
 
 }  // namespace
 
+// Workspace of one single-domain subcycle: the ping-pong state, the argument
+// block and the packed constants.
+template <typename T>
+size_t evp_core_ws_bytes(const Geom& g) {
+  return 5 * align256(field_elems(g) * sizeof(T)) + align256(sizeof(EvpArgBlock<T>)) +
+         evp_pack_bytes<T>(field_elems(g), g.NY);
+}
+
 int evp_residual_tiles(const Geom& g) {
   const int TI = evp_tile_extent_i(kEvpResidualCfg), TJ = evp_tile_extent_j(kEvpResidualCfg);
   return ((g.NX + TI - 1) / TI) * ((g.NY + TJ - 1) / TJ);
@@ -94,6 +102,9 @@ int evp_run_core(const Geom& g, const vh_settings* s, const vh_fields* fl, const
   const size_t need = 5 * fb + align256(sizeof(EvpArgBlock<T>));
   VH_REQUIRE(ws != nullptr && ws_bytes >= need, VH_EWORKSPACE,
              "evp workspace: need %zu bytes, got %zu", need, ws_bytes);
+  // packed constants when the workspace has room for them (its full size,
+  // vh_evp_workspace_bytes); a workspace of the minimal size reads full fields
+  const bool pack = ws_bytes >= evp_core_ws_bytes<T>(g);
   const void* req[] = {fl->uIce, fl->vIce, fl->sigma1, fl->sigma2, fl->sigma12,
                        fl->SeaIceStrength, fl->SeaIceMassC, fl->SeaIceMassU, fl->SeaIceMassV,
                        fl->AreaW, fl->AreaS, fl->WindForcingX, fl->WindForcingY, fl->uOcean,
@@ -148,8 +159,10 @@ int evp_run_core(const Geom& g, const vh_settings* s, const vh_fields* fl, const
   ab.g = g;
   ab.nlaunch = evp_n_launches(cfg, nsub);
   ab.members = nullptr;
+  if (pack) evp_pack_attach(&ab, (char*)ws + need, field_elems(g), g.NY);
   EvpArgBlock<T>* ab_dev = (EvpArgBlock<T>*)((char*)ws + 5 * fb);
   hipLaunchKernelGGL(evp_write_args_kernel<T>, dim3(1), dim3(64), 0, st, ab_dev, ab);
+  if (pack) evp_pack_launch(ab_dev, (long long)field_elems(g), 1, st);
   VH_TRY_HIP(hipGetLastError());
   if (nsub >= 8 && graphs_enabled()) {
     char key[160];
@@ -192,7 +205,7 @@ int run_evp_res(const Geom& g, const vh_settings* s, const vh_fields* fl, int ns
                 double* resU, double* resSig, void* ws, size_t ws_bytes, int dtype, hipStream_t st) {
   VH_REQUIRE(nsub >= 1, VH_EINVAL, "residuals need n_substeps >= 1");
   VH_REQUIRE(resU && resSig && resU != resSig, VH_EINVAL, "null or aliased residual outputs");
-  const size_t base = 5 * align256(field_elems(g) * sizeof(T)) + align256(sizeof(EvpArgBlock<T>));
+  const size_t base = evp_core_ws_bytes<T>(g);
   const int nt = evp_residual_tiles(g);
   const size_t need = base + (size_t)nsub * nt * 2 * sizeof(double);
   VH_REQUIRE(ws != nullptr && ws_bytes >= need, VH_EWORKSPACE,
@@ -217,9 +230,7 @@ extern "C" int vh_evp_workspace_bytes(const vh_domain* d, size_t* bytes) {
   int rc = vh::make_geom(d, &g);
   if (rc) return rc;
   VH_REQUIRE(bytes != nullptr, VH_EINVAL, "null bytes");
-  const size_t eb = vh::elem_bytes(d->dtype);
-  const size_t ab = d->dtype == VH_F64 ? sizeof(vh::EvpArgBlock<double>) : sizeof(vh::EvpArgBlock<float>);
-  *bytes = 5 * vh::align256(vh::field_elems(g) * eb) + vh::align256(ab);
+  *bytes = d->dtype == VH_F64 ? vh::evp_core_ws_bytes<double>(g) : vh::evp_core_ws_bytes<float>(g);
   return VH_OK;
 }
 

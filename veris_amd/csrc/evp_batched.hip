@@ -35,8 +35,12 @@ int n_tiles_of(int cfg, const Geom& g) {
   return ((g.NX + TI - 1) / TI) * ((g.NY + TJ - 1) / TJ);
 }
 
+// Workspace: ping-pong state, argument block, member table (the minimum,
+// `base`), then the packed constants (`total`; without room for them the
+// launches read the full fields).
 struct Layout {
-  size_t fb, ab, mem, total;
+  size_t fb, ab, mem, base, total;
+  int rows;  // metric rows of all members (sum of padded ny)
 };
 
 template <typename T>
@@ -45,7 +49,10 @@ Layout layout_of(const vh_batch* b) {
   L.fb = align256((size_t)b->total_elems * sizeof(T));
   L.ab = align256(sizeof(EvpArgBlock<T>));
   L.mem = align256((size_t)b->n_members * sizeof(MemberDesc));
-  L.total = 5 * L.fb + L.ab + L.mem;
+  L.base = 5 * L.fb + L.ab + L.mem;
+  L.rows = 0;
+  for (int m = 0; m < b->n_members; ++m) L.rows += b->members[m].ny + 4;
+  L.total = L.base + evp_pack_bytes<T>((size_t)b->total_elems, (size_t)L.rows);
   return L;
 }
 
@@ -97,8 +104,9 @@ template <typename T>
 int run_batched(const vh_batch* b, const vh_settings* s, const vh_fields* fl, int nsub, void* uo,
                 void* vo, void* s1o, void* s2o, void* s12o, void* ws, size_t wsb, hipStream_t st) {
   const Layout L = layout_of<T>(b);
-  VH_REQUIRE(ws != nullptr && wsb >= L.total, VH_EWORKSPACE,
+  VH_REQUIRE(ws != nullptr && wsb >= L.base, VH_EWORKSPACE,
              "batched evp workspace: need %zu bytes, got %zu", L.total, wsb);
+  const bool pack = wsb >= L.total;
   const void* req[kEvpMaxRequired];
   const int nreq = evp_required_fields(fl, req);
   for (int q = 0; q < nreq; ++q)
@@ -140,7 +148,8 @@ int run_batched(const vh_batch* b, const vh_settings* s, const vh_fields* fl, in
   MemberDesc* mem_dev = (MemberDesc*)(base + 5 * L.fb + L.ab);
 
   // member table, written by value in chunks
-  int n_main = 0, n_rem = 0;
+  int n_main = 0, n_rem = 0, roff = 0;
+  long long max_cells = 0;
   for (int first = 0; first < b->n_members; first += kMembersPerWrite) {
     MemberChunk ch{};
     ch.first = first;
@@ -151,6 +160,9 @@ int run_batched(const vh_batch* b, const vh_settings* s, const vh_fields* fl, in
       member_geom(b->members[first + q], &md.g);
       md.tb[0] = n_main;
       md.tb[1] = n_rem;
+      md.roff = roff;
+      roff += md.g.NY;
+      max_cells = std::max(max_cells, (long long)md.g.NX * md.g.NY);
       n_main += n_tiles_of(cfg, md.g);
       n_rem += n_tiles_of(kEvpRemainderCfg, md.g);
     }
@@ -171,7 +183,9 @@ int run_batched(const vh_batch* b, const vh_settings* s, const vh_fields* fl, in
   ab.nlaunch = evp_n_launches(cfg, nsub);
   ab.members = mem_dev;
   ab.n_members = b->n_members;
+  if (pack) evp_pack_attach(&ab, base + L.base, (size_t)b->total_elems, L.rows);
   hipLaunchKernelGGL(evp_write_args_kernel<T>, dim3(1), dim3(64), 0, st, ab_dev, ab);
+  if (pack) evp_pack_launch(ab_dev, max_cells, b->n_members, st);
   VH_TRY_HIP(hipGetLastError());
   const int K = kEvpTiles[cfg].k;
   const bool so = ab.sc.sobc != 0;
