@@ -391,6 +391,8 @@ def main(argv=None) -> int:
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--substeps", type=int, default=400)
+    ap.add_argument("--tile", type=int, default=0,
+                    help="EVP workgroup shape index (default 0: autotune among the exact shapes)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the batched, adjoint and 0.25-degree measurements")
@@ -442,7 +444,11 @@ def main(argv=None) -> int:
     from veris_amd.autotune import tune_evp
 
     seaice_step(g, q0, sw, n_substeps=nsub)
-    tile = tune_evp(g, exact=True, n_substeps=100)
+    if args.tile > 0:
+        tile = args.tile
+        K.EVP_TILE[(tuple(g.variables.iceMask.shape), g.variables.iceMask.dtype)] = tile
+    else:
+        tile = tune_evp(g, exact=True, n_substeps=100)
     ri, rj, kk = _native.evp_tile_shapes()[tile]
     for _ in range(args.warmup):
         seaice_step(g, q0, sw, n_substeps=nsub)
@@ -503,6 +509,7 @@ def main(argv=None) -> int:
         "roofline": {
             "bound": "hbm",
             "kernel": kernel,
+            "tile": tile,
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
