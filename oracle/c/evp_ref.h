@@ -37,4 +37,13 @@ int evp_solver_ref(const double* settings, const double* const* fields, int nsub
 /* closed-form free drift; returns 0, or -1 when out of memory */
 int freedrift_ref(const double* settings, const double* const* fields, double* u, double* v);
 
+/* The whole sea-ice step on fields and settings passed by name (step_ref.c);
+ * the fields are updated in place.  Returns 0, -1 when out of memory, -2 when
+ * a field or setting is missing, -3 when neither solver is selected. */
+int seaice_step_ref(int NX, int NY, int nf, const char* const* fnames, double* const* fields,
+                    int ns, const char* const* snames, const double* svals, int nsub);
+
+/* name of the field or setting a failed seaice_step_ref call did not find */
+const char* step_ref_missing(void);
+
 #endif

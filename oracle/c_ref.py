@@ -1,12 +1,13 @@
 """ctypes loader of the scalar-loop C restatement (``oracle/c/``) -- TEST
 INFRASTRUCTURE ONLY.
 
-The C code restates ``veris/evp_solver.py:22-319`` and
-``veris/freedrift_solver.py:5-62`` cell by cell with wrapped indices; the
-tests run it beside the NumPy oracle (``oracle/veris_np.py``) on the same
-state and require the two to agree bit for bit where only +, -, *, /, sqrt
-and comparisons are involved.  ``build()`` compiles it with gcc through
-``oracle/c/Makefile``.
+The C code restates the sea-ice step cell by cell with wrapped indices:
+``veris/evp_solver.py:22-319`` and ``veris/freedrift_solver.py:5-62``
+(``evp_ref.c``) and the rest of the step of
+``seaice_global_4deg.py:874-932`` (``step_ref.c``).  The tests run it beside
+the NumPy oracle (``oracle/veris_np.py``) on the same state and require the
+two to agree bit for bit where only +, -, *, /, sqrt and comparisons are
+involved.  ``build()`` compiles it with gcc through ``oracle/c/Makefile``.
 """
 
 from __future__ import annotations
@@ -36,6 +37,11 @@ FIELDS = ("uIce", "vIce", "sigma1", "sigma2", "sigma12",
           "AreaW", "AreaS", "SeaIceMassC", "SeaIceMassU", "SeaIceMassV",
           "WindForcingX", "WindForcingY", "Fu", "Fv")
 
+STEP_OUTPUTS = ("SeaIceMassC", "SeaIceMassU", "SeaIceMassV", "AreaW", "AreaS", "WindForcingX",
+                "WindForcingY", "SeaIceStrength", "OceanStressU", "OceanStressV", "os_hIceMean",
+                "os_hSnowMean", "EmPmR", "forc_salt_surface_ice", "Qsw", "Qnet", "SeaIceLoad",
+                "IcePenetSW", "recip_hIceMean")
+
 _lib = None
 
 
@@ -56,6 +62,11 @@ def _load():
         lib.evp_solver_ref.restype = ctypes.c_int
         lib.freedrift_ref.argtypes = [dp, ctypes.POINTER(dp), dp, dp]
         lib.freedrift_ref.restype = ctypes.c_int
+        cpp = ctypes.POINTER(ctypes.c_char_p)
+        lib.seaice_step_ref.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, cpp,
+                                        ctypes.POINTER(dp), ctypes.c_int, cpp, dp, ctypes.c_int]
+        lib.seaice_step_ref.restype = ctypes.c_int
+        lib.step_ref_missing.restype = ctypes.c_char_p
         _lib = lib
     return _lib
 
@@ -97,3 +108,47 @@ def freedrift_solver(state):
         raise MemoryError("freedrift_ref: allocation failed")
     del keep
     return tuple(out)
+
+
+def seaice_step(state, qnet_forcing=None, swdown=None, n_substeps=None, strength=None):
+    """The whole step on a copy of the state's variables; returns the updated
+    variables as a dict (the oracle's ``seaice_step`` updates them in place).
+    ``strength``: a SeaIceStrength field to use instead of the C library's
+    exp() (for bit-for-bit comparisons of everything else)."""
+    from oracle import veris_np as O
+
+    lib = _load()
+    vs, s = state.variables, state.settings
+    arrays = {k: np.array(vs[k], dtype=np.float64, copy=True) for k in vs.keys()
+              if isinstance(vs[k], np.ndarray)}
+    if qnet_forcing is not None:
+        arrays["Qnet"] = np.array(qnet_forcing, dtype=np.float64, copy=True)
+    if swdown is not None:
+        arrays["Qsw"] = -np.asarray(swdown, dtype=np.float64)
+    NX, NY = arrays["uIce"].shape
+    for k in STEP_OUTPUTS:  # fields the step writes (zero until it does)
+        arrays.setdefault(k, np.zeros((NX, NY)))
+    for k, a in arrays.items():
+        arrays[k] = np.ascontiguousarray(np.broadcast_to(a, (NX, NY))).copy()
+    sett = {k: float(s[k]) for k in s.keys() if isinstance(s[k], (bool, int, float))}
+    # the oracle's module constants of solve4temp and the McPhee taper (growth.py:258-259)
+    sett.update(AA1=O.AA1, AA2=O.AA2, BB1=O.BB1, PPASCALS=float(O.PPASCALS), CC1=O.CC1,
+                CC2=O.CC2, McPheeT0=0.4, McPheeT1=7 / 0.4)
+    if strength is not None:
+        arrays["SeaIceStrength"] = np.array(strength, dtype=np.float64, copy=True)
+        sett["keep_strength"] = 1.0
+    n = int(s.nEVPsteps if n_substeps is None else n_substeps)
+    fnames = list(arrays)
+    dp = ctypes.POINTER(ctypes.c_double)
+    fn = (ctypes.c_char_p * len(fnames))(*[k.encode() for k in fnames])
+    fp = (dp * len(fnames))(*[arrays[k].ctypes.data_as(dp) for k in fnames])
+    snames = list(sett)
+    sn = (ctypes.c_char_p * len(snames))(*[k.encode() for k in snames])
+    sv = np.array([sett[k] for k in snames], dtype=np.float64)
+    rc = lib.seaice_step_ref(NX, NY, len(fnames), fn, fp, len(snames), sn,
+                             sv.ctypes.data_as(dp), n)
+    if rc == -2:
+        raise KeyError(f"seaice_step_ref: no field or setting {lib.step_ref_missing().decode()!r}")
+    if rc != 0:
+        raise RuntimeError(f"seaice_step_ref returned {rc}")
+    return arrays

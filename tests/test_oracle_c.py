@@ -13,6 +13,15 @@ relative 1e-12 of each field's scale instead.
 The second test plants each EVP and free-drift mistake of
 ``test_oracle_mutations`` into a copy of the oracle and requires the C
 comparison alone to see it.
+
+The whole step (``oracle/c/step_ref.c``: pre-EVP, IceVelocities, ocean
+stress, advection, clean-up, ridging, Growth with solve4temp, fills) is
+compared the same way.  The sea-ice strength calls exp(), so the C step is
+handed the oracle's strength field; then every field computed before Growth
+must be equal bit for bit, and Growth's outputs, whose solve4temp and melt
+taper call pow() and exp(), must be bit-equal in at least 99% of the cells
+and within 1e-12 of scale everywhere.  Every planted mistake of the step
+functions must break that.
 """
 
 from __future__ import annotations
@@ -185,3 +194,148 @@ def test_c_comparison_catches_free_drift_mistake(mutation, tmp_path):
         got = mod.freedrift_solver(st)
         caught = caught or any(not _rel(a, b) <= 1e-12 for a, b in zip(got, want))
     assert caught, f"C comparison missed: {mutation}"
+
+
+# ---------------------------------------------------------------------------
+# the whole step
+# ---------------------------------------------------------------------------
+
+def _load_state(st):
+    st = synth.tiny(20, 14) if st is None else st
+    vs = st.variables
+    vs.SeaIceLoad = st.settings.rhoIce * vs.hIceMean + st.settings.rhoSnow * vs.hSnowMean
+    return st
+
+
+def _moving_ice(st):
+    """Non-zero initial ice velocities (relative wind, ocean stress)."""
+    r = np.random.default_rng(12)
+    st.variables.uIce = r.normal(0, 0.1, st.shape) * st.variables.iceMaskU
+    st.variables.vIce = r.normal(0, 0.1, st.shape) * st.variables.iceMaskV
+    return st
+
+
+def _melting(st):
+    """Warm air over thin snow: the snow melts away within the step."""
+    vs = st.variables
+    vs.ATemp = np.full(st.shape, 285.0)
+    vs.hSnowMean = vs.hSnowMean * 0.002
+    return st
+
+
+def _thin_ice(st):
+    """Ice exactly at, just below and just above hIce_min; with no motion
+    (no substeps from rest) advection leaves these thicknesses unchanged."""
+    vs, hmin = st.variables, st.settings.hIce_min
+    h = vs.hIceMean.copy()
+    h[2:-2:6, 2:-2] = hmin
+    h[3:-2:6, 2:-2] = np.nextafter(hmin, 0)
+    h[4:-2:6, 2:-2] = np.nextafter(hmin, 1)
+    vs.hIceMean = h
+    return st
+
+
+STEP_VARIANTS = {
+    "tiny": lambda: synth.tiny(20, 14),
+    "moving_ice": lambda: _moving_ice(synth.tiny(20, 14)),
+    "melting": lambda: _melting(synth.tiny(20, 14)),
+    "thin_ice": lambda: _thin_ice(synth.tiny(20, 14)),
+    "regional": lambda: synth.regional(7, 0, 37, 29),
+    "config0": lambda: synth.config0(),
+    "own_fill": lambda: synth.tiny(20, 14, veros_fill=False),
+    "absolute_wind": lambda: synth.tiny(20, 14, useRelativeWind=False),
+    "sea_ice_load": lambda: _load_state(synth.tiny(20, 14, useRealFreshWaterFlux=True)),
+    "adaptive_side_drag": lambda: synth.tiny(20, 14, useAdaptiveEVP=True, noSlip=False),
+    "ice_edge": lambda: _ice_edge(synth.tiny(20, 14)),
+}
+GROWTH_OUT = ("hIceMean", "hSnowMean", "Area", "TSurf", "EmPmR", "forc_salt_surface_ice", "Qsw",
+              "Qnet", "SeaIceLoad", "IcePenetSW", "recip_hIceMean")
+
+
+STEP_SUBSTEPS = {"thin_ice": 0}
+
+
+def _step_pair(oracle, variant):
+    """(C step, oracle step) from two fresh copies of the same seeded state."""
+    mk, n_sub = STEP_VARIANTS[variant], STEP_SUBSTEPS.get(variant, 15)
+    st, want = mk(), mk()
+    q0 = st.variables.Qnet.copy()
+    sw = st.variables.SWdown.copy()
+    strength = O.SeaIceStrength(pre_evp(mk()))
+    got = c_ref.seaice_step(st, q0, sw, n_substeps=n_sub, strength=strength)
+    oracle.seaice_step(want, q0, sw, n_substeps=n_sub)
+    return got, want.variables
+
+
+def _step_mismatches(got, want):
+    """Names of the fields that break the bar stated in the module docstring."""
+    bad = []
+    for k, a in got.items():
+        if k not in want.keys() or not isinstance(want[k], np.ndarray):
+            continue
+        b = np.ascontiguousarray(np.broadcast_to(np.asarray(want[k], dtype=np.float64), a.shape))
+        neq = int(np.count_nonzero(a.view(np.int64) != b.view(np.int64)))
+        if not neq:
+            continue
+        rel = float(np.max(np.abs(a - b))) / max(float(np.max(np.abs(b))), 1e-300)
+        if k not in GROWTH_OUT or neq > 0.01 * a.size or not rel <= 1e-12:
+            bad.append((k, neq, rel))
+    return bad
+
+
+@pytest.mark.parametrize("variant", sorted(STEP_VARIANTS))
+def test_step_matches_c_restatement(variant):
+    got, want = _step_pair(O, variant)
+    moving = STEP_SUBSTEPS.get(variant, 15) > 0
+    assert float(np.max(np.abs(got["uIce"]))) > 0 or not moving
+    assert float(np.max(got["hIceMean"])) > 0
+    assert not _step_mismatches(got, want), _step_mismatches(got, want)
+
+
+def test_step_strength_matches_c_restatement():
+    """The C library's exp() in the strength: within 1e-15 of scale."""
+    mk = STEP_VARIANTS["config0"]
+    st = mk()
+    got = c_ref.seaice_step(st, st.variables.Qnet.copy(), st.variables.SWdown.copy(),
+                            n_substeps=1)
+    want = O.SeaIceStrength(pre_evp(mk()))
+    assert _rel(got["SeaIceStrength"], want) <= 1e-15
+
+
+def test_free_drift_step_matches_c_restatement():
+    """Free drift calls atan2/cos/sin: every field within 1e-12 of scale."""
+    STEP_VARIANTS["free_drift"] = lambda: synth.config0(useFreedrift=True, useEVP=False)
+    try:
+        got, want = _step_pair(O, "free_drift")
+    finally:
+        del STEP_VARIANTS["free_drift"]
+    for k, a in got.items():
+        if k in want.keys() and isinstance(want[k], np.ndarray):
+            b = np.broadcast_to(np.asarray(want[k], dtype=np.float64), a.shape)
+            assert _rel(a, b) <= 1e-12, k
+
+
+_STEP_FUNCS = (O.SeaIceMass, O.AreaWS, O.SeaIceStrength, O.tauXY, O.WindForcingXY,
+               O.OceanStressUV, O.limiter, O._slope_ratio, O.calc_ZonalFlux,
+               O.calc_MeridionalFlux, O.calc_Advection, O.clean_up_advection, O.ridging,
+               O.solve4temp, O.Growth, O.seaice_step)
+_STEP_SRC = [inspect.getsource(f) for f in _STEP_FUNCS]
+STEP_MUTATIONS = sorted(n for n, (a, _) in MUTATIONS.items() if any(a in s for s in _STEP_SRC))
+
+
+def test_step_mutation_list_is_not_empty():
+    assert len(STEP_MUTATIONS) >= 20
+
+
+@pytest.mark.parametrize("mutation", STEP_MUTATIONS)
+def test_c_comparison_catches_step_mistake(mutation, tmp_path):
+    mod = _mutant(mutation, tmp_path)
+    for variant in ("tiny", "moving_ice", "melting", "thin_ice", "config0", "sea_ice_load",
+                    "ice_edge"):
+        try:
+            got, want = _step_pair(mod, variant)
+        except Exception:  # a planted mistake may also make the oracle raise
+            return
+        if _step_mismatches(got, want):
+            return
+    pytest.fail(f"C comparison missed: {mutation}")
