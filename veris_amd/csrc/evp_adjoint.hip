@@ -105,11 +105,14 @@ __device__ inline T* fwd_loc(const BlockPlan<T>& p, int k, int q) {
 // which the reverse sweep reads and never recomputes.
 template <typename T>
 __global__ void adj_write_blocks_kernel(EvpArgBlock<T>* blocks, size_t stride, EvpArgBlock<T> proto,
-                                        BlockPlan<T> p, int mode, AdjArgs<T>* aa, AdjArgs<T> av) {
+                                        BlockPlan<T> p, int mode, AdjArgs<T>* aa, AdjArgs<T> av,
+                                        const EvpArgBlock<T>* packed_from) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (mode == 1 && k == 0) *aa = av;
   if (k >= p.n) return;
   EvpArgBlock<T> b = proto;
+  // packed step constants: usable when the pack kernel left its flag set
+  b.packed = packed_from ? packed_from->packed : 0;
   if (mode == 0) {
     for (int q = 0; q < 5; ++q) {
       b.in[q] = fwd_loc(p, k, q);
@@ -554,7 +557,7 @@ __global__ __launch_bounds__(BLK) void pre_evp_adjoint_kernel(Geom g, PreAdjArgs
 struct AdjLayout {
   size_t fb, blk, total;
   int nck, ce, n;
-  size_t off_args, off_fblk, off_rblk, off_ckpt, off_pp, off_seg, off_lam;
+  size_t off_args, off_fblk, off_rblk, off_ckpt, off_pp, off_seg, off_lam, off_pblk, off_pack;
 };
 
 template <typename T>
@@ -575,9 +578,14 @@ AdjLayout adj_layout(const Geom& g, int n, int ce) {
   // two sets of segment slots (slot 0 of each unused): the recompute of one
   // segment overlaps the reverse sweep of the next
   L.off_lam = L.off_seg + (size_t)2 * 5 * (ce + 1) * L.fb;
-  L.total = L.off_lam + (size_t)10 * L.fb;
+  // argument block of the pack kernel, then the packed step constants of the
+  // forward and recompute launches
+  L.off_pblk = L.off_lam + (size_t)10 * L.fb;
+  L.off_pack = L.off_pblk + L.blk;
+  L.total = L.off_pack + evp_pack_bytes<T>(field_elems(g), g.NY);
   return L;
 }
+
 
 template <typename T>
 T* wsf(void* ws, size_t off, size_t fb, int q) {
@@ -657,6 +665,22 @@ EvpArgBlock<T> proto_block(const Geom& g, const vh_settings* s, const vh_fields*
   return ab;
 }
 
+// Packs the step constants into the workspace (evp_pack_kernel) and returns
+// the prototype block of the forward launches, pointing at them; the blocks
+// take the pack kernel's verdict from the returned device block.
+template <typename T>
+EvpArgBlock<T> packed_proto(const Geom& g, const vh_settings* s, const vh_fields* f,
+                            const AdjLayout& L, void* ws, const EvpArgBlock<T>** verdict,
+                            hipStream_t st) {
+  EvpArgBlock<T> proto = proto_block<T>(g, s, f);
+  evp_pack_attach(&proto, (char*)ws + L.off_pack, field_elems(g), g.NY);
+  EvpArgBlock<T>* pblk = (EvpArgBlock<T>*)((char*)ws + L.off_pblk);
+  hipLaunchKernelGGL(evp_write_args_kernel<T>, dim3(1), dim3(64), 0, st, pblk, proto);
+  evp_pack_launch(pblk, (long long)field_elems(g), 1, st);
+  *verdict = pblk;
+  return proto;
+}
+
 template <typename T>
 BlockPlan<T> block_plan(const AdjLayout& L, const vh_fields* f, void* const outs[5], void* ws) {
   BlockPlan<T> p{};
@@ -716,9 +740,11 @@ int forward_ckpt(const Geom& g, const vh_domain* d, const vh_settings* s, const 
   tiles_of(cfg, g, &n_tiles, &tiles_j);
   EvpArgBlock<T>* fblk = (EvpArgBlock<T>*)((char*)ws + L.off_fblk);
   AdjArgs<T> unused{};
+  const EvpArgBlock<T>* verdict = nullptr;
+  const EvpArgBlock<T> proto = packed_proto<T>(g, s, f, L, ws, &verdict, st);
   hipLaunchKernelGGL(adj_write_blocks_kernel<T>, dim3((n + 63) / 64), dim3(64), 0, st, fblk, L.blk,
-                     proto_block<T>(g, s, f), block_plan<T>(L, f, outs, ws), 0,
-                     (AdjArgs<T>*)nullptr, unused);
+                     proto, block_plan<T>(L, f, outs, ws), 0, (AdjArgs<T>*)nullptr, unused,
+                     verdict);
   VH_TRY_HIP(hipGetLastError());
   T* final_slot = wsf<T>(ws, L.off_ckpt, L.fb, 5 * L.nck);
   auto record = [&](hipStream_t cs) {
@@ -765,8 +791,10 @@ int adjoint(const Geom& g, const vh_domain* d, const vh_settings* s, const vh_fi
   }
   AdjArgs<T>* a_dev = (AdjArgs<T>*)((char*)ws + L.off_args);
   EvpArgBlock<T>* rblk = (EvpArgBlock<T>*)((char*)ws + L.off_rblk);
+  const EvpArgBlock<T>* verdict = nullptr;
+  const EvpArgBlock<T> proto = packed_proto<T>(g, s, f, L, ws, &verdict, st);
   hipLaunchKernelGGL(adj_write_blocks_kernel<T>, dim3((n + 63) / 64), dim3(64), 0, st, rblk, L.blk,
-                     proto_block<T>(g, s, f), block_plan<T>(L, f, nullptr, ws), 1, a_dev, av);
+                     proto, block_plan<T>(L, f, nullptr, ws), 1, a_dev, av, verdict);
   VH_TRY_HIP(hipGetLastError());
 
   const int cfg = pick_adj_tile(g, d->tile);
