@@ -667,17 +667,17 @@ EvpArgBlock<T> proto_block(const Geom& g, const vh_settings* s, const vh_fields*
 
 // Packs the step constants into the workspace (evp_pack_kernel) and returns
 // the prototype block of the forward launches, pointing at them; the blocks
-// take the pack kernel's verdict from the returned device block.
+// take the pack kernel's outcome from the returned device block.
 template <typename T>
 EvpArgBlock<T> packed_proto(const Geom& g, const vh_settings* s, const vh_fields* f,
-                            const AdjLayout& L, void* ws, const EvpArgBlock<T>** verdict,
+                            const AdjLayout& L, void* ws, const EvpArgBlock<T>** pack_status,
                             hipStream_t st) {
   EvpArgBlock<T> proto = proto_block<T>(g, s, f);
   evp_pack_attach(&proto, (char*)ws + L.off_pack, field_elems(g), g.NY);
   EvpArgBlock<T>* pblk = (EvpArgBlock<T>*)((char*)ws + L.off_pblk);
   hipLaunchKernelGGL(evp_write_args_kernel<T>, dim3(1), dim3(64), 0, st, pblk, proto);
   evp_pack_launch(pblk, (long long)field_elems(g), 1, st);
-  *verdict = pblk;
+  *pack_status = pblk;
   return proto;
 }
 
@@ -740,11 +740,11 @@ int forward_ckpt(const Geom& g, const vh_domain* d, const vh_settings* s, const 
   tiles_of(cfg, g, &n_tiles, &tiles_j);
   EvpArgBlock<T>* fblk = (EvpArgBlock<T>*)((char*)ws + L.off_fblk);
   AdjArgs<T> unused{};
-  const EvpArgBlock<T>* verdict = nullptr;
-  const EvpArgBlock<T> proto = packed_proto<T>(g, s, f, L, ws, &verdict, st);
+  const EvpArgBlock<T>* pack_status = nullptr;
+  const EvpArgBlock<T> proto = packed_proto<T>(g, s, f, L, ws, &pack_status, st);
   hipLaunchKernelGGL(adj_write_blocks_kernel<T>, dim3((n + 63) / 64), dim3(64), 0, st, fblk, L.blk,
                      proto, block_plan<T>(L, f, outs, ws), 0, (AdjArgs<T>*)nullptr, unused,
-                     verdict);
+                     pack_status);
   VH_TRY_HIP(hipGetLastError());
   T* final_slot = wsf<T>(ws, L.off_ckpt, L.fb, 5 * L.nck);
   auto record = [&](hipStream_t cs) {
@@ -791,10 +791,10 @@ int adjoint(const Geom& g, const vh_domain* d, const vh_settings* s, const vh_fi
   }
   AdjArgs<T>* a_dev = (AdjArgs<T>*)((char*)ws + L.off_args);
   EvpArgBlock<T>* rblk = (EvpArgBlock<T>*)((char*)ws + L.off_rblk);
-  const EvpArgBlock<T>* verdict = nullptr;
-  const EvpArgBlock<T> proto = packed_proto<T>(g, s, f, L, ws, &verdict, st);
+  const EvpArgBlock<T>* pack_status = nullptr;
+  const EvpArgBlock<T> proto = packed_proto<T>(g, s, f, L, ws, &pack_status, st);
   hipLaunchKernelGGL(adj_write_blocks_kernel<T>, dim3((n + 63) / 64), dim3(64), 0, st, rblk, L.blk,
-                     proto, block_plan<T>(L, f, nullptr, ws), 1, a_dev, av, verdict);
+                     proto, block_plan<T>(L, f, nullptr, ws), 1, a_dev, av, pack_status);
   VH_TRY_HIP(hipGetLastError());
 
   const int cfg = pick_adj_tile(g, d->tile);
