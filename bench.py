@@ -294,6 +294,41 @@ def bench_adjoint(dev, ckpt: int = 10) -> dict:
     return res
 
 
+def bench_shard(dev, st, stepped) -> dict:
+    """The EVP subcycle on one x-strip of an 8-way split of the 0.25-degree
+    grid (180 x 720 interior columns plus the 2-cell overlap), the shape one
+    GPU would own, beside the whole grid on one GPU.  The strip is a closed
+    domain here: its overlap columns stand for the neighbours' values that a
+    split run would exchange each substep (2 columns x 724 rows x u, v)."""
+    import torch
+
+    from veris_amd import _native
+    from veris_amd import kernels as K
+    from veris_amd.autotune import tune_evp
+    from veris_amd.state import VerisState, make_state_settings
+    from veris_amd.variables import VariablesNamespace
+
+    nsub = int(st.settings.nEVPsteps)
+    nx_s, NY = st.settings.nx // 8, st.settings.ny + 4
+    res = {"workload": f"{nx_s}x{st.settings.ny} + 2-cell halo (one strip of an 8-way x split of "
+                       f"the 0.25-degree grid), {nsub} EVP substeps after one full step",
+           "exchange_bytes_per_neighbour_per_substep": 2 * NY * 2 * 8}
+    for tag, full in (("fp64", stepped["fp64"]), ("fp32", stepped["fp32"])):
+        arrays = {}
+        for k, a in full.variables.items():
+            arrays[k] = (a[: nx_s + 4].clone() if isinstance(a, torch.Tensor) and a.dim() == 2
+                         and a.shape[0] == full.variables.uIce.shape[0] else a)
+        sett = make_state_settings(nx_s, st.settings.ny, enable_cyclic_x=False, nEVPsteps=nsub)
+        strip = VerisState(sett, VariablesNamespace(arrays))
+        tile = tune_evp(strip, exact=True, n_substeps=100)
+        t_strip = _sync_time(lambda: K.evp_solver(strip, nsub), 3)
+        t_full = _sync_time(lambda: K.evp_solver(full, nsub), 3)
+        res[f"{tag}_strip_us_per_substep"] = t_strip / nsub * 1e6
+        res[f"{tag}_full_grid_us_per_substep"] = t_full / nsub * 1e6
+        res[f"{tag}_strip_tile"] = list(_native.evp_tile_shapes()[tile])
+    return res
+
+
 def bench_hires(dev) -> dict:
     import torch
 
@@ -347,6 +382,7 @@ def bench_hires(dev) -> dict:
              for t in ("fp32", "fp64")]
         budgets[k] = abs(b[0] - b[1]) / max(abs(b[1]), 1e-300)
     res["fp64_budget_rel_diff"] = budgets
+    res["shard_8way"] = bench_shard(dev, st, out)
 
     # batched adjoint: 8 members at 240 x 240, FP32, concurrent on 4 streams
     members = [synth.config4_adjoint_member(mm).to(dev, torch.float32) for mm in range(8)]
