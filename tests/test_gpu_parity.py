@@ -242,6 +242,39 @@ def test_evp_packed_constants(cuda, case):
         _assert_bits(f"{name}[{case}]", a, b)
 
 
+def test_concurrent_first_calls_capture_one_graph_at_a_time(cuda):
+    """Two host threads making their first subcycle calls at once (each on its
+    own stream and workspace, so each captures its own graph) get the same
+    bits as a call made alone."""
+    import threading
+
+    st = _spun_up(synth.tiny(20, 14, config=31), cuda, n_spin=3)
+    want = [g2n(t) for t in K.evp_solver(st.to(cuda), n_substeps=37)]
+    results, errors = [None, None], []
+
+    def worker(k):
+        try:
+            s_ = torch.cuda.Stream(cuda)
+            with torch.cuda.stream(s_):
+                g = st.to(cuda)
+                out = K.evp_solver(g, n_substeps=41 + k)  # new substep counts: new graphs
+                out = K.evp_solver(g, n_substeps=37)
+            s_.synchronize()
+            results[k] = [g2n(t) for t in out]
+        except Exception as e:  # noqa: BLE001
+            errors.append(e)
+
+    threads = [threading.Thread(target=worker, args=(k,)) for k in range(2)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join(timeout=120)
+    assert not errors, errors
+    for k in range(2):
+        for name, a, b in zip(EVP_NAMES, results[k], want):
+            assert_exact(f"{name}[thread {k}]", a, b)
+
+
 @pytest.mark.parametrize("kind", ["global", "regional"])
 def test_freedrift_solver(cuda, kind):
     """Free drift (closed-form c-point solution with atan2, cos, sin): the

@@ -39,6 +39,7 @@ int make_geom(const vh_domain* d, Geom* g) {
 
 namespace {
 std::mutex g_graph_mu;
+std::mutex g_capture_mu;  // held for a whole capture on the per-device capture stream
 std::map<std::pair<int, std::string>, hipGraphExec_t> g_graphs;
 std::map<int, hipStream_t> g_capture_streams;
 }  // namespace
@@ -62,30 +63,31 @@ int graph_run(const std::string& key, const std::function<void(hipStream_t)>& re
     std::lock_guard<std::mutex> lk(g_graph_mu);
     auto it = g_graphs.find({dev, key});
     if (it != g_graphs.end()) exec = it->second;
-    auto ct = g_capture_streams.find(dev);
-    if (ct != g_capture_streams.end()) cs = ct->second;
   }
   if (!exec) {
-    if (!cs) {
-      VH_TRY_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+    // one capture at a time; another thread may have captured this key meanwhile
+    std::lock_guard<std::mutex> cap(g_capture_mu);
+    {
       std::lock_guard<std::mutex> lk(g_graph_mu);
-      auto ins = g_capture_streams.emplace(dev, cs);
-      if (!ins.second) {
-        hipStreamDestroy(cs);
-        cs = ins.first->second;
-      }
+      auto it = g_graphs.find({dev, key});
+      if (it != g_graphs.end()) exec = it->second;
+      auto ct = g_capture_streams.find(dev);
+      if (ct != g_capture_streams.end()) cs = ct->second;
     }
-    hipGraph_t graph = nullptr;
-    VH_TRY_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeRelaxed));
-    record(cs);
-    VH_TRY_HIP(hipStreamEndCapture(cs, &graph));
-    VH_TRY_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
-    VH_TRY_HIP(hipGraphDestroy(graph));
-    std::lock_guard<std::mutex> lk(g_graph_mu);
-    auto ins = g_graphs.emplace(std::make_pair(dev, key), exec);
-    if (!ins.second) {
-      hipGraphExecDestroy(exec);
-      exec = ins.first->second;
+    if (!exec) {
+      if (!cs) {
+        VH_TRY_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+        std::lock_guard<std::mutex> lk(g_graph_mu);
+        g_capture_streams.emplace(dev, cs);
+      }
+      hipGraph_t graph = nullptr;
+      VH_TRY_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeRelaxed));
+      record(cs);
+      VH_TRY_HIP(hipStreamEndCapture(cs, &graph));
+      VH_TRY_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+      VH_TRY_HIP(hipGraphDestroy(graph));
+      std::lock_guard<std::mutex> lk(g_graph_mu);
+      g_graphs.emplace(std::make_pair(dev, key), exec);
     }
   }
   VH_TRY_HIP(hipGraphLaunch(exec, st));
