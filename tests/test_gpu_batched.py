@@ -76,6 +76,48 @@ def test_batched_packed_constants_fallback(cuda, case):
             assert_exact(f"member {m} {name}[{case}]", g2n(ens.view(out[q], m)), want[q])
 
 
+def test_batched_smaller_workspaces(cuda):
+    """A workspace without room for the tile tables (each workgroup then
+    searches the member table), and one without room for the packed step
+    constants either (full fields), give the same bits as the full one."""
+    import ctypes
+
+    from veris_amd import _native as N
+
+    states = _members()
+    ens = Ensemble(states, device=cuda, dtype=torch.float64)
+    tile, n = 5, 11
+    want = [o.clone() for o in ens.evp(n, tile=tile)]
+    b = ens._batch(tile)
+    full = ctypes.c_size_t(0)
+    N.call("vh_evp_batched_workspace_bytes", ctypes.byref(b), ctypes.byref(full))
+    ri, rj, k = N.evp_tile_shapes()[tile]
+    r5, j5, k5 = N.evp_tile_shapes()[5]
+
+    def tiles(ri_, rj_, k_):
+        TI, TJ = ri_ - 4 * k_, rj_ - 4 * k_
+        return sum(-(-a // TI) * -(-b_ // TJ) for a, b_ in ens.shapes)
+
+    align = lambda x: (x + 255) // 256 * 256  # noqa: E731
+    no_tables = full.value - align((tiles(ri, rj, k) + tiles(r5, j5, k5)) * 48)
+    rows = sum(b_ for _, b_ in ens.shapes)
+    no_pack = no_tables - align(ens.total) - align(12 * rows * 8)
+    sett = N.settings_struct(ens.settings)
+    fields = ens._fields_struct()
+    stream = torch.cuda.current_stream(cuda).cuda_stream
+    for size in (no_tables, no_pack):
+        ws = torch.empty(size, dtype=torch.uint8, device=cuda)
+        out = [torch.empty(ens.total, dtype=torch.float64, device=cuda) for _ in range(5)]
+        N.call("vh_evp_solver_batched", ctypes.byref(b), ctypes.byref(sett), ctypes.byref(fields),
+               n, *[t.data_ptr() for t in out], ws.data_ptr(), size, stream)
+        for q, name in enumerate(EVP_NAMES):
+            assert torch.equal(out[q], want[q]), f"{name} differs with a {size}-byte workspace"
+    ws = torch.empty(no_pack - 256, dtype=torch.uint8, device=cuda)
+    with pytest.raises(N.VerisHipError, match="VH_EWORKSPACE"):
+        N.call("vh_evp_solver_batched", ctypes.byref(b), ctypes.byref(sett), ctypes.byref(fields),
+               n, *[t.data_ptr() for t in out], ws.data_ptr(), no_pack - 256, stream)
+
+
 def test_batched_pre_evp_and_budgets(cuda):
     states = [synth.regional(23, m, nx, ny) for m, (nx, ny) in enumerate([(40, 30), (25, 61)])]
     ens = Ensemble(states, device=cuda, dtype=torch.float64)

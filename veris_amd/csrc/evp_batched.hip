@@ -36,12 +36,17 @@ int n_tiles_of(int cfg, const Geom& g) {
 }
 
 // Workspace: ping-pong state, argument block, member table (the minimum,
-// `base`), then the packed constants (`total`; without room for them the
-// launches read the full fields).
+// `base`), then the packed constants (`packed`) and the tile tables of the
+// main and remainder shapes (`total`).  Without room for the packed constants
+// the launches read the full fields; without room for the tables they search
+// the member table per tile.
 struct Layout {
-  size_t fb, ab, mem, base, total;
-  int rows;  // metric rows of all members (sum of padded ny)
+  size_t fb, ab, mem, base, packed, total;
+  int rows;             // metric rows of all members (sum of padded ny)
+  int n_main, n_rem;    // tiles of the main and remainder shapes
 };
+
+int batch_cfg(const vh_batch* b) { return (b->tile > 0) ? b->tile : 3; }
 
 template <typename T>
 Layout layout_of(const vh_batch* b) {
@@ -51,8 +56,17 @@ Layout layout_of(const vh_batch* b) {
   L.mem = align256((size_t)b->n_members * sizeof(MemberDesc));
   L.base = 5 * L.fb + L.ab + L.mem;
   L.rows = 0;
-  for (int m = 0; m < b->n_members; ++m) L.rows += b->members[m].ny + 4;
-  L.total = L.base + evp_pack_bytes<T>((size_t)b->total_elems, (size_t)L.rows);
+  L.n_main = 0;
+  L.n_rem = 0;
+  for (int m = 0; m < b->n_members; ++m) {
+    Geom g;
+    member_geom(b->members[m], &g);
+    L.rows += g.NY;
+    L.n_main += n_tiles_of(batch_cfg(b), g);
+    L.n_rem += n_tiles_of(kEvpRemainderCfg, g);
+  }
+  L.packed = L.base + evp_pack_bytes<T>((size_t)b->total_elems, (size_t)L.rows);
+  L.total = L.packed + align256((size_t)(L.n_main + L.n_rem) * sizeof(TileDesc));
   return L;
 }
 
@@ -106,7 +120,8 @@ int run_batched(const vh_batch* b, const vh_settings* s, const vh_fields* fl, in
   const Layout L = layout_of<T>(b);
   VH_REQUIRE(ws != nullptr && wsb >= L.base, VH_EWORKSPACE,
              "batched evp workspace: need %zu bytes, got %zu", L.total, wsb);
-  const bool pack = wsb >= L.total;
+  const bool pack = wsb >= L.packed;
+  const bool tables = wsb >= L.total;
   const void* req[kEvpMaxRequired];
   const int nreq = evp_required_fields(fl, req);
   for (int q = 0; q < nreq; ++q)
@@ -139,7 +154,7 @@ int run_batched(const vh_batch* b, const vh_settings* s, const vh_fields* fl, in
     return VH_OK;
   }
 
-  const int cfg = (b->tile > 0) ? b->tile : 3;
+  const int cfg = batch_cfg(b);
   VH_REQUIRE(s->noSlip == 0.0 || s->secondOrderBC == 0.0 || kEvpTiles[cfg].k == 1,
              VH_EUNSUPPORTED, "secondOrderBC needs a single-substep tile shape (tile %d has k=%d)",
              cfg, kEvpTiles[cfg].k);
@@ -184,8 +199,21 @@ int run_batched(const vh_batch* b, const vh_settings* s, const vh_fields* fl, in
   ab.members = mem_dev;
   ab.n_members = b->n_members;
   if (pack) evp_pack_attach(&ab, base + L.base, (size_t)b->total_elems, L.rows);
+  VH_REQUIRE(n_main == L.n_main && n_rem == L.n_rem, VH_EINVAL, "batched evp: tile count mismatch");
+  if (tables) {
+    ab.tiles[0] = (const TileDesc*)(base + L.packed);
+    ab.tiles[1] = ab.tiles[0] + n_main;
+  }
   hipLaunchKernelGGL(evp_write_args_kernel<T>, dim3(1), dim3(64), 0, st, ab_dev, ab);
   if (pack) evp_pack_launch(ab_dev, max_cells, b->n_members, st);
+  if (tables) {
+    const int tabs[2][2] = {{cfg, n_main}, {kEvpRemainderCfg, n_rem}};
+    for (int w = 0; w < 2; ++w)
+      if (tabs[w][1] > 0)
+        hipLaunchKernelGGL(evp_tile_table_kernel<T>, dim3((tabs[w][1] + 255) / 256), dim3(256), 0,
+                           st, ab_dev, w, evp_tile_extent_i(tabs[w][0]),
+                           evp_tile_extent_j(tabs[w][0]), tabs[w][1]);
+  }
   VH_TRY_HIP(hipGetLastError());
   const int K = kEvpTiles[cfg].k;
   const bool so = ab.sc.sobc != 0;

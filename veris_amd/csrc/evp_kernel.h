@@ -88,6 +88,16 @@ struct MemberDesc {
   int pad;
 };
 
+// Placement of one tile of a batched launch (its member's offset, extents and
+// row offset, and the tile's origin in the member), written once per call so
+// that a workgroup finds its tile with one burst of scalar loads instead of a
+// search over the member table.
+struct TileDesc {
+  long long off;
+  int NX, NY, nx, ny, fill_x, fill_y;
+  int i0, j0, roff, pad;
+};
+
 // Up to kMembersPerWrite member descriptors, passed by value to the kernel
 // that stores them (kernel arguments are copied when the launch is enqueued).
 constexpr int kMembersPerWrite = 48;
@@ -123,6 +133,7 @@ struct EvpArgBlock {
   T* rows;
   int rstride;  // elements from one metric's rows to the next
   int packed;
+  const TileDesc* tiles[2];  // batched: per-tile placement of the main and remainder shapes (or null)
 };
 
 
@@ -304,6 +315,40 @@ __global__ __launch_bounds__(256) void evp_pack_kernel(EvpArgBlock<T>* ab) {
   if (!ok) ab->packed = 0;
 }
 
+// Tile table of a batched launch shape (`which`: 0 main, 1 remainder) with
+// tile extents TI x TJ: one thread per tile finds its member in the member
+// table (the last member whose first tile is at or before it).
+template <typename T>
+__global__ __launch_bounds__(256) void evp_tile_table_kernel(EvpArgBlock<T>* ab, int which, int TI,
+                                                             int TJ, int n_tiles) {
+  const int tile = blockIdx.x * 256 + threadIdx.x;
+  if (tile >= n_tiles) return;
+  const MemberDesc* ms = ab->members;
+  int lo = 0, hi = ab->n_members - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (ms[mid].tb[which] <= tile) lo = mid;
+    else hi = mid - 1;
+  }
+  const MemberDesc& md = ms[lo];
+  const int local = tile - md.tb[which];
+  const int tj_m = (md.g.NY + TJ - 1) / TJ;
+  const int ti = local / tj_m;
+  TileDesc d;
+  d.off = md.off;
+  d.NX = md.g.NX;
+  d.NY = md.g.NY;
+  d.nx = md.g.nx;
+  d.ny = md.g.ny;
+  d.fill_x = md.g.fill_x;
+  d.fill_y = md.g.fill_y;
+  d.i0 = ti * TI;
+  d.j0 = (local - ti * tj_m) * TJ;
+  d.roff = md.roff;
+  d.pad = 0;
+  ((TileDesc*)ab->tiles[which])[tile] = d;
+}
+
 // Points the host-side block at packed storage `buf` (evp_pack_bytes<T>(elems,
 // rows) bytes) and marks it packed; evp_pack_launch then fills it on device.
 template <typename T>
@@ -380,7 +425,22 @@ __global__ __launch_bounds__(RI* RJ) void evp_substep_kernel(const EvpArgBlock<T
     long long off = 0;
     int roff = 0;
     int i0, j0;
-    if (BATCHED) {
+    if (BATCHED && ab->tiles[which] != nullptr) {
+      // the tile's placement from the per-call tile table
+      const TileDesc* td = ab->tiles[which] + tile;
+      g.NX = uniform(td->NX);
+      g.NY = uniform(td->NY);
+      g.nx = uniform(td->nx);
+      g.ny = uniform(td->ny);
+      g.ol = 2;
+      g.fill_x = uniform(td->fill_x);
+      g.fill_y = uniform(td->fill_y);
+      const long long to = td->off;
+      off = ((long long)(unsigned)uniform((int)(to >> 32)) << 32) | (unsigned)uniform((int)to);
+      roff = uniform(td->roff);
+      i0 = uniform(td->i0);
+      j0 = uniform(td->j0);
+    } else if (BATCHED) {
       // member owning this tile: last m with tb[which] <= tile (uniform search)
       const MemberDesc* ms = ab->members;
       int lo = 0, hi = uniform(ab->n_members) - 1;
