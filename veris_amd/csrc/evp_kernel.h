@@ -8,9 +8,11 @@
 // Work decomposition.  A workgroup of RI x RJ threads owns a TI x TJ tile
 // (TI = RI - 4K, TJ = RJ - 4K, j contiguous) plus a ring of 2K cells, ONE
 // THREAD PER CELL.  Every thread issues all global loads its cell needs at
-// once (at 1 degree the grid is ~1 wave per SIMD, so memory latency, not
-// bandwidth, sets the time) and keeps its velocity and stresses in registers
-// across the K substeps.  Each substep runs in LDS-exchange stages separated
+// once (at 1 degree the grid is under two waves per SIMD, so memory latency,
+// not bandwidth, sets the time) and keeps its velocity and stresses in
+// registers across the K substeps.  On grids whose masks are 0/1 and whose
+// metrics depend on j alone, the masks come as bits of one byte per cell and
+// the metrics as rows along j in LDS (evp_pack_kernel).  Each substep runs in LDS-exchange stages separated
 // by barriers:
 //   A  u, v, masks, u-uOcean, v-vOcean -> LDS
 //   B  e12 and q = rAz*e12^2 on z-points
