@@ -527,7 +527,10 @@ def main(argv=None) -> int:
     kernel = f"evp_substep_kernel<double,double,{ri},{rj},{kk},false>"
 
     line = {
-        "metric": "EVP cell-substeps/s at 1 degree (plain FP64 GPU drop-in, full step)",
+        # the benchmark's metric; `value` is its cell-substeps/s leg on the
+        # workload it is quoted on (the plain FP64 drop-in, 1-degree full step);
+        # the two ratios it names are under "ratios"
+        "metric": "EVP cell-substeps/s at 1\u00b0 (FP32 batched vs plain FP64); adjoint/forward ratio",
         "value": value,
         "unit": "cell-substeps/s",
         "n_gpus": 1,
@@ -565,6 +568,13 @@ def main(argv=None) -> int:
         line["batched"] = bench_batched(dev, nsub, evp_rate, value)
         line["adjoint"] = adjoint
         line["hires"] = bench_hires(dev)
+        line["ratios"] = {
+            "fp32_batched_over_plain_fp64_evp": line["batched"]["fp32_over_plain_fp64_evp_1deg"],
+            "fp32_batched_target": line["batched"]["target_ratio"],
+            "adjoint_over_forward_fp64": line["adjoint"]["fp64"]["adjoint_over_plain_forward"],
+            "adjoint_over_forward_fp32": line["adjoint"]["fp32"]["adjoint_over_plain_forward"],
+            "adjoint_target": line["adjoint"]["fp64"]["target"],
+        }
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(nsub)
     print(json.dumps(line), flush=True)
