@@ -46,4 +46,11 @@ int seaice_step_ref(int NX, int NY, int nf, const char* const* fnames, double* c
 /* name of the field or setting a failed seaice_step_ref call did not find */
 const char* step_ref_missing(void);
 
+/* The open-ocean bulk fluxes on n cells (step_ref.c); out[9] = flwupa, flha,
+ * fsha, df0dT, ust, vst, evp, ssq, dEvdT. */
+int bulkf_ref(int n, const double* uw, const double* vw, const double* ta, const double* qa,
+              const double* tsf, double grav, double karman, double zvir, double latvap,
+              double gamma_blk, double rhoAir, double cpdair, double emis, double stefBoltz,
+              double aln, double rdn, double* const out[9]);
+
 #endif

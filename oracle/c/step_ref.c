@@ -647,3 +647,62 @@ int seaice_step_ref(int NX, int NY, int nf, const char* const* fnames, double* c
   for (int k = 0; k < 5; ++k) fill(S, F_(S, fills[k]));
   return S->missing ? -2 : 0;
 }
+
+/* veris/heat_flux_MITgcm.py:5-123 (bulkf_formula_lanl), cell by cell.  The
+ * scalars aln = log(ht/zref) and rdn = karman/log(zref/zice) come from the
+ * caller so that NumPy's and libm's log agree on them by construction;
+ * out[] = flwupa, flha, fsha, df0dT, ust, vst, evp, ssq, dEvdT. */
+int bulkf_ref(int n, const double* uw, const double* vw, const double* ta, const double* qa,
+              const double* tsf, double grav, double karman, double zvir, double latvap,
+              double gamma_blk, double rhoAir, double cpdair, double emis, double stefBoltz,
+              double aln, double rdn, double* const out[9]) {
+  const double ht = 2.0, zref = 10.0;
+  const double czol = zref * karman * grav;
+  for (int c = 0; c < n; ++c) {
+    const double lath = latvap;
+    const double us = sqrt(uw[c] * uw[c] + vw[c] * vw[c]);
+    const double usm = npmax(us, 1.0);
+    const double t0 = ta[c] * (1.0 + zvir * qa[c]);
+    const double ssq = 3.797915 * exp(lath * (7.93252e-6 - 2.166847e-3 / tsf[c])) / 1013.0;
+    const double deltap = ta[c] - tsf[c] + gamma_blk * ht;
+    const double delq = qa[c] - ssq;
+    const double rhn = rdn, ren = rdn;
+    double ustar = rdn * usm, tstar = rhn * deltap, qstar = ren * delq;
+    double rd = rdn, rh = rhn, re = ren;
+    for (int it = 0; it < 5; ++it) {
+      double huol = czol / (ustar * ustar) * (tstar / t0 + qstar / (1.0 / zvir + qa[c]));
+      huol = npmin(fabs(huol), 10.0) * npsign(huol);
+      const double stable = 0.5 + 0.5 * npsign(huol);
+      const double xsq = npmax(sqrt(fabs(1.0 - 16.0 * huol)), 1.0);
+      const double x = sqrt(xsq);
+      const double psimh = -5.0 * huol * stable +
+                           (1.0 - stable) * (2.0 * log(0.5 * (1.0 + x)) + 2.0 * log(0.5 * (1.0 + xsq)) -
+                                             2.0 * atan(x) + M_PI * 0.5);
+      const double psixh = -5.0 * huol * stable + (1.0 - stable) * (2.0 * log(0.5 * (1.0 + xsq)));
+      rd = rdn / (1.0 + rdn * (aln - psimh) / karman);
+      rh = rhn / (1.0 + rhn * (aln - psixh) / karman);
+      re = rh;
+      ustar = rd * usm;
+      qstar = re * delq;
+      tstar = rh * deltap;
+    }
+    const double csha = rhoAir * cpdair * us * rh * rd;
+    const double clha = rhoAir * lath * us * re * rd;
+    const double fsha = csha * deltap;
+    const double flha = clha * delq;
+    const double t = tsf[c];
+    const double devdt = clha * ssq * 2.166847e-3 / (t * t);
+    const double dflwupdt = 4.0 * emis * stefBoltz * pow(t, 3.0);
+    const double cdn = 2.7e-3 / usm + 0.142e-3 + 0.0764e-3 * usm;
+    out[0][c] = emis * stefBoltz * pow(t, 4.0);
+    out[1][c] = flha;
+    out[2][c] = fsha;
+    out[3][c] = -dflwupdt + -csha + -lath * devdt;
+    out[4][c] = rhoAir * cdn * us * uw[c];
+    out[5][c] = rhoAir * cdn * us * vw[c];
+    out[6][c] = -flha / lath;
+    out[7][c] = ssq;
+    out[8][c] = devdt;
+  }
+  return 0;
+}

@@ -67,6 +67,9 @@ def _load():
                                         ctypes.POINTER(dp), ctypes.c_int, cpp, dp, ctypes.c_int]
         lib.seaice_step_ref.restype = ctypes.c_int
         lib.step_ref_missing.restype = ctypes.c_char_p
+        lib.bulkf_ref.argtypes = ([ctypes.c_int] + [dp] * 5 + [ctypes.c_double] * 11 +
+                                  [ctypes.POINTER(dp)])
+        lib.bulkf_ref.restype = ctypes.c_int
         _lib = lib
     return _lib
 
@@ -152,3 +155,24 @@ def seaice_step(state, qnet_forcing=None, swdown=None, n_substeps=None, strength
     if rc != 0:
         raise RuntimeError(f"seaice_step_ref returned {rc}")
     return arrays
+
+
+def bulkf_formula_lanl(state, uw, vw, ta, qa, tsf, grav=9.81):
+    """The open-ocean bulk fluxes (flwupa, flha, fsha, df0dT, ust, vst, evp,
+    ssq, dEvdT) of ``veris/heat_flux_MITgcm.py:5-123``, cell by cell."""
+    lib = _load()
+    s = state.settings
+    ins = [np.ascontiguousarray(a, dtype=np.float64) for a in (uw, vw, ta, qa, tsf)]
+    n = ins[0].size
+    out = [np.empty(ins[0].shape) for _ in range(9)]
+    dp = ctypes.POINTER(ctypes.c_double)
+    outs = (dp * 9)(*[o.ctypes.data_as(dp) for o in out])
+    aln = float(np.log(2.0 / 10.0))
+    rdn = float(s.karman / np.log(10.0 / 0.0005))
+    rc = lib.bulkf_ref(n, *[a.ctypes.data_as(dp) for a in ins], float(grav), float(s.karman),
+                       float(s.zvir), float(s.latvap), float(s.gamma_blk), float(s.rhoAir),
+                       float(s.cpdair), float(s.ocean_emissivity), float(s.stefBoltz), aln, rdn,
+                       outs)
+    if rc != 0:
+        raise RuntimeError(f"bulkf_ref returned {rc}")
+    return tuple(out)

@@ -22,6 +22,11 @@ must be equal bit for bit, and Growth's outputs, whose solve4temp and melt
 taper call pow() and exp(), must be bit-equal in at least 99% of the cells
 and within 1e-12 of scale everywhere.  Every planted mistake of the step
 functions must break that.
+
+The open-ocean bulk fluxes (the forcing producer of the step), all of whose
+outputs go through exp/log/atan/pow, are compared within 1e-13 of scale
+with at least 90% of the cells bit-equal, and every planted bulk-flux
+mistake must break that too.
 """
 
 from __future__ import annotations
@@ -339,3 +344,54 @@ def test_c_comparison_catches_step_mistake(mutation, tmp_path):
         if _step_mismatches(got, want):
             return
     pytest.fail(f"C comparison missed: {mutation}")
+
+
+# ---------------------------------------------------------------------------
+# the open-ocean bulk fluxes (the forcing producer of the step)
+# ---------------------------------------------------------------------------
+
+def _bulk_inputs(seed=77):
+    """Seeded open-ocean inputs spanning stable, neutral and unstable
+    stratification and calm (including exactly zero) to gale winds."""
+    st = synth.config0()
+    r = np.random.default_rng(seed)
+    shp = st.shape
+    tsf = r.uniform(271.0, 303.0, shp)
+    ins = [r.normal(0, 8, shp), r.normal(0, 8, shp), tsf + r.normal(0, 6, shp),
+           r.uniform(1e-3, 2e-2, shp), tsf]
+    ins[0][:3, :3] = 0.0
+    ins[1][:3, :3] = 0.0
+    return st, ins
+
+
+def _bulk_mismatches(oracle):
+    st, ins = _bulk_inputs()
+    want = oracle.bulkf_formula_lanl(st, *ins, st.variables.iceMask, grav=st.settings.gravity)
+    got = c_ref.bulkf_formula_lanl(st, *ins, grav=st.settings.gravity)
+    bad = []
+    for k, (a, b) in enumerate(zip(got, want)):
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        neq = int(np.count_nonzero(a.view(np.int64) != b.view(np.int64)))
+        if neq > 0.1 * a.size or not _rel(a, b) <= 1e-13:
+            bad.append((k, neq, _rel(a, b)))
+    return bad
+
+
+def test_bulk_fluxes_match_c_restatement():
+    """Every output within 1e-13 of scale and bit-equal in at least 90% of the
+    cells (exp/log/atan/pow of NumPy and of libm differ by an ulp here and
+    there)."""
+    assert not _bulk_mismatches(O), _bulk_mismatches(O)
+
+
+BULK_MUTATIONS = sorted(n for n, (a, _) in MUTATIONS.items()
+                        if a in inspect.getsource(O.bulkf_formula_lanl))
+
+
+def test_bulk_mutation_list_is_not_empty():
+    assert len(BULK_MUTATIONS) >= 8
+
+
+@pytest.mark.parametrize("mutation", BULK_MUTATIONS)
+def test_c_comparison_catches_bulk_flux_mistake(mutation, tmp_path):
+    assert _bulk_mismatches(_mutant(mutation, tmp_path)), f"C comparison missed: {mutation}"
