@@ -308,6 +308,62 @@ int evp_solver_ref(const double* settings, const double* const* fields, int nsub
   return 0;
 }
 
+/* Residual norms of one substep over the interior, veris/evp_solver.py:192-230
+ * (the previous iterate's (sigma1 +- sigma2)/2 as the previous sig11/sig22).
+ * Sums in index order. */
+static void residuals(const Ctx* C, double* const old[5], const double* u, const double* v,
+                      const double* s1, const double* s2, const double* s12, const double* aC,
+                      const double* aZ, double* rsig, double* rvel) {
+  const int adaptive = S(ES_ADAPTIVE) != 0.0;
+  double sig = 0.0, vel = 0.0;
+  for (int i = 2; i < C->NX - 2; ++i)
+    for (int j = 2; j < C->NY - 2; ++j) {
+      const size_t c = AT(i, j);
+      const double r11 = (((0.5 * (s1[c] + s2[c])) - (0.5 * (old[2][c] + old[3][c]))) * aC[c]) * F(EF_M, i, j);
+      const double r22 = (((0.5 * (s1[c] - s2[c])) - (0.5 * (old[2][c] - old[3][c]))) * aC[c]) * F(EF_M, i, j);
+      const double r12 = (s12[c] - old[4][c]) * aZ[c];
+      const double bU = adaptive ? 0.5 * (aC[c] + aC[AT(i - 1, j)]) : S(ES_EVPBETA);
+      const double bV = adaptive ? 0.5 * (aC[c] + aC[AT(i, j - 1)]) : S(ES_EVPBETA);
+      const double ru = (F(EF_MU, i, j) * (u[c] - old[0][c])) * bU;
+      const double rv = (F(EF_MV, i, j) * (v[c] - old[1][c])) * bV;
+      sig += (r11 * r11 + r22 * r22) + r12 * r12;
+      vel += ru * ru + rv * rv;
+    }
+  *rsig = sig;
+  *rvel = vel;
+}
+
+/* The subcycle with per-substep residual norms (computeEvpResidual). */
+int evp_solver_residuals_ref(const double* settings, const double* const* fields, int nsub,
+                             double* u, double* v, double* s1, double* s2, double* s12,
+                             double* resU, double* resSig) {
+  Ctx ctx = {(int)settings[ES_NX], (int)settings[ES_NY], settings, fields};
+  const Ctx* C = &ctx;
+  const size_t n = (size_t)C->NX * C->NY;
+  memcpy(u, fields[EF_UICE], n * sizeof(double));
+  memcpy(v, fields[EF_VICE], n * sizeof(double));
+  memcpy(s1, fields[EF_S1], n * sizeof(double));
+  memcpy(s2, fields[EF_S2], n * sizeof(double));
+  memcpy(s12, fields[EF_S12], n * sizeof(double));
+  double* w[26];
+  for (int k = 0; k < 26; ++k) {
+    w[k] = (double*)calloc(n, sizeof(double));
+    if (!w[k]) {
+      for (int m = 0; m < k; ++m) free(w[m]);
+      return -1;
+    }
+  }
+  double* const old[5] = {w[21], w[22], w[23], w[24], w[25]};
+  double* const cur[5] = {u, v, s1, s2, s12};
+  for (int k = 0; k < nsub; ++k) {
+    for (int q = 0; q < 5; ++q) memcpy(old[q], cur[q], n * sizeof(double));
+    substep(C, u, v, s1, s2, s12, w);
+    residuals(C, old, u, v, s1, s2, s12, w[6], w[7], &resSig[k], &resU[k]);
+  }
+  for (int k = 0; k < 26; ++k) free(w[k]);
+  return 0;
+}
+
 /* veris/freedrift_solver.py:5-62 */
 int freedrift_ref(const double* settings, const double* const* fields, double* u, double* v) {
   Ctx ctx = {(int)settings[ES_NX], (int)settings[ES_NY], settings, fields};

@@ -34,6 +34,11 @@ enum evp_ref_field {
 int evp_solver_ref(const double* settings, const double* const* fields, int nsub, double* u,
                    double* v, double* s1, double* s2, double* s12);
 
+/* the same with the residual norms of every substep (resU[k], resSig[k]) */
+int evp_solver_residuals_ref(const double* settings, const double* const* fields, int nsub,
+                             double* u, double* v, double* s1, double* s2, double* s12,
+                             double* resU, double* resSig);
+
 /* closed-form free drift; returns 0, or -1 when out of memory */
 int freedrift_ref(const double* settings, const double* const* fields, double* u, double* v);
 

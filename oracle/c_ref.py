@@ -60,6 +60,9 @@ def _load():
         dp = ctypes.POINTER(ctypes.c_double)
         lib.evp_solver_ref.argtypes = [dp, ctypes.POINTER(dp), ctypes.c_int] + [dp] * 5
         lib.evp_solver_ref.restype = ctypes.c_int
+        lib.evp_solver_residuals_ref.argtypes = ([dp, ctypes.POINTER(dp), ctypes.c_int] +
+                                                 [dp] * 7)
+        lib.evp_solver_residuals_ref.restype = ctypes.c_int
         lib.freedrift_ref.argtypes = [dp, ctypes.POINTER(dp), dp, dp]
         lib.freedrift_ref.restype = ctypes.c_int
         cpp = ctypes.POINTER(ctypes.c_char_p)
@@ -96,6 +99,21 @@ def evp_solver(state, n_substeps):
                             *[o.ctypes.data_as(dp) for o in out])
     if rc != 0:
         raise MemoryError("evp_solver_ref: allocation failed")
+    del keep
+    return tuple(out)
+
+
+def evp_solver_residuals(state, n_substeps):
+    """Returns (uIce, vIce, sigma1, sigma2, sigma12, resU, resSig) after
+    n_substeps, resU and resSig one value per substep."""
+    lib = _load()
+    shape, sett, keep, ptrs = _pack(state)
+    out = [np.empty(shape) for _ in range(5)] + [np.empty(n_substeps) for _ in range(2)]
+    dp = ctypes.POINTER(ctypes.c_double)
+    rc = lib.evp_solver_residuals_ref(sett.ctypes.data_as(dp), ptrs, int(n_substeps),
+                                      *[o.ctypes.data_as(dp) for o in out])
+    if rc != 0:
+        raise MemoryError("evp_solver_residuals_ref: allocation failed")
     del keep
     return tuple(out)
 

@@ -23,6 +23,10 @@ taper call pow() and exp(), must be bit-equal in at least 99% of the cells
 and within 1e-12 of scale everywhere.  Every planted mistake of the step
 functions must break that.
 
+The residual norms of ``computeEvpResidual`` are compared within 1e-12 per
+substep (FP64 sums in another order), the subcycle that carries them bit for
+bit, and every planted residual mistake must break that.
+
 The open-ocean bulk fluxes (the forcing producer of the step), all of whose
 outputs go through exp/log/atan/pow, are compared within 1e-13 of scale
 with at least 90% of the cells bit-equal, and every planted bulk-flux
@@ -395,3 +399,39 @@ def test_bulk_mutation_list_is_not_empty():
 @pytest.mark.parametrize("mutation", BULK_MUTATIONS)
 def test_c_comparison_catches_bulk_flux_mistake(mutation, tmp_path):
     assert _bulk_mismatches(_mutant(mutation, tmp_path)), f"C comparison missed: {mutation}"
+
+
+# ---------------------------------------------------------------------------
+# residual norms of the subcycle (computeEvpResidual)
+# ---------------------------------------------------------------------------
+
+RES_VARIANTS = ("default", "adaptive", "adaptive_regional", "side_drag_adaptive")
+
+
+def _residual_mismatches(oracle, variant, n=9):
+    st = _state(variant)
+    want = oracle.evp_solver_residuals(st, n)
+    got = c_ref.evp_solver_residuals(st, n)
+    bad = [k for k in range(5) if not np.array_equal(got[k], want[k])]
+    for k in (5, 6):  # FP64 sums in another order: 1e-12 per substep
+        rel = np.abs(got[k] - want[k]) / np.maximum(np.abs(want[k]), 1e-300)
+        if not float(rel.max()) <= 1e-12:
+            bad.append(k)
+    return bad
+
+
+@pytest.mark.parametrize("variant", RES_VARIANTS)
+def test_residuals_match_c_restatement(variant):
+    """The subcycle bit for bit, the per-substep residual norms (sums in
+    another order) within 1e-12."""
+    assert not _residual_mismatches(O, variant)
+
+
+RES_MUTATIONS = sorted(n for n, (a, _) in MUTATIONS.items()
+                       if a in inspect.getsource(O.evp_residuals))
+
+
+@pytest.mark.parametrize("mutation", RES_MUTATIONS)
+def test_c_comparison_catches_residual_mistake(mutation, tmp_path):
+    mod = _mutant(mutation, tmp_path)
+    assert any(_residual_mismatches(mod, v) for v in RES_VARIANTS), f"missed: {mutation}"
