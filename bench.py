@@ -10,8 +10,9 @@ the GPU before timing starts.
 steps (cell-substeps/s).  ``roofline`` covers the dominant kernel, the fused
 EVP substep: algorithmic bytes per launch (38 words per interior cell, 57,600
 cells) over its average launch time measured with HIP events on the stream the
-substeps are launched on.  ``cpu_baseline`` times the NumPy FP64 oracle (one
-thread) on one 1-degree step.
+substeps are launched on.  ``cpu_baseline`` times the scalar-loop C
+restatement of the step (one thread, FP64) on full 1-degree steps, and one
+step of the NumPy FP64 restatement beside it.
 
 Beside the headline line the same JSON object carries the other measured
 workloads (skip them with ``--no-extras``):
@@ -68,9 +69,12 @@ def measured_traffic(kernel: str):
 
 
 def cpu_baseline(n_substeps: int, min_seconds: float = 10.0, max_steps: int = 12) -> dict:
-    """NumPy FP64 oracle on full 1-degree steps (single thread: elementwise
-    NumPy), repeated until at least ``min_seconds`` of CPU work."""
+    """The CPU restatements on full 1-degree steps, single thread: the
+    scalar-loop C one (gcc -O2, the value) repeated until at least
+    ``min_seconds`` of CPU work, then one step of the NumPy FP64 one for
+    comparison."""
     os.environ.setdefault("OMP_NUM_THREADS", "1")
+    from oracle import c_ref
     from oracle import veris_np as O
     from veris_amd import synth
 
@@ -78,20 +82,27 @@ def cpu_baseline(n_substeps: int, min_seconds: float = 10.0, max_steps: int = 12
     q0 = st.variables.Qnet.copy()
     sw = st.variables.SWdown.copy()
     nx, ny = st.settings.nx, st.settings.ny
+    c_ref.build()
     steps = 0
     t0 = time.perf_counter()
     while steps < max_steps and (steps == 0 or time.perf_counter() - t0 < min_seconds):
-        O.seaice_step(st, q0, sw, n_substeps=n_substeps)
+        c_ref.seaice_step(st, q0, sw, n_substeps=n_substeps)
         steps += 1
     dt = time.perf_counter() - t0
+    t1 = time.perf_counter()
+    O.seaice_step(st, q0, sw, n_substeps=n_substeps)
+    dt_np = time.perf_counter() - t1
     return {
         "value": nx * ny * n_substeps * steps / dt,
         "unit": "cell-substeps/s",
         "cores": 1,
         "kind": "port",
         "sample": f"{steps} full step{'s' if steps > 1 else ''} of the 1-degree workload "
-                  f"({n_substeps} EVP substeps each), "
-                  f"NumPy FP64 oracle, {dt:.1f} s, host affinity {len(os.sched_getaffinity(0))} cpus",
+                  f"({n_substeps} EVP substeps each) through the scalar-loop C restatement "
+                  f"(gcc -O2, FP64), {dt:.1f} s, host affinity "
+                  f"{len(os.sched_getaffinity(0))} cpus",
+        "numpy_port_value": nx * ny * n_substeps / dt_np,
+        "numpy_port_sample": f"one full step through the NumPy FP64 restatement, {dt_np:.1f} s",
     }
 
 
