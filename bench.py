@@ -11,8 +11,8 @@ steps (cell-substeps/s).  ``roofline`` covers the dominant kernel, the fused
 EVP substep: algorithmic bytes per launch (38 words per interior cell, 57,600
 cells) over its average launch time measured with HIP events on the stream the
 substeps are launched on.  ``cpu_baseline`` times the scalar-loop C
-restatement of the step (one thread, FP64) on full 1-degree steps, and one
-step of the NumPy FP64 restatement beside it.
+restatement of the step (one thread, FP64) on full 1-degree steps and one step
+of the NumPy FP64 restatement, and reports the faster.
 
 Beside the headline line the same JSON object carries the other measured
 workloads (skip them with ``--no-extras``):
@@ -70,9 +70,8 @@ def measured_traffic(kernel: str):
 
 def cpu_baseline(n_substeps: int, min_seconds: float = 10.0, max_steps: int = 12) -> dict:
     """The CPU restatements on full 1-degree steps, single thread: the
-    scalar-loop C one (gcc -O2, the value) repeated until at least
-    ``min_seconds`` of CPU work, then one step of the NumPy FP64 one for
-    comparison."""
+    scalar-loop C one (gcc -O2) repeated until at least ``min_seconds`` of CPU
+    work, then one step of the NumPy FP64 one; the value is the faster."""
     os.environ.setdefault("OMP_NUM_THREADS", "1")
     from oracle import c_ref
     from oracle import veris_np as O
@@ -92,17 +91,24 @@ def cpu_baseline(n_substeps: int, min_seconds: float = 10.0, max_steps: int = 12
     t1 = time.perf_counter()
     O.seaice_step(st, q0, sw, n_substeps=n_substeps)
     dt_np = time.perf_counter() - t1
+    c_rate = nx * ny * n_substeps * steps / dt
+    np_rate = nx * ny * n_substeps / dt_np
+    c_sample = (f"{steps} full step{'s' if steps > 1 else ''} of the 1-degree workload "
+                f"({n_substeps} EVP substeps each) through the scalar-loop C restatement "
+                f"(gcc -O2, FP64), {dt:.1f} s")
+    np_sample = (f"one full step of the 1-degree workload ({n_substeps} EVP substeps) through "
+                 f"the NumPy FP64 restatement, {dt_np:.1f} s")
+    # the value is the faster of the two single-thread CPU restatements
+    best_c = c_rate >= np_rate
     return {
-        "value": nx * ny * n_substeps * steps / dt,
+        "value": max(c_rate, np_rate),
         "unit": "cell-substeps/s",
         "cores": 1,
         "kind": "port",
-        "sample": f"{steps} full step{'s' if steps > 1 else ''} of the 1-degree workload "
-                  f"({n_substeps} EVP substeps each) through the scalar-loop C restatement "
-                  f"(gcc -O2, FP64), {dt:.1f} s, host affinity "
-                  f"{len(os.sched_getaffinity(0))} cpus",
-        "numpy_port_value": nx * ny * n_substeps / dt_np,
-        "numpy_port_sample": f"one full step through the NumPy FP64 restatement, {dt_np:.1f} s",
+        "sample": (c_sample if best_c else np_sample)
+                  + f"; host affinity {len(os.sched_getaffinity(0))} cpus",
+        "c_port": {"value": c_rate, "sample": c_sample},
+        "numpy_port": {"value": np_rate, "sample": np_sample},
     }
 
 
