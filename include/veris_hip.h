@@ -332,8 +332,10 @@ int vh_evp_tile_count(void);
 /* Launch shape of EVP tile index `tile`: ri x rj threads, one per cell of a
  * (ri-4k) x (rj-4k) tile plus a 2k-cell ring, running k substeps per launch.
  * k = 1 reproduces the reference on every cell; k > 1 keeps the velocities on
- * every cell and the stresses on every cell except the periodic overlap
- * columns whose reference values are wrap-around artefacts (see DESIGN). */
+ * every cell and the stresses on every cell except the x-periodic overlap
+ * columns: there k > 1 gives periodic values, while the reference's are
+ * numpy.roll wrap-around artefacts that need the post-fill velocities of both
+ * x edges within one substep. */
 int vh_evp_tile_shape(int32_t tile, int32_t* ri, int32_t* rj, int32_t* k);
 
 /* Workspace bytes vh_evp_solver needs for this domain. */

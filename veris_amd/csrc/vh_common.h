@@ -96,7 +96,8 @@ __device__ inline void store_filled(const Geom& g, P dst, int i, int j, T val) {
 // Replays the launch sequence that `record(stream)` issues as a hipGraph on
 // `stream`, capturing it once per (device, key) on a private stream.  The
 // key must name everything the recorded launches bake in (kernel arguments,
-// grid sizes).  Returns VH_OK or VH_EHIP.
+// grid sizes).  At most 256 graphs are kept per device; past that, new keys
+// issue their launches directly on `stream`.  Returns VH_OK or VH_EHIP.
 int graph_run(const std::string& key, const std::function<void(hipStream_t)>& record,
               hipStream_t stream);
 

@@ -44,6 +44,19 @@ std::map<std::pair<int, std::string>, hipGraphExec_t> g_graphs;
 std::map<int, hipStream_t> g_capture_streams;
 }  // namespace
 
+// Keys name the workspace address, so a caller that passes ever new
+// workspaces would grow the cache without end: once a device holds
+// kMaxGraphsPerDevice graphs, new keys launch their sequence directly on the
+// caller's stream instead (the cached graphs stay valid: another thread may
+// be about to launch one).
+constexpr size_t kMaxGraphsPerDevice = 256;
+
+size_t graphs_on(int dev) {
+  size_t n = 0;
+  for (const auto& kv : g_graphs) n += kv.first.first == dev;
+  return n;
+}
+
 int graphs_enabled() {
   static int en = -1;
   if (en < 0) {
@@ -67,12 +80,19 @@ int graph_run(const std::string& key, const std::function<void(hipStream_t)>& re
   if (!exec) {
     // one capture at a time; another thread may have captured this key meanwhile
     std::lock_guard<std::mutex> cap(g_capture_mu);
+    bool full = false;
     {
       std::lock_guard<std::mutex> lk(g_graph_mu);
       auto it = g_graphs.find({dev, key});
       if (it != g_graphs.end()) exec = it->second;
       auto ct = g_capture_streams.find(dev);
       if (ct != g_capture_streams.end()) cs = ct->second;
+      full = graphs_on(dev) >= kMaxGraphsPerDevice;
+    }
+    if (!exec && full) {
+      record(st);
+      VH_TRY_HIP(hipGetLastError());
+      return VH_OK;
     }
     if (!exec) {
       if (!cs) {
@@ -84,8 +104,10 @@ int graph_run(const std::string& key, const std::function<void(hipStream_t)>& re
       VH_TRY_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeRelaxed));
       record(cs);
       VH_TRY_HIP(hipStreamEndCapture(cs, &graph));
-      VH_TRY_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
-      VH_TRY_HIP(hipGraphDestroy(graph));
+      hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+      hipError_t de = hipGraphDestroy(graph);
+      VH_TRY_HIP(ie);
+      VH_TRY_HIP(de);
       std::lock_guard<std::mutex> lk(g_graph_mu);
       g_graphs.emplace(std::make_pair(dev, key), exec);
     }
