@@ -266,6 +266,19 @@ __global__ void evp_write_args_kernel(EvpArgBlock<T>* dst, EvpArgBlock<T> src) {
 constexpr int kEvpRowFields = 12;  // rdyU rdxV rAz rdxU rdyV rrA dyV dxU dxV dyU rrAu rrAv
 constexpr int kEvpMaskFields = 6;  // iceMask iceMaskU iceMaskV maskInC maskInU maskInV
 
+// Division and square root of the momentum update and the water drag.  FP64
+// arithmetic keeps the correctly rounded forms (the drop-in's bit-for-bit
+// parity); FP32 arithmetic takes the hardware reciprocal and square root
+// (about 1 ulp each, a third of the instructions of the rounded sequences).
+template <typename A>
+__device__ inline A evp_div(A a, A b) { return a / b; }
+template <>
+__device__ inline float evp_div<float>(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+template <typename A>
+__device__ inline A evp_sqrt(A x) { return sqrt(x); }
+template <>
+__device__ inline float evp_sqrt<float>(float x) { return __builtin_amdgcn_sqrtf(x); }
+
 __device__ inline unsigned long long bits_of(double x) { return (unsigned long long)__double_as_longlong(x); }
 __device__ inline unsigned long long bits_of(float x) { return __float_as_uint(x); }
 template <typename T>
@@ -724,7 +737,7 @@ __global__ __launch_bounds__(RI* RJ) void evp_substep_kernel(const EvpArgBlock<T
           const A D = (A)(fc < (A)0 ? sc.cwS : sc.cwN) * (A)sc.rhoSea;
           const A su = sDU[t] + sDU[t + RJ], sv = sDV[t] + sDV[t + 1];
           const A w = (A)0.25 * (su * su + sv * sv);
-          const A cd = ((D * D) * w > (A)sc.cDragMin2) ? D * sqrt(w) : (A)sc.cDragMin;
+          const A cd = ((D * D) * w > (A)sc.cDragMin2) ? D * evp_sqrt(w) : (A)sc.cDragMin;
           sCD[t] = cd * Mc;
           if (sc.basal) {
             const int cip = map_axis<K>(i + 1, g.NX, g.nx, g.fill_x) * g.NY + gj;
@@ -827,9 +840,9 @@ __global__ __launch_bounds__(RI* RJ) void evp_substep_kernel(const EvpArgBlock<T
           denV = denV + ((mV * (A)sc.sdCoeff) * sdmV) / (spV + (A)sc.sdU0);
         }
         const A un =
-            (MUc * (((bU * mU) * u + (mU * (A)sc.rdt) * uNm1 + fx) + divx)) / denU;
+            evp_div(MUc * (((bU * mU) * u + (mU * (A)sc.rdt) * uNm1 + fx) + divx), denU);
         const A vn =
-            (MVc * (((bV * mV) * v + (mV * (A)sc.rdt) * vNm1 + fy) + divy)) / denV;
+            evp_div(MVc * (((bV * mV) * v + (mV * (A)sc.rdt) * vNm1 + fy) + divy), denV);
         u = un;  // other threads read sU/sV, never these registers
         v = vn;
       }
