@@ -143,6 +143,19 @@ def _rel_l2(pairs):
     return float((num / den).sqrt()) if float(den) > 0 else 0.0
 
 
+def _rounded_to_fp32(state):
+    """The state with every floating-point field rounded to the nearest FP32
+    value (kept in FP64)."""
+    import numpy as np
+
+    vs = state.variables
+    for k in list(vs.keys()):
+        a = vs[k]
+        if isinstance(a, np.ndarray) and a.dtype == np.float64:
+            vs[k] = a.astype(np.float32).astype(np.float64)
+    return state
+
+
 def bench_batched(dev, nsub: int, fp64_evp_rate: float, fp64_step_rate: float) -> dict:
     import torch
 
@@ -178,6 +191,18 @@ def bench_batched(dev, nsub: int, fp64_evp_rate: float, fp64_step_rate: float) -
     err_a = {name: _rel_l2(list(zip(e32a.unpack(out32a[q]), e64.unpack(out64[q]), masks)))
              for q, name in enumerate(EVP_OUT)}
     del e32a
+    # the same FP64 subcycle on inputs rounded to FP32: what the rounding of
+    # the inputs alone moves the result by, and the FP32 run against FP64 on
+    # identical (FP32-representable) inputs
+    states_q = [_rounded_to_fp32(synth.config2_member(m)) for m in range(len(states))]
+    e64q = Ensemble(states_q, dev, torch.float64)
+    e64q.pre_evp()
+    out64q = e64q.evp(nsub, tile=tile64)
+    err_inputs = {name: _rel_l2(list(zip(e64q.unpack(out64q[q]), e64.unpack(out64[q]), masks)))
+                  for q, name in enumerate(EVP_OUT)}
+    err_same = {name: _rel_l2(list(zip(e32.unpack(out32[q]), e64q.unpack(out64q[q]), masks)))
+                for q, name in enumerate(EVP_OUT)}
+    del e64q
     b32 = e32.budgets()
     b64 = e64.budgets()
     budget_err = {k: float(((b32[k] - b64[k]).abs() / b64[k].abs().clamp_min(1e-300)).max())
@@ -204,6 +229,8 @@ def bench_batched(dev, nsub: int, fp64_evp_rate: float, fp64_step_rate: float) -
                           "frac": rate32 * B_ALG_WORDS * 4 / 1e9 / HBM_PEAK_GBS},
         "fp32_rel_l2_vs_fp64": err,
         "velocity_error_target": 1e-4,
+        "fp64_on_fp32_rounded_inputs_rel_l2_vs_fp64": err_inputs,
+        "fp32_rel_l2_vs_fp64_on_fp32_rounded_inputs": err_same,
         "fp32_storage_fp64_arith": {
             "value": cells * nsub / t32a,
             "tile": {"index": tile32a, "shape": shapes[tile32a]},
