@@ -125,6 +125,30 @@ def test_evp_config1_1deg(cuda):
         assert_exact(f"{name}[1deg]", a, b)
 
 
+def test_evp_config4_quarter_degree(cuda):
+    """The full 0.25-degree grid (1440 x 720 + halo, about a million cells,
+    the largest single domain here): EVP substeps bit-exact on every cell."""
+    st = _spun_up(synth.config4(), cuda, n_spin=3)
+    got, want = _evp_pair(st, 4, cuda)
+    for name, a, b in zip(EVP_NAMES, got, want):
+        assert_exact(f"{name}[0.25deg]", a, b)
+
+
+def test_full_step_config4_quarter_degree(cuda):
+    """One whole step on the full 0.25-degree grid (two EVP substeps to keep
+    the oracle quick); tolerance 1e-9 of each field's maximum, as for the
+    4-degree steps (device exp/pow in strength and growth)."""
+    st = synth.config4()
+    q0 = st.variables.Qnet.copy()
+    sw = st.variables.SWdown.copy()
+    g = st.to(cuda)
+    O.seaice_step(st, q0, sw, n_substeps=2)
+    seaice_step(g, torch.from_numpy(q0).to(cuda), torch.from_numpy(sw).to(cuda), n_substeps=2)
+    for name in ("uIce", "vIce", "sigma1", "sigma2", "sigma12", "OceanStressU", "OceanStressV",
+                 "hIceMean", "hSnowMean", "Area", "TSurf", "Qnet", "EmPmR", "SeaIceLoad"):
+        assert_close(f"0.25deg step {name}", g2n(g.variables[name]), st.variables[name], 1e-9)
+
+
 @pytest.mark.parametrize("variant", ["regional", "own_fill", "turning", "kterms", "tensile",
                                      "no_repl_pressure", "small_domain", "adaptive",
                                      "adaptive_regional", "side_drag", "side_drag_regional",
