@@ -32,7 +32,11 @@ workloads (skip them with ``--no-extras``):
 * ``hires``: the 0.25-degree 1440 x 720 grid, one full step with 800 EVP
   substeps in FP32 and in FP64, the error of every output of the FP32 step
   against the FP64 step, FP64 budget sums, and the batched adjoint of 8
-  members at 240 x 240.
+  members at 240 x 240;
+* ``variants``: the 1-degree subcycle with adaptive EVP, free slip with
+  lateral drag and the second-order boundary term, the residual-norm
+  subcycle, free drift and the open-ocean bulk fluxes, timed per substep or
+  per call.
 """
 
 from __future__ import annotations
@@ -377,6 +381,63 @@ def bench_shard(dev, st, stepped) -> dict:
     return res
 
 
+def bench_variants(dev, nsub: int) -> dict:
+    """The alternative dynamics settings and the forcing producer on the
+    1-degree workload, after one full step: microseconds per EVP substep
+    (median replay of the nsub-substep subcycle, its argument and packing
+    kernels included) with the HBM fraction at 38 words per cell-substep;
+    the residual-norm subcycle; free drift and the open-ocean bulk fluxes per
+    call, wall time of the Python drop-in call (argument marshalling
+    included; the bulk fluxes move 6 input and 9 output words per cell)."""
+    import numpy as np
+    import torch
+
+    from veris_amd import kernels as K
+    from veris_amd import synth
+    from veris_amd.heat_flux_MITgcm import bulkf_formula_lanl
+    from veris_amd.step import seaice_step
+
+    def prepared(**kw):
+        st = synth.config1(**kw)
+        g = st.to(dev)
+        seaice_step(g, torch.from_numpy(st.variables.Qnet.copy()).to(dev),
+                    torch.from_numpy(st.variables.SWdown.copy()).to(dev), n_substeps=nsub)
+        return st, g
+
+    def per_substep(fn, cells):
+        fn()
+        t = _sync_time(fn) / nsub
+        bps = B_ALG_WORDS * 8 * cells / t
+        return {"us_per_substep": t * 1e6, "cell_substeps_per_s": cells / t,
+                "hbm_frac_at_38_words": bps / 1e9 / HBM_PEAK_GBS}
+
+    out = {"workload": f"1-degree 360x160 + halo, FP64, {nsub} EVP substeps after one full step"}
+    for name, kw in (("default", {}), ("adaptive_evp", {"useAdaptiveEVP": True}),
+                     ("free_slip_side_drag", {"noSlip": False}),
+                     ("second_order_bc", {"secondOrderBC": True})):
+        st, g = prepared(**kw)
+        cells = st.settings.nx * st.settings.ny
+        out[name] = per_substep(lambda: K.evp_solver(g, nsub), cells)
+    st, g = prepared()
+    cells = st.settings.nx * st.settings.ny
+    out["residual_norms"] = per_substep(lambda: K.evp_solver_residuals(g, nsub), cells)
+    st, g = prepared(useFreedrift=True, useEVP=False)
+    K.freedrift_solver(g)
+    t = _sync_time(lambda: K.freedrift_solver(g))
+    out["free_drift"] = {"wall_us_per_call": t * 1e6, "cells_per_s": cells / t}
+    r = np.random.default_rng(77)
+    tsf = r.uniform(271.0, 303.0, st.shape)
+    ins = [r.normal(0, 8, st.shape), r.normal(0, 8, st.shape), tsf + r.normal(0, 6, st.shape),
+           r.uniform(1e-3, 2e-2, st.shape), tsf, st.variables.iceMask.copy()]
+    ins = [torch.from_numpy(a).to(dev) for a in ins]
+    bulkf_formula_lanl(g, *ins)
+    t = _sync_time(lambda: bulkf_formula_lanl(g, *ins))
+    padded = st.shape[0] * st.shape[1]
+    out["bulk_fluxes"] = {"wall_us_per_call": t * 1e6, "cells_per_s": padded / t,
+                          "hbm_frac_at_15_words": 15 * 8 * padded / t / 1e9 / HBM_PEAK_GBS}
+    return out
+
+
 def bench_hires(dev) -> dict:
     import torch
 
@@ -480,7 +541,7 @@ def main(argv=None) -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the batched, adjoint and 0.25-degree measurements")
-    ap.add_argument("--only", choices=("batched", "adjoint", "hires"),
+    ap.add_argument("--only", choices=("batched", "adjoint", "hires", "variants"),
                     help="run one of the extra workloads alone and print its JSON object "
                          "(for profiling)")
     ap.add_argument("--calibration-copy", action="store_true",
@@ -504,6 +565,8 @@ def main(argv=None) -> int:
             res = bench_batched(dev, args.substeps, float("nan"), float("nan"))
         elif args.only == "adjoint":
             res = bench_adjoint(dev)
+        elif args.only == "variants":
+            res = bench_variants(dev, args.substeps)
         else:
             res = bench_hires(dev)
         print(json.dumps({args.only: res}), flush=True)
@@ -625,6 +688,7 @@ def main(argv=None) -> int:
         line["batched"] = guarded(bench_batched, dev, nsub, evp_rate, value)
         line["adjoint"] = adjoint
         line["hires"] = guarded(bench_hires, dev)
+        line["variants"] = guarded(bench_variants, dev, nsub)
         b, a = line["batched"], line["adjoint"]
         if "error" not in b and "error" not in a:
             line["ratios"] = {
