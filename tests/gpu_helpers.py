@@ -50,3 +50,63 @@ def assert_close(name, got, want, rtol):
     assert np.isfinite(got).all(), f"{name}: non-finite values"
     rel, _ = report(name, got, want)
     assert rel <= rtol, f"{name}: relative error {rel:.3e} > {rtol:.1e}"
+
+
+def round_to_fp32(st):
+    """The state with every FP64 field rounded to the nearest FP32 value
+    (kept in FP64), in place."""
+    vs = st.variables
+    for k in list(vs.keys()):
+        a = vs[k]
+        if isinstance(a, np.ndarray) and a.dtype == np.float64:
+            vs[k] = a.astype(np.float32).astype(np.float64)
+    return st
+
+
+def _ensemble_member_oracle(args):
+    """Worker: pre-EVP fields and the EVP subcycle of one member of the
+    ragged ensemble through the oracle (run in a child process)."""
+    from veris_amd import synth
+
+    m, n, rounded = args
+    st = synth.config2_member(m)
+    if rounded:
+        round_to_fp32(st)
+    pre_evp(st)
+    return O.evp_solver(st, n_substeps=n)
+
+
+def ensemble_oracle(members, n, rounded=False, workers=12):
+    """Oracle EVP outputs of the ragged-ensemble members ``members`` after
+    ``n`` substeps from rest, computed in parallel CPU processes (spawned, so
+    no GPU state is inherited).  Returns one 5-tuple per member."""
+    import multiprocessing as mp
+    import os
+
+    from veris_amd import synth
+
+    sizes = synth.config2_sizes()
+    order = sorted(range(len(members)), key=lambda q: -sizes[members[q]][0] * sizes[members[q]][1])
+    workers = max(1, min(workers, len(members), len(os.sched_getaffinity(0))))
+    ctx = mp.get_context("spawn")
+    out = [None] * len(members)
+    with ctx.Pool(workers) as pool:  # largest members first
+        res = pool.map(_ensemble_member_oracle, [(members[q], n, rounded) for q in order],
+                       chunksize=1)
+    for q, r in zip(order, res):
+        out[q] = r
+    return out
+
+
+def evp_shapes_from_source():
+    """{tile index: (ri, rj, k)} of the EVP launch-shape list in
+    veris_amd/csrc/evp_kernel.h (the list the library's table and launch
+    switches are generated from), read at collection time without loading
+    the library."""
+    import re
+    from pathlib import Path
+
+    src = (Path(__file__).resolve().parent.parent / "veris_amd" / "csrc" / "evp_kernel.h").read_text()
+    body = re.search(r"#define VH_EVP_SHAPES\(X\)(.*?)\n\s*\n", src, re.S).group(1)
+    rows = re.findall(r"X\((\d+),\s*(\d+),\s*(\d+),\s*(\d+)\)", body)
+    return {int(a): (int(b), int(c), int(d)) for a, b, c, d in rows}

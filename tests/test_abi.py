@@ -71,6 +71,10 @@ def test_version_and_host_side_validation():
         N.workspace_bytes("vh_evp_workspace_bytes", bad)
     shapes = N.evp_tile_shapes()
     assert len(shapes) == lib.vh_evp_tile_count()
+    # the GPU tests parametrize over the shape list read from the kernel source
+    from gpu_helpers import evp_shapes_from_source
+
+    assert shapes == evp_shapes_from_source()
     for ri, rj, k in shapes.values():
         # rj a power of two (LDS row stride); ri free, so a workgroup may end in a
         # partial wave (13 x 32 = 6.5 waves)

@@ -207,3 +207,36 @@ def test_adjoint_rejects_unsupported(cuda):
     st.settings.basalDragK2 = 8.0
     with pytest.raises(VerisHipError, match="VH_EUNSUPPORTED"):
         A.evp_subcycle(st.to(cuda), n_substeps=3)
+
+
+def test_backward_after_the_state_is_reassigned(cuda):
+    """The forward saves every tensor whose device pointer went into the ABI
+    structs: reassigning the state's variables between forward and backward
+    (as a step does) and filling new allocations of the same size must not
+    change the gradients; a second backward without retain_graph raises
+    autograd's own error, and one with retain_graph gives the same result."""
+    st = _state("global")
+    n = 17
+    want, _, _ = _gpu_grads(st, n, ckpt=5)
+
+    gst = st.to(cuda)
+    p = _params(st, cuda)
+    outs = A.evp_subcycle(gst, n_substeps=n, ckpt_every=5, **p)
+    ws = [torch.tensor(w, device=cuda) for w in _weights(st, 3)]
+    J = _objective(outs, ws)
+    vs = gst.variables
+    for name in list(vs.keys()):  # the pattern of a step: every field replaced
+        vs[name] = torch.full_like(vs[name], 7.0)
+    junk = [torch.full((1 << 20,), 3.0, dtype=torch.float64, device=cuda) for _ in range(64)]
+    J.backward(retain_graph=True)
+    got = {k: g2n(v.grad) for k, v in p.items()}
+    for k in want:
+        assert np.array_equal(got[k], want[k]), k
+    for v in p.values():
+        v.grad = None
+    J.backward()
+    for k in want:
+        assert np.array_equal(g2n(p[k].grad), want[k]), k
+    with pytest.raises(RuntimeError, match="backward through the graph a second time"):
+        J.backward()
+    del junk

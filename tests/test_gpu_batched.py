@@ -13,7 +13,7 @@ import numpy as np
 import pytest
 import torch
 
-from gpu_helpers import assert_exact, g2n, pre_evp, report
+from gpu_helpers import assert_exact, evp_shapes_from_source, g2n, pre_evp, report
 from oracle import veris_np as O
 from veris_amd import synth
 from veris_amd.batched import Ensemble
@@ -34,7 +34,7 @@ def _members():
     return states
 
 
-@pytest.mark.parametrize("tile", [3, 5, 1, 8, 13, 15, 16])
+@pytest.mark.parametrize("tile", sorted(evp_shapes_from_source()))
 def test_batched_fp64_matches_oracle(cuda, tile):
     from veris_amd._native import evp_tile_shapes
 

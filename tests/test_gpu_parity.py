@@ -14,7 +14,7 @@ import numpy as np
 import pytest
 import torch
 
-from gpu_helpers import assert_close, assert_exact, g2n, pre_evp
+from gpu_helpers import assert_close, assert_exact, evp_shapes_from_source, g2n, pre_evp
 from oracle import veris_np as O
 from veris_amd import kernels as K
 from veris_amd import synth
@@ -541,7 +541,7 @@ def test_errors_are_reported(cuda):
         K.evp_solver(st, 3)
 
 
-ALL_TILES = list(range(1, 16))
+ALL_TILES = sorted(evp_shapes_from_source())  # every launch shape of the library
 
 
 def _periodic_artefact_cols(st):

@@ -1,0 +1,313 @@
+"""GPU parity at the benchmark's own workloads and launch shapes.
+
+* 1 degree (360 x 160, FP64): the 400-substep subcycle with every workgroup
+  shape against the oracle, bit for bit (the timed drop-in runs one of the
+  single-substep shapes, whichever the autotuner picks), and one whole step
+  within 1e-9 of the oracle.
+* The ragged ensemble (48 members of 64..512 x 64..384 cells): FP64 batched
+  subcycles of every shape bit for bit against the oracle for every member;
+  the FP32 ensemble after 400 substeps from rest against the FP64 oracle at
+  the bounds stated per precision mode.
+* The adjoint workload (1 degree, 120 substeps, checkpoint every 10):
+  gradients against torch.autograd of the independent torch restatement and
+  against central differences.
+* 0.25 degree (1440 x 720): each stage of the step in FP32 against the same
+  stage in FP64 on identical (FP32-representable) inputs, at stated bounds.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+
+import torch_ref as TR
+from gpu_helpers import (assert_close, assert_exact, ensemble_oracle, g2n, pre_evp,
+                         round_to_fp32)
+from oracle import veris_np as O
+from veris_amd import kernels as K
+from veris_amd import synth
+from veris_amd.batched import Ensemble
+from veris_amd.step import seaice_step
+
+pytestmark = pytest.mark.gpu
+
+EVP_NAMES = ("uIce", "vIce", "sigma1", "sigma2", "sigma12")
+
+
+def _tiles():
+    from veris_amd._native import evp_tile_shapes
+
+    return evp_tile_shapes()
+
+
+def _artefact_cols(st):
+    """x-overlap columns of an x-periodic grid whose reference stresses are
+    numpy.roll wrap-around artefacts (no other value depends on them)."""
+    if not (st.settings.enable_cyclic_x or not st.settings.veros_fill):
+        return []
+    NX = st.shape[0]
+    return [0, 1, NX - 2, NX - 1]
+
+
+# -- 1 degree ------------------------------------------------------------------
+
+
+@pytest.fixture(scope="module")
+def one_degree():
+    st = pre_evp(synth.config1())
+    print("oracle: 1-degree subcycle, 400 substeps", flush=True)
+    return st, O.evp_solver(st, n_substeps=400)
+
+
+def test_evp_1deg_400_substeps_every_shape(cuda, one_degree):
+    """The drop-in's workload: 400 substeps from rest at 1 degree.  Every
+    single-substep shape (the autotuner's candidates, 13 x 32 and 14 x 32
+    among them) equals the oracle in every bit of every padded cell;
+    multi-substep shapes equal it on every velocity and on every stress
+    outside the periodic artefact columns."""
+    st, want = one_degree
+    g = st.to(cuda)
+    for tile, (ri, rj, k) in sorted(_tiles().items()):
+        print(f"1 degree, tile {tile} ({ri}x{rj}, k={k})", flush=True)
+        got = K.evp_solver(g, n_substeps=400, tile=tile)
+        keep = np.setdiff1d(np.arange(st.shape[0]), _artefact_cols(st) if k > 1 else [])
+        for q, (name, a, b) in enumerate(zip(EVP_NAMES, got, want)):
+            a = g2n(a)
+            if q >= 2:
+                a, b = a[keep], b[keep]
+            assert_exact(f"1deg {name}[tile={tile}]", a, b)
+
+
+def test_full_step_1deg(cuda):
+    """One whole 1-degree step (pre-EVP, 400 EVP substeps, ocean stress,
+    advection, clean-up, ridging, growth, fills) with the default and with
+    the autotuner's 13 x 32 and 14 x 32 shapes: within 1e-9 of each field's
+    maximum (device exp/pow in strength and growth)."""
+    st = synth.config1()
+    q0 = st.variables.Qnet.copy()
+    sw = st.variables.SWdown.copy()
+    start = st.to(cuda)
+    print("oracle: one 1-degree step", flush=True)
+    O.seaice_step(st, q0, sw)
+    names = ("uIce", "vIce", "sigma1", "sigma2", "sigma12", "OceanStressU", "OceanStressV",
+             "hIceMean", "hSnowMean", "Area", "TSurf", "Qnet", "Qsw", "EmPmR",
+             "forc_salt_surface_ice", "SeaIceLoad", "IcePenetSW", "recip_hIceMean")
+    key = (tuple(start.variables.iceMask.shape), start.variables.iceMask.dtype)
+    try:
+        for tile in (0, 16, 17):
+            K.EVP_TILE[key] = tile
+            g = start.copy(lambda a: a.clone())
+            seaice_step(g, torch.from_numpy(q0).to(cuda), torch.from_numpy(sw).to(cuda))
+            for name in names:
+                assert_close(f"1deg step {name}[tile={tile}]", g2n(g.variables[name]),
+                             st.variables[name], 1e-9)
+    finally:
+        K.EVP_TILE.pop(key, None)
+
+
+# -- the ragged ensemble ------------------------------------------------------------
+
+
+N_MEMBERS = len(synth.config2_sizes())
+
+
+@pytest.fixture(scope="module")
+def ensemble_states():
+    return [pre_evp(synth.config2_member(m)) for m in range(N_MEMBERS)]
+
+
+def test_ensemble_fp64_every_shape(cuda, ensemble_states):
+    """All 48 members in one batched launch sequence, FP64: 13 substeps
+    (13 = 6 x 2 + 1 = 4 x 3 + 1 = 3 x 4 + 1, so every multi-substep shape also
+    runs its single-substep remainder) equal the oracle in every bit of every
+    member for every shape (the members are closed domains, so multi-substep
+    shapes are exact too)."""
+    states = ensemble_states
+    print("oracle: 48 members, 13 substeps", flush=True)
+    want = ensemble_oracle(list(range(N_MEMBERS)), 13)
+    ens = Ensemble(states, device=cuda, dtype=torch.float64)
+    out = None
+    for tile, (ri, rj, k) in sorted(_tiles().items()):
+        print(f"ensemble FP64, tile {tile} ({ri}x{rj}, k={k})", flush=True)
+        out = ens.evp(13, tile=tile, out=out)
+        for m in range(N_MEMBERS):
+            for q, name in enumerate(EVP_NAMES):
+                got = g2n(ens.view(out[q], m))
+                if not np.array_equal(got, want[m][q]):
+                    assert_exact(f"member {m} {name}[tile={tile}]", got, want[m][q])
+
+
+@pytest.fixture(scope="module")
+def ensemble_400():
+    print("oracle: 48 members, 400 substeps (parallel CPU processes)", flush=True)
+    return ensemble_oracle(list(range(N_MEMBERS)), 400)
+
+
+def _ensemble_errors(ens, out, want, states):
+    """Relative L2 error of every EVP output over the interior ocean cells of
+    all members together, and the worst member's velocity error."""
+    num = dict.fromkeys(EVP_NAMES, 0.0)
+    den = dict.fromkeys(EVP_NAMES, 0.0)
+    worst_member = 0.0
+    for m, st in enumerate(states):
+        sel = (np.asarray(st.variables.maskInC) > 0)[2:-2, 2:-2]
+        nu = de = 0.0
+        for q, name in enumerate(EVP_NAMES):
+            got = g2n(ens.view(out[q], m))[2:-2, 2:-2][sel]
+            ref = want[m][q][2:-2, 2:-2][sel]
+            d2, r2 = float(np.sum((got - ref) ** 2)), float(np.sum(ref**2))
+            num[name] += d2
+            den[name] += r2
+            if q < 2:
+                nu, de = nu + d2, de + r2
+        worst_member = max(worst_member, (nu / de) ** 0.5 if de > 0 else 0.0)
+    err = {k: (num[k] / den[k]) ** 0.5 for k in EVP_NAMES}
+    return err, worst_member
+
+
+# relative L2 bounds over the ensemble (velocity, stresses, worst member's
+# velocity) after 400 substeps from rest against the FP64 oracle, per mode
+# (measured: native 1.37e-4, 3.1e-3, 3.4e-4; fp64 1.17e-4, 2.9e-3, 3.2e-4)
+FP32_BOUNDS = {
+    "native": (1.5e-4, 4e-3, 4e-4),
+    "fp64": (1.3e-4, 4e-3, 4e-4),
+}
+
+
+@pytest.mark.parametrize("mode", sorted(FP32_BOUNDS))
+def test_ensemble_fp32_against_fp64_oracle(cuda, ensemble_states, ensemble_400, mode):
+    """The FP32 ensemble (48 members, 400 substeps from rest, the batched
+    benchmark's autotuned shape 32 x 32 x 2 and the single-substep default)
+    against the FP64 oracle on the exact inputs.  "native": FP32 storage and
+    arithmetic with Delta, zeta and press in FP64; "fp64": FP64 arithmetic on
+    FP32-stored fields.  The subcycle amplifies any perturbation of its
+    inputs: rounding a single input field to FP32 moves the FP64 velocities
+    by 6-8e-5 on this workload, so the velocity error of every mode that reads
+    FP32 inputs sits near 1e-4.  Bounds per mode in FP32_BOUNDS."""
+    states = ensemble_states
+    want = ensemble_400
+    ens = Ensemble(states, device=cuda, dtype=torch.float32, arith=mode)
+    ens.pre_evp()
+    bu, bs, bm = FP32_BOUNDS[mode]
+    for tile in (9, 5):
+        print(f"ensemble FP32 {mode}, tile {tile}", flush=True)
+        out = ens.evp(400, tile=tile)
+        err, worst = _ensemble_errors(ens, out, want, states)
+        print(f"{mode} tile {tile}: " + ", ".join(f"{k} {v:.3e}" for k, v in err.items())
+              + f", worst member velocity {worst:.3e}", flush=True)
+        assert err["uIce"] <= bu and err["vIce"] <= bu, err
+        assert max(err["sigma1"], err["sigma2"], err["sigma12"]) <= bs, err
+        assert worst <= bm, worst
+
+
+# -- the adjoint workload -----------------------------------------------------------
+
+
+def test_adjoint_config3_against_torch_autograd_and_fd(cuda):
+    """1 degree, 120 substeps, checkpoint every 10, loss = L2 misfit of the
+    final velocities to the seeded target: gradients w.r.t. the P* field,
+    hIceMean and the (tied) water and air drag coefficients against
+    torch.autograd of the torch restatement (FP64 both, 1e-8 relative L2 per
+    gradient), and the directional derivative along 5 random directions
+    against central differences of the GPU forward (1e-5)."""
+    from veris_amd import adjoint as A
+
+    st = synth.config3()
+    n = int(st.settings.nEVPsteps)
+    pstar, ut, vt = synth.config3_adjoint_inputs(st)
+    s = st.settings
+    g = st.to(cuda)
+    tgt = (torch.from_numpy(ut).to(cuda), torch.from_numpy(vt).to(cuda))
+
+    def leaves(dev):
+        mk = lambda a: torch.tensor(a, dtype=torch.float64, device=dev).requires_grad_(True)  # noqa: E731
+        return {"pStar": mk(pstar), "hIceMean": mk(np.asarray(st.variables.hIceMean)),
+                "cw": mk(s.waterIceDrag), "ca": mk(s.airIceDrag)}
+
+    def gpu_loss(p):
+        outs = A.evp_subcycle(g, pStar=p["pStar"], hIceMean=p["hIceMean"],
+                              waterIceDrag=p["cw"], waterIceDrag_south=p["cw"],
+                              airIceDrag=p["ca"], airIceDrag_south=p["ca"], n_substeps=n,
+                              ckpt_every=10)
+        return (((outs[0] - tgt[0])[2:-2, 2:-2] ** 2).sum()
+                + ((outs[1] - tgt[1])[2:-2, 2:-2] ** 2).sum())
+
+    print("GPU adjoint, 1 degree, 120 substeps", flush=True)
+    pg = leaves(cuda)
+    gpu_loss(pg).backward()
+    got = {k: g2n(v.grad) for k, v in pg.items()}
+
+    print("torch restatement, autograd on the CPU", flush=True)
+    vs = {k: torch.from_numpy(np.asarray(v, dtype=np.float64)) for k, v in st.variables.items()}
+    pc = leaves("cpu")
+    J = TR.misfit(vs, s, pc["pStar"], pc["hIceMean"], pc["cw"], pc["ca"], n,
+                  torch.from_numpy(ut), torch.from_numpy(vt))
+    J.backward()
+    for k, v in pc.items():
+        w = v.grad.numpy()
+        err = float(np.linalg.norm(got[k] - w) / max(np.linalg.norm(w), 1e-300))
+        print(f"config 3 d/d{k}: rel L2 err {err:.3e} |grad| {np.linalg.norm(w):.3e}", flush=True)
+        assert np.linalg.norm(w) > 0, k
+        assert err <= 1e-8, (k, err)
+
+    base = {k: v.detach() for k, v in pg.items()}
+    rng = np.random.default_rng(21)
+    eps, worst = 1e-6, 0.0
+    for d in range(5):
+        dirs = {k: torch.tensor(rng.standard_normal(tuple(v.shape)), device=cuda) * v
+                for k, v in base.items()}
+        with torch.no_grad():
+            jp = float(gpu_loss({k: base[k] + eps * dirs[k] for k in base}))
+            jm = float(gpu_loss({k: base[k] - eps * dirs[k] for k in base}))
+        fd = (jp - jm) / (2 * eps)
+        ad = sum(float(np.sum(got[k] * g2n(dirs[k]))) for k in base)
+        rel = abs(fd - ad) / max(abs(ad), 1e-300)
+        worst = max(worst, rel)
+        print(f"direction {d}: fd={fd:.10e} adjoint={ad:.10e} rel={rel:.2e}", flush=True)
+    assert worst <= 1e-5
+
+
+# -- 0.25 degree, FP32 per stage ---------------------------------------------------------
+
+
+# bound on max|FP32 - FP64| / max|FP64| per stage output on identical inputs
+# (measured: 1.3e-7, 1.8e-6, 1.0e-7, 1.1e-7, 2.5e-13, 5.6e-8)
+HIRES_STAGE_BOUNDS = {"pre_evp": 5e-7, "evp20": 1e-5, "ocean_stress": 5e-7, "advection": 5e-7,
+                      "clean_up": 1e-7, "growth": 5e-7}
+
+
+def test_fp32_stages_quarter_degree(cuda):
+    """Each stage of the 0.25-degree step in FP32 (FP64 Delta/zeta/press,
+    FP64 growth arithmetic) against the same stage in FP64, both on the same
+    FP32-representable inputs taken from an FP64 state after one step of 20
+    substeps: the error each FP32 kernel adds by itself, as max|diff| /
+    max|FP64| per output (bounds in HIRES_STAGE_BOUNDS; the EVP bound covers
+    20 substeps of the chaotic subcycle)."""
+    st = synth.config4()
+    g = st.to(cuda)
+    seaice_step(g, torch.from_numpy(st.variables.Qnet.copy()).to(cuda),
+                torch.from_numpy(st.variables.SWdown.copy()).to(cuda), n_substeps=20)
+    base = round_to_fp32(g.numpy())
+    s64, s32 = base.to(cuda, torch.float64), base.to(cuda, torch.float32)
+
+    def cmp(stage, names, a64, a32):
+        for name, x, y in zip(names, a64, a32):
+            assert y.dtype == torch.float32
+            assert_close(f"0.25deg {stage} {name}", g2n(y), g2n(x), HIRES_STAGE_BOUNDS[stage])
+
+    pre_names = ("SeaIceMassC", "SeaIceMassU", "SeaIceMassV", "AreaW", "AreaS", "WindForcingX",
+                 "WindForcingY", "SeaIceStrength")
+    print("0.25 degree stages: pre-EVP", flush=True)
+    cmp("pre_evp", pre_names, K.pre_evp(s64), K.pre_evp(s32))
+    print("0.25 degree stages: EVP, 20 substeps", flush=True)
+    cmp("evp20", EVP_NAMES, K.evp_solver(s64, 20), K.evp_solver(s32, 20))
+    print("0.25 degree stages: ocean stress, advection, clean-up, growth", flush=True)
+    cmp("ocean_stress", ("OceanStressU", "OceanStressV"), K.OceanStressUV(s64),
+        K.OceanStressUV(s32))
+    cmp("advection", ("hIceMean", "hSnowMean", "Area"), K.Advection(s64), K.Advection(s32))
+    cmp("clean_up", ("hIceMean", "hSnowMean", "Area", "TSurf"), K.clean_up_advection(s64)[:4],
+        K.clean_up_advection(s32)[:4])
+    from veris_amd._native import GROWTH_OUT_NAMES
+
+    cmp("growth", GROWTH_OUT_NAMES, K.Growth(s64), K.Growth(s32))

@@ -32,3 +32,19 @@ def test_ensemble_and_adjoint_member_sizes():
     m = synth.config4_adjoint_member(3)
     assert m.shape == (244, 244) and not m.settings.enable_cyclic_x
     assert int(m.settings.nEVPsteps) == 120
+
+
+def test_quarter_degree_adjoint_members_carry_smooth_thickness_noise():
+    """The 240 x 240 members draw thickness from noise with a 10-cell
+    correlation length: neighbouring ice cells are strongly correlated, where
+    the uniform 0-3 m draw of the other workloads has no correlation."""
+    def neighbour_corr(st):
+        h = np.asarray(st.variables.hIceMean)[2:-2, 2:-2]
+        ice = (h[1:] > 0) & (h[:-1] > 0)
+        return float(np.corrcoef(h[1:][ice], h[:-1][ice])[0, 1])
+
+    m = synth.config4_adjoint_member(0)
+    h = np.asarray(m.variables.hIceMean)[2:-2, 2:-2]
+    assert 0.0 <= h.min() and h.max() <= 3.0
+    assert neighbour_corr(m) > 0.9
+    assert abs(neighbour_corr(synth.regional(4, 100, 240, 240))) < 0.1

@@ -73,15 +73,31 @@ class _EvpSubcycle(torch.autograd.Function):
         out = torch.empty((5,) + shape, dtype=dt, device=c.device).unbind(0)
         N.call("vh_evp_forward_checkpointed", *c.args(), n, ckpt_every,
                *(t.data_ptr() for t in out), ws.data_ptr(), ws.numel(), c.stream)
-        # the ABI structs point into these tensors: keep them alive for backward
-        ctx.c, ctx.keep = c, (state, h, pre, pstar_field)
-        ctx.n, ctx.ckpt, ctx.ws = n, ckpt_every, ws
+        # The ABI structs hold raw pointers into the state's tensors, the
+        # pre-EVP fields and the workspace (its checkpoints).  Saving every one
+        # of them for backward keeps their memory alive however the caller
+        # reassigns the state's variables in between, makes autograd refuse a
+        # backward after an in-place change of any of them, and frees them
+        # after backward unless the graph is retained.
+        ptrs = {t.data_ptr() for t in (ws, h, pre, pstar_field)}
+        keep = [ws, h, pre, pstar_field]
+        for name in c.field_names:
+            t = state.variables.get(name)
+            if t.data_ptr() not in ptrs:
+                ptrs.add(t.data_ptr())
+                keep.append(t)
+        ctx.save_for_backward(*keep)
+        ctx.c = c
+        ctx.n, ctx.ckpt = n, ckpt_every
         ctx.meta = [(x.dtype, x.device, x.shape) for x in (pstar, cwN, cwS, caN, caS)]
         ctx.h_meta = (h.dtype, h.device)
         return tuple(out)
 
     @staticmethod
     def backward(ctx, gu, gv, gs1, gs2, gs12):
+        # raises autograd's own error when the saved tensors were freed by an
+        # earlier backward (retain_graph=False) or changed in place since forward
+        ws, _h, _pre, pstar_field = ctx.saved_tensors[:4]
         c = ctx.c
         shape, dt = c.shape, c.dtype
         grads = [None if g is None else g.to(dt).contiguous() for g in (gu, gv, gs1, gs2, gs12)]
@@ -90,11 +106,10 @@ class _EvpSubcycle(torch.autograd.Function):
         eo = N.EvpAdjointOut(**{"adj_" + k: v.data_ptr() for k, v in evp.items()})
         N.call("vh_evp_adjoint", *c.args(), ctx.n, ctx.ckpt,
                *(None if g is None else g.data_ptr() for g in grads), ctypes.byref(eo),
-               ctx.ws.data_ptr(), ctx.ws.numel(), c.stream)
+               ws.data_ptr(), ws.numel(), c.stream)
         pbuf = torch.empty((len(N.PRE_EVP_ADJOINT_NAMES),) + shape, dtype=dt, device=c.device)
         pre = dict(zip(N.PRE_EVP_ADJOINT_NAMES, pbuf.unbind(0)))
         po = N.PreEvpAdjointOut(**{"adj_" + k: v.data_ptr() for k, v in pre.items()})
-        pstar_field = ctx.keep[3]
         N.call("vh_pre_evp_adjoint", *c.args(), pstar_field.data_ptr(), ctypes.byref(eo),
                ctypes.byref(po), c.stream)
         need = ctx.needs_input_grad  # (state, n, ckpt, tile, pstar, h, cwN, cwS, caN, caS)
@@ -110,7 +125,6 @@ class _EvpSubcycle(torch.autograd.Function):
             else:
                 res.append(sums[q].reshape(x_shape).to(device=x_dev, dtype=x_dt))
         g_h = pre["hIceMean"].to(device=ctx.h_meta[1], dtype=ctx.h_meta[0]) if need[5] else None
-        ctx.ws = None
         return (None, None, None, None, res[0], g_h, res[1], res[2], res[3], res[4])
 
 

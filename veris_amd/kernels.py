@@ -61,12 +61,14 @@ class Ctx:
         self.dom = N.domain_struct(sett, _dtype_code(ref))
         self.sett = N.settings_struct(sett)
         self.fields = N.Fields()
+        self.field_names = []  # variables whose device pointers are in self.fields
         for name in VARIABLE_NAMES:
             t = vs.get(name)
             if t is None:
                 continue
             self._check(name, t)
             setattr(self.fields, name, t.data_ptr())
+            self.field_names.append(name)
         if all(vs.get(k) is None or _all_zero(vs.get(k)) for k in _KTERMS):
             for k in _KTERMS:
                 setattr(self.fields, k, None)

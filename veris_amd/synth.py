@@ -194,8 +194,12 @@ def global_latlon(config: int, nx: int, ny: int, lat_south: float, lat_north: fl
 
 
 def regional(config: int, member: int, nx: int, ny: int, dx: float = 25e3,
-             lat_south: float = 60.0, lat_north: float = 85.0, **setting_overrides) -> VerisState:
-    """Closed regional domain on a uniform Cartesian grid over a polar band."""
+             lat_south: float = 60.0, lat_north: float = 85.0, h_sigma: float | None = None,
+             **setting_overrides) -> VerisState:
+    """Closed regional domain on a uniform Cartesian grid over a polar band.
+
+    ``h_sigma`` (cells) replaces the uniform 0-3 m thickness by smooth noise
+    of that correlation length (mean 1.5 m, clipped to 0-3 m)."""
     rng = rng_for(config, member)
     sett = make_state_settings(nx, ny, enable_cyclic_x=False, **setting_overrides)
     NX, NY = nx + 2 * OL, ny + 2 * OL
@@ -206,7 +210,12 @@ def regional(config: int, member: int, nx: int, ny: int, dx: float = 25e3,
                             cyclic=False, cos_lat=False)
     vs = VariablesNamespace()
     derive_grid(vs, grid)
-    _forcing(rng, vs, lat, sett, ice_lat=0.0)
+    h_noise = None
+    if h_sigma is not None:
+        z = _gauss_smooth(rng.standard_normal((nx, ny)), h_sigma, periodic_x=False)
+        z = (z - z.mean()) / z.std()
+        h_noise = _pad_interior(np.clip(1.5 + 0.75 * z, 0.0, 3.0), cyclic=False)
+    _forcing(rng, vs, lat, sett, ice_lat=0.0, h_noise=h_noise)
     _zero_rest(vs, (NX, NY))
     return VerisState(sett, vs)
 
@@ -263,6 +272,7 @@ def config4_adjoint_member(member: int, **kw) -> VerisState:
     """Member of the batched adjoint at 0.25 degree: 240 x 240 closed regional
     domain with 10-cell thickness noise."""
     kw.setdefault("nEVPsteps", 120)
+    kw.setdefault("h_sigma", 10.0)
     return regional(4, 100 + member, 240, 240, **kw)
 
 
