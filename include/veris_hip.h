@@ -352,6 +352,27 @@ int vh_evp_solver(const vh_domain* d, const vh_settings* s, const vh_fields* f,
                   int32_t n_substeps, void* uIce, void* vIce, void* sigma1, void* sigma2,
                   void* sigma12, void* workspace, size_t workspace_bytes, void* stream);
 
+/* Largest scratch vh_evp_solver_noalloc holds per GPU: 48 MiB, enough for the
+ * 0.25-degree grid (1440 x 720) in FP64 (about 43 MB). */
+#define VH_EVP_NOALLOC_MAX_BYTES (48u * 1024u * 1024u)
+
+/* vh_evp_solver without a workspace argument: the same subcycle (the same
+ * kernels and launch shapes, the same outputs bit for bit), with its scratch
+ * in a device array of the library (one per GPU, VH_EVP_NOALLOC_MAX_BYTES,
+ * held until the process ends) and every substep launched directly on
+ * `stream` (no graph is captured or instantiated).  It allocates no device
+ * memory, so callers that cannot (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc) can still run the subcycle; the
+ * first call on a device looks up that device's array once.  Domains whose
+ * vh_evp_workspace_bytes exceeds the maximum return VH_EWORKSPACE.  All calls
+ * on one GPU share its array: calls that overlap on one GPU (other streams,
+ * other threads) race and must be serialised by the caller; nothing in the
+ * array needs to start at zero.  No counterpart in the reference: it exists
+ * for the callers named above, which cannot allocate. */
+int vh_evp_solver_noalloc(const vh_domain* d, const vh_settings* s, const vh_fields* f,
+                          int32_t n_substeps, void* uIce, void* vIce, void* sigma1, void* sigma2,
+                          void* sigma12, void* stream);
+
 /* Workspace bytes vh_evp_solver_residuals needs for this domain and count. */
 int vh_evp_residuals_workspace_bytes(const vh_domain* d, int32_t n_substeps, size_t* bytes);
 

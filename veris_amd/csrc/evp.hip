@@ -11,6 +11,7 @@
 //      workspace address), so moving inputs and outputs never forces a
 //      re-capture and a 400-substep subcycle costs one graph launch.
 #include <cstdio>
+#include <mutex>
 
 #include "evp_kernel.h"
 
@@ -94,10 +95,12 @@ int evp_residual_tiles(const Geom& g) {
 // sigma12) to `out`; fl->uIce/vIce are the previous-step velocities of the
 // momentum equation.  Shared by vh_evp_solver and the checkpointed forward
 // and segment recompute of the adjoint.
+// `direct`: launch the substeps on `st` one by one, never through a graph
+// (no capture, no instantiation: nothing that allocates).
 template <typename T>
 int evp_run_core(const Geom& g, const vh_settings* s, const vh_fields* fl, const T* const in[5],
                  T* const out[5], int nsub, void* ws, size_t ws_bytes, int tile_req, int dtype,
-                 hipStream_t st, double* res = nullptr) {
+                 hipStream_t st, double* res = nullptr, bool direct = false) {
   const size_t fb = align256(field_elems(g) * sizeof(T));
   const size_t need = 5 * fb + align256(sizeof(EvpArgBlock<T>));
   VH_REQUIRE(ws != nullptr && ws_bytes >= need, VH_EWORKSPACE,
@@ -164,7 +167,7 @@ int evp_run_core(const Geom& g, const vh_settings* s, const vh_fields* fl, const
   hipLaunchKernelGGL(evp_write_args_kernel<T>, dim3(1), dim3(64), 0, st, ab_dev, ab);
   if (pack) evp_pack_launch(ab_dev, (long long)field_elems(g), 1, st);
   VH_TRY_HIP(hipGetLastError());
-  if (nsub >= 8 && graphs_enabled()) {
+  if (nsub >= 8 && graphs_enabled() && !direct) {
     char key[160];
     snprintf(key, sizeof(key), "evp1/%d/%d/%d/%d/%d/%d/%d/%d/%d/%d/%p", dtype, cfg, (int)so,
              res ? 1 : 0, nsub, g.NX, g.NY, g.nx, g.fill_x, g.fill_y, (const void*)ab_dev);
@@ -181,10 +184,41 @@ int evp_run_core(const Geom& g, const vh_settings* s, const vh_fields* fl, const
 
 template int evp_run_core<double>(const Geom&, const vh_settings*, const vh_fields*,
                                   const double* const[5], double* const[5], int, void*, size_t,
-                                  int, int, hipStream_t, double*);
+                                  int, int, hipStream_t, double*, bool);
 template int evp_run_core<float>(const Geom&, const vh_settings*, const vh_fields*,
                                  const float* const[5], float* const[5], int, void*, size_t, int,
-                                 int, hipStream_t, double*);
+                                 int, hipStream_t, double*, bool);
+
+// Scratch of vh_evp_solver_noalloc: the workspace of one subcycle (ping-pong
+// state, argument block, packed step constants) in a device array of the
+// library, one copy per GPU, held until the process ends.  Host code never
+// passes `scratch` itself (on the host it only names the symbol): it uses the
+// device address hipGetSymbolAddress returns for the current device.
+constexpr size_t kNoAllocScratchBytes = (size_t)VH_EVP_NOALLOC_MAX_BYTES;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of the current device's scratch array, looked up once per
+// device (with that device current) and kept.
+int scratch_for_current_device(void** out) {
+  int dev = -1;
+  VH_TRY_HIP(hipGetDevice(&dev));
+  VH_REQUIRE(dev >= 0 && dev < kMaxDevices, VH_EINVAL, "device %d beyond the library's %d", dev,
+             kMaxDevices);
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    VH_TRY_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    VH_REQUIRE(p != nullptr, VH_EHIP, "hipGetSymbolAddress gave no address for the scratch array");
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return VH_OK;
+}
+}  // namespace
 
 namespace {
 
@@ -259,6 +293,46 @@ extern "C" int vh_evp_solver(const vh_domain* d, const vh_settings* s, const vh_
                                workspace_bytes, d->tile, d->dtype, st);
   return vh::run_evp<float>(g, s, f, n_substeps, uIce, vIce, sigma1, sigma2, sigma12, workspace,
                             workspace_bytes, d->tile, d->dtype, st);
+}
+
+extern "C" int vh_evp_solver_noalloc(const vh_domain* d, const vh_settings* s,
+                                     const vh_fields* f, int32_t n_substeps, void* uIce,
+                                     void* vIce, void* sigma1, void* sigma2, void* sigma12,
+                                     void* stream) {
+  vh::Geom g;
+  int rc = vh::make_geom(d, &g);
+  if (rc) return rc;
+  VH_REQUIRE(s != nullptr && f != nullptr, VH_EINVAL, "null settings or fields");
+  VH_REQUIRE(n_substeps >= 0, VH_EINVAL, "n_substeps=%d", n_substeps);
+  VH_REQUIRE(d->tile >= 0 && d->tile < vh::kEvpNumTiles, VH_EINVAL, "tile %d", d->tile);
+  // the full workspace of vh_evp_solver (packed constants included), so both
+  // entry points run the same kernels on the same launch plan
+  const size_t need = d->dtype == VH_F64 ? vh::evp_core_ws_bytes<double>(g)
+                                         : vh::evp_core_ws_bytes<float>(g);
+  VH_REQUIRE(need <= vh::kNoAllocScratchBytes, VH_EWORKSPACE,
+             "vh_evp_solver_noalloc: a %d x %d %s domain needs %zu bytes of scratch, beyond its "
+             "maximum of %zu (VH_EVP_NOALLOC_MAX_BYTES); use vh_evp_solver with a workspace",
+             d->nx, d->ny, d->dtype == VH_F64 ? "FP64" : "FP32", need,
+             vh::kNoAllocScratchBytes);
+  void* ws = nullptr;
+  rc = vh::scratch_for_current_device(&ws);
+  if (rc) return rc;
+  hipStream_t st = (hipStream_t)stream;
+  if (d->dtype == VH_F64) {
+    const double* in[5] = {(const double*)f->uIce, (const double*)f->vIce,
+                           (const double*)f->sigma1, (const double*)f->sigma2,
+                           (const double*)f->sigma12};
+    double* const out[5] = {(double*)uIce, (double*)vIce, (double*)sigma1, (double*)sigma2,
+                            (double*)sigma12};
+    return vh::evp_run_core<double>(g, s, f, in, out, n_substeps, ws, need, d->tile, d->dtype, st,
+                                    nullptr, true);
+  }
+  const float* in[5] = {(const float*)f->uIce, (const float*)f->vIce, (const float*)f->sigma1,
+                        (const float*)f->sigma2, (const float*)f->sigma12};
+  float* const out[5] = {(float*)uIce, (float*)vIce, (float*)sigma1, (float*)sigma2,
+                         (float*)sigma12};
+  return vh::evp_run_core<float>(g, s, f, in, out, n_substeps, ws, need, d->tile, d->dtype, st,
+                                 nullptr, true);
 }
 
 extern "C" int vh_evp_residuals_workspace_bytes(const vh_domain* d, int32_t n_substeps,
