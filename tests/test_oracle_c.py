@@ -50,6 +50,13 @@ from veris_amd import synth
 N_SUB = 12
 
 
+def _shallow(st):
+    """Water 8 m deep, given as a negative depth (the drag takes |R_low|):
+    keels of the 0-3 m ice ground (hCrit = 8 A / K1 < 1 m)."""
+    st.variables.R_low = -st.variables.R_low * 0.002
+    return st
+
+
 def _kterms(st, seed):
     r = np.random.default_rng(seed)
     for k in ("k1AtC", "k2AtC", "k1AtZ", "k2AtZ"):
@@ -94,7 +101,7 @@ VARIANTS = {
     "second_order_regional": lambda: synth.regional(7, 2, 37, 29, secondOrderBC=True,
                                                     useAdaptiveEVP=True),
     "second_order_kterms": lambda: _kterms(synth.tiny(20, 14, secondOrderBC=True, **TURN), 4),
-    "basal": lambda: synth.tiny(20, 14, basalDragK2=15.0),
+    "basal": lambda: _shallow(synth.tiny(20, 14, basalDragK2=15.0)),
 }
 INEXACT = {"basal"}
 _STATES = {}
@@ -167,7 +174,12 @@ _EVP_FUNCS = (O.strainrates, O.side_drag, O.viscosities, O.c_point_to_z_point, O
               O.fill_overlap)
 _EVP_SRC = [inspect.getsource(f) for f in _EVP_FUNCS]
 _FD_SRC = inspect.getsource(O.freedrift_solver)
-EVP_MUTATIONS = sorted(n for n, (a, _) in MUTATIONS.items() if any(a in s for s in _EVP_SRC))
+# planted mistakes that the closed-form pins catch but that no C-comparison
+# variant can show: no variant cell sits exactly at Area = 0.01 or has zero
+# ice mass at an ice-covered u-point
+PIN_ONLY = {"basal area gate", "turning mass gate"}
+EVP_MUTATIONS = sorted(n for n, (a, _) in MUTATIONS.items()
+                       if any(a in s for s in _EVP_SRC) and n not in PIN_ONLY)
 FD_MUTATIONS = sorted(n for n, (a, _) in MUTATIONS.items() if a in _FD_SRC)
 
 
@@ -232,6 +244,13 @@ def _melting(st):
     return st
 
 
+def _at_melting_point(st):
+    """Every surface starts the step at the melting point: wet albedos,
+    with southern values that differ from the northern ones."""
+    st.variables.TSurf = np.full(st.shape, st.settings.celsius2K)
+    return st
+
+
 def _thin_ice(st):
     """Ice exactly at, just below and just above hIce_min; with no motion
     (no substeps from rest) advection leaves these thicknesses unchanged."""
@@ -256,6 +275,13 @@ STEP_VARIANTS = {
     "sea_ice_load": lambda: _load_state(synth.tiny(20, 14, useRealFreshWaterFlux=True)),
     "adaptive_side_drag": lambda: synth.tiny(20, 14, useAdaptiveEVP=True, noSlip=False),
     "ice_edge": lambda: _ice_edge(synth.tiny(20, 14)),
+    # both hemispheres, turning angles, and southern dry albedos of their own
+    "turning_angles": lambda: _moving_ice(synth.global_latlon(
+        98, 20, 14, -70.0, 70.0, airTurnAngle=25.0, waterTurnAngle=20.0,
+        dryIceAlb_south=0.72, drySnowAlb_south=0.8)),
+    "wet_albedo": lambda: _at_melting_point(synth.global_latlon(
+        97, 20, 14, -70.0, 70.0, wetIceAlb_south=0.6, wetSnowAlb_south=0.64,
+        dryIceAlb_south=0.72, drySnowAlb_south=0.8)),
 }
 GROWTH_OUT = ("hIceMean", "hSnowMean", "Area", "TSurf", "EmPmR", "forc_salt_surface_ice", "Qsw",
               "Qnet", "SeaIceLoad", "IcePenetSW", "recip_hIceMean")
@@ -340,7 +366,7 @@ def test_step_mutation_list_is_not_empty():
 def test_c_comparison_catches_step_mistake(mutation, tmp_path):
     mod = _mutant(mutation, tmp_path)
     for variant in ("tiny", "moving_ice", "melting", "thin_ice", "config0", "sea_ice_load",
-                    "ice_edge"):
+                    "ice_edge", "turning_angles", "wet_albedo"):
         try:
             got, want = _step_pair(mod, variant)
         except Exception:  # a planted mistake may also make the oracle raise

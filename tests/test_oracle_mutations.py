@@ -100,6 +100,37 @@ MUTATIONS = {
     "lead closing": ("dAoa = np.where(grow, gOW * (1 - AP), dAoa)", "dAoa = np.where(grow, gOW * AP, dAoa)"),
     "flooding density ratio": ("hs = hs - flood * s.rhoIce2rhoSnow", "hs = hs - flood"),
     "fresh water sign": ("- (fwIce + fwSnow) / s.deltatTherm) * s.rhoFresh", "+ (fwIce + fwSnow) / s.deltatTherm) * s.rhoFresh"),
+    "basal east-face speed": ("t = 0.25 * (um**2 + ip1(um) ** 2 + vm**2 + jp1(vm) ** 2)", "t = 0.25 * (um**2 + im1(um) ** 2 + vm**2 + jp1(vm) ** 2)"),
+    "basal north-face speed": ("t = 0.25 * (um**2 + ip1(um) ** 2 + vm**2 + jp1(vm) ** 2)", "t = 0.25 * (um**2 + ip1(um) ** 2 + vm**2 + jm1(vm) ** 2)"),
+    "basal velocity scale": ("np.sqrt(t + s.basalDragU0**2)", "np.sqrt(t + s.basalDragU0)"),
+    "basal critical height": ("hcrit = np.abs(vs.R_low) * vs.Area / s.basalDragK1", "hcrit = np.abs(vs.R_low) * vs.Area * s.basalDragK1"),
+    "basal depth sign": ("hcrit = np.abs(vs.R_low) * vs.Area / s.basalDragK1", "hcrit = vs.R_low * vs.Area / s.basalDragK1"),
+    "basal soft maximum": ("rfac = 1.0 / fac", "rfac = 1.0"),
+    "basal concentration decay": ("np.exp(-s.cBasalStar * (1.0 - vs.Area))", "np.exp(-s.cBasalStar * vs.Area)"),
+    "basal area gate": ("np.where(vs.Area > 0.01, val, 0.0)", "np.where(vs.Area >= 0.01, val, 0.0)"),
+    "basal drag explicit": ("denU = vs.SeaIceMassU * bp1U + vs.AreaW * (\n        0.5 * (cd + im1(cd)) * s.cosWat + 0.5 * (cb + im1(cb)))", "denU = vs.SeaIceMassU * bp1U + vs.AreaW * (\n        0.5 * (cd + im1(cd)) * s.cosWat)"),
+    "turning x sign": ("        - sg * s.sinWat * 0.5 * (", "        + sg * s.sinWat * 0.5 * ("),
+    "turning y sign": ("        + sg * s.sinWat * 0.5 * (", "        - sg * s.sinWat * 0.5 * ("),
+    "turning x north corner": ("cd * 0.5 * (du0 + jp1(du0))", "cd * 0.5 * (du0 + jm1(du0))"),
+    "turning x west cell": ("+ im1(cd) * 0.5 * (im1(du0) + jp1(im1(du0)))", "+ im1(cd) * 0.5 * (ip1(du0) + jp1(ip1(du0)))"),
+    "turning y east corner": ("cd * 0.5 * (dv0 + ip1(dv0))", "cd * 0.5 * (dv0 + im1(dv0))"),
+    "turning y south cell": ("+ jm1(cd) * 0.5 * (jm1(dv0) + ip1(jm1(dv0)))", "+ jm1(cd) * 0.5 * (jp1(dv0) + ip1(jp1(dv0)))"),
+    "turning ice difference": ("du0 = vs.uOcean - u", "du0 = vs.uOcean + u"),
+    "turning mass gate": ("lmU = np.where(vs.SeaIceMassU != 0, 1, vs.SeaIceMassU)", "lmU = np.where(vs.SeaIceMassU != 0, 1, 1)"),
+    "ocean stress turning sign": ("tv = 0.5 * (cd + jm1(cd)) * cosW * dv + sg * sinW", "tv = 0.5 * (cd + jm1(cd)) * cosW * dv - sg * sinW"),
+    "ocean stress turning average": ("- sg * sinW * 0.5 * (cd * dvC + jm1(cd * dvC))", "- sg * sinW * 0.5 * (cd * dvC + jp1(cd * dvC))"),
+    "ocean stress c-point difference": ("duC = 0.5 * (du + ip1(du))", "duC = 0.5 * (du + im1(du))"),
+    "ocean stress angle units": ("sinW = np.sin(np.deg2rad(s.waterTurnAngle))", "sinW = np.sin(s.waterTurnAngle)"),
+    "air turning sign": ("tx = cda * (cosWin * urel - sg * sinWin * vrel)", "tx = cda * (cosWin * urel + sg * sinWin * vrel)"),
+    "air turning angle units": ("cosWin = np.cos(np.deg2rad(s.airTurnAngle))", "cosWin = np.cos(s.airTurnAngle)"),
+    "load gravity": ("(vs.surfPress + vs.SeaIceLoad * s.gravity * s.seaIceLoadFac)", "(vs.surfPress + vs.SeaIceLoad * s.seaIceLoadFac)"),
+    "load density": ("+ vs.SeaIceLoad * s.gravity * s.seaIceLoadFac) * s.recip_rhoSea", "+ vs.SeaIceLoad * s.gravity * s.seaIceLoadFac) * s.rhoSea"),
+    "wet albedo threshold": ("wet = (hI > 0) & (T >= Tmeltw)", "wet = (hI > 0) & (T > Tmeltw)"),
+    "wet ice albedo": ("aI = np.where(wet, s.wetIceAlb, aI)", "aI = np.where(wet, s.dryIceAlb, aI)"),
+    "wet snow albedo": ("aS = np.where(wet, s.wetSnowAlb, aS)", "aS = np.where(wet, s.drySnowAlb, aS)"),
+    "south dry albedo": ("aI = np.where(south, s.dryIceAlb_south, aI)", "aI = np.where(south, s.dryIceAlb, aI)"),
+    "south wet snow albedo": ("aS = np.where(wets, s.wetSnowAlb_south, aS)", "aS = np.where(wets, s.wetSnowAlb, aS)"),
+    "south wet threshold": ("wets = (hI > 0) & (vs.fCori < 0) & (T >= Tmeltw)", "wets = (hI > 0) & (vs.fCori > 0) & (T >= Tmeltw)"),
 }
 
 

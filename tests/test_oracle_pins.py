@@ -1428,3 +1428,298 @@ def test_bulk_fluxes_against_published_values():
     np.testing.assert_allclose(o["ust"][0], s.rhoAir * 1.176e-3 * 100.0, rtol=1e-12)
     np.testing.assert_allclose(o["vst"][1], s.rhoAir * 1.176e-3 * 100.0, rtol=1e-12)
     assert o["ust"][1] == 0 and o["vst"][0] == 0
+
+
+# -- pins of the non-default branches: basal drag, turning angles, the sea-ice
+#    load in the surface tilt, the wet-albedo cascade ----------------------------
+
+INNER3 = (slice(3, -3), slice(3, -3))
+
+
+def _basal_state(h, A, u0=0.0, v0=0.0, R=20.0):
+    st = uniform_state(nx=10, ny=8, basalDragK2=15.0)
+    vs = st.variables
+    vs.hIceMean = np.full(st.shape, h)
+    vs.Area = np.full(st.shape, A)
+    vs.R_low = np.full(st.shape, -R)  # the depth enters as |R_low|
+    return st, np.full(st.shape, u0), np.full(st.shape, v0)
+
+
+def test_basal_drag_closed_forms():
+    """Lemieux et al. (2015) basal drag with the soft maximum
+    ln(exp(k a) + 1)/k, k = 10, of a = h - hCrit, hCrit = |R_low| A / K1:
+    thick keels give K2 / sqrt(|u|^2_c + U0^2) (h - hCrit) exp(-C (1 - A));
+    at h = hCrit the soft maximum is ln 2 / k; far below hCrit it vanishes;
+    no drag where A <= 0.01.  |u|^2_c is the mean of the squared velocities
+    of the four faces of the c-point: (u0^2 + v0^2) / 2 for uniform fields."""
+    s = uniform_state().settings
+    A, u0, v0, R = 0.8, 0.03, -0.04, 20.0
+    hcrit = R * A / s.basalDragK1
+    pre = 15.0 / np.sqrt(0.5 * (u0**2 + v0**2) + s.basalDragU0**2) * np.exp(-s.cBasalStar * (1 - A))
+    # far below hCrit, ln(exp(-50) + 1) rounds to ln(1) = 0 in FP64
+    for dh, soft in ((5.0, 5.0), (0.0, np.log(2.0) / 10.0), (-5.0, 0.0)):
+        st, u, v = _basal_state(hcrit + dh, A, u0, v0, R)
+        cb = O.basal_drag_coeffs(st, u, v)
+        np.testing.assert_allclose(cb[INNER], pre * soft, rtol=1e-12, atol=1e-300)
+    st, u, v = _basal_state(hcrit + 5.0, 0.01, u0, v0, R)
+    assert np.all(O.basal_drag_coeffs(st, u, v) == 0)
+    st, u, v = _basal_state(hcrit + 5.0, 0.0101, u0, v0, R)
+    assert np.all(O.basal_drag_coeffs(st, u, v)[INNER] > 0)
+
+
+def test_basal_drag_speed_uses_the_four_faces_of_the_cell():
+    """A single moving u-face (i0) is the east face of c-cell i0-1 and the
+    west face of cell i0: only those two cells see |u|^2_c = u^2/4; a single
+    moving v-face (j0) only cells j0-1 and j0."""
+    s = uniform_state().settings
+    A, R = 0.8, 20.0
+    hcrit = R * A / s.basalDragK1
+    st, u, v = _basal_state(hcrit + 5.0, A, 0.0, 0.0, R)
+    rest = O.basal_drag_coeffs(st, u, v)
+    pre = lambda sq: 15.0 / np.sqrt(sq + s.basalDragU0**2) * 5.0 * np.exp(-s.cBasalStar * (1 - A))  # noqa: E731
+    np.testing.assert_allclose(rest[INNER], pre(0.0), rtol=1e-12)
+    i0, j0, w = 6, 5, 0.08
+    u1 = u.copy()
+    u1[i0, :] = w
+    cb = O.basal_drag_coeffs(st, u1, v)
+    changed = np.argwhere(np.any(cb != rest, axis=1)).ravel()
+    assert list(changed) == [i0 - 1, i0]
+    np.testing.assert_allclose(cb[i0 - 1, 2:-2], pre(0.25 * w * w), rtol=1e-12)
+    v1 = v.copy()
+    v1[:, j0] = w
+    cb = O.basal_drag_coeffs(st, u, v1)
+    changed = np.argwhere(np.any(cb != rest, axis=0)).ravel()
+    assert list(changed) == [j0 - 1, j0]
+    np.testing.assert_allclose(cb[2:-2, j0], pre(0.25 * w * w), rtol=1e-12)
+
+
+def test_basal_drag_enters_the_momentum_denominators():
+    """One substep from a uniform state with P = 0, f = 0, water drag in its
+    cDragMin regime and basal drag cB: the velocity update is
+    u' = (beta m u/dt + m uNm1/dt + F + A C cos uo) / (m (beta + 1)/dt + A (C cos + cB))
+    -- the basal drag acts implicitly, in the denominator only."""
+    s0 = uniform_state().settings
+    A, R = 0.8, 20.0
+    hcrit = R * A / s0.basalDragK1
+    st, u, v = _basal_state(hcrit + 0.3, A, 0.01, -0.02, R)
+    st.settings.veros_fill = False
+    vs, s = st.variables, st.settings
+    m, tau, uo, vo = 900.0, 2e-3, 0.005, -0.003
+    for k in ("SeaIceMassC", "SeaIceMassU", "SeaIceMassV"):
+        vs[k] = np.full(st.shape, m)
+    vs.AreaW = np.full(st.shape, A)
+    vs.AreaS = np.full(st.shape, A)
+    vs.WindForcingX = np.full(st.shape, tau)
+    vs.WindForcingY = np.full(st.shape, -tau)
+    vs.uOcean = np.full(st.shape, uo)
+    vs.vOcean = np.full(st.shape, vo)
+    vs.uIce, vs.vIce = u, v
+    cb = float(O.basal_drag_coeffs(st, u, v)[5, 5])
+    assert cb > 0
+    un, vn, *_ = O.evp_substep(st, u, v, u, v, vs.sigma1, vs.sigma2, vs.sigma12)
+    dt, beta, C = s.deltatDyn, s.evpBeta, s.cDragMin
+    den = m * (beta + 1) / dt + A * (C * s.cosWat + cb)
+    for got, x0, f, oc in ((un, 0.01, tau, uo), (vn, -0.02, -tau, vo)):
+        want = (beta * m * x0 / dt + m * x0 / dt + f + A * C * s.cosWat * oc) / den
+        np.testing.assert_allclose(got[INNER], want, rtol=1e-12)
+
+
+def _turning_state(f, angle=25.0):
+    st = _rest_state(sinWat=np.sin(np.deg2rad(angle)), cosWat=np.cos(np.deg2rad(angle)))
+    st.variables.fCori = np.full(st.shape, f)
+    return st
+
+
+def test_evp_turning_term_on_linear_ocean_currents():
+    """Water-stress turning term of the momentum forcing (the reference forms
+    it from the u-difference uOcean - uIce in ForcingX and the v-difference in
+    ForcingY): with drag in its cDragMin regime C and linear currents, the
+    four-point averages sample the velocity differences at the corner
+    (i - 1/2, j + 1/2) for u and (i + 1/2, j - 1/2) for v, so
+    Fx = WFX + A (C cos uo - sgn(f) sin C (uo - u)(i-1/2, j+1/2)) + Coriolis,
+    Fy = WFY + A (C cos vo + sgn(f) sin C (vo - v)(i+1/2, j-1/2)) - Coriolis,
+    in both hemispheres; the term vanishes where the ice mass is zero."""
+    for f in (1.3e-4, -1.1e-4):
+        st = _turning_state(f)
+        vs, s = st.variables, st.settings
+        x, y, dx, dy = xy(st)
+        I, J = x / dx, y / dy
+        uo_f = lambda i, j: 1e-3 + 2e-5 * i - 3e-5 * j  # noqa: E731
+        vo_f = lambda i, j: -2e-3 + 4e-5 * i + 1e-5 * j  # noqa: E731
+        vs.uOcean, vs.vOcean = uo_f(I, J), vo_f(I, J)
+        u0, m, A = 4e-4, 900.0, 1.0
+        u = np.full(st.shape, u0)
+        v = np.zeros(st.shape)
+        vs.WindForcingX = np.full(st.shape, 1e-3)
+        vs.WindForcingY = np.full(st.shape, 2e-3)
+        vs.SeaIceMassU = vs.SeaIceMassU.copy()
+        vs.SeaIceMassU[:, 7] = 0.0  # no ice mass at these u-points: no turning term
+        tr = {}
+        O.evp_substep(st, u, v, u, v, vs.sigma1, vs.sigma2, vs.sigma12, trace=tr)
+        C, sg = s.cDragMin, np.sign(f)
+        lmU = (vs.SeaIceMassU != 0).astype(float)
+        fx = 1e-3 + A * (C * s.cosWat * vs.uOcean - sg * s.sinWat * C * (uo_f(I - 0.5, J + 0.5) - u0) * lmU)
+        fx = fx + m * f * 0.0  # v = 0: no Coriolis term in x
+        fy = 2e-3 + A * (C * s.cosWat * vs.vOcean + sg * s.sinWat * C * vo_f(I + 0.5, J - 0.5))
+        fy = fy - m * f * u0
+        close(tr["ForcingX"][INNER3], fx[INNER3], 1e-12)
+        close(tr["ForcingY"][INNER3], fy[INNER3], 1e-12)
+
+
+def test_evp_turning_term_steady_state():
+    """P = 0, uniform fields, cDragMin regime, turning angle and Coriolis: the
+    subcycle converges to the steady balance of the momentum equation,
+    (m/dt + A C (cos - s sin)) u - m f v = m uNm1/dt + WFX + A C (cos - s sin) uo,
+    (m/dt + A C (cos + s sin)) v + m f u = m vNm1/dt + WFY + A C (cos + s sin) vo,
+    s = sgn(f) (the turning term's u-difference in x and v-difference in y)."""
+    for f in (1.2e-4, -1.2e-4):
+        st = _turning_state(f, angle=20.0)
+        vs, s = st.variables, st.settings
+        m, A, C, dt = 900.0, 1.0, s.cDragMin, s.deltatDyn
+        uo, vo, wx, wy, u0, v0 = 0.004, -0.006, 1e-3, -5e-4, 0.002, 0.001
+        vs.uOcean = np.full(st.shape, uo)
+        vs.vOcean = np.full(st.shape, vo)
+        vs.WindForcingX = np.full(st.shape, wx)
+        vs.WindForcingY = np.full(st.shape, wy)
+        vs.uIce = np.full(st.shape, u0)
+        vs.vIce = np.full(st.shape, v0)
+        sg = np.sign(f)
+        cx = A * C * (s.cosWat - sg * s.sinWat)
+        cy = A * C * (s.cosWat + sg * s.sinWat)
+        M = np.array([[m / dt + cx, -m * f], [m * f, m / dt + cy]])
+        rhs = np.array([m * u0 / dt + wx + cx * uo, m * v0 / dt + wy + cy * vo])
+        us, vs_ = np.linalg.solve(M, rhs)
+        # the Jacobi iteration contracts by about 0.97 per substep here
+        u, v, *_ = O.evp_solver(st, n_substeps=1000)
+        np.testing.assert_allclose(u[INNER], us, rtol=1e-9)
+        np.testing.assert_allclose(v[INNER], vs_, rtol=1e-9)
+
+
+def test_ocean_stress_turning_rotates_the_drag():
+    """OceanStressUV with a turning angle: on uniform fields the stress is the
+    drag coefficient times the ice-ocean velocity difference rotated by
+    sgn(f) * angle (magnitude D |du|^2, quadratic drag); on linear fields in
+    the cDragMin regime the same rotation holds cell by cell."""
+    angle = 25.0
+    th = np.deg2rad(angle)
+    for f in (1e-4, -1e-4):
+        sg = np.sign(f)
+        st = uniform_state(nx=10, ny=9, waterTurnAngle=angle)
+        vs, s = st.variables, st.settings
+        vs.fCori = np.full(st.shape, f)
+        du, dv = 0.3, -0.2
+        vs.uIce = np.full(st.shape, du)
+        vs.vIce = np.full(st.shape, dv)
+        tu, tv = O.OceanStressUV(st)
+        D = (s.waterIceDrag if f > 0 else s.waterIceDrag_south) * s.rhoSea
+        mag = D * (du * du + dv * dv)
+        np.testing.assert_allclose(np.hypot(tu, tv)[INNER], mag, rtol=1e-13)
+        ang = np.arctan2(tv, tu) - np.arctan2(dv, du)
+        np.testing.assert_allclose(ang[INNER], sg * th, rtol=1e-12)
+        x, y, dx, dy = xy(st)
+        I, J = x / dx, y / dy
+        dU = 2e-3 + 1e-4 * I - 2e-4 * J
+        dV = -1e-3 + 3e-4 * I + 1e-4 * J
+        vs.uIce, vs.vIce = dU, dV
+        tu, tv = O.OceanStressUV(st)
+        C = s.cDragMin
+        close(tu[INNER3], (C * (np.cos(th) * dU - sg * np.sin(th) * dV))[INNER3], 1e-12)
+        close(tv[INNER3], (C * (np.cos(th) * dV + sg * np.sin(th) * dU))[INNER3], 1e-12)
+
+
+def test_wind_stress_turning_rotates_the_bulk_stress():
+    """tauXY with airTurnAngle: |tau| = rhoAir Ca |U|^2 (the bulk law is
+    unchanged) and tau is the wind rotated by sgn(f) * angle; the southern
+    drag coefficient applies where f < 0."""
+    angle = 30.0
+    th = np.deg2rad(angle)
+    for f in (1e-4, -1e-4):
+        st = uniform_state(nx=6, ny=5, airTurnAngle=angle, airIceDrag_south=0.0015)
+        vs, s = st.variables, st.settings
+        vs.fCori = np.full(st.shape, f)
+        vs.uWind = np.full(st.shape, 6.0)
+        vs.vWind = np.full(st.shape, -8.0)
+        tx, ty = O.tauXY(st)
+        ca = s.airIceDrag if f > 0 else s.airIceDrag_south
+        np.testing.assert_allclose(np.hypot(tx, ty)[INNER], s.rhoAir * ca * 100.0, rtol=1e-14)
+        ang = np.arctan2(ty, tx) - np.arctan2(-8.0, 6.0)
+        np.testing.assert_allclose(ang[INNER], np.sign(f) * th, rtol=1e-13)
+
+
+def test_sea_ice_load_tilt_is_the_load_gradient():
+    """useRealFreshWaterFlux adds the sea-ice load to the surface pressure:
+    the extra forcing is -m g fac dL/dx / rhoSea at u-points and
+    -m g fac dL/dy / rhoSea at v-points, on a linear load L."""
+    st = uniform_state(nx=8, ny=7, dx=2e4, dy=3e4, seaIceLoadFac=0.7)
+    vs, s = st.variables, st.settings
+    x, y, dx, dy = xy(st)
+    a, b = 2e-3, -5e-3  # kg/m^2 per metre
+    vs.SeaIceLoad = 900.0 + a * (x + 0.5 * dx) + b * (y + 0.5 * dy)
+    vs.SeaIceMassU = np.full(st.shape, 700.0)
+    vs.SeaIceMassV = np.full(st.shape, 800.0)
+    vs.AreaW = np.ones(st.shape)
+    vs.AreaS = np.ones(st.shape)
+    fx0, fy0 = O.WindForcingXY(st)
+    st.settings.useRealFreshWaterFlux = True
+    fx1, fy1 = O.WindForcingXY(st)
+    g, fac = s.gravity, 0.7
+    close((fx1 - fx0)[INNER], np.full(st.shape, -700.0 * g * fac * a / s.rhoSea)[INNER], 1e-9)
+    close((fy1 - fy0)[INNER], np.full(st.shape, -800.0 * g * fac * b / s.rhoSea)[INNER], 1e-9)
+
+
+def test_solve4temp_wet_albedo_at_the_melting_point():
+    """A surface that starts the step at the melting point (TSurf >= Tmelt +
+    wetAlbTemp) uses the wet albedos, north or south: the surface temperature
+    is the root of the energy balance with those albedos (different southern
+    values set here), and one just below it uses the dry ones."""
+    st = uniform_state(nx=6, ny=4, wetIceAlb_south=0.6, wetSnowAlb_south=0.64,
+                       dryIceAlb_south=0.72, drySnowAlb_south=0.8)
+    s = st.settings
+    vs = st.variables
+    r = np.random.default_rng(16)
+    h = r.uniform(0.2, 3.0, st.shape)
+    hs = np.where(r.random(st.shape) < 0.4, 0.0, r.uniform(0.01, 0.4, st.shape))
+    vs.LWdown = r.uniform(150, 280, st.shape)
+    vs.ATemp = r.uniform(235, 262, st.shape)
+    vs.wSpeed = r.uniform(1, 12, st.shape)
+    vs.SWdown = r.uniform(50, 250, st.shape)
+    vs.aqh = r.uniform(1e-4, 2e-3, st.shape)
+    Tfrz = np.full(st.shape, s.tempFrz + s.celsius2K)
+    Tmelt = s.celsius2K + s.wetAlbTemp
+    checked = 0
+    for f, south in ((1e-4, False), (-1e-4, True)):
+        vs.fCori = np.full(st.shape, f)
+        for tin, wet in ((Tmelt, True), (Tmelt - 0.01, False)):
+            T, *_ = O.solve4temp(st, h, hs, np.full(st.shape, tin), Tfrz)
+            for idx in np.ndindex(st.shape):
+                args = (h[idx], hs[idx], s, vs.LWdown[idx], vs.ATemp[idx], vs.wSpeed[idx],
+                        vs.aqh[idx], vs.SWdown[idx], Tfrz[idx], south, wet)
+                lo, hi = s.celsius2K + s.minTIce, s.celsius2K
+                if _seb_residual_alb(lo, *args) * _seb_residual_alb(hi, *args) > 0:
+                    continue
+                root = brentq(_seb_residual_alb, lo, hi, args=args, xtol=1e-12)
+                assert abs(T[idx] - root) < 1e-6, (f, tin, idx, T[idx], root)
+                checked += 1
+    assert checked > 40
+
+
+def _seb_residual_alb(T, h, hs, s, LWdown, ATemp, wSpeed, aqh, SWdown, Tfrz, south, wet):
+    """_seb_residual with the albedos of the given hemisphere and surface state."""
+    suffix = "_south" if south else ""
+    kind = "wet" if wet else "dry"
+    k_eff = s.iceConduct * s.snowConduct / (s.snowConduct * h + s.iceConduct * hs)
+    snow = hs > 0
+    emis = s.snowEmiss if snow else s.iceEmiss
+    alb_i = s[f"{kind}IceAlb{suffix}"]
+    alb_s = s[f"{kind}SnowAlb{suffix}"]
+    alb = alb_i if hs == 0 else (alb_s if hs > s.hCut else alb_i + hs / s.hCut * (alb_s - alb_i))
+    pen = 0.0 if snow else s.shortwave * np.exp(-1.5 * h)
+    absorbed = (1 - alb) * (1 - pen) * SWdown
+    es = 10 ** (12.537 - 2663.5 / T)
+    qs = 0.622 * es / (1e5 - 0.378 * es)
+    u = max(wSpeed, s.wSpeedMin)
+    ta = max(ATemp, s.celsius2K + s.minTAir)
+    f_up = (emis * s.stefBoltz * T**4 - emis * max(LWdown, s.minLWdown) - absorbed
+            + s.rhoAir * s.cpAir * s.dalton * u * (T - ta)
+            + s.rhoAir * s.lhSublim * s.dalton * u * (qs - aqh))
+    return k_eff * (Tfrz - T) - f_up
