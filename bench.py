@@ -211,6 +211,35 @@ def bench_batched(dev, nsub: int, fp64_evp_rate: float, fp64_step_rate: float) -
     err_same = {name: _rel_l2(list(zip(e32.unpack(out32[q]), e64q.unpack(out64q[q]), masks)))
                 for q, name in enumerate(EVP_OUT)}
     del e64q
+    # how well the FP64 reference itself is determined: the FP64 subcycle
+    # with the ice strength perturbed by 1e-15 (a few ulps), and where the
+    # difference sits; the FP32 error over the cells that perturbation leaves
+    # stable (below the 99th percentile of its velocity response)
+    P = e64.fields["SeaIceStrength"]
+    P0 = P.clone()
+    gen = torch.Generator(device=dev).manual_seed(5)
+    P.mul_(1 + 1e-15 * torch.randn(P.shape, generator=gen, device=dev, dtype=P.dtype))
+    outp = e64.evp(nsub, tile=tile64)
+    P.copy_(P0)
+    err_pert = {name: _rel_l2(list(zip(e64.unpack(outp[q]), e64.unpack(out64[q]), masks)))
+                for q, name in enumerate(EVP_OUT[:2])}
+    def interior(buf, m):
+        v = e64.view(buf, m)[2:-2, 2:-2]
+        return v[e64.view(e64.fields["maskInC"], m)[2:-2, 2:-2] > 0]
+
+    resp = torch.cat([interior((outp[0] - out64[0]).abs() + (outp[1] - out64[1]).abs(), m)
+                      for m in range(e64.n)])
+    cut = float(torch.quantile(resp.double()[:: max(1, resp.numel() // 2_000_000)], 0.99))
+    stable = ((outp[0] - out64[0]).abs() + (outp[1] - out64[1]).abs() <= cut).to(
+        out64[0].dtype) * e64.fields["maskInC"]
+    err_stable = {name: _rel_l2(list(zip(e32.unpack(out32[q]), e64.unpack(out64[q]),
+                                         e64.unpack(stable))))
+                  for q, name in enumerate(EVP_OUT[:2])}
+    d2 = torch.cat([interior((out32[0].double() - out64[0]) ** 2
+                             + (out32[1].double() - out64[1]) ** 2, m) for m in range(e64.n)])
+    top = torch.sort(d2, descending=True).values
+    top_share = float(top[: max(1, top.numel() // 100)].sum() / top.sum().clamp_min(1e-300))
+    del outp
     b32 = e32.budgets()
     b64 = e64.budgets()
     budget_err = {k: float(((b32[k] - b64[k]).abs() / b64[k].abs().clamp_min(1e-300)).max())
@@ -239,6 +268,9 @@ def bench_batched(dev, nsub: int, fp64_evp_rate: float, fp64_step_rate: float) -
         "velocity_error_target": 1e-4,
         "fp64_on_fp32_rounded_inputs_rel_l2_vs_fp64": err_inputs,
         "fp32_rel_l2_vs_fp64_on_fp32_rounded_inputs": err_same,
+        "fp64_strength_perturbed_1e-15_rel_l2_vs_fp64": err_pert,
+        "fp32_squared_velocity_error_share_of_worst_1pct_cells": top_share,
+        "fp32_rel_l2_vs_fp64_on_cells_stable_under_1e-15": err_stable,
         "fp32_storage_fp64_arith": {
             "value": cells * nsub / t32a,
             "tile": {"index": tile32a, "shape": shapes[tile32a]},
@@ -493,40 +525,49 @@ def bench_hires(dev) -> dict:
     res["fp64_budget_rel_diff"] = budgets
     res["shard_8way"] = bench_shard(dev, st, out)
 
-    # batched adjoint: 8 members at 240 x 240, FP32, concurrent on 4 streams
-    members = [synth.config4_adjoint_member(mm).to(dev, torch.float32) for mm in range(8)]
-    n_adj = int(members[0].settings.nEVPsteps)
-    streams = [torch.cuda.Stream(dev) for _ in range(4)]
+    # batched adjoint: 8 members at 240 x 240, FP32, all members in one launch
+    # sequence forward and backward; the forward it is divided by is the
+    # batched drop-in (one pre-EVP launch + the batched subcycle), timed the
+    # same way on the same stream
+    from veris_amd.batched import Ensemble
 
-    def member_loss(mm, stt):
-        h = stt.variables.hIceMean.detach().clone().requires_grad_(True)
-        ps = torch.tensor(stt.settings.pStar, dtype=torch.float64, device=dev, requires_grad=True)
-        outs = A.evp_subcycle(stt, pStar=ps, hIceMean=h, n_substeps=n_adj, ckpt_every=10)
-        return ((outs[0][2:-2, 2:-2]) ** 2).sum() + ((outs[1][2:-2, 2:-2]) ** 2).sum()
+    ens = Ensemble([synth.config4_adjoint_member(mm) for mm in range(8)], dev, torch.float32)
+    n_adj = int(ens.settings.nEVPsteps)
+    ps = torch.tensor(float(ens.settings.pStar), dtype=torch.float64, device=dev)
 
     def batched_adjoint():
-        cur = torch.cuda.current_stream(dev)
-        for mm, stt in enumerate(members):
-            s2 = streams[mm % len(streams)]
-            s2.wait_stream(cur)
-            with torch.cuda.stream(s2):
-                member_loss(mm, stt).backward()
-        for s2 in streams:
-            cur.wait_stream(s2)
+        h = ens.fields["hIceMean"].detach().clone().requires_grad_(True)
+        p_ = ps.clone().requires_grad_(True)
+        outs = A.evp_subcycle_batched(ens, pStar=p_, hIceMean=h, n_substeps=n_adj, ckpt_every=10)
+        ((outs[0].double() ** 2).sum() + (outs[1].double() ** 2).sum()).backward()
+
+    def batched_ckpt_forward():
+        with torch.no_grad():
+            A.evp_subcycle_batched(ens, n_substeps=n_adj, ckpt_every=10)
+
+    out_b = None
+    from veris_amd.autotune import tune_batched
+
+    tile_b, _ = tune_batched(ens, exact=False)  # closed members: every shape is exact
 
     def batched_forward():
-        for stt in members:
-            K.evp_solver(stt, n_adj)
+        nonlocal out_b
+        ens.pre_evp()
+        out_b = ens.evp(n_adj, tile=tile_b, out=out_b)
 
-    batched_adjoint()
-    batched_forward()
-    t_adj = _sync_time(batched_adjoint, 3)
-    t_fwd = _sync_time(batched_forward, 3)
+    for fn in (batched_adjoint, batched_ckpt_forward, batched_forward):
+        fn()
+    t_adj = _sync_time(batched_adjoint, 5)
+    t_ck = _sync_time(batched_ckpt_forward, 5)
+    t_fwd = _sync_time(batched_forward, 5)
     res["batched_adjoint"] = {
-        "workload": f"8 regional members 240x240, FP32, {n_adj} EVP substeps, checkpoint every "
-                    "10, gradient of a velocity norm w.r.t. P* and hIceMean; members on 4 streams",
-        "forward_backward_ms": t_adj * 1e3, "plain_forward_ms": t_fwd * 1e3,
-        "adjoint_over_plain_forward": t_adj / t_fwd}
+        "workload": f"8 regional members 240x240 (10-cell thickness noise), FP32, {n_adj} EVP "
+                    "substeps, checkpoint every 10, gradient of a velocity norm w.r.t. P* and "
+                    "hIceMean; forward and reverse each one batched launch sequence",
+        "forward_backward_ms": t_adj * 1e3, "checkpointed_forward_ms": t_ck * 1e3,
+        "plain_forward_ms": t_fwd * 1e3, "plain_forward_tile": tile_b,
+        "adjoint_over_plain_forward": t_adj / t_fwd,
+        "adjoint_over_checkpointed_forward": t_adj / t_ck}
     return res
 
 

@@ -486,6 +486,34 @@ int vh_pre_evp_adjoint(const vh_domain* d, const vh_settings* s, const vh_fields
                        const void* pStarField, const vh_evp_adjoint_out* evp_adj,
                        const vh_pre_evp_adjoint_out* out, void* stream);
 
+/* Batched adjoint of n_members equal members: every field of `f`, every
+ * output and every adjoint holds n_members padded boxes of the geometry `d`
+ * back to back (member m at element m * (nx+4) * (ny+4)), and one launch
+ * sequence runs all members (launch grids carry the member index).  Each
+ * member's results equal the single-domain entry point's on that member bit
+ * for bit; the settings (drag coefficients among them) are shared.  The
+ * reference has no batching: these are vh_evp_adjoint_workspace_bytes,
+ * vh_evp_forward_checkpointed, vh_evp_adjoint and vh_pre_evp_adjoint (the
+ * reverse of veris/evp_solver.py:22-319 and of the pre-EVP forcing) for
+ * ensembles of equal regional domains. */
+int vh_evp_adjoint_batched_workspace_bytes(const vh_domain* d, int32_t n_members,
+                                           int32_t n_substeps, int32_t ckpt_every, size_t* bytes);
+int vh_evp_forward_checkpointed_batched(const vh_domain* d, int32_t n_members,
+                                        const vh_settings* s, const vh_fields* f,
+                                        int32_t n_substeps, int32_t ckpt_every, void* uIce,
+                                        void* vIce, void* sigma1, void* sigma2, void* sigma12,
+                                        void* workspace, size_t workspace_bytes, void* stream);
+int vh_evp_adjoint_batched(const vh_domain* d, int32_t n_members, const vh_settings* s,
+                           const vh_fields* f, int32_t n_substeps, int32_t ckpt_every,
+                           const void* adj_uIce, const void* adj_vIce, const void* adj_sigma1,
+                           const void* adj_sigma2, const void* adj_sigma12,
+                           const vh_evp_adjoint_out* out, void* workspace,
+                           size_t workspace_bytes, void* stream);
+int vh_pre_evp_adjoint_batched(const vh_domain* d, int32_t n_members, const vh_settings* s,
+                               const vh_fields* f, const void* pStarField,
+                               const vh_evp_adjoint_out* evp_adj,
+                               const vh_pre_evp_adjoint_out* out, void* stream);
+
 /* Outputs of vh_pre_evp (caller-owned device fields). */
 typedef struct vh_pre_evp_out {
   void *SeaIceMassC, *SeaIceMassU, *SeaIceMassV, *AreaW, *AreaS, *WindForcingX, *WindForcingY,
@@ -498,6 +526,26 @@ typedef struct vh_pre_evp_out {
  * the same results as the four separate entry points. */
 int vh_pre_evp(const vh_domain* d, const vh_settings* s, const vh_fields* f,
                const void* pStarField, const vh_pre_evp_out* out, void* stream);
+
+/* vh_pre_evp of every member of a batch (vh_batch, packed fields as for
+ * vh_evp_solver_batched; pStarField packed or NULL) in one launch per 64
+ * members: each member's outputs equal vh_pre_evp's on that member bit for
+ * bit.  No counterpart in the reference, which has no batching: it runs
+ * SeaIceMass, AreaWS, WindForcingXY and SeaIceStrength per domain
+ * (veris/area_mass.py:5-28, veris/dynsolver.py:7-84,
+ * veris/dynamics_routines.py:5-18). */
+int vh_pre_evp_batched(const vh_batch* b, const vh_settings* s, const vh_fields* f,
+                       const void* pStarField, const vh_pre_evp_out* out, void* stream);
+
+/* FP64 budgets of a batch in one launch per 64 members: for each of the
+ * n_fields (1..8) packed fields (host array of device pointers) and each
+ * member m, the FP64 sum over m's interior cells of field times `weight`
+ * (packed, or NULL for 1), written to result[q * n_members + m] (device
+ * memory, n_fields * n_members doubles).  The batched form of global_sum
+ * (veris/global_sum.py:1-4, a stub in the reference) for the mass, snow,
+ * area and heat budgets (weights rA). */
+int vh_global_sums_batched(const vh_batch* b, int32_t n_fields, const void* const* fields,
+                           const void* weight, double* result, void* stream);
 
 /* Outputs of vh_ocean_to_ice. */
 typedef struct vh_ocean_to_ice_out {

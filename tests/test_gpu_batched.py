@@ -216,3 +216,39 @@ def test_batched_branches_fp64_match_oracle(cuda, branch, tile):
         for q, name in enumerate(EVP_NAMES):
             assert_exact(f"{branch} member {m} {name}[tile={tile}]", g2n(ens.view(out[q], m)),
                          want[q])
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_batched_pre_evp_and_budgets_one_launch(cuda, dtype):
+    """vh_pre_evp_batched equals vh_pre_evp on every member bit for bit (with
+    a packed P* field too), and vh_global_sums_batched gives each member's
+    FP64 budgets of h, hs, A and Qnet (weight rA) within 1e-13 of math.fsum;
+    70 members, so the launches run in two chunks of at most 64."""
+    import math
+
+    from veris_amd import kernels as K
+
+    sizes = [(9 + (m * 7) % 23, 8 + (m * 5) % 19) for m in range(70)]
+    states = [synth.regional(24, m, nx, ny) for m, (nx, ny) in enumerate(sizes)]
+    states[3] = synth.tiny(20, 14, config=25)  # one x-periodic member
+    ens = Ensemble(states, device=cuda, dtype=dtype)
+    rng = np.random.default_rng(3)
+    pstar = torch.from_numpy(rng.lognormal(np.log(27500.0), 0.3, ens.total)).to(cuda, dtype)
+    for ps in (None, pstar):
+        ens.pre_evp(ps)
+        for m in range(ens.n):
+            st = ens.member_state(m)
+            want = K.pre_evp(st, None if ps is None else ens.view(ps, m))
+            for name, w in zip(("SeaIceMassC", "SeaIceMassU", "SeaIceMassV", "AreaW", "AreaS",
+                                "WindForcingX", "WindForcingY", "SeaIceStrength"), want):
+                got = ens.view(ens.fields[name], m)
+                assert torch.equal(got, w), f"member {m} {name} (P* field: {ps is not None})"
+    b = ens.budgets()
+    for m, st in enumerate(states):
+        vs = ens.member_state(m).variables
+        rA = vs.rA[2:-2, 2:-2].double().cpu().numpy().ravel()
+        for name in ("hIceMean", "hSnowMean", "Area", "Qnet"):
+            x = vs[name][2:-2, 2:-2].double().cpu().numpy().ravel()
+            want = math.fsum(x * rA)
+            scale = math.fsum(np.abs(x * rA))
+            assert abs(b[name][m].item() - want) <= 1e-13 * max(scale, 1e-300), (m, name)

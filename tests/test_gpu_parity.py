@@ -608,3 +608,33 @@ def test_evp_every_tile_shape(cuda, tile):
                 if q >= 2:
                     a, b = a[keep], b[keep]
                 assert_exact(f"{name}[tile={tile}, k={k}, n={n}]", a, b)
+
+
+def test_argument_structs_follow_the_state(cuda):
+    """The drop-ins reuse their ABI structs between calls; every change a
+    caller can make must still reach the kernels: a reassigned variable (a
+    new tensor, possibly at a freed tensor's address), a changed setting, and
+    an in-place change of the metric-term fields (which decides whether they
+    are read at all).  Each call equals the oracle bit for bit."""
+    st = _spun_up(synth.tiny(20, 14), cuda, n_spin=5)
+    g = st.to(cuda)
+    for name, a, b in zip(EVP_NAMES, K.evp_solver(g, n_substeps=9), O.evp_solver(st, n_substeps=9)):
+        assert_exact(f"first call {name}", g2n(a), b)
+    # reassign the ice velocities (new tensors) and the ocean currents
+    r = np.random.default_rng(41)
+    for k in ("uIce", "vIce", "uOcean"):
+        st.variables[k] = st.variables[k] + r.normal(0, 0.01, st.shape) * st.variables.iceMaskU
+        g.variables[k] = torch.from_numpy(st.variables[k]).to(cuda)
+    for name, a, b in zip(EVP_NAMES, K.evp_solver(g, n_substeps=9), O.evp_solver(st, n_substeps=9)):
+        assert_exact(f"after reassignment {name}", g2n(a), b)
+    # a setting
+    st.settings.evpAlpha = g.settings.evpAlpha = 450.0
+    for name, a, b in zip(EVP_NAMES, K.evp_solver(g, n_substeps=9), O.evp_solver(st, n_substeps=9)):
+        assert_exact(f"after a setting {name}", g2n(a), b)
+    # the metric terms, zero so far, changed in place
+    k = r.normal(0, 1e-7, st.shape)
+    for name in ("k1AtC", "k2AtC", "k1AtZ", "k2AtZ"):
+        st.variables[name] = k.copy()
+        g.variables[name].copy_(torch.from_numpy(k))
+    for name, a, b in zip(EVP_NAMES, K.evp_solver(g, n_substeps=9), O.evp_solver(st, n_substeps=9)):
+        assert_exact(f"after metric terms {name}", g2n(a), b)

@@ -311,3 +311,14 @@ def test_fp32_stages_quarter_degree(cuda):
     from veris_amd._native import GROWTH_OUT_NAMES
 
     cmp("growth", GROWTH_OUT_NAMES, K.Growth(s64), K.Growth(s32))
+
+
+def test_batched_adjoint_quarter_degree_members(cuda):
+    """The 0.25-degree batched adjoint (8 regional members of 240 x 240 with
+    10-cell thickness noise, FP32, 120 substeps, checkpoint every 10): members
+    0 and 7 bit for bit against the single-domain adjoint, drag gradients as
+    the sum over the members."""
+    from test_gpu_adjoint import _batched_vs_single
+
+    states = [pre_evp(synth.config4_adjoint_member(m)) for m in range(8)]
+    _batched_vs_single(states, torch.float32, 120, 10, members_to_check=(0, 7))

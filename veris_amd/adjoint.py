@@ -164,3 +164,116 @@ def evp_subcycle(state, *, pStar=None, hIceMean=None, waterIceDrag=None,
     if args[0].dim() not in (0, 2):
         raise ValueError("pStar must be 0-d or a field")
     return _EvpSubcycle.apply(state, n, int(ckpt_every), int(tile), *args)
+
+
+class _EvpSubcycleBatched(torch.autograd.Function):
+    """Forward and reverse of the subcycle of every member of an ensemble of
+    equal members in one launch sequence each (the batched entry points)."""
+
+    @staticmethod
+    def forward(ctx, ens, n, ckpt_every, tile, pstar, h, cwN, cwS, caN, caS):
+        from . import batched as B
+
+        dev, dt, total = ens.device, ens.dtype, ens.total
+        NX, NY = ens.shapes[0]
+        m0 = ens.member_settings[0]
+        dom = N.domain_struct(m0, K._dtype_code(ens.fields["iceMask"]), tile)
+        sett = N.settings_struct(ens.settings)
+        for name, val in zip(_DRAGS, _scalars([cwN, cwS, caN, caS])):
+            setattr(sett, name, val)
+        h = h.detach().to(device=dev, dtype=dt).contiguous()
+        if h.shape != (total,):
+            raise ValueError("hIceMean must be a packed field of the ensemble")
+        pstar_field = pstar.detach().to(device=dev, dtype=dt).expand(total).contiguous()
+        fields = ens._fields_struct()
+        fields.hIceMean = h.data_ptr()
+        pre = torch.empty((len(_PRE_OUT), total), dtype=dt, device=dev)
+        po = N.PreEvpOut(*[t.data_ptr() for t in pre.unbind(0)])
+        stream = torch.cuda.current_stream(dev).cuda_stream
+        b = ens._batch(0)
+        N.call("vh_pre_evp_batched", ctypes.byref(b), ctypes.byref(sett), ctypes.byref(fields),
+               pstar_field.data_ptr(), ctypes.byref(po), stream)
+        for name, t in zip(_PRE_OUT, pre.unbind(0)):
+            setattr(fields, name, t.data_ptr())
+        nbytes = ctypes.c_size_t(0)
+        N.call("vh_evp_adjoint_batched_workspace_bytes", ctypes.byref(dom), ens.n, n, ckpt_every,
+               ctypes.byref(nbytes))
+        ws = torch.empty(max(int(nbytes.value), 256), dtype=torch.uint8, device=dev)
+        out = torch.empty((5, total), dtype=dt, device=dev).unbind(0)
+        N.call("vh_evp_forward_checkpointed_batched", ctypes.byref(dom), ens.n, ctypes.byref(sett),
+               ctypes.byref(fields), n, ckpt_every, *(t.data_ptr() for t in out), ws.data_ptr(),
+               ws.numel(), stream)
+        # every tensor behind a pointer of the ABI structs stays alive until backward
+        ctx.save_for_backward(ws, h, pre, pstar_field,
+                              *[ens.fields[k] for k in B.VARIABLE_NAMES])
+        ctx.abi = (dom, sett, fields)
+        ctx.n, ctx.ckpt, ctx.nm, ctx.total = n, ckpt_every, ens.n, total
+        ctx.meta = [(x.dtype, x.device, x.shape) for x in (pstar, cwN, cwS, caN, caS)]
+        ctx.h_meta = (h.dtype, h.device)
+        return tuple(out)
+
+    @staticmethod
+    def backward(ctx, gu, gv, gs1, gs2, gs12):
+        ws, _h, _pre, pstar_field = ctx.saved_tensors[:4]
+        dom, sett, fields = ctx.abi
+        dev, dt, total = ws.device, pstar_field.dtype, ctx.total
+        stream = torch.cuda.current_stream(dev).cuda_stream
+        grads = [None if g is None else g.to(dt).contiguous() for g in (gu, gv, gs1, gs2, gs12)]
+        ebuf = torch.empty((len(N.EVP_ADJOINT_NAMES), total), dtype=dt, device=dev)
+        evp = dict(zip(N.EVP_ADJOINT_NAMES, ebuf.unbind(0)))
+        eo = N.EvpAdjointOut(**{"adj_" + k: v.data_ptr() for k, v in evp.items()})
+        N.call("vh_evp_adjoint_batched", ctypes.byref(dom), ctx.nm, ctypes.byref(sett),
+               ctypes.byref(fields), ctx.n, ctx.ckpt,
+               *(None if g is None else g.data_ptr() for g in grads), ctypes.byref(eo),
+               ws.data_ptr(), ws.numel(), stream)
+        pbuf = torch.empty((len(N.PRE_EVP_ADJOINT_NAMES), total), dtype=dt, device=dev)
+        pre = dict(zip(N.PRE_EVP_ADJOINT_NAMES, pbuf.unbind(0)))
+        po = N.PreEvpAdjointOut(**{"adj_" + k: v.data_ptr() for k, v in pre.items()})
+        N.call("vh_pre_evp_adjoint_batched", ctypes.byref(dom), ctx.nm, ctypes.byref(sett),
+               ctypes.byref(fields), pstar_field.data_ptr(), ctypes.byref(eo), ctypes.byref(po),
+               stream)
+        need = ctx.needs_input_grad  # (ens, n, ckpt, tile, pstar, h, cwN, cwS, caN, caS)
+        sums = torch.stack([pre["pStar"], evp["waterDrag_north"], evp["waterDrag_south"],
+                            pre["airDrag_north"], pre["airDrag_south"]]).double().sum(dim=1)
+        res = []
+        for q, (x_dt, x_dev, x_shape) in enumerate(ctx.meta):
+            if not need[4 if q == 0 else q + 5]:
+                res.append(None)
+            elif q == 0 and len(x_shape) == 1:
+                res.append(pre["pStar"].to(device=x_dev, dtype=x_dt))
+            else:
+                res.append(sums[q].reshape(x_shape).to(device=x_dev, dtype=x_dt))
+        g_h = pre["hIceMean"].to(device=ctx.h_meta[1], dtype=ctx.h_meta[0]) if need[5] else None
+        return (None, None, None, None, res[0], g_h, res[1], res[2], res[3], res[4])
+
+
+def evp_subcycle_batched(ens, *, pStar=None, hIceMean=None, waterIceDrag=None,
+                         waterIceDrag_south=None, airIceDrag=None, airIceDrag_south=None,
+                         n_substeps: int | None = None, ckpt_every: int = 10, tile: int = 0):
+    """:func:`evp_subcycle` of every member of an ensemble of equal members
+    (a :class:`~veris_amd.batched.Ensemble` whose members share one padded
+    shape, cyclicity and settings) -> packed (uIce, vIce, sigma1, sigma2,
+    sigma12), each of the ensemble's packed length.
+
+    Forward and reverse each run all members in one launch sequence; member
+    by member the results and gradients equal :func:`evp_subcycle`'s bit for
+    bit.  ``pStar`` is 0-d (shared) or a packed per-cell field, ``hIceMean``
+    a packed field (default: the ensemble's), the four drag coefficients are
+    0-d and shared by all members (their gradients sum over the members)."""
+    if len(set(ens.shapes)) != 1 or len(set(ens.cyclic)) != 1:
+        raise ValueError("evp_subcycle_batched needs members of one shape and cyclicity")
+    s = ens.settings
+    n = int(s.nEVPsteps if n_substeps is None else n_substeps)
+    if n < 1 or ckpt_every < 1:
+        raise ValueError("n_substeps and ckpt_every must be >= 1")
+    args = (
+        _as_tensor(s.pStar if pStar is None else pStar),
+        ens.fields["hIceMean"] if hIceMean is None else hIceMean,
+        _as_tensor(s.waterIceDrag if waterIceDrag is None else waterIceDrag),
+        _as_tensor(s.waterIceDrag_south if waterIceDrag_south is None else waterIceDrag_south),
+        _as_tensor(s.airIceDrag if airIceDrag is None else airIceDrag),
+        _as_tensor(s.airIceDrag_south if airIceDrag_south is None else airIceDrag_south),
+    )
+    if args[0].dim() not in (0, 1):
+        raise ValueError("pStar must be 0-d or a packed field")
+    return _EvpSubcycleBatched.apply(ens, n, int(ckpt_every), int(tile), *args)

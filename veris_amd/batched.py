@@ -95,18 +95,18 @@ class Ensemble:
 
     # -- kernels ------------------------------------------------------------
 
-    def pre_evp(self):
-        """SeaIceMass, AreaWS, WindForcingXY, SeaIceStrength of every member,
-        written into the packed arrays (single-domain kernels on the views)."""
-        for m in range(self.n):
-            st = self.member_state(m)
-            vs = st.variables
-            vs.SeaIceMassC, vs.SeaIceMassU, vs.SeaIceMassV = K.SeaIceMass(st)
-            vs.AreaW, vs.AreaS = K.AreaWS(st)
-            vs.WindForcingX, vs.WindForcingY = K.WindForcingXY(st)
-            vs.SeaIceStrength = K.SeaIceStrength(st)
-            for name in _PRE_EVP:
-                self.view(self.fields[name], m).copy_(vs[name])
+    def pre_evp(self, pStar: torch.Tensor | None = None):
+        """SeaIceMass, AreaWS, WindForcingXY and SeaIceStrength of every member
+        (veris/area_mass.py:5-28, veris/dynsolver.py:7-84,
+        veris/dynamics_routines.py:5-18) in one batched launch, written into
+        the packed arrays; ``pStar`` optionally a packed per-cell P* field."""
+        if pStar is not None and (pStar.shape != (self.total,) or pStar.dtype != self.dtype):
+            raise ValueError("pStar must be a packed field of the ensemble's dtype")
+        b = self._batch(0)
+        out = N.PreEvpOut(*[self.fields[name].data_ptr() for name in _PRE_EVP])
+        N.call("vh_pre_evp_batched", ctypes.byref(b), ctypes.byref(N.settings_struct(self.settings)),
+               ctypes.byref(self._fields_struct()), None if pStar is None else pStar.data_ptr(),
+               ctypes.byref(out), torch.cuda.current_stream(self.device).cuda_stream)
 
     def _batch(self, tile):
         return N.Batch(n_members=self.n, dtype=N.VH_F64 if self.dtype == torch.float64 else N.VH_F32,
@@ -143,14 +143,17 @@ class Ensemble:
     def interior_cells(self) -> int:
         return int(sum((a - 4) * (b - 4) for a, b in self.shapes))
 
-    def budgets(self, names=("hIceMean", "hSnowMean", "Area")):
-        """FP64 device sums of rA * field over each member's interior:
-        returns {name: float64 tensor of shape (n_members,)}."""
-        res = {}
-        for name in names:
-            vals = []
-            for m in range(self.n):
-                st = self.member_state(m)
-                vals.append(K.global_sum(st, st.variables[name], st.variables.rA))
-            res[name] = torch.stack(vals)
-        return res
+    def budgets(self, names=("hIceMean", "hSnowMean", "Area", "Qnet")):
+        """FP64 budgets of every member in one launch: the sums of rA * field
+        over each member's interior (ice and snow volume, ice area, and the
+        heat budget of the net heat flux Qnet), accumulated in FP64 whatever
+        the storage type; returns {name: float64 tensor of shape (n_members,)}."""
+        if not 1 <= len(names) <= 8:
+            raise ValueError("1 to 8 budget fields per call")
+        res = torch.empty((len(names), self.n), dtype=torch.float64, device=self.device)
+        ptrs = (ctypes.c_void_p * len(names))(*[self.fields[n].data_ptr() for n in names])
+        b = self._batch(0)
+        N.call("vh_global_sums_batched", ctypes.byref(b), len(names), ptrs,
+               self.fields["rA"].data_ptr(), res.data_ptr(),
+               torch.cuda.current_stream(self.device).cuda_stream)
+        return dict(zip(names, res.unbind(0)))

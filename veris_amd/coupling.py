@@ -22,7 +22,7 @@ def ocean_to_ice(state, u_surface, v_surface, temp_surface, salt_surface, forc_t
     degrees Celsius; ``forc_temp_surface`` is the ocean's surface heat
     forcing.  ``Qsw`` and ``wSpeed`` come from ``state.variables.SWdown`` and
     the wind (taken before its overlap fill, as in the reference)."""
-    c = Ctx(state)
+    c = Ctx.of(state)
     for name, t in (("u_surface", u_surface), ("v_surface", v_surface),
                     ("temp_surface", temp_surface), ("salt_surface", salt_surface),
                     ("forc_temp_surface", forc_temp_surface)):
@@ -38,7 +38,7 @@ def ice_to_ocean(state, surface_taux, surface_tauy, forc_salt_surface_ice, forc_
     """-> (surface_taux, surface_tauy, forc_temp_surface, forc_salt_surface):
     the wind stress blended with OceanStressU/V by AreaW/AreaS, the heat
     forcing from Qnet and the salt forcing (ice flux + restoring)."""
-    c = Ctx(state)
+    c = Ctx.of(state)
     for name, t in (("surface_taux", surface_taux), ("surface_tauy", surface_tauy),
                     ("forc_salt_surface_ice", forc_salt_surface_ice),
                     ("forc_salt_surface_res", forc_salt_surface_res)):

@@ -103,6 +103,9 @@ int evp_run_core(const Geom& g, const vh_settings* s, const vh_fields* fl, const
                  hipStream_t st, double* res = nullptr, bool direct = false) {
   const size_t fb = align256(field_elems(g) * sizeof(T));
   const size_t need = 5 * fb + align256(sizeof(EvpArgBlock<T>));
+  // the substep kernel addresses the fields with 32-bit byte offsets
+  VH_REQUIRE(field_elems(g) * sizeof(T) < ((size_t)1 << 32), VH_ESHAPE,
+             "domain too large: %d x %d cells exceed 4 GiB per field", g.nx, g.ny);
   VH_REQUIRE(ws != nullptr && ws_bytes >= need, VH_EWORKSPACE,
              "evp workspace: need %zu bytes, got %zu", need, ws_bytes);
   // packed constants when the workspace has room for them (its full size,

@@ -135,19 +135,21 @@ __global__ void adj_write_blocks_kernel(EvpArgBlock<T>* blocks, size_t stride, E
 // copy of the final state (the forward's last outputs) into checkpoint slot nck
 template <typename T>
 __global__ __launch_bounds__(BLK) void adj_keep_final_kernel(const EvpArgBlock<T>* __restrict__ last,
-                                                             Geom g, T* dst, size_t fb) {
+                                                             int n_all, T* dst, size_t fb) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= g.NX * g.NY) return;
+  if (c >= n_all) return;
 #pragma unroll
   for (int q = 0; q < 5; ++q)
     stg((T*)((char*)dst + (size_t)q * fb), (unsigned)c, ldg((const T*)last->out[q], (unsigned)c));
 }
 
 // ---- reverse sweep: set-up and result copy
+// pointwise over every cell of every member
 template <typename T>
 __global__ __launch_bounds__(BLK) void adj_init_kernel(const AdjArgs<T>* __restrict__ a,
-                                                       LamSet<T> L) {
-  CELL_SETUP
+                                                       LamSet<T> L, int n_all) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n_all) return;
   const AdjAcc<T>& acc = a->acc;
   T* const accp[kAdjAccFields] = {acc.P, acc.mC, acc.mU, acc.mV, acc.AW, acc.AS,
                                   acc.WFX, acc.WFY, acc.u0, acc.v0, acc.DN, acc.DS};
@@ -160,8 +162,9 @@ __global__ __launch_bounds__(BLK) void adj_init_kernel(const AdjArgs<T>* __restr
 
 template <typename T>
 __global__ __launch_bounds__(BLK) void adj_finish_kernel(const AdjArgs<T>* __restrict__ a,
-                                                         LamSet<T> L) {
-  CELL_SETUP
+                                                         LamSet<T> L, int n_all) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n_all) return;
 #pragma unroll
   for (int q = 0; q < 5; ++q) a->outs[q][c] = L.in[q][c];
 }
@@ -236,9 +239,11 @@ __global__ __launch_bounds__(RI* RJ) void adj_substep_kernel(const AdjArgs<T>* _
   const int i = i0 + ar - R, j = j0 + br - R;
   const int gi = wrap(i, g.NX), gj = wrap(j, g.NY);
   const int gip = wp(gi, g.NX), gjp = wp(gj, g.NY), gim = wm(gi, g.NX), gjm = wm(gj, g.NY);
-  const unsigned c = gi * g.NY + gj;
-  const unsigned cip = gip * g.NY + gj, cjp = gi * g.NY + gjp;
-  const unsigned cim = gim * g.NY + gj, cjm = gi * g.NY + gjm;
+  // equal members packed back to back: member blockIdx.y starts at element mo
+  const unsigned mo = blockIdx.y * (unsigned)(g.NX * g.NY);
+  const unsigned c = mo + gi * g.NY + gj;
+  const unsigned cip = mo + gip * g.NY + gj, cjp = mo + gi * g.NY + gjp;
+  const unsigned cim = mo + gim * g.NY + gj, cjm = mo + gi * g.NY + gjm;
   const bool own = (ar >= R) && (ar < R + TI) && (br >= R) && (br < R + TJ) && (i < g.NX) &&
                    (j < g.NY);
   const bool comp = is_computed(g, gi, gj);
@@ -258,7 +263,7 @@ __global__ __launch_bounds__(RI* RJ) void adj_substep_kernel(const AdjArgs<T>* _
   const T rrAu = ldg(k.rrAu, c), rrAv = ldg(k.rrAv, c), mC = ldg(k.massC, c);
   const T dyV = ldg(k.dyV, c), dxU = ldg(k.dxU, c), dxV = ldg(k.dxV, c), dyU = ldg(k.dyU, c);
   const T l1 = ldg(L.in[2], c), l2 = ldg(L.in[3], c), l12 = ldg(L.in[4], c);
-  const T lu = gather_filled(g, L.in[0], gi, gj), lv = gather_filled(g, L.in[1], gi, gj);
+  const T lu = gather_filled(g, L.in[0] + mo, gi, gj), lv = gather_filled(g, L.in[1] + mo, gi, gj);
   // water drag factor of either hemisphere, selected by value (a select of
   // the two setting addresses would become a per-lane load)
   const T Dn = (T)sc.cwN * (T)sc.rhoSea, Ds = (T)sc.cwS * (T)sc.rhoSea;
@@ -470,22 +475,22 @@ constexpr int kAdjNumTiles = sizeof(kAdjTiles) / sizeof(kAdjTiles[0]);
 
 template <typename T, int RI, int RJ>
 void adj_launch_shape(const AdjArgs<T>* a, const EvpArgBlock<T>* b, LamSet<T> L, const Geom& g,
-                      hipStream_t st) {
+                      int n_members, hipStream_t st) {
   constexpr int TI = RI - 8, TJ = RJ - 8;
   const int tiles_j = (g.NY + TJ - 1) / TJ;
   const int n_tiles = ((g.NX + TI - 1) / TI) * tiles_j;
   const int per_xcd = (n_tiles + 7) / 8;
-  hipLaunchKernelGGL((adj_substep_kernel<T, double, RI, RJ>), dim3(per_xcd * 8), dim3(RI * RJ), 0,
-                     st, a, b, L, n_tiles, tiles_j, per_xcd);
+  hipLaunchKernelGGL((adj_substep_kernel<T, double, RI, RJ>), dim3(per_xcd * 8, n_members),
+                     dim3(RI * RJ), 0, st, a, b, L, n_tiles, tiles_j, per_xcd);
 }
 
 template <typename T>
 void adj_launch(int cfg, const AdjArgs<T>* a, const EvpArgBlock<T>* b, LamSet<T> L, const Geom& g,
-                hipStream_t st) {
+                int n_members, hipStream_t st) {
   switch (cfg) {
-    case 1: adj_launch_shape<T, 24, 32>(a, b, L, g, st); break;
-    case 2: adj_launch_shape<T, 12, 64>(a, b, L, g, st); break;
-    default: adj_launch_shape<T, 16, 32>(a, b, L, g, st); break;
+    case 1: adj_launch_shape<T, 24, 32>(a, b, L, g, n_members, st); break;
+    case 2: adj_launch_shape<T, 12, 64>(a, b, L, g, n_members, st); break;
+    default: adj_launch_shape<T, 16, 32>(a, b, L, g, n_members, st); break;
   }
 }
 
@@ -504,14 +509,14 @@ struct PreAdjArgs {
 
 // d tau_c / d Ca at c-point (i, j) (veris/dynsolver.py:7-47)
 template <typename T>
-__device__ inline void tau_c_dca(const Geom& g, const PreAdjArgs<T>& a, int i, int j, T* dtx,
-                                 T* dty) {
+__device__ inline void tau_c_dca(const Geom& g, const PreAdjArgs<T>& a, size_t mo, int i, int j,
+                                 T* dtx, T* dty) {
   const int ip = wp(i, g.NX), jp = wp(j, g.NY);
-  const int c = AT(i, j);
+  const size_t c = mo + AT(i, j);
   T urel = a.uW[c], vrel = a.vW[c];
   if (a.relWind) {
-    urel = urel - (T)0.5 * (a.u[c] + a.u[AT(ip, j)]);
-    vrel = vrel - (T)0.5 * (a.v[c] + a.v[AT(i, jp)]);
+    urel = urel - (T)0.5 * (a.u[c] + a.u[mo + AT(ip, j)]);
+    vrel = vrel - (T)0.5 * (a.v[c] + a.v[mo + AT(i, jp)]);
   }
   const T wsq = urel * urel + vrel * vrel;
   const T ws = wsq < a.wmin2 ? a.wmin : sqrt(wsq);
@@ -523,34 +528,36 @@ __device__ inline void tau_c_dca(const Geom& g, const PreAdjArgs<T>& a, int i, i
 template <typename T>
 __global__ __launch_bounds__(BLK) void pre_evp_adjoint_kernel(Geom g, PreAdjArgs<T> a) {
   CELL_SETUP_G
+  // equal members packed back to back: member blockIdx.y starts at element mo
+  const size_t mo = (size_t)blockIdx.y * n_;
   // strength: P = P* h exp(-C (1 - A)) M
-  const T E = exp((-a.cStar) * ((T)1 - a.A[c]));
-  const T ps = a.pstarF ? a.pstarF[c] : a.pstar;
-  a.apstar[c] = a.aP[c] * a.h[c] * E * a.M[c];
-  T hb = a.aP[c] * ps * E * a.M[c];
+  const T E = exp((-a.cStar) * ((T)1 - a.A[mo + c]));
+  const T ps = a.pstarF ? a.pstarF[mo + c] : a.pstar;
+  a.apstar[mo + c] = a.aP[mo + c] * a.h[mo + c] * E * a.M[mo + c];
+  T hb = a.aP[mo + c] * ps * E * a.M[mo + c];
   // wind forcing: the tilt term -massU rdxC (phi - phi[i-1]) adds to the mass adjoints
   auto phi_at = [&](int ii, int jj) {
-    return a.grav * a.ssh[AT(ii, jj)] + a.sp[AT(ii, jj)] * a.rrho;
+    return a.grav * a.ssh[mo + AT(ii, jj)] + a.sp[mo + AT(ii, jj)] * a.rrho;
   };
   const T phi = phi_at(i, j);
-  const T amU_c = a.amU[c] - a.aWFX[c] * a.rdxC[c] * (phi - phi_at(im, j));
-  const T amU_ip = a.amU[AT(ip, j)] - a.aWFX[AT(ip, j)] * a.rdxC[AT(ip, j)] * (phi_at(ip, j) - phi);
-  const T amV_c = a.amV[c] - a.aWFY[c] * a.rdyC[c] * (phi - phi_at(i, jm));
-  const T amV_jp = a.amV[AT(i, jp)] - a.aWFY[AT(i, jp)] * a.rdyC[AT(i, jp)] * (phi_at(i, jp) - phi);
+  const T amU_c = a.amU[mo + c] - a.aWFX[mo + c] * a.rdxC[mo + c] * (phi - phi_at(im, j));
+  const T amU_ip = a.amU[mo + AT(ip, j)] - a.aWFX[mo + AT(ip, j)] * a.rdxC[mo + AT(ip, j)] * (phi_at(ip, j) - phi);
+  const T amV_c = a.amV[mo + c] - a.aWFY[mo + c] * a.rdyC[mo + c] * (phi - phi_at(i, jm));
+  const T amV_jp = a.amV[mo + AT(i, jp)] - a.aWFY[mo + AT(i, jp)] * a.rdyC[mo + AT(i, jp)] * (phi_at(i, jp) - phi);
   // mass: massC = rhoI h + rhoS hs, massU = (massC + massC[i-1])/2, massV likewise in j
-  const T amC = a.amC[c] + (T)0.5 * (amU_c + amU_ip) + (T)0.5 * (amV_c + amV_jp);
-  a.ah[c] = hb + a.rhoI * amC;
-  a.ahs[c] = a.rhoS * amC;
+  const T amC = a.amC[mo + c] + (T)0.5 * (amU_c + amU_ip) + (T)0.5 * (amV_c + amV_jp);
+  a.ah[mo + c] = hb + a.rhoI * amC;
+  a.ahs[mo + c] = a.rhoS * amC;
   // air drag: WindForcingX = 0.5 (tau_c + tau_c[i-1]) MU AreaW + tilt, tau_c linear in Ca
   T dtx, dty;
-  tau_c_dca(g, a, i, j, &dtx, &dty);
-  const T gx = (T)0.5 * (a.aWFX[c] * a.AW[c] * a.MU[c] +
-                         a.aWFX[AT(ip, j)] * a.AW[AT(ip, j)] * a.MU[AT(ip, j)]);
-  const T gy = (T)0.5 * (a.aWFY[c] * a.AS[c] * a.MV[c] +
-                         a.aWFY[AT(i, jp)] * a.AS[AT(i, jp)] * a.MV[AT(i, jp)]);
+  tau_c_dca(g, a, mo, i, j, &dtx, &dty);
+  const T gx = (T)0.5 * (a.aWFX[mo + c] * a.AW[mo + c] * a.MU[mo + c] +
+                         a.aWFX[mo + AT(ip, j)] * a.AW[mo + AT(ip, j)] * a.MU[mo + AT(ip, j)]);
+  const T gy = (T)0.5 * (a.aWFY[mo + c] * a.AS[mo + c] * a.MV[mo + c] +
+                         a.aWFY[mo + AT(i, jp)] * a.AS[mo + AT(i, jp)] * a.MV[mo + AT(i, jp)]);
   const T contrib = gx * dtx + gy * dty;
-  a.aCaN[c] = a.fC[c] < (T)0 ? (T)0 : contrib;
-  a.aCaS[c] = a.fC[c] < (T)0 ? contrib : (T)0;
+  a.aCaN[mo + c] = a.fC[mo + c] < (T)0 ? (T)0 : contrib;
+  a.aCaS[mo + c] = a.fC[mo + c] < (T)0 ? contrib : (T)0;
 }
 
 // ---- host side
@@ -560,10 +567,12 @@ struct AdjLayout {
   size_t off_args, off_fblk, off_rblk, off_ckpt, off_pp, off_seg, off_lam, off_pblk, off_pack;
 };
 
+// Workspace of a subcycle of n substeps with a checkpoint every ce, for nm
+// equal members packed back to back (1 = one domain).
 template <typename T>
-AdjLayout adj_layout(const Geom& g, int n, int ce) {
+AdjLayout adj_layout(const Geom& g, int n, int ce, int nm = 1) {
   AdjLayout L;
-  L.fb = align256(field_elems(g) * sizeof(T));
+  L.fb = align256(field_elems(g) * (size_t)nm * sizeof(T));
   L.blk = align256(sizeof(EvpArgBlock<T>));
   L.ce = ce;
   L.n = n;
@@ -582,7 +591,7 @@ AdjLayout adj_layout(const Geom& g, int n, int ce) {
   // forward and recompute launches
   L.off_pblk = L.off_lam + (size_t)10 * L.fb;
   L.off_pack = L.off_pblk + L.blk;
-  L.total = L.off_pack + evp_pack_bytes<T>(field_elems(g), g.NY);
+  L.total = L.off_pack + evp_pack_bytes<T>(field_elems(g) * (size_t)nm, (size_t)g.NY * nm);
   return L;
 }
 
@@ -662,6 +671,7 @@ EvpArgBlock<T> proto_block(const Geom& g, const vh_settings* s, const vh_fields*
   ab.g = g;
   ab.nlaunch = 1;
   ab.members = nullptr;
+  ab.mstride = (long long)field_elems(g);  // member blockIdx.y of equal packed members
   return ab;
 }
 
@@ -671,12 +681,12 @@ EvpArgBlock<T> proto_block(const Geom& g, const vh_settings* s, const vh_fields*
 template <typename T>
 EvpArgBlock<T> packed_proto(const Geom& g, const vh_settings* s, const vh_fields* f,
                             const AdjLayout& L, void* ws, const EvpArgBlock<T>** pack_status,
-                            hipStream_t st) {
+                            int nm, hipStream_t st) {
   EvpArgBlock<T> proto = proto_block<T>(g, s, f);
-  evp_pack_attach(&proto, (char*)ws + L.off_pack, field_elems(g), g.NY);
+  evp_pack_attach(&proto, (char*)ws + L.off_pack, field_elems(g) * (size_t)nm, g.NY * nm);
   EvpArgBlock<T>* pblk = (EvpArgBlock<T>*)((char*)ws + L.off_pblk);
   hipLaunchKernelGGL(evp_write_args_kernel<T>, dim3(1), dim3(64), 0, st, pblk, proto);
-  evp_pack_launch(pblk, (long long)field_elems(g), 1, st);
+  evp_pack_launch(pblk, (long long)field_elems(g), nm, st);
   *pack_status = pblk;
   return proto;
 }
@@ -708,12 +718,12 @@ void tiles_of(int cfg, const Geom& g, int* n_tiles, int* tiles_j) {
 }
 
 template <typename T>
-int run_recorded(const char* kind, const Geom& g, const AdjLayout& L, int cfg, void* ws,
+int run_recorded(const char* kind, const Geom& g, const AdjLayout& L, int cfg, int nm, void* ws,
                  const std::function<void(hipStream_t)>& record, hipStream_t st) {
   if (graphs_enabled()) {
     char key[192];
-    snprintf(key, sizeof(key), "%s/%d/%d/%d/%d/%d/%d/%d/%d/%d/%p", kind, (int)sizeof(T), cfg, L.n,
-             L.ce, g.NX, g.NY, g.nx, g.fill_x, g.fill_y, ws);
+    snprintf(key, sizeof(key), "%s/%d/%d/%d/%d/%d/%d/%d/%d/%d/%d/%p", kind, (int)sizeof(T), cfg,
+             L.n, L.ce, nm, g.NX, g.NY, g.nx, g.fill_x, g.fill_y, ws);
     return graph_run(key, record, st);
   }
   record(st);
@@ -723,8 +733,8 @@ int run_recorded(const char* kind, const Geom& g, const AdjLayout& L, int cfg, v
 
 template <typename T>
 int forward_ckpt(const Geom& g, const vh_domain* d, const vh_settings* s, const vh_fields* f,
-                 int n, int ce, void* const outs[5], void* ws, size_t wsb, hipStream_t st) {
-  const AdjLayout L = adj_layout<T>(g, n, ce);
+                 int n, int ce, int nm, void* const outs[5], void* ws, size_t wsb, hipStream_t st) {
+  const AdjLayout L = adj_layout<T>(g, n, ce, nm);
   VH_REQUIRE(ws && wsb >= L.total, VH_EWORKSPACE, "adjoint workspace: need %zu bytes, got %zu",
              L.total, wsb);
   const void* req[kEvpMaxRequired];
@@ -741,29 +751,31 @@ int forward_ckpt(const Geom& g, const vh_domain* d, const vh_settings* s, const 
   EvpArgBlock<T>* fblk = (EvpArgBlock<T>*)((char*)ws + L.off_fblk);
   AdjArgs<T> unused{};
   const EvpArgBlock<T>* pack_status = nullptr;
-  const EvpArgBlock<T> proto = packed_proto<T>(g, s, f, L, ws, &pack_status, st);
+  const EvpArgBlock<T> proto = packed_proto<T>(g, s, f, L, ws, &pack_status, nm, st);
   hipLaunchKernelGGL(adj_write_blocks_kernel<T>, dim3((n + 63) / 64), dim3(64), 0, st, fblk, L.blk,
                      proto, block_plan<T>(L, f, outs, ws), 0, (AdjArgs<T>*)nullptr, unused,
                      pack_status);
   VH_TRY_HIP(hipGetLastError());
   T* final_slot = wsf<T>(ws, L.off_ckpt, L.fb, 5 * L.nck);
+  const int n_all = (int)(field_elems(g) * (size_t)nm);
   auto record = [&](hipStream_t cs) {
     for (int kk = 0; kk < n; ++kk)
       evp_launch_single<T, double>(
-          cfg, (const EvpArgBlock<T>*)((char*)fblk + (size_t)kk * L.blk), n_tiles, tiles_j, cs);
-    hipLaunchKernelGGL(adj_keep_final_kernel<T>, dim3((unsigned)((field_elems(g) + BLK - 1) / BLK)),
+          cfg, (const EvpArgBlock<T>*)((char*)fblk + (size_t)kk * L.blk), n_tiles, tiles_j, cs,
+          nm);
+    hipLaunchKernelGGL(adj_keep_final_kernel<T>, dim3((unsigned)((n_all + BLK - 1) / BLK)),
                        dim3(BLK), 0, cs,
-                       (const EvpArgBlock<T>*)((char*)fblk + (size_t)(n - 1) * L.blk), g,
+                       (const EvpArgBlock<T>*)((char*)fblk + (size_t)(n - 1) * L.blk), n_all,
                        final_slot, L.fb);
   };
-  return run_recorded<T>("adjf", g, L, cfg, ws, record, st);
+  return run_recorded<T>("adjf", g, L, cfg, nm, ws, record, st);
 }
 
 template <typename T>
 int adjoint(const Geom& g, const vh_domain* d, const vh_settings* s, const vh_fields* f, int n,
-            int ce, const void* const lam_final[5], const vh_evp_adjoint_out* o, void* ws,
+            int ce, int nm, const void* const lam_final[5], const vh_evp_adjoint_out* o, void* ws,
             size_t wsb, hipStream_t st) {
-  const AdjLayout L = adj_layout<T>(g, n, ce);
+  const AdjLayout L = adj_layout<T>(g, n, ce, nm);
   VH_REQUIRE(ws && wsb >= L.total, VH_EWORKSPACE, "adjoint workspace: need %zu bytes, got %zu",
              L.total, wsb);
   AdjArgs<T> av{};
@@ -792,7 +804,7 @@ int adjoint(const Geom& g, const vh_domain* d, const vh_settings* s, const vh_fi
   AdjArgs<T>* a_dev = (AdjArgs<T>*)((char*)ws + L.off_args);
   EvpArgBlock<T>* rblk = (EvpArgBlock<T>*)((char*)ws + L.off_rblk);
   const EvpArgBlock<T>* pack_status = nullptr;
-  const EvpArgBlock<T> proto = packed_proto<T>(g, s, f, L, ws, &pack_status, st);
+  const EvpArgBlock<T> proto = packed_proto<T>(g, s, f, L, ws, &pack_status, nm, st);
   hipLaunchKernelGGL(adj_write_blocks_kernel<T>, dim3((n + 63) / 64), dim3(64), 0, st, rblk, L.blk,
                      proto, block_plan<T>(L, f, nullptr, ws), 1, a_dev, av, pack_status);
   VH_TRY_HIP(hipGetLastError());
@@ -806,7 +818,8 @@ int adjoint(const Geom& g, const vh_domain* d, const vh_settings* s, const vh_fi
     lam[0][q] = wsf<T>(ws, L.off_lam, L.fb, q);
     lam[1][q] = wsf<T>(ws, L.off_lam, L.fb, 5 + q);
   }
-  const dim3 grid((unsigned)((field_elems(g) + BLK - 1) / BLK));
+  const int n_all = (int)(field_elems(g) * (size_t)nm);
+  const dim3 grid((unsigned)((n_all + BLK - 1) / BLK));
   auto sets = [&](int cur) {
     LamSet<T> ls;
     for (int q = 0; q < 5; ++q) {
@@ -823,7 +836,7 @@ int adjoint(const Geom& g, const vh_domain* d, const vh_settings* s, const vh_fi
     for (int jj = 0; jj + 1 < len; ++jj)
       evp_launch_single<T, double>(
           cfg, (const EvpArgBlock<T>*)((char*)rblk + (size_t)(k0 + jj) * L.blk), n_tiles, tiles_j,
-          cs);
+          cs, nm);
   };
   // Segment sgm - 1 is recomputed on a second stream while segment sgm is
   // reversed (they use different slot sets); the fork follows the reverse of
@@ -835,7 +848,7 @@ int adjoint(const Geom& g, const vh_domain* d, const vh_settings* s, const vh_fi
       init.in[q] = nullptr;
       init.out[q] = lam[cur][q];
     }
-    hipLaunchKernelGGL(adj_init_kernel<T>, grid, dim3(BLK), 0, cs, a_dev, init);
+    hipLaunchKernelGGL(adj_init_kernel<T>, grid, dim3(BLK), 0, cs, a_dev, init, n_all);
     recompute(L.nck - 1, cs);
     for (int sgm = L.nck - 1; sgm >= 0; --sgm) {
       if (sgm >= 1) {
@@ -847,19 +860,20 @@ int adjoint(const Geom& g, const vh_domain* d, const vh_settings* s, const vh_fi
       const int k0 = sgm * ce, len = std::min(ce, n - k0);
       for (int jj = len - 1; jj >= 0; --jj) {
         const EvpArgBlock<T>* b = (const EvpArgBlock<T>*)((char*)rblk + (size_t)(k0 + jj) * L.blk);
-        adj_launch<T>(acfg, a_dev, b, sets(cur), g, cs);
+        adj_launch<T>(acfg, a_dev, b, sets(cur), g, nm, cs);
         cur = 1 - cur;
       }
       if (sgm >= 1) hipStreamWaitEvent(cs, side.join, 0);
     }
-    hipLaunchKernelGGL(adj_finish_kernel<T>, grid, dim3(BLK), 0, cs, a_dev, sets(cur));
+    hipLaunchKernelGGL(adj_finish_kernel<T>, grid, dim3(BLK), 0, cs, a_dev, sets(cur), n_all);
   };
-  return run_recorded<T>("adjr", g, L, cfg * 16 + acfg, ws, record, st);
+  return run_recorded<T>("adjr", g, L, cfg * 16 + acfg, nm, ws, record, st);
 }
 
 template <typename T>
 int pre_evp_adjoint(const Geom& g, const vh_settings* s, const vh_fields* f, const void* pstarF,
-                    const vh_evp_adjoint_out* in, const vh_pre_evp_adjoint_out* o, hipStream_t st) {
+                    const vh_evp_adjoint_out* in, const vh_pre_evp_adjoint_out* o, int nm,
+                    hipStream_t st) {
   PreAdjArgs<T> a;
   a.h = evp_ptr<T>(f->hIceMean);
   a.A = evp_ptr<T>(f->Area);
@@ -909,7 +923,8 @@ int pre_evp_adjoint(const Geom& g, const vh_settings* s, const vh_fields* f, con
   a.cosWin = (T)cos(ang);
   a.relWind = s->useRelativeWind != 0.0;
   const int n = g.NX * g.NY;
-  hipLaunchKernelGGL(pre_evp_adjoint_kernel<T>, dim3((n + BLK - 1) / BLK), dim3(BLK), 0, st, g, a);
+  hipLaunchKernelGGL(pre_evp_adjoint_kernel<T>, dim3((n + BLK - 1) / BLK, nm), dim3(BLK), 0, st, g,
+                     a);
   VH_TRY_HIP(hipGetLastError());
   return VH_OK;
 }
@@ -921,15 +936,76 @@ int pre_evp_adjoint(const Geom& g, const vh_settings* s, const vh_fields* f, con
 }  // namespace
 }  // namespace vh
 
+namespace vh {
+namespace {
+
+// Geometry of one member and the member count of an equal-member batch
+// (every field holds n_members padded boxes back to back).
+int batch_geom(const vh_domain* d, int nm, Geom* g) {
+  int rc = make_geom(d, g);
+  if (rc) return rc;
+  VH_REQUIRE(nm >= 1 && nm <= 65535, VH_EINVAL, "n_members=%d", nm);
+  // the kernels address every packed field with 32-bit byte offsets
+  VH_REQUIRE(field_elems(*g) * (size_t)nm * (d->dtype == VH_F64 ? 8 : 4) < ((size_t)1 << 32),
+             VH_ESHAPE, "%d members of %d x %d cells exceed 4 GiB per packed field", nm, g->nx,
+             g->ny);
+  return VH_OK;
+}
+
+int ws_bytes(const vh_domain* d, int nm, int n, int ce, size_t* bytes) {
+  Geom g;
+  int rc = batch_geom(d, nm, &g);
+  if (rc) return rc;
+  VH_REQUIRE(bytes && n >= 1 && ce >= 1, VH_EINVAL, "bad arguments");
+  *bytes = d->dtype == VH_F64 ? adj_layout<double>(g, n, ce, nm).total
+                              : adj_layout<float>(g, n, ce, nm).total;
+  return VH_OK;
+}
+
+int fwd(const vh_domain* d, int nm, const vh_settings* s, const vh_fields* f, int n, int ce,
+        void* const outs[5], void* ws, size_t wsb, void* stream) {
+  Geom g;
+  int rc = batch_geom(d, nm, &g);
+  if (rc) return rc;
+  rc = check_adj(d, s, f, n, ce);
+  if (rc) return rc;
+  hipStream_t st = (hipStream_t)stream;
+  if (d->dtype == VH_F64) return forward_ckpt<double>(g, d, s, f, n, ce, nm, outs, ws, wsb, st);
+  return forward_ckpt<float>(g, d, s, f, n, ce, nm, outs, ws, wsb, st);
+}
+
+int rev(const vh_domain* d, int nm, const vh_settings* s, const vh_fields* f, int n, int ce,
+        const void* const lam[5], const vh_evp_adjoint_out* out, void* ws, size_t wsb,
+        void* stream) {
+  Geom g;
+  int rc = batch_geom(d, nm, &g);
+  if (rc) return rc;
+  rc = check_adj(d, s, f, n, ce);
+  if (rc) return rc;
+  VH_REQUIRE(out != nullptr, VH_EINVAL, "null output struct");
+  hipStream_t st = (hipStream_t)stream;
+  if (d->dtype == VH_F64) return adjoint<double>(g, d, s, f, n, ce, nm, lam, out, ws, wsb, st);
+  return adjoint<float>(g, d, s, f, n, ce, nm, lam, out, ws, wsb, st);
+}
+
+int pre_rev(const vh_domain* d, int nm, const vh_settings* s, const vh_fields* f,
+            const void* psf, const vh_evp_adjoint_out* evp_adj, const vh_pre_evp_adjoint_out* out,
+            void* stream) {
+  Geom g;
+  int rc = batch_geom(d, nm, &g);
+  if (rc) return rc;
+  VH_REQUIRE(s && f && evp_adj && out, VH_EINVAL, "bad arguments");
+  hipStream_t st = (hipStream_t)stream;
+  if (d->dtype == VH_F64) return pre_evp_adjoint<double>(g, s, f, psf, evp_adj, out, nm, st);
+  return pre_evp_adjoint<float>(g, s, f, psf, evp_adj, out, nm, st);
+}
+
+}  // namespace
+}  // namespace vh
+
 extern "C" int vh_evp_adjoint_workspace_bytes(const vh_domain* d, int32_t n_substeps,
                                               int32_t ckpt_every, size_t* bytes) {
-  vh::Geom g;
-  int rc = vh::make_geom(d, &g);
-  if (rc) return rc;
-  VH_REQUIRE(bytes && n_substeps >= 1 && ckpt_every >= 1, VH_EINVAL, "bad arguments");
-  *bytes = d->dtype == VH_F64 ? vh::adj_layout<double>(g, n_substeps, ckpt_every).total
-                              : vh::adj_layout<float>(g, n_substeps, ckpt_every).total;
-  return VH_OK;
+  return vh::ws_bytes(d, 1, n_substeps, ckpt_every, bytes);
 }
 
 extern "C" int vh_evp_forward_checkpointed(const vh_domain* d, const vh_settings* s,
@@ -937,18 +1013,8 @@ extern "C" int vh_evp_forward_checkpointed(const vh_domain* d, const vh_settings
                                            int32_t ckpt_every, void* uIce, void* vIce,
                                            void* sigma1, void* sigma2, void* sigma12,
                                            void* workspace, size_t workspace_bytes, void* stream) {
-  vh::Geom g;
-  int rc = vh::make_geom(d, &g);
-  if (rc) return rc;
-  rc = vh::check_adj(d, s, f, n_substeps, ckpt_every);
-  if (rc) return rc;
   void* outs[5] = {uIce, vIce, sigma1, sigma2, sigma12};
-  hipStream_t st = (hipStream_t)stream;
-  if (d->dtype == VH_F64)
-    return vh::forward_ckpt<double>(g, d, s, f, n_substeps, ckpt_every, outs, workspace,
-                                    workspace_bytes, st);
-  return vh::forward_ckpt<float>(g, d, s, f, n_substeps, ckpt_every, outs, workspace,
-                                 workspace_bytes, st);
+  return vh::fwd(d, 1, s, f, n_substeps, ckpt_every, outs, workspace, workspace_bytes, stream);
 }
 
 extern "C" int vh_evp_adjoint(const vh_domain* d, const vh_settings* s, const vh_fields* f,
@@ -956,29 +1022,48 @@ extern "C" int vh_evp_adjoint(const vh_domain* d, const vh_settings* s, const vh
                               const void* adj_vIce, const void* adj_sigma1, const void* adj_sigma2,
                               const void* adj_sigma12, const vh_evp_adjoint_out* out,
                               void* workspace, size_t workspace_bytes, void* stream) {
-  vh::Geom g;
-  int rc = vh::make_geom(d, &g);
-  if (rc) return rc;
-  rc = vh::check_adj(d, s, f, n_substeps, ckpt_every);
-  if (rc) return rc;
-  VH_REQUIRE(out != nullptr, VH_EINVAL, "null output struct");
   const void* lam[5] = {adj_uIce, adj_vIce, adj_sigma1, adj_sigma2, adj_sigma12};
-  hipStream_t st = (hipStream_t)stream;
-  if (d->dtype == VH_F64)
-    return vh::adjoint<double>(g, d, s, f, n_substeps, ckpt_every, lam, out, workspace,
-                               workspace_bytes, st);
-  return vh::adjoint<float>(g, d, s, f, n_substeps, ckpt_every, lam, out, workspace,
-                            workspace_bytes, st);
+  return vh::rev(d, 1, s, f, n_substeps, ckpt_every, lam, out, workspace, workspace_bytes, stream);
 }
 
 extern "C" int vh_pre_evp_adjoint(const vh_domain* d, const vh_settings* s, const vh_fields* f,
                                   const void* pStarField, const vh_evp_adjoint_out* evp_adj,
                                   const vh_pre_evp_adjoint_out* out, void* stream) {
-  vh::Geom g;
-  int rc = vh::make_geom(d, &g);
-  if (rc) return rc;
-  VH_REQUIRE(s && f && evp_adj && out, VH_EINVAL, "bad arguments");
-  hipStream_t st = (hipStream_t)stream;
-  if (d->dtype == VH_F64) return vh::pre_evp_adjoint<double>(g, s, f, pStarField, evp_adj, out, st);
-  return vh::pre_evp_adjoint<float>(g, s, f, pStarField, evp_adj, out, st);
+  return vh::pre_rev(d, 1, s, f, pStarField, evp_adj, out, stream);
+}
+
+extern "C" int vh_evp_adjoint_batched_workspace_bytes(const vh_domain* d, int32_t n_members,
+                                                      int32_t n_substeps, int32_t ckpt_every,
+                                                      size_t* bytes) {
+  return vh::ws_bytes(d, n_members, n_substeps, ckpt_every, bytes);
+}
+
+extern "C" int vh_evp_forward_checkpointed_batched(const vh_domain* d, int32_t n_members,
+                                                   const vh_settings* s, const vh_fields* f,
+                                                   int32_t n_substeps, int32_t ckpt_every,
+                                                   void* uIce, void* vIce, void* sigma1,
+                                                   void* sigma2, void* sigma12, void* workspace,
+                                                   size_t workspace_bytes, void* stream) {
+  void* outs[5] = {uIce, vIce, sigma1, sigma2, sigma12};
+  return vh::fwd(d, n_members, s, f, n_substeps, ckpt_every, outs, workspace, workspace_bytes,
+                 stream);
+}
+
+extern "C" int vh_evp_adjoint_batched(const vh_domain* d, int32_t n_members, const vh_settings* s,
+                                      const vh_fields* f, int32_t n_substeps, int32_t ckpt_every,
+                                      const void* adj_uIce, const void* adj_vIce,
+                                      const void* adj_sigma1, const void* adj_sigma2,
+                                      const void* adj_sigma12, const vh_evp_adjoint_out* out,
+                                      void* workspace, size_t workspace_bytes, void* stream) {
+  const void* lam[5] = {adj_uIce, adj_vIce, adj_sigma1, adj_sigma2, adj_sigma12};
+  return vh::rev(d, n_members, s, f, n_substeps, ckpt_every, lam, out, workspace, workspace_bytes,
+                 stream);
+}
+
+extern "C" int vh_pre_evp_adjoint_batched(const vh_domain* d, int32_t n_members,
+                                          const vh_settings* s, const vh_fields* f,
+                                          const void* pStarField,
+                                          const vh_evp_adjoint_out* evp_adj,
+                                          const vh_pre_evp_adjoint_out* out, void* stream) {
+  return vh::pre_rev(d, n_members, s, f, pStarField, evp_adj, out, stream);
 }

@@ -91,6 +91,11 @@ int check_batch(const vh_batch* b) {
     tiles += n_tiles_of(7, g);  // the shape with the most tiles
   }
   VH_REQUIRE(tiles < (1LL << 30), VH_ESHAPE, "batch too large (%lld tiles)", tiles);
+  // the substep kernel addresses every packed field with 32-bit byte offsets
+  const long long esz = b->dtype == VH_F64 ? 8 : 4;
+  VH_REQUIRE(b->total_elems * esz < (1LL << 32), VH_ESHAPE,
+             "batch too large: %lld elements of %lld bytes exceed 4 GiB per packed field",
+             (long long)b->total_elems, esz);
   std::sort(spans.begin(), spans.end());
   for (size_t q = 1; q < spans.size(); ++q)
     VH_REQUIRE(spans[q].first >= spans[q - 1].second, VH_EINVAL, "member boxes overlap");

@@ -136,6 +136,10 @@ struct EvpArgBlock {
   int rstride;  // elements from one metric's rows to the next
   int packed;
   const TileDesc* tiles[2];  // batched: per-tile placement of the main and remainder shapes (or null)
+  // single-domain launches: elements from one member to the next of a batch
+  // of equal members packed back to back (launch grid y = member; 0 or any
+  // value for one domain, whose grid has y = 1)
+  long long mstride;
 };
 
 
@@ -300,8 +304,10 @@ __global__ __launch_bounds__(256) void evp_pack_kernel(EvpArgBlock<T>* ab) {
     g = md.g;
     off = md.off;
     roff = md.roff;
-  } else {
+  } else {  // one domain, or equal members packed back to back
     g = ab->g;
+    off = (long long)blockIdx.y * ab->mstride;
+    roff = (int)blockIdx.y * g.NY;
   }
   const int e = blockIdx.x * 256 + threadIdx.x;
   if (e >= g.NX * g.NY) return;
@@ -409,12 +415,22 @@ __device__ inline void stg(T* base, unsigned idx, T v) {
 
 __device__ inline int uniform(int x) { return __builtin_amdgcn_readfirstlane(x); }
 
+// x folded into [0, m) by whole periods.  Region coordinates lie within a
+// ring of a few cells around the domain, so one period added or taken away
+// suffices for every domain larger than a workgroup's region; only smaller
+// ones (lanes left out of range) take the integer remainder.
+__device__ inline int fold(int x, int m) {
+  x = x < 0 ? x + m : (x >= m ? x - m : x);
+  if ((unsigned)x >= (unsigned)m) x = ((x % m) + m) % m;
+  return x;
+}
+
 // padded index of region coordinate p along an axis of padded length N,
 // interior length n, overlap 2: roll wrap, or periodic interior if filled
 template <int K>
 __device__ inline int map_axis(int p, int N, int n, int filled) {
-  if (K > 1 && filled) return 2 + wrap(p - 2, n);
-  return wrap(p, N);
+  if (K > 1 && filled) return 2 + fold(p - 2, n);
+  return fold(p, N);
 }
 
 template <typename T, typename TD, int RI, int RJ, int K, bool BATCHED, typename A = T,
@@ -481,8 +497,10 @@ __global__ __launch_bounds__(RI* RJ) void evp_substep_kernel(const EvpArgBlock<T
       const int ti = local / tj_m;
       i0 = ti * TI;
       j0 = (local - ti * tj_m) * TJ;
-    } else {
+    } else {  // one domain, or member blockIdx.y of equal members packed back to back
       g = ab->g;
+      off = (long long)blockIdx.y * ab->mstride;
+      roff = (int)blockIdx.y * g.NY;
       const int ti = tile / tiles_j;
       i0 = ti * TI;
       j0 = (tile - ti * tiles_j) * TJ;
@@ -501,6 +519,11 @@ __global__ __launch_bounds__(RI* RJ) void evp_substep_kernel(const EvpArgBlock<T
     const int gi = map_axis<K>(i, g.NX, g.nx, g.fill_x);
     const int gj = map_axis<K>(j, g.NY, g.ny, g.fill_y);
     const int c = gi * g.NY + gj;  // member-local element index (every region cell is in bounds)
+    // element offsets into the packed fields fit 32 bits (the host checks the
+    // batch size), so every load below uses the field's uniform base pointer
+    // plus one per-lane 32-bit offset shared by all fields
+    const unsigned oo = (unsigned)off;
+    const unsigned ce = oo + (unsigned)c;
 
     // final output cell of this thread
     const bool own = (ar >= R) && (ar < R + TI) && (br >= R) && (br < R + TJ) && (i < g.NX) &&
@@ -528,11 +551,11 @@ __global__ __launch_bounds__(RI* RJ) void evp_substep_kernel(const EvpArgBlock<T
         }
       }
     }
-    A u = ldg(src[0] + off, c), v = ldg(src[1] + off, c);
-    const A uoc = ldg(f.uOc + off, c), voc = ldg(f.vOc + off, c);
+    A u = ldg(src[0] , ce), v = ldg(src[1] , ce);
+    const A uoc = ldg(f.uOc , ce), voc = ldg(f.vOc , ce);
     A Mc, MUc, MVc, mInC, mInUc, mInVc;
     if (packed) {
-      const unsigned pkb = ldg(ab->pk + off, c);
+      const unsigned pkb = ldg(ab->pk , ce);
       Mc = (A)(pkb & 1u);
       MUc = (A)((pkb >> 1) & 1u);
       MVc = (A)((pkb >> 2) & 1u);
@@ -540,12 +563,12 @@ __global__ __launch_bounds__(RI* RJ) void evp_substep_kernel(const EvpArgBlock<T
       mInUc = (A)((pkb >> 4) & 1u);
       mInVc = (A)((pkb >> 5) & 1u);
     } else {
-      Mc = ldg(f.M + off, c);
-      MUc = ldg(f.MU + off, c);
-      MVc = ldg(f.MV + off, c);
-      mInUc = ldg(f.mInU + off, c);
-      mInVc = ldg(f.mInV + off, c);
-      mInC = ldg(f.mInC + off, c);
+      Mc = ldg(f.M , ce);
+      MUc = ldg(f.MU , ce);
+      MVc = ldg(f.MV , ce);
+      mInUc = ldg(f.mInU , ce);
+      mInVc = ldg(f.mInV , ce);
+      mInC = ldg(f.mInC , ce);
     }
     // issue order = order of use: stage A waits only for the loads above, the
     // rest stream in while A and B run
@@ -553,37 +576,37 @@ __global__ __launch_bounds__(RI* RJ) void evp_substep_kernel(const EvpArgBlock<T
     A g_rdyU = 0, g_rdxV = 0, g_rAz = 0, g_rdxU = 0, g_rdyV = 0, g_rrA = 0;
     A g_dyV = 0, g_dxU = 0, g_dxV = 0, g_dyU = 0, g_rrAu = 0, g_rrAv = 0;
     if (!packed) {
-      g_rdyU = ldg(f.rdyU + off, c);
-      g_rdxV = ldg(f.rdxV + off, c);
-      g_rAz = ldg(f.rAz + off, c);
-      g_rdxU = ldg(f.rdxU + off, c);
-      g_rdyV = ldg(f.rdyV + off, c);
-      g_rrA = ldg(f.rrA + off, c);
-      g_dyV = ldg(f.dyV + off, c);
-      g_dxU = ldg(f.dxU + off, c);
-      g_dxV = ldg(f.dxV + off, c);
-      g_dyU = ldg(f.dyU + off, c);
-      g_rrAu = ldg(f.rrAu + off, c);
-      g_rrAv = ldg(f.rrAv + off, c);
+      g_rdyU = ldg(f.rdyU , ce);
+      g_rdxV = ldg(f.rdxV , ce);
+      g_rAz = ldg(f.rAz , ce);
+      g_rdxU = ldg(f.rdxU , ce);
+      g_rdyV = ldg(f.rdyV , ce);
+      g_rrA = ldg(f.rrA , ce);
+      g_dyV = ldg(f.dyV , ce);
+      g_dxU = ldg(f.dxU , ce);
+      g_dxV = ldg(f.dxV , ce);
+      g_dyU = ldg(f.dyU , ce);
+      g_rrAu = ldg(f.rrAu , ce);
+      g_rrAv = ldg(f.rrAv , ce);
     }
     A k1Z = 0, k2Z = 0, k1C = 0, k2C = 0;  // metric terms
     if (sc.kterms) {
-      k1Z = ldg(f.k1Z + off, c);
-      k2Z = ldg(f.k2Z + off, c);
+      k1Z = ldg(f.k1Z , ce);
+      k2Z = ldg(f.k2Z , ce);
     }
     __builtin_amdgcn_sched_barrier(0);
-    const A P = ldg(f.P + off, c);
-    A s1 = ldg(src[2] + off, c), s2 = ldg(src[3] + off, c);
-    const A fc = ldg(f.fC + off, c);
-    const A mC = ldg(f.massC + off, c);
-    A s12 = ldg(src[4] + off, c);
-    const A mU = ldg(f.massU + off, c), mV = ldg(f.massV + off, c);
-    const A AW = ldg(f.AreaW + off, c), AS = ldg(f.AreaS + off, c);
-    const A wfx = ldg(f.WFX + off, c), wfy = ldg(f.WFY + off, c);
-    const A uNm1 = ldg(f.uNm1 + off, c), vNm1 = ldg(f.vNm1 + off, c);
+    const A P = ldg(f.P , ce);
+    A s1 = ldg(src[2] , ce), s2 = ldg(src[3] , ce);
+    const A fc = ldg(f.fC , ce);
+    const A mC = ldg(f.massC , ce);
+    A s12 = ldg(src[4] , ce);
+    const A mU = ldg(f.massU , ce), mV = ldg(f.massV , ce);
+    const A AW = ldg(f.AreaW , ce), AS = ldg(f.AreaS , ce);
+    const A wfx = ldg(f.WFX , ce), wfy = ldg(f.WFY , ce);
+    const A uNm1 = ldg(f.uNm1 , ce), vNm1 = ldg(f.vNm1 , ce);
     if (sc.kterms) {
-      k1C = ldg(f.k1C + off, c);
-      k2C = ldg(f.k2C + off, c);
+      k1C = ldg(f.k1C , ce);
+      k2C = ldg(f.k2C , ce);
     }
     A cbc = (A)0;  // basal drag coefficient inputs are step constants: load lazily
     const A sg = sgn(fc);
@@ -594,8 +617,8 @@ __global__ __launch_bounds__(RI* RJ) void evp_substep_kernel(const EvpArgBlock<T
     // lateral drag form factors at this u- and v-point (veris/dynamics_routines.py:119-130)
     A sdmU = (A)0, sdmV = (A)0;
     if (sc.sidedrag && sc.coast) {
-      sdmU = ldg(f.Fu + off, c);
-      sdmV = ldg(f.Fv + off, c);
+      sdmU = ldg(f.Fu , ce);
+      sdmV = ldg(f.Fv , ce);
     }
     // metric rows in LDS (stored in stage A of the first substep)
     __shared__ A sRow[kEvpRowFields * RJ];
@@ -657,12 +680,12 @@ __global__ __launch_bounds__(RI* RJ) void evp_substep_kernel(const EvpArgBlock<T
           const int cip1 = wrap(i + 1, g.NX) * g.NY + gj;
           const int cjm2 = gi * g.NY + wrap(j - 2, g.NY), cjm1 = gi * g.NY + wrap(j - 1, g.NY);
           const int cjp1 = gi * g.NY + wrap(j + 1, g.NY);
-          const A uim2 = ldg(src[0] + off, cim2), uip1 = ldg(src[0] + off, cip1);
-          const A MUim2 = ldg(f.MU + off, cim2), MUim1 = ldg(f.MU + off, cim1);
-          const A MUip1 = ldg(f.MU + off, cip1);
-          const A vjm2 = ldg(src[1] + off, cjm2), vjp1 = ldg(src[1] + off, cjp1);
-          const A MVjm2 = ldg(f.MV + off, cjm2), MVjm1 = ldg(f.MV + off, cjm1);
-          const A MVjp1 = ldg(f.MV + off, cjp1);
+          const A uim2 = ldg(src[0] , oo + (unsigned)cim2), uip1 = ldg(src[0] , oo + (unsigned)cip1);
+          const A MUim2 = ldg(f.MU , oo + (unsigned)cim2), MUim1 = ldg(f.MU , oo + (unsigned)cim1);
+          const A MUip1 = ldg(f.MU , oo + (unsigned)cip1);
+          const A vjm2 = ldg(src[1] , oo + (unsigned)cjm2), vjp1 = ldg(src[1] , oo + (unsigned)cjp1);
+          const A MVjm2 = ldg(f.MV , oo + (unsigned)cjm2), MVjm1 = ldg(f.MV , oo + (unsigned)cjm1);
+          const A MVjp1 = ldg(f.MV , oo + (unsigned)cjp1);
           A hu2 = (MUc - MUim1) / (A)3.0;
           A hv2 = (MVc - MVjm1) / (A)3.0;
           hu2 = hu2 * (MUim2 * MUim1 + MUip1 * MUc);
@@ -742,13 +765,13 @@ __global__ __launch_bounds__(RI* RJ) void evp_substep_kernel(const EvpArgBlock<T
           if (sc.basal) {
             const int cip = map_axis<K>(i + 1, g.NX, g.nx, g.fill_x) * g.NY + gj;
             const int cjp = gi * g.NY + map_axis<K>(j + 1, g.NY, g.ny, g.fill_y);
-            const A um = u * mInUc, umip = uip * ldg(f.mInU + off, cip);
-            const A vm = v * mInVc, vmjp = vjp * ldg(f.mInV + off, cjp);
+            const A um = u * mInUc, umip = uip * ldg(f.mInU , oo + (unsigned)cip);
+            const A vm = v * mInVc, vmjp = vjp * ldg(f.mInV , oo + (unsigned)cjp);
             A tb = (A)0.25 * (((um * um + umip * umip) + vm * vm) + vmjp * vmjp);
             tb = (A)sc.K2 / sqrt(tb + (A)sc.U0sq);
-            const A Ar = ldg(f.Area + off, c);
-            const A hc = (fabs(ldg(f.Rlow + off, c)) * Ar) / (A)sc.K1;
-            const A val = ((tb * log(exp((A)10.0 * (ldg(f.hIce + off, c) - hc)) + (A)1.0)) * (A)0.1) *
+            const A Ar = ldg(f.Area , ce);
+            const A hc = (fabs(ldg(f.Rlow , ce)) * Ar) / (A)sc.K1;
+            const A val = ((tb * log(exp((A)10.0 * (ldg(f.hIce , ce) - hc)) + (A)1.0)) * (A)0.1) *
                           exp((A)(-sc.cBasalStar) * ((A)1.0 - Ar));
             cbc = Ar > (A)0.01 ? val : (A)0;
           }
@@ -893,12 +916,12 @@ __global__ __launch_bounds__(RI* RJ) void evp_substep_kernel(const EvpArgBlock<T
           store_filled(g, GW(dst[1] + off), i, j, v);
         }
       } else {
-        stg(dst[0] + off, cp, (T)u);
-        stg(dst[1] + off, cp, (T)v);
+        stg(dst[0], oo + (unsigned)cp, (T)u);
+        stg(dst[1], oo + (unsigned)cp, (T)v);
       }
-      stg(dst[2] + off, cp, (T)s1);
-      stg(dst[3] + off, cp, (T)s2);
-      stg(dst[4] + off, cp, (T)s12);
+      stg(dst[2], oo + (unsigned)cp, (T)s1);
+      stg(dst[3], oo + (unsigned)cp, (T)s2);
+      stg(dst[4], oo + (unsigned)cp, (T)s12);
     }
   };
   if constexpr (BATCHED) {
@@ -1048,15 +1071,16 @@ inline void evp_launch_cfg(int cfg, const EvpArgBlock<T>* ab, int which, int lau
 }
 
 // Single-domain, single-substep launch of shape `cfg` (a shape with k = 1)
-// reading its arguments from `ab` (nlaunch = 1: in -> out).
+// reading its arguments from `ab` (nlaunch = 1: in -> out), for n_members
+// equal members packed back to back (ab->mstride apart; 1 = one domain).
 template <typename T, typename TD>
 inline void evp_launch_single(int cfg, const EvpArgBlock<T>* ab, int n_tiles, int tiles_j,
-                              hipStream_t st) {
+                              hipStream_t st, int n_members = 1) {
   const int per_xcd = (n_tiles + 7) / 8;
   evp_with_shape(cfg, [&](auto sh) {
     using S = decltype(sh);
     constexpr int RI = S::k == 1 ? S::ri : 16, RJ = S::k == 1 ? S::rj : 32;
-    hipLaunchKernelGGL((evp_substep_kernel<T, TD, RI, RJ, 1, false>), dim3(per_xcd * 8),
+    hipLaunchKernelGGL((evp_substep_kernel<T, TD, RI, RJ, 1, false>), dim3(per_xcd * 8, n_members),
                        dim3(RI * RJ), 0, st, ab, 0, 0, n_tiles, tiles_j, per_xcd);
   });
 }

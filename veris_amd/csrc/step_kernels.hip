@@ -9,6 +9,8 @@
 // kernels run once per model step and move < 1% of the step's bytes, so they
 // are plain coalesced sweeps.  Built with -ffp-contract=off: every expression
 // keeps the reference's association order.
+#include <vector>
+
 #include "vh_common.h"
 
 namespace vh {
@@ -485,8 +487,12 @@ struct PreEvpIO {
 };
 
 template <typename T>
-__global__ __launch_bounds__(BLK) void pre_evp_kernel(Geom g, WindArgs<T> w, PreEvpIO<T> o) {
-  CELL_PROLOGUE
+__device__ inline void pre_evp_cell(const Geom& g, const WindArgs<T>& w, const PreEvpIO<T>& o,
+                                    size_t idx) {
+  const int i = (int)(idx / g.NY);
+  const int j = (int)(idx - (size_t)i * g.NY);
+  const int im = (i == 0 ? g.NX - 1 : i - 1);
+  const int jm = (j == 0 ? g.NY - 1 : j - 1);
   const T c = o.rhoI * o.h[idx] + o.rhoS * o.hs[idx];
   const T cim = o.rhoI * o.h[AT(im, j)] + o.rhoS * o.hs[AT(im, j)];
   const T cjm = o.rhoI * o.h[AT(i, jm)] + o.rhoS * o.hs[AT(i, jm)];
@@ -518,10 +524,126 @@ __global__ __launch_bounds__(BLK) void pre_evp_kernel(Geom g, WindArgs<T> w, Pre
 }
 
 template <typename T>
-int pre_evp(const Geom& g, const vh_settings* s, const vh_fields* f, const void* psf,
-            const vh_pre_evp_out* out, hipStream_t st) {
+__global__ __launch_bounds__(BLK) void pre_evp_kernel(Geom g, WindArgs<T> w, PreEvpIO<T> o) {
+  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (size_t)g.NX * g.NY) return;
+  pre_evp_cell(g, w, o, idx);
+}
+
+// Members of a batch (packed fields, each member's padded box at its element
+// offset), passed by value: up to kBatchChunk per launch.
+struct BatchMember {
+  long long off;
+  Geom g;
+};
+constexpr int kBatchChunk = 64;
+struct BatchChunk {
+  int count;
+  BatchMember m[kBatchChunk];
+};
+
+template <typename P>
+__device__ inline P* at_off(P* p, long long off) {
+  return p ? p + off : p;
+}
+
+template <typename T>
+__device__ inline WindArgs<T> shifted(WindArgs<T> w, long long off) {
+  w.u = at_off(w.u, off);
+  w.v = at_off(w.v, off);
+  w.uW = at_off(w.uW, off);
+  w.vW = at_off(w.vW, off);
+  w.fC = at_off(w.fC, off);
+  w.MU = at_off(w.MU, off);
+  w.MV = at_off(w.MV, off);
+  w.AW = at_off(w.AW, off);
+  w.AS = at_off(w.AS, off);
+  w.mU = at_off(w.mU, off);
+  w.mV = at_off(w.mV, off);
+  w.rdxC = at_off(w.rdxC, off);
+  w.rdyC = at_off(w.rdyC, off);
+  w.ssh = at_off(w.ssh, off);
+  w.sp = at_off(w.sp, off);
+  w.load = at_off(w.load, off);
+  return w;
+}
+
+template <typename T>
+__device__ inline PreEvpIO<T> shifted(PreEvpIO<T> o, long long off) {
+  o.h = at_off(o.h, off);
+  o.hs = at_off(o.hs, off);
+  o.A = at_off(o.A, off);
+  o.M = at_off(o.M, off);
+  o.psf = at_off(o.psf, off);
+  o.mC = at_off(o.mC, off);
+  o.mU = at_off(o.mU, off);
+  o.mV = at_off(o.mV, off);
+  o.AW = at_off(o.AW, off);
+  o.AS = at_off(o.AS, off);
+  o.WX = at_off(o.WX, off);
+  o.WY = at_off(o.WY, off);
+  o.P = at_off(o.P, off);
+  return o;
+}
+
+// The pre-EVP sweep of every member of a chunk in one launch: blockIdx.y is
+// the member, blockIdx.x runs over its padded cells.
+template <typename T>
+__global__ __launch_bounds__(BLK) void pre_evp_batched_kernel(BatchChunk ch, WindArgs<T> w,
+                                                              PreEvpIO<T> o) {
+  const BatchMember& m = ch.m[blockIdx.y];
+  const Geom g = m.g;
+  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (size_t)g.NX * g.NY) return;
+  pre_evp_cell(g, shifted(w, m.off), shifted(o, m.off), idx);
+}
+
+// FP64 sums over each member's interior of up to kMaxSumFields packed fields
+// (times `w` when not null): one workgroup per (member, field), a fixed
+// summation order, result [field * n_members + first + member].
+constexpr int kMaxSumFields = 8;
+template <typename T>
+struct SumFields {
+  const T* x[kMaxSumFields];
+  const T* w;
+  double* out;
+  int n_members, first;
+};
+
+template <typename T>
+__global__ __launch_bounds__(BLK) void sums_batched_kernel(BatchChunk ch, SumFields<T> a) {
+  const BatchMember& m = ch.m[blockIdx.x];
+  const Geom g = m.g;
+  const T* x = a.x[blockIdx.y] + m.off;
+  const T* w = a.w ? a.w + m.off : nullptr;
+  __shared__ double red[BLK / 64];
+  const int n = g.nx * g.ny;
+  double acc = 0.0;
+  for (int k = threadIdx.x; k < n; k += BLK) {
+    const int ii = k / g.ny + g.ol, jj = k - (k / g.ny) * g.ny + g.ol;
+    const size_t c = AT(ii, jj);
+    double v = (double)x[c];
+    if (w) v = v * (double)w[c];
+    acc += v;
+  }
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s = 0.0;
+    for (int q = 0; q < BLK / 64; ++q) s += red[q];
+    a.out[(size_t)blockIdx.y * a.n_members + a.first + blockIdx.x] = s;
+  }
+}
+
+// Arguments of the pre-EVP sweep (validated), shared by the single-domain and
+// the batched launch.
+template <typename T>
+int pre_evp_args(const vh_settings* s, const vh_fields* f, const void* psf,
+                 const vh_pre_evp_out* out, WindArgs<T>* wp, PreEvpIO<T>* op) {
   VH_REQUIRE(out != nullptr, VH_EINVAL, "pre_evp: null output struct");
-  WindArgs<T> w;
+  WindArgs<T>& w = *wp;
+  PreEvpIO<T>& o = *op;
   w.u = C_<T>(f->uIce);
   w.v = C_<T>(f->vIce);
   w.uW = C_<T>(f->uWind);
@@ -540,7 +662,6 @@ int pre_evp(const Geom& g, const vh_settings* s, const vh_fields* f, const void*
   w.load = C_<T>(f->SeaIceLoad);
   w.relWind = s->useRelativeWind != 0.0;
   w.realFW = s->useRealFreshWaterFlux != 0.0;
-  PreEvpIO<T> o;
   o.h = C_<T>(f->hIceMean);
   o.hs = C_<T>(f->hSnowMean);
   o.A = C_<T>(f->Area);
@@ -581,7 +702,87 @@ int pre_evp(const Geom& g, const vh_settings* s, const vh_fields* f, const void*
   o.cStar = (T)s->cStar;
   o.rhoI = (T)s->rhoIce;
   o.rhoS = (T)s->rhoSnow;
+  return VH_OK;
+}
+
+template <typename T>
+int pre_evp(const Geom& g, const vh_settings* s, const vh_fields* f, const void* psf,
+            const vh_pre_evp_out* out, hipStream_t st) {
+  WindArgs<T> w;
+  PreEvpIO<T> o;
+  int rc = pre_evp_args<T>(s, f, psf, out, &w, &o);
+  if (rc) return rc;
   hipLaunchKernelGGL(pre_evp_kernel<T>, cell_grid(g, BLK), dim3(BLK), 0, st, g, w, o);
+  VH_TRY_HIP(hipGetLastError());
+  return VH_OK;
+}
+
+// Members of a batch in chunks of kBatchChunk (validated like the batched EVP).
+int batch_chunks(const vh_batch* b, std::vector<BatchChunk>* chunks, long long* max_cells) {
+  VH_REQUIRE(b != nullptr && b->members != nullptr && b->n_members > 0, VH_EINVAL, "empty batch");
+  VH_REQUIRE(b->dtype == VH_F64 || b->dtype == VH_F32, VH_EDTYPE, "dtype %d", b->dtype);
+  *max_cells = 0;
+  chunks->clear();
+  for (int m = 0; m < b->n_members; ++m) {
+    if (m % kBatchChunk == 0) chunks->push_back(BatchChunk{});
+    BatchChunk& ch = chunks->back();
+    BatchMember& bm = ch.m[ch.count++];
+    const vh_member& mm = b->members[m];
+    vh_domain d{mm.nx, mm.ny, 2, 2, mm.cyclic_x, mm.fill_mode, b->dtype, 0};
+    int rc = make_geom(&d, &bm.g);
+    if (rc) return rc;
+    bm.off = mm.offset;
+    const long long n = (long long)bm.g.NX * bm.g.NY;
+    VH_REQUIRE(mm.offset >= 0 && mm.offset + n <= b->total_elems, VH_ESHAPE,
+               "member %d box [%lld, %lld) outside the packed fields (%lld)", m,
+               (long long)mm.offset, (long long)mm.offset + n, (long long)b->total_elems);
+    *max_cells = n > *max_cells ? n : *max_cells;
+  }
+  return VH_OK;
+}
+
+template <typename T>
+int pre_evp_batched(const vh_batch* b, const vh_settings* s, const vh_fields* f, const void* psf,
+                    const vh_pre_evp_out* out, hipStream_t st) {
+  std::vector<BatchChunk> chunks;
+  long long max_cells = 0;
+  int rc = batch_chunks(b, &chunks, &max_cells);
+  if (rc) return rc;
+  WindArgs<T> w;
+  PreEvpIO<T> o;
+  rc = pre_evp_args<T>(s, f, psf, out, &w, &o);
+  if (rc) return rc;
+  for (const BatchChunk& ch : chunks)
+    hipLaunchKernelGGL(pre_evp_batched_kernel<T>,
+                       dim3((unsigned)((max_cells + BLK - 1) / BLK), ch.count), dim3(BLK), 0, st,
+                       ch, w, o);
+  VH_TRY_HIP(hipGetLastError());
+  return VH_OK;
+}
+
+template <typename T>
+int sums_batched(const vh_batch* b, int nf, const void* const* fields, const void* wgt,
+                 double* result, hipStream_t st) {
+  std::vector<BatchChunk> chunks;
+  long long max_cells = 0;
+  int rc = batch_chunks(b, &chunks, &max_cells);
+  if (rc) return rc;
+  VH_REQUIRE(nf >= 1 && nf <= kMaxSumFields, VH_EINVAL, "n_fields %d (1..%d)", nf, kMaxSumFields);
+  VH_REQUIRE(fields != nullptr && result != nullptr, VH_EINVAL, "global sums: null pointer");
+  SumFields<T> a{};
+  for (int q = 0; q < nf; ++q) {
+    VH_REQUIRE(fields[q] != nullptr, VH_EINVAL, "global sums: field %d is null", q);
+    a.x[q] = C_<T>(fields[q]);
+  }
+  a.w = C_<T>(wgt);
+  a.out = result;
+  a.n_members = b->n_members;
+  int first = 0;
+  for (const BatchChunk& ch : chunks) {
+    a.first = first;
+    hipLaunchKernelGGL(sums_batched_kernel<T>, dim3(ch.count, nf), dim3(BLK), 0, st, ch, a);
+    first += ch.count;
+  }
   VH_TRY_HIP(hipGetLastError());
   return VH_OK;
 }
@@ -730,6 +931,24 @@ extern "C" int vh_pre_evp(const vh_domain* d, const vh_settings* s, const vh_fie
   VH_PRE(d, s, f)
   return VH_DISPATCH(d, vh::pre_evp<double>(g, s, f, pStarField, out, st),
                      vh::pre_evp<float>(g, s, f, pStarField, out, st));
+}
+
+extern "C" int vh_pre_evp_batched(const vh_batch* b, const vh_settings* s, const vh_fields* f,
+                                  const void* pStarField, const vh_pre_evp_out* out,
+                                  void* stream) {
+  VH_REQUIRE(b != nullptr && s != nullptr && f != nullptr, VH_EINVAL, "null batch, settings or fields");
+  hipStream_t st = (hipStream_t)stream;
+  if (b->dtype == VH_F32) return vh::pre_evp_batched<float>(b, s, f, pStarField, out, st);
+  return vh::pre_evp_batched<double>(b, s, f, pStarField, out, st);
+}
+
+extern "C" int vh_global_sums_batched(const vh_batch* b, int32_t n_fields,
+                                      const void* const* fields, const void* weight,
+                                      double* result, void* stream) {
+  VH_REQUIRE(b != nullptr, VH_EINVAL, "null batch");
+  hipStream_t st = (hipStream_t)stream;
+  if (b->dtype == VH_F32) return vh::sums_batched<float>(b, n_fields, fields, weight, result, st);
+  return vh::sums_batched<double>(b, n_fields, fields, weight, result, st);
 }
 
 namespace {

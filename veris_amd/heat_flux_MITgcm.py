@@ -16,7 +16,7 @@ def bulkf_formula_lanl(state, uw, vw, ta, qa, tsf, ocn_mask):
     Inputs are device tensors of the state's padded shape and dtype; gravity
     is the host model's ``settings.grav`` (Veros), or the plugin's
     ``gravity`` when the settings carry no ``grav``."""
-    c = Ctx(state)
+    c = Ctx.of(state)
     for name, t in (("uw", uw), ("vw", vw), ("ta", ta), ("qa", qa), ("tsf", tsf),
                     ("ocn_mask", ocn_mask)):
         c._check(name, t)

@@ -11,6 +11,7 @@ device; there is no CPU path.
 from __future__ import annotations
 
 import ctypes
+import weakref
 
 import torch
 
@@ -62,6 +63,7 @@ class Ctx:
         self.sett = N.settings_struct(sett)
         self.fields = N.Fields()
         self.field_names = []  # variables whose device pointers are in self.fields
+        self._src = {}  # name -> weak reference to the tensor behind its pointer
         for name in VARIABLE_NAMES:
             t = vs.get(name)
             if t is None:
@@ -69,10 +71,69 @@ class Ctx:
             self._check(name, t)
             setattr(self.fields, name, t.data_ptr())
             self.field_names.append(name)
+            self._src[name] = weakref.ref(t)
+        self._kterms(vs)
+        self.stream = torch.cuda.current_stream(self.device).cuda_stream
+
+    def _kterms(self, vs):
+        """Metric-term fields that are absent or all zero are passed as NULL
+        (the kernels then skip them)."""
         if all(vs.get(k) is None or _all_zero(vs.get(k)) for k in _KTERMS):
             for k in _KTERMS:
                 setattr(self.fields, k, None)
-        self.stream = torch.cuda.current_stream(self.device).cuda_stream
+        else:
+            for k in _KTERMS:
+                setattr(self.fields, k, vs.get(k).data_ptr())
+
+    def _refresh(self, vs) -> bool:
+        """Re-point the fields of the variables that were reassigned since
+        this Ctx was built (each new tensor checked as in the constructor);
+        False if the set of variables changed (then rebuild)."""
+        names = []
+        for name in VARIABLE_NAMES:
+            t = vs.get(name)
+            if t is None:
+                continue
+            names.append(name)
+            r = self._src.get(name)
+            if r is not None and r() is t:
+                continue
+            self._check(name, t)
+            setattr(self.fields, name, t.data_ptr())
+            self._src[name] = weakref.ref(t)
+        if names != self.field_names:
+            return False
+        self._kterms(vs)
+        return True
+
+    @classmethod
+    def of(cls, state, ref: torch.Tensor | None = None) -> "Ctx":
+        """The Ctx of ``state``, built once and reused by later calls on the
+        same stream while no variable or setting of the state has been
+        assigned since (the namespaces count assignments); in-place changes of
+        the metric-term fields, which decide whether they are read at all, are
+        noticed through their version counters.  Callers must not modify the
+        returned structs except ``dom.tile``, which every EVP call sets."""
+        vs, sett = state.variables, state.settings
+        t = vs.get("iceMask") if ref is None else ref
+        vver, sver = getattr(vs, "_version", None), getattr(sett, "_version", None)
+        if not isinstance(t, torch.Tensor) or t.device.type != "cuda" or vver is None or sver is None:
+            return cls(state, ref)
+        kv = tuple(None if x is None else (id(x), x._version) for x in map(vs.get, _KTERMS))
+        stream = torch.cuda.current_stream(t.device).cuda_stream
+        slot = (t.dtype, t.device, stream)
+        cache = vs.__dict__.setdefault("_ctx_cache", {})
+        hit = cache.get(slot)
+        if hit is not None and hit[1] is sett and hit[0][1] == sver:
+            if hit[0] == (vver, sver, kv):
+                return hit[2]
+            c = hit[2]
+            if c._refresh(vs):  # only some variables were reassigned
+                cache[slot] = ((vver, sver, kv), sett, c)
+                return c
+        c = cls(state, ref)
+        cache[slot] = ((vver, sver, kv), sett, c)
+        return c
 
     def _check(self, name, t):
         if not isinstance(t, torch.Tensor):
@@ -85,8 +146,10 @@ class Ctx:
                              f"{t.is_contiguous()}), expected contiguous {self.shape}")
 
     def empty(self, n=1):
-        out = [torch.empty(self.shape, dtype=self.dtype, device=self.device) for _ in range(n)]
-        return out if n > 1 else out[0]
+        """One output field, or a list of n (views of one allocation)."""
+        if n == 1:
+            return torch.empty(self.shape, dtype=self.dtype, device=self.device)
+        return list(torch.empty((n,) + self.shape, dtype=self.dtype, device=self.device).unbind(0))
 
     def workspace(self, query: str) -> torch.Tensor:
         nbytes = N.workspace_bytes(query, self.dom)
@@ -110,7 +173,7 @@ def _p(t):
 
 def SeaIceMass(state):
     """veris/area_mass.py:17-28 -> (SeaIceMassC, SeaIceMassU, SeaIceMassV)."""
-    c = Ctx(state)
+    c = Ctx.of(state)
     out = c.empty(3)
     N.call("vh_seaice_mass", *c.args(), *map(_p, out), c.stream)
     return tuple(out)
@@ -118,7 +181,7 @@ def SeaIceMass(state):
 
 def AreaWS(state):
     """veris/area_mass.py:5-14 -> (AreaW, AreaS)."""
-    c = Ctx(state)
+    c = Ctx.of(state)
     out = c.empty(2)
     N.call("vh_area_ws", *c.args(), *map(_p, out), c.stream)
     return tuple(out)
@@ -128,7 +191,7 @@ def SeaIceStrength(state, pStar: torch.Tensor | None = None):
     """veris/dynamics_routines.py:5-18 -> SeaIceStrength.
 
     ``pStar`` optionally replaces the scalar setting by a per-cell field."""
-    c = Ctx(state)
+    c = Ctx.of(state)
     if pStar is not None:
         c._check("pStar", pStar)
     out = c.empty()
@@ -138,7 +201,7 @@ def SeaIceStrength(state, pStar: torch.Tensor | None = None):
 
 def WindForcingXY(state):
     """veris/dynsolver.py:50-84 (with tauXY :7-47) -> (WindForcingX, WindForcingY)."""
-    c = Ctx(state)
+    c = Ctx.of(state)
     out = c.empty(2)
     N.call("vh_wind_forcing_xy", *c.args(), *map(_p, out), c.stream)
     return tuple(out)
@@ -150,7 +213,7 @@ def pre_evp(state, pStar: torch.Tensor | None = None):
     veris/dynamics_routines.py:5-18) -> (SeaIceMassC, SeaIceMassU, SeaIceMassV,
     AreaW, AreaS, WindForcingX, WindForcingY, SeaIceStrength); results equal
     the four separate drop-ins, in that order of the reference setup."""
-    c = Ctx(state)
+    c = Ctx.of(state)
     if pStar is not None:
         c._check("pStar", pStar)
     buf = torch.empty((len(N.PRE_EVP_NAMES),) + c.shape, dtype=c.dtype, device=c.device)
@@ -178,7 +241,7 @@ def evp_solver(state, n_substeps: int | None = None, tile: int | None = None):
     """veris/evp_solver.py:257-319 -> (uIce, vIce, sigma1, sigma2, sigma12).
 
     ``tile`` selects the workgroup shape (see :func:`veris_amd.autotune.tune_evp`)."""
-    c = Ctx(state)
+    c = Ctx.of(state)
     if tile is None:
         tile = EVP_TILE.get((c.shape, c.dtype), 0)
         if tile and not single_substep_ok(state.settings, tile):
@@ -196,7 +259,7 @@ def evp_solver_residuals(state, n_substeps: int | None = None):
     (uIce, vIce, sigma1, sigma2, sigma12, resU, resSig): the same subcycle as
     :func:`evp_solver` (bit for bit) plus the per-substep residual norms as
     FP64 device tensors of length n_substeps."""
-    c = Ctx(state)
+    c = Ctx.of(state)
     n = int(state.settings.nEVPsteps if n_substeps is None else n_substeps)
     out = c.empty(5)
     res = torch.empty((2, n), dtype=torch.float64, device=c.device)
@@ -210,7 +273,7 @@ def evp_solver_residuals(state, n_substeps: int | None = None):
 
 def freedrift_solver(state):
     """veris/freedrift_solver.py:5-62 -> (uIce, vIce): steady free drift."""
-    c = Ctx(state)
+    c = Ctx.of(state)
     out = c.empty(2)
     N.call("vh_freedrift_solver", *c.args(), *map(_p, out), c.stream)
     return tuple(out)
@@ -234,7 +297,7 @@ def IceVelocities(state, n_substeps: int | None = None):
 
 def OceanStressUV(state):
     """veris/ocean_stress.py:8-40 -> (OceanStressU, OceanStressV)."""
-    c = Ctx(state)
+    c = Ctx.of(state)
     out = c.empty(2)
     N.call("vh_ocean_stress_uv", *c.args(), *map(_p, out), c.stream)
     return tuple(out)
@@ -242,7 +305,7 @@ def OceanStressUV(state):
 
 def Advection(state):
     """veris/advection.py:15-25 -> (hIceMean, hSnowMean, Area)."""
-    c = Ctx(state)
+    c = Ctx.of(state)
     out = c.empty(3)
     ws = c.workspace("vh_advection_workspace_bytes")
     N.call("vh_advection", *c.args(), *map(_p, out), _p(ws), ws.numel(), c.stream)
@@ -252,7 +315,7 @@ def Advection(state):
 def clean_up_advection(state):
     """veris/clean_up.py:5-39 ->
     (hIceMean, hSnowMean, Area, TSurf, os_hIceMean, os_hSnowMean)."""
-    c = Ctx(state)
+    c = Ctx.of(state)
     out = c.empty(6)
     N.call("vh_clean_up_advection", *c.args(), *map(_p, out), c.stream)
     return tuple(out)
@@ -260,7 +323,7 @@ def clean_up_advection(state):
 
 def ridging(state):
     """veris/clean_up.py:42-47 -> Area."""
-    c = Ctx(state)
+    c = Ctx.of(state)
     out = c.empty()
     N.call("vh_ridging", *c.args(), _p(out), c.stream)
     return out
@@ -269,7 +332,7 @@ def ridging(state):
 def Growth(state):
     """veris/growth.py:8-397 -> (hIceMean, hSnowMean, Area, TSurf, EmPmR,
     forc_salt_surface, Qsw, Qnet, SeaIceLoad, IcePenetSW, recip_hIceMean)."""
-    c = Ctx(state)
+    c = Ctx.of(state)
     out = c.empty(len(N.GROWTH_OUT_NAMES))
     go = N.GrowthOut(*[t.data_ptr() for t in out])
     N.call("vh_growth", *c.args(), ctypes.byref(go), c.stream)
@@ -278,7 +341,7 @@ def Growth(state):
 
 def fill_overlap(state, a: torch.Tensor) -> torch.Tensor:
     """veris/fill_overlap.py:6-18 -> filled copy of ``a``."""
-    c = Ctx(state, ref=a)
+    c = Ctx.of(state, ref=a)
     c._check("field", a)
     out = a.clone()
     N.call("vh_fill_overlap", ctypes.byref(c.dom), _p(out), c.stream)
@@ -294,7 +357,7 @@ def global_sum(state, a: torch.Tensor, weight: torch.Tensor | None = None) -> to
     """FP64 sum of ``a`` (times ``weight``) over the interior cells, as a 0-d
     float64 device tensor.  The reference's ``global_sum`` (veris/global_sum.py:1-4)
     is an identity stub; this is the reduction it stands for."""
-    c = Ctx(state, ref=a)
+    c = Ctx.of(state, ref=a)
     c._check("field", a)
     if weight is not None:
         c._check("weight", weight)

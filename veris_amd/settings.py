@@ -181,10 +181,12 @@ _RECIPROCALS = {
 
 
 class SettingsNamespace:
-    """Attribute namespace of setting values (``state.settings.<name>``)."""
+    """Attribute namespace of setting values (``state.settings.<name>``);
+    ``_version`` counts assignments."""
 
     def __init__(self, values: dict):
         object.__setattr__(self, "_values", dict(values))
+        object.__setattr__(self, "_version", 0)
 
     def __getattr__(self, name):
         try:
@@ -196,6 +198,7 @@ class SettingsNamespace:
         if name not in SETTINGS and name not in self._values:
             raise AttributeError(f"unknown setting {name!r}")
         self._values[name] = value
+        object.__setattr__(self, "_version", self._version + 1)
 
     def __getitem__(self, name):
         return self._values[name]

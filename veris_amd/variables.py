@@ -128,10 +128,15 @@ GRID_NAMES = tuple(n for n, _, _ in _MASKS + _GRIDVARS)
 
 
 class VariablesNamespace:
-    """Attribute namespace of arrays (``state.variables.<name>``)."""
+    """Attribute namespace of arrays (``state.variables.<name>``).
+
+    ``_version`` counts assignments, so a caller can tell whether any
+    variable has been replaced since it last looked (in-place changes of an
+    array keep its identity and do not count)."""
 
     def __init__(self, arrays: dict | None = None):
         object.__setattr__(self, "_arrays", dict(arrays or {}))
+        object.__setattr__(self, "_version", 0)
 
     def __getattr__(self, name):
         try:
@@ -141,12 +146,14 @@ class VariablesNamespace:
 
     def __setattr__(self, name, value):
         self._arrays[name] = value
+        object.__setattr__(self, "_version", self._version + 1)
 
     def __getitem__(self, name):
         return self._arrays[name]
 
     def __setitem__(self, name, value):
         self._arrays[name] = value
+        object.__setattr__(self, "_version", self._version + 1)
 
     def __contains__(self, name):
         return name in self._arrays
