@@ -616,6 +616,23 @@ int vh_advection(const vh_domain* d, const vh_settings* s, const vh_fields* f,
                  void* hIceMean, void* hSnowMean, void* Area, void* workspace,
                  size_t workspace_bytes, void* stream);
 
+/* Workspace bytes vh_advection_adjoint needs for this domain. */
+int vh_advection_adjoint_workspace_bytes(const vh_domain* d, size_t* bytes);
+
+/* Reverse mode of vh_advection (the derivative jax.grad takes through
+ * veris/advection.py:15-175 in the reference): given the adjoints of the
+ * advected hIceMean, hSnowMean and Area (any may be NULL = zero), writes the
+ * adjoints of the three input fields and of the ice velocities uIce, vIce
+ * (f->uIce, f->vIce are the advecting velocities, f->hIceMean, hSnowMean,
+ * Area the advected fields).  Each flux's derivative follows the branch its
+ * forward evaluation takes (upwind side, ratio or clipped slope ratio,
+ * superbee segment); d|x|/dx = sgn(x), 0 at 0. */
+int vh_advection_adjoint(const vh_domain* d, const vh_settings* s, const vh_fields* f,
+                         const void* adj_hIceMean, const void* adj_hSnowMean,
+                         const void* adj_Area, void* adj_hIceMean_in, void* adj_hSnowMean_in,
+                         void* adj_Area_in, void* adj_uIce, void* adj_vIce, void* workspace,
+                         size_t workspace_bytes, void* stream);
+
 /* clean_up_advection (veris/clean_up.py:5-39). */
 int vh_clean_up_advection(const vh_domain* d, const vh_settings* s, const vh_fields* f,
                           void* hIceMean, void* hSnowMean, void* Area, void* TSurf,

@@ -296,3 +296,104 @@ def test_batched_adjoint_equals_single_domain(cuda):
         _batched_vs_single(closed, dtype, 25, 10, members_to_check=(0, 1, 2))
         cyclic = [pre_evp(synth.global_latlon(34, 36, 20, -75.0, 80.0, member=m)) for m in range(2)]
         _batched_vs_single(cyclic, dtype, 23, 7, members_to_check=(0, 1))
+
+
+def _moving(kind):
+    """A state whose ice has moved for 60 substeps (the advecting velocities)."""
+    st = _state(kind)
+    vs = st.variables
+    vs.uIce, vs.vIce, vs.sigma1, vs.sigma2, vs.sigma12 = O.evp_solver(st, n_substeps=60)
+    return st
+
+
+@pytest.mark.parametrize("kind", ["global", "doubly_periodic", "regional"])
+def test_advection_adjoint_matches_torch_autograd(cuda, kind):
+    """vh_advection_adjoint (through veris_amd.adjoint.advect) against
+    torch.autograd of the torch restatement of the advection: gradients of a
+    weighted sum of the three advected fields w.r.t. the ice velocities and
+    the three input fields, FP64, 1e-12 relative L2 each; the forward equals
+    the drop-in bit for bit."""
+    st = _moving(kind)
+    gst = st.to(cuda)
+    rng = np.random.default_rng(19)
+    wts = [rng.standard_normal(st.shape) for _ in range(3)]
+    names = ("uIce", "vIce", "hIceMean", "hSnowMean", "Area")
+    leaves = {n: gst.variables[n].detach().clone().requires_grad_(True) for n in names}
+    outs = A.advect(gst, **leaves)
+    for a, b in zip(outs, K.Advection(gst)):
+        assert torch.equal(a, b)
+    sum((o * torch.from_numpy(w).to(cuda)).sum() for o, w in zip(outs, wts)).backward()
+    tv = {k: torch.from_numpy(np.asarray(v, dtype=np.float64)) for k, v in st.variables.items()}
+    tl = {n: tv[n].clone().requires_grad_(True) for n in names}
+    J = sum((TR.advection(tv, st.settings, tl["uIce"], tl["vIce"], tl[f]) * torch.from_numpy(w)).sum()
+            for f, w in zip(("hIceMean", "hSnowMean", "Area"), wts))
+    J.backward()
+    for n in names:
+        g, w = g2n(leaves[n].grad), tl[n].grad.numpy()
+        err = float(np.linalg.norm(g - w) / max(np.linalg.norm(w), 1e-300))
+        print(f"{kind} advection d/d{n}: rel L2 err {err:.3e} |grad| {np.linalg.norm(w):.3e}",
+              flush=True)
+        assert np.linalg.norm(w) > 0 and err <= 1e-12, (n, err)
+
+
+def test_thickness_misfit_gradients(cuda):
+    """The thickness misfit of a step's dynamics: the EVP subcycle moves the
+    ice (velocities depend on P*, the drags and the initial thickness through
+    mass and strength), the advection carries hIceMean and Area; loss =
+    |h' - h_target|^2 + |A' - A_target|^2 over the interior.  Gradients w.r.t.
+    P* (per cell), the initial hIceMean and the four drags against
+    torch.autograd of the torch restatements (1e-9), and central differences
+    of the GPU forward along 10 random directions (1e-5)."""
+    st = _state("global")
+    n = 40
+    gst = st.to(cuda)
+    rng = np.random.default_rng(23)
+    ht = np.asarray(st.variables.hIceMean) * (1 + 0.1 * rng.standard_normal(st.shape))
+    At = np.asarray(st.variables.Area) * (1 + 0.1 * rng.standard_normal(st.shape))
+    ht_g, At_g = torch.from_numpy(ht).to(cuda), torch.from_numpy(At).to(cuda)
+
+    def gpu_loss(p):
+        u, v, *_ = A.evp_subcycle(gst, n_substeps=n, ckpt_every=10, **p)
+        h1, _hs1, A1 = A.advect(gst, uIce=u, vIce=v, hIceMean=p["hIceMean"])
+        return (((h1 - ht_g)[2:-2, 2:-2] ** 2).sum() + ((A1 - At_g)[2:-2, 2:-2] ** 2).sum())
+
+    p = _params(st, cuda)
+    gpu_loss(p).backward()
+    got = {k: g2n(v.grad) for k, v in p.items()}
+
+    vs = {k: torch.from_numpy(np.asarray(v, dtype=np.float64)) for k, v in st.variables.items()}
+    s = st.settings
+    pc = _params(st, "cpu")
+    f = vs["fCori"]
+    ca = torch.where(f < 0, pc["airIceDrag_south"], pc["airIceDrag"])
+    cw = torch.where(f < 0, pc["waterIceDrag_south"], pc["waterIceDrag"])
+    pre = TR.pre_evp(vs, s, pc["pStar"], pc["hIceMean"], ca)
+    u, v, *_ = TR.evp(vs, s, pre, n, cw)
+    h1 = TR.advection(vs, s, u, v, pc["hIceMean"])
+    A1 = TR.advection(vs, s, u, v, vs["Area"])
+    J = (((h1 - torch.from_numpy(ht))[2:-2, 2:-2] ** 2).sum()
+         + ((A1 - torch.from_numpy(At))[2:-2, 2:-2] ** 2).sum())
+    J.backward()
+    for k, t in pc.items():
+        w = t.grad.numpy()
+        err = float(np.linalg.norm(got[k] - w) / max(np.linalg.norm(w), 1e-300))
+        print(f"thickness misfit d/d{k}: rel L2 err {err:.3e} |grad| {np.linalg.norm(w):.3e}",
+              flush=True)
+        assert np.linalg.norm(w) > 0 and err <= 1e-9, (k, err)
+
+    base = _params(st, cuda, requires_grad=False)
+    rng = np.random.default_rng(29)
+    worst = 0.0
+    for d in range(10):
+        dirs = {k: torch.tensor(rng.standard_normal(tuple(v.shape)), device=cuda,
+                                dtype=torch.float64) * v for k, v in base.items()}
+        eps = 1e-6
+        with torch.no_grad():
+            jp = float(gpu_loss({k: base[k] + eps * dirs[k] for k in base}))
+            jm = float(gpu_loss({k: base[k] - eps * dirs[k] for k in base}))
+        fd = (jp - jm) / (2 * eps)
+        ad = sum(float(np.sum(got[k] * g2n(dirs[k]))) for k in base)
+        rel = abs(fd - ad) / max(abs(ad), 1e-300)
+        worst = max(worst, rel)
+        print(f"direction {d}: fd={fd:.10e} adjoint={ad:.10e} rel={rel:.2e}", flush=True)
+    assert worst <= 1e-5

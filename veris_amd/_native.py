@@ -150,6 +150,8 @@ SIGNATURES = {
     "vh_pre_evp_adjoint_batched": [_DP, _I32, _SP, _FP, _P, ctypes.POINTER(EvpAdjointOut),
                                    ctypes.POINTER(PreEvpAdjointOut), _P],
     "vh_pre_evp": [_DP, _SP, _FP, _P, ctypes.POINTER(PreEvpOut), _P],
+    "vh_advection_adjoint_workspace_bytes": [_DP, ctypes.POINTER(_SZ)],
+    "vh_advection_adjoint": [_DP, _SP, _FP, _P, _P, _P, _P, _P, _P, _P, _P, _P, _SZ, _P],
     "vh_pre_evp_batched": [ctypes.POINTER(Batch), _SP, _FP, _P, ctypes.POINTER(PreEvpOut), _P],
     "vh_global_sums_batched": [ctypes.POINTER(Batch), _I32, ctypes.POINTER(_P), _P, _P, _P],
     "vh_ocean_to_ice": [_DP, _SP, _FP, _P, _P, _P, _P, _P, ctypes.POINTER(OceanToIceOut), _P],

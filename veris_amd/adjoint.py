@@ -277,3 +277,57 @@ def evp_subcycle_batched(ens, *, pStar=None, hIceMean=None, waterIceDrag=None,
     if args[0].dim() not in (0, 1):
         raise ValueError("pStar must be 0-d or a packed field")
     return _EvpSubcycleBatched.apply(ens, n, int(ckpt_every), int(tile), *args)
+
+
+_ADV_IN = ("uIce", "vIce", "hIceMean", "hSnowMean", "Area")
+
+
+class _Advect(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, state, u, v, h, hs, A):
+        c = K.Ctx(state)
+        keep = []
+        for name, t in zip(_ADV_IN, (u, v, h, hs, A)):
+            t = t.detach().to(c.dtype).contiguous()
+            c._check(name, t)
+            setattr(c.fields, name, t.data_ptr())
+            keep.append(t)
+        out = c.empty(3)
+        ws = c.workspace("vh_advection_workspace_bytes")
+        N.call("vh_advection", *c.args(), *(t.data_ptr() for t in out), ws.data_ptr(),
+               ws.numel(), c.stream)
+        # the reverse recomputes the forward fluxes from these inputs
+        ctx.save_for_backward(*keep, *[state.variables.get(n) for n in c.field_names])
+        ctx.c = c
+        ctx.metas = [(x.dtype, x.device) for x in (u, v, h, hs, A)]
+        return tuple(out)
+
+    @staticmethod
+    def backward(ctx, gh, ghs, gA):
+        ctx.saved_tensors  # raises if freed by an earlier backward or changed in place
+        c = ctx.c
+        lo = [None if g is None else g.to(c.dtype).contiguous() for g in (gh, ghs, gA)]
+        res = torch.empty((5,) + c.shape, dtype=c.dtype, device=c.device)
+        lu, lv, lh, lhs, lA = res.unbind(0)
+        ws = c.workspace("vh_advection_adjoint_workspace_bytes")
+        N.call("vh_advection_adjoint", *c.args(), *(None if g is None else g.data_ptr() for g in lo),
+               lh.data_ptr(), lhs.data_ptr(), lA.data_ptr(), lu.data_ptr(), lv.data_ptr(),
+               ws.data_ptr(), ws.numel(), c.stream)
+        grads = [t.to(dtype=dt, device=dev) if need else None
+                 for t, (dt, dev), need in zip((lu, lv, lh, lhs, lA), ctx.metas,
+                                               ctx.needs_input_grad[1:])]
+        return (None, *grads)
+
+
+def advect(state, uIce=None, vIce=None, hIceMean=None, hSnowMean=None, Area=None):
+    """Advection (veris/advection.py:15-175) of hIceMean, hSnowMean and Area
+    by the ice velocities -> (hIceMean, hSnowMean, Area), the drop-in's results,
+    differentiable through ``torch.autograd`` with respect to all five inputs
+    (each defaults to the state's variable).  The backward is the
+    hand-written adjoint of the flux-limited scheme: with
+    :func:`evp_subcycle` in front it gives the gradients of a thickness misfit
+    with respect to P*, the drag coefficients and the initial thickness."""
+    vs = state.variables
+    args = [vs[n] if x is None else x for n, x in zip(_ADV_IN, (uIce, vIce, hIceMean, hSnowMean,
+                                                                Area))]
+    return _Advect.apply(state, *args)
