@@ -58,18 +58,27 @@ B_ALG_WORDS = 38  # algorithmic words per EVP cell-substep (33 reads + 5 writes)
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "evp_substep_1deg_fp64_traffic.md")
 
 
-def measured_traffic(kernel: str):
-    """HBM bytes per launch of ``kernel`` from the rocprofv3 FETCH_SIZE /
-    WRITE_SIZE summary (copy-calibrated), or None if it was not measured."""
+KERNEL_PROFILE_FILE = os.path.join(ROOT, "profiles", "evp_substep_1deg_fp64.md")
+
+
+def _profile_value(path: str, key: str, kernel: str):
+    """The number on the line ``<key> <kernel> <value>`` of a profile summary,
+    or None if the summary has no such line."""
     try:
-        with open(TRAFFIC_FILE) as fh:
+        with open(path) as fh:
             for line in fh:
                 parts = line.split()
-                if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch" and parts[1] == kernel:
+                if len(parts) == 3 and parts[0] == key and parts[1] == kernel:
                     return float(parts[2])
     except OSError:
         pass
     return None
+
+
+def measured_traffic(kernel: str):
+    """HBM bytes per launch of ``kernel`` from the rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary (copy-calibrated), or None if it was not measured."""
+    return _profile_value(TRAFFIC_FILE, "traffic_bytes_per_launch", kernel)
 
 
 def cpu_baseline(n_substeps: int, min_seconds: float = 10.0, max_steps: int = 12) -> dict:
@@ -708,10 +717,18 @@ def main(argv=None) -> int:
             "traffic": measured_traffic(kernel),
             "traffic_source": os.path.relpath(TRAFFIC_FILE, ROOT),
             "avg_launch_us": per_launch_s * 1e6,
+            "frac_basis": "HIP-event time per substep launch over the timed region on the "
+                          "launch stream: kernel plus the gap to the next dependent launch",
             "algorithmic_bytes_per_launch": bytes_per_launch,
         },
         "finite": finite,
     }
+    kavg = _profile_value(KERNEL_PROFILE_FILE, "kernel_avg_us_rocprof", kernel)
+    if kavg:  # the kernel's own duration, from the committed rocprofv3 kernel trace
+        line["roofline"]["kernel_only"] = {
+            "avg_kernel_us": kavg, "achieved": bytes_per_launch / (kavg * 1e-6) / 1e9,
+            "frac": bytes_per_launch / (kavg * 1e-6) / 1e9 / HBM_PEAK_GBS,
+            "source": os.path.relpath(KERNEL_PROFILE_FILE, ROOT)}
     def guarded(fn, *a):
         # an extra measurement that fails is reported in its place; the
         # headline figures above still print
