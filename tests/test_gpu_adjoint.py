@@ -4,7 +4,9 @@ vh_evp_adjoint, vh_pre_evp_adjoint) through veris_amd.adjoint.evp_subcycle.
 * The checkpointed forward equals the drop-in path (pre-EVP kernels +
   vh_evp_solver) bit for bit.
 * Gradients of a scalar objective of all five outputs w.r.t. P* (per cell),
-  hIceMean (per cell) and the four drag coefficients equal torch.autograd of
+  hIceMean (per cell) and the four drag coefficients, at default settings and
+  with the water-stress turning angle and the metric terms, equal
+  torch.autograd of
   an independent torch FP64 restatement (tests/torch_ref.py, the stand-in for
   the reference's jax.grad) to 1e-9 relative: both are FP64 and differ only
   in rounding order.
@@ -41,6 +43,13 @@ def _state(kind):
         st = synth.global_latlon(31, 36, 20, -75.0, 80.0, veros_fill=False)
     elif kind == "regional":
         st = synth.regional(32, 0, 30, 22)
+    elif kind == "turning_metric":
+        # water-stress turning angle of 25 degrees and random metric terms
+        st = synth.global_latlon(31, 36, 20, -75.0, 80.0, sinWat=0.42261826174069944,
+                                 cosWat=0.9063077870366499)
+        r = np.random.default_rng(4)
+        for k in ("k1AtC", "k2AtC", "k1AtZ", "k2AtZ"):
+            st.variables[k] = r.normal(0, 1e-6, st.shape)
     else:
         raise ValueError(kind)
     pre_evp(st)
@@ -105,7 +114,7 @@ def _ref_grads(st, n, seed=3):
     return {k: v.grad.numpy() for k, v in p.items()}, [o.detach().numpy() for o in outs], float(J.detach())
 
 
-@pytest.mark.parametrize("kind", ["global", "doubly_periodic", "regional"])
+@pytest.mark.parametrize("kind", ["global", "doubly_periodic", "regional", "turning_metric"])
 def test_checkpointed_forward_is_the_drop_in(cuda, kind):
     st = _state(kind)
     gst = st.to(cuda)
@@ -127,7 +136,7 @@ def test_checkpointed_forward_is_the_drop_in(cuda, kind):
             assert_exact(f"{kind} {NAMES[q]} n={n} ckpt={ckpt}", g2n(outs[q]), g2n(want[q]))
 
 
-@pytest.mark.parametrize("kind", ["global", "doubly_periodic", "regional"])
+@pytest.mark.parametrize("kind", ["global", "doubly_periodic", "regional", "turning_metric"])
 def test_adjoint_matches_torch_autograd(cuda, kind):
     st = _state(kind)
     n = 25
@@ -156,10 +165,11 @@ def test_checkpoint_interval_is_irrelevant(cuda):
         assert np.array_equal(g1[k], g2[k]) and np.array_equal(g1[k], g3[k]), k
 
 
-def test_adjoint_finite_differences(cuda):
+@pytest.mark.parametrize("kind", ["global", "turning_metric"])
+def test_adjoint_finite_differences(cuda, kind):
     """20 random directions in (P*, h, Cw_N, Cw_S, Ca_N, Ca_S) space; central
     differences of the GPU forward with relative step 1e-6."""
-    st = _state("global")
+    st = _state(kind)
     n = 40
     got, _, _ = _gpu_grads(st, n, ckpt=10)
     gst = st.to(cuda)

@@ -22,9 +22,18 @@ from oracle import veris_np as O
 from veris_amd import synth
 
 
+TURN = dict(sinWat=0.42261826174069944, cosWat=0.9063077870366499)
+
+
 def _state(kind):
     if kind == "global":
         st = synth.global_latlon(31, 36, 20, -75.0, 80.0)
+    elif kind == "turning_metric":
+        # water-stress turning angle of 25 degrees and random metric terms
+        st = synth.global_latlon(31, 36, 20, -75.0, 80.0, **TURN)
+        r = np.random.default_rng(4)
+        for k in ("k1AtC", "k2AtC", "k1AtZ", "k2AtZ"):
+            st.variables[k] = r.normal(0, 1e-6, st.shape)
     else:
         st = synth.regional(32, 0, 30, 22)
     pre_evp(st)
@@ -37,7 +46,7 @@ def _torch_vars(st):
     return {k: torch.from_numpy(np.asarray(v, dtype=np.float64)) for k, v in st.variables.items()}
 
 
-@pytest.mark.parametrize("kind", ["global", "regional"])
+@pytest.mark.parametrize("kind", ["global", "regional", "turning_metric"])
 def test_forward_matches_oracle(kind):
     st = _state(kind)
     vs, s = _torch_vars(st), st.settings
@@ -59,7 +68,7 @@ def test_forward_matches_oracle(kind):
         assert np.abs(out[q].detach().numpy() - want[q]).max() <= 1e-13 * scale, q
 
 
-@pytest.mark.parametrize("kind", ["global", "regional"])
+@pytest.mark.parametrize("kind", ["global", "regional", "turning_metric"])
 def test_autograd_matches_finite_differences(kind):
     st = _state(kind)
     vs, s = _torch_vars(st), st.settings

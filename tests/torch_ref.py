@@ -5,8 +5,8 @@ It stands in for ``jax.grad`` of the reference (JAX is absent here) when the
 hand-written HIP adjoint is checked.  It follows the reference's expressions
 (veris/evp_solver.py:22-319, veris/dynamics_routines.py:5-302,
 veris/averaging.py:4-19, veris/area_mass.py:5-28, veris/dynsolver.py:7-84)
-at default settings without the turning term and basal drag, written with
-torch.roll, and defines the derivative of sqrt at 0 as 0 (the reference's
+at default settings (without basal drag), optionally with the water-stress
+turning term and the metric terms, written with torch.roll, and defines the derivative of sqrt at 0 as 0 (the reference's
 Delta = sqrt(Delta^2) and drag sqrt are not differentiable there; the HIP
 adjoint uses the same convention).
 """
@@ -96,6 +96,10 @@ def evp_substep(vs, sett, p, u, v, u0, v0, s1, s2, s12, cw):
     M, MU, MV = vs["iceMask"], vs["iceMaskU"], vs["iceMaskV"]
     dudx = (ip1(u) - u) * vs["recip_dxU"]
     dvdy = (jp1(v) - v) * vs["recip_dyV"]
+    kt = {k: vs.get(k) for k in ("k1AtC", "k2AtC", "k1AtZ", "k2AtZ")}
+    if kt["k1AtC"] is not None:  # metric terms (veris/dynamics_routines.py:151-169)
+        dudx = dudx + 0.5 * (jp1(v) + v) * kt["k2AtC"]
+        dvdy = dvdy + 0.5 * (ip1(u) + u) * kt["k1AtC"]
     e11 = dudx * vs["maskInC"]
     e22 = dvdy * vs["maskInC"]
     dudy = (u - jm1(u)) * vs["recip_dyU"]
@@ -104,7 +108,10 @@ def evp_substep(vs, sett, p, u, v, u0, v0, s1, s2, s12, cw):
     vavz = (v + im1(v)) * 0.5
     mz = M * im1(M)
     mz = mz * jm1(mz)
-    e12 = 0.5 * (dudy + dvdx) * mz
+    inner = dudy + dvdx
+    if kt["k1AtZ"] is not None:
+        inner = inner - kt["k1AtZ"] * vavz - kt["k2AtZ"] * uavz
+    e12 = 0.5 * inner * mz
     e12 = e12 + (2.0 * uavz * vs["recip_dyU"] * (MU - jm1(MU))
                  + 2.0 * vavz * vs["recip_dxV"] * (MV - im1(MV)))
     rpd = 1.0 / sett.PlasDefCoeff**2
@@ -146,8 +153,20 @@ def evp_substep(vs, sett, p, u, v, u0, v0, s1, s2, s12, cw):
     fu = mC * f * 0.5 * (u + ip1(u))
     cdU = 0.5 * (cd + im1(cd))
     cdV = 0.5 * (cd + jm1(cd))
-    fx = p["WindForcingX"] + cdU * sett.cosWat * vs["uOcean"] * p["AreaW"] + 0.5 * (fv + im1(fv))
-    fy = p["WindForcingY"] + cdV * sett.cosWat * vs["vOcean"] * p["AreaS"] - 0.5 * (fu + jm1(fu))
+    inx = cdU * sett.cosWat * vs["uOcean"]
+    iny = cdV * sett.cosWat * vs["vOcean"]
+    if sett.sinWat != 0:  # turning term (veris/evp_solver.py:119-139)
+        sg = torch.sign(f)
+        lmU = (mU != 0).to(mU.dtype)
+        lmV = (mV != 0).to(mV.dtype)
+        du0 = vs["uOcean"] - u
+        dv0 = vs["vOcean"] - v
+        X = cd * 0.5 * (du0 + jp1(du0)) + im1(cd) * 0.5 * (im1(du0) + jp1(im1(du0)))
+        Y = cd * 0.5 * (dv0 + ip1(dv0)) + jm1(cd) * 0.5 * (jm1(dv0) + ip1(jm1(dv0)))
+        inx = inx - sg * sett.sinWat * 0.5 * X * lmU
+        iny = iny + sg * sett.sinWat * 0.5 * Y * lmV
+    fx = p["WindForcingX"] + inx * p["AreaW"] + 0.5 * (fv + im1(fv))
+    fy = p["WindForcingY"] + iny * p["AreaS"] - 0.5 * (fu + jm1(fu))
     bU = sett.evpBeta * sett.recip_deltatDyn
     bP1 = bU + sett.recip_deltatDyn
     denU = mU * bP1 + p["AreaW"] * (cdU * sett.cosWat)

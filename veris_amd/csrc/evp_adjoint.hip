@@ -17,8 +17,8 @@
 // Non-smooth points: d sqrt(x)/dx is taken as 0 at x = 0 (Delta = 0 on
 // motionless or masked cells, zero relative velocity); the denominator guard
 // and the drag regime switch contribute no derivative.  Supported settings:
-// non-adaptive EVP, noSlip, no turning angle (sinWat = 0), no basal drag, no
-// metric terms -- the defaults.
+// non-adaptive EVP, noSlip, no basal drag, with or without the turning angle
+// and the metric terms.
 #include <algorithm>
 #include <cstdio>
 #include <cstdlib>
@@ -223,6 +223,11 @@ __global__ __launch_bounds__(RI* RJ) void adj_substep_kernel(const AdjArgs<T>* _
   static_assert(TI >= 1 && TJ >= 1, "tile too small");
   __shared__ T sU[NT], sV[NT], sM[NT], sMU[NT], sMV[NT], sA[NT], sB[NT], sE12[NT], sQ[NT], sZ[NT];
   __shared__ T sCD[NT], sNV[NT], sCXU[NT], sCXV[NT], sAB[NT], sBB[NT], sFU[NT], sFV[NT], sEP[NT];
+  // The water-stress turning term (sinWat != 0) borrows arrays outside their
+  // own lifetimes: the ocean-minus-ice velocities sit in sFU/sFV (A to C),
+  // their face sums in sAB/sBB (C to S2), the u/v-point factors
+  // nb sgn(f) lm Area of its adjoint in sU/sV (S1 to S2) and the velocity
+  // adjoint partials in sU/sV again (S3 to S4).
 
   const int bid = blockIdx.x;
   const int tile = (bid & 7) * per_xcd + (bid >> 3);
@@ -276,6 +281,10 @@ __global__ __launch_bounds__(RI* RJ) void adj_substep_kernel(const AdjArgs<T>* _
   sMV[t] = MVc;
   sA[t] = (u - uoc) * mInUc;
   sB[t] = (v - voc) * mInVc;
+  if (sc.turn) {
+    sFU[t] = uoc - u;
+    sFV[t] = voc - v;
+  }
   __syncthreads();
 
   // ---- B: e12 at z-points (the forward kernel's expression order)
@@ -287,7 +296,9 @@ __global__ __launch_bounds__(RI* RJ) void adj_substep_kernel(const AdjArgs<T>* _
     const T dvdx = (v - vim) * rdxV;
     const T vavz = (v + vim) * (T)0.5;
     const T mz = (Mc * sM[t - RJ]) * (sM[t - 1] * sM[t - RJ - 1]);
-    T e12 = ((T)0.5 * (dudy + dvdx)) * mz;
+    T inner = dudy + dvdx;
+    if (sc.kterms) inner = (inner - ldg(k.k1Z, c) * vavz) - ldg(k.k2Z, c) * uavz;
+    T e12 = ((T)0.5 * inner) * mz;
     const T hu = MUc - sMU[t - 1];
     const T hv = MVc - sMV[t - RJ];
     e12 = e12 + ((((T)2.0 * uavz) * rdyU) * hu + (((T)2.0 * vavz) * rdxV) * hv);
@@ -301,8 +312,14 @@ __global__ __launch_bounds__(RI* RJ) void adj_substep_kernel(const AdjArgs<T>* _
   T zeta = 0, dC = 0, ep = 0, em = 0, sa = 0, sb = 0, ww = 0, sw = 0;
   if (ar >= 1 && br >= 1 && ar <= RI - 2 && br <= RJ - 2) {
     const T uip = sU[t + RJ], vjp = sV[t + 1];
-    const T e11 = ((uip - u) * rdxU) * mInC;
-    const T e22 = ((vjp - v) * rdyV) * mInC;
+    const T dudx = (uip - u) * rdxU, dvdy = (vjp - v) * rdyV;
+    T e11 = dudx, e22 = dvdy;
+    if (sc.kterms) {
+      e11 = dudx + ((vjp + v) * (T)0.5) * ldg(k.k2C, c);
+      e22 = dvdy + ((uip + u) * (T)0.5) * ldg(k.k1C, c);
+    }
+    e11 = e11 * mInC;
+    e22 = e22 * mInC;
     const T qa = sQ[t] + sQ[t + RJ];
     const T qb = sQ[t + 1] + sQ[t + RJ + 1];
     const T e12c = ((T)0.25 * rrA) * (qa + qb);
@@ -320,6 +337,10 @@ __global__ __launch_bounds__(RI* RJ) void adj_substep_kernel(const AdjArgs<T>* _
     sw = sqrt(ww);
     const T cd = ((D * D) * ww > (T)sc.cDragMin2) ? D * sw : (T)sc.cDragMin;
     sCD[t] = cd * Mc;
+    if (sc.turn) {  // (uOcean - u) summed over j, j+1 and (vOcean - v) over i, i+1
+      sAB[t] = sFU[t] + sFU[t + 1];
+      sBB[t] = sFV[t] + sFV[t + RJ];
+    }
   }
   __syncthreads();
 
@@ -335,10 +356,10 @@ __global__ __launch_bounds__(RI* RJ) void adj_substep_kernel(const AdjArgs<T>* _
 
   // ---- S1: momentum update at the u- and v-point of this cell
   const T cw = (T)sc.cosWat, bU = (T)sc.betaU, bP1 = (T)sc.betaP1, rdt = (T)sc.rdt;
-  T nbU = 0, dbU = 0, cdAU = 0, nbV = 0, dbV = 0, cdAV = 0;
+  T nbU = 0, dbU = 0, nbV = 0, dbV = 0, aAW = 0, aAS = 0, aMU = 0, aMV = 0;
   if (ar >= 2 && br >= 2 && ar <= RI - 2 && br <= RJ - 2) {
-    cdAU = (T)0.5 * (sCD[t] + sCD[t - RJ]);
-    cdAV = (T)0.5 * (sCD[t] + sCD[t - 1]);
+    const T cdAU = (T)0.5 * (sCD[t] + sCD[t - RJ]);
+    const T cdAV = (T)0.5 * (sCD[t] + sCD[t - 1]);
     const T denU = mU * bP1 + AW * (cdAU * cw);
     const T denV = mV * bP1 + AS * (cdAV * cw);
     const bool gU = denU == (T)0, gV = denV == (T)0;
@@ -354,19 +375,43 @@ __global__ __launch_bounds__(RI* RJ) void adj_substep_kernel(const AdjArgs<T>* _
     sNV[t] = nbV;
     sCXU[t] = (nbU * uoc + dbU) * (AW * cw);
     sCXV[t] = (nbV * voc + dbV) * (AS * cw);
+    T turnU = 0, turnV = 0;
+    if (sc.turn) {  // turning term of the forcing (veris/evp_solver.py:119-139)
+      const T sgs = sgn(fc) * (T)sc.sinWat;
+      const T lmU = mU != (T)0 ? (T)1 : mU, lmV = mV != (T)0 ? (T)1 : mV;
+      const T Xt = (sCD[t] * (T)0.5) * sAB[t] + (sCD[t - RJ] * (T)0.5) * sAB[t - RJ];
+      const T Yt = (sCD[t] * (T)0.5) * sBB[t] + (sCD[t - 1] * (T)0.5) * sBB[t - 1];
+      turnU = -((sgs * (T)0.5) * Xt) * lmU;  // d ForcingX / d AreaW, turning part
+      turnV = ((sgs * (T)0.5) * Yt) * lmV;
+      sU[t] = nbU * sgn(fc) * lmU * AW;
+      sV[t] = nbV * sgn(fc) * lmV * AS;
+    }
+    aAW = nbU * (cdAU * cw * uoc + turnU) + dbU * (cdAU * cw);  // AreaW, AreaS adjoints
+    aAS = nbV * (cdAV * cw * voc + turnV) + dbV * (cdAV * cw);
+    aMU = nbU * (bU * u + rdt * uNm1) + dbU * bP1;  // SeaIceMassU, SeaIceMassV adjoints
+    aMV = nbV * (bU * v + rdt * vNm1) + dbV * bP1;
   }
   __syncthreads();
 
   // ---- S2: stresses, drag and Coriolis at c, sigma12 at z
   T Padd = 0, mCadd = 0, Db = 0, zbp = 0, epb0 = 0, emb0 = 0, dbp = 0;
-  T lo1 = 0, lo2 = 0, lo12 = 0;
+  T lo1 = 0, lo2 = 0, lo12 = 0, wU = 0, wV = 0;
   if (ar >= 3 && br >= 3 && ar <= RI - 3 && br <= RJ - 3) {
     const T Xc = sA[t], Xip = sA[t + RJ], Xjm = sA[t - 1];
     const T Yc = sB[t], Yjp = sB[t + 1], Yim = sB[t - RJ];
-    const T cdb = (T)0.5 * (sCXU[t] + sCXU[t + RJ]) + (T)0.5 * (sCXV[t] + sCXV[t + 1]);
+    T cdb = (T)0.5 * (sCXU[t] + sCXU[t + RJ]) + (T)0.5 * (sCXV[t] + sCXV[t + 1]);
+    if (sc.turn) {  // the drag of this c-point in the turning terms of its four faces
+      // (sAB/sBB of this cell are read before this thread overwrites them)
+      const T kus = sU[t] + sU[t + RJ], kvs = sV[t] + sV[t + 1];
+      cdb = cdb + (T)0.25 * (T)sc.sinWat * (sBB[t] * kvs - sAB[t] * kus);
+      wU = sCD[t] * kus;  // and the ice velocities in them, weighted by the drag
+      wV = sCD[t] * kvs;
+    }
     const T fvb = (T)0.5 * (sQ[t] + sQ[t + RJ]);
     const T fub = -(T)0.5 * (sNV[t] + sNV[t + 1]);
-    const T uip = sU[t + RJ], vjp = sV[t + 1];
+    // neighbour velocities (sU/sV hold the turning factors when sinWat != 0)
+    const T uip = sc.turn ? ldg(xu, cip) : sU[t + RJ];
+    const T vjp = sc.turn ? ldg(xv, cjp) : sV[t + 1];
     mCadd = fvb * fc * (T)0.5 * (v + vjp) + fub * fc * (T)0.5 * (u + uip);
     sFV[t] = fvb;
     sFU[t] = fub;
@@ -417,6 +462,14 @@ __global__ __launch_bounds__(RI* RJ) void adj_substep_kernel(const AdjArgs<T>* _
     sA[t] = (epb + emb) * rdxU * mInC;  // e11 adjoint (times its stencil factor)
     sB[t] = (epb - emb) * rdyV * mInC;  // e22 adjoint
     sQ[t] = dsqb * (T)sc.rpd * rrA;     // adjoint of rAz e12^2 at the four z-corners
+    if (sc.kterms) {  // e22 += uave k1C, e11 += vave k2C (veris/dynamics_routines.py:151-158)
+      sNV[t] = ((epb - emb) * mInC) * ldg(k.k1C, c) * (T)0.5;
+      sCXU[t] = ((epb + emb) * mInC) * ldg(k.k2C, c) * (T)0.5;
+    }
+    if (sc.turn) {
+      sU[t] = wU;
+      sV[t] = wV;
+    }
   }
   __syncthreads();
 
@@ -436,10 +489,20 @@ __global__ __launch_bounds__(RI* RJ) void adj_substep_kernel(const AdjArgs<T>* _
   const T A3 = (T)0.5 * mzc * rdxV + hvc * rdxV;
   const T A4ip = -(T)0.5 * mzip * rdxV_ip + hvip * rdxV_ip;
   const T fmc = fc * mC;
-  const T lu_out = nbU * bU * mU + (sAB[t] + sAB[t - RJ]) * mInUc + (sA[t - RJ] - sA[t]) +
-                   ec * A1 + ejp * A2jp + (T)0.5 * fmc * sFU[t] + (T)0.5 * fmc_im * sFU[t - RJ];
-  const T lv_out = nbV * bU * mV + (sBB[t] + sBB[t - 1]) * mInVc + (sB[t - 1] - sB[t]) +
-                   ec * A3 + eip * A4ip + (T)0.5 * fmc * sFV[t] + (T)0.5 * fmc_jm * sFV[t - 1];
+  T lu_out = nbU * bU * mU + (sAB[t] + sAB[t - RJ]) * mInUc + (sA[t - RJ] - sA[t]) +
+             ec * A1 + ejp * A2jp + (T)0.5 * fmc * sFU[t] + (T)0.5 * fmc_im * sFU[t - RJ];
+  T lv_out = nbV * bU * mV + (sBB[t] + sBB[t - 1]) * mInVc + (sB[t - 1] - sB[t]) +
+             ec * A3 + eip * A4ip + (T)0.5 * fmc * sFV[t] + (T)0.5 * fmc_jm * sFV[t - 1];
+  if (sc.kterms) {  // the metric terms of e11, e22 (c) and e12 (z)
+    const T k1Z = ldg(k.k1Z, c), k2Z = ldg(k.k2Z, c);
+    const T k2Z_jp = ldg(k.k2Z, cjp), k1Z_ip = ldg(k.k1Z, cip);
+    lu_out = lu_out + (sNV[t] + sNV[t - RJ]) - (T)0.25 * (ec * mzc * k2Z + ejp * mzjp * k2Z_jp);
+    lv_out = lv_out + (sCXU[t] + sCXU[t - 1]) - (T)0.25 * (ec * mzc * k1Z + eip * mzip * k1Z_ip);
+  }
+  if (sc.turn) {  // the ice velocity in the turning terms of the four neighbouring faces
+    lu_out = lu_out + (T)0.25 * (T)sc.sinWat * (sU[t] + sU[t - 1]);
+    lv_out = lv_out - (T)0.25 * (T)sc.sinWat * (sV[t] + sV[t - RJ]);
+  }
   stg(L.out[0], c, lu_out);
   stg(L.out[1], c, lv_out);
   stg(L.out[2], c, lo1);
@@ -450,17 +513,17 @@ __global__ __launch_bounds__(RI* RJ) void adj_substep_kernel(const AdjArgs<T>* _
   if (fc < (T)0) stg(acc.DS, c, aDS + Db);
   else stg(acc.DN, c, aDN + Db);
   if (comp) {  // nbU = nbV = 0 elsewhere
-    stg(acc.mU, c, amU + (nbU * (bU * u + rdt * uNm1) + dbU * bP1));
+    stg(acc.mU, c, amU + aMU);
     stg(acc.WFX, c, aWX + nbU);
-    stg(acc.mV, c, amV + (nbV * (bU * v + rdt * vNm1) + dbV * bP1));
+    stg(acc.mV, c, amV + aMV);
     stg(acc.WFY, c, aWY + nbV);
     if (acc.u0) {  // optional outputs (uniform branch)
       stg(acc.u0, c, ldg(acc.u0, c) + nbU * mU * rdt);
       stg(acc.v0, c, ldg(acc.v0, c) + nbV * mV * rdt);
     }
     if (acc.AW) {
-      stg(acc.AW, c, ldg(acc.AW, c) + (nbU * (cdAU * cw * uoc) + dbU * (cdAU * cw)));
-      stg(acc.AS, c, ldg(acc.AS, c) + (nbV * (cdAV * cw * voc) + dbV * (cdAV * cw)));
+      stg(acc.AW, c, ldg(acc.AW, c) + aAW);
+      stg(acc.AS, c, ldg(acc.AS, c) + aAS);
     }
   }
 }
@@ -652,10 +715,10 @@ int check_adj(const vh_domain* d, const vh_settings* s, const vh_fields* f, int 
   VH_REQUIRE(s->useAdaptiveEVP == 0.0 && s->noSlip != 0.0 && s->secondOrderBC == 0.0 &&
                  s->useEVP != 0.0,
              VH_EUNSUPPORTED, "adjoint: only non-adaptive noSlip EVP is supported");
-  VH_REQUIRE(s->sinWat == 0.0, VH_EUNSUPPORTED, "adjoint: turning angle (sinWat != 0) not supported");
   VH_REQUIRE(s->basalDragK2 == 0.0, VH_EUNSUPPORTED, "adjoint: basal drag not supported");
-  VH_REQUIRE(!f->k1AtC && !f->k2AtC && !f->k1AtZ && !f->k2AtZ, VH_EUNSUPPORTED,
-             "adjoint: metric terms k1/k2 not supported (pass NULL for all-zero fields)");
+  const bool kt = f->k1AtC || f->k2AtC || f->k1AtZ || f->k2AtZ;
+  VH_REQUIRE(!kt || (f->k1AtC && f->k2AtC && f->k1AtZ && f->k2AtZ), VH_EINVAL,
+             "adjoint: k1AtC/k2AtC/k1AtZ/k2AtZ must be all set or all NULL");
   const void* req[kEvpMaxRequired];
   const int nreq = evp_required_fields(f, req);
   for (int q = 0; q < nreq; ++q)
@@ -663,10 +726,14 @@ int check_adj(const vh_domain* d, const vh_settings* s, const vh_fields* f, int 
   return VH_OK;
 }
 
+inline int kterms_of(const vh_fields* f) {
+  return (f->k1AtC || f->k2AtC || f->k1AtZ || f->k2AtZ) ? 1 : 0;
+}
+
 template <typename T>
 EvpArgBlock<T> proto_block(const Geom& g, const vh_settings* s, const vh_fields* f) {
   EvpArgBlock<T> ab{};
-  fill_evp_scalars(s, 0, 0, &ab.sc);
+  fill_evp_scalars(s, kterms_of(f), 0, &ab.sc);
   fill_evp_const(f, &ab.c);
   ab.g = g;
   ab.nlaunch = 1;
@@ -780,7 +847,7 @@ int adjoint(const Geom& g, const vh_domain* d, const vh_settings* s, const vh_fi
              L.total, wsb);
   AdjArgs<T> av{};
   fill_evp_const(f, &av.c);
-  fill_evp_scalars(s, 0, 0, &av.sc);
+  fill_evp_scalars(s, kterms_of(f), 0, &av.sc);
   av.g = g;
   av.acc = AdjAcc<T>{(T*)o->adj_SeaIceStrength, (T*)o->adj_SeaIceMassC, (T*)o->adj_SeaIceMassU,
                      (T*)o->adj_SeaIceMassV, (T*)o->adj_AreaW, (T*)o->adj_AreaS,
