@@ -20,6 +20,8 @@ vh_evp_adjoint, vh_pre_evp_adjoint) through veris_amd.adjoint.evp_subcycle.
 
 from __future__ import annotations
 
+import ctypes
+
 import numpy as np
 import pytest
 import torch
@@ -27,6 +29,7 @@ import torch
 import torch_ref as TR
 from gpu_helpers import assert_close, assert_exact, g2n, pre_evp, report
 from oracle import veris_np as O
+from veris_amd import _native as N
 from veris_amd import adjoint as A
 from veris_amd import kernels as K
 from veris_amd import synth
@@ -43,6 +46,13 @@ def _state(kind):
         st = synth.global_latlon(31, 36, 20, -75.0, 80.0, veros_fill=False)
     elif kind == "regional":
         st = synth.regional(32, 0, 30, 22)
+    elif kind == "unpacked":
+        # metrics varying along x: the kernels read the full constant fields
+        # instead of the packed mask bytes and metric rows
+        st = synth.global_latlon(31, 36, 20, -75.0, 80.0)
+        ramp = 1.0 + 1e-3 * np.arange(st.shape[0])[:, None]
+        for k in ("recip_dxU", "recip_dyV", "rAz", "dxV"):
+            st.variables[k] = np.asarray(st.variables[k]) * ramp
     elif kind == "turning_metric":
         # water-stress turning angle of 25 degrees and random metric terms
         st = synth.global_latlon(31, 36, 20, -75.0, 80.0, sinWat=0.42261826174069944,
@@ -136,7 +146,8 @@ def test_checkpointed_forward_is_the_drop_in(cuda, kind):
             assert_exact(f"{kind} {NAMES[q]} n={n} ckpt={ckpt}", g2n(outs[q]), g2n(want[q]))
 
 
-@pytest.mark.parametrize("kind", ["global", "doubly_periodic", "regional", "turning_metric"])
+@pytest.mark.parametrize("kind", ["global", "doubly_periodic", "regional", "turning_metric",
+                                  "unpacked"])
 def test_adjoint_matches_torch_autograd(cuda, kind):
     st = _state(kind)
     n = 25
@@ -154,6 +165,68 @@ def test_adjoint_matches_torch_autograd(cuda, kind):
             continue
         assert np.linalg.norm(w) > 0, k
         assert err <= 1e-9, f"{kind} d/d{k}: {err:.3e}"
+
+
+def test_every_reverse_output_through_the_c_abi(cuda):
+    """vh_evp_adjoint called directly with every output requested, including
+    the optional AreaW/AreaS and uNm1/vNm1 adjoints that the autograd wrapper
+    leaves out, against torch autograd with the pre-EVP fields, the initial
+    state and the velocities of the previous time level as separate leaves;
+    turning angle and metric terms on.  Bound: 1e-9 relative L2 (both FP64)."""
+    st = _state("turning_metric")
+    n, ce = 12, 5
+    gst = st.to(cuda)
+    vs = gst.variables
+    vs.SeaIceMassC, vs.SeaIceMassU, vs.SeaIceMassV = K.SeaIceMass(gst)
+    vs.AreaW, vs.AreaS = K.AreaWS(gst)
+    vs.WindForcingX, vs.WindForcingY = K.WindForcingXY(gst)
+    vs.SeaIceStrength = K.SeaIceStrength(gst)
+    c = K.Ctx(gst)
+    nbytes = ctypes.c_size_t(0)
+    N.call("vh_evp_adjoint_workspace_bytes", ctypes.byref(c.dom), n, ce, ctypes.byref(nbytes))
+    ws = torch.empty(int(nbytes.value), dtype=torch.uint8, device=cuda)
+    out = torch.empty((5,) + c.shape, dtype=torch.float64, device=cuda).unbind(0)
+    N.call("vh_evp_forward_checkpointed", *c.args(), n, ce, *(t.data_ptr() for t in out),
+           ws.data_ptr(), ws.numel(), c.stream)
+    w = [torch.tensor(x, device=cuda) for x in _weights(st, 5)]
+    ebuf = torch.empty((len(N.EVP_ADJOINT_NAMES),) + c.shape, dtype=torch.float64, device=cuda)
+    evp = dict(zip(N.EVP_ADJOINT_NAMES, ebuf.unbind(0)))
+    eo = N.EvpAdjointOut(**{"adj_" + k: v.data_ptr() for k, v in evp.items()})
+    N.call("vh_evp_adjoint", *c.args(), n, ce, *(x.data_ptr() for x in w), ctypes.byref(eo),
+           ws.data_ptr(), ws.numel(), c.stream)
+    torch.cuda.synchronize()
+
+    # torch: J = sum of the outputs weighted by w, every input a leaf
+    tv = {k: torch.from_numpy(np.asarray(v, dtype=np.float64)) for k, v in st.variables.items()}
+    s = st.settings
+    f = tv["fCori"]
+    pre = {k: torch.from_numpy(np.asarray(st.variables[k], dtype=np.float64)).requires_grad_(True)
+           for k in ("SeaIceMassC", "SeaIceMassU", "SeaIceMassV", "AreaW", "AreaS",
+                     "WindForcingX", "WindForcingY", "SeaIceStrength")}
+    cwN = torch.tensor(float(s.waterIceDrag), dtype=torch.float64, requires_grad=True)
+    cwS = torch.tensor(float(s.waterIceDrag_south), dtype=torch.float64, requires_grad=True)
+    cw = torch.where(f < 0, cwS, cwN)
+    st0 = {k: tv[k].clone().requires_grad_(True) for k in NAMES}
+    u0 = tv["uIce"].clone().requires_grad_(True)
+    v0 = tv["vIce"].clone().requires_grad_(True)
+    q = [st0[k] for k in NAMES]
+    for _ in range(n):
+        q = list(TR.evp_substep(tv, s, pre, *q[:2], u0, v0, *q[2:], cw))
+    J = sum((o * torch.from_numpy(g2n(x))).sum() for o, x in zip(q, w))
+    J.backward()
+    want = {**{k: st0[k].grad for k in NAMES}, "uNm1": u0.grad, "vNm1": v0.grad,
+            **{k: pre[k].grad for k in pre}}
+    for k, ref in want.items():
+        got = g2n(evp[k])
+        err = float(np.linalg.norm(got - ref.numpy()) / max(float(ref.norm()), 1e-300))
+        print(f"adj_{k}: rel L2 err {err:.3e} |grad| {float(ref.norm()):.3e}", flush=True)
+        assert float(ref.norm()) > 0, k
+        assert err <= 1e-9, k
+    for k, leaf in (("waterDrag_north", cwN), ("waterDrag_south", cwS)):
+        got = float(evp[k].double().sum())
+        ref = float(leaf.grad)
+        print(f"adj_{k} summed: {got:.12e} torch {ref:.12e}", flush=True)
+        assert abs(got - ref) <= 1e-9 * abs(ref), k
 
 
 def test_checkpoint_interval_is_irrelevant(cuda):

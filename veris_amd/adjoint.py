@@ -26,6 +26,7 @@ from . import _native as N
 from . import kernels as K
 
 _DRAGS = ("waterIceDrag", "waterIceDrag_south", "airIceDrag", "airIceDrag_south")
+_EVP_UNUSED = ("AreaW", "AreaS", "uNm1", "vNm1")
 _PRE_OUT = ("SeaIceMassC", "SeaIceMassU", "SeaIceMassV", "AreaW", "AreaS", "WindForcingX",
             "WindForcingY", "SeaIceStrength")
 
@@ -103,7 +104,9 @@ class _EvpSubcycle(torch.autograd.Function):
         grads = [None if g is None else g.to(dt).contiguous() for g in (gu, gv, gs1, gs2, gs12)]
         ebuf = torch.empty((len(N.EVP_ADJOINT_NAMES),) + shape, dtype=dt, device=c.device)
         evp = dict(zip(N.EVP_ADJOINT_NAMES, ebuf.unbind(0)))
-        eo = N.EvpAdjointOut(**{"adj_" + k: v.data_ptr() for k, v in evp.items()})
+        # the AreaW/AreaS and uNm1/vNm1 adjoints feed no gradient returned here
+        eo = N.EvpAdjointOut(**{"adj_" + k: (None if k in _EVP_UNUSED else v.data_ptr())
+                                for k, v in evp.items()})
         N.call("vh_evp_adjoint", *c.args(), ctx.n, ctx.ckpt,
                *(None if g is None else g.data_ptr() for g in grads), ctypes.byref(eo),
                ws.data_ptr(), ws.numel(), c.stream)
@@ -222,7 +225,9 @@ class _EvpSubcycleBatched(torch.autograd.Function):
         grads = [None if g is None else g.to(dt).contiguous() for g in (gu, gv, gs1, gs2, gs12)]
         ebuf = torch.empty((len(N.EVP_ADJOINT_NAMES), total), dtype=dt, device=dev)
         evp = dict(zip(N.EVP_ADJOINT_NAMES, ebuf.unbind(0)))
-        eo = N.EvpAdjointOut(**{"adj_" + k: v.data_ptr() for k, v in evp.items()})
+        # the AreaW/AreaS and uNm1/vNm1 adjoints feed no gradient returned here
+        eo = N.EvpAdjointOut(**{"adj_" + k: (None if k in _EVP_UNUSED else v.data_ptr())
+                                for k, v in evp.items()})
         N.call("vh_evp_adjoint_batched", ctypes.byref(dom), ctx.nm, ctypes.byref(sett),
                ctypes.byref(fields), ctx.n, ctx.ckpt,
                *(None if g is None else g.data_ptr() for g in grads), ctypes.byref(eo),

@@ -253,20 +253,61 @@ __global__ __launch_bounds__(RI* RJ) void adj_substep_kernel(const AdjArgs<T>* _
                    (j < g.NY);
   const bool comp = is_computed(g, gi, gj);
 
-  // ---- A: every load of this cell, issued together
+  // ---- A: every load of this cell, issued together.  With the packed step
+  //      constants of the forward launches (uniform flag, set when every mask
+  //      is 0/1 and every metric constant along i), the masks come from one
+  //      byte and the metrics from rows in LDS, as in the forward kernel.
+  const bool packed = blk->packed != 0;
+  constexpr int RPT = (kEvpRowFields * RJ + NT - 1) / NT;  // rows per thread
+  __shared__ T sRow[kEvpRowFields * RJ];
+  T rowv[RPT];
+  if (packed) {
+    const int roff = (int)blockIdx.y * g.NY;  // rows of member blockIdx.y
+#pragma unroll
+    for (int r = 0; r < RPT; ++r) {
+      const int q = t + r * NT;
+      rowv[r] = (T)0;
+      if (q < kEvpRowFields * RJ) {
+        const int mq = q / RJ, bq = q & (RJ - 1);
+        rowv[r] = ldg(blk->rows + (long long)mq * blk->rstride + roff,
+                      (unsigned)wrap(j0 + bq - R, g.NY));
+      }
+    }
+  }
   const T* xu = blk->in[0];
   const T* xv = blk->in[1];
   const T u = ldg(xu, c), v = ldg(xv, c);
-  const T Mc = ldg(k.M, c), MUc = ldg(k.MU, c), MVc = ldg(k.MV, c);
+  T Mc, MUc, MVc, mInC, mInUc, mInVc;
+  if (packed) {
+    const unsigned pkb = ldg(blk->pk, c);
+    Mc = (T)(pkb & 1u);
+    MUc = (T)((pkb >> 1) & 1u);
+    MVc = (T)((pkb >> 2) & 1u);
+    mInC = (T)((pkb >> 3) & 1u);
+    mInUc = (T)((pkb >> 4) & 1u);
+    mInVc = (T)((pkb >> 5) & 1u);
+  } else {
+    Mc = ldg(k.M, c);
+    MUc = ldg(k.MU, c);
+    MVc = ldg(k.MV, c);
+    mInC = ldg(k.mInC, c);
+    mInUc = ldg(k.mInU, c);
+    mInVc = ldg(k.mInV, c);
+  }
   const T uoc = ldg(k.uOc, c), voc = ldg(k.vOc, c);
-  const T mInUc = ldg(k.mInU, c), mInVc = ldg(k.mInV, c);
-  const T rdyU = ldg(k.rdyU, c), rdxV = ldg(k.rdxV, c), rAz = ldg(k.rAz, c);
-  const T rdxU = ldg(k.rdxU, c), rdyV = ldg(k.rdyV, c), mInC = ldg(k.mInC, c);
-  const T rrA = ldg(k.rrA, c), P = ldg(k.P, c), fc = ldg(k.fC, c);
+  // metrics from the full fields (not packed); met(q, .) picks the source
+  T g_met[kEvpRowFields] = {};
+  if (!packed) {
+    const T* const mp[kEvpRowFields] = {k.rdyU, k.rdxV, k.rAz, k.rdxU, k.rdyV, k.rrA,
+                                        k.dyV,  k.dxU,  k.dxV, k.dyU,  k.rrAu, k.rrAv};
+#pragma unroll
+    for (int q = 0; q < kEvpRowFields; ++q) g_met[q] = ldg(mp[q], c);
+  }
+  auto met = [&](int q) -> T { return packed ? sRow[q * RJ + br] : g_met[q]; };
+  const T P = ldg(k.P, c), fc = ldg(k.fC, c);
   const T un = ldg(blk->out[0], c), vn = ldg(blk->out[1], c);
   const T mU = ldg(k.massU, c), mV = ldg(k.massV, c), AW = ldg(k.AreaW, c), AS = ldg(k.AreaS, c);
-  const T rrAu = ldg(k.rrAu, c), rrAv = ldg(k.rrAv, c), mC = ldg(k.massC, c);
-  const T dyV = ldg(k.dyV, c), dxU = ldg(k.dxU, c), dxV = ldg(k.dxV, c), dyU = ldg(k.dyU, c);
+  const T mC = ldg(k.massC, c);
   const T l1 = ldg(L.in[2], c), l2 = ldg(L.in[3], c), l12 = ldg(L.in[4], c);
   const T lu = gather_filled(g, L.in[0] + mo, gi, gj), lv = gather_filled(g, L.in[1] + mo, gi, gj);
   // water drag factor of either hemisphere, selected by value (a select of
@@ -274,6 +315,11 @@ __global__ __launch_bounds__(RI* RJ) void adj_substep_kernel(const AdjArgs<T>* _
   const T Dn = (T)sc.cwN * (T)sc.rhoSea, Ds = (T)sc.cwS * (T)sc.rhoSea;
   const T D = fc < (T)0 ? Ds : Dn;
 
+  if (packed) {
+#pragma unroll
+    for (int r = 0; r < RPT; ++r)
+      if (t + r * NT < kEvpRowFields * RJ) sRow[t + r * NT] = rowv[r];
+  }
   sU[t] = u;
   sV[t] = v;
   sM[t] = Mc;
@@ -290,6 +336,7 @@ __global__ __launch_bounds__(RI* RJ) void adj_substep_kernel(const AdjArgs<T>* _
   // ---- B: e12 at z-points (the forward kernel's expression order)
   T e12r = 0;
   if (ar >= 1 && br >= 1) {
+    const T rdyU = met(0), rdxV = met(1), rAz = met(2);
     const T ujm = sU[t - 1], vim = sV[t - RJ];
     const T dudy = (u - ujm) * rdyU;
     const T uavz = (u + ujm) * (T)0.5;
@@ -312,6 +359,7 @@ __global__ __launch_bounds__(RI* RJ) void adj_substep_kernel(const AdjArgs<T>* _
   T zeta = 0, dC = 0, ep = 0, em = 0, sa = 0, sb = 0, ww = 0, sw = 0;
   if (ar >= 1 && br >= 1 && ar <= RI - 2 && br <= RJ - 2) {
     const T uip = sU[t + RJ], vjp = sV[t + 1];
+    const T rdxU = met(3), rdyV = met(4);
     const T dudx = (uip - u) * rdxU, dvdy = (vjp - v) * rdyV;
     T e11 = dudx, e22 = dvdy;
     if (sc.kterms) {
@@ -322,7 +370,7 @@ __global__ __launch_bounds__(RI* RJ) void adj_substep_kernel(const AdjArgs<T>* _
     e22 = e22 * mInC;
     const T qa = sQ[t] + sQ[t + RJ];
     const T qb = sQ[t + 1] + sQ[t + RJ + 1];
-    const T e12c = ((T)0.25 * rrA) * (qa + qb);
+    const T e12c = ((T)0.25 * met(5)) * (qa + qb);
     ep = e11 + e22;
     em = e11 - e22;
     const TD dsq = (TD)ep * (TD)ep + (TD)sc.rpd * ((TD)em * (TD)em + (TD)4.0 * (TD)e12c);
@@ -350,8 +398,14 @@ __global__ __launch_bounds__(RI* RJ) void adj_substep_kernel(const AdjArgs<T>* _
   const T aWX = ldg(acc.WFX, c), aWY = ldg(acc.WFY, c), aDN = ldg(acc.DN, c), aDS = ldg(acc.DS, c);
   const T uNm1 = ldg(k.uNm1, c), vNm1 = ldg(k.vNm1, c);
   // neighbours' constants used by S4
-  const T rAz_jp = ldg(k.rAz, cjp), rdyU_jp = ldg(k.rdyU, cjp);
-  const T rAz_ip = ldg(k.rAz, cip), rdxV_ip = ldg(k.rdxV, cip);
+  // (the packed rows hold these at column br + 1 and, constant along i, br)
+  T rAz_jp = 0, rdyU_jp = 0, rAz_ip = 0, rdxV_ip = 0;
+  if (!packed) {
+    rAz_jp = ldg(k.rAz, cjp);
+    rdyU_jp = ldg(k.rdyU, cjp);
+    rAz_ip = ldg(k.rAz, cip);
+    rdxV_ip = ldg(k.rdxV, cip);
+  }
   const T fmc_im = ldg(k.fC, cim) * ldg(k.massC, cim), fmc_jm = ldg(k.fC, cjm) * ldg(k.massC, cjm);
 
   // ---- S1: momentum update at the u- and v-point of this cell
@@ -369,8 +423,8 @@ __global__ __launch_bounds__(RI* RJ) void adj_substep_kernel(const AdjArgs<T>* _
     dbU = gU ? (T)0 : -lcu * un * rU;
     nbV = lcv * MVc * rV;
     dbV = gV ? (T)0 : -lcv * vn * rV;
-    sA[t] = nbU * rrAu;  // X: adjoint of the u-point stress divergence
-    sB[t] = nbV * rrAv;  // Y: of the v-point one
+    sA[t] = nbU * met(10);  // X: adjoint of the u-point stress divergence (rrAu)
+    sB[t] = nbV * met(11);  // Y: of the v-point one (rrAv)
     sQ[t] = nbU;
     sNV[t] = nbV;
     sCXU[t] = (nbU * uoc + dbU) * (AW * cw);
@@ -421,8 +475,8 @@ __global__ __launch_bounds__(RI* RJ) void adj_substep_kernel(const AdjArgs<T>* _
     Db = drag ? cbe * sw * (T)sc.rhoSea : (T)0;
     sAB[t] = wb * (T)0.5 * sa;
     sBB[t] = wb * (T)0.5 * sb;
-    const T s11b = dyV * (Xc - Xip);
-    const T s22b = dxU * (Yc - Yjp);
+    const T s11b = met(6) * (Xc - Xip);  // dyV
+    const T s22b = met(7) * (Yc - Yjp);  // dxU
     const T s1t = l1 + (T)0.5 * (s11b + s22b);
     const T s2t = l2 + (T)0.5 * (s11b - s22b);
     const T H1 = s1t * (T)sc.den1 * Mc, H2 = s2t * (T)sc.den2 * Mc;
@@ -435,7 +489,7 @@ __global__ __launch_bounds__(RI* RJ) void adj_substep_kernel(const AdjArgs<T>* _
     epb0 = H1 * (T)2 * zeta;
     emb0 = H2 * (T)0.5 * zeta;
     dbp = pb * pfac * zeta;
-    const T s12t = l12 + dxV * (Xjm - Xc) + dyU * (Yim - Yc);
+    const T s12t = l12 + met(8) * (Xjm - Xc) + met(9) * (Yim - Yc);  // dxV, dyU
     const T G = s12t * (T)sc.den2;
     lo12 = G * (T)sc.alphaM1;
     const T nz = (Mc + sM[t - RJ]) + (sM[t - 1] + sM[t - RJ - 1]);
@@ -459,9 +513,9 @@ __global__ __launch_bounds__(RI* RJ) void adj_substep_kernel(const AdjArgs<T>* _
     const T dsqb = dC > (T)0 ? Dbar * (T)0.5 / dC : (T)0;
     const T epb = epb0 + dsqb * (T)2 * ep;
     const T emb = emb0 + dsqb * (T)sc.rpd * (T)2 * em;
-    sA[t] = (epb + emb) * rdxU * mInC;  // e11 adjoint (times its stencil factor)
-    sB[t] = (epb - emb) * rdyV * mInC;  // e22 adjoint
-    sQ[t] = dsqb * (T)sc.rpd * rrA;     // adjoint of rAz e12^2 at the four z-corners
+    sA[t] = (epb + emb) * met(3) * mInC;  // e11 adjoint (times its stencil factor)
+    sB[t] = (epb - emb) * met(4) * mInC;  // e22 adjoint
+    sQ[t] = dsqb * (T)sc.rpd * met(5);    // adjoint of rAz e12^2 at the four z-corners
     if (sc.kterms) {  // e22 += uave k1C, e11 += vave k2C (veris/dynamics_routines.py:151-158)
       sNV[t] = ((epb - emb) * mInC) * ldg(k.k1C, c) * (T)0.5;
       sCXU[t] = ((epb + emb) * mInC) * ldg(k.k2C, c) * (T)0.5;
@@ -480,6 +534,13 @@ __global__ __launch_bounds__(RI* RJ) void adj_substep_kernel(const AdjArgs<T>* _
     return sEP[z] + qs * (T)2 * rz * sE12[z];
   };
   auto mzf = [&](int z) { return (sM[z] * sM[z - RJ]) * (sM[z - 1] * sM[z - RJ - 1]); };
+  const T rdyU = met(0), rdxV = met(1), rAz = met(2);
+  if (packed) {
+    rAz_jp = sRow[2 * RJ + br + 1];
+    rdyU_jp = sRow[br + 1];
+    rAz_ip = rAz;
+    rdxV_ip = rdxV;
+  }
   const T ec = eadj(t, rAz), ejp = eadj(t + 1, rAz_jp), eip = eadj(t + RJ, rAz_ip);
   const T mzc = mzf(t), mzjp = mzf(t + 1), mzip = mzf(t + RJ);
   const T huc = MUc - sMU[t - 1], hujp = sMU[t + 1] - MUc;
@@ -533,7 +594,7 @@ struct AdjTileCfg {
 };
 // 512- and 768-thread shapes: the FP64 kernel needs ~156 VGPRs, more than a
 // 1024-thread workgroup can give each lane.
-constexpr AdjTileCfg kAdjTiles[] = {{16, 32}, {24, 32}, {12, 64}};
+constexpr AdjTileCfg kAdjTiles[] = {{16, 32}, {24, 32}, {12, 64}, {32, 32}};
 constexpr int kAdjNumTiles = sizeof(kAdjTiles) / sizeof(kAdjTiles[0]);
 
 template <typename T, int RI, int RJ>
@@ -553,6 +614,10 @@ void adj_launch(int cfg, const AdjArgs<T>* a, const EvpArgBlock<T>* b, LamSet<T>
   switch (cfg) {
     case 1: adj_launch_shape<T, 24, 32>(a, b, L, g, n_members, st); break;
     case 2: adj_launch_shape<T, 12, 64>(a, b, L, g, n_members, st); break;
+    case 3:  // 1024 threads: FP32 only (the FP64 kernel's registers do not fit)
+      if constexpr (sizeof(T) == 4) adj_launch_shape<T, 32, 32>(a, b, L, g, n_members, st);
+      else adj_launch_shape<T, 24, 32>(a, b, L, g, n_members, st);
+      break;
     default: adj_launch_shape<T, 16, 32>(a, b, L, g, n_members, st); break;
   }
 }
@@ -664,16 +729,32 @@ T* wsf(void* ws, size_t off, size_t fb, int q) {
   return (T*)((char*)ws + off + (size_t)q * fb);
 }
 
-// Tile shape of the reverse kernel (index into kAdjTiles, default 24 x 32,
-// the fastest measured at 1 degree; environment VERIS_ADJ_TILE overrides it).
-int adj_tile_cfg() {
-  static int cfg = -1;
-  if (cfg < 0) {
+// Tile shape of the reverse kernel (index into kAdjTiles; environment
+// VERIS_ADJ_TILE overrides the choice): 24 x 32, the fastest measured where
+// latency bounds the kernel (1 degree, 60k cells, either precision); in FP32
+// from 200k cells on, 32 x 32, whose smaller ring of recomputed cells pays
+// once the launch fills the GPU (8 members of 240 x 240: 6% faster).
+template <typename T>
+int adj_tile_cfg(size_t cells) {
+  static int env = -2;
+  if (env == -2) {
     const char* e = getenv("VERIS_ADJ_TILE");
-    const int v = e ? atoi(e) : 1;
-    cfg = (v >= 0 && v < kAdjNumTiles) ? v : 1;
+    const int v = e ? atoi(e) : -1;
+    env = (v >= 0 && v < kAdjNumTiles) ? v : -1;
   }
-  return cfg;
+  if (env >= 0) return env;
+  return (sizeof(T) == 4 && cells >= 200000) ? 3 : 1;
+}
+
+// Whether the recompute of a segment runs on a second stream beside the
+// reverse sweep of the next (environment VERIS_ADJ_OVERLAP=0 serialises it).
+bool adj_overlap() {
+  static int v = -1;
+  if (v < 0) {
+    const char* e = getenv("VERIS_ADJ_OVERLAP");
+    v = (e && atoi(e) == 0) ? 0 : 1;
+  }
+  return v == 1;
 }
 
 // A second stream and two events per device, for the fork/join of the
@@ -879,7 +960,8 @@ int adjoint(const Geom& g, const vh_domain* d, const vh_settings* s, const vh_fi
   const int cfg = pick_adj_tile(g, d->tile);
   int n_tiles, tiles_j;
   tiles_of(cfg, g, &n_tiles, &tiles_j);
-  const int acfg = adj_tile_cfg();
+  const int acfg = adj_tile_cfg<T>(field_elems(g) * (size_t)nm);
+  const bool overlap = adj_overlap();
   T* lam[2][5];
   for (int q = 0; q < 5; ++q) {
     lam[0][q] = wsf<T>(ws, L.off_lam, L.fb, q);
@@ -918,7 +1000,7 @@ int adjoint(const Geom& g, const vh_domain* d, const vh_settings* s, const vh_fi
     hipLaunchKernelGGL(adj_init_kernel<T>, grid, dim3(BLK), 0, cs, a_dev, init, n_all);
     recompute(L.nck - 1, cs);
     for (int sgm = L.nck - 1; sgm >= 0; --sgm) {
-      if (sgm >= 1) {
+      if (sgm >= 1 && overlap) {
         hipEventRecord(side.fork, cs);
         hipStreamWaitEvent(side.stream, side.fork, 0);
         recompute(sgm - 1, side.stream);
@@ -930,11 +1012,13 @@ int adjoint(const Geom& g, const vh_domain* d, const vh_settings* s, const vh_fi
         adj_launch<T>(acfg, a_dev, b, sets(cur), g, nm, cs);
         cur = 1 - cur;
       }
-      if (sgm >= 1) hipStreamWaitEvent(cs, side.join, 0);
+      if (sgm >= 1 && overlap) hipStreamWaitEvent(cs, side.join, 0);
+      if (sgm >= 1 && !overlap) recompute(sgm - 1, cs);
     }
     hipLaunchKernelGGL(adj_finish_kernel<T>, grid, dim3(BLK), 0, cs, a_dev, sets(cur), n_all);
   };
-  return run_recorded<T>("adjr", g, L, cfg * 16 + acfg, nm, ws, record, st);
+  return run_recorded<T>("adjr", g, L, (cfg * 16 + acfg) * 2 + (overlap ? 1 : 0), nm, ws, record,
+                         st);
 }
 
 template <typename T>
