@@ -4,8 +4,9 @@ vh_evp_adjoint, vh_pre_evp_adjoint) through veris_amd.adjoint.evp_subcycle.
 * The checkpointed forward equals the drop-in path (pre-EVP kernels +
   vh_evp_solver) bit for bit.
 * Gradients of a scalar objective of all five outputs w.r.t. P* (per cell),
-  hIceMean (per cell) and the four drag coefficients, at default settings and
-  with the water-stress turning angle and the metric terms, equal
+  hIceMean (per cell) and the four drag coefficients, at default settings,
+  with the water-stress turning angle and the metric terms, and under free
+  slip with lateral drag (land-neighbour and coastline form factors), equal
   torch.autograd of
   an independent torch FP64 restatement (tests/torch_ref.py, the stand-in for
   the reference's jax.grad) to 1e-9 relative: both are FP64 and differ only
@@ -46,6 +47,13 @@ def _state(kind):
         st = synth.global_latlon(31, 36, 20, -75.0, 80.0, veros_fill=False)
     elif kind == "regional":
         st = synth.regional(32, 0, 30, 22)
+    elif kind == "side_drag":  # free slip with lateral drag from land neighbours
+        st = synth.regional(32, 0, 30, 22, noSlip=False, sideDragCoeff=0.004, sideDragU0=0.02)
+    elif kind == "coastline":  # free slip with lateral drag from coastline form factors
+        st = synth.global_latlon(31, 36, 20, -75.0, 80.0, noSlip=False, use_coastline=True)
+        r = np.random.default_rng(8)
+        st.variables.Fu = r.uniform(0.0, 2.0, st.shape) * st.variables.iceMaskU
+        st.variables.Fv = r.uniform(0.0, 2.0, st.shape) * st.variables.iceMaskV
     elif kind == "unpacked":
         # metrics varying along x: the kernels read the full constant fields
         # instead of the packed mask bytes and metric rows
@@ -124,7 +132,8 @@ def _ref_grads(st, n, seed=3):
     return {k: v.grad.numpy() for k, v in p.items()}, [o.detach().numpy() for o in outs], float(J.detach())
 
 
-@pytest.mark.parametrize("kind", ["global", "doubly_periodic", "regional", "turning_metric"])
+@pytest.mark.parametrize("kind", ["global", "doubly_periodic", "regional", "turning_metric",
+                                  "side_drag", "coastline"])
 def test_checkpointed_forward_is_the_drop_in(cuda, kind):
     st = _state(kind)
     gst = st.to(cuda)
@@ -147,7 +156,7 @@ def test_checkpointed_forward_is_the_drop_in(cuda, kind):
 
 
 @pytest.mark.parametrize("kind", ["global", "doubly_periodic", "regional", "turning_metric",
-                                  "unpacked"])
+                                  "unpacked", "side_drag", "coastline"])
 def test_adjoint_matches_torch_autograd(cuda, kind):
     st = _state(kind)
     n = 25
@@ -160,20 +169,23 @@ def test_adjoint_matches_torch_autograd(cuda, kind):
         g, w = np.asarray(got[k]), np.asarray(want[k])
         err = float(np.linalg.norm(g - w) / max(np.linalg.norm(w), 1e-300))
         print(f"{kind} d/d{k}: rel L2 err {err:.3e} |grad| {np.linalg.norm(w):.3e}", flush=True)
-        if k.endswith("_south") and kind == "regional":  # no southern cells
+        if k.endswith("_south") and not (np.asarray(st.variables.fCori) < 0).any():
+            # a domain without southern cells
             assert np.linalg.norm(w) == 0 and np.linalg.norm(g) == 0
             continue
         assert np.linalg.norm(w) > 0, k
         assert err <= 1e-9, f"{kind} d/d{k}: {err:.3e}"
 
 
-def test_every_reverse_output_through_the_c_abi(cuda):
+@pytest.mark.parametrize("kind", ["turning_metric", "coastline"])
+def test_every_reverse_output_through_the_c_abi(cuda, kind):
     """vh_evp_adjoint called directly with every output requested, including
     the optional AreaW/AreaS and uNm1/vNm1 adjoints that the autograd wrapper
     leaves out, against torch autograd with the pre-EVP fields, the initial
     state and the velocities of the previous time level as separate leaves;
-    turning angle and metric terms on.  Bound: 1e-9 relative L2 (both FP64)."""
-    st = _state("turning_metric")
+    turning angle and metric terms, or free slip with coastline side drag.
+    Bound: 1e-9 relative L2 (both FP64)."""
+    st = _state(kind)
     n, ce = 12, 5
     gst = st.to(cuda)
     vs = gst.variables
@@ -238,7 +250,7 @@ def test_checkpoint_interval_is_irrelevant(cuda):
         assert np.array_equal(g1[k], g2[k]) and np.array_equal(g1[k], g3[k]), k
 
 
-@pytest.mark.parametrize("kind", ["global", "turning_metric"])
+@pytest.mark.parametrize("kind", ["global", "turning_metric", "coastline"])
 def test_adjoint_finite_differences(cuda, kind):
     """20 random directions in (P*, h, Cw_N, Cw_S, Ca_N, Ca_S) space; central
     differences of the GPU forward with relative step 1e-6."""
@@ -286,10 +298,11 @@ def test_fp32_adjoint_against_fp64(cuda):
 def test_adjoint_rejects_unsupported(cuda):
     from veris_amd._native import VerisHipError
 
-    st = _state("global")
-    st.settings.basalDragK2 = 8.0
-    with pytest.raises(VerisHipError, match="VH_EUNSUPPORTED"):
-        A.evp_subcycle(st.to(cuda), n_substeps=3)
+    for name, value in (("basalDragK2", 8.0), ("useAdaptiveEVP", True), ("secondOrderBC", True)):
+        st = _state("global")
+        setattr(st.settings, name, value)
+        with pytest.raises(VerisHipError, match="VH_EUNSUPPORTED"):
+            A.evp_subcycle(st.to(cuda), n_substeps=3)
 
 
 def test_backward_after_the_state_is_reassigned(cuda):

@@ -34,6 +34,13 @@ def _state(kind):
         r = np.random.default_rng(4)
         for k in ("k1AtC", "k2AtC", "k1AtZ", "k2AtZ"):
             st.variables[k] = r.normal(0, 1e-6, st.shape)
+    elif kind == "side_drag":
+        st = synth.regional(32, 0, 30, 22, noSlip=False, sideDragCoeff=0.004, sideDragU0=0.02)
+    elif kind == "coastline":
+        st = synth.global_latlon(31, 36, 20, -75.0, 80.0, noSlip=False, use_coastline=True)
+        r = np.random.default_rng(8)
+        st.variables.Fu = r.uniform(0.0, 2.0, st.shape) * st.variables.iceMaskU
+        st.variables.Fv = r.uniform(0.0, 2.0, st.shape) * st.variables.iceMaskV
     else:
         st = synth.regional(32, 0, 30, 22)
     pre_evp(st)
@@ -46,7 +53,8 @@ def _torch_vars(st):
     return {k: torch.from_numpy(np.asarray(v, dtype=np.float64)) for k, v in st.variables.items()}
 
 
-@pytest.mark.parametrize("kind", ["global", "regional", "turning_metric"])
+@pytest.mark.parametrize("kind", ["global", "regional", "turning_metric", "side_drag",
+                                  "coastline"])
 def test_forward_matches_oracle(kind):
     st = _state(kind)
     vs, s = _torch_vars(st), st.settings
@@ -59,8 +67,9 @@ def test_forward_matches_oracle(kind):
         np.testing.assert_array_equal(p[name].numpy(), st.variables[name], err_msg=name)
     # torch's pow/exp and NumPy's differ by an ulp here and there
     for name in ("WindForcingX", "WindForcingY", "SeaIceStrength"):
-        np.testing.assert_allclose(p[name].numpy(), st.variables[name], rtol=2e-15, atol=0,
-                                   err_msg=name)
+        ref = np.asarray(st.variables[name])
+        np.testing.assert_allclose(p[name].numpy(), ref, rtol=2e-15,
+                                   atol=1e-15 * np.abs(ref).max(), err_msg=name)
     out = TR.evp(vs, s, p, 30, cw)
     want = O.evp_solver(st, n_substeps=30)
     for q in range(5):
@@ -68,7 +77,8 @@ def test_forward_matches_oracle(kind):
         assert np.abs(out[q].detach().numpy() - want[q]).max() <= 1e-13 * scale, q
 
 
-@pytest.mark.parametrize("kind", ["global", "regional", "turning_metric"])
+@pytest.mark.parametrize("kind", ["global", "regional", "turning_metric", "side_drag",
+                                  "coastline"])
 def test_autograd_matches_finite_differences(kind):
     st = _state(kind)
     vs, s = _torch_vars(st), st.settings

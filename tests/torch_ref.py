@@ -5,8 +5,9 @@ It stands in for ``jax.grad`` of the reference (JAX is absent here) when the
 hand-written HIP adjoint is checked.  It follows the reference's expressions
 (veris/evp_solver.py:22-319, veris/dynamics_routines.py:5-302,
 veris/averaging.py:4-19, veris/area_mass.py:5-28, veris/dynsolver.py:7-84)
-at default settings (without basal drag), optionally with the water-stress
-turning term and the metric terms, written with torch.roll, and defines the derivative of sqrt at 0 as 0 (the reference's
+without basal drag, adaptive EVP or the second-order boundary term,
+optionally with the water-stress turning term, the metric terms and free
+slip with lateral drag, written with torch.roll, and defines the derivative of sqrt at 0 as 0 (the reference's
 Delta = sqrt(Delta^2) and drag sqrt are not differentiable there; the HIP
 adjoint uses the same convention).
 """
@@ -112,8 +113,9 @@ def evp_substep(vs, sett, p, u, v, u0, v0, s1, s2, s12, cw):
     if kt["k1AtZ"] is not None:
         inner = inner - kt["k1AtZ"] * vavz - kt["k2AtZ"] * uavz
     e12 = 0.5 * inner * mz
-    e12 = e12 + (2.0 * uavz * vs["recip_dyU"] * (MU - jm1(MU))
-                 + 2.0 * vavz * vs["recip_dxV"] * (MV - im1(MV)))
+    if sett.noSlip:
+        e12 = e12 + (2.0 * uavz * vs["recip_dyU"] * (MU - jm1(MU))
+                     + 2.0 * vavz * vs["recip_dxV"] * (MV - im1(MV)))
     rpd = 1.0 / sett.PlasDefCoeff**2
     q = vs["rAz"] * e12**2
     q = q + ip1(q)
@@ -173,6 +175,18 @@ def evp_substep(vs, sett, p, u, v, u0, v0, s1, s2, s12, cw):
     denV = mV * bP1 + p["AreaS"] * (cdV * sett.cosWat)
     denU = torch.where(denU == 0, torch.ones_like(denU), denU)
     denV = torch.where(denV == 0, torch.ones_like(denV), denV)
+    if not sett.noSlip:  # lateral drag (veris/dynamics_routines.py:98-140, evp_solver.py:169-176)
+        speed = 0.5 * ssqrt((u + ip1(u)) ** 2 + (v + jp1(v)) ** 2)
+        z = torch.zeros_like(speed)
+        spU = torch.where(p["AreaW"] > 0, 0.5 * (speed + im1(speed)), z)
+        spV = torch.where(p["AreaS"] > 0, 0.5 * (speed + jm1(speed)), z)
+        if sett.use_coastline:
+            mskU, mskV = vs["Fu"], vs["Fv"]
+        else:
+            mskU = (2 - im1(MU) - ip1(MU)) * MU
+            mskV = (2 - jm1(MV) - jp1(MV)) * MV
+        denU = denU + mU * sett.sideDragCoeff * mskU / (spU + sett.sideDragU0)
+        denV = denV + mV * sett.sideDragCoeff * mskV / (spV + sett.sideDragU0)
     un = MU * (bU * mU * u + mU * sett.recip_deltatDyn * u0 + fx + divx) / denU
     vn = MV * (bU * mV * v + mV * sett.recip_deltatDyn * v0 + fy + divy) / denV
     return fill(sett, un), fill(sett, vn), s1, s2, s12

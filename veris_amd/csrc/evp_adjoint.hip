@@ -17,8 +17,9 @@
 // Non-smooth points: d sqrt(x)/dx is taken as 0 at x = 0 (Delta = 0 on
 // motionless or masked cells, zero relative velocity); the denominator guard
 // and the drag regime switch contribute no derivative.  Supported settings:
-// non-adaptive EVP, noSlip, no basal drag, with or without the turning angle
-// and the metric terms.
+// non-adaptive EVP without basal drag or the second-order no-slip term, with
+// no-slip boundaries or free slip with lateral drag, with or without the
+// turning angle and the metric terms.
 #include <algorithm>
 #include <cstdio>
 #include <cstdlib>
@@ -213,7 +214,12 @@ __device__ inline T gather_filled(const Geom& g, const T* a, int i, int j) {
 // as the forward kernel computes them; nothing but the two states and the
 // state adjoints is read from or written to memory per substep, plus the
 // parameter adjoints of the tile's own cells.
-template <typename T, typename TD, int RI, int RJ>
+//
+// SD selects the instantiation for free slip with lateral drag (noSlip =
+// False): no boundary term in e12, and the lateral drag of each u/v point in
+// its momentum denominator, whose adjoint reaches the ice speed at c
+// (veris/dynamics_routines.py:98-140, veris/evp_solver.py:169-176).
+template <typename T, typename TD, int RI, int RJ, bool SD>
 __global__ __launch_bounds__(RI* RJ) void adj_substep_kernel(const AdjArgs<T>* __restrict__ a,
                                                              const EvpArgBlock<T>* __restrict__ blk,
                                                              LamSet<T> L, int n_tiles, int tiles_j,
@@ -228,6 +234,10 @@ __global__ __launch_bounds__(RI* RJ) void adj_substep_kernel(const AdjArgs<T>* _
   // their face sums in sAB/sBB (C to S2), the u/v-point factors
   // nb sgn(f) lm Area of its adjoint in sU/sV (S1 to S2) and the velocity
   // adjoint partials in sU/sV again (S3 to S4).
+  // Lateral drag (SD): the ice speed at c (C to S2) and the speed adjoints of
+  // the u/v points (S1 to S2); the velocity adjoint partials of the speed go
+  // through sZ/sCXV, free after S2.
+  __shared__ T sSPD[SD ? NT : 1], sLSU[SD ? NT : 1], sLSV[SD ? NT : 1];
 
   const int bid = blockIdx.x;
   const int tile = (bid & 7) * per_xcd + (bid >> 3);
@@ -308,6 +318,11 @@ __global__ __launch_bounds__(RI* RJ) void adj_substep_kernel(const AdjArgs<T>* _
   const T un = ldg(blk->out[0], c), vn = ldg(blk->out[1], c);
   const T mU = ldg(k.massU, c), mV = ldg(k.massV, c), AW = ldg(k.AreaW, c), AS = ldg(k.AreaS, c);
   const T mC = ldg(k.massC, c);
+  T FuC = 0, FvC = 0;  // coastline form factors
+  if (SD && sc.coast) {
+    FuC = ldg(k.Fu, c);
+    FvC = ldg(k.Fv, c);
+  }
   const T l1 = ldg(L.in[2], c), l2 = ldg(L.in[3], c), l12 = ldg(L.in[4], c);
   const T lu = gather_filled(g, L.in[0] + mo, gi, gj), lv = gather_filled(g, L.in[1] + mo, gi, gj);
   // water drag factor of either hemisphere, selected by value (a select of
@@ -346,9 +361,11 @@ __global__ __launch_bounds__(RI* RJ) void adj_substep_kernel(const AdjArgs<T>* _
     T inner = dudy + dvdx;
     if (sc.kterms) inner = (inner - ldg(k.k1Z, c) * vavz) - ldg(k.k2Z, c) * uavz;
     T e12 = ((T)0.5 * inner) * mz;
-    const T hu = MUc - sMU[t - 1];
-    const T hv = MVc - sMV[t - RJ];
-    e12 = e12 + ((((T)2.0 * uavz) * rdyU) * hu + (((T)2.0 * vavz) * rdxV) * hv);
+    if (!SD) {  // no-slip boundary term
+      const T hu = MUc - sMU[t - 1];
+      const T hv = MVc - sMV[t - RJ];
+      e12 = e12 + ((((T)2.0 * uavz) * rdyU) * hu + (((T)2.0 * vavz) * rdxV) * hv);
+    }
     sE12[t] = e12;
     sQ[t] = rAz * (e12 * e12);
     e12r = e12;
@@ -357,6 +374,7 @@ __global__ __launch_bounds__(RI* RJ) void adj_substep_kernel(const AdjArgs<T>* _
 
   // ---- C: zeta, Delta, strain invariants, water drag at c-points
   T zeta = 0, dC = 0, ep = 0, em = 0, sa = 0, sb = 0, ww = 0, sw = 0;
+  T su = 0, sv = 0, spd = 0;  // lateral drag: u + u(i+1), v + v(j+1) and the speed at c
   if (ar >= 1 && br >= 1 && ar <= RI - 2 && br <= RJ - 2) {
     const T uip = sU[t + RJ], vjp = sV[t + 1];
     const T rdxU = met(3), rdyV = met(4);
@@ -371,6 +389,12 @@ __global__ __launch_bounds__(RI* RJ) void adj_substep_kernel(const AdjArgs<T>* _
     const T qa = sQ[t] + sQ[t + RJ];
     const T qb = sQ[t + 1] + sQ[t + RJ + 1];
     const T e12c = ((T)0.25 * met(5)) * (qa + qb);
+    if (SD) {  // ice speed at c (veris/dynamics_routines.py:109-111)
+      su = u + uip;
+      sv = v + vjp;
+      spd = (T)0.5 * sqrt(su * su + sv * sv);
+      sSPD[t] = spd;
+    }
     ep = e11 + e22;
     em = e11 - e22;
     const TD dsq = (TD)ep * (TD)ep + (TD)sc.rpd * ((TD)em * (TD)em + (TD)4.0 * (TD)e12c);
@@ -417,12 +441,28 @@ __global__ __launch_bounds__(RI* RJ) void adj_substep_kernel(const AdjArgs<T>* _
     const T denU = mU * bP1 + AW * (cdAU * cw);
     const T denV = mV * bP1 + AS * (cdAV * cw);
     const bool gU = denU == (T)0, gV = denV == (T)0;
-    const T rU = (T)1 / (gU ? (T)1 : denU), rV = (T)1 / (gV ? (T)1 : denV);
+    T dU = gU ? (T)1 : denU, dV = gV ? (T)1 : denV;
+    // lateral drag (SD), added after the zero guard as the forward adds it
+    T spU = 0, spV = 0, sdnU = 1, sdnV = 1, sdU = 0, sdV = 0, sdmU = 0, sdmV = 0;
+    if (SD) {
+      spU = AW > (T)0 ? (T)0.5 * (sSPD[t] + sSPD[t - RJ]) : (T)0;
+      spV = AS > (T)0 ? (T)0.5 * (sSPD[t] + sSPD[t - 1]) : (T)0;
+      sdmU = sc.coast ? FuC : (((T)2 - sMU[t - RJ]) - sMU[t + RJ]) * MUc;
+      sdmV = sc.coast ? FvC : (((T)2 - sMV[t - 1]) - sMV[t + 1]) * MVc;
+      sdnU = spU + (T)sc.sdU0;
+      sdnV = spV + (T)sc.sdU0;
+      sdU = ((mU * (T)sc.sdCoeff) * sdmU) / sdnU;
+      sdV = ((mV * (T)sc.sdCoeff) * sdmV) / sdnV;
+      dU = dU + sdU;
+      dV = dV + sdV;
+    }
+    const T rU = (T)1 / dU, rV = (T)1 / dV;
     const T lcu = comp ? lu : (T)0, lcv = comp ? lv : (T)0;
     nbU = lcu * MUc * rU;
-    dbU = gU ? (T)0 : -lcu * un * rU;
+    const T ddU = -lcu * un * rU, ddV = -lcv * vn * rV;  // adjoints of the denominators
+    dbU = gU ? (T)0 : ddU;
     nbV = lcv * MVc * rV;
-    dbV = gV ? (T)0 : -lcv * vn * rV;
+    dbV = gV ? (T)0 : ddV;
     sA[t] = nbU * met(10);  // X: adjoint of the u-point stress divergence (rrAu)
     sB[t] = nbV * met(11);  // Y: of the v-point one (rrAv)
     sQ[t] = nbU;
@@ -444,13 +484,25 @@ __global__ __launch_bounds__(RI* RJ) void adj_substep_kernel(const AdjArgs<T>* _
     aAS = nbV * (cdAV * cw * voc + turnV) + dbV * (cdAV * cw);
     aMU = nbU * (bU * u + rdt * uNm1) + dbU * bP1;  // SeaIceMassU, SeaIceMassV adjoints
     aMV = nbV * (bU * v + rdt * vNm1) + dbV * bP1;
+    if (SD) {  // the lateral drag's mass factor and its speed (zero where AreaW/S = 0)
+      aMU = aMU + ddU * (((T)sc.sdCoeff * sdmU) / sdnU);
+      aMV = aMV + ddV * (((T)sc.sdCoeff * sdmV) / sdnV);
+      sLSU[t] = AW > (T)0 ? (T)-0.5 * (ddU * sdU / sdnU) : (T)0;
+      sLSV[t] = AS > (T)0 ? (T)-0.5 * (ddV * sdV / sdnV) : (T)0;
+    }
   }
   __syncthreads();
 
   // ---- S2: stresses, drag and Coriolis at c, sigma12 at z
   T Padd = 0, mCadd = 0, Db = 0, zbp = 0, epb0 = 0, emb0 = 0, dbp = 0;
-  T lo1 = 0, lo2 = 0, lo12 = 0, wU = 0, wV = 0;
+  T lo1 = 0, lo2 = 0, lo12 = 0, wU = 0, wV = 0, qU = 0, qV = 0;
   if (ar >= 3 && br >= 3 && ar <= RI - 3 && br <= RJ - 3) {
+    if (SD) {  // speed adjoint at c from its four faces, then d speed / d (u, v)
+      const T lsp = (sLSU[t] + sLSU[t + RJ]) + (sLSV[t] + sLSV[t + 1]);
+      const T lq = spd > (T)0 ? lsp * (T)0.125 / spd : (T)0;  // speed = sqrt(q) / 2
+      qU = (T)2 * lq * su;
+      qV = (T)2 * lq * sv;
+    }
     const T Xc = sA[t], Xip = sA[t + RJ], Xjm = sA[t - 1];
     const T Yc = sB[t], Yjp = sB[t + 1], Yim = sB[t - RJ];
     T cdb = (T)0.5 * (sCXU[t] + sCXU[t + RJ]) + (T)0.5 * (sCXV[t] + sCXV[t + 1]);
@@ -524,6 +576,10 @@ __global__ __launch_bounds__(RI* RJ) void adj_substep_kernel(const AdjArgs<T>* _
       sU[t] = wU;
       sV[t] = wV;
     }
+    if (SD) {
+      sZ[t] = qU;
+      sCXV[t] = qV;
+    }
   }
   __syncthreads();
 
@@ -543,8 +599,9 @@ __global__ __launch_bounds__(RI* RJ) void adj_substep_kernel(const AdjArgs<T>* _
   }
   const T ec = eadj(t, rAz), ejp = eadj(t + 1, rAz_jp), eip = eadj(t + RJ, rAz_ip);
   const T mzc = mzf(t), mzjp = mzf(t + 1), mzip = mzf(t + RJ);
-  const T huc = MUc - sMU[t - 1], hujp = sMU[t + 1] - MUc;
-  const T hvc = MVc - sMV[t - RJ], hvip = sMV[t + RJ] - MVc;
+  // no-slip boundary term of e12 (absent under free slip)
+  const T huc = SD ? (T)0 : MUc - sMU[t - 1], hujp = SD ? (T)0 : sMU[t + 1] - MUc;
+  const T hvc = SD ? (T)0 : MVc - sMV[t - RJ], hvip = SD ? (T)0 : sMV[t + RJ] - MVc;
   const T A1 = (T)0.5 * mzc * rdyU + huc * rdyU;
   const T A2jp = -(T)0.5 * mzjp * rdyU_jp + hujp * rdyU_jp;
   const T A3 = (T)0.5 * mzc * rdxV + hvc * rdxV;
@@ -563,6 +620,10 @@ __global__ __launch_bounds__(RI* RJ) void adj_substep_kernel(const AdjArgs<T>* _
   if (sc.turn) {  // the ice velocity in the turning terms of the four neighbouring faces
     lu_out = lu_out + (T)0.25 * (T)sc.sinWat * (sU[t] + sU[t - 1]);
     lv_out = lv_out - (T)0.25 * (T)sc.sinWat * (sV[t] + sV[t - RJ]);
+  }
+  if (SD) {  // the ice speed at the two c-points of this u (v) point
+    lu_out = lu_out + (sZ[t] + sZ[t - RJ]);
+    lv_out = lv_out + (sCXV[t] + sCXV[t - 1]);
   }
   stg(L.out[0], c, lu_out);
   stg(L.out[1], c, lv_out);
@@ -599,26 +660,30 @@ constexpr int kAdjNumTiles = sizeof(kAdjTiles) / sizeof(kAdjTiles[0]);
 
 template <typename T, int RI, int RJ>
 void adj_launch_shape(const AdjArgs<T>* a, const EvpArgBlock<T>* b, LamSet<T> L, const Geom& g,
-                      int n_members, hipStream_t st) {
+                      int n_members, bool sd, hipStream_t st) {
   constexpr int TI = RI - 8, TJ = RJ - 8;
   const int tiles_j = (g.NY + TJ - 1) / TJ;
   const int n_tiles = ((g.NX + TI - 1) / TI) * tiles_j;
   const int per_xcd = (n_tiles + 7) / 8;
-  hipLaunchKernelGGL((adj_substep_kernel<T, double, RI, RJ>), dim3(per_xcd * 8, n_members),
-                     dim3(RI * RJ), 0, st, a, b, L, n_tiles, tiles_j, per_xcd);
+  if (sd)
+    hipLaunchKernelGGL((adj_substep_kernel<T, double, RI, RJ, true>), dim3(per_xcd * 8, n_members),
+                       dim3(RI * RJ), 0, st, a, b, L, n_tiles, tiles_j, per_xcd);
+  else
+    hipLaunchKernelGGL((adj_substep_kernel<T, double, RI, RJ, false>), dim3(per_xcd * 8, n_members),
+                       dim3(RI * RJ), 0, st, a, b, L, n_tiles, tiles_j, per_xcd);
 }
 
 template <typename T>
 void adj_launch(int cfg, const AdjArgs<T>* a, const EvpArgBlock<T>* b, LamSet<T> L, const Geom& g,
-                int n_members, hipStream_t st) {
+                int n_members, bool sd, hipStream_t st) {
   switch (cfg) {
-    case 1: adj_launch_shape<T, 24, 32>(a, b, L, g, n_members, st); break;
-    case 2: adj_launch_shape<T, 12, 64>(a, b, L, g, n_members, st); break;
+    case 1: adj_launch_shape<T, 24, 32>(a, b, L, g, n_members, sd, st); break;
+    case 2: adj_launch_shape<T, 12, 64>(a, b, L, g, n_members, sd, st); break;
     case 3:  // 1024 threads: FP32 only (the FP64 kernel's registers do not fit)
-      if constexpr (sizeof(T) == 4) adj_launch_shape<T, 32, 32>(a, b, L, g, n_members, st);
-      else adj_launch_shape<T, 24, 32>(a, b, L, g, n_members, st);
+      if constexpr (sizeof(T) == 4) adj_launch_shape<T, 32, 32>(a, b, L, g, n_members, sd, st);
+      else adj_launch_shape<T, 24, 32>(a, b, L, g, n_members, sd, st);
       break;
-    default: adj_launch_shape<T, 16, 32>(a, b, L, g, n_members, st); break;
+    default: adj_launch_shape<T, 16, 32>(a, b, L, g, n_members, sd, st); break;
   }
 }
 
@@ -793,9 +858,10 @@ int check_adj(const vh_domain* d, const vh_settings* s, const vh_fields* f, int 
   VH_REQUIRE(n <= (1 << 20), VH_EINVAL, "n_substeps=%d too large", n);
   VH_REQUIRE(d->tile >= 0 && d->tile < kEvpNumTiles && (d->tile == 0 || kEvpTiles[d->tile].k == 1),
              VH_EINVAL, "adjoint: tile %d (needs a single-substep tile shape or 0)", d->tile);
-  VH_REQUIRE(s->useAdaptiveEVP == 0.0 && s->noSlip != 0.0 && s->secondOrderBC == 0.0 &&
-                 s->useEVP != 0.0,
-             VH_EUNSUPPORTED, "adjoint: only non-adaptive noSlip EVP is supported");
+  VH_REQUIRE(s->useAdaptiveEVP == 0.0 && s->useEVP != 0.0, VH_EUNSUPPORTED,
+             "adjoint: only non-adaptive EVP is supported");
+  VH_REQUIRE(!(s->noSlip != 0.0 && s->secondOrderBC != 0.0), VH_EUNSUPPORTED,
+             "adjoint: the second-order no-slip boundary term is not supported");
   VH_REQUIRE(s->basalDragK2 == 0.0, VH_EUNSUPPORTED, "adjoint: basal drag not supported");
   const bool kt = f->k1AtC || f->k2AtC || f->k1AtZ || f->k2AtZ;
   VH_REQUIRE(!kt || (f->k1AtC && f->k2AtC && f->k1AtZ && f->k2AtZ), VH_EINVAL,
@@ -962,6 +1028,7 @@ int adjoint(const Geom& g, const vh_domain* d, const vh_settings* s, const vh_fi
   tiles_of(cfg, g, &n_tiles, &tiles_j);
   const int acfg = adj_tile_cfg<T>(field_elems(g) * (size_t)nm);
   const bool overlap = adj_overlap();
+  const bool sd = s->noSlip == 0.0;  // free slip with lateral drag
   T* lam[2][5];
   for (int q = 0; q < 5; ++q) {
     lam[0][q] = wsf<T>(ws, L.off_lam, L.fb, q);
@@ -1009,7 +1076,7 @@ int adjoint(const Geom& g, const vh_domain* d, const vh_settings* s, const vh_fi
       const int k0 = sgm * ce, len = std::min(ce, n - k0);
       for (int jj = len - 1; jj >= 0; --jj) {
         const EvpArgBlock<T>* b = (const EvpArgBlock<T>*)((char*)rblk + (size_t)(k0 + jj) * L.blk);
-        adj_launch<T>(acfg, a_dev, b, sets(cur), g, nm, cs);
+        adj_launch<T>(acfg, a_dev, b, sets(cur), g, nm, sd, cs);
         cur = 1 - cur;
       }
       if (sgm >= 1 && overlap) hipStreamWaitEvent(cs, side.join, 0);
@@ -1017,8 +1084,10 @@ int adjoint(const Geom& g, const vh_domain* d, const vh_settings* s, const vh_fi
     }
     hipLaunchKernelGGL(adj_finish_kernel<T>, grid, dim3(BLK), 0, cs, a_dev, sets(cur), n_all);
   };
-  return run_recorded<T>("adjr", g, L, (cfg * 16 + acfg) * 2 + (overlap ? 1 : 0), nm, ws, record,
-                         st);
+  // the key holds every choice baked into the recorded launches: shapes,
+  // overlap and the lateral-drag instantiation
+  return run_recorded<T>("adjr", g, L, ((cfg * 16 + acfg) * 2 + (overlap ? 1 : 0)) * 2 + (sd ? 1 : 0),
+                         nm, ws, record, st);
 }
 
 template <typename T>
