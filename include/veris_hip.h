@@ -438,6 +438,10 @@ typedef struct vh_evp_adjoint_out {
   void *adj_SeaIceStrength, *adj_SeaIceMassC, *adj_SeaIceMassU, *adj_SeaIceMassV;
   void *adj_AreaW, *adj_AreaS, *adj_WindForcingX, *adj_WindForcingY;
   void *adj_waterDrag_north, *adj_waterDrag_south;
+  /* thickness adjoint through the basal drag coefficient (accumulated when
+   * basalDragK2 != 0, may be NULL otherwise); vh_pre_evp_adjoint adds it to
+   * its adj_hIceMean when non-NULL */
+  void* adj_hIceMean;
 } vh_evp_adjoint_out;
 
 /* Adjoints of the pre-EVP inputs (outputs of vh_pre_evp_adjoint): per-cell
@@ -457,9 +461,9 @@ int vh_evp_adjoint_workspace_bytes(const vh_domain* d, int32_t n_substeps, int32
 /* evp_solver (veris/evp_solver.py:257-319) that also keeps the state every
  * ckpt_every substeps in `workspace` for a later vh_evp_adjoint with the same
  * inputs.  Results equal vh_evp_solver's.  Supported: non-adaptive EVP without
- * basal drag or the second-order no-slip term; no-slip boundaries or free
- * slip with lateral drag; with or without the water-stress turning angle
- * (sinWat, cosWat) and the metric terms (k1AtC, k2AtC, k1AtZ, k2AtZ). */
+ * the second-order no-slip term; no-slip boundaries or free slip with lateral
+ * drag; with or without basal drag, the water-stress turning angle (sinWat,
+ * cosWat) and the metric terms (k1AtC, k2AtC, k1AtZ, k2AtZ). */
 int vh_evp_forward_checkpointed(const vh_domain* d, const vh_settings* s, const vh_fields* f,
                                 int32_t n_substeps, int32_t ckpt_every, void* uIce, void* vIce,
                                 void* sigma1, void* sigma2, void* sigma12, void* workspace,

@@ -47,6 +47,9 @@ def _state(kind):
         st = synth.global_latlon(31, 36, 20, -75.0, 80.0, veros_fill=False)
     elif kind == "regional":
         st = synth.regional(32, 0, 30, 22)
+    elif kind == "basal":  # grounded keels: water 8 m deep, given as a negative depth
+        st = synth.global_latlon(31, 36, 20, -75.0, 80.0, basalDragK2=15.0)
+        st.variables.R_low = -np.asarray(st.variables.R_low) * 0.002
     elif kind == "side_drag":  # free slip with lateral drag from land neighbours
         st = synth.regional(32, 0, 30, 22, noSlip=False, sideDragCoeff=0.004, sideDragU0=0.02)
     elif kind == "coastline":  # free slip with lateral drag from coastline form factors
@@ -133,7 +136,7 @@ def _ref_grads(st, n, seed=3):
 
 
 @pytest.mark.parametrize("kind", ["global", "doubly_periodic", "regional", "turning_metric",
-                                  "side_drag", "coastline"])
+                                  "side_drag", "coastline", "basal"])
 def test_checkpointed_forward_is_the_drop_in(cuda, kind):
     st = _state(kind)
     gst = st.to(cuda)
@@ -156,7 +159,7 @@ def test_checkpointed_forward_is_the_drop_in(cuda, kind):
 
 
 @pytest.mark.parametrize("kind", ["global", "doubly_periodic", "regional", "turning_metric",
-                                  "unpacked", "side_drag", "coastline"])
+                                  "unpacked", "side_drag", "coastline", "basal"])
 def test_adjoint_matches_torch_autograd(cuda, kind):
     st = _state(kind)
     n = 25
@@ -177,14 +180,15 @@ def test_adjoint_matches_torch_autograd(cuda, kind):
         assert err <= 1e-9, f"{kind} d/d{k}: {err:.3e}"
 
 
-@pytest.mark.parametrize("kind", ["turning_metric", "coastline"])
+@pytest.mark.parametrize("kind", ["turning_metric", "coastline", "basal"])
 def test_every_reverse_output_through_the_c_abi(cuda, kind):
     """vh_evp_adjoint called directly with every output requested, including
     the optional AreaW/AreaS and uNm1/vNm1 adjoints that the autograd wrapper
     leaves out, against torch autograd with the pre-EVP fields, the initial
     state and the velocities of the previous time level as separate leaves;
-    turning angle and metric terms, or free slip with coastline side drag.
-    Bound: 1e-9 relative L2 (both FP64)."""
+    turning angle and metric terms, free slip with coastline side drag, or
+    basal drag (with the thickness adjoint of its coefficient).  Bound: 1e-9
+    relative L2 (both FP64)."""
     st = _state(kind)
     n, ce = 12, 5
     gst = st.to(cuda)
@@ -214,7 +218,7 @@ def test_every_reverse_output_through_the_c_abi(cuda, kind):
     f = tv["fCori"]
     pre = {k: torch.from_numpy(np.asarray(st.variables[k], dtype=np.float64)).requires_grad_(True)
            for k in ("SeaIceMassC", "SeaIceMassU", "SeaIceMassV", "AreaW", "AreaS",
-                     "WindForcingX", "WindForcingY", "SeaIceStrength")}
+                     "WindForcingX", "WindForcingY", "SeaIceStrength", "hIceMean")}
     cwN = torch.tensor(float(s.waterIceDrag), dtype=torch.float64, requires_grad=True)
     cwS = torch.tensor(float(s.waterIceDrag_south), dtype=torch.float64, requires_grad=True)
     cw = torch.where(f < 0, cwS, cwN)
@@ -228,6 +232,8 @@ def test_every_reverse_output_through_the_c_abi(cuda, kind):
     J.backward()
     want = {**{k: st0[k].grad for k in NAMES}, "uNm1": u0.grad, "vNm1": v0.grad,
             **{k: pre[k].grad for k in pre}}
+    if not st.settings.basalDragK2:  # the thickness enters through the basal drag only
+        assert want.pop("hIceMean") is None
     for k, ref in want.items():
         got = g2n(evp[k])
         err = float(np.linalg.norm(got - ref.numpy()) / max(float(ref.norm()), 1e-300))
@@ -250,7 +256,7 @@ def test_checkpoint_interval_is_irrelevant(cuda):
         assert np.array_equal(g1[k], g2[k]) and np.array_equal(g1[k], g3[k]), k
 
 
-@pytest.mark.parametrize("kind", ["global", "turning_metric", "coastline"])
+@pytest.mark.parametrize("kind", ["global", "turning_metric", "coastline", "basal"])
 def test_adjoint_finite_differences(cuda, kind):
     """20 random directions in (P*, h, Cw_N, Cw_S, Ca_N, Ca_S) space; central
     differences of the GPU forward with relative step 1e-6."""
@@ -298,7 +304,7 @@ def test_fp32_adjoint_against_fp64(cuda):
 def test_adjoint_rejects_unsupported(cuda):
     from veris_amd._native import VerisHipError
 
-    for name, value in (("basalDragK2", 8.0), ("useAdaptiveEVP", True), ("secondOrderBC", True)):
+    for name, value in (("useAdaptiveEVP", True), ("secondOrderBC", True)):
         st = _state("global")
         setattr(st.settings, name, value)
         with pytest.raises(VerisHipError, match="VH_EUNSUPPORTED"):

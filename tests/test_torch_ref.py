@@ -34,6 +34,9 @@ def _state(kind):
         r = np.random.default_rng(4)
         for k in ("k1AtC", "k2AtC", "k1AtZ", "k2AtZ"):
             st.variables[k] = r.normal(0, 1e-6, st.shape)
+    elif kind == "basal":  # grounded keels: water 8 m deep, given as a negative depth
+        st = synth.global_latlon(31, 36, 20, -75.0, 80.0, basalDragK2=15.0)
+        st.variables.R_low = -np.asarray(st.variables.R_low) * 0.002
     elif kind == "side_drag":
         st = synth.regional(32, 0, 30, 22, noSlip=False, sideDragCoeff=0.004, sideDragU0=0.02)
     elif kind == "coastline":
@@ -54,7 +57,7 @@ def _torch_vars(st):
 
 
 @pytest.mark.parametrize("kind", ["global", "regional", "turning_metric", "side_drag",
-                                  "coastline"])
+                                  "coastline", "basal"])
 def test_forward_matches_oracle(kind):
     st = _state(kind)
     vs, s = _torch_vars(st), st.settings
@@ -78,7 +81,7 @@ def test_forward_matches_oracle(kind):
 
 
 @pytest.mark.parametrize("kind", ["global", "regional", "turning_metric", "side_drag",
-                                  "coastline"])
+                                  "coastline", "basal"])
 def test_autograd_matches_finite_differences(kind):
     st = _state(kind)
     vs, s = _torch_vars(st), st.settings

@@ -5,9 +5,9 @@ It stands in for ``jax.grad`` of the reference (JAX is absent here) when the
 hand-written HIP adjoint is checked.  It follows the reference's expressions
 (veris/evp_solver.py:22-319, veris/dynamics_routines.py:5-302,
 veris/averaging.py:4-19, veris/area_mass.py:5-28, veris/dynsolver.py:7-84)
-without basal drag, adaptive EVP or the second-order boundary term,
-optionally with the water-stress turning term, the metric terms and free
-slip with lateral drag, written with torch.roll, and defines the derivative of sqrt at 0 as 0 (the reference's
+without adaptive EVP or the second-order boundary term, optionally with the
+water-stress turning term, the metric terms, free slip with lateral drag
+and basal drag, written with torch.roll, and defines the derivative of sqrt at 0 as 0 (the reference's
 Delta = sqrt(Delta^2) and drag sqrt are not differentiable there; the HIP
 adjoint uses the same convention).
 """
@@ -89,7 +89,7 @@ def pre_evp(vs, sett, pstar, h, ca):
     fy = ty * AS - mV * vs["recip_dyC"] * (phi - jm1(phi))
     P = pstar * h * torch.exp(-sett.cStar * (1 - A)) * vs["iceMask"]
     return dict(SeaIceMassC=mC, SeaIceMassU=mU, SeaIceMassV=mV, AreaW=AW, AreaS=AS,
-                WindForcingX=fx, WindForcingY=fy, SeaIceStrength=P)
+                WindForcingX=fx, WindForcingY=fy, SeaIceStrength=P, hIceMean=h)
 
 
 def evp_substep(vs, sett, p, u, v, u0, v0, s1, s2, s12, cw):
@@ -171,8 +171,19 @@ def evp_substep(vs, sett, p, u, v, u0, v0, s1, s2, s12, cw):
     fy = p["WindForcingY"] + iny * p["AreaS"] - 0.5 * (fu + jm1(fu))
     bU = sett.evpBeta * sett.recip_deltatDyn
     bP1 = bU + sett.recip_deltatDyn
-    denU = mU * bP1 + p["AreaW"] * (cdU * sett.cosWat)
-    denV = mV * bP1 + p["AreaS"] * (cdV * sett.cosWat)
+    cbU = cbV = 0.0
+    if sett.basalDragK2 != 0:  # basal drag (veris/dynamics_routines.py:55-96)
+        A, h = vs["Area"], p.get("hIceMean", vs["hIceMean"])
+        um, vm = u * vs["maskInU"], v * vs["maskInV"]
+        tb = 0.25 * (um**2 + ip1(um) ** 2 + vm**2 + jp1(vm) ** 2)
+        tb = sett.basalDragK2 / torch.sqrt(tb + sett.basalDragU0**2)
+        hc = torch.abs(vs["R_low"]) * A / sett.basalDragK1
+        val = (tb * torch.log(torch.exp(10.0 * (h - hc)) + 1.0) * 0.1
+               * torch.exp(-sett.cBasalStar * (1.0 - A)))
+        cb = torch.where(A > 0.01, val, torch.zeros_like(val))
+        cbU, cbV = 0.5 * (cb + im1(cb)), 0.5 * (cb + jm1(cb))
+    denU = mU * bP1 + p["AreaW"] * (cdU * sett.cosWat + cbU)
+    denV = mV * bP1 + p["AreaS"] * (cdV * sett.cosWat + cbV)
     denU = torch.where(denU == 0, torch.ones_like(denU), denU)
     denV = torch.where(denV == 0, torch.ones_like(denV), denV)
     if not sett.noSlip:  # lateral drag (veris/dynamics_routines.py:98-140, evp_solver.py:169-176)

@@ -59,7 +59,7 @@ class GrowthOut(ctypes.Structure):
 EVP_ADJOINT_NAMES = ("uIce", "vIce", "sigma1", "sigma2", "sigma12", "uNm1", "vNm1",
                      "SeaIceStrength", "SeaIceMassC", "SeaIceMassU", "SeaIceMassV", "AreaW",
                      "AreaS", "WindForcingX", "WindForcingY", "waterDrag_north",
-                     "waterDrag_south")
+                     "waterDrag_south", "hIceMean")
 
 
 class EvpAdjointOut(ctypes.Structure):

@@ -17,8 +17,8 @@
 // Non-smooth points: d sqrt(x)/dx is taken as 0 at x = 0 (Delta = 0 on
 // motionless or masked cells, zero relative velocity); the denominator guard
 // and the drag regime switch contribute no derivative.  Supported settings:
-// non-adaptive EVP without basal drag or the second-order no-slip term, with
-// no-slip boundaries or free slip with lateral drag, with or without the
+// non-adaptive EVP without the second-order no-slip term, with no-slip
+// boundaries or free slip with lateral drag, with or without basal drag, the
 // turning angle and the metric terms.
 #include <algorithm>
 #include <cstdio>
@@ -35,9 +35,9 @@ constexpr int BLK = 256;
 
 template <typename T>
 struct AdjAcc {  // parameter adjoints, accumulated over the substeps
-  T *P, *mC, *mU, *mV, *AW, *AS, *WFX, *WFY, *u0, *v0, *DN, *DS;
+  T *P, *mC, *mU, *mV, *AW, *AS, *WFX, *WFY, *u0, *v0, *DN, *DS, *h;
 };
-constexpr int kAdjAccFields = 12;
+constexpr int kAdjAccFields = 13;
 
 // What every reverse launch reads besides workspace-relative pointers.
 template <typename T>
@@ -152,8 +152,8 @@ __global__ __launch_bounds__(BLK) void adj_init_kernel(const AdjArgs<T>* __restr
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= n_all) return;
   const AdjAcc<T>& acc = a->acc;
-  T* const accp[kAdjAccFields] = {acc.P, acc.mC, acc.mU, acc.mV, acc.AW, acc.AS,
-                                  acc.WFX, acc.WFY, acc.u0, acc.v0, acc.DN, acc.DS};
+  T* const accp[kAdjAccFields] = {acc.P,   acc.mC,  acc.mU, acc.mV, acc.AW, acc.AS, acc.WFX,
+                                  acc.WFY, acc.u0, acc.v0, acc.DN, acc.DS, acc.h};
 #pragma unroll
   for (int q = 0; q < kAdjAccFields; ++q)
     if (accp[q]) accp[q][c] = (T)0;
@@ -215,11 +215,14 @@ __device__ inline T gather_filled(const Geom& g, const T* a, int i, int j) {
 // state adjoints is read from or written to memory per substep, plus the
 // parameter adjoints of the tile's own cells.
 //
-// SD selects the instantiation for free slip with lateral drag (noSlip =
-// False): no boundary term in e12, and the lateral drag of each u/v point in
-// its momentum denominator, whose adjoint reaches the ice speed at c
-// (veris/dynamics_routines.py:98-140, veris/evp_solver.py:169-176).
-template <typename T, typename TD, int RI, int RJ, bool SD>
+// XB selects the instantiation for the extra drag branches, each a runtime
+// flag inside it: free slip with lateral drag (noSlip = False: no boundary
+// term in e12, and the lateral drag of each u/v point in its momentum
+// denominator, whose adjoint reaches the ice speed at c;
+// veris/dynamics_routines.py:98-140, veris/evp_solver.py:169-176), and basal
+// drag (the c-point coefficient in the denominators, whose adjoint reaches
+// the velocities and the thickness; veris/dynamics_routines.py:55-96).
+template <typename T, typename TD, int RI, int RJ, bool XB>
 __global__ __launch_bounds__(RI* RJ) void adj_substep_kernel(const AdjArgs<T>* __restrict__ a,
                                                              const EvpArgBlock<T>* __restrict__ blk,
                                                              LamSet<T> L, int n_tiles, int tiles_j,
@@ -234,10 +237,13 @@ __global__ __launch_bounds__(RI* RJ) void adj_substep_kernel(const AdjArgs<T>* _
   // their face sums in sAB/sBB (C to S2), the u/v-point factors
   // nb sgn(f) lm Area of its adjoint in sU/sV (S1 to S2) and the velocity
   // adjoint partials in sU/sV again (S3 to S4).
-  // Lateral drag (SD): the ice speed at c (C to S2) and the speed adjoints of
-  // the u/v points (S1 to S2); the velocity adjoint partials of the speed go
-  // through sZ/sCXV, free after S2.
-  __shared__ T sSPD[SD ? NT : 1], sLSU[SD ? NT : 1], sLSV[SD ? NT : 1];
+  // Lateral drag: the ice speed at c (C to S1) and the speed adjoints of the
+  // u/v points (S1 to S2); the velocity adjoint partials of the speed go
+  // through sZ/sCXV, free after S2.  Basal drag: the coefficient at c (C to
+  // S1), the denominator adjoints times the area at the u/v points (S1 to
+  // S2) and the adjoint of the coefficient's velocity argument at c (S3 to S4).
+  constexpr int NX_ = XB ? NT : 1;
+  __shared__ T sSPD[NX_], sLSU[NX_], sLSV[NX_], sCBc[NX_], sCBU[NX_], sCBV[NX_], sLTB[NX_];
 
   const int bid = blockIdx.x;
   const int tile = (bid & 7) * per_xcd + (bid >> 3);
@@ -318,10 +324,20 @@ __global__ __launch_bounds__(RI* RJ) void adj_substep_kernel(const AdjArgs<T>* _
   const T un = ldg(blk->out[0], c), vn = ldg(blk->out[1], c);
   const T mU = ldg(k.massU, c), mV = ldg(k.massV, c), AW = ldg(k.AreaW, c), AS = ldg(k.AreaS, c);
   const T mC = ldg(k.massC, c);
+  const bool sdr = XB && sc.sidedrag;  // uniform; false in the default instantiation
+  const bool bas = XB && sc.basal;
   T FuC = 0, FvC = 0;  // coastline form factors
-  if (SD && sc.coast) {
+  if (sdr && sc.coast) {
     FuC = ldg(k.Fu, c);
     FvC = ldg(k.Fv, c);
+  }
+  T bAr = 0, bRl = 0, bh = 0, bmUip = 0, bmVjp = 0;  // basal drag inputs
+  if (bas) {
+    bAr = ldg(k.Area, c);
+    bRl = ldg(k.Rlow, c);
+    bh = ldg(k.hIce, c);
+    bmUip = ldg(k.mInU, cip);
+    bmVjp = ldg(k.mInV, cjp);
   }
   const T l1 = ldg(L.in[2], c), l2 = ldg(L.in[3], c), l12 = ldg(L.in[4], c);
   const T lu = gather_filled(g, L.in[0] + mo, gi, gj), lv = gather_filled(g, L.in[1] + mo, gi, gj);
@@ -361,7 +377,7 @@ __global__ __launch_bounds__(RI* RJ) void adj_substep_kernel(const AdjArgs<T>* _
     T inner = dudy + dvdx;
     if (sc.kterms) inner = (inner - ldg(k.k1Z, c) * vavz) - ldg(k.k2Z, c) * uavz;
     T e12 = ((T)0.5 * inner) * mz;
-    if (!SD) {  // no-slip boundary term
+    if (!sdr) {  // no-slip boundary term
       const T hu = MUc - sMU[t - 1];
       const T hv = MVc - sMV[t - RJ];
       e12 = e12 + ((((T)2.0 * uavz) * rdyU) * hu + (((T)2.0 * vavz) * rdxV) * hv);
@@ -375,6 +391,7 @@ __global__ __launch_bounds__(RI* RJ) void adj_substep_kernel(const AdjArgs<T>* _
   // ---- C: zeta, Delta, strain invariants, water drag at c-points
   T zeta = 0, dC = 0, ep = 0, em = 0, sa = 0, sb = 0, ww = 0, sw = 0;
   T su = 0, sv = 0, spd = 0;  // lateral drag: u + u(i+1), v + v(j+1) and the speed at c
+  T cTB = 0, cH = 0;          // basal drag: its derivatives at c
   if (ar >= 1 && br >= 1 && ar <= RI - 2 && br <= RJ - 2) {
     const T uip = sU[t + RJ], vjp = sV[t + 1];
     const T rdxU = met(3), rdyV = met(4);
@@ -389,11 +406,27 @@ __global__ __launch_bounds__(RI* RJ) void adj_substep_kernel(const AdjArgs<T>* _
     const T qa = sQ[t] + sQ[t + RJ];
     const T qb = sQ[t + 1] + sQ[t + RJ + 1];
     const T e12c = ((T)0.25 * met(5)) * (qa + qb);
-    if (SD) {  // ice speed at c (veris/dynamics_routines.py:109-111)
+    if (sdr) {  // ice speed at c (veris/dynamics_routines.py:109-111)
       su = u + uip;
       sv = v + vjp;
       spd = (T)0.5 * sqrt(su * su + sv * sv);
       sSPD[t] = spd;
+    }
+    if (bas) {  // basal drag coefficient at c, the forward kernel's expression order
+      const T um = u * mInUc, umip = uip * bmUip;
+      const T vm = v * mInVc, vmjp = vjp * bmVjp;
+      const T tbs = (T)0.25 * (((um * um + umip * umip) + vm * vm) + vmjp * vmjp) + (T)sc.U0sq;
+      const T tb = (T)sc.K2 / sqrt(tbs);
+      const T hc = (fabs(bRl) * bAr) / (T)sc.K1;
+      const T ex = exp((T)10.0 * (bh - hc));
+      const T lg = log(ex + (T)1.0);
+      const T eb = exp((T)(-sc.cBasalStar) * ((T)1.0 - bAr));
+      const T val = ((tb * lg) * (T)0.1) * eb;
+      const bool on = bAr > (T)0.01;
+      sCBc[t] = on ? val : (T)0;
+      // d cb / d tb-argument and d cb / d h (the soft maximum's logistic slope)
+      cTB = on ? ((T)-0.5 * (((tb * lg) * (T)0.1) * eb)) / tbs : (T)0;
+      cH = on ? (tb * eb) * (ex / (ex + (T)1.0)) : (T)0;
     }
     ep = e11 + e22;
     em = e11 - e22;
@@ -438,13 +471,16 @@ __global__ __launch_bounds__(RI* RJ) void adj_substep_kernel(const AdjArgs<T>* _
   if (ar >= 2 && br >= 2 && ar <= RI - 2 && br <= RJ - 2) {
     const T cdAU = (T)0.5 * (sCD[t] + sCD[t - RJ]);
     const T cdAV = (T)0.5 * (sCD[t] + sCD[t - 1]);
-    const T denU = mU * bP1 + AW * (cdAU * cw);
-    const T denV = mV * bP1 + AS * (cdAV * cw);
+    // basal drag at the u/v point (zero without it: the same sum as the forward's)
+    const T cbU = bas ? (T)0.5 * (sCBc[t] + sCBc[t - RJ]) : (T)0;
+    const T cbV = bas ? (T)0.5 * (sCBc[t] + sCBc[t - 1]) : (T)0;
+    const T denU = mU * bP1 + AW * ((cdAU * cw) + cbU);
+    const T denV = mV * bP1 + AS * ((cdAV * cw) + cbV);
     const bool gU = denU == (T)0, gV = denV == (T)0;
     T dU = gU ? (T)1 : denU, dV = gV ? (T)1 : denV;
-    // lateral drag (SD), added after the zero guard as the forward adds it
+    // lateral drag, added after the zero guard as the forward adds it
     T spU = 0, spV = 0, sdnU = 1, sdnV = 1, sdU = 0, sdV = 0, sdmU = 0, sdmV = 0;
-    if (SD) {
+    if (sdr) {
       spU = AW > (T)0 ? (T)0.5 * (sSPD[t] + sSPD[t - RJ]) : (T)0;
       spV = AS > (T)0 ? (T)0.5 * (sSPD[t] + sSPD[t - 1]) : (T)0;
       sdmU = sc.coast ? FuC : (((T)2 - sMU[t - RJ]) - sMU[t + RJ]) * MUc;
@@ -480,11 +516,15 @@ __global__ __launch_bounds__(RI* RJ) void adj_substep_kernel(const AdjArgs<T>* _
       sU[t] = nbU * sgn(fc) * lmU * AW;
       sV[t] = nbV * sgn(fc) * lmV * AS;
     }
-    aAW = nbU * (cdAU * cw * uoc + turnU) + dbU * (cdAU * cw);  // AreaW, AreaS adjoints
-    aAS = nbV * (cdAV * cw * voc + turnV) + dbV * (cdAV * cw);
+    aAW = nbU * (cdAU * cw * uoc + turnU) + dbU * (cdAU * cw + cbU);  // AreaW, AreaS adjoints
+    aAS = nbV * (cdAV * cw * voc + turnV) + dbV * (cdAV * cw + cbV);
     aMU = nbU * (bU * u + rdt * uNm1) + dbU * bP1;  // SeaIceMassU, SeaIceMassV adjoints
     aMV = nbV * (bU * v + rdt * vNm1) + dbV * bP1;
-    if (SD) {  // the lateral drag's mass factor and its speed (zero where AreaW/S = 0)
+    if (bas) {  // the basal coefficient's share of the denominators
+      sCBU[t] = dbU * AW;
+      sCBV[t] = dbV * AS;
+    }
+    if (sdr) {  // the lateral drag's mass factor and its speed (zero where AreaW/S = 0)
       aMU = aMU + ddU * (((T)sc.sdCoeff * sdmU) / sdnU);
       aMV = aMV + ddV * (((T)sc.sdCoeff * sdmV) / sdnV);
       sLSU[t] = AW > (T)0 ? (T)-0.5 * (ddU * sdU / sdnU) : (T)0;
@@ -495,9 +535,14 @@ __global__ __launch_bounds__(RI* RJ) void adj_substep_kernel(const AdjArgs<T>* _
 
   // ---- S2: stresses, drag and Coriolis at c, sigma12 at z
   T Padd = 0, mCadd = 0, Db = 0, zbp = 0, epb0 = 0, emb0 = 0, dbp = 0;
-  T lo1 = 0, lo2 = 0, lo12 = 0, wU = 0, wV = 0, qU = 0, qV = 0;
+  T lo1 = 0, lo2 = 0, lo12 = 0, wU = 0, wV = 0, qU = 0, qV = 0, ltb = 0, aHadd = 0;
   if (ar >= 3 && br >= 3 && ar <= RI - 3 && br <= RJ - 3) {
-    if (SD) {  // speed adjoint at c from its four faces, then d speed / d (u, v)
+    if (bas) {  // basal coefficient adjoint at c from its four faces
+      const T lcb = (T)0.5 * (sCBU[t] + sCBU[t + RJ]) + (T)0.5 * (sCBV[t] + sCBV[t + 1]);
+      ltb = lcb * cTB;
+      aHadd = lcb * cH;
+    }
+    if (sdr) {  // speed adjoint at c from its four faces, then d speed / d (u, v)
       const T lsp = (sLSU[t] + sLSU[t + RJ]) + (sLSV[t] + sLSV[t + 1]);
       const T lq = spd > (T)0 ? lsp * (T)0.125 / spd : (T)0;  // speed = sqrt(q) / 2
       qU = (T)2 * lq * su;
@@ -576,10 +621,11 @@ __global__ __launch_bounds__(RI* RJ) void adj_substep_kernel(const AdjArgs<T>* _
       sU[t] = wU;
       sV[t] = wV;
     }
-    if (SD) {
+    if (sdr) {
       sZ[t] = qU;
       sCXV[t] = qV;
     }
+    if (bas) sLTB[t] = ltb;
   }
   __syncthreads();
 
@@ -600,8 +646,8 @@ __global__ __launch_bounds__(RI* RJ) void adj_substep_kernel(const AdjArgs<T>* _
   const T ec = eadj(t, rAz), ejp = eadj(t + 1, rAz_jp), eip = eadj(t + RJ, rAz_ip);
   const T mzc = mzf(t), mzjp = mzf(t + 1), mzip = mzf(t + RJ);
   // no-slip boundary term of e12 (absent under free slip)
-  const T huc = SD ? (T)0 : MUc - sMU[t - 1], hujp = SD ? (T)0 : sMU[t + 1] - MUc;
-  const T hvc = SD ? (T)0 : MVc - sMV[t - RJ], hvip = SD ? (T)0 : sMV[t + RJ] - MVc;
+  const T huc = sdr ? (T)0 : MUc - sMU[t - 1], hujp = sdr ? (T)0 : sMU[t + 1] - MUc;
+  const T hvc = sdr ? (T)0 : MVc - sMV[t - RJ], hvip = sdr ? (T)0 : sMV[t + RJ] - MVc;
   const T A1 = (T)0.5 * mzc * rdyU + huc * rdyU;
   const T A2jp = -(T)0.5 * mzjp * rdyU_jp + hujp * rdyU_jp;
   const T A3 = (T)0.5 * mzc * rdxV + hvc * rdxV;
@@ -621,9 +667,14 @@ __global__ __launch_bounds__(RI* RJ) void adj_substep_kernel(const AdjArgs<T>* _
     lu_out = lu_out + (T)0.25 * (T)sc.sinWat * (sU[t] + sU[t - 1]);
     lv_out = lv_out - (T)0.25 * (T)sc.sinWat * (sV[t] + sV[t - RJ]);
   }
-  if (SD) {  // the ice speed at the two c-points of this u (v) point
+  if (sdr) {  // the ice speed at the two c-points of this u (v) point
     lu_out = lu_out + (sZ[t] + sZ[t - RJ]);
     lv_out = lv_out + (sCXV[t] + sCXV[t - 1]);
+  }
+  if (bas) {  // the basal coefficient's velocity argument at the two c-points
+    lu_out = lu_out + (((T)0.5 * (u * mInUc)) * mInUc) * (sLTB[t] + sLTB[t - RJ]);
+    lv_out = lv_out + (((T)0.5 * (v * mInVc)) * mInVc) * (sLTB[t] + sLTB[t - 1]);
+    stg(acc.h, c, ldg(acc.h, c) + aHadd);
   }
   stg(L.out[0], c, lu_out);
   stg(L.out[1], c, lv_out);
@@ -658,32 +709,45 @@ struct AdjTileCfg {
 constexpr AdjTileCfg kAdjTiles[] = {{16, 32}, {24, 32}, {12, 64}, {32, 32}};
 constexpr int kAdjNumTiles = sizeof(kAdjTiles) / sizeof(kAdjTiles[0]);
 
-template <typename T, int RI, int RJ>
+template <typename T, int RI, int RJ, bool XB>
 void adj_launch_shape(const AdjArgs<T>* a, const EvpArgBlock<T>* b, LamSet<T> L, const Geom& g,
-                      int n_members, bool sd, hipStream_t st) {
+                      int n_members, hipStream_t st) {
   constexpr int TI = RI - 8, TJ = RJ - 8;
   const int tiles_j = (g.NY + TJ - 1) / TJ;
   const int n_tiles = ((g.NX + TI - 1) / TI) * tiles_j;
   const int per_xcd = (n_tiles + 7) / 8;
-  if (sd)
-    hipLaunchKernelGGL((adj_substep_kernel<T, double, RI, RJ, true>), dim3(per_xcd * 8, n_members),
-                       dim3(RI * RJ), 0, st, a, b, L, n_tiles, tiles_j, per_xcd);
-  else
-    hipLaunchKernelGGL((adj_substep_kernel<T, double, RI, RJ, false>), dim3(per_xcd * 8, n_members),
-                       dim3(RI * RJ), 0, st, a, b, L, n_tiles, tiles_j, per_xcd);
+  hipLaunchKernelGGL((adj_substep_kernel<T, double, RI, RJ, XB>), dim3(per_xcd * 8, n_members),
+                     dim3(RI * RJ), 0, st, a, b, L, n_tiles, tiles_j, per_xcd);
 }
 
 template <typename T>
 void adj_launch(int cfg, const AdjArgs<T>* a, const EvpArgBlock<T>* b, LamSet<T> L, const Geom& g,
-                int n_members, bool sd, hipStream_t st) {
+                int n_members, bool xb, hipStream_t st) {
+  if (xb && sizeof(T) == 8) {
+    // the extra-branch FP64 kernel: its LDS does not fit a 768-thread
+    // workgroup, so it exists as 16 x 32 only
+    adj_launch_shape<T, 16, 32, true>(a, b, L, g, n_members, st);
+    return;
+  }
+  if (xb) {
+    if constexpr (sizeof(T) == 4) {
+      switch (cfg) {
+        case 1: adj_launch_shape<T, 24, 32, true>(a, b, L, g, n_members, st); break;
+        case 2: adj_launch_shape<T, 12, 64, true>(a, b, L, g, n_members, st); break;
+        case 3: adj_launch_shape<T, 32, 32, true>(a, b, L, g, n_members, st); break;
+        default: adj_launch_shape<T, 16, 32, true>(a, b, L, g, n_members, st); break;
+      }
+    }
+    return;
+  }
   switch (cfg) {
-    case 1: adj_launch_shape<T, 24, 32>(a, b, L, g, n_members, sd, st); break;
-    case 2: adj_launch_shape<T, 12, 64>(a, b, L, g, n_members, sd, st); break;
+    case 1: adj_launch_shape<T, 24, 32, false>(a, b, L, g, n_members, st); break;
+    case 2: adj_launch_shape<T, 12, 64, false>(a, b, L, g, n_members, st); break;
     case 3:  // 1024 threads: FP32 only (the FP64 kernel's registers do not fit)
-      if constexpr (sizeof(T) == 4) adj_launch_shape<T, 32, 32>(a, b, L, g, n_members, sd, st);
-      else adj_launch_shape<T, 24, 32>(a, b, L, g, n_members, sd, st);
+      if constexpr (sizeof(T) == 4) adj_launch_shape<T, 32, 32, false>(a, b, L, g, n_members, st);
+      else adj_launch_shape<T, 24, 32, false>(a, b, L, g, n_members, st);
       break;
-    default: adj_launch_shape<T, 16, 32>(a, b, L, g, n_members, sd, st); break;
+    default: adj_launch_shape<T, 16, 32, false>(a, b, L, g, n_members, st); break;
   }
 }
 
@@ -694,6 +758,7 @@ struct PreAdjArgs {
   const T *h, *A, *M, *MU, *MV, *pstarF, *u, *v, *uW, *vW, *fC, *rdxC, *rdyC, *ssh, *sp, *AW, *AS;
   // adjoints of the EVP inputs
   const T *aP, *amC, *amU, *amV, *aWFX, *aWFY;
+  const T* ahb;  // thickness adjoint of the basal drag (NULL: none)
   // outputs
   T *apstar, *ah, *ahs, *aCaN, *aCaS;
   T pstar, cStar, rhoI, rhoS, rhoAir, wmin, wmin2, grav, rrho, sinWin, cosWin;
@@ -739,7 +804,7 @@ __global__ __launch_bounds__(BLK) void pre_evp_adjoint_kernel(Geom g, PreAdjArgs
   const T amV_jp = a.amV[mo + AT(i, jp)] - a.aWFY[mo + AT(i, jp)] * a.rdyC[mo + AT(i, jp)] * (phi_at(i, jp) - phi);
   // mass: massC = rhoI h + rhoS hs, massU = (massC + massC[i-1])/2, massV likewise in j
   const T amC = a.amC[mo + c] + (T)0.5 * (amU_c + amU_ip) + (T)0.5 * (amV_c + amV_jp);
-  a.ah[mo + c] = hb + a.rhoI * amC;
+  a.ah[mo + c] = (hb + a.rhoI * amC) + (a.ahb ? a.ahb[mo + c] : (T)0);
   a.ahs[mo + c] = a.rhoS * amC;
   // air drag: WindForcingX = 0.5 (tau_c + tau_c[i-1]) MU AreaW + tilt, tau_c linear in Ca
   T dtx, dty;
@@ -862,7 +927,8 @@ int check_adj(const vh_domain* d, const vh_settings* s, const vh_fields* f, int 
              "adjoint: only non-adaptive EVP is supported");
   VH_REQUIRE(!(s->noSlip != 0.0 && s->secondOrderBC != 0.0), VH_EUNSUPPORTED,
              "adjoint: the second-order no-slip boundary term is not supported");
-  VH_REQUIRE(s->basalDragK2 == 0.0, VH_EUNSUPPORTED, "adjoint: basal drag not supported");
+  VH_REQUIRE(s->basalDragK2 == 0.0 || (f->R_low && f->Area && f->hIceMean), VH_EINVAL,
+             "adjoint: basal drag needs R_low, Area, hIceMean");
   const bool kt = f->k1AtC || f->k2AtC || f->k1AtZ || f->k2AtZ;
   VH_REQUIRE(!kt || (f->k1AtC && f->k2AtC && f->k1AtZ && f->k2AtZ), VH_EINVAL,
              "adjoint: k1AtC/k2AtC/k1AtZ/k2AtZ must be all set or all NULL");
@@ -880,7 +946,7 @@ inline int kterms_of(const vh_fields* f) {
 template <typename T>
 EvpArgBlock<T> proto_block(const Geom& g, const vh_settings* s, const vh_fields* f) {
   EvpArgBlock<T> ab{};
-  fill_evp_scalars(s, kterms_of(f), 0, &ab.sc);
+  fill_evp_scalars(s, kterms_of(f), s->basalDragK2 != 0.0 ? 1 : 0, &ab.sc);
   fill_evp_const(f, &ab.c);
   ab.g = g;
   ab.nlaunch = 1;
@@ -994,19 +1060,21 @@ int adjoint(const Geom& g, const vh_domain* d, const vh_settings* s, const vh_fi
              L.total, wsb);
   AdjArgs<T> av{};
   fill_evp_const(f, &av.c);
-  fill_evp_scalars(s, kterms_of(f), 0, &av.sc);
+  fill_evp_scalars(s, kterms_of(f), s->basalDragK2 != 0.0 ? 1 : 0, &av.sc);
   av.g = g;
   av.acc = AdjAcc<T>{(T*)o->adj_SeaIceStrength, (T*)o->adj_SeaIceMassC, (T*)o->adj_SeaIceMassU,
                      (T*)o->adj_SeaIceMassV, (T*)o->adj_AreaW, (T*)o->adj_AreaS,
                      (T*)o->adj_WindForcingX, (T*)o->adj_WindForcingY, (T*)o->adj_uNm1,
-                     (T*)o->adj_vNm1, (T*)o->adj_waterDrag_north, (T*)o->adj_waterDrag_south};
-  const void* accp[kAdjAccFields] = {av.acc.P, av.acc.mC, av.acc.mU, av.acc.mV,
-                                     av.acc.AW, av.acc.AS, av.acc.WFX, av.acc.WFY,
-                                     av.acc.u0, av.acc.v0, av.acc.DN, av.acc.DS};
+                     (T*)o->adj_vNm1, (T*)o->adj_waterDrag_north, (T*)o->adj_waterDrag_south,
+                     (T*)o->adj_hIceMean};
+  const void* accp[kAdjAccFields] = {av.acc.P,   av.acc.mC, av.acc.mU, av.acc.mV, av.acc.AW,
+                                     av.acc.AS,  av.acc.WFX, av.acc.WFY, av.acc.u0, av.acc.v0,
+                                     av.acc.DN,  av.acc.DS, av.acc.h};
   const void* outs[5] = {o->adj_uIce, o->adj_vIce, o->adj_sigma1, o->adj_sigma2, o->adj_sigma12};
+  const bool basal = s->basalDragK2 != 0.0;
   for (int q = 0; q < kAdjAccFields; ++q)
-    VH_REQUIRE(accp[q] != nullptr || q == 4 || q == 5 || q == 8 || q == 9, VH_EINVAL,
-               "adjoint: null output");
+    VH_REQUIRE(accp[q] != nullptr || q == 4 || q == 5 || q == 8 || q == 9 || (q == 12 && !basal),
+               VH_EINVAL, "adjoint: null output");
   VH_REQUIRE((av.acc.AW == nullptr) == (av.acc.AS == nullptr) &&
                  (av.acc.u0 == nullptr) == (av.acc.v0 == nullptr),
              VH_EINVAL, "adjoint: adj_AreaW/adj_AreaS and adj_uNm1/adj_vNm1 go in pairs");
@@ -1028,7 +1096,8 @@ int adjoint(const Geom& g, const vh_domain* d, const vh_settings* s, const vh_fi
   tiles_of(cfg, g, &n_tiles, &tiles_j);
   const int acfg = adj_tile_cfg<T>(field_elems(g) * (size_t)nm);
   const bool overlap = adj_overlap();
-  const bool sd = s->noSlip == 0.0;  // free slip with lateral drag
+  // the extra-branch instantiation: free slip with lateral drag, basal drag
+  const bool sd = s->noSlip == 0.0 || s->basalDragK2 != 0.0;
   T* lam[2][5];
   for (int q = 0; q < 5; ++q) {
     lam[0][q] = wsf<T>(ws, L.off_lam, L.fb, q);
@@ -1085,7 +1154,7 @@ int adjoint(const Geom& g, const vh_domain* d, const vh_settings* s, const vh_fi
     hipLaunchKernelGGL(adj_finish_kernel<T>, grid, dim3(BLK), 0, cs, a_dev, sets(cur), n_all);
   };
   // the key holds every choice baked into the recorded launches: shapes,
-  // overlap and the lateral-drag instantiation
+  // overlap and the extra-branch instantiation
   return run_recorded<T>("adjr", g, L, ((cfg * 16 + acfg) * 2 + (overlap ? 1 : 0)) * 2 + (sd ? 1 : 0),
                          nm, ws, record, st);
 }
@@ -1118,6 +1187,7 @@ int pre_evp_adjoint(const Geom& g, const vh_settings* s, const vh_fields* f, con
   a.amV = evp_ptr<T>(in->adj_SeaIceMassV);
   a.aWFX = evp_ptr<T>(in->adj_WindForcingX);
   a.aWFY = evp_ptr<T>(in->adj_WindForcingY);
+  a.ahb = evp_ptr<T>(in->adj_hIceMean);
   a.apstar = (T*)o->adj_pStar;
   a.ah = (T*)o->adj_hIceMean;
   a.ahs = (T*)o->adj_hSnowMean;
