@@ -119,11 +119,14 @@ def test_adjoint_host_side_validation():
     assert m.value > n.value  # every state kept
     with pytest.raises(N.VerisHipError, match="VH_EINVAL"):
         N.call("vh_evp_adjoint_workspace_bytes", ctypes.byref(dom), 0, 10, ctypes.byref(n))
-    sett = N.settings_struct(make_settings(basalDragK2=8.0))
     fields = N.Fields()
-    with pytest.raises(N.VerisHipError, match="VH_EUNSUPPORTED.*basal"):
-        N.call("vh_evp_forward_checkpointed", ctypes.byref(dom), ctypes.byref(sett),
-               ctypes.byref(fields), 120, 10, None, None, None, None, None, None, 0, None)
+    for kw, msg in (({"useAdaptiveEVP": True}, "VH_EUNSUPPORTED.*non-adaptive"),
+                    ({"secondOrderBC": True}, "VH_EUNSUPPORTED.*second-order"),
+                    ({"basalDragK2": 8.0}, "VH_EINVAL.*basal drag needs")):
+        sett = N.settings_struct(make_settings(**kw))
+        with pytest.raises(N.VerisHipError, match=msg):
+            N.call("vh_evp_forward_checkpointed", ctypes.byref(dom), ctypes.byref(sett),
+                   ctypes.byref(fields), 120, 10, None, None, None, None, None, None, 0, None)
     sett = N.settings_struct(make_settings())
     with pytest.raises(N.VerisHipError, match="VH_EINVAL.*missing input"):
         N.call("vh_evp_forward_checkpointed", ctypes.byref(dom), ctypes.byref(sett),
