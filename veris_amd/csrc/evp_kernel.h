@@ -433,6 +433,15 @@ __device__ inline int map_axis(int p, int N, int n, int filled) {
   return fold(p, N);
 }
 
+// map_axis<K>(p + 1, ...) from q = map_axis<K>(p, ...): the next index of the
+// same axis, without the remainder (hoisted out of the substep loop, the
+// remainder's rare path would otherwise run for every lane of every tile)
+template <int K>
+__device__ inline int next_axis(int q, int N, int n, int filled) {
+  if (K > 1 && filled) return q + 1 == 2 + n ? 2 : q + 1;
+  return q + 1 == N ? 0 : q + 1;
+}
+
 template <typename T, typename TD, int RI, int RJ, int K, bool BATCHED, typename A = T,
           bool SO = false, bool RES = false>
 __global__ __launch_bounds__(RI* RJ) void evp_substep_kernel(const EvpArgBlock<T>* __restrict__ ab,
@@ -763,8 +772,8 @@ __global__ __launch_bounds__(RI* RJ) void evp_substep_kernel(const EvpArgBlock<T
           const A cd = ((D * D) * w > (A)sc.cDragMin2) ? D * evp_sqrt(w) : (A)sc.cDragMin;
           sCD[t] = cd * Mc;
           if (sc.basal) {
-            const int cip = map_axis<K>(i + 1, g.NX, g.nx, g.fill_x) * g.NY + gj;
-            const int cjp = gi * g.NY + map_axis<K>(j + 1, g.NY, g.ny, g.fill_y);
+            const int cip = next_axis<K>(gi, g.NX, g.nx, g.fill_x) * g.NY + gj;
+            const int cjp = gi * g.NY + next_axis<K>(gj, g.NY, g.ny, g.fill_y);
             const A um = u * mInUc, umip = uip * ldg(f.mInU , oo + (unsigned)cip);
             const A vm = v * mInVc, vmjp = vjp * ldg(f.mInV , oo + (unsigned)cjp);
             A tb = (A)0.25 * (((um * um + umip * umip) + vm * vm) + vmjp * vmjp);
