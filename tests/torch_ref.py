@@ -5,7 +5,7 @@ It stands in for ``jax.grad`` of the reference (JAX is absent here) when the
 hand-written HIP adjoint is checked.  It follows the reference's expressions
 (veris/evp_solver.py:22-319, veris/dynamics_routines.py:5-302,
 veris/averaging.py:4-19, veris/area_mass.py:5-28, veris/dynsolver.py:7-84)
-without adaptive EVP or the second-order boundary term, optionally with the
+without the second-order boundary term, optionally with adaptive EVP, the
 water-stress turning term, the metric terms, free slip with lateral drag
 and basal drag, written with torch.roll, and defines the derivative of sqrt at 0 as 0 (the reference's
 Delta = sqrt(Delta^2) and drag sqrt are not differentiable there; the HIP
@@ -13,6 +13,8 @@ adjoint uses the same convention).
 """
 
 from __future__ import annotations
+
+import math
 
 import torch
 
@@ -133,9 +135,17 @@ def evp_substep(vs, sett, p, u, v, u0, v0, s1, s2, s12, cw):
     nrm = torch.where(n > 0, 1.0 / torch.where(n > 0, n, torch.ones_like(n)), torch.zeros_like(n))
     zz = zeta + im1(zeta)
     zz = nrm * (zz + jm1(zz))
-    s1 = (s1 * (alpha - 1) + (2 * zeta * ep - press)) / alpha * M
-    s2 = (s2 * (alpha - 1) + 2 * zeta * em * 0.25) / alpha * M
-    s12 = (s12 * (alpha - 1) + 2 * zz * e12 * 0.25) / alpha
+    aC = aZ = alpha
+    if sett.useAdaptiveEVP:  # veris/evp_solver.py:65-73, 97-100, 266-271
+        cfac = sett.deltatDyn * 4 * (math.pi * sett.aEvpCoeff) ** 2
+        mC0 = torch.clamp(p["SeaIceMassC"], min=1e-4)
+        aC = ssqrt(zeta * cfac / mC0 * vs["recip_rA"]) * M
+        aC = torch.maximum(aC, torch.full_like(aC, sett.aEVPalphaMin))
+        aZ = 0.5 * (aC + jm1(aC))
+        aZ = 0.5 * (aZ + im1(aZ))
+    s1 = (s1 * (aC - 1) + (2 * zeta * ep - press)) / aC * M
+    s2 = (s2 * (aC - 1) + 2 * zeta * em * 0.25) / aC * M
+    s12 = (s12 * (aZ - 1) + 2 * zz * e12 * 0.25) / aZ
     s11 = 0.5 * (s1 + s2)
     s22 = 0.5 * (s1 - s2)
     a = s11 * vs["dyV"]
@@ -169,8 +179,12 @@ def evp_substep(vs, sett, p, u, v, u0, v0, s1, s2, s12, cw):
         iny = iny + sg * sett.sinWat * 0.5 * Y * lmV
     fx = p["WindForcingX"] + inx * p["AreaW"] + 0.5 * (fv + im1(fv))
     fy = p["WindForcingY"] + iny * p["AreaS"] - 0.5 * (fu + jm1(fu))
-    bU = sett.evpBeta * sett.recip_deltatDyn
+    bU = bV = sett.evpBeta * sett.recip_deltatDyn
+    if sett.useAdaptiveEVP:  # veris/evp_solver.py:148-155
+        bU = 0.5 * (aC + im1(aC)) * sett.recip_deltatDyn
+        bV = 0.5 * (aC + jm1(aC)) * sett.recip_deltatDyn
     bP1 = bU + sett.recip_deltatDyn
+    bP1V = bV + sett.recip_deltatDyn
     cbU = cbV = 0.0
     if sett.basalDragK2 != 0:  # basal drag (veris/dynamics_routines.py:55-96)
         A, h = vs["Area"], p.get("hIceMean", vs["hIceMean"])
@@ -183,7 +197,7 @@ def evp_substep(vs, sett, p, u, v, u0, v0, s1, s2, s12, cw):
         cb = torch.where(A > 0.01, val, torch.zeros_like(val))
         cbU, cbV = 0.5 * (cb + im1(cb)), 0.5 * (cb + jm1(cb))
     denU = mU * bP1 + p["AreaW"] * (cdU * sett.cosWat + cbU)
-    denV = mV * bP1 + p["AreaS"] * (cdV * sett.cosWat + cbV)
+    denV = mV * bP1V + p["AreaS"] * (cdV * sett.cosWat + cbV)
     denU = torch.where(denU == 0, torch.ones_like(denU), denU)
     denV = torch.where(denV == 0, torch.ones_like(denV), denV)
     if not sett.noSlip:  # lateral drag (veris/dynamics_routines.py:98-140, evp_solver.py:169-176)
@@ -199,7 +213,7 @@ def evp_substep(vs, sett, p, u, v, u0, v0, s1, s2, s12, cw):
         denU = denU + mU * sett.sideDragCoeff * mskU / (spU + sett.sideDragU0)
         denV = denV + mV * sett.sideDragCoeff * mskV / (spV + sett.sideDragU0)
     un = MU * (bU * mU * u + mU * sett.recip_deltatDyn * u0 + fx + divx) / denU
-    vn = MV * (bU * mV * v + mV * sett.recip_deltatDyn * v0 + fy + divy) / denV
+    vn = MV * (bV * mV * v + mV * sett.recip_deltatDyn * v0 + fy + divy) / denV
     return fill(sett, un), fill(sett, vn), s1, s2, s12
 
 

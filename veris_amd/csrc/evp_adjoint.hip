@@ -17,9 +17,9 @@
 // Non-smooth points: d sqrt(x)/dx is taken as 0 at x = 0 (Delta = 0 on
 // motionless or masked cells, zero relative velocity); the denominator guard
 // and the drag regime switch contribute no derivative.  Supported settings:
-// non-adaptive EVP without the second-order no-slip term, with no-slip
-// boundaries or free slip with lateral drag, with or without basal drag, the
-// turning angle and the metric terms.
+// EVP without the second-order no-slip term, fixed or adaptive relaxation,
+// with no-slip boundaries or free slip with lateral drag, with or without
+// basal drag, the turning angle and the metric terms.
 #include <algorithm>
 #include <cstdio>
 #include <cstdlib>
@@ -221,7 +221,9 @@ __device__ inline T gather_filled(const Geom& g, const T* a, int i, int j) {
 // denominator, whose adjoint reaches the ice speed at c;
 // veris/dynamics_routines.py:98-140, veris/evp_solver.py:169-176), and basal
 // drag (the c-point coefficient in the denominators, whose adjoint reaches
-// the velocities and the thickness; veris/dynamics_routines.py:55-96).
+// the velocities and the thickness; veris/dynamics_routines.py:55-96), and
+// adaptive EVP (relaxation weights alpha at c and z and beta at u/v points
+// from zeta and the mass; veris/evp_solver.py:65-73, 97-100, 148-155).
 template <typename T, typename TD, int RI, int RJ, bool XB>
 __global__ __launch_bounds__(RI* RJ) void adj_substep_kernel(const AdjArgs<T>* __restrict__ a,
                                                              const EvpArgBlock<T>* __restrict__ blk,
@@ -242,8 +244,11 @@ __global__ __launch_bounds__(RI* RJ) void adj_substep_kernel(const AdjArgs<T>* _
   // through sZ/sCXV, free after S2.  Basal drag: the coefficient at c (C to
   // S1), the denominator adjoints times the area at the u/v points (S1 to
   // S2) and the adjoint of the coefficient's velocity argument at c (S3 to S4).
+  // Adaptive EVP: alpha at c (C to S2), the beta adjoints of the u/v points
+  // (S1 to S2) and the alpha adjoints of the z-points (S2 to S3).
   constexpr int NX_ = XB ? NT : 1;
   __shared__ T sSPD[NX_], sLSU[NX_], sLSV[NX_], sCBc[NX_], sCBU[NX_], sCBV[NX_], sLTB[NX_];
+  __shared__ T sAl[NX_], sLBU[NX_], sLBV[NX_], sLAZ[NX_];
 
   const int bid = blockIdx.x;
   const int tile = (bid & 7) * per_xcd + (bid >> 3);
@@ -326,6 +331,13 @@ __global__ __launch_bounds__(RI* RJ) void adj_substep_kernel(const AdjArgs<T>* _
   const T mC = ldg(k.massC, c);
   const bool sdr = XB && sc.sidedrag;  // uniform; false in the default instantiation
   const bool bas = XB && sc.basal;
+  const bool ada = XB && sc.adaptive;
+  T so1 = 0, so2 = 0, so12 = 0;  // input stresses (adaptive: the alpha derivatives)
+  if (ada) {
+    so1 = ldg(blk->in[2], c);
+    so2 = ldg(blk->in[3], c);
+    so12 = ldg(blk->in[4], c);
+  }
   T FuC = 0, FvC = 0;  // coastline form factors
   if (sdr && sc.coast) {
     FuC = ldg(k.Fu, c);
@@ -392,6 +404,7 @@ __global__ __launch_bounds__(RI* RJ) void adj_substep_kernel(const AdjArgs<T>* _
   T zeta = 0, dC = 0, ep = 0, em = 0, sa = 0, sb = 0, ww = 0, sw = 0;
   T su = 0, sv = 0, spd = 0;  // lateral drag: u + u(i+1), v + v(j+1) and the speed at c
   T cTB = 0, cH = 0;          // basal drag: its derivatives at c
+  T aC = 0, cZa = 0, cMa = 0, prs = 0;  // adaptive: alpha at c, d alpha / d (zeta, mC), press
   if (ar >= 1 && br >= 1 && ar <= RI - 2 && br <= RJ - 2) {
     const T uip = sU[t + RJ], vjp = sV[t + 1];
     const T rdxU = met(3), rdyV = met(4);
@@ -436,6 +449,23 @@ __global__ __launch_bounds__(RI* RJ) void adj_substep_kernel(const AdjArgs<T>* _
     zeta = (T)zd;
     dC = (T)dCd;
     sZ[t] = zeta;
+    if (ada) {  // alpha at c (the forward kernel's expression order) and its derivatives
+      const T mmax = fmax(mC, (T)1e-4);
+      const T Y = ((zeta * (T)sc.evpcfac) / mmax) * met(5);
+      const T sY = sqrt(Y);
+      const T raw = sY * Mc;
+      aC = fmax(raw, (T)sc.alphaMin);
+      sAl[t] = aC;
+      const bool act = raw > (T)sc.alphaMin && sY > (T)0;
+      const T dY = act ? (Mc * (T)0.5) / sY : (T)0;  // d alpha / d Y
+      cZa = dY * (((T)sc.evpcfac / mmax) * met(5));
+      cMa = mC > (T)1e-4 ? -dY * (Y / mmax) : (T)0;
+      // the replacement pressure, for the sigma1 relaxation's alpha derivative
+      T prep = (((T)2.0 * zeta) * dC) * (T)sc.pr;
+      if (!sc.tens0) prep = prep / (T)sc.tens1p;
+      prs = (P * (T)sc.pr1m) + prep;
+      if (!sc.tens0) prs = prs * (T)sc.tens1m;
+    }
     sa = sA[t] + sA[t + RJ];
     sb = sB[t] + sB[t + 1];
     ww = (T)0.25 * (sa * sa + sb * sb);
@@ -468,14 +498,22 @@ __global__ __launch_bounds__(RI* RJ) void adj_substep_kernel(const AdjArgs<T>* _
   // ---- S1: momentum update at the u- and v-point of this cell
   const T cw = (T)sc.cosWat, bU = (T)sc.betaU, bP1 = (T)sc.betaP1, rdt = (T)sc.rdt;
   T nbU = 0, dbU = 0, nbV = 0, dbV = 0, aAW = 0, aAS = 0, aMU = 0, aMV = 0;
+  T bUu = bU, bVv = bU;  // beta / deltatDyn at this u- and v-point (adaptive: from alpha)
   if (ar >= 2 && br >= 2 && ar <= RI - 2 && br <= RJ - 2) {
+    T bP1u = bP1, bP1v = bP1;
+    if (ada) {  // veris/evp_solver.py:148-155
+      bUu = ((T)0.5 * (sAl[t] + sAl[t - RJ])) * rdt;
+      bVv = ((T)0.5 * (sAl[t] + sAl[t - 1])) * rdt;
+      bP1u = bUu + rdt;
+      bP1v = bVv + rdt;
+    }
     const T cdAU = (T)0.5 * (sCD[t] + sCD[t - RJ]);
     const T cdAV = (T)0.5 * (sCD[t] + sCD[t - 1]);
     // basal drag at the u/v point (zero without it: the same sum as the forward's)
     const T cbU = bas ? (T)0.5 * (sCBc[t] + sCBc[t - RJ]) : (T)0;
     const T cbV = bas ? (T)0.5 * (sCBc[t] + sCBc[t - 1]) : (T)0;
-    const T denU = mU * bP1 + AW * ((cdAU * cw) + cbU);
-    const T denV = mV * bP1 + AS * ((cdAV * cw) + cbV);
+    const T denU = mU * bP1u + AW * ((cdAU * cw) + cbU);
+    const T denV = mV * bP1v + AS * ((cdAV * cw) + cbV);
     const bool gU = denU == (T)0, gV = denV == (T)0;
     T dU = gU ? (T)1 : denU, dV = gV ? (T)1 : denV;
     // lateral drag, added after the zero guard as the forward adds it
@@ -518,8 +556,12 @@ __global__ __launch_bounds__(RI* RJ) void adj_substep_kernel(const AdjArgs<T>* _
     }
     aAW = nbU * (cdAU * cw * uoc + turnU) + dbU * (cdAU * cw + cbU);  // AreaW, AreaS adjoints
     aAS = nbV * (cdAV * cw * voc + turnV) + dbV * (cdAV * cw + cbV);
-    aMU = nbU * (bU * u + rdt * uNm1) + dbU * bP1;  // SeaIceMassU, SeaIceMassV adjoints
-    aMV = nbV * (bU * v + rdt * vNm1) + dbV * bP1;
+    aMU = nbU * (bUu * u + rdt * uNm1) + dbU * bP1u;  // SeaIceMassU, SeaIceMassV adjoints
+    aMV = nbV * (bVv * v + rdt * vNm1) + dbV * bP1v;
+    if (ada) {  // beta adjoints, times the averaging weight and 1 / deltatDyn
+      sLBU[t] = ((T)0.5 * rdt) * (nbU * mU * u + dbU * mU);
+      sLBV[t] = ((T)0.5 * rdt) * (nbV * mV * v + dbV * mV);
+    }
     if (bas) {  // the basal coefficient's share of the denominators
       sCBU[t] = dbU * AW;
       sCBV[t] = dbV * AS;
@@ -535,7 +577,7 @@ __global__ __launch_bounds__(RI* RJ) void adj_substep_kernel(const AdjArgs<T>* _
 
   // ---- S2: stresses, drag and Coriolis at c, sigma12 at z
   T Padd = 0, mCadd = 0, Db = 0, zbp = 0, epb0 = 0, emb0 = 0, dbp = 0;
-  T lo1 = 0, lo2 = 0, lo12 = 0, wU = 0, wV = 0, qU = 0, qV = 0, ltb = 0, aHadd = 0;
+  T lo1 = 0, lo2 = 0, lo12 = 0, wU = 0, wV = 0, qU = 0, qV = 0, ltb = 0, aHadd = 0, laC = 0;
   if (ar >= 3 && br >= 3 && ar <= RI - 3 && br <= RJ - 3) {
     if (bas) {  // basal coefficient adjoint at c from its four faces
       const T lcb = (T)0.5 * (sCBU[t] + sCBU[t + RJ]) + (T)0.5 * (sCBV[t] + sCBV[t + 1]);
@@ -576,9 +618,11 @@ __global__ __launch_bounds__(RI* RJ) void adj_substep_kernel(const AdjArgs<T>* _
     const T s22b = met(7) * (Yc - Yjp);  // dxU
     const T s1t = l1 + (T)0.5 * (s11b + s22b);
     const T s2t = l2 + (T)0.5 * (s11b - s22b);
-    const T H1 = s1t * (T)sc.den1 * Mc, H2 = s2t * (T)sc.den2 * Mc;
-    lo1 = H1 * (T)sc.alphaM1;
-    lo2 = H2 * (T)sc.alphaM1;
+    // relaxation weights: fixed, or this substep's alpha at c (adaptive)
+    const T den1c = ada ? (T)1.0 / aC : (T)sc.den1, am1c = ada ? aC - (T)1 : (T)sc.alphaM1;
+    const T H1 = s1t * den1c * Mc, H2 = s2t * den1c * Mc;
+    lo1 = H1 * am1c;
+    lo2 = H2 * am1c;
     const T pb = -H1;
     const T pfac = (T)sc.tens1m * (T)2 * (T)sc.pr / (T)sc.tens1p;
     Padd = pb * (T)sc.tens1m * (T)sc.pr1m;
@@ -587,8 +631,15 @@ __global__ __launch_bounds__(RI* RJ) void adj_substep_kernel(const AdjArgs<T>* _
     emb0 = H2 * (T)0.5 * zeta;
     dbp = pb * pfac * zeta;
     const T s12t = l12 + met(8) * (Xjm - Xc) + met(9) * (Yim - Yc);  // dxV, dyU
-    const T G = s12t * (T)sc.den2;
-    lo12 = G * (T)sc.alphaM1;
+    T den2z = (T)sc.den2, am1z = (T)sc.alphaM1, aZ = 0;
+    if (ada) {  // alpha at z (veris/evp_solver.py:97-100)
+      aZ = (T)0.5 * (sAl[t] + sAl[t - 1]);
+      aZ = (T)0.5 * (aZ + (T)0.5 * (sAl[t - RJ] + sAl[t - RJ - 1]));
+      den2z = (T)1.0 / aZ;
+      am1z = aZ - (T)1;
+    }
+    const T G = s12t * den2z;
+    lo12 = G * am1z;
     const T nz = (Mc + sM[t - RJ]) + (sM[t - 1] + sM[t - RJ - 1]);
     // 1 / nz for the 0/1 mask sums 1..4 (0 -> 0), without a division
     const T nrm = nz == (T)4 ? (T)0.25
@@ -598,12 +649,27 @@ __global__ __launch_bounds__(RI* RJ) void adj_substep_kernel(const AdjArgs<T>* _
     const T zz = nrm * ((sZ[t] + sZ[t - RJ]) + (sZ[t - 1] + sZ[t - RJ - 1]));
     sCD[t] = G * (T)0.5 * e12r * nrm;  // zeta adjoint share of z = this cell, weighted
     sEP[t] = G * (T)0.5 * zz;          // e12 adjoint from sigma12
+    if (ada) {
+      // d sigma_new / d alpha = (sigma_old - forcing) / alpha^2 for sigma1, sigma2 at c
+      // and sigma12 at z; beta of the four faces of this c-point
+      const T divg = (((T)2 * zeta) * ep) - prs;
+      const T tens = ((T)2 * zeta) * em;
+      laC = (H1 * (so1 - divg) + H2 * (so2 - tens * (T)0.25)) * den1c +
+            ((sLBU[t] + sLBU[t + RJ]) + (sLBV[t] + sLBV[t + 1]));
+      const T shear = ((T)2 * zz) * e12r;
+      sLAZ[t] = G * (so12 - shear * (T)0.25) * den2z;
+    }
   }
   __syncthreads();
 
   // ---- S3: zeta and Delta at c
   if (ar >= 3 && br >= 3 && ar <= RI - 4 && br <= RJ - 4) {
-    const T zb = zbp + ((sCD[t] + sCD[t + RJ]) + (sCD[t + 1] + sCD[t + RJ + 1]));
+    T zb = zbp + ((sCD[t] + sCD[t + RJ]) + (sCD[t + 1] + sCD[t + RJ + 1]));
+    if (ada) {  // alpha at c from its four z-points, then to zeta and the mass
+      const T la = laC + (T)0.25 * ((sLAZ[t] + sLAZ[t + 1]) + (sLAZ[t + RJ] + sLAZ[t + RJ + 1]));
+      zb = zb + la * cZa;
+      mCadd = mCadd + la * cMa;
+    }
     const T rreg = (T)1 / (dC + (T)sc.deltaMin);
     Padd += zb * (T)0.5 * (T)sc.tens1p * rreg;
     const T Dbar = dbp - zb * zeta * rreg;
@@ -653,9 +719,9 @@ __global__ __launch_bounds__(RI* RJ) void adj_substep_kernel(const AdjArgs<T>* _
   const T A3 = (T)0.5 * mzc * rdxV + hvc * rdxV;
   const T A4ip = -(T)0.5 * mzip * rdxV_ip + hvip * rdxV_ip;
   const T fmc = fc * mC;
-  T lu_out = nbU * bU * mU + (sAB[t] + sAB[t - RJ]) * mInUc + (sA[t - RJ] - sA[t]) +
+  T lu_out = nbU * bUu * mU + (sAB[t] + sAB[t - RJ]) * mInUc + (sA[t - RJ] - sA[t]) +
              ec * A1 + ejp * A2jp + (T)0.5 * fmc * sFU[t] + (T)0.5 * fmc_im * sFU[t - RJ];
-  T lv_out = nbV * bU * mV + (sBB[t] + sBB[t - 1]) * mInVc + (sB[t - 1] - sB[t]) +
+  T lv_out = nbV * bVv * mV + (sBB[t] + sBB[t - 1]) * mInVc + (sB[t - 1] - sB[t]) +
              ec * A3 + eip * A4ip + (T)0.5 * fmc * sFV[t] + (T)0.5 * fmc_jm * sFV[t - 1];
   if (sc.kterms) {  // the metric terms of e11, e22 (c) and e12 (z)
     const T k1Z = ldg(k.k1Z, c), k2Z = ldg(k.k2Z, c);
@@ -923,8 +989,7 @@ int check_adj(const vh_domain* d, const vh_settings* s, const vh_fields* f, int 
   VH_REQUIRE(n <= (1 << 20), VH_EINVAL, "n_substeps=%d too large", n);
   VH_REQUIRE(d->tile >= 0 && d->tile < kEvpNumTiles && (d->tile == 0 || kEvpTiles[d->tile].k == 1),
              VH_EINVAL, "adjoint: tile %d (needs a single-substep tile shape or 0)", d->tile);
-  VH_REQUIRE(s->useAdaptiveEVP == 0.0 && s->useEVP != 0.0, VH_EUNSUPPORTED,
-             "adjoint: only non-adaptive EVP is supported");
+  VH_REQUIRE(s->useEVP != 0.0, VH_EUNSUPPORTED, "adjoint: the EVP solver must be on");
   VH_REQUIRE(!(s->noSlip != 0.0 && s->secondOrderBC != 0.0), VH_EUNSUPPORTED,
              "adjoint: the second-order no-slip boundary term is not supported");
   VH_REQUIRE(s->basalDragK2 == 0.0 || (f->R_low && f->Area && f->hIceMean), VH_EINVAL,
@@ -1096,8 +1161,9 @@ int adjoint(const Geom& g, const vh_domain* d, const vh_settings* s, const vh_fi
   tiles_of(cfg, g, &n_tiles, &tiles_j);
   const int acfg = adj_tile_cfg<T>(field_elems(g) * (size_t)nm);
   const bool overlap = adj_overlap();
-  // the extra-branch instantiation: free slip with lateral drag, basal drag
-  const bool sd = s->noSlip == 0.0 || s->basalDragK2 != 0.0;
+  // the extra-branch instantiation: free slip with lateral drag, basal drag,
+  // adaptive EVP
+  const bool sd = s->noSlip == 0.0 || s->basalDragK2 != 0.0 || s->useAdaptiveEVP != 0.0;
   T* lam[2][5];
   for (int q = 0; q < 5; ++q) {
     lam[0][q] = wsf<T>(ws, L.off_lam, L.fb, q);
