@@ -460,11 +460,11 @@ int vh_evp_adjoint_workspace_bytes(const vh_domain* d, int32_t n_substeps, int32
 
 /* evp_solver (veris/evp_solver.py:257-319) that also keeps the state every
  * ckpt_every substeps in `workspace` for a later vh_evp_adjoint with the same
- * inputs.  Results equal vh_evp_solver's.  Supported: EVP with fixed or
- * adaptive relaxation, without the second-order no-slip term; no-slip
- * boundaries or free slip with lateral drag; with or without basal drag, the
- * water-stress turning angle (sinWat, cosWat) and the metric terms (k1AtC,
- * k2AtC, k1AtZ, k2AtZ). */
+ * inputs.  Results equal vh_evp_solver's.  Every setting of the subcycle is
+ * supported: fixed or adaptive relaxation; no-slip boundaries (with or
+ * without the second-order term) or free slip with lateral drag; with or
+ * without basal drag, the water-stress turning angle (sinWat, cosWat) and the
+ * metric terms (k1AtC, k2AtC, k1AtZ, k2AtZ). */
 int vh_evp_forward_checkpointed(const vh_domain* d, const vh_settings* s, const vh_fields* f,
                                 int32_t n_substeps, int32_t ckpt_every, void* uIce, void* vIce,
                                 void* sigma1, void* sigma2, void* sigma12, void* workspace,

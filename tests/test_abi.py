@@ -121,7 +121,6 @@ def test_adjoint_host_side_validation():
         N.call("vh_evp_adjoint_workspace_bytes", ctypes.byref(dom), 0, 10, ctypes.byref(n))
     fields = N.Fields()
     for kw, msg in (({"useEVP": False}, "VH_EUNSUPPORTED.*EVP solver"),
-                    ({"secondOrderBC": True}, "VH_EUNSUPPORTED.*second-order"),
                     ({"basalDragK2": 8.0}, "VH_EINVAL.*basal drag needs")):
         sett = N.settings_struct(make_settings(**kw))
         with pytest.raises(N.VerisHipError, match=msg):

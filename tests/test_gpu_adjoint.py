@@ -47,6 +47,8 @@ def _state(kind):
         st = synth.global_latlon(31, 36, 20, -75.0, 80.0, veros_fill=False)
     elif kind == "regional":
         st = synth.regional(32, 0, 30, 22)
+    elif kind == "second_order":  # the second-order no-slip boundary term at the coasts
+        st = synth.global_latlon(31, 36, 20, -75.0, 80.0, secondOrderBC=True)
     elif kind == "adaptive":  # about a tenth of the ice cells above alpha's floor
         st = synth.regional(32, 0, 30, 22, useAdaptiveEVP=True, aEvpCoeff=0.5, aEVPalphaMin=20.0)
     elif kind == "basal":  # grounded keels: water 8 m deep, given as a negative depth
@@ -138,7 +140,7 @@ def _ref_grads(st, n, seed=3):
 
 
 @pytest.mark.parametrize("kind", ["global", "doubly_periodic", "regional", "turning_metric",
-                                  "side_drag", "coastline", "basal", "adaptive"])
+                                  "side_drag", "coastline", "basal", "adaptive", "second_order"])
 def test_checkpointed_forward_is_the_drop_in(cuda, kind):
     st = _state(kind)
     gst = st.to(cuda)
@@ -161,7 +163,8 @@ def test_checkpointed_forward_is_the_drop_in(cuda, kind):
 
 
 @pytest.mark.parametrize("kind", ["global", "doubly_periodic", "regional", "turning_metric",
-                                  "unpacked", "side_drag", "coastline", "basal", "adaptive"])
+                                  "unpacked", "side_drag", "coastline", "basal", "adaptive",
+                                  "second_order"])
 def test_adjoint_matches_torch_autograd(cuda, kind):
     st = _state(kind)
     n = 25
@@ -182,7 +185,8 @@ def test_adjoint_matches_torch_autograd(cuda, kind):
         assert err <= 1e-9, f"{kind} d/d{k}: {err:.3e}"
 
 
-@pytest.mark.parametrize("kind", ["turning_metric", "coastline", "basal", "adaptive"])
+@pytest.mark.parametrize("kind", ["turning_metric", "coastline", "basal", "adaptive",
+                                  "second_order"])
 def test_every_reverse_output_through_the_c_abi(cuda, kind):
     """vh_evp_adjoint called directly with every output requested, including
     the optional AreaW/AreaS and uNm1/vNm1 adjoints that the autograd wrapper
@@ -258,7 +262,8 @@ def test_checkpoint_interval_is_irrelevant(cuda):
         assert np.array_equal(g1[k], g2[k]) and np.array_equal(g1[k], g3[k]), k
 
 
-@pytest.mark.parametrize("kind", ["global", "turning_metric", "coastline", "basal", "adaptive"])
+@pytest.mark.parametrize("kind", ["global", "turning_metric", "coastline", "basal", "adaptive",
+                                  "second_order"])
 def test_adjoint_finite_differences(cuda, kind):
     """20 random directions in (P*, h, Cw_N, Cw_S, Ca_N, Ca_S) space; central
     differences of the GPU forward with relative step 1e-6."""
@@ -308,11 +313,10 @@ def test_fp32_adjoint_against_fp64(cuda):
 def test_adjoint_rejects_unsupported(cuda):
     from veris_amd._native import VerisHipError
 
-    for name, value in (("secondOrderBC", True),):
-        st = _state("global")
-        setattr(st.settings, name, value)
-        with pytest.raises(VerisHipError, match="VH_EUNSUPPORTED"):
-            A.evp_subcycle(st.to(cuda), n_substeps=3)
+    st = _state("global")
+    st.settings.useEVP = False  # free drift has no subcycle to differentiate
+    with pytest.raises(VerisHipError, match="VH_EUNSUPPORTED"):
+        A.evp_subcycle(st.to(cuda), n_substeps=3)
 
 
 def test_backward_after_the_state_is_reassigned(cuda):

@@ -37,6 +37,8 @@ def _state(kind):
     elif kind == "basal":  # grounded keels: water 8 m deep, given as a negative depth
         st = synth.global_latlon(31, 36, 20, -75.0, 80.0, basalDragK2=15.0)
         st.variables.R_low = -np.asarray(st.variables.R_low) * 0.002
+    elif kind == "second_order":
+        st = synth.global_latlon(31, 36, 20, -75.0, 80.0, secondOrderBC=True)
     elif kind == "adaptive":
         # about a tenth of the ice cells above alpha's floor
         st = synth.regional(32, 0, 30, 22, useAdaptiveEVP=True, aEvpCoeff=0.5, aEVPalphaMin=20.0)
@@ -60,7 +62,7 @@ def _torch_vars(st):
 
 
 @pytest.mark.parametrize("kind", ["global", "regional", "turning_metric", "side_drag",
-                                  "coastline", "basal", "adaptive"])
+                                  "coastline", "basal", "adaptive", "second_order"])
 def test_forward_matches_oracle(kind):
     st = _state(kind)
     vs, s = _torch_vars(st), st.settings
@@ -84,7 +86,7 @@ def test_forward_matches_oracle(kind):
 
 
 @pytest.mark.parametrize("kind", ["global", "regional", "turning_metric", "side_drag",
-                                  "coastline", "basal", "adaptive"])
+                                  "coastline", "basal", "adaptive", "second_order"])
 def test_autograd_matches_finite_differences(kind):
     st = _state(kind)
     vs, s = _torch_vars(st), st.settings

@@ -5,7 +5,7 @@ It stands in for ``jax.grad`` of the reference (JAX is absent here) when the
 hand-written HIP adjoint is checked.  It follows the reference's expressions
 (veris/evp_solver.py:22-319, veris/dynamics_routines.py:5-302,
 veris/averaging.py:4-19, veris/area_mass.py:5-28, veris/dynsolver.py:7-84)
-without the second-order boundary term, optionally with adaptive EVP, the
+optionally with adaptive EVP, the second-order boundary term, the
 water-stress turning term, the metric terms, free slip with lateral drag
 and basal drag, written with torch.roll, and defines the derivative of sqrt at 0 as 0 (the reference's
 Delta = sqrt(Delta^2) and drag sqrt are not differentiable there; the HIP
@@ -49,6 +49,14 @@ def jm1(a):
 
 def jp1(a):
     return torch.roll(a, -1, 1)
+
+
+def im2(a):
+    return torch.roll(a, 2, 0)
+
+
+def jm2(a):
+    return torch.roll(a, 2, 1)
 
 
 def fill(sett, a):
@@ -118,6 +126,11 @@ def evp_substep(vs, sett, p, u, v, u0, v0, s1, s2, s12, cw):
     if sett.noSlip:
         e12 = e12 + (2.0 * uavz * vs["recip_dyU"] * (MU - jm1(MU))
                      + 2.0 * vavz * vs["recip_dxV"] * (MV - im1(MV)))
+    if sett.noSlip and sett.secondOrderBC:  # veris/dynamics_routines.py:177-215
+        hu2 = (MU - im1(MU)) / 3.0 * (im2(MU) * im1(MU) + ip1(MU) * MU)
+        hv2 = (MV - jm1(MV)) / 3.0 * (jm2(MV) * jm1(MV) + jp1(MV) * MV)
+        e12 = e12 + 0.5 * (vs["recip_dyU"] * (6.0 * uavz - im2(u) * im1(MU) - ip1(u) * MU) * hu2
+                           + vs["recip_dxV"] * (6.0 * vavz - jm2(v) * jm1(MV) - jp1(v) * MV) * hv2)
     rpd = 1.0 / sett.PlasDefCoeff**2
     q = vs["rAz"] * e12**2
     q = q + ip1(q)

@@ -148,13 +148,13 @@ def evp_subcycle(state, *, pStar=None, hIceMean=None, waterIceDrag=None,
     state's value and are constants; tensors with ``requires_grad`` receive
     gradients in ``backward``: ``pStar`` (0-d or per-cell), ``hIceMean``
     (field), the four drag coefficients (0-d).  Gradients come back on the
-    parameter's device and dtype.  Supported settings: EVP with fixed or
-    adaptive relaxation, without the second-order no-slip term or sea-ice
-    load in the surface tilt; no-slip boundaries or free slip with lateral
-    drag (land neighbours or coastline form factors); with or without basal
-    drag (its thickness dependence included in the hIceMean gradient), the
-    water-stress turning angle (sinWat, cosWat) and the metric terms (k1AtC,
-    k2AtC, k1AtZ, k2AtZ).
+    parameter's device and dtype.  Every setting of the EVP subcycle is
+    supported: fixed or adaptive relaxation; no-slip boundaries (with or
+    without the second-order term) or free slip with lateral drag (land
+    neighbours or coastline form factors); with or without basal drag (its
+    thickness dependence included in the hIceMean gradient), the water-stress
+    turning angle (sinWat, cosWat) and the metric terms (k1AtC, k2AtC, k1AtZ,
+    k2AtZ).  The pre-EVP part does not take sea-ice load in the surface tilt.
     """
     vs, s = state.variables, state.settings
     n = int(s.nEVPsteps if n_substeps is None else n_substeps)

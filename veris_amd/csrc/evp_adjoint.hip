@@ -16,10 +16,10 @@
 //
 // Non-smooth points: d sqrt(x)/dx is taken as 0 at x = 0 (Delta = 0 on
 // motionless or masked cells, zero relative velocity); the denominator guard
-// and the drag regime switch contribute no derivative.  Supported settings:
-// EVP without the second-order no-slip term, fixed or adaptive relaxation,
-// with no-slip boundaries or free slip with lateral drag, with or without
-// basal drag, the turning angle and the metric terms.
+// and the drag regime switch contribute no derivative.  Every setting of the
+// subcycle is supported: fixed or adaptive relaxation, no-slip boundaries
+// (with or without the second-order term) or free slip with lateral drag,
+// with or without basal drag, the turning angle and the metric terms.
 #include <algorithm>
 #include <cstdio>
 #include <cstdlib>
@@ -224,12 +224,18 @@ __device__ inline T gather_filled(const Geom& g, const T* a, int i, int j) {
 // the velocities and the thickness; veris/dynamics_routines.py:55-96), and
 // adaptive EVP (relaxation weights alpha at c and z and beta at u/v points
 // from zeta and the mass; veris/evp_solver.py:65-73, 97-100, 148-155).
-template <typename T, typename TD, int RI, int RJ, bool XB>
+//
+// MODE 0 is the default instantiation, 1 the extra branches, 2 the extra
+// branches and the second-order no-slip boundary term
+// (veris/dynamics_routines.py:177-215): its stencil reaches two cells along
+// the face normal, so MODE 2 recomputes one more ring of cells (R = 5).
+template <typename T, typename TD, int RI, int RJ, int MODE>
 __global__ __launch_bounds__(RI* RJ) void adj_substep_kernel(const AdjArgs<T>* __restrict__ a,
                                                              const EvpArgBlock<T>* __restrict__ blk,
                                                              LamSet<T> L, int n_tiles, int tiles_j,
                                                              int per_xcd) {
-  constexpr int R = 4, TI = RI - 2 * R, TJ = RJ - 2 * R, NT = RI * RJ;
+  constexpr bool XB = MODE >= 1, SO = MODE == 2;
+  constexpr int R = SO ? 5 : 4, TI = RI - 2 * R, TJ = RJ - 2 * R, NT = RI * RJ;
   static_assert((RJ & (RJ - 1)) == 0, "RJ must be a power of two");
   static_assert(TI >= 1 && TJ >= 1, "tile too small");
   __shared__ T sU[NT], sV[NT], sM[NT], sMU[NT], sMV[NT], sA[NT], sB[NT], sE12[NT], sQ[NT], sZ[NT];
@@ -393,6 +399,23 @@ __global__ __launch_bounds__(RI* RJ) void adj_substep_kernel(const AdjArgs<T>* _
       const T hu = MUc - sMU[t - 1];
       const T hv = MVc - sMV[t - RJ];
       e12 = e12 + ((((T)2.0 * uavz) * rdyU) * hu + (((T)2.0 * vavz) * rdxV) * hv);
+    }
+    if (SO && sc.sobc) {
+      // second-order boundary shear, read from memory as the forward kernel
+      // reads it (two cells along the face normal reach past this region's edge)
+      const unsigned cim2 = mo + wrap(i - 2, g.NX) * g.NY + gj, cim1 = mo + wrap(i - 1, g.NX) * g.NY + gj;
+      const unsigned cjm2 = mo + gi * g.NY + wrap(j - 2, g.NY), cjm1 = mo + gi * g.NY + wrap(j - 1, g.NY);
+      const T uim2 = ldg(xu, cim2), uip1 = ldg(xu, cip);
+      const T MUim2 = ldg(k.MU, cim2), MUim1 = ldg(k.MU, cim1), MUip1 = ldg(k.MU, cip);
+      const T vjm2 = ldg(xv, cjm2), vjp1 = ldg(xv, cjp);
+      const T MVjm2 = ldg(k.MV, cjm2), MVjm1 = ldg(k.MV, cjm1), MVjp1 = ldg(k.MV, cjp);
+      T hu2 = (MUc - MUim1) / (T)3.0;
+      T hv2 = (MVc - MVjm1) / (T)3.0;
+      hu2 = hu2 * (MUim2 * MUim1 + MUip1 * MUc);
+      hv2 = hv2 * (MVjm2 * MVjm1 + MVjp1 * MVc);
+      const T tu = (rdyU * ((((T)6.0 * uavz) - uim2 * MUim1) - uip1 * MUc)) * hu2;
+      const T tv = (rdxV * ((((T)6.0 * vavz) - vjm2 * MVjm1) - vjp1 * MVc)) * hv2;
+      e12 = e12 + (T)0.5 * (tu + tv);
     }
     sE12[t] = e12;
     sQ[t] = rAz * (e12 * e12);
@@ -737,6 +760,43 @@ __global__ __launch_bounds__(RI* RJ) void adj_substep_kernel(const AdjArgs<T>* _
     lu_out = lu_out + (sZ[t] + sZ[t - RJ]);
     lv_out = lv_out + (sCXV[t] + sCXV[t - 1]);
   }
+  if (SO && sc.sobc) {
+    // the second-order boundary shear of the z-points whose stencil holds this
+    // u (v): uavz at z = c, c+1 (v: c, c+RJ), u two cells down at z = c+2RJ
+    // and one up at z = c-RJ (v: c+2, c-1), as the reference shifts them
+    auto hu2f = [&](int z) {
+      return ((sMU[z] - sMU[z - RJ]) / (T)3.0) * (sMU[z - 2 * RJ] * sMU[z - RJ] + sMU[z + RJ] * sMU[z]);
+    };
+    auto hv2f = [&](int z) {
+      return ((sMV[z] - sMV[z - 1]) / (T)3.0) * (sMV[z - 2] * sMV[z - 1] + sMV[z + 1] * sMV[z]);
+    };
+    T rAz_i2 = rAz, rAz_im = rAz, rdyU_i2 = rdyU, rdyU_im = rdyU;  // along i: rows hold them
+    T rAz_j2, rAz_jm, rdxV_j2, rdxV_jm;
+    if (packed) {
+      rAz_j2 = sRow[2 * RJ + br + 2];
+      rAz_jm = sRow[2 * RJ + br - 1];
+      rdxV_j2 = sRow[RJ + br + 2];
+      rdxV_jm = sRow[RJ + br - 1];
+    } else {
+      const unsigned ci2 = mo + wrap(gi + 2, g.NX) * g.NY + gj, cj2 = mo + gi * g.NY + wrap(gj + 2, g.NY);
+      rAz_i2 = ldg(k.rAz, ci2);
+      rAz_im = ldg(k.rAz, cim);
+      rdyU_i2 = ldg(k.rdyU, ci2);
+      rdyU_im = ldg(k.rdyU, cim);
+      rAz_j2 = ldg(k.rAz, cj2);
+      rAz_jm = ldg(k.rAz, cjm);
+      rdxV_j2 = ldg(k.rdxV, cj2);
+      rdxV_jm = ldg(k.rdxV, cjm);
+    }
+    const T e_i2 = eadj(t + 2 * RJ, rAz_i2), e_im = eadj(t - RJ, rAz_im);
+    const T e_j2 = eadj(t + 2, rAz_j2), e_jm = eadj(t - 1, rAz_jm);
+    lu_out = lu_out + (T)1.5 * (ec * rdyU * hu2f(t) + ejp * rdyU_jp * hu2f(t + 1)) -
+             (T)0.5 * (e_i2 * rdyU_i2 * hu2f(t + 2 * RJ) * sMU[t + RJ] +
+                       e_im * rdyU_im * hu2f(t - RJ) * sMU[t - RJ]);
+    lv_out = lv_out + (T)1.5 * (ec * rdxV * hv2f(t) + eip * rdxV_ip * hv2f(t + RJ)) -
+             (T)0.5 * (e_j2 * rdxV_j2 * hv2f(t + 2) * sMV[t + 1] +
+                       e_jm * rdxV_jm * hv2f(t - 1) * sMV[t - 1]);
+  }
   if (bas) {  // the basal coefficient's velocity argument at the two c-points
     lu_out = lu_out + (((T)0.5 * (u * mInUc)) * mInUc) * (sLTB[t] + sLTB[t - RJ]);
     lv_out = lv_out + (((T)0.5 * (v * mInVc)) * mInVc) * (sLTB[t] + sLTB[t - 1]);
@@ -775,45 +835,52 @@ struct AdjTileCfg {
 constexpr AdjTileCfg kAdjTiles[] = {{16, 32}, {24, 32}, {12, 64}, {32, 32}};
 constexpr int kAdjNumTiles = sizeof(kAdjTiles) / sizeof(kAdjTiles[0]);
 
-template <typename T, int RI, int RJ, bool XB>
+template <typename T, int RI, int RJ, int MODE>
 void adj_launch_shape(const AdjArgs<T>* a, const EvpArgBlock<T>* b, LamSet<T> L, const Geom& g,
                       int n_members, hipStream_t st) {
-  constexpr int TI = RI - 8, TJ = RJ - 8;
+  constexpr int R = MODE == 2 ? 5 : 4, TI = RI - 2 * R, TJ = RJ - 2 * R;
   const int tiles_j = (g.NY + TJ - 1) / TJ;
   const int n_tiles = ((g.NX + TI - 1) / TI) * tiles_j;
   const int per_xcd = (n_tiles + 7) / 8;
-  hipLaunchKernelGGL((adj_substep_kernel<T, double, RI, RJ, XB>), dim3(per_xcd * 8, n_members),
+  hipLaunchKernelGGL((adj_substep_kernel<T, double, RI, RJ, MODE>), dim3(per_xcd * 8, n_members),
                      dim3(RI * RJ), 0, st, a, b, L, n_tiles, tiles_j, per_xcd);
 }
 
 template <typename T>
 void adj_launch(int cfg, const AdjArgs<T>* a, const EvpArgBlock<T>* b, LamSet<T> L, const Geom& g,
-                int n_members, bool xb, hipStream_t st) {
-  if (xb && sizeof(T) == 8) {
-    // the extra-branch FP64 kernel: its LDS does not fit a 768-thread
-    // workgroup, so it exists as 16 x 32 only
-    adj_launch_shape<T, 16, 32, true>(a, b, L, g, n_members, st);
+                int n_members, int mode, hipStream_t st) {
+  if constexpr (sizeof(T) == 8) {
+    if (mode >= 1) {
+      // the extra-branch FP64 kernels: their LDS does not fit a 768-thread
+      // workgroup, so they exist as 16 x 32 only
+      if (mode == 2) adj_launch_shape<T, 16, 32, 2>(a, b, L, g, n_members, st);
+      else adj_launch_shape<T, 16, 32, 1>(a, b, L, g, n_members, st);
+      return;
+    }
+  }
+  if (mode == 2) {  // FP32 with the second-order term: 24 x 32 only
+    if constexpr (sizeof(T) == 4) adj_launch_shape<T, 24, 32, 2>(a, b, L, g, n_members, st);
     return;
   }
-  if (xb) {
+  if (mode == 1) {
     if constexpr (sizeof(T) == 4) {
       switch (cfg) {
-        case 1: adj_launch_shape<T, 24, 32, true>(a, b, L, g, n_members, st); break;
-        case 2: adj_launch_shape<T, 12, 64, true>(a, b, L, g, n_members, st); break;
-        case 3: adj_launch_shape<T, 32, 32, true>(a, b, L, g, n_members, st); break;
-        default: adj_launch_shape<T, 16, 32, true>(a, b, L, g, n_members, st); break;
+        case 1: adj_launch_shape<T, 24, 32, 1>(a, b, L, g, n_members, st); break;
+        case 2: adj_launch_shape<T, 12, 64, 1>(a, b, L, g, n_members, st); break;
+        case 3: adj_launch_shape<T, 32, 32, 1>(a, b, L, g, n_members, st); break;
+        default: adj_launch_shape<T, 16, 32, 1>(a, b, L, g, n_members, st); break;
       }
     }
     return;
   }
   switch (cfg) {
-    case 1: adj_launch_shape<T, 24, 32, false>(a, b, L, g, n_members, st); break;
-    case 2: adj_launch_shape<T, 12, 64, false>(a, b, L, g, n_members, st); break;
+    case 1: adj_launch_shape<T, 24, 32, 0>(a, b, L, g, n_members, st); break;
+    case 2: adj_launch_shape<T, 12, 64, 0>(a, b, L, g, n_members, st); break;
     case 3:  // 1024 threads: FP32 only (the FP64 kernel's registers do not fit)
-      if constexpr (sizeof(T) == 4) adj_launch_shape<T, 32, 32, false>(a, b, L, g, n_members, st);
-      else adj_launch_shape<T, 24, 32, false>(a, b, L, g, n_members, st);
+      if constexpr (sizeof(T) == 4) adj_launch_shape<T, 32, 32, 0>(a, b, L, g, n_members, st);
+      else adj_launch_shape<T, 24, 32, 0>(a, b, L, g, n_members, st);
       break;
-    default: adj_launch_shape<T, 16, 32, false>(a, b, L, g, n_members, st); break;
+    default: adj_launch_shape<T, 16, 32, 0>(a, b, L, g, n_members, st); break;
   }
 }
 
@@ -990,8 +1057,7 @@ int check_adj(const vh_domain* d, const vh_settings* s, const vh_fields* f, int 
   VH_REQUIRE(d->tile >= 0 && d->tile < kEvpNumTiles && (d->tile == 0 || kEvpTiles[d->tile].k == 1),
              VH_EINVAL, "adjoint: tile %d (needs a single-substep tile shape or 0)", d->tile);
   VH_REQUIRE(s->useEVP != 0.0, VH_EUNSUPPORTED, "adjoint: the EVP solver must be on");
-  VH_REQUIRE(!(s->noSlip != 0.0 && s->secondOrderBC != 0.0), VH_EUNSUPPORTED,
-             "adjoint: the second-order no-slip boundary term is not supported");
+
   VH_REQUIRE(s->basalDragK2 == 0.0 || (f->R_low && f->Area && f->hIceMean), VH_EINVAL,
              "adjoint: basal drag needs R_low, Area, hIceMean");
   const bool kt = f->k1AtC || f->k2AtC || f->k1AtZ || f->k2AtZ;
@@ -1103,17 +1169,18 @@ int forward_ckpt(const Geom& g, const vh_domain* d, const vh_settings* s, const 
   VH_TRY_HIP(hipGetLastError());
   T* final_slot = wsf<T>(ws, L.off_ckpt, L.fb, 5 * L.nck);
   const int n_all = (int)(field_elems(g) * (size_t)nm);
+  const bool so = s->noSlip != 0.0 && s->secondOrderBC != 0.0;
   auto record = [&](hipStream_t cs) {
     for (int kk = 0; kk < n; ++kk)
       evp_launch_single<T, double>(
           cfg, (const EvpArgBlock<T>*)((char*)fblk + (size_t)kk * L.blk), n_tiles, tiles_j, cs,
-          nm);
+          nm, so);
     hipLaunchKernelGGL(adj_keep_final_kernel<T>, dim3((unsigned)((n_all + BLK - 1) / BLK)),
                        dim3(BLK), 0, cs,
                        (const EvpArgBlock<T>*)((char*)fblk + (size_t)(n - 1) * L.blk), n_all,
                        final_slot, L.fb);
   };
-  return run_recorded<T>("adjf", g, L, cfg, nm, ws, record, st);
+  return run_recorded<T>("adjf", g, L, cfg * 2 + (so ? 1 : 0), nm, ws, record, st);
 }
 
 template <typename T>
@@ -1161,9 +1228,11 @@ int adjoint(const Geom& g, const vh_domain* d, const vh_settings* s, const vh_fi
   tiles_of(cfg, g, &n_tiles, &tiles_j);
   const int acfg = adj_tile_cfg<T>(field_elems(g) * (size_t)nm);
   const bool overlap = adj_overlap();
-  // the extra-branch instantiation: free slip with lateral drag, basal drag,
-  // adaptive EVP
-  const bool sd = s->noSlip == 0.0 || s->basalDragK2 != 0.0 || s->useAdaptiveEVP != 0.0;
+  // the instantiation: 1 for free slip with lateral drag, basal drag or
+  // adaptive EVP, 2 when the second-order no-slip term is on as well
+  const bool sobc = s->noSlip != 0.0 && s->secondOrderBC != 0.0;
+  const int sd = sobc ? 2
+                 : (s->noSlip == 0.0 || s->basalDragK2 != 0.0 || s->useAdaptiveEVP != 0.0) ? 1 : 0;
   T* lam[2][5];
   for (int q = 0; q < 5; ++q) {
     lam[0][q] = wsf<T>(ws, L.off_lam, L.fb, q);
@@ -1187,7 +1256,7 @@ int adjoint(const Geom& g, const vh_domain* d, const vh_settings* s, const vh_fi
     for (int jj = 0; jj + 1 < len; ++jj)
       evp_launch_single<T, double>(
           cfg, (const EvpArgBlock<T>*)((char*)rblk + (size_t)(k0 + jj) * L.blk), n_tiles, tiles_j,
-          cs, nm);
+          cs, nm, sobc);
   };
   // Segment sgm - 1 is recomputed on a second stream while segment sgm is
   // reversed (they use different slot sets); the fork follows the reverse of
@@ -1221,8 +1290,8 @@ int adjoint(const Geom& g, const vh_domain* d, const vh_settings* s, const vh_fi
   };
   // the key holds every choice baked into the recorded launches: shapes,
   // overlap and the extra-branch instantiation
-  return run_recorded<T>("adjr", g, L, ((cfg * 16 + acfg) * 2 + (overlap ? 1 : 0)) * 2 + (sd ? 1 : 0),
-                         nm, ws, record, st);
+  return run_recorded<T>("adjr", g, L, ((cfg * 16 + acfg) * 2 + (overlap ? 1 : 0)) * 4 + sd, nm,
+                         ws, record, st);
 }
 
 template <typename T>

@@ -1084,13 +1084,18 @@ inline void evp_launch_cfg(int cfg, const EvpArgBlock<T>* ab, int which, int lau
 // equal members packed back to back (ab->mstride apart; 1 = one domain).
 template <typename T, typename TD>
 inline void evp_launch_single(int cfg, const EvpArgBlock<T>* ab, int n_tiles, int tiles_j,
-                              hipStream_t st, int n_members = 1) {
+                              hipStream_t st, int n_members = 1, bool so = false) {
   const int per_xcd = (n_tiles + 7) / 8;
   evp_with_shape(cfg, [&](auto sh) {
     using S = decltype(sh);
     constexpr int RI = S::k == 1 ? S::ri : 16, RJ = S::k == 1 ? S::rj : 32;
-    hipLaunchKernelGGL((evp_substep_kernel<T, TD, RI, RJ, 1, false>), dim3(per_xcd * 8, n_members),
-                       dim3(RI * RJ), 0, st, ab, 0, 0, n_tiles, tiles_j, per_xcd);
+    if (so)  // the second-order boundary term's instantiation
+      hipLaunchKernelGGL((evp_substep_kernel<T, TD, RI, RJ, 1, false, T, true>),
+                         dim3(per_xcd * 8, n_members), dim3(RI * RJ), 0, st, ab, 0, 0, n_tiles,
+                         tiles_j, per_xcd);
+    else
+      hipLaunchKernelGGL((evp_substep_kernel<T, TD, RI, RJ, 1, false>), dim3(per_xcd * 8, n_members),
+                         dim3(RI * RJ), 0, st, ab, 0, 0, n_tiles, tiles_j, per_xcd);
   });
 }
 
