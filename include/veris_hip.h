@@ -487,8 +487,9 @@ int vh_evp_adjoint(const vh_domain* d, const vh_settings* s, const vh_fields* f,
  * pStarField per cell or NULL for the pStar setting), SeaIceMass
  * (veris/area_mass.py:17-28) and WindForcingXY (veris/dynsolver.py:7-84):
  * maps the strength, mass and wind-forcing adjoints of `evp_adj` to P*,
- * hIceMean, hSnowMean and the air drag.  useRealFreshWaterFlux is not
- * supported. */
+ * hIceMean, hSnowMean and the air drag.  With useRealFreshWaterFlux the
+ * surface tilt holds the SeaIceLoad field, a state input here as in the
+ * reference's kernel. */
 int vh_pre_evp_adjoint(const vh_domain* d, const vh_settings* s, const vh_fields* f,
                        const void* pStarField, const vh_evp_adjoint_out* evp_adj,
                        const vh_pre_evp_adjoint_out* out, void* stream);

@@ -47,6 +47,13 @@ def _state(kind):
         st = synth.global_latlon(31, 36, 20, -75.0, 80.0, veros_fill=False)
     elif kind == "regional":
         st = synth.regional(32, 0, 30, 22)
+    elif kind == "sea_ice_load":  # the load in the surface tilt, with a tilted surface
+        st = synth.global_latlon(31, 36, 20, -75.0, 80.0, useRealFreshWaterFlux=True)
+        vs0, s0 = st.variables, st.settings
+        r = np.random.default_rng(6)
+        vs0.SeaIceLoad = (s0.rhoIce * np.asarray(vs0.hIceMean) + s0.rhoSnow * np.asarray(vs0.hSnowMean)
+                          ) * r.uniform(0.8, 1.2, st.shape)
+        vs0.ssh_an = r.normal(0.0, 0.05, st.shape)
     elif kind == "second_order":  # the second-order no-slip boundary term at the coasts
         st = synth.global_latlon(31, 36, 20, -75.0, 80.0, secondOrderBC=True)
     elif kind == "adaptive":  # about a tenth of the ice cells above alpha's floor
@@ -164,7 +171,7 @@ def test_checkpointed_forward_is_the_drop_in(cuda, kind):
 
 @pytest.mark.parametrize("kind", ["global", "doubly_periodic", "regional", "turning_metric",
                                   "unpacked", "side_drag", "coastline", "basal", "adaptive",
-                                  "second_order"])
+                                  "second_order", "sea_ice_load"])
 def test_adjoint_matches_torch_autograd(cuda, kind):
     st = _state(kind)
     n = 25

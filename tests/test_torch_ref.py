@@ -37,6 +37,13 @@ def _state(kind):
     elif kind == "basal":  # grounded keels: water 8 m deep, given as a negative depth
         st = synth.global_latlon(31, 36, 20, -75.0, 80.0, basalDragK2=15.0)
         st.variables.R_low = -np.asarray(st.variables.R_low) * 0.002
+    elif kind == "sea_ice_load":  # the load in the surface tilt, with a tilted surface
+        st = synth.global_latlon(31, 36, 20, -75.0, 80.0, useRealFreshWaterFlux=True)
+        vs0, s0 = st.variables, st.settings
+        r = np.random.default_rng(6)
+        vs0.SeaIceLoad = (s0.rhoIce * np.asarray(vs0.hIceMean) + s0.rhoSnow * np.asarray(vs0.hSnowMean)
+                          ) * r.uniform(0.8, 1.2, st.shape)
+        vs0.ssh_an = r.normal(0.0, 0.05, st.shape)
     elif kind == "second_order":
         st = synth.global_latlon(31, 36, 20, -75.0, 80.0, secondOrderBC=True)
     elif kind == "adaptive":
@@ -62,7 +69,8 @@ def _torch_vars(st):
 
 
 @pytest.mark.parametrize("kind", ["global", "regional", "turning_metric", "side_drag",
-                                  "coastline", "basal", "adaptive", "second_order"])
+                                  "coastline", "basal", "adaptive", "second_order",
+                                  "sea_ice_load"])
 def test_forward_matches_oracle(kind):
     st = _state(kind)
     vs, s = _torch_vars(st), st.settings
@@ -86,7 +94,8 @@ def test_forward_matches_oracle(kind):
 
 
 @pytest.mark.parametrize("kind", ["global", "regional", "turning_metric", "side_drag",
-                                  "coastline", "basal", "adaptive", "second_order"])
+                                  "coastline", "basal", "adaptive", "second_order",
+                                  "sea_ice_load"])
 def test_autograd_matches_finite_differences(kind):
     st = _state(kind)
     vs, s = _torch_vars(st), st.settings

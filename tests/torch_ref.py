@@ -94,7 +94,11 @@ def pre_evp(vs, sett, pstar, h, ca):
     ty = cda * vrel
     tx = 0.5 * (tx + im1(tx)) * vs["iceMaskU"]
     ty = 0.5 * (ty + jm1(ty)) * vs["iceMaskV"]
-    phi = sett.gravity * vs["ssh_an"] + vs["surfPress"] * sett.recip_rhoSea
+    if sett.useRealFreshWaterFlux:  # sea-ice load in the tilt (veris/dynsolver.py:65-72)
+        phi = sett.gravity * vs["ssh_an"] + (vs["surfPress"] + vs["SeaIceLoad"] * sett.gravity
+                                             * sett.seaIceLoadFac) * sett.recip_rhoSea
+    else:
+        phi = sett.gravity * vs["ssh_an"] + vs["surfPress"] * sett.recip_rhoSea
     fx = tx * AW - mU * vs["recip_dxC"] * (phi - im1(phi))
     fy = ty * AS - mV * vs["recip_dyC"] * (phi - jm1(phi))
     P = pstar * h * torch.exp(-sett.cStar * (1 - A)) * vs["iceMask"]

@@ -154,7 +154,8 @@ def evp_subcycle(state, *, pStar=None, hIceMean=None, waterIceDrag=None,
     neighbours or coastline form factors); with or without basal drag (its
     thickness dependence included in the hIceMean gradient), the water-stress
     turning angle (sinWat, cosWat) and the metric terms (k1AtC, k2AtC, k1AtZ,
-    k2AtZ).  The pre-EVP part does not take sea-ice load in the surface tilt.
+    k2AtZ); the pre-EVP part takes the sea-ice load in the surface tilt
+    (useRealFreshWaterFlux) as a state input, as the reference's kernel reads it.
     """
     vs, s = state.variables, state.settings
     n = int(s.nEVPsteps if n_substeps is None else n_substeps)

@@ -895,6 +895,8 @@ struct PreAdjArgs {
   // outputs
   T *apstar, *ah, *ahs, *aCaN, *aCaS;
   T pstar, cStar, rhoI, rhoS, rhoAir, wmin, wmin2, grav, rrho, sinWin, cosWin;
+  const T* load;  // SeaIceLoad in the surface tilt (useRealFreshWaterFlux), else NULL
+  T loadFac;      // gravity * seaIceLoadFac
   int relWind;
 };
 
@@ -927,8 +929,11 @@ __global__ __launch_bounds__(BLK) void pre_evp_adjoint_kernel(Geom g, PreAdjArgs
   a.apstar[mo + c] = a.aP[mo + c] * a.h[mo + c] * E * a.M[mo + c];
   T hb = a.aP[mo + c] * ps * E * a.M[mo + c];
   // wind forcing: the tilt term -massU rdxC (phi - phi[i-1]) adds to the mass adjoints
+  // (the load of useRealFreshWaterFlux is a state input here: Growth wrote it)
   auto phi_at = [&](int ii, int jj) {
-    return a.grav * a.ssh[mo + AT(ii, jj)] + a.sp[mo + AT(ii, jj)] * a.rrho;
+    const T p = a.load ? a.sp[mo + AT(ii, jj)] + a.load[mo + AT(ii, jj)] * a.loadFac
+                       : a.sp[mo + AT(ii, jj)];
+    return a.grav * a.ssh[mo + AT(ii, jj)] + p * a.rrho;
   };
   const T phi = phi_at(i, j);
   const T amU_c = a.amU[mo + c] - a.aWFX[mo + c] * a.rdxC[mo + c] * (phi - phi_at(im, j));
@@ -1332,8 +1337,12 @@ int pre_evp_adjoint(const Geom& g, const vh_settings* s, const vh_fields* f, con
                        a.ssh, a.sp, a.AW, a.AS, a.aP, a.amC, a.amU, a.amV, a.aWFX, a.aWFY,
                        a.apstar, a.ah, a.ahs, a.aCaN, a.aCaS};
   for (const void* p : req) VH_REQUIRE(p != nullptr, VH_EINVAL, "pre_evp_adjoint: null pointer");
-  VH_REQUIRE(s->useRealFreshWaterFlux == 0.0, VH_EUNSUPPORTED,
-             "pre_evp_adjoint: useRealFreshWaterFlux not supported");
+  a.load = nullptr;
+  a.loadFac = (T)(s->gravity * s->seaIceLoadFac);
+  if (s->useRealFreshWaterFlux != 0.0) {
+    a.load = evp_ptr<T>(f->SeaIceLoad);
+    VH_REQUIRE(a.load != nullptr, VH_EINVAL, "pre_evp_adjoint: null SeaIceLoad");
+  }
   a.pstar = (T)s->pStar;
   a.cStar = (T)s->cStar;
   a.rhoI = (T)s->rhoIce;
