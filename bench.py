@@ -173,6 +173,46 @@ def _rounded_to_fp32(state):
     return state
 
 
+EVP_FP_INPUTS = ("SeaIceStrength", "SeaIceMassC", "SeaIceMassU", "SeaIceMassV", "AreaW", "AreaS",
+                 "WindForcingX", "WindForcingY", "uOcean", "vOcean", "fCori")
+EVP_METRICS = ("recip_dxU", "recip_dyV", "recip_dyU", "recip_dxV", "dxU", "dyU", "dxV", "dyV",
+               "recip_rAu", "recip_rAv", "rAz", "recip_rA")
+
+
+def _fp32_field_sensitivity(e64, tile, nsub, ref, masks) -> dict:
+    """Velocity error (relative L2 of uIce and vIce together, interior ocean
+    cells) of the FP64 batched subcycle when exactly one of its floating-point
+    inputs is rounded to FP32 (the masks, the zero initial state and every
+    other field exact), of the twelve grid metrics rounded together, and of
+    every input rounded except the three fields that move the result most."""
+    import torch
+
+    def run_with(rounded):
+        saved = {}
+        for name in rounded:
+            buf = e64.fields[name]
+            saved[name] = buf.clone()
+            buf.copy_(buf.float().double())
+        out = e64.evp(nsub, tile=tile)
+        for name, keep in saved.items():
+            e64.fields[name].copy_(keep)
+        pairs = list(zip(e64.unpack(out[0]), e64.unpack(ref[0]), masks))
+        pairs += list(zip(e64.unpack(out[1]), e64.unpack(ref[1]), masks))
+        del out
+        torch.cuda.synchronize()
+        return _rel_l2(pairs)
+
+    one = {name: run_with((name,)) for name in EVP_FP_INPUTS + EVP_METRICS}
+    worst = sorted(one, key=one.get, reverse=True)[:3]
+    return {
+        "one_field": one,
+        "metrics_together": run_with(EVP_METRICS),
+        "all_but_top3": {"kept_fp64": worst,
+                         "rel_l2": run_with(tuple(n for n in one if n not in worst))},
+        "all": run_with(tuple(one)),
+    }
+
+
 def bench_batched(dev, nsub: int, fp64_evp_rate: float, fp64_step_rate: float) -> dict:
     import torch
 
@@ -208,6 +248,15 @@ def bench_batched(dev, nsub: int, fp64_evp_rate: float, fp64_step_rate: float) -
     err_a = {name: _rel_l2(list(zip(e32a.unpack(out32a[q]), e64.unpack(out64[q]), masks)))
              for q, name in enumerate(EVP_OUT)}
     del e32a
+    # FP32-stored step constants around an FP64 state, FP64 arithmetic
+    e32s = Ensemble(states, dev, torch.float32, arith="fp64_state")
+    e32s.pre_evp()
+    tile32s, _ = tune_batched(e32s, exact=False)
+    out32s = e32s.evp(nsub, tile=tile32s)
+    t32s = _sync_time(lambda: e32s.evp(nsub, tile=tile32s, out=out32s))
+    err_s = {name: _rel_l2(list(zip(e32s.unpack(out32s[q]), e64.unpack(out64[q]), masks)))
+             for q, name in enumerate(EVP_OUT)}
+    del e32s, out32s
     # the same FP64 subcycle on inputs rounded to FP32: what the rounding of
     # the inputs alone moves the result by, and the FP32 run against FP64 on
     # identical (FP32-representable) inputs
@@ -249,6 +298,7 @@ def bench_batched(dev, nsub: int, fp64_evp_rate: float, fp64_step_rate: float) -
     top = torch.sort(d2, descending=True).values
     top_share = float(top[: max(1, top.numel() // 100)].sum() / top.sum().clamp_min(1e-300))
     del outp
+    sens = _fp32_field_sensitivity(e64, tile64, nsub, out64, masks)
     b32 = e32.budgets()
     b64 = e64.budgets()
     budget_err = {k: float(((b32[k] - b64[k]).abs() / b64[k].abs().clamp_min(1e-300)).max())
@@ -280,11 +330,17 @@ def bench_batched(dev, nsub: int, fp64_evp_rate: float, fp64_step_rate: float) -
         "fp64_strength_perturbed_1e-15_rel_l2_vs_fp64": err_pert,
         "fp32_squared_velocity_error_share_of_worst_1pct_cells": top_share,
         "fp32_rel_l2_vs_fp64_on_cells_stable_under_1e-15": err_stable,
+        "fp64_with_one_input_rounded_to_fp32": sens,
         "fp32_storage_fp64_arith": {
             "value": cells * nsub / t32a,
             "tile": {"index": tile32a, "shape": shapes[tile32a]},
             "over_plain_fp64_evp_1deg": cells * nsub / t32a / fp64_evp_rate,
             "rel_l2_vs_fp64": err_a},
+        "fp32_constants_fp64_state": {
+            "value": cells * nsub / t32s,
+            "tile": {"index": tile32s, "shape": shapes[tile32s]},
+            "over_plain_fp64_evp_1deg": cells * nsub / t32s / fp64_evp_rate,
+            "rel_l2_vs_fp64": err_s},
         "fp64_budget_rel_diff_of_fp32_fields": budget_err,
         "setup_s": t_gen,
     }

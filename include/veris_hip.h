@@ -404,7 +404,10 @@ typedef struct vh_batch {
   int32_t dtype;            /* VH_F64 | VH_F32 */
   int32_t tile;             /* EVP tile index, 0 = library default */
   int32_t arith;            /* 0: arithmetic in dtype (FP32: Delta, zeta, press in FP64);
-                               1 (VH_F32 only): FP64 arithmetic on FP32-stored fields */
+                               1 (VH_F32 only): FP64 arithmetic on FP32-stored fields;
+                               2 (VH_F32 only): FP32-stored step constants, FP64 state
+                                 (uIce, vIce, sigma1, sigma2, sigma12 fields and the five
+                                 outputs are FP64) and FP64 arithmetic */
   const vh_member* members; /* host array of n_members */
   int64_t total_elems;      /* elements of every packed field */
 } vh_batch;
@@ -418,7 +421,8 @@ int vh_evp_batched_workspace_bytes(const vh_batch* b, size_t* bytes);
  * state.  Members run the same substep kernel as vh_evp_solver; with
  * dtype VH_F32 the bulk is FP32 and Delta, zeta and the replacement pressure
  * are FP64 (arith 0), or every substep computes in FP64 on the FP32 fields
- * (arith 1).  No extension of the reference (which has no batching). */
+ * (arith 1), or the state is FP64 and only the step constants are FP32
+ * (arith 2).  No extension of the reference (which has no batching). */
 int vh_evp_solver_batched(const vh_batch* b, const vh_settings* s, const vh_fields* f,
                           int32_t n_substeps, void* uIce, void* vIce, void* sigma1,
                           void* sigma2, void* sigma12, void* workspace, size_t workspace_bytes,

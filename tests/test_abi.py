@@ -147,6 +147,18 @@ def test_batch_arith_validation():
                   total_elems=68 * 68)
     with pytest.raises(N.VerisHipError, match="VH_EINVAL.*arith"):
         N.call("vh_evp_batched_workspace_bytes", ctypes.byref(bad), ctypes.byref(n))
+    # arith 2: FP32 step constants around an FP64 state, whose five ping-pong
+    # slots are FP64 (4 more bytes per element each)
+    n2 = ctypes.c_size_t(0)
+    st64 = N.Batch(n_members=1, dtype=N.VH_F32, tile=0, arith=2, members=members,
+                   total_elems=68 * 68)
+    N.call("vh_evp_batched_workspace_bytes", ctypes.byref(st64), ctypes.byref(n2))
+    assert n2.value >= n.value + 5 * (68 * 68 * 4 - 256)
+    for dtype, arith in ((N.VH_F64, 2), (N.VH_F32, 3)):
+        bad = N.Batch(n_members=1, dtype=dtype, tile=0, arith=arith, members=members,
+                      total_elems=68 * 68)
+        with pytest.raises(N.VerisHipError, match="VH_EINVAL.*arith"):
+            N.call("vh_evp_batched_workspace_bytes", ctypes.byref(bad), ctypes.byref(n))
 
 
 def test_forcing_residual_and_free_drift_validation():

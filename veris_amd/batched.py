@@ -34,15 +34,22 @@ class Ensemble:
 
     ``states``: member states (NumPy or torch variables); their settings must
     agree except for ``nx``, ``ny`` and ``enable_cyclic_x``.  ``arith``
-    ("native" or "fp64") chooses, for float32 fields, FP32 arithmetic (with
-    FP64 Delta, zeta, press) or FP64 arithmetic on the FP32-stored fields.
+    chooses, for float32 fields, FP32 arithmetic with FP64 Delta, zeta and
+    press ("native"), FP64 arithmetic on the FP32-stored fields ("fp64"), or
+    FP32-stored step constants around an FP64 state with FP64 arithmetic
+    ("fp64_state": every field is also held in FP64, ``fields64``; the
+    pre-EVP forcing runs in FP64 on those and is rounded once into the FP32
+    step constants; the EVP state uIce, vIce, sigma1, sigma2, sigma12 stays
+    FP64, ``state64``, and the subcycle returns FP64).
     """
 
+    _ARITH = {"native": 0, "fp64": 1, "fp64_state": 2}
+
     def __init__(self, states, device="cuda", dtype=torch.float32, arith="native"):
-        if arith not in ("native", "fp64"):
-            raise ValueError("arith must be 'native' or 'fp64'")
-        if arith == "fp64" and dtype != torch.float32:
-            raise ValueError("arith='fp64' applies to float32 ensembles")
+        if arith not in self._ARITH:
+            raise ValueError("arith must be 'native', 'fp64' or 'fp64_state'")
+        if arith != "native" and dtype != torch.float32:
+            raise ValueError(f"arith={arith!r} applies to float32 ensembles")
         self.arith = arith
         if not states:
             raise ValueError("empty ensemble")
@@ -61,20 +68,26 @@ class Ensemble:
         self.n = len(states)
         self.cyclic = [bool(st.settings.enable_cyclic_x) for st in states]
         self.fill_mode = 0 if self.settings.veros_fill else 1
-        self.fields = {}
-        for name in VARIABLE_NAMES:
-            buf = torch.empty(self.total, dtype=dtype, device=self.device)
-            for m, st in enumerate(states):
-                a = st.variables[name]
-                t = a if isinstance(a, torch.Tensor) else torch.from_numpy(np.ascontiguousarray(a))
-                self.view(buf, m).copy_(t.to(device=self.device, dtype=dtype))
-            self.fields[name] = buf
+        self.fields = self._pack(states, dtype)
+        self.fields64 = self._pack(states, torch.float64) if arith == "fp64_state" else {}
+        self.state64 = {name: self.fields64[name] for name in EVP_OUT} if self.fields64 else {}
         self._members = (N.Member * self.n)()
         for m, (NX, NY) in enumerate(self.shapes):
             self._members[m] = N.Member(nx=NX - 4, ny=NY - 4, cyclic_x=int(self.cyclic[m]),
                                         fill_mode=self.fill_mode, offset=int(self.offsets[m]))
         self._ws = None
         self.member_settings = [st.settings for st in states]
+
+    def _pack(self, states, dtype):
+        fields = {}
+        for name in VARIABLE_NAMES:
+            buf = torch.empty(self.total, dtype=dtype, device=self.device)
+            for m, st in enumerate(states):
+                a = st.variables[name]
+                t = a if isinstance(a, torch.Tensor) else torch.from_numpy(np.ascontiguousarray(a))
+                self.view(buf, m).copy_(t.to(device=self.device, dtype=dtype))
+            fields[name] = buf
+        return fields
 
     # -- views --------------------------------------------------------------
 
@@ -99,32 +112,49 @@ class Ensemble:
         """SeaIceMass, AreaWS, WindForcingXY and SeaIceStrength of every member
         (veris/area_mass.py:5-28, veris/dynsolver.py:7-84,
         veris/dynamics_routines.py:5-18) in one batched launch, written into
-        the packed arrays; ``pStar`` optionally a packed per-cell P* field."""
-        if pStar is not None and (pStar.shape != (self.total,) or pStar.dtype != self.dtype):
-            raise ValueError("pStar must be a packed field of the ensemble's dtype")
-        b = self._batch(0)
-        out = N.PreEvpOut(*[self.fields[name].data_ptr() for name in _PRE_EVP])
+        the packed arrays; ``pStar`` optionally a packed per-cell P* field.
+        With arith "fp64_state" the forcing is computed in FP64 from the FP64
+        fields and rounded once into the FP32 step constants."""
+        fdt = torch.float64 if self.fields64 else self.dtype
+        if pStar is not None and (pStar.shape != (self.total,) or pStar.dtype != fdt):
+            raise ValueError("pStar must be a packed field of the ensemble's (forcing) dtype")
+        fields = self.fields64 or self.fields
+        if self.fields64:
+            b = N.Batch(n_members=self.n, dtype=N.VH_F64, tile=0, arith=0, members=self._members,
+                        total_elems=self.total)
+        else:
+            b = self._batch(0)
+        out = N.PreEvpOut(*[fields[name].data_ptr() for name in _PRE_EVP])
         N.call("vh_pre_evp_batched", ctypes.byref(b), ctypes.byref(N.settings_struct(self.settings)),
-               ctypes.byref(self._fields_struct()), None if pStar is None else pStar.data_ptr(),
+               ctypes.byref(self._fields_struct(fields)), None if pStar is None else pStar.data_ptr(),
                ctypes.byref(out), torch.cuda.current_stream(self.device).cuda_stream)
+        for name in _PRE_EVP if self.fields64 else ():
+            self.fields[name].copy_(self.fields64[name])
 
     def _batch(self, tile):
         return N.Batch(n_members=self.n, dtype=N.VH_F64 if self.dtype == torch.float64 else N.VH_F32,
-                       tile=int(tile), arith=1 if self.arith == "fp64" else 0,
+                       tile=int(tile), arith=self._ARITH[self.arith],
                        members=self._members, total_elems=self.total)
 
-    def _fields_struct(self):
+    def _fields_struct(self, fields=None, state64=False):
+        """Packed field pointers (of ``fields``, default the ensemble's own);
+        with ``state64`` the EVP state points at the FP64 state arrays (arith
+        "fp64_state")."""
+        fields = self.fields if fields is None else fields
         f = N.Fields()
         for name in VARIABLE_NAMES:
-            setattr(f, name, self.fields[name].data_ptr())
-        if all(K._all_zero(self.fields[k]) for k in _KTERMS):
+            setattr(f, name, fields[name].data_ptr())
+        if state64:
+            for name, buf in self.state64.items():
+                setattr(f, name, buf.data_ptr())
+        if all(K._all_zero(fields[k]) for k in _KTERMS):
             for k in _KTERMS:
                 setattr(f, k, None)
         return f
 
     def evp(self, n_substeps=None, tile=0, out=None):
         """Batched EVP subcycle; returns the five packed outputs
-        (uIce, vIce, sigma1, sigma2, sigma12)."""
+        (uIce, vIce, sigma1, sigma2, sigma12), FP64 with arith "fp64_state"."""
         n = int(self.settings.nEVPsteps if n_substeps is None else n_substeps)
         b = self._batch(tile)
         nbytes = ctypes.c_size_t(0)
@@ -132,9 +162,10 @@ class Ensemble:
         if self._ws is None or self._ws.numel() < nbytes.value:
             self._ws = torch.empty(nbytes.value, dtype=torch.uint8, device=self.device)
         if out is None:
-            out = [torch.empty(self.total, dtype=self.dtype, device=self.device) for _ in range(5)]
+            sdt = torch.float64 if self.state64 else self.dtype
+            out = [torch.empty(self.total, dtype=sdt, device=self.device) for _ in range(5)]
         sett = N.settings_struct(self.settings)
-        fields = self._fields_struct()
+        fields = self._fields_struct(state64=True)
         stream = torch.cuda.current_stream(self.device).cuda_stream
         N.call("vh_evp_solver_batched", ctypes.byref(b), ctypes.byref(sett), ctypes.byref(fields),
                n, *[t.data_ptr() for t in out], self._ws.data_ptr(), self._ws.numel(), stream)

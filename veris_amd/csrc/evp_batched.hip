@@ -22,6 +22,8 @@ void evp_batched_launch_f32(int cfg, const EvpArgBlock<float>* ab, int which, in
                             int n_tiles, bool so, hipStream_t st);
 void evp_batched_launch_f32_f64(int cfg, const EvpArgBlock<float>* ab, int which, int launch,
                                 int n_tiles, bool so, hipStream_t st);
+void evp_batched_launch_f32_s64(int cfg, const EvpArgBlock<float, double>* ab, int which,
+                                int launch, int n_tiles, bool so, hipStream_t st);
 
 namespace {
 
@@ -48,11 +50,12 @@ struct Layout {
 
 int batch_cfg(const vh_batch* b) { return (b->tile > 0) ? b->tile : 3; }
 
-template <typename T>
+// T: storage of the step constants, S: of the state (ping-pong slots)
+template <typename T, typename S = T>
 Layout layout_of(const vh_batch* b) {
   Layout L;
-  L.fb = align256((size_t)b->total_elems * sizeof(T));
-  L.ab = align256(sizeof(EvpArgBlock<T>));
+  L.fb = align256((size_t)b->total_elems * sizeof(S));
+  L.ab = align256(sizeof(EvpArgBlock<T, S>));
   L.mem = align256((size_t)b->n_members * sizeof(MemberDesc));
   L.base = 5 * L.fb + L.ab + L.mem;
   L.rows = 0;
@@ -74,8 +77,9 @@ int check_batch(const vh_batch* b) {
   VH_REQUIRE(b != nullptr && b->members != nullptr && b->n_members > 0, VH_EINVAL, "empty batch");
   VH_REQUIRE(b->dtype == VH_F64 || b->dtype == VH_F32, VH_EDTYPE, "dtype %d", b->dtype);
   VH_REQUIRE(b->tile >= 0 && b->tile < kEvpNumTiles, VH_EINVAL, "tile %d", b->tile);
-  VH_REQUIRE(b->arith == 0 || (b->arith == 1 && b->dtype == VH_F32), VH_EINVAL,
-             "arith %d (1 = FP64 arithmetic, FP32 storage only)", b->arith);
+  VH_REQUIRE(b->arith == 0 || ((b->arith == 1 || b->arith == 2) && b->dtype == VH_F32), VH_EINVAL,
+             "arith %d (1 = FP64 arithmetic, 2 = FP64 state and arithmetic: FP32 storage only)",
+             b->arith);
   std::vector<std::pair<long long, long long>> spans;
   long long tiles = 0;
   for (int m = 0; m < b->n_members; ++m) {
@@ -92,7 +96,7 @@ int check_batch(const vh_batch* b) {
   }
   VH_REQUIRE(tiles < (1LL << 30), VH_ESHAPE, "batch too large (%lld tiles)", tiles);
   // the substep kernel addresses every packed field with 32-bit byte offsets
-  const long long esz = b->dtype == VH_F64 ? 8 : 4;
+  const long long esz = (b->dtype == VH_F64 || b->arith == 2) ? 8 : 4;
   VH_REQUIRE(b->total_elems * esz < (1LL << 32), VH_ESHAPE,
              "batch too large: %lld elements of %lld bytes exceed 4 GiB per packed field",
              (long long)b->total_elems, esz);
@@ -102,27 +106,26 @@ int check_batch(const vh_batch* b) {
   return VH_OK;
 }
 
-template <typename T>
-void launch_batched(int cfg, int arith, bool so, const EvpArgBlock<T>* ab, int which, int launch,
-                    int n_tiles, hipStream_t st);
-
-template <>
-void launch_batched<double>(int cfg, int, bool so, const EvpArgBlock<double>* ab, int which,
-                            int launch, int n_tiles, hipStream_t st) {
+inline void launch_batched(int cfg, int, bool so, const EvpArgBlock<double>* ab, int which,
+                           int launch, int n_tiles, hipStream_t st) {
   evp_launch_cfg<double, double, true>(cfg, ab, which, launch, n_tiles, 0, st, so);
 }
 
-template <>
-void launch_batched<float>(int cfg, int arith, bool so, const EvpArgBlock<float>* ab, int which,
+inline void launch_batched(int cfg, int arith, bool so, const EvpArgBlock<float>* ab, int which,
                            int launch, int n_tiles, hipStream_t st) {
   if (arith == 1) evp_batched_launch_f32_f64(cfg, ab, which, launch, n_tiles, so, st);
   else evp_batched_launch_f32(cfg, ab, which, launch, n_tiles, so, st);
 }
 
-template <typename T>
+inline void launch_batched(int cfg, int, bool so, const EvpArgBlock<float, double>* ab, int which,
+                           int launch, int n_tiles, hipStream_t st) {
+  evp_batched_launch_f32_s64(cfg, ab, which, launch, n_tiles, so, st);
+}
+
+template <typename T, typename S = T>
 int run_batched(const vh_batch* b, const vh_settings* s, const vh_fields* fl, int nsub, void* uo,
                 void* vo, void* s1o, void* s2o, void* s12o, void* ws, size_t wsb, hipStream_t st) {
-  const Layout L = layout_of<T>(b);
+  const Layout L = layout_of<T, S>(b);
   VH_REQUIRE(ws != nullptr && wsb >= L.base, VH_EWORKSPACE,
              "batched evp workspace: need %zu bytes, got %zu", L.total, wsb);
   const bool pack = wsb >= L.packed;
@@ -149,12 +152,12 @@ int run_batched(const vh_batch* b, const vh_settings* s, const vh_fields* fl, in
     VH_REQUIRE(fl->R_low && fl->Area && fl->hIceMean, VH_EINVAL,
                "basal drag needs R_low, Area, hIceMean");
 
-  const T* inp[5] = {evp_ptr<T>(fl->uIce), evp_ptr<T>(fl->vIce), evp_ptr<T>(fl->sigma1),
-                     evp_ptr<T>(fl->sigma2), evp_ptr<T>(fl->sigma12)};
-  T* outp[5] = {(T*)uo, (T*)vo, (T*)s1o, (T*)s2o, (T*)s12o};
+  const S* inp[5] = {evp_ptr<S>(fl->uIce), evp_ptr<S>(fl->vIce), evp_ptr<S>(fl->sigma1),
+                     evp_ptr<S>(fl->sigma2), evp_ptr<S>(fl->sigma12)};
+  S* outp[5] = {(S*)uo, (S*)vo, (S*)s1o, (S*)s2o, (S*)s12o};
   if (nsub == 0) {
     for (int q = 0; q < 5; ++q)
-      VH_TRY_HIP(hipMemcpyAsync(outp[q], inp[q], (size_t)b->total_elems * sizeof(T),
+      VH_TRY_HIP(hipMemcpyAsync(outp[q], inp[q], (size_t)b->total_elems * sizeof(S),
                                 hipMemcpyDeviceToDevice, st));
     return VH_OK;
   }
@@ -164,7 +167,7 @@ int run_batched(const vh_batch* b, const vh_settings* s, const vh_fields* fl, in
              VH_EUNSUPPORTED, "secondOrderBC needs a single-substep tile shape (tile %d has k=%d)",
              cfg, kEvpTiles[cfg].k);
   char* base = (char*)ws;
-  EvpArgBlock<T>* ab_dev = (EvpArgBlock<T>*)(base + 5 * L.fb);
+  EvpArgBlock<T, S>* ab_dev = (EvpArgBlock<T, S>*)(base + 5 * L.fb);
   MemberDesc* mem_dev = (MemberDesc*)(base + 5 * L.fb + L.ab);
 
   // member table, written by value in chunks
@@ -191,13 +194,13 @@ int run_batched(const vh_batch* b, const vh_settings* s, const vh_fields* fl, in
   }
   VH_TRY_HIP(hipGetLastError());
 
-  EvpArgBlock<T> ab{};
+  EvpArgBlock<T, S> ab{};
   fill_evp_scalars(s, kt, basal, &ab.sc);
   fill_evp_const(fl, &ab.c);
   for (int q = 0; q < 5; ++q) {
     ab.in[q] = inp[q];
     ab.out[q] = outp[q];
-    ab.alt[q] = (T*)(base + q * L.fb);
+    ab.alt[q] = (S*)(base + q * L.fb);
   }
   ab.g = Geom{};
   ab.nlaunch = evp_n_launches(cfg, nsub);
@@ -209,13 +212,13 @@ int run_batched(const vh_batch* b, const vh_settings* s, const vh_fields* fl, in
     ab.tiles[0] = (const TileDesc*)(base + L.packed);
     ab.tiles[1] = ab.tiles[0] + n_main;
   }
-  hipLaunchKernelGGL(evp_write_args_kernel<T>, dim3(1), dim3(64), 0, st, ab_dev, ab);
+  hipLaunchKernelGGL((evp_write_args_kernel<T, S>), dim3(1), dim3(64), 0, st, ab_dev, ab);
   if (pack) evp_pack_launch(ab_dev, max_cells, b->n_members, st);
   if (tables) {
     const int tabs[2][2] = {{cfg, n_main}, {kEvpRemainderCfg, n_rem}};
     for (int w = 0; w < 2; ++w)
       if (tabs[w][1] > 0)
-        hipLaunchKernelGGL(evp_tile_table_kernel<T>, dim3((tabs[w][1] + 255) / 256), dim3(256), 0,
+        hipLaunchKernelGGL((evp_tile_table_kernel<T, S>), dim3((tabs[w][1] + 255) / 256), dim3(256), 0,
                            st, ab_dev, w, evp_tile_extent_i(tabs[w][0]),
                            evp_tile_extent_j(tabs[w][0]), tabs[w][1]);
   }
@@ -225,9 +228,9 @@ int run_batched(const vh_batch* b, const vh_settings* s, const vh_fields* fl, in
   auto record = [&](hipStream_t cs) {
     int l = 0;
     for (int q = 0; q < nsub / K; ++q)
-      launch_batched<T>(cfg, b->arith, so, ab_dev, 0, l++, n_main, cs);
+      launch_batched(cfg, b->arith, so, ab_dev, 0, l++, n_main, cs);
     for (int q = 0; q < nsub % K; ++q)
-      launch_batched<T>(kEvpRemainderCfg, b->arith, so, ab_dev, 1, l++, n_rem, cs);
+      launch_batched(kEvpRemainderCfg, b->arith, so, ab_dev, 1, l++, n_rem, cs);
   };
   if (nsub >= 8 && graphs_enabled()) {
     char key[192];
@@ -249,7 +252,9 @@ extern "C" int vh_evp_batched_workspace_bytes(const vh_batch* b, size_t* bytes) 
   int rc = vh::check_batch(b);
   if (rc) return rc;
   VH_REQUIRE(bytes != nullptr, VH_EINVAL, "null bytes");
-  *bytes = b->dtype == VH_F64 ? vh::layout_of<double>(b).total : vh::layout_of<float>(b).total;
+  *bytes = b->dtype == VH_F64 ? vh::layout_of<double>(b).total
+           : b->arith == 2     ? vh::layout_of<float, double>(b).total
+                               : vh::layout_of<float>(b).total;
   return VH_OK;
 }
 
@@ -265,6 +270,9 @@ extern "C" int vh_evp_solver_batched(const vh_batch* b, const vh_settings* s, co
   if (b->dtype == VH_F64)
     return vh::run_batched<double>(b, s, f, n_substeps, uIce, vIce, sigma1, sigma2, sigma12,
                                    workspace, workspace_bytes, st);
+  if (b->arith == 2)
+    return vh::run_batched<float, double>(b, s, f, n_substeps, uIce, vIce, sigma1, sigma2,
+                                          sigma12, workspace, workspace_bytes, st);
   return vh::run_batched<float>(b, s, f, n_substeps, uIce, vIce, sigma1, sigma2, sigma12,
                                 workspace, workspace_bytes, st);
 }

@@ -47,9 +47,13 @@
 
 namespace vh {
 
-template <typename T>
+// S is the storage type of the state (u, v, sigma1, sigma2, sigma12 and the
+// previous-step velocities); T that of every other field (S = T except in the
+// batched mode with FP32 step constants around an FP64 state).
+template <typename T, typename S = T>
 struct EvpConst {
-  const T *uNm1, *vNm1, *P, *massC, *massU, *massV, *AreaW, *AreaS, *WFX, *WFY, *uOc, *vOc, *fC;
+  const S *uNm1, *vNm1;
+  const T *P, *massC, *massU, *massV, *AreaW, *AreaS, *WFX, *WFY, *uOc, *vOc, *fC;
   const T *M, *MU, *MV, *mInC, *mInU, *mInV;
   const T *rdxU, *rdyV, *rdyU, *rdxV, *dxU, *dyU, *dxV, *dyV, *rrAu, *rrAv, *rAz, *rrA;
   const T *k1C, *k2C, *k1Z, *k2Z;  // metric terms (read when kterms)
@@ -115,12 +119,12 @@ static __global__ void evp_write_members_kernel(MemberDesc* dst, MemberChunk chu
 
 // Everything a launch reads, in device memory (written once per solver call,
 // so a captured graph of launches stays valid when the caller's buffers move).
-template <typename T>
+template <typename T, typename S = T>
 struct EvpArgBlock {
-  EvpConst<T> c;
-  const T* in[5];  // u, v, sigma1, sigma2, sigma12 at the start of the subcycle
-  T* out[5];       // final state (written by the last launch)
-  T* alt[5];       // ping-pong partner
+  EvpConst<T, S> c;
+  const S* in[5];  // u, v, sigma1, sigma2, sigma12 at the start of the subcycle
+  S* out[5];       // final state (written by the last launch)
+  S* alt[5];       // ping-pong partner
   EvpScalars sc;
   Geom g;          // single-domain geometry (unused when batched)
   int nlaunch;     // launches in the subcycle
@@ -192,11 +196,11 @@ inline void fill_evp_scalars(const vh_settings* s, int kterms, int basal, EvpSca
 }
 
 // Step-constant field pointers of one subcycle (host side).
-template <typename T>
-inline void fill_evp_const(const vh_fields* fl, EvpConst<T>* out) {
-  EvpConst<T>& f = *out;
-  f.uNm1 = evp_ptr<T>(fl->uIce);
-  f.vNm1 = evp_ptr<T>(fl->vIce);
+template <typename T, typename S>
+inline void fill_evp_const(const vh_fields* fl, EvpConst<T, S>* out) {
+  EvpConst<T, S>& f = *out;
+  f.uNm1 = evp_ptr<S>(fl->uIce);
+  f.vNm1 = evp_ptr<S>(fl->vIce);
   f.P = evp_ptr<T>(fl->SeaIceStrength);
   f.massC = evp_ptr<T>(fl->SeaIceMassC);
   f.massU = evp_ptr<T>(fl->SeaIceMassU);
@@ -253,8 +257,8 @@ inline int evp_required_fields(const vh_fields* fl, const void** req) {
 
 constexpr int kEvpMaxRequired = 34;
 
-template <typename T>
-__global__ void evp_write_args_kernel(EvpArgBlock<T>* dst, EvpArgBlock<T> src) {
+template <typename T, typename S = T>
+__global__ void evp_write_args_kernel(EvpArgBlock<T, S>* dst, EvpArgBlock<T, S> src) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *dst = src;
 }
 
@@ -294,8 +298,8 @@ inline size_t evp_pack_bytes(size_t elems, size_t rows) {
   return align256(elems) + align256((size_t)kEvpRowFields * rows * sizeof(T));
 }
 
-template <typename T>
-__global__ __launch_bounds__(256) void evp_pack_kernel(EvpArgBlock<T>* ab) {
+template <typename T, typename S>
+__global__ __launch_bounds__(256) void evp_pack_kernel(EvpArgBlock<T, S>* ab) {
   Geom g;
   long long off = 0;
   int roff = 0;
@@ -312,7 +316,7 @@ __global__ __launch_bounds__(256) void evp_pack_kernel(EvpArgBlock<T>* ab) {
   const int e = blockIdx.x * 256 + threadIdx.x;
   if (e >= g.NX * g.NY) return;
   const int i = e / g.NY, j = e - i * g.NY;
-  const EvpConst<T>& f = ab->c;
+  const EvpConst<T, S>& f = ab->c;
   const T* const masks[kEvpMaskFields] = {f.M, f.MU, f.MV, f.mInC, f.mInU, f.mInV};
   const T* const mets[kEvpRowFields] = {f.rdyU, f.rdxV, f.rAz, f.rdxU, f.rdyV, f.rrA,
                                         f.dyV,  f.dxU,  f.dxV, f.dyU,  f.rrAu, f.rrAv};
@@ -339,8 +343,8 @@ __global__ __launch_bounds__(256) void evp_pack_kernel(EvpArgBlock<T>* ab) {
 // Tile table of a batched launch shape (`which`: 0 main, 1 remainder) with
 // tile extents TI x TJ: one thread per tile finds its member in the member
 // table (the last member whose first tile is at or before it).
-template <typename T>
-__global__ __launch_bounds__(256) void evp_tile_table_kernel(EvpArgBlock<T>* ab, int which, int TI,
+template <typename T, typename S = T>
+__global__ __launch_bounds__(256) void evp_tile_table_kernel(EvpArgBlock<T, S>* ab, int which, int TI,
                                                              int TJ, int n_tiles) {
   const int tile = blockIdx.x * 256 + threadIdx.x;
   if (tile >= n_tiles) return;
@@ -372,18 +376,18 @@ __global__ __launch_bounds__(256) void evp_tile_table_kernel(EvpArgBlock<T>* ab,
 
 // Points the host-side block at packed storage `buf` (evp_pack_bytes<T>(elems,
 // rows) bytes) and marks it packed; evp_pack_launch then fills it on device.
-template <typename T>
-inline void evp_pack_attach(EvpArgBlock<T>* ab, void* buf, size_t elems, int rows) {
+template <typename T, typename S>
+inline void evp_pack_attach(EvpArgBlock<T, S>* ab, void* buf, size_t elems, int rows) {
   ab->pk = (const unsigned char*)buf;
   ab->rows = (T*)((char*)buf + align256(elems));
   ab->rstride = rows;
   ab->packed = 1;
 }
 
-template <typename T>
-inline void evp_pack_launch(EvpArgBlock<T>* ab_dev, long long max_cells, int n_members,
+template <typename T, typename S>
+inline void evp_pack_launch(EvpArgBlock<T, S>* ab_dev, long long max_cells, int n_members,
                             hipStream_t st) {
-  hipLaunchKernelGGL(evp_pack_kernel<T>, dim3((unsigned)((max_cells + 255) / 256), n_members),
+  hipLaunchKernelGGL((evp_pack_kernel<T, S>), dim3((unsigned)((max_cells + 255) / 256), n_members),
                      dim3(256), 0, st, ab_dev);
 }
 
@@ -443,8 +447,8 @@ __device__ inline int next_axis(int q, int N, int n, int filled) {
 }
 
 template <typename T, typename TD, int RI, int RJ, int K, bool BATCHED, typename A = T,
-          bool SO = false, bool RES = false>
-__global__ __launch_bounds__(RI* RJ) void evp_substep_kernel(const EvpArgBlock<T>* __restrict__ ab,
+          bool SO = false, bool RES = false, typename S = T>
+__global__ __launch_bounds__(RI* RJ) void evp_substep_kernel(const EvpArgBlock<T, S>* __restrict__ ab,
                                                              int which, int launch, int n_tiles,
                                                              int tiles_j, int per_xcd) {
   constexpr int R = 2 * K;
@@ -515,11 +519,11 @@ __global__ __launch_bounds__(RI* RJ) void evp_substep_kernel(const EvpArgBlock<T
       j0 = (tile - ti * tiles_j) * TJ;
     }
     const int nl = ab->nlaunch;
-    const T* const* src = (launch == 0) ? ab->in
-                                        : (((nl - launch) % 2 == 0) ? (const T* const*)ab->out
-                                                                    : (const T* const*)ab->alt);
-    T* const* dst = ((nl - 1 - launch) % 2 == 0) ? ab->out : ab->alt;
-    const EvpConst<T>& f = ab->c;
+    const S* const* src = (launch == 0) ? ab->in
+                                        : (((nl - launch) % 2 == 0) ? (const S* const*)ab->out
+                                                                    : (const S* const*)ab->alt);
+    S* const* dst = ((nl - 1 - launch) % 2 == 0) ? ab->out : ab->alt;
+    const EvpConst<T, S>& f = ab->c;
     const EvpScalars& sc = ab->sc;
 
     const int t = threadIdx.x;
@@ -925,12 +929,12 @@ __global__ __launch_bounds__(RI* RJ) void evp_substep_kernel(const EvpArgBlock<T
           store_filled(g, GW(dst[1] + off), i, j, v);
         }
       } else {
-        stg(dst[0], oo + (unsigned)cp, (T)u);
-        stg(dst[1], oo + (unsigned)cp, (T)v);
+        stg(dst[0], oo + (unsigned)cp, (S)u);
+        stg(dst[1], oo + (unsigned)cp, (S)v);
       }
-      stg(dst[2], oo + (unsigned)cp, (T)s1);
-      stg(dst[3], oo + (unsigned)cp, (T)s2);
-      stg(dst[4], oo + (unsigned)cp, (T)s12);
+      stg(dst[2], oo + (unsigned)cp, (S)s1);
+      stg(dst[3], oo + (unsigned)cp, (S)s2);
+      stg(dst[4], oo + (unsigned)cp, (S)s12);
     }
   };
   if constexpr (BATCHED) {
@@ -1026,12 +1030,12 @@ inline int evp_grid(F kernel, int threads, int n_tiles, std::atomic<int>* cache)
 }
 
 template <typename T, typename TD, int RI, int RJ, int K, bool BATCHED, typename A = T,
-          bool SO = false>
-inline void evp_launch_shape(const EvpArgBlock<T>* ab, int which, int launch, int n_tiles,
+          bool SO = false, typename S = T>
+inline void evp_launch_shape(const EvpArgBlock<T, S>* ab, int which, int launch, int n_tiles,
                              int tiles_j, hipStream_t st) {
   static std::atomic<int> cache[kMaxDevices];
   const int per_xcd = (n_tiles + 7) / 8;
-  auto kernel = evp_substep_kernel<T, TD, RI, RJ, K, BATCHED, A, SO>;
+  auto kernel = evp_substep_kernel<T, TD, RI, RJ, K, BATCHED, A, SO, false, S>;
   const int grid = BATCHED ? evp_grid(kernel, RI * RJ, n_tiles, cache) : 8 * per_xcd;
   hipLaunchKernelGGL(kernel, dim3(grid), dim3(RI * RJ), 0, st, ab, which, launch, n_tiles,
                      tiles_j, per_xcd);
@@ -1039,13 +1043,14 @@ inline void evp_launch_shape(const EvpArgBlock<T>* ab, int which, int launch, in
 
 // Single-substep launch with the second-order no-slip boundary term (cfg must
 // be a k = 1 shape; the host checks).
-template <typename T, typename TD, bool BATCHED, typename A = T>
-inline void evp_launch_so(int cfg, const EvpArgBlock<T>* ab, int which, int launch, int n_tiles,
+template <typename T, typename TD, bool BATCHED, typename A = T, typename S = T>
+inline void evp_launch_so(int cfg, const EvpArgBlock<T, S>* ab, int which, int launch, int n_tiles,
                           int tiles_j, hipStream_t st) {
   evp_with_shape(cfg, [&](auto sh) {
-    using S = decltype(sh);
-    constexpr int RI = S::k == 1 ? S::ri : 16, RJ = S::k == 1 ? S::rj : 32;
-    evp_launch_shape<T, TD, RI, RJ, 1, BATCHED, A, true>(ab, which, launch, n_tiles, tiles_j, st);
+    using Sh = decltype(sh);
+    constexpr int RI = Sh::k == 1 ? Sh::ri : 16, RJ = Sh::k == 1 ? Sh::rj : 32;
+    evp_launch_shape<T, TD, RI, RJ, 1, BATCHED, A, true, S>(ab, which, launch, n_tiles, tiles_j,
+                                                            st);
   });
 }
 
@@ -1065,17 +1070,17 @@ constexpr int kEvpResidualCfg = 5;
 // (0 main shape, 1 remainder shape) by batched ones.  T is the storage type
 // of the fields, A the arithmetic type (default T), TD that of Delta/zeta;
 // `so` selects the second-order boundary variant (k = 1 shapes only).
-template <typename T, typename TD, bool BATCHED, typename A = T>
-inline void evp_launch_cfg(int cfg, const EvpArgBlock<T>* ab, int which, int launch, int n_tiles,
+template <typename T, typename TD, bool BATCHED, typename A = T, typename S = T>
+inline void evp_launch_cfg(int cfg, const EvpArgBlock<T, S>* ab, int which, int launch, int n_tiles,
                            int tiles_j, hipStream_t st, bool so = false) {
   if (so) {
-    evp_launch_so<T, TD, BATCHED, A>(cfg, ab, which, launch, n_tiles, tiles_j, st);
+    evp_launch_so<T, TD, BATCHED, A, S>(cfg, ab, which, launch, n_tiles, tiles_j, st);
     return;
   }
   evp_with_shape(cfg, [&](auto sh) {
-    using S = decltype(sh);
-    evp_launch_shape<T, TD, S::ri, S::rj, S::k, BATCHED, A>(ab, which, launch, n_tiles, tiles_j,
-                                                            st);
+    using Sh = decltype(sh);
+    evp_launch_shape<T, TD, Sh::ri, Sh::rj, Sh::k, BATCHED, A, false, S>(ab, which, launch,
+                                                                         n_tiles, tiles_j, st);
   });
 }
 
