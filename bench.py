@@ -11,8 +11,9 @@ steps (cell-substeps/s).  ``roofline`` covers the dominant kernel, the fused
 EVP substep: algorithmic bytes per launch (38 words per interior cell, 57,600
 cells) over its average launch time measured with HIP events on the stream the
 substeps are launched on.  ``cpu_baseline`` times the scalar-loop C
-restatement of the step and the NumPy FP64 one (one thread each) on about 10 s
-of full 1-degree steps each, and reports the faster.
+restatement of the step (on one thread and on every host CPU) and the NumPy
+FP64 one (one thread) on about 8 s of full 1-degree steps each, and reports
+the fastest with its thread count.
 
 Beside the headline line the same JSON object carries the other measured
 workloads (skip them with ``--no-extras``):
@@ -81,11 +82,12 @@ def measured_traffic(kernel: str):
     return _profile_value(TRAFFIC_FILE, "traffic_bytes_per_launch", kernel)
 
 
-def cpu_baseline(n_substeps: int, min_seconds: float = 10.0, max_steps: int = 12) -> dict:
-    """The CPU restatements on full 1-degree steps, single thread: the
-    scalar-loop C one (gcc -O2) and the NumPy FP64 one, each repeated until at
-    least ``min_seconds`` of CPU work; the value is the faster."""
-    os.environ.setdefault("OMP_NUM_THREADS", "1")
+def cpu_baseline(n_substeps: int, min_seconds: float = 8.0, max_steps: int = 12) -> dict:
+    """The CPU restatements on full 1-degree steps: the scalar-loop C one
+    (gcc -O2) on one thread and on every host CPU this process may use
+    (OpenMP over the cell loops), and the NumPy FP64 one (one thread), each
+    repeated until at least ``min_seconds`` of CPU work; the value is the
+    fastest, with the threads it used as ``cores``."""
     from oracle import c_ref
     from oracle import veris_np as O
     from veris_amd import synth
@@ -106,25 +108,29 @@ def cpu_baseline(n_substeps: int, min_seconds: float = 10.0, max_steps: int = 12
         return (f"{n} full step{'s' if n > 1 else ''} of the 1-degree workload "
                 f"({n_substeps} EVP substeps each) through {what}, {dt:.1f} s")
 
+    ncpu = len(os.sched_getaffinity(0))
     legs = {}
-    try:  # the C leg needs gcc where the library is not built yet
+    try:  # the C legs need gcc where the library is not built yet
         c_ref.build()
-        n, dt = timed(lambda: c_ref.seaice_step(st, q0, sw, n_substeps=n_substeps))
-        legs["c_port"] = {"value": nx * ny * n_substeps * n / dt,
-                          "sample": sample(n, dt, "the scalar-loop C restatement (gcc -O2, FP64)")}
+        for key, threads in (("c_port", 1), ("c_port_openmp", ncpu)):
+            used = c_ref.set_threads(threads)
+            n, dt = timed(lambda: c_ref.seaice_step(st, q0, sw, n_substeps=n_substeps))
+            legs[key] = {"value": nx * ny * n_substeps * n / dt, "cores": used,
+                         "sample": sample(n, dt, f"the scalar-loop C restatement (gcc -O2, FP64, "
+                                                 f"{used} OpenMP thread{'s' if used > 1 else ''})")}
     except Exception as e:  # noqa: BLE001
         legs["c_port"] = {"error": repr(e)}
     n, dt = timed(lambda: O.seaice_step(st, q0, sw, n_substeps=n_substeps))
-    legs["numpy_port"] = {"value": nx * ny * n_substeps * n / dt,
-                          "sample": sample(n, dt, "the NumPy FP64 restatement")}
-    # the value is the faster of the single-thread CPU restatements
+    legs["numpy_port"] = {"value": nx * ny * n_substeps * n / dt, "cores": 1,
+                          "sample": sample(n, dt, "the NumPy FP64 restatement (one thread)")}
+    # the value is the fastest CPU restatement
     best = max((v for v in legs.values() if "value" in v), key=lambda v: v["value"])
     return {
         "value": best["value"],
         "unit": "cell-substeps/s",
-        "cores": 1,
+        "cores": best["cores"],
         "kind": "port",
-        "sample": best["sample"] + f"; host affinity {len(os.sched_getaffinity(0))} cpus",
+        "sample": best["sample"] + f"; host affinity {ncpu} cpus",
         **legs,
     }
 

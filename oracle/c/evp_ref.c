@@ -19,6 +19,20 @@
 
 #include "evp_ref.h"
 
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+int evp_ref_set_threads(int n) {
+#ifdef _OPENMP
+  if (n > 0) omp_set_num_threads(n);
+  return omp_get_max_threads();
+#else
+  (void)n;
+  return 1;
+#endif
+}
+
 typedef struct {
   int NX, NY;
   const double* s;
@@ -29,7 +43,11 @@ static int wrap(int i, int n) { return ((i % n) + n) % n; }
 #define AT(x, y) ((size_t)wrap((x), C->NX) * (size_t)C->NY + (size_t)wrap((y), C->NY))
 #define F(k, x, y) (C->f[k][AT(x, y)])
 #define S(k) (C->s[k])
-#define FOR_CELLS(C) for (int i = 0; i < (C)->NX; ++i) for (int j = 0; j < (C)->NY; ++j)
+/* every FOR_CELLS body writes only its own cell: rows are split over OpenMP
+ * threads on grids large enough to pay for it (bits do not depend on it) */
+#define FOR_CELLS(C)                                                                 \
+  _Pragma("omp parallel for schedule(static) if ((C)->NX * (C)->NY >= 16384)")       \
+  for (int i = 0; i < (C)->NX; ++i) for (int j = 0; j < (C)->NY; ++j)
 
 static double sgn(double x) { return (double)((x > 0) - (x < 0)); }
 

@@ -58,4 +58,8 @@ int bulkf_ref(int n, const double* uw, const double* vw, const double* ta, const
               double gamma_blk, double rhoAir, double cpdair, double emis, double stefBoltz,
               double aln, double rdn, double* const out[9]);
 
+/* OpenMP threads of the cell loops (n > 0 sets them); returns the count in
+ * effect (1 in a build without OpenMP) */
+int evp_ref_set_threads(int n);
+
 #endif

@@ -71,7 +71,10 @@ static double Popt(Step* S, const char* name, double dflt) {
 static int wr(int i, int n) { return ((i % n) + n) % n; }
 #define NXY (S->NX * S->NY)
 #define AT(x, y) ((size_t)wr((x), S->NX) * (size_t)S->NY + (size_t)wr((y), S->NY))
-#define EACH for (int i = 0; i < S->NX; ++i) for (int j = 0; j < S->NY; ++j)
+/* every EACH body writes only its own cell (see FOR_CELLS in evp_ref.c) */
+#define EACH                                                                         \
+  _Pragma("omp parallel for schedule(static) if (S->NX * S->NY >= 16384)")          \
+  for (int i = 0; i < S->NX; ++i) for (int j = 0; j < S->NY; ++j)
 
 static double npmax(double a, double b) { return a != a ? a : (b != b ? b : (a > b ? a : b)); }
 static double npmin(double a, double b) { return a != a ? a : (b != b ? b : (a < b ? a : b)); }

@@ -51,6 +51,12 @@ def build() -> str:
     return LIB
 
 
+def set_threads(n: int = 0) -> int:
+    """OpenMP threads of the C restatement's cell loops (``n`` > 0 sets
+    them); returns the count in effect."""
+    return int(_load().evp_ref_set_threads(int(n)))
+
+
 def _load():
     global _lib
     if _lib is None:
@@ -73,6 +79,8 @@ def _load():
         lib.bulkf_ref.argtypes = ([ctypes.c_int] + [dp] * 5 + [ctypes.c_double] * 11 +
                                   [ctypes.POINTER(dp)])
         lib.bulkf_ref.restype = ctypes.c_int
+        lib.evp_ref_set_threads.argtypes = [ctypes.c_int]
+        lib.evp_ref_set_threads.restype = ctypes.c_int
         _lib = lib
     return _lib
 

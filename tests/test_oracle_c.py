@@ -461,3 +461,22 @@ RES_MUTATIONS = sorted(n for n, (a, _) in MUTATIONS.items()
 def test_c_comparison_catches_residual_mistake(mutation, tmp_path):
     mod = _mutant(mutation, tmp_path)
     assert any(_residual_mismatches(mod, v) for v in RES_VARIANTS), f"missed: {mutation}"
+
+
+def test_openmp_threads_change_no_bit():
+    """On a grid large enough for the OpenMP cell loops (1 degree, 59,696
+    padded cells), a whole step of the C restatement on one thread and on
+    four threads gives the same bits in every field."""
+    st = synth.config1()
+    q0, sw = st.variables.Qnet.copy(), st.variables.SWdown.copy()
+    outs = []
+    try:
+        for threads in (1, 4):
+            assert c_ref.set_threads(threads) == threads
+            outs.append(c_ref.seaice_step(synth.config1(), q0, sw, n_substeps=6))
+    finally:
+        c_ref.set_threads(1)
+    one, four = outs
+    assert one.keys() == four.keys()
+    for k in one:
+        assert np.array_equal(one[k].view(np.int64), four[k].view(np.int64)), k
