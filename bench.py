@@ -62,6 +62,15 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "evp_substep_1deg_fp64_traffic.md"
 KERNEL_PROFILE_FILE = os.path.join(ROOT, "profiles", "evp_substep_1deg_fp64.md")
 
 
+def evp_kernel_name(ri: int, rj: int, k: int, plain: bool = True, batched: bool = False,
+                    t: str = "double", a: str = "double", s: str = "double") -> str:
+    """The EVP substep instantiation a launch of shape ri x rj x k runs (plain:
+    the default-settings one): its template arguments as rocprofv3 lists them,
+    without the spaces (the key of the profile summaries)."""
+    return (f"evp_substep_kernel<{t},double,{ri},{rj},{k},{str(batched).lower()},{a},"
+            f"false,false,{s},{str(plain).lower()}>")
+
+
 def _profile_value(path: str, key: str, kernel: str):
     """The number on the line ``<key> <kernel> <value>`` of a profile summary,
     or None if the summary has no such line."""
@@ -665,7 +674,13 @@ def main(argv=None) -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--substeps", type=int, default=400)
     ap.add_argument("--tile", type=int, default=0,
-                    help="EVP workgroup shape index (default 0: autotune among the exact shapes)")
+                    help="EVP workgroup shape index (default 0: autotune)")
+    ap.add_argument("--exact", action="store_true",
+                    help="autotune among the single-substep shapes only (every padded cell "
+                         "bit for bit, the x-overlap stresses included)")
+    ap.add_argument("--general", action="store_true",
+                    help="launch the general substep instantiation (settings branches "
+                         "evaluated at run time) instead of the default-settings one")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the batched, adjoint and 0.25-degree measurements")
@@ -681,6 +696,8 @@ def main(argv=None) -> int:
               "run with --gpus 1", file=sys.stderr)
         return 2
 
+    if args.general:
+        os.environ["VERIS_EVP_PLAIN"] = "0"
     import torch
 
     from veris_amd import kernels as K
@@ -713,8 +730,12 @@ def main(argv=None) -> int:
     g = st.to(dev)
     nsub = args.substeps
 
-    # one step first (pre-EVP fields in place), then pick the fastest exact EVP
-    # workgroup shape on this state (every exact shape gives identical bits)
+    # one step first (pre-EVP fields in place), then pick the fastest EVP
+    # workgroup shape on this state: among every shape (multi-substep shapes
+    # give every velocity and every interior stress bit for bit, and periodic
+    # values in the stresses of the four x-overlap columns, where the
+    # reference holds numpy.roll wrap-around values nothing else reads), or
+    # with --exact among the single-substep shapes only (every padded cell)
     from veris_amd import _native
     from veris_amd.autotune import tune_evp
 
@@ -723,8 +744,9 @@ def main(argv=None) -> int:
         tile = args.tile
         K.EVP_TILE[(tuple(g.variables.iceMask.shape), g.variables.iceMask.dtype)] = tile
     else:
-        tile = tune_evp(g, exact=True, n_substeps=100)
+        tile = tune_evp(g, exact=args.exact, n_substeps=100)
     ri, rj, kk = _native.evp_tile_shapes()[tile]
+    n_launch = nsub // kk + nsub % kk
     for _ in range(args.warmup):
         seaice_step(g, q0, sw, n_substeps=nsub)
     torch.cuda.synchronize()
@@ -757,13 +779,15 @@ def main(argv=None) -> int:
     step_mod.K.IceVelocities = orig
 
     evp_ms = sum(a.elapsed_time(b) for a, b in ev)
-    per_launch_s = evp_ms * 1e-3 / (args.steps * nsub)
+    per_substep_s = evp_ms * 1e-3 / (args.steps * nsub)
+    per_launch_s = evp_ms * 1e-3 / (args.steps * n_launch)
     cells = nx * ny
     value = cells * nsub * args.steps / wall
-    bytes_per_launch = B_ALG_WORDS * 8 * cells
-    achieved = bytes_per_launch / per_launch_s / 1e9
+    # algorithmic bytes: 38 words per interior cell and substep, kk substeps per launch
+    bytes_per_launch = B_ALG_WORDS * 8 * cells * kk
+    achieved = B_ALG_WORDS * 8 * cells / per_substep_s / 1e9
     finite = all(bool(torch.isfinite(g.variables[k]).all()) for k in ("uIce", "hIceMean", "Qnet"))
-    kernel = f"evp_substep_kernel<double,double,{ri},{rj},{kk},false>"
+    kernel = evp_kernel_name(ri, rj, kk, plain=not args.general)
 
     line = {
         # the benchmark's metric; `value` is its cell-substeps/s leg on the
@@ -783,7 +807,13 @@ def main(argv=None) -> int:
         "data": "synthetic (seeded fields, see veris_amd/synth.py)",
         "config": {"workload": "1-degree global grid 360x160 + 2-cell halo, FP64, full sea-ice "
                                "step with 400 EVP substeps + advection + growth",
-                   "nx": nx, "ny": ny, "evp_substeps": nsub, "parallelism": "none (1 GPU)"},
+                   "nx": nx, "ny": ny, "evp_substeps": nsub, "parallelism": "none (1 GPU)",
+                   "evp_launch_shape": {"tile": tile, "threads": [ri, rj],
+                                        "substeps_per_launch": kk},
+                   "parity": ("bit-exact on every padded cell" if kk == 1 else
+                              "bit-exact velocities on every cell and stresses on every cell "
+                              "but the 4 x-overlap columns (periodic values there instead of "
+                              "the reference's numpy.roll wrap-around values)")},
         "roofline": {
             "bound": "hbm",
             "kernel": kernel,
@@ -795,8 +825,9 @@ def main(argv=None) -> int:
             "traffic": measured_traffic(kernel),
             "traffic_source": os.path.relpath(TRAFFIC_FILE, ROOT),
             "avg_launch_us": per_launch_s * 1e6,
-            "frac_basis": "HIP-event time per substep launch over the timed region on the "
-                          "launch stream: kernel plus the gap to the next dependent launch",
+            "substeps_per_launch": kk,
+            "frac_basis": "HIP-event time per launch over the timed region on the launch "
+                          "stream: kernel plus the gap to the next dependent launch",
             "algorithmic_bytes_per_launch": bytes_per_launch,
         },
         "finite": finite,
@@ -817,7 +848,7 @@ def main(argv=None) -> int:
             return {"error": repr(e)}
 
     if not args.no_extras:
-        evp_rate = cells / per_launch_s
+        evp_rate = cells / per_substep_s
         # the small adjoint workload first: right after the bandwidth-heavy
         # ensemble runs its FP64 timing read about 10% slower
         adjoint = guarded(bench_adjoint, dev)

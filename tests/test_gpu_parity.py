@@ -610,6 +610,31 @@ def test_evp_every_tile_shape(cuda, tile):
                 assert_exact(f"{name}[tile={tile}, k={k}, n={n}]", a, b)
 
 
+@pytest.mark.parametrize("tile", ALL_TILES)
+def test_plain_instantiation_equals_general(cuda, tile, monkeypatch):
+    """Under the default settings the subcycle launches the kernel
+    instantiation with every optional branch compiled out; the general
+    instantiation (VERIS_EVP_PLAIN=0) gives the same bits on every cell, for
+    every launch shape, and both equal the oracle (velocities everywhere,
+    stresses outside the periodic artefact columns for k > 1)."""
+    from veris_amd._native import evp_tile_shapes
+
+    k = evp_tile_shapes()[tile][2]
+    st = _spun_up(synth.config0(), cuda, n_spin=5)
+    g = st.to(cuda)
+    monkeypatch.setenv("VERIS_EVP_PLAIN", "1")
+    plain = [g2n(t) for t in K.evp_solver(g, n_substeps=17, tile=tile)]
+    monkeypatch.setenv("VERIS_EVP_PLAIN", "0")
+    general = [g2n(t) for t in K.evp_solver(g, n_substeps=17, tile=tile)]
+    want = O.evp_solver(st, n_substeps=17)
+    keep = np.setdiff1d(np.arange(st.shape[0]), _periodic_artefact_cols(st) if k > 1 else [])
+    for q, (name, a, b, c) in enumerate(zip(EVP_NAMES, plain, general, want)):
+        assert_exact(f"plain vs general {name}[tile={tile}]", a, b)
+        if q >= 2:
+            a, c = a[keep], c[keep]
+        assert_exact(f"plain {name}[tile={tile}]", a, c)
+
+
 def test_argument_structs_follow_the_state(cuda):
     """The drop-ins reuse their ABI structs between calls; every change a
     caller can make must still reach the kernels: a reassigned variable (a

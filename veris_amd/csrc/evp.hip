@@ -26,7 +26,7 @@ int pick_tile(const Geom& g, int req) {
 
 template <typename T, typename TD>
 void launch_substeps(int cfg, const EvpArgBlock<T>* ab, int nsub, const Geom& g, bool so,
-                     bool res, hipStream_t st) {
+                     bool res, bool plain, hipStream_t st) {
   if (res) {  // one residual-writing launch per substep
     const int TI = evp_tile_extent_i(kEvpResidualCfg), TJ = evp_tile_extent_j(kEvpResidualCfg);
     const int tiles_j = (g.NY + TJ - 1) / TJ;
@@ -44,7 +44,7 @@ void launch_substeps(int cfg, const EvpArgBlock<T>* ab, int nsub, const Geom& g,
     const int TI = evp_tile_extent_i(c), TJ = evp_tile_extent_j(c);
     const int tiles_j = (g.NY + TJ - 1) / TJ;
     const int n_tiles = ((g.NX + TI - 1) / TI) * tiles_j;
-    evp_launch_cfg<T, TD, false>(c, ab, 0, l++, n_tiles, tiles_j, st, so);
+    evp_launch_cfg<T, TD, false>(c, ab, 0, l++, n_tiles, tiles_j, st, so, plain);
   }
 }
 
@@ -162,6 +162,7 @@ int evp_run_core(const Geom& g, const vh_settings* s, const vh_fields* fl, const
              "secondOrderBC needs a single-substep tile shape (tile %d has k=%d)", cfg,
              kEvpTiles[cfg].k);
   const bool so = ab.sc.sobc != 0;
+  const bool plain = !res && evp_plain_kernels_enabled() && evp_plain_settings(ab.sc);
   ab.g = g;
   ab.nlaunch = evp_n_launches(cfg, nsub);
   ab.members = nullptr;
@@ -172,14 +173,16 @@ int evp_run_core(const Geom& g, const vh_settings* s, const vh_fields* fl, const
   VH_TRY_HIP(hipGetLastError());
   if (nsub >= 8 && graphs_enabled() && !direct) {
     char key[160];
-    snprintf(key, sizeof(key), "evp1/%d/%d/%d/%d/%d/%d/%d/%d/%d/%d/%p", dtype, cfg, (int)so,
-             res ? 1 : 0, nsub, g.NX, g.NY, g.nx, g.fill_x, g.fill_y, (const void*)ab_dev);
+    snprintf(key, sizeof(key), "evp1/%d/%d/%d/%d/%d/%d/%d/%d/%d/%d/%d/%p", dtype, cfg, (int)so,
+             res ? 1 : 0, (int)plain, nsub, g.NX, g.NY, g.nx, g.fill_x, g.fill_y,
+             (const void*)ab_dev);
     int rc = graph_run(
-        key, [&](hipStream_t cs) { launch_substeps<T, double>(cfg, ab_dev, nsub, g, so, res, cs); },
+        key,
+        [&](hipStream_t cs) { launch_substeps<T, double>(cfg, ab_dev, nsub, g, so, res, plain, cs); },
         st);
     if (rc) return rc;
   } else {
-    launch_substeps<T, double>(cfg, ab_dev, nsub, g, so, res, st);
+    launch_substeps<T, double>(cfg, ab_dev, nsub, g, so, res, plain, st);
   }
   VH_TRY_HIP(hipGetLastError());
   return VH_OK;

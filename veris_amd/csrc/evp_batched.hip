@@ -19,11 +19,11 @@
 namespace vh {
 
 void evp_batched_launch_f32(int cfg, const EvpArgBlock<float>* ab, int which, int launch,
-                            int n_tiles, bool so, hipStream_t st);
+                            int n_tiles, bool so, bool plain, hipStream_t st);
 void evp_batched_launch_f32_f64(int cfg, const EvpArgBlock<float>* ab, int which, int launch,
-                                int n_tiles, bool so, hipStream_t st);
+                                int n_tiles, bool so, bool plain, hipStream_t st);
 void evp_batched_launch_f32_s64(int cfg, const EvpArgBlock<float, double>* ab, int which,
-                                int launch, int n_tiles, bool so, hipStream_t st);
+                                int launch, int n_tiles, bool so, bool plain, hipStream_t st);
 
 namespace {
 
@@ -106,20 +106,21 @@ int check_batch(const vh_batch* b) {
   return VH_OK;
 }
 
-inline void launch_batched(int cfg, int, bool so, const EvpArgBlock<double>* ab, int which,
-                           int launch, int n_tiles, hipStream_t st) {
-  evp_launch_cfg<double, double, true>(cfg, ab, which, launch, n_tiles, 0, st, so);
+inline void launch_batched(int cfg, int, bool so, bool plain, const EvpArgBlock<double>* ab,
+                           int which, int launch, int n_tiles, hipStream_t st) {
+  evp_launch_cfg<double, double, true>(cfg, ab, which, launch, n_tiles, 0, st, so, plain);
 }
 
-inline void launch_batched(int cfg, int arith, bool so, const EvpArgBlock<float>* ab, int which,
-                           int launch, int n_tiles, hipStream_t st) {
-  if (arith == 1) evp_batched_launch_f32_f64(cfg, ab, which, launch, n_tiles, so, st);
-  else evp_batched_launch_f32(cfg, ab, which, launch, n_tiles, so, st);
+inline void launch_batched(int cfg, int arith, bool so, bool plain, const EvpArgBlock<float>* ab,
+                           int which, int launch, int n_tiles, hipStream_t st) {
+  if (arith == 1) evp_batched_launch_f32_f64(cfg, ab, which, launch, n_tiles, so, plain, st);
+  else evp_batched_launch_f32(cfg, ab, which, launch, n_tiles, so, plain, st);
 }
 
-inline void launch_batched(int cfg, int, bool so, const EvpArgBlock<float, double>* ab, int which,
-                           int launch, int n_tiles, hipStream_t st) {
-  evp_batched_launch_f32_s64(cfg, ab, which, launch, n_tiles, so, st);
+inline void launch_batched(int cfg, int, bool so, bool plain,
+                           const EvpArgBlock<float, double>* ab, int which, int launch,
+                           int n_tiles, hipStream_t st) {
+  evp_batched_launch_f32_s64(cfg, ab, which, launch, n_tiles, so, plain, st);
 }
 
 template <typename T, typename S = T>
@@ -225,17 +226,18 @@ int run_batched(const vh_batch* b, const vh_settings* s, const vh_fields* fl, in
   VH_TRY_HIP(hipGetLastError());
   const int K = kEvpTiles[cfg].k;
   const bool so = ab.sc.sobc != 0;
+  const bool plain = evp_plain_kernels_enabled() && evp_plain_settings(ab.sc);
   auto record = [&](hipStream_t cs) {
     int l = 0;
     for (int q = 0; q < nsub / K; ++q)
-      launch_batched(cfg, b->arith, so, ab_dev, 0, l++, n_main, cs);
+      launch_batched(cfg, b->arith, so, plain, ab_dev, 0, l++, n_main, cs);
     for (int q = 0; q < nsub % K; ++q)
-      launch_batched(kEvpRemainderCfg, b->arith, so, ab_dev, 1, l++, n_rem, cs);
+      launch_batched(kEvpRemainderCfg, b->arith, so, plain, ab_dev, 1, l++, n_rem, cs);
   };
   if (nsub >= 8 && graphs_enabled()) {
     char key[192];
-    snprintf(key, sizeof(key), "evpb/%d/%d/%d/%d/%d/%d/%d/%p", (int)sizeof(T), b->arith, (int)so,
-             cfg, nsub, n_main, n_rem, (const void*)ab_dev);
+    snprintf(key, sizeof(key), "evpb/%d/%d/%d/%d/%d/%d/%d/%d/%p", (int)sizeof(T), b->arith,
+             (int)so, (int)plain, cfg, nsub, n_main, n_rem, (const void*)ab_dev);
     int rc = graph_run(key, record, st);
     if (rc) return rc;
   } else {

@@ -12,18 +12,18 @@
 namespace vh {
 
 void evp_batched_launch_f32(int cfg, const EvpArgBlock<float>* ab, int which, int launch,
-                            int n_tiles, bool so, hipStream_t st) {
-  evp_launch_cfg<float, double, true>(cfg, ab, which, launch, n_tiles, 0, st, so);
+                            int n_tiles, bool so, bool plain, hipStream_t st) {
+  evp_launch_cfg<float, double, true>(cfg, ab, which, launch, n_tiles, 0, st, so, plain);
 }
 
 void evp_batched_launch_f32_f64(int cfg, const EvpArgBlock<float>* ab, int which, int launch,
-                                int n_tiles, bool so, hipStream_t st) {
-  evp_launch_cfg<float, double, true, double>(cfg, ab, which, launch, n_tiles, 0, st, so);
+                                int n_tiles, bool so, bool plain, hipStream_t st) {
+  evp_launch_cfg<float, double, true, double>(cfg, ab, which, launch, n_tiles, 0, st, so, plain);
 }
 
 void evp_batched_launch_f32_s64(int cfg, const EvpArgBlock<float, double>* ab, int which,
-                                int launch, int n_tiles, bool so, hipStream_t st) {
-  evp_launch_cfg<float, double, true, double, double>(cfg, ab, which, launch, n_tiles, 0, st, so);
+                                int launch, int n_tiles, bool so, bool plain, hipStream_t st) {
+  evp_launch_cfg<float, double, true, double, double>(cfg, ab, which, launch, n_tiles, 0, st, so, plain);
 }
 
 }  // namespace vh

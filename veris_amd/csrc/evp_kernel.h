@@ -447,7 +447,7 @@ __device__ inline int next_axis(int q, int N, int n, int filled) {
 }
 
 template <typename T, typename TD, int RI, int RJ, int K, bool BATCHED, typename A = T,
-          bool SO = false, bool RES = false, typename S = T>
+          bool SO = false, bool RES = false, typename S = T, bool PL = false>
 __global__ __launch_bounds__(RI* RJ) void evp_substep_kernel(const EvpArgBlock<T, S>* __restrict__ ab,
                                                              int which, int launch, int n_tiles,
                                                              int tiles_j, int per_xcd) {
@@ -525,6 +525,11 @@ __global__ __launch_bounds__(RI* RJ) void evp_substep_kernel(const EvpArgBlock<T
     S* const* dst = ((nl - 1 - launch) % 2 == 0) ? ab->out : ab->alt;
     const EvpConst<T, S>& f = ab->c;
     const EvpScalars& sc = ab->sc;
+    // settings branches; PL instantiates the default settings (none of them,
+    // tensileStrFac = 0) with every branch compiled out
+    const bool f_kterms = !PL && sc.kterms, f_turn = !PL && sc.turn, f_basal = !PL && sc.basal;
+    const bool f_adaptive = !PL && sc.adaptive, f_sidedrag = !PL && sc.sidedrag;
+    const bool f_coast = !PL && sc.coast, f_tens0 = PL || sc.tens0;
 
     const int t = threadIdx.x;
     const int ar = t / RJ, br = t & (RJ - 1);  // region coordinates
@@ -603,7 +608,7 @@ __global__ __launch_bounds__(RI* RJ) void evp_substep_kernel(const EvpArgBlock<T
       g_rrAv = ldg(f.rrAv , ce);
     }
     A k1Z = 0, k2Z = 0, k1C = 0, k2C = 0;  // metric terms
-    if (sc.kterms) {
+    if (f_kterms) {
       k1Z = ldg(f.k1Z , ce);
       k2Z = ldg(f.k2Z , ce);
     }
@@ -617,7 +622,7 @@ __global__ __launch_bounds__(RI* RJ) void evp_substep_kernel(const EvpArgBlock<T
     const A AW = ldg(f.AreaW , ce), AS = ldg(f.AreaS , ce);
     const A wfx = ldg(f.WFX , ce), wfy = ldg(f.WFY , ce);
     const A uNm1 = ldg(f.uNm1 , ce), vNm1 = ldg(f.vNm1 , ce);
-    if (sc.kterms) {
+    if (f_kterms) {
       k1C = ldg(f.k1C , ce);
       k2C = ldg(f.k2C , ce);
     }
@@ -629,7 +634,7 @@ __global__ __launch_bounds__(RI* RJ) void evp_substep_kernel(const EvpArgBlock<T
     A r_aC = (A)sc.alpha, r_aZ = (A)sc.alpha, r_bU = (A)sc.beta, r_bV = (A)sc.beta;
     // lateral drag form factors at this u- and v-point (veris/dynamics_routines.py:119-130)
     A sdmU = (A)0, sdmV = (A)0;
-    if (sc.sidedrag && sc.coast) {
+    if (f_sidedrag && f_coast) {
       sdmU = ldg(f.Fu , ce);
       sdmV = ldg(f.Fv , ce);
     }
@@ -676,9 +681,9 @@ __global__ __launch_bounds__(RI* RJ) void evp_substep_kernel(const EvpArgBlock<T
         const A vavz = (v + vim) * (A)0.5;
         const A mz = (Mc * sM[t - RJ]) * (sM[t - 1] * sM[t - RJ - 1]);
         A inner = dudy + dvdx;
-        if (sc.kterms) inner = (inner - k1Z * vavz) - k2Z * uavz;
+        if (f_kterms) inner = (inner - k1Z * vavz) - k2Z * uavz;
         A e12 = ((A)0.5 * inner) * mz;
-        if (!sc.sidedrag) {  // no-slip boundary term (veris/dynamics_routines.py:170-175)
+        if (!f_sidedrag) {  // no-slip boundary term (veris/dynamics_routines.py:170-175)
           const A hu = MUc - sMU[t - 1];
           const A hv = MVc - sMV[t - RJ];
           e12 = e12 + ((((A)2.0 * uavz) * rdyU) * hu + (((A)2.0 * vavz) * rdxV) * hv);
@@ -710,7 +715,7 @@ __global__ __launch_bounds__(RI* RJ) void evp_substep_kernel(const EvpArgBlock<T
         sE12[t] = e12;
         sQ[t] = rAz * (e12 * e12);
       }
-      if (sc.sidedrag && !sc.coast && upd) {  // land neighbours along the face normal
+      if (f_sidedrag && !f_coast && upd) {  // land neighbours along the face normal
         sdmU = (((A)2 - sMU[t - RJ]) - sMU[t + RJ]) * MUc;
         sdmV = (((A)2 - sMV[t - 1]) - sMV[t + 1]) * MVc;
       }
@@ -726,7 +731,7 @@ __global__ __launch_bounds__(RI* RJ) void evp_substep_kernel(const EvpArgBlock<T
         const A dvdy = (vjp - v) * rdyV;
         const A vave = (vjp + v) * (A)0.5;
         A e11 = dudx, e22 = dvdy;
-        if (sc.kterms) {
+        if (f_kterms) {
           e11 = dudx + vave * k2C;
           e22 = dvdy + uave * k1C;
         }
@@ -735,7 +740,7 @@ __global__ __launch_bounds__(RI* RJ) void evp_substep_kernel(const EvpArgBlock<T
         const A qa = sQ[t] + sQ[t + RJ];
         const A qb = sQ[t + 1] + sQ[t + RJ + 1];
         const A e12c = ((A)0.25 * rrA) * (qa + qb);
-        if (sc.sidedrag) {  // ice speed at c (veris/dynamics_routines.py:109-111)
+        if (f_sidedrag) {  // ice speed at c (veris/dynamics_routines.py:109-111)
           const A su = u + uip, sv = v + vjp;
           spd = (A)0.5 * sqrt(su * su + sv * sv);
         }
@@ -745,12 +750,12 @@ __global__ __launch_bounds__(RI* RJ) void evp_substep_kernel(const EvpArgBlock<T
         const TD dreg = dC + (TD)sc.deltaMin;
         // with t = 0 the products and the quotient by 1 + t = 1 are exact
         // identities, so skipping them keeps every bit
-        const TD Pt = sc.tens0 ? (TD)P : (TD)P * (TD)sc.tens1p;
+        const TD Pt = f_tens0 ? (TD)P : (TD)P * (TD)sc.tens1p;
         const TD zeta = ((TD)0.5 * Pt) / dreg;
         sZ[t] = (A)zeta;
         // relaxation parameter at c (veris/evp_solver.py:65-73 when adaptive)
         A am1 = (A)sc.alphaM1, den1 = (A)sc.den1;
-        if (sc.adaptive) {
+        if (f_adaptive) {
           A aC = sqrt((((A)zeta * (A)sc.evpcfac) / fmax(mC, (A)1e-4)) * rrA) * Mc;
           aC = fmax(aC, (A)sc.alphaMin);
           sAl[t] = aC;
@@ -760,9 +765,9 @@ __global__ __launch_bounds__(RI* RJ) void evp_substep_kernel(const EvpArgBlock<T
         }
         if (cs) {
           TD prep = (((TD)2.0 * zeta) * dC) * (TD)sc.pr;
-          if (!sc.tens0) prep = prep / (TD)sc.tens1p;
+          if (!f_tens0) prep = prep / (TD)sc.tens1p;
           TD pressD = ((TD)P * (TD)sc.pr1m) + prep;
-          if (!sc.tens0) pressD = pressD * (TD)sc.tens1m;
+          if (!f_tens0) pressD = pressD * (TD)sc.tens1m;
           const A zt = (A)zeta, press = (A)pressD;
           const A divg = (((A)2 * zt) * ep) - press;
           const A tens = ((A)2 * zt) * em;
@@ -775,7 +780,7 @@ __global__ __launch_bounds__(RI* RJ) void evp_substep_kernel(const EvpArgBlock<T
           const A w = (A)0.25 * (su * su + sv * sv);
           const A cd = ((D * D) * w > (A)sc.cDragMin2) ? D * evp_sqrt(w) : (A)sc.cDragMin;
           sCD[t] = cd * Mc;
-          if (sc.basal) {
+          if (f_basal) {
             const int cip = next_axis<K>(gi, g.NX, g.nx, g.fill_x) * g.NY + gj;
             const int cjp = gi * g.NY + next_axis<K>(gj, g.NY, g.ny, g.fill_y);
             const A um = u * mInUc, umip = uip * ldg(f.mInU , oo + (unsigned)cip);
@@ -788,20 +793,20 @@ __global__ __launch_bounds__(RI* RJ) void evp_substep_kernel(const EvpArgBlock<T
                           exp((A)(-sc.cBasalStar) * ((A)1.0 - Ar));
             cbc = Ar > (A)0.01 ? val : (A)0;
           }
-          sCB[t] = cbc;
+          if (!PL) sCB[t] = cbc;
           const A mf = (mC * fc) * (A)0.5;
           sFV[t] = mf * (v + vjp);
           sFU[t] = mf * (u + uip);
         }
       }
-      if (sc.turn) {  // ocean velocity for the turning term (sMU/sMV are free after B)
+      if (f_turn) {  // ocean velocity for the turning term (sMU/sMV are free after B)
         sMU[t] = uoc;
         sMV[t] = voc;
       }
       __syncthreads();
 
       // ---- D: sigma12 on z-points (veris/averaging.py:4-19, veris/evp_solver.py:86,103)
-      if (sc.sidedrag && cc) sDV[t] = spd;  // dv is dead after C
+      if (f_sidedrag && cc) sDV[t] = spd;  // dv is dead after C
       if (zs) {
         const A n = (Mc + sM[t - RJ]) + (sM[t - 1] + sM[t - RJ - 1]);
         // 1/n: 0/1 masks give n in {0, ..., 4}, whose reciprocals are exact
@@ -815,7 +820,7 @@ __global__ __launch_bounds__(RI* RJ) void evp_substep_kernel(const EvpArgBlock<T
         const A zz = nrm * ((sZ[t] + sZ[t - RJ]) + (sZ[t - 1] + sZ[t - RJ - 1]));
         const A shear = ((A)2 * zz) * sE12[t];
         A am1z = (A)sc.alphaM1, den2 = (A)sc.den2;
-        if (sc.adaptive) {  // veris/evp_solver.py:97-100
+        if (f_adaptive) {  // veris/evp_solver.py:97-100
           A aZ = (A)0.5 * (sAl[t] + sAl[t - 1]);
           aZ = (A)0.5 * (aZ + (A)0.5 * (sAl[t - RJ] + sAl[t - RJ - 1]));
           am1z = aZ - (A)1;
@@ -838,7 +843,7 @@ __global__ __launch_bounds__(RI* RJ) void evp_substep_kernel(const EvpArgBlock<T
         const A cdV = (A)0.5 * (cdc + sCD[t - 1]);
         A inx = (cdU * (A)sc.cosWat) * uoc;
         A iny = (cdV * (A)sc.cosWat) * voc;
-        if (sc.turn) {
+        if (f_turn) {
           const A lmU = mU != (A)0 ? (A)1 : mU;
           const A lmV = mV != (A)0 ? (A)1 : mV;
           const A X = (cdc * (A)0.5) * ((uoc - u) + (sMU[t + 1] - sU[t + 1])) +
@@ -855,7 +860,7 @@ __global__ __launch_bounds__(RI* RJ) void evp_substep_kernel(const EvpArgBlock<T
         fx = fx + (A)0.5 * (sFV[t] + sFV[t - RJ]);
         fy = fy - (A)0.5 * (sFU[t] + sFU[t - 1]);
         A bU = (A)sc.betaU, bV = (A)sc.betaU, bP1U = (A)sc.betaP1, bP1V = (A)sc.betaP1;
-        if (sc.adaptive) {  // veris/evp_solver.py:148-155
+        if (f_adaptive) {  // veris/evp_solver.py:148-155
           bU = ((A)0.5 * (sAl[t] + sAl[t - RJ])) * (A)sc.rdt;
           bV = ((A)0.5 * (sAl[t] + sAl[t - 1])) * (A)sc.rdt;
           bP1U = bU + (A)sc.rdt;
@@ -865,11 +870,14 @@ __global__ __launch_bounds__(RI* RJ) void evp_substep_kernel(const EvpArgBlock<T
             r_bV = (A)0.5 * (sAl[t] + sAl[t - 1]);
           }
         }
-        A denU = mU * bP1U + AW * ((cdU * (A)sc.cosWat) + (A)0.5 * (sCB[t] + sCB[t - RJ]));
-        A denV = mV * bP1V + AS * ((cdV * (A)sc.cosWat) + (A)0.5 * (sCB[t] + sCB[t - 1]));
+        // basal drag at u and v (+0 without it, as the averages of the +0 coefficients)
+        const A cbU = PL ? (A)0 : (A)0.5 * (sCB[t] + sCB[t - RJ]);
+        const A cbV = PL ? (A)0 : (A)0.5 * (sCB[t] + sCB[t - 1]);
+        A denU = mU * bP1U + AW * ((cdU * (A)sc.cosWat) + cbU);
+        A denV = mV * bP1V + AS * ((cdV * (A)sc.cosWat) + cbV);
         denU = denU == (A)0 ? (A)1 : denU;
         denV = denV == (A)0 ? (A)1 : denV;
-        if (sc.sidedrag) {  // lateral drag (veris/dynamics_routines.py:113-138, evp_solver.py:169-175)
+        if (f_sidedrag) {  // lateral drag (veris/dynamics_routines.py:113-138, evp_solver.py:169-175)
           const A spU = AW > (A)0 ? (A)0.5 * (sDV[t] + sDV[t - RJ]) : (A)0;
           const A spV = AS > (A)0 ? (A)0.5 * (sDV[t] + sDV[t - 1]) : (A)0;
           denU = denU + ((mU * (A)sc.sdCoeff) * sdmU) / (spU + (A)sc.sdU0);
@@ -965,7 +973,7 @@ constexpr int kMaxDevices = 64;
   X(1, 8, 64, 1) X(2, 12, 64, 1) X(3, 16, 64, 1) X(4, 8, 32, 1) X(5, 16, 32, 1)                \
   X(6, 32, 32, 1) X(7, 6, 64, 1) X(8, 16, 64, 2) X(9, 32, 32, 2) X(10, 12, 64, 2)              \
   X(11, 32, 32, 3) X(12, 16, 64, 3) X(13, 32, 32, 4) X(14, 16, 32, 2) X(15, 24, 32, 2)         \
-  X(16, 13, 32, 1) X(17, 14, 32, 1)
+  X(16, 13, 32, 1) X(17, 14, 32, 1) X(18, 20, 32, 2) X(19, 28, 32, 2) X(20, 28, 32, 3)
 
 struct TileCfg {
   int ri, rj, k;
@@ -1030,12 +1038,12 @@ inline int evp_grid(F kernel, int threads, int n_tiles, std::atomic<int>* cache)
 }
 
 template <typename T, typename TD, int RI, int RJ, int K, bool BATCHED, typename A = T,
-          bool SO = false, typename S = T>
+          bool SO = false, typename S = T, bool PL = false>
 inline void evp_launch_shape(const EvpArgBlock<T, S>* ab, int which, int launch, int n_tiles,
                              int tiles_j, hipStream_t st) {
   static std::atomic<int> cache[kMaxDevices];
   const int per_xcd = (n_tiles + 7) / 8;
-  auto kernel = evp_substep_kernel<T, TD, RI, RJ, K, BATCHED, A, SO, false, S>;
+  auto kernel = evp_substep_kernel<T, TD, RI, RJ, K, BATCHED, A, SO, false, S, PL>;
   const int grid = BATCHED ? evp_grid(kernel, RI * RJ, n_tiles, cache) : 8 * per_xcd;
   hipLaunchKernelGGL(kernel, dim3(grid), dim3(RI * RJ), 0, st, ab, which, launch, n_tiles,
                      tiles_j, per_xcd);
@@ -1066,21 +1074,36 @@ inline void evp_launch_res(const EvpArgBlock<T>* ab, int launch, int n_tiles, in
 }
 constexpr int kEvpResidualCfg = 5;
 
+// True when the settings take none of the optional branches of the substep
+// (metric terms, turning angle, basal drag, adaptive EVP, free slip, the
+// second-order boundary term) and tensileStrFac = 0: the default settings,
+// which launch the kernel instantiation with those branches compiled out.
+inline bool evp_plain_settings(const EvpScalars& sc) {
+  return !sc.kterms && !sc.turn && !sc.basal && !sc.adaptive && !sc.sidedrag && !sc.sobc &&
+         sc.tens0;
+}
+
 // One launch of shape `cfg`: tiles_j is used by single-domain launches, `which`
 // (0 main shape, 1 remainder shape) by batched ones.  T is the storage type
 // of the fields, A the arithmetic type (default T), TD that of Delta/zeta;
-// `so` selects the second-order boundary variant (k = 1 shapes only).
+// `so` selects the second-order boundary variant (k = 1 shapes only), `plain`
+// the default-settings instantiation (evp_plain_settings of the launch's
+// scalars; the same results bit for bit as the general one).
 template <typename T, typename TD, bool BATCHED, typename A = T, typename S = T>
 inline void evp_launch_cfg(int cfg, const EvpArgBlock<T, S>* ab, int which, int launch, int n_tiles,
-                           int tiles_j, hipStream_t st, bool so = false) {
+                           int tiles_j, hipStream_t st, bool so = false, bool plain = false) {
   if (so) {
     evp_launch_so<T, TD, BATCHED, A, S>(cfg, ab, which, launch, n_tiles, tiles_j, st);
     return;
   }
   evp_with_shape(cfg, [&](auto sh) {
     using Sh = decltype(sh);
-    evp_launch_shape<T, TD, Sh::ri, Sh::rj, Sh::k, BATCHED, A, false, S>(ab, which, launch,
-                                                                         n_tiles, tiles_j, st);
+    if (plain)
+      evp_launch_shape<T, TD, Sh::ri, Sh::rj, Sh::k, BATCHED, A, false, S, true>(
+          ab, which, launch, n_tiles, tiles_j, st);
+    else
+      evp_launch_shape<T, TD, Sh::ri, Sh::rj, Sh::k, BATCHED, A, false, S>(ab, which, launch,
+                                                                           n_tiles, tiles_j, st);
   });
 }
 

@@ -104,6 +104,12 @@ int graph_run(const std::string& key, const std::function<void(hipStream_t)>& re
 // Whether graph replay is enabled (environment VERIS_EVP_GRAPH=0 disables it).
 int graphs_enabled();
 
+// Whether EVP subcycles under the default settings launch the substep
+// instantiation with the optional branches compiled out (read at every call;
+// environment VERIS_EVP_PLAIN=0 selects the general instantiation, which gives
+// the same results bit for bit).
+int evp_plain_kernels_enabled();
+
 // 1-D launch helper: one thread per padded cell.
 inline dim3 cell_grid(const Geom& g, int block) {
   size_t n = field_elems(g);

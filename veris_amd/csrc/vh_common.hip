@@ -66,6 +66,11 @@ int graphs_enabled() {
   return en;
 }
 
+int evp_plain_kernels_enabled() {
+  const char* e = getenv("VERIS_EVP_PLAIN");
+  return (e && e[0] == '0') ? 0 : 1;
+}
+
 int graph_run(const std::string& key, const std::function<void(hipStream_t)>& record,
               hipStream_t st) {
   int dev = 0;
