@@ -42,6 +42,7 @@
 #pragma once
 
 #include <atomic>
+#include <type_traits>
 
 #include "vh_common.h"
 
@@ -464,7 +465,11 @@ __global__ __launch_bounds__(RI* RJ) void evp_substep_kernel(const EvpArgBlock<T
   // over its XCD's run with a stride of the XCD's workgroup count
   const int bid = blockIdx.x;
   const int xcd = bid & 7, k0 = bid >> 3;
-  auto run_tile = [&](const int tile) {
+  const bool packed = ab->packed != 0;  // uniform: step constants in packed form
+  // the tile body is instantiated twice, for packed step constants and for
+  // full fields (pk: std::true_type / std::false_type), each with its own
+  // schedule and registers; the packed flag of the argument block picks one
+  auto run_tile = [&](const int tile, auto pk) {
     Geom g;
     long long off = 0;
     int roff = 0;
@@ -552,7 +557,7 @@ __global__ __launch_bounds__(RI* RJ) void evp_substep_kernel(const EvpArgBlock<T
     // ---- all global loads of this cell, issued together.  Packed constants
     //      (uniform flag): the masks come from one byte, the metrics from rows
     //      held in LDS (sRow, filled below); otherwise from the full fields.
-    const bool packed = ab->packed != 0;
+    constexpr bool packed = decltype(pk)::value;
     // metric rows of this tile's columns: issued first, so that stage A, which
     // stores them to LDS before its barrier, waits for no later load
     constexpr int RPT = (kEvpRowFields * RJ + NT - 1) / NT;  // rows per thread
@@ -591,22 +596,6 @@ __global__ __launch_bounds__(RI* RJ) void evp_substep_kernel(const EvpArgBlock<T
     // issue order = order of use: stage A waits only for the loads above, the
     // rest stream in while A and B run
     __builtin_amdgcn_sched_barrier(0);
-    A g_rdyU = 0, g_rdxV = 0, g_rAz = 0, g_rdxU = 0, g_rdyV = 0, g_rrA = 0;
-    A g_dyV = 0, g_dxU = 0, g_dxV = 0, g_dyU = 0, g_rrAu = 0, g_rrAv = 0;
-    if (!packed) {
-      g_rdyU = ldg(f.rdyU , ce);
-      g_rdxV = ldg(f.rdxV , ce);
-      g_rAz = ldg(f.rAz , ce);
-      g_rdxU = ldg(f.rdxU , ce);
-      g_rdyV = ldg(f.rdyV , ce);
-      g_rrA = ldg(f.rrA , ce);
-      g_dyV = ldg(f.dyV , ce);
-      g_dxU = ldg(f.dxU , ce);
-      g_dxV = ldg(f.dxV , ce);
-      g_dyU = ldg(f.dyU , ce);
-      g_rrAu = ldg(f.rrAu , ce);
-      g_rrAv = ldg(f.rrAv , ce);
-    }
     A k1Z = 0, k2Z = 0, k1C = 0, k2C = 0;  // metric terms
     if (f_kterms) {
       k1Z = ldg(f.k1Z , ce);
@@ -640,8 +629,12 @@ __global__ __launch_bounds__(RI* RJ) void evp_substep_kernel(const EvpArgBlock<T
     }
     // metric rows in LDS (stored in stage A of the first substep)
     __shared__ A sRow[kEvpRowFields * RJ];
-    // metric q of this thread's column
-    auto met = [&](int q, A full) -> A { return packed ? sRow[q * RJ + br] : full; };
+    // metric q (the order of the pack kernel's rows) at this cell: from the
+    // rows in LDS, or, on grids whose constants do not pack, loaded from its
+    // field where it is used (no registers held for it across the stages)
+    auto met = [&](int q, const T* full) -> A {
+      return packed ? sRow[q * RJ + br] : (A)ldg(full, ce);
+    };
 
   #pragma unroll 1
     for (int s = 0; s < K; ++s) {
@@ -673,7 +666,7 @@ __global__ __launch_bounds__(RI* RJ) void evp_substep_kernel(const EvpArgBlock<T
 
       // ---- B: strain rate at z-points (veris/dynamics_routines.py:160-175)
       if (zc) {
-        const A rdyU = met(0, g_rdyU), rdxV = met(1, g_rdxV), rAz = met(2, g_rAz);
+        const A rdyU = met(0, f.rdyU), rdxV = met(1, f.rdxV), rAz = met(2, f.rAz);
         const A ujm = sU[t - 1], vim = sV[t - RJ];
         const A dudy = (u - ujm) * rdyU;
         const A uavz = (u + ujm) * (A)0.5;
@@ -724,7 +717,7 @@ __global__ __launch_bounds__(RI* RJ) void evp_substep_kernel(const EvpArgBlock<T
       // ---- C: c-point rheology (veris/dynamics_routines.py:151-158, 220-268),
       //         stresses (veris/evp_solver.py:79-94), drags (:108-110), Coriolis (:142-143)
       if (cc) {
-        const A rdxU = met(3, g_rdxU), rdyV = met(4, g_rdyV), rrA = met(5, g_rrA);
+        const A rdxU = met(3, f.rdxU), rdyV = met(4, f.rdyV), rrA = met(5, f.rrA);
         const A uip = sU[t + RJ], vjp = sV[t + 1];
         const A dudx = (uip - u) * rdxU;
         const A uave = (uip + u) * (A)0.5;
@@ -773,8 +766,8 @@ __global__ __launch_bounds__(RI* RJ) void evp_substep_kernel(const EvpArgBlock<T
           const A tens = ((A)2 * zt) * em;
           s1 = (((s1 * am1) + divg) * den1) * Mc;
           s2 = (((s2 * am1) + tens * (A)0.25) * den1) * Mc;
-          sS11[t] = ((A)0.5 * (s1 + s2)) * met(6, g_dyV);
-          sS22[t] = ((A)0.5 * (s1 - s2)) * met(7, g_dxU);
+          sS11[t] = ((A)0.5 * (s1 + s2)) * met(6, f.dyV);
+          sS22[t] = ((A)0.5 * (s1 - s2)) * met(7, f.dxU);
           const A D = (A)(fc < (A)0 ? sc.cwS : sc.cwN) * (A)sc.rhoSea;
           const A su = sDU[t] + sDU[t + RJ], sv = sDV[t] + sDV[t + 1];
           const A w = (A)0.25 * (su * su + sv * sv);
@@ -828,16 +821,16 @@ __global__ __launch_bounds__(RI* RJ) void evp_substep_kernel(const EvpArgBlock<T
           if constexpr (RES) r_aZ = aZ;
         }
         s12 = ((s12 * am1z) + shear * (A)0.25) * den2;
-        sQ[t] = s12 * met(8, g_dxV);   // sigma12*dxV (q is dead after C)
-        sDU[t] = s12 * met(9, g_dyU);  // sigma12*dyU (du is dead after C)
+        sQ[t] = s12 * met(8, f.dxV);   // sigma12*dxV (q is dead after C)
+        sDU[t] = s12 * met(9, f.dyU);  // sigma12*dyU (du is dead after C)
       }
       __syncthreads();
 
       // ---- E: stress divergence (veris/dynamics_routines.py:284-302), forcing
       //         (veris/evp_solver.py:112-145) and momentum update (:152-187)
       if (upd) {
-        const A divx = (((sS11[t] - sS11[t - RJ]) - sQ[t]) + sQ[t + 1]) * met(10, g_rrAu);
-        const A divy = (((sS22[t] - sS22[t - 1]) - sDU[t]) + sDU[t + RJ]) * met(11, g_rrAv);
+        const A divx = (((sS11[t] - sS11[t - RJ]) - sQ[t]) + sQ[t + 1]) * met(10, f.rrAu);
+        const A divy = (((sS22[t] - sS22[t - 1]) - sDU[t]) + sDU[t + RJ]) * met(11, f.rrAv);
         const A cdc = sCD[t];
         const A cdU = (A)0.5 * (cdc + sCD[t - RJ]);
         const A cdV = (A)0.5 * (cdc + sCD[t - 1]);
@@ -951,11 +944,15 @@ __global__ __launch_bounds__(RI* RJ) void evp_substep_kernel(const EvpArgBlock<T
       const int tile = xcd * per_xcd + k;
       if (tile >= n_tiles) break;
       if (k != k0) __syncthreads();  // the previous tile's last stage still reads LDS
-      run_tile(tile);
+      if (packed) run_tile(tile, std::true_type{});
+      else run_tile(tile, std::false_type{});
     }
   } else {
     const int tile = xcd * per_xcd + k0;
-    if (tile < n_tiles) run_tile(tile);
+    if (tile < n_tiles) {
+      if (packed) run_tile(tile, std::true_type{});
+      else run_tile(tile, std::false_type{});
+    }
   }
 }
 
