@@ -361,10 +361,10 @@ def bench_batched(dev, nsub: int, fp64_evp_rate: float, fp64_step_rate: float) -
     }
 
 
-def _adjoint_run(state, params, target, n, ckpt):
+def _adjoint_run(state, params, target, n, ckpt, tile=0):
     from veris_amd import adjoint as A
 
-    outs = A.evp_subcycle(state, n_substeps=n, ckpt_every=ckpt, **params)
+    outs = A.evp_subcycle(state, n_substeps=n, ckpt_every=ckpt, tile=tile, **params)
     ut, vt = target
     return ((outs[0] - ut)[2:-2, 2:-2] ** 2).sum() + ((outs[1] - vt)[2:-2, 2:-2] ** 2).sum()
 
@@ -400,6 +400,11 @@ def bench_adjoint(dev, ckpt: int = 10) -> dict:
 
         p = leafs()
         pst = torch.tensor(pstar_np, dtype=dtype, device=dev)
+        # the inputs are periodic in the x overlap: the forward and the
+        # recompute run two substeps per launch (same bits as one)
+        from veris_amd.adjoint import pair_tile
+
+        tile = pair_tile(st, pStar=pst)
 
         def plain_forward():
             vs = st.variables
@@ -411,20 +416,26 @@ def bench_adjoint(dev, ckpt: int = 10) -> dict:
 
         def ckpt_forward():
             with torch.no_grad():
-                _adjoint_run(st, p, target, n, ckpt)
+                _adjoint_run(st, p, target, n, ckpt, tile)
 
         def fwd_bwd():
             for v in p.values():
                 v.grad = None
-            _adjoint_run(st, p, target, n, ckpt).backward()
+            _adjoint_run(st, p, target, n, ckpt, tile).backward()
 
+        # the plain forward on its fastest launch shape for this state and precision
+        from veris_amd.autotune import tune_evp
+
+        plain_forward()
+        tune_evp(st, exact=False, n_substeps=n)
         for fn in (plain_forward, ckpt_forward, fwd_bwd):
             fn()
         t_plain = _sync_time(plain_forward, 7)
         t_fwd = _sync_time(ckpt_forward, 7)
         t_fb = _sync_time(fwd_bwd, 7)
         r = {"plain_forward_ms": t_plain * 1e3, "checkpointed_forward_ms": t_fwd * 1e3,
-             "forward_backward_ms": t_fb * 1e3,
+             "forward_backward_ms": t_fb * 1e3, "forward_tile": tile,
+             "plain_forward_tile": K.EVP_TILE.get((tuple(st.variables.iceMask.shape), dtype), 0),
              "adjoint_over_plain_forward": t_fb / t_plain,
              "adjoint_over_checkpointed_forward": t_fb / t_fwd, "target": 4.0}
         if tag == "fp64":

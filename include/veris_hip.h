@@ -331,12 +331,28 @@ int vh_evp_tile_count(void);
 
 /* Launch shape of EVP tile index `tile`: ri x rj threads, one per cell of a
  * (ri-4k) x (rj-4k) tile plus a 2k-cell ring, running k substeps per launch.
- * k = 1 reproduces the reference on every cell; k > 1 keeps the velocities on
- * every cell and the stresses on every cell except the x-periodic overlap
- * columns: there k > 1 gives periodic values, while the reference's are
- * numpy.roll wrap-around artefacts that need the post-fill velocities of both
- * x edges within one substep. */
+ * k = 1 reproduces the reference on every cell.  k > 1 reads each filled
+ * overlap through the interior cells the fill copies from: on inputs for
+ * which vh_evp_overlaps_periodic gives 1 (every domain without a fill), it
+ * keeps the velocities on every cell and the stresses on every cell except
+ * the filled overlap columns (rows), where it gives periodic values while the
+ * reference's are numpy.roll wrap-around artefacts (they need the post-fill
+ * velocities of both edges within one substep, and nothing else reads them). */
 int vh_evp_tile_shape(int32_t tile, int32_t* ri, int32_t* rj, int32_t* k);
+
+/* Whether multi-substep tile shapes (k > 1) give the single-substep results
+ * on these inputs: writes 1 to the device int `flag` (on `stream`, no host
+ * synchronisation) when every field such a launch reads at neighbouring
+ * cells holds, in each filled overlap, the value of the cell the fill copies
+ * from (bit for bit): uIce, vIce, uOcean, vOcean, the six masks, the ten
+ * strain/stress metrics, SeaIceStrength, fCori, SeaIceMassC, the metric terms
+ * when set, and with basal drag R_low, Area and hIceMean; and sigma1, sigma2
+ * in overlap column (row) 1 and sigma12 in column (row) NX-2 (NY-2), the only
+ * overlap stresses the interior reads.  Otherwise writes 0.  A domain
+ * without a filled overlap always gets 1.  No counterpart in the reference:
+ * it guards the multi-substep launches of this library. */
+int vh_evp_overlaps_periodic(const vh_domain* d, const vh_settings* s, const vh_fields* f,
+                             int32_t* flag, void* stream);
 
 /* Workspace bytes vh_evp_solver needs for this domain. */
 int vh_evp_workspace_bytes(const vh_domain* d, size_t* bytes);

@@ -130,8 +130,8 @@ def test_adjoint_host_side_validation():
     with pytest.raises(N.VerisHipError, match="VH_EINVAL.*missing input"):
         N.call("vh_evp_forward_checkpointed", ctypes.byref(dom), ctypes.byref(sett),
                ctypes.byref(fields), 120, 10, None, None, None, None, None, None, 0, None)
-    multi = N.Domain(nx=360, ny=160, olx=2, oly=2, cyclic_x=1, fill_mode=0, dtype=N.VH_F64, tile=8)
-    with pytest.raises(N.VerisHipError, match="VH_EINVAL.*single-substep"):
+    multi = N.Domain(nx=360, ny=160, olx=2, oly=2, cyclic_x=1, fill_mode=0, dtype=N.VH_F64, tile=11)
+    with pytest.raises(N.VerisHipError, match="VH_EINVAL.*one or two substeps"):
         N.call("vh_evp_adjoint", ctypes.byref(multi), ctypes.byref(sett), ctypes.byref(fields),
                120, 10, None, None, None, None, None, ctypes.byref(N.EvpAdjointOut()), None, 0,
                None)

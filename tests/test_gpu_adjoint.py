@@ -260,6 +260,74 @@ def test_every_reverse_output_through_the_c_abi(cuda, kind):
         assert abs(got - ref) <= 1e-9 * abs(ref), k
 
 
+def _periodic_params(st, p):
+    """The per-cell parameters with their x overlap filled periodically (as the
+    fill leaves a model field), so that two-substep launches apply."""
+    for k in ("pStar", "hIceMean"):
+        a = p[k].detach().cpu().numpy().copy()
+        a[:2] = a[-4:-2]
+        a[-2:] = a[2:4]
+        p[k] = torch.tensor(a, dtype=p[k].dtype, device=p[k].device).requires_grad_(True)
+    return p
+
+
+@pytest.mark.parametrize("kind", ["global", "regional"])
+def test_two_substep_forward_launches_change_no_gradient(cuda, kind, monkeypatch):
+    """Under the default settings with an even subcycle and checkpoint
+    interval, the checkpointed forward and the recompute may run two
+    substeps per launch (the recompute keeps the intermediate state): on a
+    domain without a filled overlap by themselves, on a filled one when the
+    caller names a two-substep tile that pair_tile confirms.  Every gradient
+    equals the one-substep-per-launch path bit for bit, and so do the
+    returned velocities; the returned stresses too, except in the filled
+    overlap columns (periodic values there instead of the reference's
+    wrap-around values)."""
+    dev = torch.device("cuda")
+    st = _state(kind)
+    gst = st.to(dev)
+    p = _periodic_params(st, _params(st, dev))
+    tile = A.pair_tile(gst, pStar=p["pStar"], hIceMean=p["hIceMean"])
+    assert tile > 0
+    ws = [torch.tensor(w, device=dev) for w in _weights(st, 3)]
+
+    def run(t):
+        q = {k: v.detach().clone().requires_grad_(True) for k, v in p.items()}
+        outs = A.evp_subcycle(gst, n_substeps=24, ckpt_every=10, tile=t, **q)
+        _objective(outs, ws).backward()
+        return {k: g2n(v.grad) for k, v in q.items()}, [g2n(o) for o in outs]
+
+    monkeypatch.setenv("VERIS_ADJ_PAIRS", "0")
+    g1, o1 = run(0)
+    monkeypatch.setenv("VERIS_ADJ_PAIRS", "1")
+    g2, o2 = run(tile if kind == "global" else 0)
+    for k in g1:
+        assert np.array_equal(g1[k], g2[k]), k
+    cyclic = st.settings.enable_cyclic_x or not st.settings.veros_fill
+    keep = np.arange(2, st.shape[0] - 2) if cyclic else np.arange(st.shape[0])
+    for q in range(5):
+        a, b = (o1[q], o2[q]) if q < 2 else (o1[q][keep], o2[q][keep])
+        assert_exact(f"{kind} {NAMES[q]} pairs", b, a)
+
+
+def test_pair_tile_refuses_inputs_without_periodic_overlaps(cuda):
+    """A per-cell P* that is not periodic in the x overlap (or settings with
+    an optional branch) gives no two-substep tile; an explicit two-substep
+    tile under such settings is refused."""
+    dev = torch.device("cuda")
+    st = _state("global")
+    gst = st.to(dev)
+    p = _params(st, dev)
+    assert A.pair_tile(gst, pStar=p["pStar"], hIceMean=p["hIceMean"]) == 0
+    p = _periodic_params(st, p)
+    tile = A.pair_tile(gst, pStar=p["pStar"], hIceMean=p["hIceMean"])
+    assert tile > 0
+    st2 = _state("adaptive")
+    g2 = st2.to(dev)
+    assert A.pair_tile(g2) == 0
+    with pytest.raises(N.VerisHipError, match="VH_EUNSUPPORTED"):
+        A.evp_subcycle(g2, n_substeps=24, ckpt_every=10, tile=tile)
+
+
 def test_checkpoint_interval_is_irrelevant(cuda):
     st = _state("global")
     g1, _, _ = _gpu_grads(st, 24, ckpt=1)

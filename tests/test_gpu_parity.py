@@ -610,6 +610,51 @@ def test_evp_every_tile_shape(cuda, tile):
                 assert_exact(f"{name}[tile={tile}, k={k}, n={n}]", a, b)
 
 
+def test_overlaps_periodic_guards_multi_substep_shapes(cuda):
+    """vh_evp_overlaps_periodic: 1 on the x-cyclic 4-degree state (its
+    inputs are filled), unchanged by a wrap-around value in an overlap stress
+    that nothing interior reads (sigma1 in column 0), 0 for a value the
+    interior reads (sigma1 in column 1, sigma12 in column NX-2, an ocean
+    current anywhere in the overlap); always 1 without a filled overlap.
+    Where it gives 0, a multi-substep shape indeed departs from the oracle."""
+    from veris_amd.autotune import candidate_tiles
+
+    st = _spun_up(synth.config0(), cuda, n_spin=5)
+    g = st.to(cuda)
+    assert K.overlaps_periodic(g)
+    NX = st.shape[0]
+    for name, col, want in (("sigma1", 0, True), ("sigma1", 1, False), ("sigma12", NX - 2, False),
+                            ("sigma12", 1, True), ("uOcean", NX - 1, False), ("uIce", 0, False)):
+        keep = g.variables[name]
+        t = keep.clone()
+        t[col, 5] += 0.125
+        g.variables[name] = t
+        assert K.overlaps_periodic(g) == want, (name, col)
+        g.variables[name] = keep
+    assert K.overlaps_periodic(g)
+    # a current that is not periodic in the overlap: k > 1 shapes would read the
+    # interior partner where the reference reads the overlap value
+    uoc = g.variables.uOcean.clone()
+    uoc[1, :] += 0.01
+    g.variables.uOcean = uoc
+    st.variables.uOcean = g2n(uoc)
+    assert not K.overlaps_periodic(g)
+    assert all(k == 1 for t, (_, _, k) in N_shapes().items()
+               if t in candidate_tiles(False, st.settings, K.overlaps_periodic(g)))
+    want = O.evp_solver(st, n_substeps=4)
+    got = K.evp_solver(g, n_substeps=4, tile=13)
+    assert np.count_nonzero(g2n(got[0]) != want[0]) > 0
+    # without a filled overlap the check always passes
+    r = _spun_up(synth.regional(9, 0, 45, 61), cuda, n_spin=3)
+    assert K.overlaps_periodic(r.to(cuda))
+
+
+def N_shapes():
+    from veris_amd._native import evp_tile_shapes
+
+    return evp_tile_shapes()
+
+
 @pytest.mark.parametrize("tile", ALL_TILES)
 def test_plain_instantiation_equals_general(cuda, tile, monkeypatch):
     """Under the default settings the subcycle launches the kernel

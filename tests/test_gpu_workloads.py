@@ -234,11 +234,11 @@ def test_adjoint_config3_against_torch_autograd_and_fd(cuda):
         return {"pStar": mk(pstar), "hIceMean": mk(np.asarray(st.variables.hIceMean)),
                 "cw": mk(s.waterIceDrag), "ca": mk(s.airIceDrag)}
 
-    def gpu_loss(p):
+    def gpu_loss(p, tile=0):
         outs = A.evp_subcycle(g, pStar=p["pStar"], hIceMean=p["hIceMean"],
                               waterIceDrag=p["cw"], waterIceDrag_south=p["cw"],
                               airIceDrag=p["ca"], airIceDrag_south=p["ca"], n_substeps=n,
-                              ckpt_every=10)
+                              ckpt_every=10, tile=tile)
         return (((outs[0] - tgt[0])[2:-2, 2:-2] ** 2).sum()
                 + ((outs[1] - tgt[1])[2:-2, 2:-2] ** 2).sum())
 
@@ -246,6 +246,14 @@ def test_adjoint_config3_against_torch_autograd_and_fd(cuda):
     pg = leaves(cuda)
     gpu_loss(pg).backward()
     got = {k: g2n(v.grad) for k, v in pg.items()}
+    # the timed path: the inputs are periodic in the x overlap, so the forward
+    # and the recompute may run two substeps per launch, which changes no bit
+    tile = A.pair_tile(g, pStar=torch.from_numpy(pstar).to(cuda))
+    assert tile > 0
+    pp = leaves(cuda)
+    gpu_loss(pp, tile).backward()
+    for k, v in pp.items():
+        assert np.array_equal(g2n(v.grad), got[k]), k
 
     print("torch restatement, autograd on the CPU", flush=True)
     vs = {k: torch.from_numpy(np.asarray(v, dtype=np.float64)) for k, v in st.variables.items()}

@@ -131,6 +131,7 @@ SIGNATURES = {
     "vh_evp_tile_count": [],
     "vh_evp_tile_shape": [_I32, ctypes.POINTER(_I32), ctypes.POINTER(_I32), ctypes.POINTER(_I32)],
     "vh_evp_workspace_bytes": [_DP, ctypes.POINTER(_SZ)],
+    "vh_evp_overlaps_periodic": [_DP, _SP, _FP, _P, _P],
     "vh_evp_solver": [_DP, _SP, _FP, _I32, _P, _P, _P, _P, _P, _P, _SZ, _P],
     "vh_evp_solver_noalloc": [_DP, _SP, _FP, _I32, _P, _P, _P, _P, _P, _P],
     "vh_evp_batched_workspace_bytes": [ctypes.POINTER(Batch), ctypes.POINTER(_SZ)],

@@ -44,29 +44,69 @@ def _scalars(values):
     return out
 
 
+def _forward_fields(state, tile, pstar, h, drags):
+    """ABI arguments of the subcycle with the given parameters: a fresh Ctx of
+    ``state`` whose hIceMean, drag settings and pre-EVP fields (computed here
+    on the device) are the parameters', with the tensors behind them."""
+    c = K.Ctx(state)
+    c.dom.tile = int(tile)
+    dt, shape = c.dtype, c.shape
+    for name, val in zip(_DRAGS, _scalars(drags)):
+        setattr(c.sett, name, val)
+    h = h.detach().to(dt).contiguous()
+    c._check("hIceMean", h)
+    c.fields.hIceMean = h.data_ptr()
+    pstar_field = pstar.detach().to(device=c.device, dtype=dt).expand(shape).contiguous()
+    pre = torch.empty((len(_PRE_OUT),) + shape, dtype=dt, device=c.device)
+    mC, mU, mV, AW, AS, WX, WY, P = pre.unbind(0)
+    N.call("vh_seaice_mass", *c.args(), mC.data_ptr(), mU.data_ptr(), mV.data_ptr(), c.stream)
+    N.call("vh_area_ws", *c.args(), AW.data_ptr(), AS.data_ptr(), c.stream)
+    for name, t in (("SeaIceMassC", mC), ("SeaIceMassU", mU), ("SeaIceMassV", mV),
+                    ("AreaW", AW), ("AreaS", AS)):
+        setattr(c.fields, name, t.data_ptr())
+    N.call("vh_wind_forcing_xy", *c.args(), WX.data_ptr(), WY.data_ptr(), c.stream)
+    N.call("vh_seaice_strength", *c.args(), pstar_field.data_ptr(), P.data_ptr(), c.stream)
+    for name, t in (("WindForcingX", WX), ("WindForcingY", WY), ("SeaIceStrength", P)):
+        setattr(c.fields, name, t.data_ptr())
+    return c, h, pre, pstar_field
+
+
+def _default_settings(s) -> bool:
+    """The settings under which the subcycle takes none of its optional
+    branches (those of the library's default-settings kernels)."""
+    return (not s.useAdaptiveEVP and bool(s.noSlip) and not s.secondOrderBC
+            and float(s.sinWat) == 0.0 and float(s.basalDragK2) == 0.0
+            and float(s.tensileStrFac) == 0.0)
+
+
+def pair_tile(state, *, pStar=None, hIceMean=None) -> int:
+    """A two-substep tile shape for :func:`evp_subcycle` (``tile=``) when
+    its forward and recompute may run two substeps per launch on these
+    inputs, else 0: the default settings, no metric terms, and every field the
+    launches read at neighbouring cells periodic in the filled overlaps
+    (:func:`veris_amd.kernels.overlaps_periodic`, evaluated on the pre-EVP
+    fields these parameters give).  Waits for the device."""
+    vs, s = state.variables, state.settings
+    if not _default_settings(s) or any(
+            vs.get(k) is not None and not K._all_zero(vs.get(k)) for k in K._KTERMS):
+        return 0
+    pstar = _as_tensor(s.pStar if pStar is None else pStar)
+    h = vs.hIceMean if hIceMean is None else hIceMean
+    c, *keep = _forward_fields(state, 0, pstar, h, [float(s[k]) for k in _DRAGS])
+    flag = torch.zeros((), dtype=torch.int32, device=c.device)
+    N.call("vh_evp_overlaps_periodic", *c.args(), flag.data_ptr(), c.stream)
+    if not flag.item():
+        return 0
+    shapes = N.evp_tile_shapes()
+    want = (20, 32, 2) if c.shape[0] * c.shape[1] < 400_000 else (32, 32, 2)
+    return next(t for t, sh in shapes.items() if sh == want)
+
+
 class _EvpSubcycle(torch.autograd.Function):
     @staticmethod
     def forward(ctx, state, n, ckpt_every, tile, pstar, h, cwN, cwS, caN, caS):
-        c = K.Ctx(state)
-        c.dom.tile = int(tile)
+        c, h, pre, pstar_field = _forward_fields(state, tile, pstar, h, [cwN, cwS, caN, caS])
         dt, shape = c.dtype, c.shape
-        for name, val in zip(_DRAGS, _scalars([cwN, cwS, caN, caS])):
-            setattr(c.sett, name, val)
-        h = h.detach().to(dt).contiguous()
-        c._check("hIceMean", h)
-        c.fields.hIceMean = h.data_ptr()
-        pstar_field = pstar.detach().to(device=c.device, dtype=dt).expand(shape).contiguous()
-        pre = torch.empty((len(_PRE_OUT),) + shape, dtype=dt, device=c.device)
-        mC, mU, mV, AW, AS, WX, WY, P = pre.unbind(0)
-        N.call("vh_seaice_mass", *c.args(), mC.data_ptr(), mU.data_ptr(), mV.data_ptr(), c.stream)
-        N.call("vh_area_ws", *c.args(), AW.data_ptr(), AS.data_ptr(), c.stream)
-        for name, t in (("SeaIceMassC", mC), ("SeaIceMassU", mU), ("SeaIceMassV", mV),
-                        ("AreaW", AW), ("AreaS", AS)):
-            setattr(c.fields, name, t.data_ptr())
-        N.call("vh_wind_forcing_xy", *c.args(), WX.data_ptr(), WY.data_ptr(), c.stream)
-        N.call("vh_seaice_strength", *c.args(), pstar_field.data_ptr(), P.data_ptr(), c.stream)
-        for name, t in (("WindForcingX", WX), ("WindForcingY", WY), ("SeaIceStrength", P)):
-            setattr(c.fields, name, t.data_ptr())
         nbytes = ctypes.c_size_t(0)
         N.call("vh_evp_adjoint_workspace_bytes", ctypes.byref(c.dom), n, ckpt_every,
                ctypes.byref(nbytes))
@@ -156,6 +196,14 @@ def evp_subcycle(state, *, pStar=None, hIceMean=None, waterIceDrag=None,
     turning angle (sinWat, cosWat) and the metric terms (k1AtC, k2AtC, k1AtZ,
     k2AtZ); the pre-EVP part takes the sea-ice load in the surface tilt
     (useRealFreshWaterFlux) as a state input, as the reference's kernel reads it.
+
+    ``tile`` (0: the library's choice) names the forward's workgroup shape.
+    A two-substep shape (k = 2, e.g. from :func:`pair_tile`) runs the
+    checkpointed forward and the recompute two substeps per launch; the
+    caller states with it that the inputs are periodic in their filled
+    overlaps (what :func:`pair_tile` checks), under which the gradients and
+    velocities are those of one substep per launch bit for bit.  With tile 0
+    the library does so by itself on domains without a filled overlap.
     """
     vs, s = state.variables, state.settings
     n = int(s.nEVPsteps if n_substeps is None else n_substeps)

@@ -265,6 +265,109 @@ int run_evp_res(const Geom& g, const vh_settings* s, const vh_fields* fl, int ns
 }  // namespace
 }  // namespace vh
 
+namespace vh {
+namespace {
+
+// Fields a multi-substep launch reads at neighbouring cells: every cell of a
+// filled overlap must hold what the fill copies there.
+constexpr int kPeriodicFull = 30;
+template <typename T>
+struct PeriodicCheck {
+  const T* full[kPeriodicFull];  // null entries are skipped
+  const T *s1, *s2, *s12;        // stresses: only the overlap cells that reach the interior
+  int* flag;
+};
+
+__device__ inline bool same_bits(double a, double b) {
+  return __double_as_longlong(a) == __double_as_longlong(b);
+}
+__device__ inline bool same_bits(float a, float b) { return __float_as_uint(a) == __float_as_uint(b); }
+
+template <typename T>
+__global__ __launch_bounds__(256) void overlaps_init_kernel(int* flag) {
+  if (threadIdx.x == 0) *flag = 1;
+}
+
+// One thread per padded cell; cells the fill overwrites compare each field
+// with the cell the fill copies from (x first, then y, as the fill does) and
+// clear the flag on any difference in any bit.
+template <typename T>
+__global__ __launch_bounds__(256) void overlaps_periodic_kernel(Geom g, PeriodicCheck<T> c) {
+  const size_t e = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (e >= (size_t)g.NX * g.NY) return;
+  const int i = (int)(e / g.NY), j = (int)(e - (size_t)i * g.NY);
+  if (is_computed(g, i, j)) return;
+  int si = i, sj = j;
+  if (g.fill_x) si = i < g.ol ? i + g.nx : (i >= g.nx + g.ol ? i - g.nx : i);
+  if (g.fill_y) sj = j < g.ol ? j + g.ny : (j >= g.ny + g.ol ? j - g.ny : j);
+  const size_t src = (size_t)si * g.NY + sj;
+  bool ok = true;
+#pragma unroll 1
+  for (int q = 0; q < kPeriodicFull; ++q)
+    if (c.full[q] && !same_bits(c.full[q][e], c.full[q][src])) ok = false;
+  // sigma1, sigma2 at c-points of overlap column (row) 1 and sigma12 at z-points
+  // of column (row) NX-2 (NY-2) enter the interior's next substep; the other
+  // overlap stresses are wrap-around values nothing interior reads
+  const bool c1 = (g.fill_x && i == 1) || (g.fill_y && j == 1);
+  const bool z2 = (g.fill_x && i == g.NX - 2) || (g.fill_y && j == g.NY - 2);
+  if (c1 && (!same_bits(c.s1[e], c.s1[src]) || !same_bits(c.s2[e], c.s2[src]))) ok = false;
+  if (z2 && !same_bits(c.s12[e], c.s12[src])) ok = false;
+  if (!ok) *c.flag = 0;
+}
+
+template <typename T>
+int overlaps_periodic(const Geom& g, const vh_settings* s, const vh_fields* f, int* flag,
+                      hipStream_t st) {
+  PeriodicCheck<T> c{};
+  const void* full[] = {f->uIce,     f->vIce,      f->uOcean,    f->vOcean,    f->iceMask,
+                        f->iceMaskU, f->iceMaskV,  f->maskInC,   f->maskInU,   f->maskInV,
+                        f->recip_dyU, f->recip_dxV, f->rAz,      f->recip_dxU, f->recip_dyV,
+                        f->recip_rA, f->dyV,       f->dxU,       f->dxV,       f->dyU,
+                        f->SeaIceStrength, f->fCori, f->SeaIceMassC, f->k1AtC, f->k2AtC,
+                        f->k1AtZ,    f->k2AtZ,     nullptr,      nullptr,      nullptr};
+  static_assert(sizeof(full) / sizeof(full[0]) == kPeriodicFull, "field list");
+  for (int q = 0; q < kPeriodicFull; ++q) c.full[q] = (const T*)full[q];
+  if (s->basalDragK2 != 0.0) {  // the basal drag's inputs at the c-points
+    c.full[27] = (const T*)f->R_low;
+    c.full[28] = (const T*)f->Area;
+    c.full[29] = (const T*)f->hIceMean;
+  }
+  const void* req[] = {f->uIce, f->vIce, f->sigma1, f->sigma2, f->sigma12, f->uOcean,
+                       f->vOcean, f->iceMask, f->iceMaskU, f->iceMaskV, f->maskInC, f->maskInU,
+                       f->maskInV, f->recip_dyU, f->recip_dxV, f->rAz, f->recip_dxU,
+                       f->recip_dyV, f->recip_rA, f->dyV, f->dxU, f->dxV, f->dyU,
+                       f->SeaIceStrength, f->fCori, f->SeaIceMassC};
+  for (const void* p : req) VH_REQUIRE(p != nullptr, VH_EINVAL, "overlaps_periodic: missing field");
+  if (s->basalDragK2 != 0.0)
+    VH_REQUIRE(f->R_low && f->Area && f->hIceMean, VH_EINVAL,
+               "overlaps_periodic: basal drag needs R_low, Area, hIceMean");
+  c.s1 = (const T*)f->sigma1;
+  c.s2 = (const T*)f->sigma2;
+  c.s12 = (const T*)f->sigma12;
+  c.flag = flag;
+  hipLaunchKernelGGL(overlaps_init_kernel<T>, dim3(1), dim3(64), 0, st, flag);
+  if (g.fill_x || g.fill_y)
+    hipLaunchKernelGGL(overlaps_periodic_kernel<T>, dim3((unsigned)((field_elems(g) + 255) / 256)),
+                       dim3(256), 0, st, g, c);
+  VH_TRY_HIP(hipGetLastError());
+  return VH_OK;
+}
+
+}  // namespace
+}  // namespace vh
+
+extern "C" int vh_evp_overlaps_periodic(const vh_domain* d, const vh_settings* s,
+                                        const vh_fields* f, int32_t* flag, void* stream) {
+  vh::Geom g;
+  int rc = vh::make_geom(d, &g);
+  if (rc) return rc;
+  VH_REQUIRE(s != nullptr && f != nullptr && flag != nullptr, VH_EINVAL,
+             "null settings, fields or flag");
+  hipStream_t st = (hipStream_t)stream;
+  if (d->dtype == VH_F64) return vh::overlaps_periodic<double>(g, s, f, (int*)flag, st);
+  return vh::overlaps_periodic<float>(g, s, f, (int*)flag, st);
+}
+
 extern "C" int vh_evp_workspace_bytes(const vh_domain* d, size_t* bytes) {
   vh::Geom g;
   int rc = vh::make_geom(d, &g);

@@ -65,6 +65,7 @@ struct BlockPlan {
   char* ws;
   size_t fb, off_ckpt, off_pp, off_seg, seg_stride;
   int n, ce, nck;
+  int kstep;  // substeps per forward launch (1, or 2: ce and n even)
 };
 
 #define CELL_SETUP_G                                                                \
@@ -89,13 +90,27 @@ __device__ inline T* slot(const BlockPlan<T>& p, size_t off, int idx) {
   return (T*)(p.ws + off + (size_t)idx * p.fb);
 }
 
-// forward location of field q of the state after k substeps
+// Segment slots of segment sgm: the last segment's states, kept from the
+// checkpointed forward for every reverse sweep, have a set of their own; the
+// other segments alternate between two sets (one is recomputed while the
+// other is reversed).
+template <typename T>
+__device__ inline size_t seg_set(const BlockPlan<T>& p, int sgm) {
+  return p.off_seg + (size_t)(sgm == p.nck - 1 ? 2 : sgm % 2) * p.seg_stride;
+}
+
+// forward location of field q of the state after k substeps: the states
+// inside the last segment go to that segment's slots, where the reverse
+// sweep reads them without recomputing them; between earlier checkpoints the
+// states (k a multiple of p.kstep) alternate between two slots
 template <typename T>
 __device__ inline T* fwd_loc(const BlockPlan<T>& p, int k, int q) {
   if (k == 0) return (T*)p.in0[q];
   if (k == p.n) return p.outs[q];
   if (k % p.ce == 0) return slot(p, p.off_ckpt, 5 * (k / p.ce) + q);
-  return slot(p, p.off_pp, 5 * ((k - 1) % 2) + q);
+  const int last = p.nck - 1;
+  if (k / p.ce == last) return slot(p, seg_set(p, last), 5 * (k - last * p.ce) + q);
+  return slot(p, p.off_pp, 5 * (((k - 1) / p.kstep) % 2) + q);
 }
 
 // mode 0: forward blocks (substep k: state k -> state k+1);
@@ -103,7 +118,12 @@ __device__ inline T* fwd_loc(const BlockPlan<T>& p, int k, int q) {
 // sgm reads state k from the checkpoint (first substep) or a segment slot and
 // names state k+1 as its output: a segment slot, or for the segment's last
 // substep the next checkpoint (slot nck holds a copy of the final state),
-// which the reverse sweep reads and never recomputes.
+// which the reverse sweep reads and never recomputes;
+// mode 2: two-substep forward blocks (block m: state 2m -> state 2m+2);
+// mode 3: two-substep recompute blocks, at every even substep k of a segment
+// short of its last: state k -> state k+2 with state k+1 kept (mid), all in
+// the segment's slots (state k+2 goes to the spare slot len when it is the
+// next checkpoint, which the reverse sweep reads from its own slot).
 template <typename T>
 __global__ void adj_write_blocks_kernel(EvpArgBlock<T>* blocks, size_t stride, EvpArgBlock<T> proto,
                                         BlockPlan<T> p, int mode, AdjArgs<T>* aa, AdjArgs<T> av,
@@ -119,10 +139,30 @@ __global__ void adj_write_blocks_kernel(EvpArgBlock<T>* blocks, size_t stride, E
       b.in[q] = fwd_loc(p, k, q);
       b.out[q] = fwd_loc(p, k + 1, q);
     }
+  } else if (mode == 2) {
+    if (2 * k >= p.n) return;
+    // the intermediate state is kept where the last segment's states go
+    const bool keep_mid = (2 * k + 1) / p.ce == p.nck - 1;
+    for (int q = 0; q < 5; ++q) {
+      b.in[q] = fwd_loc(p, 2 * k, q);
+      b.mid[q] = keep_mid ? fwd_loc(p, 2 * k + 1, q) : nullptr;
+      b.out[q] = fwd_loc(p, 2 * k + 2, q);
+    }
+  } else if (mode == 3) {
+    const int jj = k % p.ce, sgm = k / p.ce;
+    const int len = min(p.ce, p.n - sgm * p.ce);
+    if (jj % 2 != 0 || jj + 1 > len - 1) return;
+    const size_t seg = seg_set(p, sgm);
+    for (int q = 0; q < 5; ++q) {
+      b.in[q] = jj == 0 ? (k == 0 ? p.in0[q] : slot(p, p.off_ckpt, 5 * sgm + q))
+                        : slot(p, seg, 5 * jj + q);
+      b.mid[q] = slot(p, seg, 5 * (jj + 1) + q);
+      b.out[q] = slot(p, seg, 5 * (jj + 2) + q);
+    }
   } else {
     const int jj = k % p.ce, sgm = k / p.ce;
     const int len = min(p.ce, p.n - sgm * p.ce);
-    const size_t seg = p.off_seg + (size_t)(sgm % 2) * p.seg_stride;  // two slot sets
+    const size_t seg = seg_set(p, sgm);
     for (int q = 0; q < 5; ++q) {
       b.in[q] = jj == 0 ? (k == 0 ? p.in0[q] : slot(p, p.off_ckpt, 5 * sgm + q))
                         : slot(p, seg, 5 * jj + q);
@@ -284,547 +324,554 @@ __global__ __launch_bounds__(RI* RJ) void adj_substep_kernel(const AdjArgs<T>* _
   //      constants of the forward launches (uniform flag, set when every mask
   //      is 0/1 and every metric constant along i), the masks come from one
   //      byte and the metrics from rows in LDS, as in the forward kernel.
-  const bool packed = blk->packed != 0;
-  constexpr int RPT = (kEvpRowFields * RJ + NT - 1) / NT;  // rows per thread
-  __shared__ T sRow[kEvpRowFields * RJ];
-  T rowv[RPT];
-  if (packed) {
-    const int roff = (int)blockIdx.y * g.NY;  // rows of member blockIdx.y
-#pragma unroll
-    for (int r = 0; r < RPT; ++r) {
-      const int q = t + r * NT;
-      rowv[r] = (T)0;
-      if (q < kEvpRowFields * RJ) {
-        const int mq = q / RJ, bq = q & (RJ - 1);
-        rowv[r] = ldg(blk->rows + (long long)mq * blk->rstride + roff,
-                      (unsigned)wrap(j0 + bq - R, g.NY));
+  // the body is instantiated twice, for packed step constants and for full
+  // fields (pk: std::true_type / std::false_type), each with its own schedule
+  // and registers; the packed flag of the forward launches picks one
+  auto body = [&](auto pk) {
+    constexpr bool packed = decltype(pk)::value;
+    constexpr int RPT = (kEvpRowFields * RJ + NT - 1) / NT;  // rows per thread
+    __shared__ T sRow[kEvpRowFields * RJ];
+    T rowv[RPT];
+    if (packed) {
+      const int roff = (int)blockIdx.y * g.NY;  // rows of member blockIdx.y
+  #pragma unroll
+      for (int r = 0; r < RPT; ++r) {
+        const int q = t + r * NT;
+        rowv[r] = (T)0;
+        if (q < kEvpRowFields * RJ) {
+          const int mq = q / RJ, bq = q & (RJ - 1);
+          rowv[r] = ldg(blk->rows + (long long)mq * blk->rstride + roff,
+                        (unsigned)wrap(j0 + bq - R, g.NY));
+        }
       }
     }
-  }
-  const T* xu = blk->in[0];
-  const T* xv = blk->in[1];
-  const T u = ldg(xu, c), v = ldg(xv, c);
-  T Mc, MUc, MVc, mInC, mInUc, mInVc;
-  if (packed) {
-    const unsigned pkb = ldg(blk->pk, c);
-    Mc = (T)(pkb & 1u);
-    MUc = (T)((pkb >> 1) & 1u);
-    MVc = (T)((pkb >> 2) & 1u);
-    mInC = (T)((pkb >> 3) & 1u);
-    mInUc = (T)((pkb >> 4) & 1u);
-    mInVc = (T)((pkb >> 5) & 1u);
-  } else {
-    Mc = ldg(k.M, c);
-    MUc = ldg(k.MU, c);
-    MVc = ldg(k.MV, c);
-    mInC = ldg(k.mInC, c);
-    mInUc = ldg(k.mInU, c);
-    mInVc = ldg(k.mInV, c);
-  }
-  const T uoc = ldg(k.uOc, c), voc = ldg(k.vOc, c);
-  // metrics from the full fields (not packed); met(q, .) picks the source
-  T g_met[kEvpRowFields] = {};
-  if (!packed) {
-    const T* const mp[kEvpRowFields] = {k.rdyU, k.rdxV, k.rAz, k.rdxU, k.rdyV, k.rrA,
-                                        k.dyV,  k.dxU,  k.dxV, k.dyU,  k.rrAu, k.rrAv};
-#pragma unroll
-    for (int q = 0; q < kEvpRowFields; ++q) g_met[q] = ldg(mp[q], c);
-  }
-  auto met = [&](int q) -> T { return packed ? sRow[q * RJ + br] : g_met[q]; };
-  const T P = ldg(k.P, c), fc = ldg(k.fC, c);
-  const T un = ldg(blk->out[0], c), vn = ldg(blk->out[1], c);
-  const T mU = ldg(k.massU, c), mV = ldg(k.massV, c), AW = ldg(k.AreaW, c), AS = ldg(k.AreaS, c);
-  const T mC = ldg(k.massC, c);
-  const bool sdr = XB && sc.sidedrag;  // uniform; false in the default instantiation
-  const bool bas = XB && sc.basal;
-  const bool ada = XB && sc.adaptive;
-  T so1 = 0, so2 = 0, so12 = 0;  // input stresses (adaptive: the alpha derivatives)
-  if (ada) {
-    so1 = ldg(blk->in[2], c);
-    so2 = ldg(blk->in[3], c);
-    so12 = ldg(blk->in[4], c);
-  }
-  T FuC = 0, FvC = 0;  // coastline form factors
-  if (sdr && sc.coast) {
-    FuC = ldg(k.Fu, c);
-    FvC = ldg(k.Fv, c);
-  }
-  T bAr = 0, bRl = 0, bh = 0, bmUip = 0, bmVjp = 0;  // basal drag inputs
-  if (bas) {
-    bAr = ldg(k.Area, c);
-    bRl = ldg(k.Rlow, c);
-    bh = ldg(k.hIce, c);
-    bmUip = ldg(k.mInU, cip);
-    bmVjp = ldg(k.mInV, cjp);
-  }
-  const T l1 = ldg(L.in[2], c), l2 = ldg(L.in[3], c), l12 = ldg(L.in[4], c);
-  const T lu = gather_filled(g, L.in[0] + mo, gi, gj), lv = gather_filled(g, L.in[1] + mo, gi, gj);
-  // water drag factor of either hemisphere, selected by value (a select of
-  // the two setting addresses would become a per-lane load)
-  const T Dn = (T)sc.cwN * (T)sc.rhoSea, Ds = (T)sc.cwS * (T)sc.rhoSea;
-  const T D = fc < (T)0 ? Ds : Dn;
+    const T* xu = blk->in[0];
+    const T* xv = blk->in[1];
+    const T u = ldg(xu, c), v = ldg(xv, c);
+    T Mc, MUc, MVc, mInC, mInUc, mInVc;
+    if (packed) {
+      const unsigned pkb = ldg(blk->pk, c);
+      Mc = (T)(pkb & 1u);
+      MUc = (T)((pkb >> 1) & 1u);
+      MVc = (T)((pkb >> 2) & 1u);
+      mInC = (T)((pkb >> 3) & 1u);
+      mInUc = (T)((pkb >> 4) & 1u);
+      mInVc = (T)((pkb >> 5) & 1u);
+    } else {
+      Mc = ldg(k.M, c);
+      MUc = ldg(k.MU, c);
+      MVc = ldg(k.MV, c);
+      mInC = ldg(k.mInC, c);
+      mInUc = ldg(k.mInU, c);
+      mInVc = ldg(k.mInV, c);
+    }
+    const T uoc = ldg(k.uOc, c), voc = ldg(k.vOc, c);
+    // metrics from the full fields (not packed); met(q, .) picks the source
+    T g_met[kEvpRowFields] = {};
+    if (!packed) {
+      const T* const mp[kEvpRowFields] = {k.rdyU, k.rdxV, k.rAz, k.rdxU, k.rdyV, k.rrA,
+                                          k.dyV,  k.dxU,  k.dxV, k.dyU,  k.rrAu, k.rrAv};
+  #pragma unroll
+      for (int q = 0; q < kEvpRowFields; ++q) g_met[q] = ldg(mp[q], c);
+    }
+    auto met = [&](int q) -> T { return packed ? sRow[q * RJ + br] : g_met[q]; };
+    const T P = ldg(k.P, c), fc = ldg(k.fC, c);
+    const T un = ldg(blk->out[0], c), vn = ldg(blk->out[1], c);
+    const T mU = ldg(k.massU, c), mV = ldg(k.massV, c), AW = ldg(k.AreaW, c), AS = ldg(k.AreaS, c);
+    const T mC = ldg(k.massC, c);
+    const bool sdr = XB && sc.sidedrag;  // uniform; false in the default instantiation
+    const bool bas = XB && sc.basal;
+    const bool ada = XB && sc.adaptive;
+    T so1 = 0, so2 = 0, so12 = 0;  // input stresses (adaptive: the alpha derivatives)
+    if (ada) {
+      so1 = ldg(blk->in[2], c);
+      so2 = ldg(blk->in[3], c);
+      so12 = ldg(blk->in[4], c);
+    }
+    T FuC = 0, FvC = 0;  // coastline form factors
+    if (sdr && sc.coast) {
+      FuC = ldg(k.Fu, c);
+      FvC = ldg(k.Fv, c);
+    }
+    T bAr = 0, bRl = 0, bh = 0, bmUip = 0, bmVjp = 0;  // basal drag inputs
+    if (bas) {
+      bAr = ldg(k.Area, c);
+      bRl = ldg(k.Rlow, c);
+      bh = ldg(k.hIce, c);
+      bmUip = ldg(k.mInU, cip);
+      bmVjp = ldg(k.mInV, cjp);
+    }
+    const T l1 = ldg(L.in[2], c), l2 = ldg(L.in[3], c), l12 = ldg(L.in[4], c);
+    const T lu = gather_filled(g, L.in[0] + mo, gi, gj), lv = gather_filled(g, L.in[1] + mo, gi, gj);
+    // water drag factor of either hemisphere, selected by value (a select of
+    // the two setting addresses would become a per-lane load)
+    const T Dn = (T)sc.cwN * (T)sc.rhoSea, Ds = (T)sc.cwS * (T)sc.rhoSea;
+    const T D = fc < (T)0 ? Ds : Dn;
 
-  if (packed) {
-#pragma unroll
-    for (int r = 0; r < RPT; ++r)
-      if (t + r * NT < kEvpRowFields * RJ) sRow[t + r * NT] = rowv[r];
-  }
-  sU[t] = u;
-  sV[t] = v;
-  sM[t] = Mc;
-  sMU[t] = MUc;
-  sMV[t] = MVc;
-  sA[t] = (u - uoc) * mInUc;
-  sB[t] = (v - voc) * mInVc;
-  if (sc.turn) {
-    sFU[t] = uoc - u;
-    sFV[t] = voc - v;
-  }
-  __syncthreads();
+    if (packed) {
+  #pragma unroll
+      for (int r = 0; r < RPT; ++r)
+        if (t + r * NT < kEvpRowFields * RJ) sRow[t + r * NT] = rowv[r];
+    }
+    sU[t] = u;
+    sV[t] = v;
+    sM[t] = Mc;
+    sMU[t] = MUc;
+    sMV[t] = MVc;
+    sA[t] = (u - uoc) * mInUc;
+    sB[t] = (v - voc) * mInVc;
+    if (sc.turn) {
+      sFU[t] = uoc - u;
+      sFV[t] = voc - v;
+    }
+    __syncthreads();
 
-  // ---- B: e12 at z-points (the forward kernel's expression order)
-  T e12r = 0;
-  if (ar >= 1 && br >= 1) {
+    // ---- B: e12 at z-points (the forward kernel's expression order)
+    T e12r = 0;
+    if (ar >= 1 && br >= 1) {
+      const T rdyU = met(0), rdxV = met(1), rAz = met(2);
+      const T ujm = sU[t - 1], vim = sV[t - RJ];
+      const T dudy = (u - ujm) * rdyU;
+      const T uavz = (u + ujm) * (T)0.5;
+      const T dvdx = (v - vim) * rdxV;
+      const T vavz = (v + vim) * (T)0.5;
+      const T mz = (Mc * sM[t - RJ]) * (sM[t - 1] * sM[t - RJ - 1]);
+      T inner = dudy + dvdx;
+      if (sc.kterms) inner = (inner - ldg(k.k1Z, c) * vavz) - ldg(k.k2Z, c) * uavz;
+      T e12 = ((T)0.5 * inner) * mz;
+      if (!sdr) {  // no-slip boundary term
+        const T hu = MUc - sMU[t - 1];
+        const T hv = MVc - sMV[t - RJ];
+        e12 = e12 + ((((T)2.0 * uavz) * rdyU) * hu + (((T)2.0 * vavz) * rdxV) * hv);
+      }
+      if (SO && sc.sobc) {
+        // second-order boundary shear, read from memory as the forward kernel
+        // reads it (two cells along the face normal reach past this region's edge)
+        const unsigned cim2 = mo + wrap(i - 2, g.NX) * g.NY + gj, cim1 = mo + wrap(i - 1, g.NX) * g.NY + gj;
+        const unsigned cjm2 = mo + gi * g.NY + wrap(j - 2, g.NY), cjm1 = mo + gi * g.NY + wrap(j - 1, g.NY);
+        const T uim2 = ldg(xu, cim2), uip1 = ldg(xu, cip);
+        const T MUim2 = ldg(k.MU, cim2), MUim1 = ldg(k.MU, cim1), MUip1 = ldg(k.MU, cip);
+        const T vjm2 = ldg(xv, cjm2), vjp1 = ldg(xv, cjp);
+        const T MVjm2 = ldg(k.MV, cjm2), MVjm1 = ldg(k.MV, cjm1), MVjp1 = ldg(k.MV, cjp);
+        T hu2 = (MUc - MUim1) / (T)3.0;
+        T hv2 = (MVc - MVjm1) / (T)3.0;
+        hu2 = hu2 * (MUim2 * MUim1 + MUip1 * MUc);
+        hv2 = hv2 * (MVjm2 * MVjm1 + MVjp1 * MVc);
+        const T tu = (rdyU * ((((T)6.0 * uavz) - uim2 * MUim1) - uip1 * MUc)) * hu2;
+        const T tv = (rdxV * ((((T)6.0 * vavz) - vjm2 * MVjm1) - vjp1 * MVc)) * hv2;
+        e12 = e12 + (T)0.5 * (tu + tv);
+      }
+      sE12[t] = e12;
+      sQ[t] = rAz * (e12 * e12);
+      e12r = e12;
+    }
+    __syncthreads();
+
+    // ---- C: zeta, Delta, strain invariants, water drag at c-points
+    T zeta = 0, dC = 0, ep = 0, em = 0, sa = 0, sb = 0, ww = 0, sw = 0;
+    T su = 0, sv = 0, spd = 0;  // lateral drag: u + u(i+1), v + v(j+1) and the speed at c
+    T cTB = 0, cH = 0;          // basal drag: its derivatives at c
+    T aC = 0, cZa = 0, cMa = 0, prs = 0;  // adaptive: alpha at c, d alpha / d (zeta, mC), press
+    if (ar >= 1 && br >= 1 && ar <= RI - 2 && br <= RJ - 2) {
+      const T uip = sU[t + RJ], vjp = sV[t + 1];
+      const T rdxU = met(3), rdyV = met(4);
+      const T dudx = (uip - u) * rdxU, dvdy = (vjp - v) * rdyV;
+      T e11 = dudx, e22 = dvdy;
+      if (sc.kterms) {
+        e11 = dudx + ((vjp + v) * (T)0.5) * ldg(k.k2C, c);
+        e22 = dvdy + ((uip + u) * (T)0.5) * ldg(k.k1C, c);
+      }
+      e11 = e11 * mInC;
+      e22 = e22 * mInC;
+      const T qa = sQ[t] + sQ[t + RJ];
+      const T qb = sQ[t + 1] + sQ[t + RJ + 1];
+      const T e12c = ((T)0.25 * met(5)) * (qa + qb);
+      if (sdr) {  // ice speed at c (veris/dynamics_routines.py:109-111)
+        su = u + uip;
+        sv = v + vjp;
+        spd = (T)0.5 * sqrt(su * su + sv * sv);
+        sSPD[t] = spd;
+      }
+      if (bas) {  // basal drag coefficient at c, the forward kernel's expression order
+        const T um = u * mInUc, umip = uip * bmUip;
+        const T vm = v * mInVc, vmjp = vjp * bmVjp;
+        const T tbs = (T)0.25 * (((um * um + umip * umip) + vm * vm) + vmjp * vmjp) + (T)sc.U0sq;
+        const T tb = (T)sc.K2 / sqrt(tbs);
+        const T hc = (fabs(bRl) * bAr) / (T)sc.K1;
+        const T ex = exp((T)10.0 * (bh - hc));
+        const T lg = log(ex + (T)1.0);
+        const T eb = exp((T)(-sc.cBasalStar) * ((T)1.0 - bAr));
+        const T val = ((tb * lg) * (T)0.1) * eb;
+        const bool on = bAr > (T)0.01;
+        sCBc[t] = on ? val : (T)0;
+        // d cb / d tb-argument and d cb / d h (the soft maximum's logistic slope)
+        cTB = on ? ((T)-0.5 * (((tb * lg) * (T)0.1) * eb)) / tbs : (T)0;
+        cH = on ? (tb * eb) * (ex / (ex + (T)1.0)) : (T)0;
+      }
+      ep = e11 + e22;
+      em = e11 - e22;
+      const TD dsq = (TD)ep * (TD)ep + (TD)sc.rpd * ((TD)em * (TD)em + (TD)4.0 * (TD)e12c);
+      const TD dCd = sqrt(dsq);
+      const TD zd = ((TD)0.5 * ((TD)P * (TD)sc.tens1p)) / (dCd + (TD)sc.deltaMin);
+      zeta = (T)zd;
+      dC = (T)dCd;
+      sZ[t] = zeta;
+      if (ada) {  // alpha at c (the forward kernel's expression order) and its derivatives
+        const T mmax = fmax(mC, (T)1e-4);
+        const T Y = ((zeta * (T)sc.evpcfac) / mmax) * met(5);
+        const T sY = sqrt(Y);
+        const T raw = sY * Mc;
+        aC = fmax(raw, (T)sc.alphaMin);
+        sAl[t] = aC;
+        const bool act = raw > (T)sc.alphaMin && sY > (T)0;
+        const T dY = act ? (Mc * (T)0.5) / sY : (T)0;  // d alpha / d Y
+        cZa = dY * (((T)sc.evpcfac / mmax) * met(5));
+        cMa = mC > (T)1e-4 ? -dY * (Y / mmax) : (T)0;
+        // the replacement pressure, for the sigma1 relaxation's alpha derivative
+        T prep = (((T)2.0 * zeta) * dC) * (T)sc.pr;
+        if (!sc.tens0) prep = prep / (T)sc.tens1p;
+        prs = (P * (T)sc.pr1m) + prep;
+        if (!sc.tens0) prs = prs * (T)sc.tens1m;
+      }
+      sa = sA[t] + sA[t + RJ];
+      sb = sB[t] + sB[t + 1];
+      ww = (T)0.25 * (sa * sa + sb * sb);
+      sw = sqrt(ww);
+      const T cd = ((D * D) * ww > (T)sc.cDragMin2) ? D * sw : (T)sc.cDragMin;
+      sCD[t] = cd * Mc;
+      if (sc.turn) {  // (uOcean - u) summed over j, j+1 and (vOcean - v) over i, i+1
+        sAB[t] = sFU[t] + sFU[t + 1];
+        sBB[t] = sFV[t] + sFV[t + RJ];
+      }
+    }
+    __syncthreads();
+
+    // parameter adjoints of this cell, read-modify-written at the end: issued
+    // here, after the forward recompute has released its registers
+    const T aP = ldg(acc.P, c), amC = ldg(acc.mC, c), amU = ldg(acc.mU, c), amV = ldg(acc.mV, c);
+    const T aWX = ldg(acc.WFX, c), aWY = ldg(acc.WFY, c), aDN = ldg(acc.DN, c), aDS = ldg(acc.DS, c);
+    const T uNm1 = ldg(k.uNm1, c), vNm1 = ldg(k.vNm1, c);
+    // neighbours' constants used by S4
+    // (the packed rows hold these at column br + 1 and, constant along i, br)
+    T rAz_jp = 0, rdyU_jp = 0, rAz_ip = 0, rdxV_ip = 0;
+    if (!packed) {
+      rAz_jp = ldg(k.rAz, cjp);
+      rdyU_jp = ldg(k.rdyU, cjp);
+      rAz_ip = ldg(k.rAz, cip);
+      rdxV_ip = ldg(k.rdxV, cip);
+    }
+    const T fmc_im = ldg(k.fC, cim) * ldg(k.massC, cim), fmc_jm = ldg(k.fC, cjm) * ldg(k.massC, cjm);
+
+    // ---- S1: momentum update at the u- and v-point of this cell
+    const T cw = (T)sc.cosWat, bU = (T)sc.betaU, bP1 = (T)sc.betaP1, rdt = (T)sc.rdt;
+    T nbU = 0, dbU = 0, nbV = 0, dbV = 0, aAW = 0, aAS = 0, aMU = 0, aMV = 0;
+    T bUu = bU, bVv = bU;  // beta / deltatDyn at this u- and v-point (adaptive: from alpha)
+    if (ar >= 2 && br >= 2 && ar <= RI - 2 && br <= RJ - 2) {
+      T bP1u = bP1, bP1v = bP1;
+      if (ada) {  // veris/evp_solver.py:148-155
+        bUu = ((T)0.5 * (sAl[t] + sAl[t - RJ])) * rdt;
+        bVv = ((T)0.5 * (sAl[t] + sAl[t - 1])) * rdt;
+        bP1u = bUu + rdt;
+        bP1v = bVv + rdt;
+      }
+      const T cdAU = (T)0.5 * (sCD[t] + sCD[t - RJ]);
+      const T cdAV = (T)0.5 * (sCD[t] + sCD[t - 1]);
+      // basal drag at the u/v point (zero without it: the same sum as the forward's)
+      const T cbU = bas ? (T)0.5 * (sCBc[t] + sCBc[t - RJ]) : (T)0;
+      const T cbV = bas ? (T)0.5 * (sCBc[t] + sCBc[t - 1]) : (T)0;
+      const T denU = mU * bP1u + AW * ((cdAU * cw) + cbU);
+      const T denV = mV * bP1v + AS * ((cdAV * cw) + cbV);
+      const bool gU = denU == (T)0, gV = denV == (T)0;
+      T dU = gU ? (T)1 : denU, dV = gV ? (T)1 : denV;
+      // lateral drag, added after the zero guard as the forward adds it
+      T spU = 0, spV = 0, sdnU = 1, sdnV = 1, sdU = 0, sdV = 0, sdmU = 0, sdmV = 0;
+      if (sdr) {
+        spU = AW > (T)0 ? (T)0.5 * (sSPD[t] + sSPD[t - RJ]) : (T)0;
+        spV = AS > (T)0 ? (T)0.5 * (sSPD[t] + sSPD[t - 1]) : (T)0;
+        sdmU = sc.coast ? FuC : (((T)2 - sMU[t - RJ]) - sMU[t + RJ]) * MUc;
+        sdmV = sc.coast ? FvC : (((T)2 - sMV[t - 1]) - sMV[t + 1]) * MVc;
+        sdnU = spU + (T)sc.sdU0;
+        sdnV = spV + (T)sc.sdU0;
+        sdU = ((mU * (T)sc.sdCoeff) * sdmU) / sdnU;
+        sdV = ((mV * (T)sc.sdCoeff) * sdmV) / sdnV;
+        dU = dU + sdU;
+        dV = dV + sdV;
+      }
+      const T rU = (T)1 / dU, rV = (T)1 / dV;
+      const T lcu = comp ? lu : (T)0, lcv = comp ? lv : (T)0;
+      nbU = lcu * MUc * rU;
+      const T ddU = -lcu * un * rU, ddV = -lcv * vn * rV;  // adjoints of the denominators
+      dbU = gU ? (T)0 : ddU;
+      nbV = lcv * MVc * rV;
+      dbV = gV ? (T)0 : ddV;
+      sA[t] = nbU * met(10);  // X: adjoint of the u-point stress divergence (rrAu)
+      sB[t] = nbV * met(11);  // Y: of the v-point one (rrAv)
+      sQ[t] = nbU;
+      sNV[t] = nbV;
+      sCXU[t] = (nbU * uoc + dbU) * (AW * cw);
+      sCXV[t] = (nbV * voc + dbV) * (AS * cw);
+      T turnU = 0, turnV = 0;
+      if (sc.turn) {  // turning term of the forcing (veris/evp_solver.py:119-139)
+        const T sgs = sgn(fc) * (T)sc.sinWat;
+        const T lmU = mU != (T)0 ? (T)1 : mU, lmV = mV != (T)0 ? (T)1 : mV;
+        const T Xt = (sCD[t] * (T)0.5) * sAB[t] + (sCD[t - RJ] * (T)0.5) * sAB[t - RJ];
+        const T Yt = (sCD[t] * (T)0.5) * sBB[t] + (sCD[t - 1] * (T)0.5) * sBB[t - 1];
+        turnU = -((sgs * (T)0.5) * Xt) * lmU;  // d ForcingX / d AreaW, turning part
+        turnV = ((sgs * (T)0.5) * Yt) * lmV;
+        sU[t] = nbU * sgn(fc) * lmU * AW;
+        sV[t] = nbV * sgn(fc) * lmV * AS;
+      }
+      aAW = nbU * (cdAU * cw * uoc + turnU) + dbU * (cdAU * cw + cbU);  // AreaW, AreaS adjoints
+      aAS = nbV * (cdAV * cw * voc + turnV) + dbV * (cdAV * cw + cbV);
+      aMU = nbU * (bUu * u + rdt * uNm1) + dbU * bP1u;  // SeaIceMassU, SeaIceMassV adjoints
+      aMV = nbV * (bVv * v + rdt * vNm1) + dbV * bP1v;
+      if (ada) {  // beta adjoints, times the averaging weight and 1 / deltatDyn
+        sLBU[t] = ((T)0.5 * rdt) * (nbU * mU * u + dbU * mU);
+        sLBV[t] = ((T)0.5 * rdt) * (nbV * mV * v + dbV * mV);
+      }
+      if (bas) {  // the basal coefficient's share of the denominators
+        sCBU[t] = dbU * AW;
+        sCBV[t] = dbV * AS;
+      }
+      if (sdr) {  // the lateral drag's mass factor and its speed (zero where AreaW/S = 0)
+        aMU = aMU + ddU * (((T)sc.sdCoeff * sdmU) / sdnU);
+        aMV = aMV + ddV * (((T)sc.sdCoeff * sdmV) / sdnV);
+        sLSU[t] = AW > (T)0 ? (T)-0.5 * (ddU * sdU / sdnU) : (T)0;
+        sLSV[t] = AS > (T)0 ? (T)-0.5 * (ddV * sdV / sdnV) : (T)0;
+      }
+    }
+    __syncthreads();
+
+    // ---- S2: stresses, drag and Coriolis at c, sigma12 at z
+    T Padd = 0, mCadd = 0, Db = 0, zbp = 0, epb0 = 0, emb0 = 0, dbp = 0;
+    T lo1 = 0, lo2 = 0, lo12 = 0, wU = 0, wV = 0, qU = 0, qV = 0, ltb = 0, aHadd = 0, laC = 0;
+    if (ar >= 3 && br >= 3 && ar <= RI - 3 && br <= RJ - 3) {
+      if (bas) {  // basal coefficient adjoint at c from its four faces
+        const T lcb = (T)0.5 * (sCBU[t] + sCBU[t + RJ]) + (T)0.5 * (sCBV[t] + sCBV[t + 1]);
+        ltb = lcb * cTB;
+        aHadd = lcb * cH;
+      }
+      if (sdr) {  // speed adjoint at c from its four faces, then d speed / d (u, v)
+        const T lsp = (sLSU[t] + sLSU[t + RJ]) + (sLSV[t] + sLSV[t + 1]);
+        const T lq = spd > (T)0 ? lsp * (T)0.125 / spd : (T)0;  // speed = sqrt(q) / 2
+        qU = (T)2 * lq * su;
+        qV = (T)2 * lq * sv;
+      }
+      const T Xc = sA[t], Xip = sA[t + RJ], Xjm = sA[t - 1];
+      const T Yc = sB[t], Yjp = sB[t + 1], Yim = sB[t - RJ];
+      T cdb = (T)0.5 * (sCXU[t] + sCXU[t + RJ]) + (T)0.5 * (sCXV[t] + sCXV[t + 1]);
+      if (sc.turn) {  // the drag of this c-point in the turning terms of its four faces
+        // (sAB/sBB of this cell are read before this thread overwrites them)
+        const T kus = sU[t] + sU[t + RJ], kvs = sV[t] + sV[t + 1];
+        cdb = cdb + (T)0.25 * (T)sc.sinWat * (sBB[t] * kvs - sAB[t] * kus);
+        wU = sCD[t] * kus;  // and the ice velocities in them, weighted by the drag
+        wV = sCD[t] * kvs;
+      }
+      const T fvb = (T)0.5 * (sQ[t] + sQ[t + RJ]);
+      const T fub = -(T)0.5 * (sNV[t] + sNV[t + 1]);
+      // neighbour velocities (sU/sV hold the turning factors when sinWat != 0)
+      const T uip = sc.turn ? ldg(xu, cip) : sU[t + RJ];
+      const T vjp = sc.turn ? ldg(xv, cjp) : sV[t + 1];
+      mCadd = fvb * fc * (T)0.5 * (v + vjp) + fub * fc * (T)0.5 * (u + uip);
+      sFV[t] = fvb;
+      sFU[t] = fub;
+      const T cbe = cdb * Mc;
+      const bool drag = (D * D) * ww > (T)sc.cDragMin2;
+      const T wb = (drag && sw > (T)0) ? cbe * D * (T)0.5 / sw : (T)0;
+      Db = drag ? cbe * sw * (T)sc.rhoSea : (T)0;
+      sAB[t] = wb * (T)0.5 * sa;
+      sBB[t] = wb * (T)0.5 * sb;
+      const T s11b = met(6) * (Xc - Xip);  // dyV
+      const T s22b = met(7) * (Yc - Yjp);  // dxU
+      const T s1t = l1 + (T)0.5 * (s11b + s22b);
+      const T s2t = l2 + (T)0.5 * (s11b - s22b);
+      // relaxation weights: fixed, or this substep's alpha at c (adaptive)
+      const T den1c = ada ? (T)1.0 / aC : (T)sc.den1, am1c = ada ? aC - (T)1 : (T)sc.alphaM1;
+      const T H1 = s1t * den1c * Mc, H2 = s2t * den1c * Mc;
+      lo1 = H1 * am1c;
+      lo2 = H2 * am1c;
+      const T pb = -H1;
+      const T pfac = (T)sc.tens1m * (T)2 * (T)sc.pr / (T)sc.tens1p;
+      Padd = pb * (T)sc.tens1m * (T)sc.pr1m;
+      zbp = H1 * (T)2 * ep + H2 * (T)0.5 * em + pb * pfac * dC;
+      epb0 = H1 * (T)2 * zeta;
+      emb0 = H2 * (T)0.5 * zeta;
+      dbp = pb * pfac * zeta;
+      const T s12t = l12 + met(8) * (Xjm - Xc) + met(9) * (Yim - Yc);  // dxV, dyU
+      T den2z = (T)sc.den2, am1z = (T)sc.alphaM1, aZ = 0;
+      if (ada) {  // alpha at z (veris/evp_solver.py:97-100)
+        aZ = (T)0.5 * (sAl[t] + sAl[t - 1]);
+        aZ = (T)0.5 * (aZ + (T)0.5 * (sAl[t - RJ] + sAl[t - RJ - 1]));
+        den2z = (T)1.0 / aZ;
+        am1z = aZ - (T)1;
+      }
+      const T G = s12t * den2z;
+      lo12 = G * am1z;
+      const T nz = (Mc + sM[t - RJ]) + (sM[t - 1] + sM[t - RJ - 1]);
+      // 1 / nz for the 0/1 mask sums 1..4 (0 -> 0), without a division
+      const T nrm = nz == (T)4 ? (T)0.25
+                    : nz == (T)3 ? (T)(1.0 / 3.0)
+                    : nz == (T)2 ? (T)0.5
+                    : nz == (T)1 ? (T)1 : (nz > (T)0 ? (T)1 / nz : (T)0);
+      const T zz = nrm * ((sZ[t] + sZ[t - RJ]) + (sZ[t - 1] + sZ[t - RJ - 1]));
+      sCD[t] = G * (T)0.5 * e12r * nrm;  // zeta adjoint share of z = this cell, weighted
+      sEP[t] = G * (T)0.5 * zz;          // e12 adjoint from sigma12
+      if (ada) {
+        // d sigma_new / d alpha = (sigma_old - forcing) / alpha^2 for sigma1, sigma2 at c
+        // and sigma12 at z; beta of the four faces of this c-point
+        const T divg = (((T)2 * zeta) * ep) - prs;
+        const T tens = ((T)2 * zeta) * em;
+        laC = (H1 * (so1 - divg) + H2 * (so2 - tens * (T)0.25)) * den1c +
+              ((sLBU[t] + sLBU[t + RJ]) + (sLBV[t] + sLBV[t + 1]));
+        const T shear = ((T)2 * zz) * e12r;
+        sLAZ[t] = G * (so12 - shear * (T)0.25) * den2z;
+      }
+    }
+    __syncthreads();
+
+    // ---- S3: zeta and Delta at c
+    if (ar >= 3 && br >= 3 && ar <= RI - 4 && br <= RJ - 4) {
+      T zb = zbp + ((sCD[t] + sCD[t + RJ]) + (sCD[t + 1] + sCD[t + RJ + 1]));
+      if (ada) {  // alpha at c from its four z-points, then to zeta and the mass
+        const T la = laC + (T)0.25 * ((sLAZ[t] + sLAZ[t + 1]) + (sLAZ[t + RJ] + sLAZ[t + RJ + 1]));
+        zb = zb + la * cZa;
+        mCadd = mCadd + la * cMa;
+      }
+      const T rreg = (T)1 / (dC + (T)sc.deltaMin);
+      Padd += zb * (T)0.5 * (T)sc.tens1p * rreg;
+      const T Dbar = dbp - zb * zeta * rreg;
+      const T dsqb = dC > (T)0 ? Dbar * (T)0.5 / dC : (T)0;
+      const T epb = epb0 + dsqb * (T)2 * ep;
+      const T emb = emb0 + dsqb * (T)sc.rpd * (T)2 * em;
+      sA[t] = (epb + emb) * met(3) * mInC;  // e11 adjoint (times its stencil factor)
+      sB[t] = (epb - emb) * met(4) * mInC;  // e22 adjoint
+      sQ[t] = dsqb * (T)sc.rpd * met(5);    // adjoint of rAz e12^2 at the four z-corners
+      if (sc.kterms) {  // e22 += uave k1C, e11 += vave k2C (veris/dynamics_routines.py:151-158)
+        sNV[t] = ((epb - emb) * mInC) * ldg(k.k1C, c) * (T)0.5;
+        sCXU[t] = ((epb + emb) * mInC) * ldg(k.k2C, c) * (T)0.5;
+      }
+      if (sc.turn) {
+        sU[t] = wU;
+        sV[t] = wV;
+      }
+      if (sdr) {
+        sZ[t] = qU;
+        sCXV[t] = qV;
+      }
+      if (bas) sLTB[t] = ltb;
+    }
+    __syncthreads();
+
+    // ---- S4: input-velocity adjoints; results of the tile's own cells
+    if (!own) return;
+    auto eadj = [&](int z, T rz) {
+      const T qs = (sQ[z] + sQ[z - RJ]) + (sQ[z - 1] + sQ[z - RJ - 1]);
+      return sEP[z] + qs * (T)2 * rz * sE12[z];
+    };
+    auto mzf = [&](int z) { return (sM[z] * sM[z - RJ]) * (sM[z - 1] * sM[z - RJ - 1]); };
     const T rdyU = met(0), rdxV = met(1), rAz = met(2);
-    const T ujm = sU[t - 1], vim = sV[t - RJ];
-    const T dudy = (u - ujm) * rdyU;
-    const T uavz = (u + ujm) * (T)0.5;
-    const T dvdx = (v - vim) * rdxV;
-    const T vavz = (v + vim) * (T)0.5;
-    const T mz = (Mc * sM[t - RJ]) * (sM[t - 1] * sM[t - RJ - 1]);
-    T inner = dudy + dvdx;
-    if (sc.kterms) inner = (inner - ldg(k.k1Z, c) * vavz) - ldg(k.k2Z, c) * uavz;
-    T e12 = ((T)0.5 * inner) * mz;
-    if (!sdr) {  // no-slip boundary term
-      const T hu = MUc - sMU[t - 1];
-      const T hv = MVc - sMV[t - RJ];
-      e12 = e12 + ((((T)2.0 * uavz) * rdyU) * hu + (((T)2.0 * vavz) * rdxV) * hv);
+    if (packed) {
+      rAz_jp = sRow[2 * RJ + br + 1];
+      rdyU_jp = sRow[br + 1];
+      rAz_ip = rAz;
+      rdxV_ip = rdxV;
+    }
+    const T ec = eadj(t, rAz), ejp = eadj(t + 1, rAz_jp), eip = eadj(t + RJ, rAz_ip);
+    const T mzc = mzf(t), mzjp = mzf(t + 1), mzip = mzf(t + RJ);
+    // no-slip boundary term of e12 (absent under free slip)
+    const T huc = sdr ? (T)0 : MUc - sMU[t - 1], hujp = sdr ? (T)0 : sMU[t + 1] - MUc;
+    const T hvc = sdr ? (T)0 : MVc - sMV[t - RJ], hvip = sdr ? (T)0 : sMV[t + RJ] - MVc;
+    const T A1 = (T)0.5 * mzc * rdyU + huc * rdyU;
+    const T A2jp = -(T)0.5 * mzjp * rdyU_jp + hujp * rdyU_jp;
+    const T A3 = (T)0.5 * mzc * rdxV + hvc * rdxV;
+    const T A4ip = -(T)0.5 * mzip * rdxV_ip + hvip * rdxV_ip;
+    const T fmc = fc * mC;
+    T lu_out = nbU * bUu * mU + (sAB[t] + sAB[t - RJ]) * mInUc + (sA[t - RJ] - sA[t]) +
+               ec * A1 + ejp * A2jp + (T)0.5 * fmc * sFU[t] + (T)0.5 * fmc_im * sFU[t - RJ];
+    T lv_out = nbV * bVv * mV + (sBB[t] + sBB[t - 1]) * mInVc + (sB[t - 1] - sB[t]) +
+               ec * A3 + eip * A4ip + (T)0.5 * fmc * sFV[t] + (T)0.5 * fmc_jm * sFV[t - 1];
+    if (sc.kterms) {  // the metric terms of e11, e22 (c) and e12 (z)
+      const T k1Z = ldg(k.k1Z, c), k2Z = ldg(k.k2Z, c);
+      const T k2Z_jp = ldg(k.k2Z, cjp), k1Z_ip = ldg(k.k1Z, cip);
+      lu_out = lu_out + (sNV[t] + sNV[t - RJ]) - (T)0.25 * (ec * mzc * k2Z + ejp * mzjp * k2Z_jp);
+      lv_out = lv_out + (sCXU[t] + sCXU[t - 1]) - (T)0.25 * (ec * mzc * k1Z + eip * mzip * k1Z_ip);
+    }
+    if (sc.turn) {  // the ice velocity in the turning terms of the four neighbouring faces
+      lu_out = lu_out + (T)0.25 * (T)sc.sinWat * (sU[t] + sU[t - 1]);
+      lv_out = lv_out - (T)0.25 * (T)sc.sinWat * (sV[t] + sV[t - RJ]);
+    }
+    if (sdr) {  // the ice speed at the two c-points of this u (v) point
+      lu_out = lu_out + (sZ[t] + sZ[t - RJ]);
+      lv_out = lv_out + (sCXV[t] + sCXV[t - 1]);
     }
     if (SO && sc.sobc) {
-      // second-order boundary shear, read from memory as the forward kernel
-      // reads it (two cells along the face normal reach past this region's edge)
-      const unsigned cim2 = mo + wrap(i - 2, g.NX) * g.NY + gj, cim1 = mo + wrap(i - 1, g.NX) * g.NY + gj;
-      const unsigned cjm2 = mo + gi * g.NY + wrap(j - 2, g.NY), cjm1 = mo + gi * g.NY + wrap(j - 1, g.NY);
-      const T uim2 = ldg(xu, cim2), uip1 = ldg(xu, cip);
-      const T MUim2 = ldg(k.MU, cim2), MUim1 = ldg(k.MU, cim1), MUip1 = ldg(k.MU, cip);
-      const T vjm2 = ldg(xv, cjm2), vjp1 = ldg(xv, cjp);
-      const T MVjm2 = ldg(k.MV, cjm2), MVjm1 = ldg(k.MV, cjm1), MVjp1 = ldg(k.MV, cjp);
-      T hu2 = (MUc - MUim1) / (T)3.0;
-      T hv2 = (MVc - MVjm1) / (T)3.0;
-      hu2 = hu2 * (MUim2 * MUim1 + MUip1 * MUc);
-      hv2 = hv2 * (MVjm2 * MVjm1 + MVjp1 * MVc);
-      const T tu = (rdyU * ((((T)6.0 * uavz) - uim2 * MUim1) - uip1 * MUc)) * hu2;
-      const T tv = (rdxV * ((((T)6.0 * vavz) - vjm2 * MVjm1) - vjp1 * MVc)) * hv2;
-      e12 = e12 + (T)0.5 * (tu + tv);
+      // the second-order boundary shear of the z-points whose stencil holds this
+      // u (v): uavz at z = c, c+1 (v: c, c+RJ), u two cells down at z = c+2RJ
+      // and one up at z = c-RJ (v: c+2, c-1), as the reference shifts them
+      auto hu2f = [&](int z) {
+        return ((sMU[z] - sMU[z - RJ]) / (T)3.0) * (sMU[z - 2 * RJ] * sMU[z - RJ] + sMU[z + RJ] * sMU[z]);
+      };
+      auto hv2f = [&](int z) {
+        return ((sMV[z] - sMV[z - 1]) / (T)3.0) * (sMV[z - 2] * sMV[z - 1] + sMV[z + 1] * sMV[z]);
+      };
+      T rAz_i2 = rAz, rAz_im = rAz, rdyU_i2 = rdyU, rdyU_im = rdyU;  // along i: rows hold them
+      T rAz_j2, rAz_jm, rdxV_j2, rdxV_jm;
+      if (packed) {
+        rAz_j2 = sRow[2 * RJ + br + 2];
+        rAz_jm = sRow[2 * RJ + br - 1];
+        rdxV_j2 = sRow[RJ + br + 2];
+        rdxV_jm = sRow[RJ + br - 1];
+      } else {
+        const unsigned ci2 = mo + wrap(gi + 2, g.NX) * g.NY + gj, cj2 = mo + gi * g.NY + wrap(gj + 2, g.NY);
+        rAz_i2 = ldg(k.rAz, ci2);
+        rAz_im = ldg(k.rAz, cim);
+        rdyU_i2 = ldg(k.rdyU, ci2);
+        rdyU_im = ldg(k.rdyU, cim);
+        rAz_j2 = ldg(k.rAz, cj2);
+        rAz_jm = ldg(k.rAz, cjm);
+        rdxV_j2 = ldg(k.rdxV, cj2);
+        rdxV_jm = ldg(k.rdxV, cjm);
+      }
+      const T e_i2 = eadj(t + 2 * RJ, rAz_i2), e_im = eadj(t - RJ, rAz_im);
+      const T e_j2 = eadj(t + 2, rAz_j2), e_jm = eadj(t - 1, rAz_jm);
+      lu_out = lu_out + (T)1.5 * (ec * rdyU * hu2f(t) + ejp * rdyU_jp * hu2f(t + 1)) -
+               (T)0.5 * (e_i2 * rdyU_i2 * hu2f(t + 2 * RJ) * sMU[t + RJ] +
+                         e_im * rdyU_im * hu2f(t - RJ) * sMU[t - RJ]);
+      lv_out = lv_out + (T)1.5 * (ec * rdxV * hv2f(t) + eip * rdxV_ip * hv2f(t + RJ)) -
+               (T)0.5 * (e_j2 * rdxV_j2 * hv2f(t + 2) * sMV[t + 1] +
+                         e_jm * rdxV_jm * hv2f(t - 1) * sMV[t - 1]);
     }
-    sE12[t] = e12;
-    sQ[t] = rAz * (e12 * e12);
-    e12r = e12;
-  }
-  __syncthreads();
-
-  // ---- C: zeta, Delta, strain invariants, water drag at c-points
-  T zeta = 0, dC = 0, ep = 0, em = 0, sa = 0, sb = 0, ww = 0, sw = 0;
-  T su = 0, sv = 0, spd = 0;  // lateral drag: u + u(i+1), v + v(j+1) and the speed at c
-  T cTB = 0, cH = 0;          // basal drag: its derivatives at c
-  T aC = 0, cZa = 0, cMa = 0, prs = 0;  // adaptive: alpha at c, d alpha / d (zeta, mC), press
-  if (ar >= 1 && br >= 1 && ar <= RI - 2 && br <= RJ - 2) {
-    const T uip = sU[t + RJ], vjp = sV[t + 1];
-    const T rdxU = met(3), rdyV = met(4);
-    const T dudx = (uip - u) * rdxU, dvdy = (vjp - v) * rdyV;
-    T e11 = dudx, e22 = dvdy;
-    if (sc.kterms) {
-      e11 = dudx + ((vjp + v) * (T)0.5) * ldg(k.k2C, c);
-      e22 = dvdy + ((uip + u) * (T)0.5) * ldg(k.k1C, c);
+    if (bas) {  // the basal coefficient's velocity argument at the two c-points
+      lu_out = lu_out + (((T)0.5 * (u * mInUc)) * mInUc) * (sLTB[t] + sLTB[t - RJ]);
+      lv_out = lv_out + (((T)0.5 * (v * mInVc)) * mInVc) * (sLTB[t] + sLTB[t - 1]);
+      stg(acc.h, c, ldg(acc.h, c) + aHadd);
     }
-    e11 = e11 * mInC;
-    e22 = e22 * mInC;
-    const T qa = sQ[t] + sQ[t + RJ];
-    const T qb = sQ[t + 1] + sQ[t + RJ + 1];
-    const T e12c = ((T)0.25 * met(5)) * (qa + qb);
-    if (sdr) {  // ice speed at c (veris/dynamics_routines.py:109-111)
-      su = u + uip;
-      sv = v + vjp;
-      spd = (T)0.5 * sqrt(su * su + sv * sv);
-      sSPD[t] = spd;
+    stg(L.out[0], c, lu_out);
+    stg(L.out[1], c, lv_out);
+    stg(L.out[2], c, lo1);
+    stg(L.out[3], c, lo2);
+    stg(L.out[4], c, lo12);
+    stg(acc.P, c, aP + Padd);
+    stg(acc.mC, c, amC + mCadd);
+    if (fc < (T)0) stg(acc.DS, c, aDS + Db);
+    else stg(acc.DN, c, aDN + Db);
+    if (comp) {  // nbU = nbV = 0 elsewhere
+      stg(acc.mU, c, amU + aMU);
+      stg(acc.WFX, c, aWX + nbU);
+      stg(acc.mV, c, amV + aMV);
+      stg(acc.WFY, c, aWY + nbV);
+      if (acc.u0) {  // optional outputs (uniform branch)
+        stg(acc.u0, c, ldg(acc.u0, c) + nbU * mU * rdt);
+        stg(acc.v0, c, ldg(acc.v0, c) + nbV * mV * rdt);
+      }
+      if (acc.AW) {
+        stg(acc.AW, c, ldg(acc.AW, c) + aAW);
+        stg(acc.AS, c, ldg(acc.AS, c) + aAS);
+      }
     }
-    if (bas) {  // basal drag coefficient at c, the forward kernel's expression order
-      const T um = u * mInUc, umip = uip * bmUip;
-      const T vm = v * mInVc, vmjp = vjp * bmVjp;
-      const T tbs = (T)0.25 * (((um * um + umip * umip) + vm * vm) + vmjp * vmjp) + (T)sc.U0sq;
-      const T tb = (T)sc.K2 / sqrt(tbs);
-      const T hc = (fabs(bRl) * bAr) / (T)sc.K1;
-      const T ex = exp((T)10.0 * (bh - hc));
-      const T lg = log(ex + (T)1.0);
-      const T eb = exp((T)(-sc.cBasalStar) * ((T)1.0 - bAr));
-      const T val = ((tb * lg) * (T)0.1) * eb;
-      const bool on = bAr > (T)0.01;
-      sCBc[t] = on ? val : (T)0;
-      // d cb / d tb-argument and d cb / d h (the soft maximum's logistic slope)
-      cTB = on ? ((T)-0.5 * (((tb * lg) * (T)0.1) * eb)) / tbs : (T)0;
-      cH = on ? (tb * eb) * (ex / (ex + (T)1.0)) : (T)0;
-    }
-    ep = e11 + e22;
-    em = e11 - e22;
-    const TD dsq = (TD)ep * (TD)ep + (TD)sc.rpd * ((TD)em * (TD)em + (TD)4.0 * (TD)e12c);
-    const TD dCd = sqrt(dsq);
-    const TD zd = ((TD)0.5 * ((TD)P * (TD)sc.tens1p)) / (dCd + (TD)sc.deltaMin);
-    zeta = (T)zd;
-    dC = (T)dCd;
-    sZ[t] = zeta;
-    if (ada) {  // alpha at c (the forward kernel's expression order) and its derivatives
-      const T mmax = fmax(mC, (T)1e-4);
-      const T Y = ((zeta * (T)sc.evpcfac) / mmax) * met(5);
-      const T sY = sqrt(Y);
-      const T raw = sY * Mc;
-      aC = fmax(raw, (T)sc.alphaMin);
-      sAl[t] = aC;
-      const bool act = raw > (T)sc.alphaMin && sY > (T)0;
-      const T dY = act ? (Mc * (T)0.5) / sY : (T)0;  // d alpha / d Y
-      cZa = dY * (((T)sc.evpcfac / mmax) * met(5));
-      cMa = mC > (T)1e-4 ? -dY * (Y / mmax) : (T)0;
-      // the replacement pressure, for the sigma1 relaxation's alpha derivative
-      T prep = (((T)2.0 * zeta) * dC) * (T)sc.pr;
-      if (!sc.tens0) prep = prep / (T)sc.tens1p;
-      prs = (P * (T)sc.pr1m) + prep;
-      if (!sc.tens0) prs = prs * (T)sc.tens1m;
-    }
-    sa = sA[t] + sA[t + RJ];
-    sb = sB[t] + sB[t + 1];
-    ww = (T)0.25 * (sa * sa + sb * sb);
-    sw = sqrt(ww);
-    const T cd = ((D * D) * ww > (T)sc.cDragMin2) ? D * sw : (T)sc.cDragMin;
-    sCD[t] = cd * Mc;
-    if (sc.turn) {  // (uOcean - u) summed over j, j+1 and (vOcean - v) over i, i+1
-      sAB[t] = sFU[t] + sFU[t + 1];
-      sBB[t] = sFV[t] + sFV[t + RJ];
-    }
-  }
-  __syncthreads();
-
-  // parameter adjoints of this cell, read-modify-written at the end: issued
-  // here, after the forward recompute has released its registers
-  const T aP = ldg(acc.P, c), amC = ldg(acc.mC, c), amU = ldg(acc.mU, c), amV = ldg(acc.mV, c);
-  const T aWX = ldg(acc.WFX, c), aWY = ldg(acc.WFY, c), aDN = ldg(acc.DN, c), aDS = ldg(acc.DS, c);
-  const T uNm1 = ldg(k.uNm1, c), vNm1 = ldg(k.vNm1, c);
-  // neighbours' constants used by S4
-  // (the packed rows hold these at column br + 1 and, constant along i, br)
-  T rAz_jp = 0, rdyU_jp = 0, rAz_ip = 0, rdxV_ip = 0;
-  if (!packed) {
-    rAz_jp = ldg(k.rAz, cjp);
-    rdyU_jp = ldg(k.rdyU, cjp);
-    rAz_ip = ldg(k.rAz, cip);
-    rdxV_ip = ldg(k.rdxV, cip);
-  }
-  const T fmc_im = ldg(k.fC, cim) * ldg(k.massC, cim), fmc_jm = ldg(k.fC, cjm) * ldg(k.massC, cjm);
-
-  // ---- S1: momentum update at the u- and v-point of this cell
-  const T cw = (T)sc.cosWat, bU = (T)sc.betaU, bP1 = (T)sc.betaP1, rdt = (T)sc.rdt;
-  T nbU = 0, dbU = 0, nbV = 0, dbV = 0, aAW = 0, aAS = 0, aMU = 0, aMV = 0;
-  T bUu = bU, bVv = bU;  // beta / deltatDyn at this u- and v-point (adaptive: from alpha)
-  if (ar >= 2 && br >= 2 && ar <= RI - 2 && br <= RJ - 2) {
-    T bP1u = bP1, bP1v = bP1;
-    if (ada) {  // veris/evp_solver.py:148-155
-      bUu = ((T)0.5 * (sAl[t] + sAl[t - RJ])) * rdt;
-      bVv = ((T)0.5 * (sAl[t] + sAl[t - 1])) * rdt;
-      bP1u = bUu + rdt;
-      bP1v = bVv + rdt;
-    }
-    const T cdAU = (T)0.5 * (sCD[t] + sCD[t - RJ]);
-    const T cdAV = (T)0.5 * (sCD[t] + sCD[t - 1]);
-    // basal drag at the u/v point (zero without it: the same sum as the forward's)
-    const T cbU = bas ? (T)0.5 * (sCBc[t] + sCBc[t - RJ]) : (T)0;
-    const T cbV = bas ? (T)0.5 * (sCBc[t] + sCBc[t - 1]) : (T)0;
-    const T denU = mU * bP1u + AW * ((cdAU * cw) + cbU);
-    const T denV = mV * bP1v + AS * ((cdAV * cw) + cbV);
-    const bool gU = denU == (T)0, gV = denV == (T)0;
-    T dU = gU ? (T)1 : denU, dV = gV ? (T)1 : denV;
-    // lateral drag, added after the zero guard as the forward adds it
-    T spU = 0, spV = 0, sdnU = 1, sdnV = 1, sdU = 0, sdV = 0, sdmU = 0, sdmV = 0;
-    if (sdr) {
-      spU = AW > (T)0 ? (T)0.5 * (sSPD[t] + sSPD[t - RJ]) : (T)0;
-      spV = AS > (T)0 ? (T)0.5 * (sSPD[t] + sSPD[t - 1]) : (T)0;
-      sdmU = sc.coast ? FuC : (((T)2 - sMU[t - RJ]) - sMU[t + RJ]) * MUc;
-      sdmV = sc.coast ? FvC : (((T)2 - sMV[t - 1]) - sMV[t + 1]) * MVc;
-      sdnU = spU + (T)sc.sdU0;
-      sdnV = spV + (T)sc.sdU0;
-      sdU = ((mU * (T)sc.sdCoeff) * sdmU) / sdnU;
-      sdV = ((mV * (T)sc.sdCoeff) * sdmV) / sdnV;
-      dU = dU + sdU;
-      dV = dV + sdV;
-    }
-    const T rU = (T)1 / dU, rV = (T)1 / dV;
-    const T lcu = comp ? lu : (T)0, lcv = comp ? lv : (T)0;
-    nbU = lcu * MUc * rU;
-    const T ddU = -lcu * un * rU, ddV = -lcv * vn * rV;  // adjoints of the denominators
-    dbU = gU ? (T)0 : ddU;
-    nbV = lcv * MVc * rV;
-    dbV = gV ? (T)0 : ddV;
-    sA[t] = nbU * met(10);  // X: adjoint of the u-point stress divergence (rrAu)
-    sB[t] = nbV * met(11);  // Y: of the v-point one (rrAv)
-    sQ[t] = nbU;
-    sNV[t] = nbV;
-    sCXU[t] = (nbU * uoc + dbU) * (AW * cw);
-    sCXV[t] = (nbV * voc + dbV) * (AS * cw);
-    T turnU = 0, turnV = 0;
-    if (sc.turn) {  // turning term of the forcing (veris/evp_solver.py:119-139)
-      const T sgs = sgn(fc) * (T)sc.sinWat;
-      const T lmU = mU != (T)0 ? (T)1 : mU, lmV = mV != (T)0 ? (T)1 : mV;
-      const T Xt = (sCD[t] * (T)0.5) * sAB[t] + (sCD[t - RJ] * (T)0.5) * sAB[t - RJ];
-      const T Yt = (sCD[t] * (T)0.5) * sBB[t] + (sCD[t - 1] * (T)0.5) * sBB[t - 1];
-      turnU = -((sgs * (T)0.5) * Xt) * lmU;  // d ForcingX / d AreaW, turning part
-      turnV = ((sgs * (T)0.5) * Yt) * lmV;
-      sU[t] = nbU * sgn(fc) * lmU * AW;
-      sV[t] = nbV * sgn(fc) * lmV * AS;
-    }
-    aAW = nbU * (cdAU * cw * uoc + turnU) + dbU * (cdAU * cw + cbU);  // AreaW, AreaS adjoints
-    aAS = nbV * (cdAV * cw * voc + turnV) + dbV * (cdAV * cw + cbV);
-    aMU = nbU * (bUu * u + rdt * uNm1) + dbU * bP1u;  // SeaIceMassU, SeaIceMassV adjoints
-    aMV = nbV * (bVv * v + rdt * vNm1) + dbV * bP1v;
-    if (ada) {  // beta adjoints, times the averaging weight and 1 / deltatDyn
-      sLBU[t] = ((T)0.5 * rdt) * (nbU * mU * u + dbU * mU);
-      sLBV[t] = ((T)0.5 * rdt) * (nbV * mV * v + dbV * mV);
-    }
-    if (bas) {  // the basal coefficient's share of the denominators
-      sCBU[t] = dbU * AW;
-      sCBV[t] = dbV * AS;
-    }
-    if (sdr) {  // the lateral drag's mass factor and its speed (zero where AreaW/S = 0)
-      aMU = aMU + ddU * (((T)sc.sdCoeff * sdmU) / sdnU);
-      aMV = aMV + ddV * (((T)sc.sdCoeff * sdmV) / sdnV);
-      sLSU[t] = AW > (T)0 ? (T)-0.5 * (ddU * sdU / sdnU) : (T)0;
-      sLSV[t] = AS > (T)0 ? (T)-0.5 * (ddV * sdV / sdnV) : (T)0;
-    }
-  }
-  __syncthreads();
-
-  // ---- S2: stresses, drag and Coriolis at c, sigma12 at z
-  T Padd = 0, mCadd = 0, Db = 0, zbp = 0, epb0 = 0, emb0 = 0, dbp = 0;
-  T lo1 = 0, lo2 = 0, lo12 = 0, wU = 0, wV = 0, qU = 0, qV = 0, ltb = 0, aHadd = 0, laC = 0;
-  if (ar >= 3 && br >= 3 && ar <= RI - 3 && br <= RJ - 3) {
-    if (bas) {  // basal coefficient adjoint at c from its four faces
-      const T lcb = (T)0.5 * (sCBU[t] + sCBU[t + RJ]) + (T)0.5 * (sCBV[t] + sCBV[t + 1]);
-      ltb = lcb * cTB;
-      aHadd = lcb * cH;
-    }
-    if (sdr) {  // speed adjoint at c from its four faces, then d speed / d (u, v)
-      const T lsp = (sLSU[t] + sLSU[t + RJ]) + (sLSV[t] + sLSV[t + 1]);
-      const T lq = spd > (T)0 ? lsp * (T)0.125 / spd : (T)0;  // speed = sqrt(q) / 2
-      qU = (T)2 * lq * su;
-      qV = (T)2 * lq * sv;
-    }
-    const T Xc = sA[t], Xip = sA[t + RJ], Xjm = sA[t - 1];
-    const T Yc = sB[t], Yjp = sB[t + 1], Yim = sB[t - RJ];
-    T cdb = (T)0.5 * (sCXU[t] + sCXU[t + RJ]) + (T)0.5 * (sCXV[t] + sCXV[t + 1]);
-    if (sc.turn) {  // the drag of this c-point in the turning terms of its four faces
-      // (sAB/sBB of this cell are read before this thread overwrites them)
-      const T kus = sU[t] + sU[t + RJ], kvs = sV[t] + sV[t + 1];
-      cdb = cdb + (T)0.25 * (T)sc.sinWat * (sBB[t] * kvs - sAB[t] * kus);
-      wU = sCD[t] * kus;  // and the ice velocities in them, weighted by the drag
-      wV = sCD[t] * kvs;
-    }
-    const T fvb = (T)0.5 * (sQ[t] + sQ[t + RJ]);
-    const T fub = -(T)0.5 * (sNV[t] + sNV[t + 1]);
-    // neighbour velocities (sU/sV hold the turning factors when sinWat != 0)
-    const T uip = sc.turn ? ldg(xu, cip) : sU[t + RJ];
-    const T vjp = sc.turn ? ldg(xv, cjp) : sV[t + 1];
-    mCadd = fvb * fc * (T)0.5 * (v + vjp) + fub * fc * (T)0.5 * (u + uip);
-    sFV[t] = fvb;
-    sFU[t] = fub;
-    const T cbe = cdb * Mc;
-    const bool drag = (D * D) * ww > (T)sc.cDragMin2;
-    const T wb = (drag && sw > (T)0) ? cbe * D * (T)0.5 / sw : (T)0;
-    Db = drag ? cbe * sw * (T)sc.rhoSea : (T)0;
-    sAB[t] = wb * (T)0.5 * sa;
-    sBB[t] = wb * (T)0.5 * sb;
-    const T s11b = met(6) * (Xc - Xip);  // dyV
-    const T s22b = met(7) * (Yc - Yjp);  // dxU
-    const T s1t = l1 + (T)0.5 * (s11b + s22b);
-    const T s2t = l2 + (T)0.5 * (s11b - s22b);
-    // relaxation weights: fixed, or this substep's alpha at c (adaptive)
-    const T den1c = ada ? (T)1.0 / aC : (T)sc.den1, am1c = ada ? aC - (T)1 : (T)sc.alphaM1;
-    const T H1 = s1t * den1c * Mc, H2 = s2t * den1c * Mc;
-    lo1 = H1 * am1c;
-    lo2 = H2 * am1c;
-    const T pb = -H1;
-    const T pfac = (T)sc.tens1m * (T)2 * (T)sc.pr / (T)sc.tens1p;
-    Padd = pb * (T)sc.tens1m * (T)sc.pr1m;
-    zbp = H1 * (T)2 * ep + H2 * (T)0.5 * em + pb * pfac * dC;
-    epb0 = H1 * (T)2 * zeta;
-    emb0 = H2 * (T)0.5 * zeta;
-    dbp = pb * pfac * zeta;
-    const T s12t = l12 + met(8) * (Xjm - Xc) + met(9) * (Yim - Yc);  // dxV, dyU
-    T den2z = (T)sc.den2, am1z = (T)sc.alphaM1, aZ = 0;
-    if (ada) {  // alpha at z (veris/evp_solver.py:97-100)
-      aZ = (T)0.5 * (sAl[t] + sAl[t - 1]);
-      aZ = (T)0.5 * (aZ + (T)0.5 * (sAl[t - RJ] + sAl[t - RJ - 1]));
-      den2z = (T)1.0 / aZ;
-      am1z = aZ - (T)1;
-    }
-    const T G = s12t * den2z;
-    lo12 = G * am1z;
-    const T nz = (Mc + sM[t - RJ]) + (sM[t - 1] + sM[t - RJ - 1]);
-    // 1 / nz for the 0/1 mask sums 1..4 (0 -> 0), without a division
-    const T nrm = nz == (T)4 ? (T)0.25
-                  : nz == (T)3 ? (T)(1.0 / 3.0)
-                  : nz == (T)2 ? (T)0.5
-                  : nz == (T)1 ? (T)1 : (nz > (T)0 ? (T)1 / nz : (T)0);
-    const T zz = nrm * ((sZ[t] + sZ[t - RJ]) + (sZ[t - 1] + sZ[t - RJ - 1]));
-    sCD[t] = G * (T)0.5 * e12r * nrm;  // zeta adjoint share of z = this cell, weighted
-    sEP[t] = G * (T)0.5 * zz;          // e12 adjoint from sigma12
-    if (ada) {
-      // d sigma_new / d alpha = (sigma_old - forcing) / alpha^2 for sigma1, sigma2 at c
-      // and sigma12 at z; beta of the four faces of this c-point
-      const T divg = (((T)2 * zeta) * ep) - prs;
-      const T tens = ((T)2 * zeta) * em;
-      laC = (H1 * (so1 - divg) + H2 * (so2 - tens * (T)0.25)) * den1c +
-            ((sLBU[t] + sLBU[t + RJ]) + (sLBV[t] + sLBV[t + 1]));
-      const T shear = ((T)2 * zz) * e12r;
-      sLAZ[t] = G * (so12 - shear * (T)0.25) * den2z;
-    }
-  }
-  __syncthreads();
-
-  // ---- S3: zeta and Delta at c
-  if (ar >= 3 && br >= 3 && ar <= RI - 4 && br <= RJ - 4) {
-    T zb = zbp + ((sCD[t] + sCD[t + RJ]) + (sCD[t + 1] + sCD[t + RJ + 1]));
-    if (ada) {  // alpha at c from its four z-points, then to zeta and the mass
-      const T la = laC + (T)0.25 * ((sLAZ[t] + sLAZ[t + 1]) + (sLAZ[t + RJ] + sLAZ[t + RJ + 1]));
-      zb = zb + la * cZa;
-      mCadd = mCadd + la * cMa;
-    }
-    const T rreg = (T)1 / (dC + (T)sc.deltaMin);
-    Padd += zb * (T)0.5 * (T)sc.tens1p * rreg;
-    const T Dbar = dbp - zb * zeta * rreg;
-    const T dsqb = dC > (T)0 ? Dbar * (T)0.5 / dC : (T)0;
-    const T epb = epb0 + dsqb * (T)2 * ep;
-    const T emb = emb0 + dsqb * (T)sc.rpd * (T)2 * em;
-    sA[t] = (epb + emb) * met(3) * mInC;  // e11 adjoint (times its stencil factor)
-    sB[t] = (epb - emb) * met(4) * mInC;  // e22 adjoint
-    sQ[t] = dsqb * (T)sc.rpd * met(5);    // adjoint of rAz e12^2 at the four z-corners
-    if (sc.kterms) {  // e22 += uave k1C, e11 += vave k2C (veris/dynamics_routines.py:151-158)
-      sNV[t] = ((epb - emb) * mInC) * ldg(k.k1C, c) * (T)0.5;
-      sCXU[t] = ((epb + emb) * mInC) * ldg(k.k2C, c) * (T)0.5;
-    }
-    if (sc.turn) {
-      sU[t] = wU;
-      sV[t] = wV;
-    }
-    if (sdr) {
-      sZ[t] = qU;
-      sCXV[t] = qV;
-    }
-    if (bas) sLTB[t] = ltb;
-  }
-  __syncthreads();
-
-  // ---- S4: input-velocity adjoints; results of the tile's own cells
-  if (!own) return;
-  auto eadj = [&](int z, T rz) {
-    const T qs = (sQ[z] + sQ[z - RJ]) + (sQ[z - 1] + sQ[z - RJ - 1]);
-    return sEP[z] + qs * (T)2 * rz * sE12[z];
   };
-  auto mzf = [&](int z) { return (sM[z] * sM[z - RJ]) * (sM[z - 1] * sM[z - RJ - 1]); };
-  const T rdyU = met(0), rdxV = met(1), rAz = met(2);
-  if (packed) {
-    rAz_jp = sRow[2 * RJ + br + 1];
-    rdyU_jp = sRow[br + 1];
-    rAz_ip = rAz;
-    rdxV_ip = rdxV;
-  }
-  const T ec = eadj(t, rAz), ejp = eadj(t + 1, rAz_jp), eip = eadj(t + RJ, rAz_ip);
-  const T mzc = mzf(t), mzjp = mzf(t + 1), mzip = mzf(t + RJ);
-  // no-slip boundary term of e12 (absent under free slip)
-  const T huc = sdr ? (T)0 : MUc - sMU[t - 1], hujp = sdr ? (T)0 : sMU[t + 1] - MUc;
-  const T hvc = sdr ? (T)0 : MVc - sMV[t - RJ], hvip = sdr ? (T)0 : sMV[t + RJ] - MVc;
-  const T A1 = (T)0.5 * mzc * rdyU + huc * rdyU;
-  const T A2jp = -(T)0.5 * mzjp * rdyU_jp + hujp * rdyU_jp;
-  const T A3 = (T)0.5 * mzc * rdxV + hvc * rdxV;
-  const T A4ip = -(T)0.5 * mzip * rdxV_ip + hvip * rdxV_ip;
-  const T fmc = fc * mC;
-  T lu_out = nbU * bUu * mU + (sAB[t] + sAB[t - RJ]) * mInUc + (sA[t - RJ] - sA[t]) +
-             ec * A1 + ejp * A2jp + (T)0.5 * fmc * sFU[t] + (T)0.5 * fmc_im * sFU[t - RJ];
-  T lv_out = nbV * bVv * mV + (sBB[t] + sBB[t - 1]) * mInVc + (sB[t - 1] - sB[t]) +
-             ec * A3 + eip * A4ip + (T)0.5 * fmc * sFV[t] + (T)0.5 * fmc_jm * sFV[t - 1];
-  if (sc.kterms) {  // the metric terms of e11, e22 (c) and e12 (z)
-    const T k1Z = ldg(k.k1Z, c), k2Z = ldg(k.k2Z, c);
-    const T k2Z_jp = ldg(k.k2Z, cjp), k1Z_ip = ldg(k.k1Z, cip);
-    lu_out = lu_out + (sNV[t] + sNV[t - RJ]) - (T)0.25 * (ec * mzc * k2Z + ejp * mzjp * k2Z_jp);
-    lv_out = lv_out + (sCXU[t] + sCXU[t - 1]) - (T)0.25 * (ec * mzc * k1Z + eip * mzip * k1Z_ip);
-  }
-  if (sc.turn) {  // the ice velocity in the turning terms of the four neighbouring faces
-    lu_out = lu_out + (T)0.25 * (T)sc.sinWat * (sU[t] + sU[t - 1]);
-    lv_out = lv_out - (T)0.25 * (T)sc.sinWat * (sV[t] + sV[t - RJ]);
-  }
-  if (sdr) {  // the ice speed at the two c-points of this u (v) point
-    lu_out = lu_out + (sZ[t] + sZ[t - RJ]);
-    lv_out = lv_out + (sCXV[t] + sCXV[t - 1]);
-  }
-  if (SO && sc.sobc) {
-    // the second-order boundary shear of the z-points whose stencil holds this
-    // u (v): uavz at z = c, c+1 (v: c, c+RJ), u two cells down at z = c+2RJ
-    // and one up at z = c-RJ (v: c+2, c-1), as the reference shifts them
-    auto hu2f = [&](int z) {
-      return ((sMU[z] - sMU[z - RJ]) / (T)3.0) * (sMU[z - 2 * RJ] * sMU[z - RJ] + sMU[z + RJ] * sMU[z]);
-    };
-    auto hv2f = [&](int z) {
-      return ((sMV[z] - sMV[z - 1]) / (T)3.0) * (sMV[z - 2] * sMV[z - 1] + sMV[z + 1] * sMV[z]);
-    };
-    T rAz_i2 = rAz, rAz_im = rAz, rdyU_i2 = rdyU, rdyU_im = rdyU;  // along i: rows hold them
-    T rAz_j2, rAz_jm, rdxV_j2, rdxV_jm;
-    if (packed) {
-      rAz_j2 = sRow[2 * RJ + br + 2];
-      rAz_jm = sRow[2 * RJ + br - 1];
-      rdxV_j2 = sRow[RJ + br + 2];
-      rdxV_jm = sRow[RJ + br - 1];
-    } else {
-      const unsigned ci2 = mo + wrap(gi + 2, g.NX) * g.NY + gj, cj2 = mo + gi * g.NY + wrap(gj + 2, g.NY);
-      rAz_i2 = ldg(k.rAz, ci2);
-      rAz_im = ldg(k.rAz, cim);
-      rdyU_i2 = ldg(k.rdyU, ci2);
-      rdyU_im = ldg(k.rdyU, cim);
-      rAz_j2 = ldg(k.rAz, cj2);
-      rAz_jm = ldg(k.rAz, cjm);
-      rdxV_j2 = ldg(k.rdxV, cj2);
-      rdxV_jm = ldg(k.rdxV, cjm);
-    }
-    const T e_i2 = eadj(t + 2 * RJ, rAz_i2), e_im = eadj(t - RJ, rAz_im);
-    const T e_j2 = eadj(t + 2, rAz_j2), e_jm = eadj(t - 1, rAz_jm);
-    lu_out = lu_out + (T)1.5 * (ec * rdyU * hu2f(t) + ejp * rdyU_jp * hu2f(t + 1)) -
-             (T)0.5 * (e_i2 * rdyU_i2 * hu2f(t + 2 * RJ) * sMU[t + RJ] +
-                       e_im * rdyU_im * hu2f(t - RJ) * sMU[t - RJ]);
-    lv_out = lv_out + (T)1.5 * (ec * rdxV * hv2f(t) + eip * rdxV_ip * hv2f(t + RJ)) -
-             (T)0.5 * (e_j2 * rdxV_j2 * hv2f(t + 2) * sMV[t + 1] +
-                       e_jm * rdxV_jm * hv2f(t - 1) * sMV[t - 1]);
-  }
-  if (bas) {  // the basal coefficient's velocity argument at the two c-points
-    lu_out = lu_out + (((T)0.5 * (u * mInUc)) * mInUc) * (sLTB[t] + sLTB[t - RJ]);
-    lv_out = lv_out + (((T)0.5 * (v * mInVc)) * mInVc) * (sLTB[t] + sLTB[t - 1]);
-    stg(acc.h, c, ldg(acc.h, c) + aHadd);
-  }
-  stg(L.out[0], c, lu_out);
-  stg(L.out[1], c, lv_out);
-  stg(L.out[2], c, lo1);
-  stg(L.out[3], c, lo2);
-  stg(L.out[4], c, lo12);
-  stg(acc.P, c, aP + Padd);
-  stg(acc.mC, c, amC + mCadd);
-  if (fc < (T)0) stg(acc.DS, c, aDS + Db);
-  else stg(acc.DN, c, aDN + Db);
-  if (comp) {  // nbU = nbV = 0 elsewhere
-    stg(acc.mU, c, amU + aMU);
-    stg(acc.WFX, c, aWX + nbU);
-    stg(acc.mV, c, amV + aMV);
-    stg(acc.WFY, c, aWY + nbV);
-    if (acc.u0) {  // optional outputs (uniform branch)
-      stg(acc.u0, c, ldg(acc.u0, c) + nbU * mU * rdt);
-      stg(acc.v0, c, ldg(acc.v0, c) + nbV * mV * rdt);
-    }
-    if (acc.AW) {
-      stg(acc.AW, c, ldg(acc.AW, c) + aAW);
-      stg(acc.AS, c, ldg(acc.AS, c) + aAS);
-    }
-  }
+  if (blk->packed != 0) body(std::true_type{});
+  else body(std::false_type{});
 }
 
 struct AdjTileCfg {
@@ -832,7 +879,7 @@ struct AdjTileCfg {
 };
 // 512- and 768-thread shapes: the FP64 kernel needs ~156 VGPRs, more than a
 // 1024-thread workgroup can give each lane.
-constexpr AdjTileCfg kAdjTiles[] = {{16, 32}, {24, 32}, {12, 64}, {32, 32}};
+constexpr AdjTileCfg kAdjTiles[] = {{16, 32}, {24, 32}, {12, 64}, {32, 32}, {20, 32}};
 constexpr int kAdjNumTiles = sizeof(kAdjTiles) / sizeof(kAdjTiles[0]);
 
 template <typename T, int RI, int RJ, int MODE>
@@ -865,7 +912,8 @@ void adj_launch(int cfg, const AdjArgs<T>* a, const EvpArgBlock<T>* b, LamSet<T>
   if (mode == 1) {
     if constexpr (sizeof(T) == 4) {
       switch (cfg) {
-        case 1: adj_launch_shape<T, 24, 32, 1>(a, b, L, g, n_members, st); break;
+        case 1:
+        case 4: adj_launch_shape<T, 24, 32, 1>(a, b, L, g, n_members, st); break;
         case 2: adj_launch_shape<T, 12, 64, 1>(a, b, L, g, n_members, st); break;
         case 3: adj_launch_shape<T, 32, 32, 1>(a, b, L, g, n_members, st); break;
         default: adj_launch_shape<T, 16, 32, 1>(a, b, L, g, n_members, st); break;
@@ -876,6 +924,7 @@ void adj_launch(int cfg, const AdjArgs<T>* a, const EvpArgBlock<T>* b, LamSet<T>
   switch (cfg) {
     case 1: adj_launch_shape<T, 24, 32, 0>(a, b, L, g, n_members, st); break;
     case 2: adj_launch_shape<T, 12, 64, 0>(a, b, L, g, n_members, st); break;
+    case 4: adj_launch_shape<T, 20, 32, 0>(a, b, L, g, n_members, st); break;
     case 3:  // 1024 threads: FP32 only (the FP64 kernel's registers do not fit)
       if constexpr (sizeof(T) == 4) adj_launch_shape<T, 32, 32, 0>(a, b, L, g, n_members, st);
       else adj_launch_shape<T, 24, 32, 0>(a, b, L, g, n_members, st);
@@ -960,7 +1009,8 @@ __global__ __launch_bounds__(BLK) void pre_evp_adjoint_kernel(Geom g, PreAdjArgs
 struct AdjLayout {
   size_t fb, blk, total;
   int nck, ce, n;
-  size_t off_args, off_fblk, off_rblk, off_ckpt, off_pp, off_seg, off_lam, off_pblk, off_pack;
+  size_t off_args, off_fblk, off_rblk, off_r2blk, off_ckpt, off_pp, off_seg, off_lam, off_pblk,
+      off_pack;
 };
 
 // Workspace of a subcycle of n substeps with a checkpoint every ce, for nm
@@ -976,13 +1026,15 @@ AdjLayout adj_layout(const Geom& g, int n, int ce, int nm = 1) {
   L.off_args = 0;
   L.off_fblk = align256(sizeof(AdjArgs<T>));
   L.off_rblk = L.off_fblk + (size_t)n * L.blk;
-  L.off_ckpt = L.off_rblk + (size_t)n * L.blk;
+  L.off_r2blk = L.off_rblk + (size_t)n * L.blk;  // two-substep recompute blocks
+  L.off_ckpt = L.off_r2blk + (size_t)n * L.blk;
   // checkpoint slots 1 .. nck-1, slot nck = copy of the final state (slot 0 unused)
   L.off_pp = L.off_ckpt + (size_t)5 * (L.nck + 1) * L.fb;
   L.off_seg = L.off_pp + (size_t)10 * L.fb;
-  // two sets of segment slots (slot 0 of each unused): the recompute of one
-  // segment overlaps the reverse sweep of the next
-  L.off_lam = L.off_seg + (size_t)2 * 5 * (ce + 1) * L.fb;
+  // three sets of segment slots (slot 0 of each unused): two alternate so that
+  // the recompute of one segment overlaps the reverse sweep of the next, the
+  // third keeps the last segment's states from the forward
+  L.off_lam = L.off_seg + (size_t)3 * 5 * (ce + 1) * L.fb;
   // argument block of the pack kernel, then the packed step constants of the
   // forward and recompute launches
   L.off_pblk = L.off_lam + (size_t)10 * L.fb;
@@ -998,10 +1050,12 @@ T* wsf(void* ws, size_t off, size_t fb, int q) {
 }
 
 // Tile shape of the reverse kernel (index into kAdjTiles; environment
-// VERIS_ADJ_TILE overrides the choice): 24 x 32, the fastest measured where
-// latency bounds the kernel (1 degree, 60k cells, either precision); in FP32
-// from 200k cells on, 32 x 32, whose smaller ring of recomputed cells pays
-// once the launch fills the GPU (8 members of 240 x 240: 6% faster).
+// VERIS_ADJ_TILE overrides the choice): 20 x 32, the fastest measured where
+// latency bounds the kernel (1 degree, 60k cells, either precision: 217
+// workgroups where 24 x 32 gives 161, 3% faster); in FP32 from 200k cells
+// on, 32 x 32, whose smaller ring of recomputed cells pays once the launch
+// fills the GPU (8 members of 240 x 240: 6% faster).  The extra-branch
+// instantiations run 16 x 32 (FP64) or 24 x 32 (FP32) for either small shape.
 template <typename T>
 int adj_tile_cfg(size_t cells) {
   static int env = -2;
@@ -1011,7 +1065,7 @@ int adj_tile_cfg(size_t cells) {
     env = (v >= 0 && v < kAdjNumTiles) ? v : -1;
   }
   if (env >= 0) return env;
-  return (sizeof(T) == 4 && cells >= 200000) ? 3 : 1;
+  return (sizeof(T) == 4 && cells >= 200000) ? 3 : 4;
 }
 
 // Whether the recompute of a segment runs on a second stream beside the
@@ -1050,8 +1104,10 @@ int side_stream(SideStream* out) {
   return VH_OK;
 }
 
+// Single-substep shape of the forward and recompute launches: the requested
+// one when it has k = 1, else the default.
 int pick_adj_tile(const Geom& g, int req) {
-  if (req > 0) return req;
+  if (req > 0 && kEvpTiles[req].k == 1) return req;
   return (long long)g.NX * g.NY < 400000 ? 5 : 3;
 }
 
@@ -1059,8 +1115,8 @@ int check_adj(const vh_domain* d, const vh_settings* s, const vh_fields* f, int 
   VH_REQUIRE(d && s && f, VH_EINVAL, "null domain, settings or fields");
   VH_REQUIRE(n >= 1 && ce >= 1, VH_EINVAL, "n_substeps=%d ckpt_every=%d", n, ce);
   VH_REQUIRE(n <= (1 << 20), VH_EINVAL, "n_substeps=%d too large", n);
-  VH_REQUIRE(d->tile >= 0 && d->tile < kEvpNumTiles && (d->tile == 0 || kEvpTiles[d->tile].k == 1),
-             VH_EINVAL, "adjoint: tile %d (needs a single-substep tile shape or 0)", d->tile);
+  VH_REQUIRE(d->tile >= 0 && d->tile < kEvpNumTiles && (d->tile == 0 || kEvpTiles[d->tile].k <= 2),
+             VH_EINVAL, "adjoint: tile %d (needs a shape of one or two substeps, or 0)", d->tile);
   VH_REQUIRE(s->useEVP != 0.0, VH_EUNSUPPORTED, "adjoint: the EVP solver must be on");
 
   VH_REQUIRE(s->basalDragK2 == 0.0 || (f->R_low && f->Area && f->hIceMean), VH_EINVAL,
@@ -1093,16 +1149,21 @@ EvpArgBlock<T> proto_block(const Geom& g, const vh_settings* s, const vh_fields*
 
 // Packs the step constants into the workspace (evp_pack_kernel) and returns
 // the prototype block of the forward launches, pointing at them; the blocks
-// take the pack kernel's outcome from the returned device block.
+// take the pack kernel's outcome from the returned device block.  With
+// `pack` false the packed constants and their outcome are the ones the
+// checkpointed forward left in the same workspace (the reverse sweep runs on
+// the forward's fields).
 template <typename T>
 EvpArgBlock<T> packed_proto(const Geom& g, const vh_settings* s, const vh_fields* f,
                             const AdjLayout& L, void* ws, const EvpArgBlock<T>** pack_status,
-                            int nm, hipStream_t st) {
+                            int nm, hipStream_t st, bool pack = true) {
   EvpArgBlock<T> proto = proto_block<T>(g, s, f);
   evp_pack_attach(&proto, (char*)ws + L.off_pack, field_elems(g) * (size_t)nm, g.NY * nm);
   EvpArgBlock<T>* pblk = (EvpArgBlock<T>*)((char*)ws + L.off_pblk);
-  hipLaunchKernelGGL(evp_write_args_kernel<T>, dim3(1), dim3(64), 0, st, pblk, proto);
-  evp_pack_launch(pblk, (long long)field_elems(g), nm, st);
+  if (pack) {
+    hipLaunchKernelGGL(evp_write_args_kernel<T>, dim3(1), dim3(64), 0, st, pblk, proto);
+    evp_pack_launch(pblk, (long long)field_elems(g), nm, st);
+  }
   *pack_status = pblk;
   return proto;
 }
@@ -1124,7 +1185,28 @@ BlockPlan<T> block_plan(const AdjLayout& L, const vh_fields* f, void* const outs
   p.seg_stride = (size_t)5 * (L.ce + 1) * L.fb;
   p.n = L.n;
   p.ce = L.ce;
+  p.kstep = 1;
   return p;
+}
+
+// Shape of the two-substep forward and recompute launches of the adjoint
+// (a k = 2 shape), or 0 when they run one substep per launch.  Two substeps
+// per launch need the default settings (the default reverse kernel reads only
+// the velocities of the states, which two-substep launches then give bit for
+// bit) and an even checkpoint interval and subcycle; they run when the caller
+// names a k = 2 shape in vh_domain.tile (stating that vh_evp_overlaps_periodic
+// holds for the inputs) or, with tile 0, on a domain without a filled overlap
+// (20 x 32 x 2 below 400k cells, where one launch is latency bound, 32 x 32 x 2
+// above).  Environment VERIS_ADJ_PAIRS=0 (read at every call) turns them off.
+// Returns -1 for a k = 2 request that the settings or sizes rule out.
+int adj_pair_cfg(const Geom& g, int tile, bool plain, int n, int ce, size_t cells) {
+  const char* e = getenv("VERIS_ADJ_PAIRS");
+  const bool requested = tile > 0 && kEvpTiles[tile].k == 2;
+  if (e && atoi(e) == 0) return 0;
+  if (!plain || n % 2 != 0 || ce % 2 != 0) return requested ? -1 : 0;
+  if (requested) return tile;
+  if (tile != 0 || g.fill_x || g.fill_y) return 0;
+  return cells < 400000 ? 18 : 9;
 }
 
 void tiles_of(int cfg, const Geom& g, int* n_tiles, int* tiles_j) {
@@ -1168,24 +1250,36 @@ int forward_ckpt(const Geom& g, const vh_domain* d, const vh_settings* s, const 
   AdjArgs<T> unused{};
   const EvpArgBlock<T>* pack_status = nullptr;
   const EvpArgBlock<T> proto = packed_proto<T>(g, s, f, L, ws, &pack_status, nm, st);
+  const bool so = s->noSlip != 0.0 && s->secondOrderBC != 0.0;
+  const bool plain = evp_plain_kernels_enabled() && evp_plain_settings(proto.sc);
+  // two substeps per launch when the adjoint allows it (adj_pair_cfg)
+  const int pcfg = adj_pair_cfg(g, d->tile, plain, n, ce, field_elems(g) * (size_t)nm);
+  VH_REQUIRE(pcfg >= 0, VH_EUNSUPPORTED,
+             "adjoint: tile %d runs two substeps per launch, which needs the default settings "
+             "and even n_substeps (%d) and ckpt_every (%d)", d->tile, n, ce);
+  BlockPlan<T> plan = block_plan<T>(L, f, outs, ws);
+  if (pcfg) plan.kstep = 2;
   hipLaunchKernelGGL(adj_write_blocks_kernel<T>, dim3((n + 63) / 64), dim3(64), 0, st, fblk, L.blk,
-                     proto, block_plan<T>(L, f, outs, ws), 0, (AdjArgs<T>*)nullptr, unused,
-                     pack_status);
+                     proto, plan, pcfg ? 2 : 0, (AdjArgs<T>*)nullptr, unused, pack_status);
   VH_TRY_HIP(hipGetLastError());
   T* final_slot = wsf<T>(ws, L.off_ckpt, L.fb, 5 * L.nck);
   const int n_all = (int)(field_elems(g) * (size_t)nm);
-  const bool so = s->noSlip != 0.0 && s->secondOrderBC != 0.0;
+  int p_tiles = 0, p_tiles_j = 0;
+  if (pcfg) tiles_of(pcfg, g, &p_tiles, &p_tiles_j);
+  const int n_launch = pcfg ? n / 2 : n;
   auto record = [&](hipStream_t cs) {
-    for (int kk = 0; kk < n; ++kk)
-      evp_launch_single<T, double>(
-          cfg, (const EvpArgBlock<T>*)((char*)fblk + (size_t)kk * L.blk), n_tiles, tiles_j, cs,
-          nm, so);
+    for (int kk = 0; kk < n_launch; ++kk) {
+      const EvpArgBlock<T>* b = (const EvpArgBlock<T>*)((char*)fblk + (size_t)kk * L.blk);
+      if (pcfg) evp_launch_pair<T>(pcfg, b, p_tiles, p_tiles_j, cs, nm);
+      else evp_launch_single<T, double>(cfg, b, n_tiles, tiles_j, cs, nm, so, plain);
+    }
     hipLaunchKernelGGL(adj_keep_final_kernel<T>, dim3((unsigned)((n_all + BLK - 1) / BLK)),
                        dim3(BLK), 0, cs,
-                       (const EvpArgBlock<T>*)((char*)fblk + (size_t)(n - 1) * L.blk), n_all,
-                       final_slot, L.fb);
+                       (const EvpArgBlock<T>*)((char*)fblk + (size_t)(n_launch - 1) * L.blk),
+                       n_all, final_slot, L.fb);
   };
-  return run_recorded<T>("adjf", g, L, cfg * 2 + (so ? 1 : 0), nm, ws, record, st);
+  return run_recorded<T>("adjf", g, L, ((cfg * 2 + (so ? 1 : 0)) * 2 + (plain ? 1 : 0)) * 32 + pcfg,
+                         nm, ws, record, st);
 }
 
 template <typename T>
@@ -1223,10 +1317,25 @@ int adjoint(const Geom& g, const vh_domain* d, const vh_settings* s, const vh_fi
   AdjArgs<T>* a_dev = (AdjArgs<T>*)((char*)ws + L.off_args);
   EvpArgBlock<T>* rblk = (EvpArgBlock<T>*)((char*)ws + L.off_rblk);
   const EvpArgBlock<T>* pack_status = nullptr;
-  const EvpArgBlock<T> proto = packed_proto<T>(g, s, f, L, ws, &pack_status, nm, st);
+  const EvpArgBlock<T> proto = packed_proto<T>(g, s, f, L, ws, &pack_status, nm, st, false);
   hipLaunchKernelGGL(adj_write_blocks_kernel<T>, dim3((n + 63) / 64), dim3(64), 0, st, rblk, L.blk,
                      proto, block_plan<T>(L, f, nullptr, ws), 1, a_dev, av, pack_status);
   VH_TRY_HIP(hipGetLastError());
+  const bool plain = evp_plain_kernels_enabled() && evp_plain_settings(av.sc);
+  // the recompute runs two substeps per launch when the adjoint allows it
+  const int pcfg = adj_pair_cfg(g, d->tile, plain, n, ce, field_elems(g) * (size_t)nm);
+  VH_REQUIRE(pcfg >= 0, VH_EUNSUPPORTED,
+             "adjoint: tile %d runs two substeps per launch, which needs the default settings "
+             "and even n_substeps (%d) and ckpt_every (%d)", d->tile, n, ce);
+  EvpArgBlock<T>* r2blk = (EvpArgBlock<T>*)((char*)ws + L.off_r2blk);
+  if (pcfg) {
+    hipLaunchKernelGGL(adj_write_blocks_kernel<T>, dim3((n + 63) / 64), dim3(64), 0, st, r2blk,
+                       L.blk, proto, block_plan<T>(L, f, nullptr, ws), 3, (AdjArgs<T>*)nullptr,
+                       av, pack_status);
+    VH_TRY_HIP(hipGetLastError());
+  }
+  int p_tiles = 0, p_tiles_j = 0;
+  if (pcfg) tiles_of(pcfg, g, &p_tiles, &p_tiles_j);
 
   const int cfg = pick_adj_tile(g, d->tile);
   int n_tiles, tiles_j;
@@ -1258,10 +1367,16 @@ int adjoint(const Geom& g, const vh_domain* d, const vh_settings* s, const vh_fi
   if (rc) return rc;
   auto recompute = [&](int sgm, hipStream_t cs) {
     const int k0 = sgm * ce, len = std::min(ce, n - k0);
+    if (pcfg) {  // states jj+1 and jj+2 from state jj, up to state len-1
+      for (int jj = 0; jj + 1 < len; jj += 2)
+        evp_launch_pair<T>(pcfg, (const EvpArgBlock<T>*)((char*)r2blk + (size_t)(k0 + jj) * L.blk),
+                           p_tiles, p_tiles_j, cs, nm);
+      return;
+    }
     for (int jj = 0; jj + 1 < len; ++jj)
       evp_launch_single<T, double>(
           cfg, (const EvpArgBlock<T>*)((char*)rblk + (size_t)(k0 + jj) * L.blk), n_tiles, tiles_j,
-          cs, nm, sobc);
+          cs, nm, sobc, plain);
   };
   // Segment sgm - 1 is recomputed on a second stream while segment sgm is
   // reversed (they use different slot sets); the fork follows the reverse of
@@ -1274,7 +1389,7 @@ int adjoint(const Geom& g, const vh_domain* d, const vh_settings* s, const vh_fi
       init.out[q] = lam[cur][q];
     }
     hipLaunchKernelGGL(adj_init_kernel<T>, grid, dim3(BLK), 0, cs, a_dev, init, n_all);
-    recompute(L.nck - 1, cs);
+    // the last segment's states are where the checkpointed forward left them
     for (int sgm = L.nck - 1; sgm >= 0; --sgm) {
       if (sgm >= 1 && overlap) {
         hipEventRecord(side.fork, cs);
@@ -1295,8 +1410,10 @@ int adjoint(const Geom& g, const vh_domain* d, const vh_settings* s, const vh_fi
   };
   // the key holds every choice baked into the recorded launches: shapes,
   // overlap and the extra-branch instantiation
-  return run_recorded<T>("adjr", g, L, ((cfg * 16 + acfg) * 2 + (overlap ? 1 : 0)) * 4 + sd, nm,
-                         ws, record, st);
+  return run_recorded<T>(
+      "adjr", g, L,
+      ((((cfg * 16 + acfg) * 2 + (overlap ? 1 : 0)) * 4 + sd) * 2 + (plain ? 1 : 0)) * 32 + pcfg,
+      nm, ws, record, st);
 }
 
 template <typename T>

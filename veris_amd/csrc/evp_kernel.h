@@ -145,6 +145,9 @@ struct EvpArgBlock {
   // of equal members packed back to back (launch grid y = member; 0 or any
   // value for one domain, whose grid has y = 1)
   long long mstride;
+  // two-substep launches: where the state after the first substep goes (every
+  // cell a launch owns; null: not kept)
+  S* mid[5];
 };
 
 
@@ -466,10 +469,14 @@ __global__ __launch_bounds__(RI* RJ) void evp_substep_kernel(const EvpArgBlock<T
   const int bid = blockIdx.x;
   const int xcd = bid & 7, k0 = bid >> 3;
   const bool packed = ab->packed != 0;  // uniform: step constants in packed form
-  // the tile body is instantiated twice, for packed step constants and for
-  // full fields (pk: std::true_type / std::false_type), each with its own
-  // schedule and registers; the packed flag of the argument block picks one
-  auto run_tile = [&](const int tile, auto pk) {
+  // placement of a tile: its member's geometry and element offset, the row
+  // offset of its metric rows, its origin and its index
+  struct Place {
+    Geom g;
+    long long off;
+    int roff, i0, j0, tile;
+  };
+  auto place = [&](const int tile) -> Place {
     Geom g;
     long long off = 0;
     int roff = 0;
@@ -523,11 +530,80 @@ __global__ __launch_bounds__(RI* RJ) void evp_substep_kernel(const EvpArgBlock<T
       i0 = ti * TI;
       j0 = (tile - ti * tiles_j) * TJ;
     }
-    const int nl = ab->nlaunch;
-    const S* const* src = (launch == 0) ? ab->in
-                                        : (((nl - launch) % 2 == 0) ? (const S* const*)ab->out
-                                                                    : (const S* const*)ab->alt);
-    S* const* dst = ((nl - 1 - launch) % 2 == 0) ? ab->out : ab->alt;
+    return Place{g, off, roff, i0, j0, tile};
+  };
+  constexpr int RPT = (kEvpRowFields * RJ + NT - 1) / NT;  // metric-row loads per thread
+  // global loads of one cell of a tile with packed step constants, issued ahead
+  // of the tile's computation by the pipelined loop below
+  struct Pre {
+    A rowv[RPT];
+    A u, v, uoc, voc;
+    unsigned pkb;
+    A P, s1, s2, fc, mC, s12, mU, mV, AW, AS, wfx, wfy, uNm1, vNm1;
+  };
+  const int nl = ab->nlaunch;
+  const S* const* src = (launch == 0) ? ab->in
+                                      : (((nl - launch) % 2 == 0) ? (const S* const*)ab->out
+                                                                  : (const S* const*)ab->alt);
+  S* const* dst = ((nl - 1 - launch) % 2 == 0) ? ab->out : ab->alt;
+  const int t = threadIdx.x;
+  const int ar = t / RJ, br = t & (RJ - 1);  // region coordinates
+  // padded position of this thread's cell in a tile, and its element offset
+  // (member offset + folded index) into every packed field
+  auto cell_of = [&](const Place& pl, int& i, int& j, int& gi, int& gj, unsigned& ce) {
+    i = pl.i0 + ar - R;
+    j = pl.j0 + br - R;
+    gi = map_axis<K>(i, pl.g.NX, pl.g.nx, pl.g.fill_x);
+    gj = map_axis<K>(j, pl.g.NY, pl.g.ny, pl.g.fill_y);
+    ce = (unsigned)pl.off + (unsigned)(gi * pl.g.NY + gj);
+  };
+  // the loads of Pre for tile placement pl, in the order run_tile issues them
+  auto preload = [&](const Place& pl) -> Pre {
+    int i, j, gi, gj;
+    unsigned ce;
+    cell_of(pl, i, j, gi, gj, ce);
+    const EvpConst<T, S>& f = ab->c;
+    Pre q;
+#pragma unroll
+    for (int r = 0; r < RPT; ++r) {
+      const int e = t + r * NT;
+      q.rowv[r] = (A)0;
+      if (e < kEvpRowFields * RJ) {
+        const int mq = e / RJ, bq = e & (RJ - 1);
+        const int gjq = map_axis<K>(pl.j0 + bq - R, pl.g.NY, pl.g.ny, pl.g.fill_y);
+        q.rowv[r] = (A)ldg(ab->rows + (long long)mq * ab->rstride + pl.roff, (unsigned)gjq);
+      }
+    }
+    q.u = ldg(src[0], ce);
+    q.v = ldg(src[1], ce);
+    q.uoc = ldg(f.uOc, ce);
+    q.voc = ldg(f.vOc, ce);
+    q.pkb = ldg(ab->pk, ce);
+    q.P = ldg(f.P, ce);
+    q.s1 = ldg(src[2], ce);
+    q.s2 = ldg(src[3], ce);
+    q.fc = ldg(f.fC, ce);
+    q.mC = ldg(f.massC, ce);
+    q.s12 = ldg(src[4], ce);
+    q.mU = ldg(f.massU, ce);
+    q.mV = ldg(f.massV, ce);
+    q.AW = ldg(f.AreaW, ce);
+    q.AS = ldg(f.AreaS, ce);
+    q.wfx = ldg(f.WFX, ce);
+    q.wfy = ldg(f.WFY, ce);
+    q.uNm1 = ldg(f.uNm1, ce);
+    q.vNm1 = ldg(f.vNm1, ce);
+    return q;
+  };
+  // the tile body is instantiated for packed step constants and for full
+  // fields (pk: std::true_type / std::false_type), each with its own schedule
+  // and registers, the packed flag of the argument block picks one; pf
+  // (std::true_type) takes the cell's loads from `pre` instead of issuing them
+  auto run_tile = [&](const Place& pl, auto pk, auto pf, const Pre& pre) {
+    constexpr bool PF = decltype(pf)::value;
+    const Geom& g = pl.g;
+    const int roff = pl.roff, i0 = pl.i0, j0 = pl.j0;
+    const long long off = pl.off;
     const EvpConst<T, S>& f = ab->c;
     const EvpScalars& sc = ab->sc;
     // settings branches; PL instantiates the default settings (none of them,
@@ -536,17 +612,14 @@ __global__ __launch_bounds__(RI* RJ) void evp_substep_kernel(const EvpArgBlock<T
     const bool f_adaptive = !PL && sc.adaptive, f_sidedrag = !PL && sc.sidedrag;
     const bool f_coast = !PL && sc.coast, f_tens0 = PL || sc.tens0;
 
-    const int t = threadIdx.x;
-    const int ar = t / RJ, br = t & (RJ - 1);  // region coordinates
-    const int i = i0 + ar - R, j = j0 + br - R;  // padded position
-    const int gi = map_axis<K>(i, g.NX, g.nx, g.fill_x);
-    const int gj = map_axis<K>(j, g.NY, g.ny, g.fill_y);
-    const int c = gi * g.NY + gj;  // member-local element index (every region cell is in bounds)
-    // element offsets into the packed fields fit 32 bits (the host checks the
-    // batch size), so every load below uses the field's uniform base pointer
-    // plus one per-lane 32-bit offset shared by all fields
+    // padded position; member-local element index gi * NY + gj (every region
+    // cell is in bounds).  Element offsets into the packed fields fit 32 bits
+    // (the host checks the batch size), so every load below uses the field's
+    // uniform base pointer plus one per-lane 32-bit offset shared by all fields
+    int i, j, gi, gj;
+    unsigned ce;
+    cell_of(pl, i, j, gi, gj, ce);
     const unsigned oo = (unsigned)off;
-    const unsigned ce = oo + (unsigned)c;
 
     // final output cell of this thread
     const bool own = (ar >= R) && (ar < R + TI) && (br >= R) && (br < R + TJ) && (i < g.NX) &&
@@ -560,9 +633,11 @@ __global__ __launch_bounds__(RI* RJ) void evp_substep_kernel(const EvpArgBlock<T
     constexpr bool packed = decltype(pk)::value;
     // metric rows of this tile's columns: issued first, so that stage A, which
     // stores them to LDS before its barrier, waits for no later load
-    constexpr int RPT = (kEvpRowFields * RJ + NT - 1) / NT;  // rows per thread
     A rowv[RPT];
-    if (packed) {
+    if constexpr (PF) {
+#pragma unroll
+      for (int r = 0; r < RPT; ++r) rowv[r] = pre.rowv[r];
+    } else if (packed) {
 #pragma unroll
       for (int r = 0; r < RPT; ++r) {
         const int q = t + r * NT;
@@ -574,11 +649,11 @@ __global__ __launch_bounds__(RI* RJ) void evp_substep_kernel(const EvpArgBlock<T
         }
       }
     }
-    A u = ldg(src[0] , ce), v = ldg(src[1] , ce);
-    const A uoc = ldg(f.uOc , ce), voc = ldg(f.vOc , ce);
+    A u = PF ? pre.u : (A)ldg(src[0], ce), v = PF ? pre.v : (A)ldg(src[1], ce);
+    const A uoc = PF ? pre.uoc : (A)ldg(f.uOc, ce), voc = PF ? pre.voc : (A)ldg(f.vOc, ce);
     A Mc, MUc, MVc, mInC, mInUc, mInVc;
     if (packed) {
-      const unsigned pkb = ldg(ab->pk , ce);
+      const unsigned pkb = PF ? pre.pkb : (unsigned)ldg(ab->pk, ce);
       Mc = (A)(pkb & 1u);
       MUc = (A)((pkb >> 1) & 1u);
       MVc = (A)((pkb >> 2) & 1u);
@@ -602,15 +677,15 @@ __global__ __launch_bounds__(RI* RJ) void evp_substep_kernel(const EvpArgBlock<T
       k2Z = ldg(f.k2Z , ce);
     }
     __builtin_amdgcn_sched_barrier(0);
-    const A P = ldg(f.P , ce);
-    A s1 = ldg(src[2] , ce), s2 = ldg(src[3] , ce);
-    const A fc = ldg(f.fC , ce);
-    const A mC = ldg(f.massC , ce);
-    A s12 = ldg(src[4] , ce);
-    const A mU = ldg(f.massU , ce), mV = ldg(f.massV , ce);
-    const A AW = ldg(f.AreaW , ce), AS = ldg(f.AreaS , ce);
-    const A wfx = ldg(f.WFX , ce), wfy = ldg(f.WFY , ce);
-    const A uNm1 = ldg(f.uNm1 , ce), vNm1 = ldg(f.vNm1 , ce);
+    const A P = PF ? pre.P : (A)ldg(f.P, ce);
+    A s1 = PF ? pre.s1 : (A)ldg(src[2], ce), s2 = PF ? pre.s2 : (A)ldg(src[3], ce);
+    const A fc = PF ? pre.fc : (A)ldg(f.fC, ce);
+    const A mC = PF ? pre.mC : (A)ldg(f.massC, ce);
+    A s12 = PF ? pre.s12 : (A)ldg(src[4], ce);
+    const A mU = PF ? pre.mU : (A)ldg(f.massU, ce), mV = PF ? pre.mV : (A)ldg(f.massV, ce);
+    const A AW = PF ? pre.AW : (A)ldg(f.AreaW, ce), AS = PF ? pre.AS : (A)ldg(f.AreaS, ce);
+    const A wfx = PF ? pre.wfx : (A)ldg(f.WFX, ce), wfy = PF ? pre.wfy : (A)ldg(f.WFY, ce);
+    const A uNm1 = PF ? pre.uNm1 : (A)ldg(f.uNm1, ce), vNm1 = PF ? pre.vNm1 : (A)ldg(f.vNm1, ce);
     if (f_kterms) {
       k1C = ldg(f.k1C , ce);
       k2C = ldg(f.k2C , ce);
@@ -883,6 +958,17 @@ __global__ __launch_bounds__(RI* RJ) void evp_substep_kernel(const EvpArgBlock<T
         u = un;  // other threads read sU/sV, never these registers
         v = vn;
       }
+      if constexpr (K == 2) {
+        // the intermediate state, for callers that need every substep's state
+        if (s == 0 && ab->mid[0] != nullptr && own) {
+          const unsigned cp = oo + (unsigned)(i * g.NY + j);
+          stg(ab->mid[0], cp, (S)u);
+          stg(ab->mid[1], cp, (S)v);
+          stg(ab->mid[2], cp, (S)s1);
+          stg(ab->mid[3], cp, (S)s2);
+          stg(ab->mid[4], cp, (S)s12);
+        }
+      }
     }
 
     // ---- residual norms of the substep (veris/evp_solver.py:192-230) over the
@@ -915,7 +1001,7 @@ __global__ __launch_bounds__(RI* RJ) void evp_substep_kernel(const EvpArgBlock<T
           a0 += sRed[2 * q];
           a1 += sRed[2 * q + 1];
         }
-        double* dp = ab->res + ((size_t)launch * n_tiles + tile) * 2;
+        double* dp = ab->res + ((size_t)launch * n_tiles + pl.tile) * 2;
         dp[0] = a0;
         dp[1] = a1;
       }
@@ -938,20 +1024,47 @@ __global__ __launch_bounds__(RI* RJ) void evp_substep_kernel(const EvpArgBlock<T
       stg(dst[4], oo + (unsigned)cp, (S)s12);
     }
   };
+  const Pre none{};
   if constexpr (BATCHED) {
     const int nwx = (int)(gridDim.x >> 3);
+    // FP32 arithmetic with packed constants under the default settings: a
+    // software pipeline whose next tile's loads are in flight while the current
+    // tile computes (the Pre registers fit beside the body's at one
+    // 1024-thread workgroup per compute unit; FP64 arithmetic would not)
+    constexpr bool PIPE = PL && sizeof(A) == 4;
+    if (PIPE && packed) {
+      int tile = xcd * per_xcd + k0;
+      if (k0 >= per_xcd || tile >= n_tiles) return;
+      Place pl = place(tile);
+      Pre cur = preload(pl);
+      for (int k = k0;; k += nwx) {
+        const int tn = xcd * per_xcd + k + nwx;
+        const bool more = k + nwx < per_xcd && tn < n_tiles;  // uniform
+        // the last tile prefetches itself again (in bounds, never used)
+        const Place pn = place(more ? tn : tile);
+        const Pre nxt = preload(pn);
+        __builtin_amdgcn_sched_barrier(0);
+        if (k != k0) __syncthreads();  // the previous tile's last stage still reads LDS
+        run_tile(pl, std::true_type{}, std::true_type{}, cur);
+        if (!more) break;
+        tile = tn;
+        pl = pn;
+        cur = nxt;
+      }
+      return;
+    }
     for (int k = k0; k < per_xcd; k += nwx) {
       const int tile = xcd * per_xcd + k;
       if (tile >= n_tiles) break;
       if (k != k0) __syncthreads();  // the previous tile's last stage still reads LDS
-      if (packed) run_tile(tile, std::true_type{});
-      else run_tile(tile, std::false_type{});
+      if (packed) run_tile(place(tile), std::true_type{}, std::false_type{}, none);
+      else run_tile(place(tile), std::false_type{}, std::false_type{}, none);
     }
   } else {
     const int tile = xcd * per_xcd + k0;
     if (tile < n_tiles) {
-      if (packed) run_tile(tile, std::true_type{});
-      else run_tile(tile, std::false_type{});
+      if (packed) run_tile(place(tile), std::true_type{}, std::false_type{}, none);
+      else run_tile(place(tile), std::false_type{}, std::false_type{}, none);
     }
   }
 }
@@ -1104,18 +1217,42 @@ inline void evp_launch_cfg(int cfg, const EvpArgBlock<T, S>* ab, int which, int 
   });
 }
 
+// Single-domain launch of the default-settings instantiation of shape `cfg`
+// (a shape with k = 2; anything else launches nothing) reading its arguments
+// from `ab` (nlaunch = 1: in -> out, the state after the first substep to
+// ab->mid when set), for n_members equal members packed back to back.
+template <typename T>
+inline void evp_launch_pair(int cfg, const EvpArgBlock<T>* ab, int n_tiles, int tiles_j,
+                            hipStream_t st, int n_members = 1) {
+  const int per_xcd = (n_tiles + 7) / 8;
+  evp_with_shape(cfg, [&](auto sh) {
+    using Sh = decltype(sh);
+    if constexpr (Sh::k == 2)
+      hipLaunchKernelGGL(
+          (evp_substep_kernel<T, double, Sh::ri, Sh::rj, 2, false, T, false, false, T, true>),
+          dim3(per_xcd * 8, n_members), dim3(Sh::ri * Sh::rj), 0, st, ab, 0, 0, n_tiles, tiles_j,
+          per_xcd);
+  });
+}
+
 // Single-domain, single-substep launch of shape `cfg` (a shape with k = 1)
 // reading its arguments from `ab` (nlaunch = 1: in -> out), for n_members
-// equal members packed back to back (ab->mstride apart; 1 = one domain).
+// equal members packed back to back (ab->mstride apart; 1 = one domain);
+// `so` and `plain` select the instantiation as for evp_launch_cfg.
 template <typename T, typename TD>
 inline void evp_launch_single(int cfg, const EvpArgBlock<T>* ab, int n_tiles, int tiles_j,
-                              hipStream_t st, int n_members = 1, bool so = false) {
+                              hipStream_t st, int n_members = 1, bool so = false,
+                              bool plain = false) {
   const int per_xcd = (n_tiles + 7) / 8;
   evp_with_shape(cfg, [&](auto sh) {
     using S = decltype(sh);
     constexpr int RI = S::k == 1 ? S::ri : 16, RJ = S::k == 1 ? S::rj : 32;
     if (so)  // the second-order boundary term's instantiation
       hipLaunchKernelGGL((evp_substep_kernel<T, TD, RI, RJ, 1, false, T, true>),
+                         dim3(per_xcd * 8, n_members), dim3(RI * RJ), 0, st, ab, 0, 0, n_tiles,
+                         tiles_j, per_xcd);
+    else if (plain)  // the default settings' instantiation
+      hipLaunchKernelGGL((evp_substep_kernel<T, TD, RI, RJ, 1, false, T, false, false, T, true>),
                          dim3(per_xcd * 8, n_members), dim3(RI * RJ), 0, st, ab, 0, 0, n_tiles,
                          tiles_j, per_xcd);
     else

@@ -228,6 +228,18 @@ def pre_evp(state, pStar: torch.Tensor | None = None):
 EVP_TILE = {}
 
 
+def overlaps_periodic(state) -> bool:
+    """Whether multi-substep EVP tile shapes give the single-substep results
+    on ``state`` (vh_evp_overlaps_periodic): every field they read at
+    neighbouring cells holds, in each filled overlap, the value the fill
+    copies there.  True for every domain without a filled overlap.  Waits for
+    the device."""
+    c = Ctx.of(state)
+    flag = torch.zeros((), dtype=torch.int32, device=c.device)
+    N.call("vh_evp_overlaps_periodic", *c.args(), flag.data_ptr(), c.stream)
+    return bool(flag.item())
+
+
 def single_substep_ok(settings, tile: int) -> bool:
     """False when ``settings`` need single-substep shapes (the second-order
     no-slip boundary term reads two cells beyond the face) and ``tile`` is a
