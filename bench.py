@@ -642,15 +642,15 @@ def bench_hires(dev) -> dict:
     n_adj = int(ens.settings.nEVPsteps)
     ps = torch.tensor(float(ens.settings.pStar), dtype=torch.float64, device=dev)
 
-    def batched_adjoint():
+    def batched_adjoint(ckpt):
         h = ens.fields["hIceMean"].detach().clone().requires_grad_(True)
         p_ = ps.clone().requires_grad_(True)
-        outs = A.evp_subcycle_batched(ens, pStar=p_, hIceMean=h, n_substeps=n_adj, ckpt_every=10)
+        outs = A.evp_subcycle_batched(ens, pStar=p_, hIceMean=h, n_substeps=n_adj, ckpt_every=ckpt)
         ((outs[0].double() ** 2).sum() + (outs[1].double() ** 2).sum()).backward()
 
-    def batched_ckpt_forward():
+    def batched_ckpt_forward(ckpt):
         with torch.no_grad():
-            A.evp_subcycle_batched(ens, n_substeps=n_adj, ckpt_every=10)
+            A.evp_subcycle_batched(ens, n_substeps=n_adj, ckpt_every=ckpt)
 
     out_b = None
     from veris_amd.autotune import tune_batched
@@ -662,19 +662,30 @@ def bench_hires(dev) -> dict:
         ens.pre_evp()
         out_b = ens.evp(n_adj, tile=tile_b, out=out_b)
 
-    for fn in (batched_adjoint, batched_ckpt_forward, batched_forward):
-        fn()
-    t_adj = _sync_time(batched_adjoint, 5)
-    t_ck = _sync_time(batched_ckpt_forward, 5)
+    # every state kept (one segment: nothing recomputed; 8 x 244 x 244 FP32
+    # states take 9.5 MB each, 1.1 GB for the subcycle) and a checkpoint every
+    # 10 substeps (each segment recomputed from its checkpoint)
+    t = {}
+    for ckpt in (n_adj, 10):
+        for fn in (lambda: batched_adjoint(ckpt), lambda: batched_ckpt_forward(ckpt)):
+            fn()
+        t[ckpt] = (_sync_time(lambda: batched_adjoint(ckpt), 5),
+                   _sync_time(lambda: batched_ckpt_forward(ckpt), 5))
+    batched_forward()
     t_fwd = _sync_time(batched_forward, 5)
+    t_adj, t_ck = t[n_adj]
     res["batched_adjoint"] = {
         "workload": f"8 regional members 240x240 (10-cell thickness noise), FP32, {n_adj} EVP "
-                    "substeps, checkpoint every 10, gradient of a velocity norm w.r.t. P* and "
-                    "hIceMean; forward and reverse each one batched launch sequence",
+                    "substeps, gradient of a velocity norm w.r.t. P* and hIceMean; forward "
+                    "and reverse each one batched launch sequence; every state kept (1.1 GB "
+                    "of HBM, nothing recomputed), and beside it a checkpoint every 10 "
+                    "substeps",
         "forward_backward_ms": t_adj * 1e3, "checkpointed_forward_ms": t_ck * 1e3,
         "plain_forward_ms": t_fwd * 1e3, "plain_forward_tile": tile_b,
         "adjoint_over_plain_forward": t_adj / t_fwd,
-        "adjoint_over_checkpointed_forward": t_adj / t_ck}
+        "adjoint_over_checkpointed_forward": t_adj / t_ck,
+        "ckpt10": {"forward_backward_ms": t[10][0] * 1e3, "checkpointed_forward_ms": t[10][1] * 1e3,
+                   "adjoint_over_plain_forward": t[10][0] / t_fwd}}
     return res
 
 

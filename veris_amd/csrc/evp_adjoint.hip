@@ -96,7 +96,8 @@ __device__ inline T* slot(const BlockPlan<T>& p, size_t off, int idx) {
 // other is reversed).
 template <typename T>
 __device__ inline size_t seg_set(const BlockPlan<T>& p, int sgm) {
-  return p.off_seg + (size_t)(sgm == p.nck - 1 ? 2 : sgm % 2) * p.seg_stride;
+  const int set = sgm != p.nck - 1 ? sgm % 2 : (p.nck == 1 ? 0 : 2);
+  return p.off_seg + (size_t)set * p.seg_stride;
 }
 
 // forward location of field q of the state after k substeps: the states
@@ -268,13 +269,15 @@ __device__ inline T gather_filled(const Geom& g, const T* a, int i, int j) {
 // MODE 0 is the default instantiation, 1 the extra branches, 2 the extra
 // branches and the second-order no-slip boundary term
 // (veris/dynamics_routines.py:177-215): its stencil reaches two cells along
-// the face normal, so MODE 2 recomputes one more ring of cells (R = 5).
+// the face normal, so MODE 2 recomputes one more ring of cells (R = 5);
+// MODE 3 is MODE 0 under the default settings (no turning angle, no metric
+// terms, tensileStrFac = 0) with those runtime branches compiled out.
 template <typename T, typename TD, int RI, int RJ, int MODE>
 __global__ __launch_bounds__(RI* RJ) void adj_substep_kernel(const AdjArgs<T>* __restrict__ a,
                                                              const EvpArgBlock<T>* __restrict__ blk,
                                                              LamSet<T> L, int n_tiles, int tiles_j,
                                                              int per_xcd) {
-  constexpr bool XB = MODE >= 1, SO = MODE == 2;
+  constexpr bool XB = MODE == 1 || MODE == 2, SO = MODE == 2, PLN = MODE == 3;
   constexpr int R = SO ? 5 : 4, TI = RI - 2 * R, TJ = RJ - 2 * R, NT = RI * RJ;
   static_assert((RJ & (RJ - 1)) == 0, "RJ must be a power of two");
   static_assert(TI >= 1 && TJ >= 1, "tile too small");
@@ -305,6 +308,7 @@ __global__ __launch_bounds__(RI* RJ) void adj_substep_kernel(const AdjArgs<T>* _
   const EvpConst<T>& k = a->c;
   const EvpScalars& sc = a->sc;
   const AdjAcc<T>& acc = a->acc;
+  const bool f_turn = !PLN && sc.turn, f_kterms = !PLN && sc.kterms, f_tens0 = PLN || sc.tens0;
 
   const int t = threadIdx.x;
   const int ar = t / RJ, br = t & (RJ - 1);
@@ -420,7 +424,7 @@ __global__ __launch_bounds__(RI* RJ) void adj_substep_kernel(const AdjArgs<T>* _
     sMV[t] = MVc;
     sA[t] = (u - uoc) * mInUc;
     sB[t] = (v - voc) * mInVc;
-    if (sc.turn) {
+    if (f_turn) {
       sFU[t] = uoc - u;
       sFV[t] = voc - v;
     }
@@ -437,7 +441,7 @@ __global__ __launch_bounds__(RI* RJ) void adj_substep_kernel(const AdjArgs<T>* _
       const T vavz = (v + vim) * (T)0.5;
       const T mz = (Mc * sM[t - RJ]) * (sM[t - 1] * sM[t - RJ - 1]);
       T inner = dudy + dvdx;
-      if (sc.kterms) inner = (inner - ldg(k.k1Z, c) * vavz) - ldg(k.k2Z, c) * uavz;
+      if (f_kterms) inner = (inner - ldg(k.k1Z, c) * vavz) - ldg(k.k2Z, c) * uavz;
       T e12 = ((T)0.5 * inner) * mz;
       if (!sdr) {  // no-slip boundary term
         const T hu = MUc - sMU[t - 1];
@@ -477,7 +481,7 @@ __global__ __launch_bounds__(RI* RJ) void adj_substep_kernel(const AdjArgs<T>* _
       const T rdxU = met(3), rdyV = met(4);
       const T dudx = (uip - u) * rdxU, dvdy = (vjp - v) * rdyV;
       T e11 = dudx, e22 = dvdy;
-      if (sc.kterms) {
+      if (f_kterms) {
         e11 = dudx + ((vjp + v) * (T)0.5) * ldg(k.k2C, c);
         e22 = dvdy + ((uip + u) * (T)0.5) * ldg(k.k1C, c);
       }
@@ -529,9 +533,9 @@ __global__ __launch_bounds__(RI* RJ) void adj_substep_kernel(const AdjArgs<T>* _
         cMa = mC > (T)1e-4 ? -dY * (Y / mmax) : (T)0;
         // the replacement pressure, for the sigma1 relaxation's alpha derivative
         T prep = (((T)2.0 * zeta) * dC) * (T)sc.pr;
-        if (!sc.tens0) prep = prep / (T)sc.tens1p;
+        if (!f_tens0) prep = prep / (T)sc.tens1p;
         prs = (P * (T)sc.pr1m) + prep;
-        if (!sc.tens0) prs = prs * (T)sc.tens1m;
+        if (!f_tens0) prs = prs * (T)sc.tens1m;
       }
       sa = sA[t] + sA[t + RJ];
       sb = sB[t] + sB[t + 1];
@@ -539,7 +543,7 @@ __global__ __launch_bounds__(RI* RJ) void adj_substep_kernel(const AdjArgs<T>* _
       sw = sqrt(ww);
       const T cd = ((D * D) * ww > (T)sc.cDragMin2) ? D * sw : (T)sc.cDragMin;
       sCD[t] = cd * Mc;
-      if (sc.turn) {  // (uOcean - u) summed over j, j+1 and (vOcean - v) over i, i+1
+      if (f_turn) {  // (uOcean - u) summed over j, j+1 and (vOcean - v) over i, i+1
         sAB[t] = sFU[t] + sFU[t + 1];
         sBB[t] = sFV[t] + sFV[t + RJ];
       }
@@ -611,7 +615,7 @@ __global__ __launch_bounds__(RI* RJ) void adj_substep_kernel(const AdjArgs<T>* _
       sCXU[t] = (nbU * uoc + dbU) * (AW * cw);
       sCXV[t] = (nbV * voc + dbV) * (AS * cw);
       T turnU = 0, turnV = 0;
-      if (sc.turn) {  // turning term of the forcing (veris/evp_solver.py:119-139)
+      if (f_turn) {  // turning term of the forcing (veris/evp_solver.py:119-139)
         const T sgs = sgn(fc) * (T)sc.sinWat;
         const T lmU = mU != (T)0 ? (T)1 : mU, lmV = mV != (T)0 ? (T)1 : mV;
         const T Xt = (sCD[t] * (T)0.5) * sAB[t] + (sCD[t - RJ] * (T)0.5) * sAB[t - RJ];
@@ -660,7 +664,7 @@ __global__ __launch_bounds__(RI* RJ) void adj_substep_kernel(const AdjArgs<T>* _
       const T Xc = sA[t], Xip = sA[t + RJ], Xjm = sA[t - 1];
       const T Yc = sB[t], Yjp = sB[t + 1], Yim = sB[t - RJ];
       T cdb = (T)0.5 * (sCXU[t] + sCXU[t + RJ]) + (T)0.5 * (sCXV[t] + sCXV[t + 1]);
-      if (sc.turn) {  // the drag of this c-point in the turning terms of its four faces
+      if (f_turn) {  // the drag of this c-point in the turning terms of its four faces
         // (sAB/sBB of this cell are read before this thread overwrites them)
         const T kus = sU[t] + sU[t + RJ], kvs = sV[t] + sV[t + 1];
         cdb = cdb + (T)0.25 * (T)sc.sinWat * (sBB[t] * kvs - sAB[t] * kus);
@@ -670,8 +674,8 @@ __global__ __launch_bounds__(RI* RJ) void adj_substep_kernel(const AdjArgs<T>* _
       const T fvb = (T)0.5 * (sQ[t] + sQ[t + RJ]);
       const T fub = -(T)0.5 * (sNV[t] + sNV[t + 1]);
       // neighbour velocities (sU/sV hold the turning factors when sinWat != 0)
-      const T uip = sc.turn ? ldg(xu, cip) : sU[t + RJ];
-      const T vjp = sc.turn ? ldg(xv, cjp) : sV[t + 1];
+      const T uip = f_turn ? ldg(xu, cip) : sU[t + RJ];
+      const T vjp = f_turn ? ldg(xv, cjp) : sV[t + 1];
       mCadd = fvb * fc * (T)0.5 * (v + vjp) + fub * fc * (T)0.5 * (u + uip);
       sFV[t] = fvb;
       sFU[t] = fub;
@@ -746,11 +750,11 @@ __global__ __launch_bounds__(RI* RJ) void adj_substep_kernel(const AdjArgs<T>* _
       sA[t] = (epb + emb) * met(3) * mInC;  // e11 adjoint (times its stencil factor)
       sB[t] = (epb - emb) * met(4) * mInC;  // e22 adjoint
       sQ[t] = dsqb * (T)sc.rpd * met(5);    // adjoint of rAz e12^2 at the four z-corners
-      if (sc.kterms) {  // e22 += uave k1C, e11 += vave k2C (veris/dynamics_routines.py:151-158)
+      if (f_kterms) {  // e22 += uave k1C, e11 += vave k2C (veris/dynamics_routines.py:151-158)
         sNV[t] = ((epb - emb) * mInC) * ldg(k.k1C, c) * (T)0.5;
         sCXU[t] = ((epb + emb) * mInC) * ldg(k.k2C, c) * (T)0.5;
       }
-      if (sc.turn) {
+      if (f_turn) {
         sU[t] = wU;
         sV[t] = wV;
       }
@@ -790,13 +794,13 @@ __global__ __launch_bounds__(RI* RJ) void adj_substep_kernel(const AdjArgs<T>* _
                ec * A1 + ejp * A2jp + (T)0.5 * fmc * sFU[t] + (T)0.5 * fmc_im * sFU[t - RJ];
     T lv_out = nbV * bVv * mV + (sBB[t] + sBB[t - 1]) * mInVc + (sB[t - 1] - sB[t]) +
                ec * A3 + eip * A4ip + (T)0.5 * fmc * sFV[t] + (T)0.5 * fmc_jm * sFV[t - 1];
-    if (sc.kterms) {  // the metric terms of e11, e22 (c) and e12 (z)
+    if (f_kterms) {  // the metric terms of e11, e22 (c) and e12 (z)
       const T k1Z = ldg(k.k1Z, c), k2Z = ldg(k.k2Z, c);
       const T k2Z_jp = ldg(k.k2Z, cjp), k1Z_ip = ldg(k.k1Z, cip);
       lu_out = lu_out + (sNV[t] + sNV[t - RJ]) - (T)0.25 * (ec * mzc * k2Z + ejp * mzjp * k2Z_jp);
       lv_out = lv_out + (sCXU[t] + sCXU[t - 1]) - (T)0.25 * (ec * mzc * k1Z + eip * mzip * k1Z_ip);
     }
-    if (sc.turn) {  // the ice velocity in the turning terms of the four neighbouring faces
+    if (f_turn) {  // the ice velocity in the turning terms of the four neighbouring faces
       lu_out = lu_out + (T)0.25 * (T)sc.sinWat * (sU[t] + sU[t - 1]);
       lv_out = lv_out - (T)0.25 * (T)sc.sinWat * (sV[t] + sV[t - RJ]);
     }
@@ -921,16 +925,21 @@ void adj_launch(int cfg, const AdjArgs<T>* a, const EvpArgBlock<T>* b, LamSet<T>
     }
     return;
   }
-  switch (cfg) {
-    case 1: adj_launch_shape<T, 24, 32, 0>(a, b, L, g, n_members, st); break;
-    case 2: adj_launch_shape<T, 12, 64, 0>(a, b, L, g, n_members, st); break;
-    case 4: adj_launch_shape<T, 20, 32, 0>(a, b, L, g, n_members, st); break;
-    case 3:  // 1024 threads: FP32 only (the FP64 kernel's registers do not fit)
-      if constexpr (sizeof(T) == 4) adj_launch_shape<T, 32, 32, 0>(a, b, L, g, n_members, st);
-      else adj_launch_shape<T, 24, 32, 0>(a, b, L, g, n_members, st);
-      break;
-    default: adj_launch_shape<T, 16, 32, 0>(a, b, L, g, n_members, st); break;
-  }
+  auto by_shape = [&](auto md) {
+    constexpr int M = decltype(md)::value;
+    switch (cfg) {
+      case 1: adj_launch_shape<T, 24, 32, M>(a, b, L, g, n_members, st); break;
+      case 2: adj_launch_shape<T, 12, 64, M>(a, b, L, g, n_members, st); break;
+      case 4: adj_launch_shape<T, 20, 32, M>(a, b, L, g, n_members, st); break;
+      case 3:
+        if constexpr (sizeof(T) == 4) adj_launch_shape<T, 32, 32, M>(a, b, L, g, n_members, st);
+        else adj_launch_shape<T, 24, 32, M>(a, b, L, g, n_members, st);
+        break;
+      default: adj_launch_shape<T, 16, 32, M>(a, b, L, g, n_members, st); break;
+    }
+  };
+  if (mode == 3) by_shape(std::integral_constant<int, 3>{});
+  else by_shape(std::integral_constant<int, 0>{});
 }
 
 // ---- pre-EVP adjoint: strength, mass, wind forcing
@@ -1010,7 +1019,7 @@ struct AdjLayout {
   size_t fb, blk, total;
   int nck, ce, n;
   size_t off_args, off_fblk, off_rblk, off_r2blk, off_ckpt, off_pp, off_seg, off_lam, off_pblk,
-      off_pack;
+      off_pack, seg_stride;
 };
 
 // Workspace of a subcycle of n substeps with a checkpoint every ce, for nm
@@ -1031,10 +1040,13 @@ AdjLayout adj_layout(const Geom& g, int n, int ce, int nm = 1) {
   // checkpoint slots 1 .. nck-1, slot nck = copy of the final state (slot 0 unused)
   L.off_pp = L.off_ckpt + (size_t)5 * (L.nck + 1) * L.fb;
   L.off_seg = L.off_pp + (size_t)10 * L.fb;
+  L.seg_stride = (size_t)5 * (std::min(ce, n) + 1) * L.fb;  // one set of segment slots
   // three sets of segment slots (slot 0 of each unused): two alternate so that
   // the recompute of one segment overlaps the reverse sweep of the next, the
-  // third keeps the last segment's states from the forward
-  L.off_lam = L.off_seg + (size_t)3 * 5 * (ce + 1) * L.fb;
+  // third keeps the last segment's states from the forward; a subcycle of one
+  // segment (ckpt_every >= n_substeps: every state kept, nothing recomputed)
+  // needs only that one
+  L.off_lam = L.off_seg + (size_t)(L.nck == 1 ? 1 : 3) * L.seg_stride;
   // argument block of the pack kernel, then the packed step constants of the
   // forward and recompute launches
   L.off_pblk = L.off_lam + (size_t)10 * L.fb;
@@ -1182,7 +1194,7 @@ BlockPlan<T> block_plan(const AdjLayout& L, const vh_fields* f, void* const outs
   p.nck = L.nck;
   p.off_pp = L.off_pp;
   p.off_seg = L.off_seg;
-  p.seg_stride = (size_t)5 * (L.ce + 1) * L.fb;
+  p.seg_stride = L.seg_stride;
   p.n = L.n;
   p.ce = L.ce;
   p.kstep = 1;
@@ -1365,6 +1377,11 @@ int adjoint(const Geom& g, const vh_domain* d, const vh_settings* s, const vh_fi
   SideStream side;
   int rc = side_stream(&side);
   if (rc) return rc;
+  // the reverse instantiation: MODE 3 (the default settings with their runtime
+  // branches compiled out) in FP32, where it is 2-3% faster; in FP64 the
+  // default instantiation, which measured faster than MODE 3 (2.6 against 3.2
+  // ms for the 1-degree 120-substep adjoint)
+  const int rmode = sd == 0 && plain && sizeof(T) == 4 ? 3 : sd;
   auto recompute = [&](int sgm, hipStream_t cs) {
     const int k0 = sgm * ce, len = std::min(ce, n - k0);
     if (pcfg) {  // states jj+1 and jj+2 from state jj, up to state len-1
@@ -1400,7 +1417,7 @@ int adjoint(const Geom& g, const vh_domain* d, const vh_settings* s, const vh_fi
       const int k0 = sgm * ce, len = std::min(ce, n - k0);
       for (int jj = len - 1; jj >= 0; --jj) {
         const EvpArgBlock<T>* b = (const EvpArgBlock<T>*)((char*)rblk + (size_t)(k0 + jj) * L.blk);
-        adj_launch<T>(acfg, a_dev, b, sets(cur), g, nm, sd, cs);
+        adj_launch<T>(acfg, a_dev, b, sets(cur), g, nm, rmode, cs);
         cur = 1 - cur;
       }
       if (sgm >= 1 && overlap) hipStreamWaitEvent(cs, side.join, 0);
