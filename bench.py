@@ -495,7 +495,7 @@ def bench_shard(dev, st, stepped) -> dict:
                          and a.shape[0] == full.variables.uIce.shape[0] else a)
         sett = make_state_settings(nx_s, st.settings.ny, enable_cyclic_x=False, nEVPsteps=nsub)
         strip = VerisState(sett, VariablesNamespace(arrays))
-        tile = tune_evp(strip, exact=True, n_substeps=100)
+        tile = tune_evp(strip, exact=False, n_substeps=100)  # a closed strip: every shape exact
         t_strip = _sync_time(lambda: K.evp_solver(strip, nsub), 3)
         t_full = _sync_time(lambda: K.evp_solver(full, nsub), 3)
         res[f"{tag}_strip_us_per_substep"] = t_strip / nsub * 1e6
@@ -580,9 +580,11 @@ def bench_variants(dev, nsub: int) -> dict:
 def bench_hires(dev) -> dict:
     import torch
 
+    from veris_amd import _native
     from veris_amd import adjoint as A
     from veris_amd import kernels as K
     from veris_amd import synth
+    from veris_amd.autotune import tune_evp
     from veris_amd.step import seaice_step
 
     st = synth.config4()
@@ -599,6 +601,11 @@ def bench_hires(dev) -> dict:
         s_ = torch.from_numpy(sw).to(dev, dtype)
         start = g.copy(lambda a: a.clone())
         seaice_step(g, q, s_, n_substeps=nsub)  # first step (graph capture)
+        # the EVP launch shape for this grid and precision, among every shape
+        # (its inputs are periodic in the x overlap)
+        tile = tune_evp(g, exact=False, n_substeps=100)
+        res[f"{tag}_evp_tile"] = list(_native.evp_tile_shapes()[tile])
+        seaice_step(start.copy(lambda a: a.clone()), q, s_, n_substeps=nsub)  # capture
         out[tag] = g
         g2 = start.copy(lambda a: a.clone())
         t = _sync_time(lambda: seaice_step(g2, q, s_, n_substeps=nsub), 3)

@@ -127,11 +127,24 @@ def test_evp_config1_1deg(cuda):
 
 def test_evp_config4_quarter_degree(cuda):
     """The full 0.25-degree grid (1440 x 720 + halo, about a million cells,
-    the largest single domain here): EVP substeps bit-exact on every cell."""
+    the largest single domain here): EVP substeps bit-exact on every cell
+    with the default shape, and with the two-substep shapes the autotuner
+    picks there (16 x 32 x 2 in FP64, 24 x 32 x 2 in FP32) on every velocity
+    and every stress outside the periodic artefact columns."""
     st = _spun_up(synth.config4(), cuda, n_spin=3)
     got, want = _evp_pair(st, 4, cuda)
     for name, a, b in zip(EVP_NAMES, got, want):
         assert_exact(f"{name}[0.25deg]", a, b)
+    g = st.to(cuda)
+    assert K.overlaps_periodic(g)
+    keep = np.setdiff1d(np.arange(st.shape[0]), _periodic_artefact_cols(st))
+    shapes = N_shapes()
+    for tile in (t for t, sh in shapes.items() if sh in ((16, 32, 2), (24, 32, 2))):
+        got = [g2n(x) for x in K.evp_solver(g, n_substeps=4, tile=tile)]
+        for q, (name, a, b) in enumerate(zip(EVP_NAMES, got, want)):
+            if q >= 2:
+                a, b = a[keep], b[keep]
+            assert_exact(f"{name}[0.25deg, tile={tile}]", a, b)
 
 
 def test_full_step_config4_quarter_degree(cuda):
