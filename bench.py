@@ -435,7 +435,7 @@ def bench_adjoint(dev, ckpt: int = 10) -> dict:
         t_fb = _sync_time(fwd_bwd, 7)
         r = {"plain_forward_ms": t_plain * 1e3, "checkpointed_forward_ms": t_fwd * 1e3,
              "forward_backward_ms": t_fb * 1e3, "forward_tile": tile,
-             "plain_forward_tile": K.EVP_TILE.get((tuple(st.variables.iceMask.shape), dtype), 0),
+             "plain_forward_tile": K.EVP_TILE.get(K.evp_tile_key(st), 0),
              "adjoint_over_plain_forward": t_fb / t_plain,
              "adjoint_over_checkpointed_forward": t_fb / t_fwd, "target": 4.0}
         if tag == "fp64":
@@ -546,13 +546,19 @@ def bench_variants(dev, nsub: int) -> dict:
         return {"us_per_substep": t * 1e6, "cell_substeps_per_s": cells / t,
                 "hbm_frac_at_38_words": bps / 1e9 / HBM_PEAK_GBS}
 
-    out = {"workload": f"1-degree 360x160 + halo, FP64, {nsub} EVP substeps after one full step"}
+    from veris_amd import _native
+    from veris_amd.autotune import tune_evp
+
+    out = {"workload": f"1-degree 360x160 + halo, FP64, {nsub} EVP substeps after one full step, "
+                       "each setting on its autotuned launch shape"}
     for name, kw in (("default", {}), ("adaptive_evp", {"useAdaptiveEVP": True}),
                      ("free_slip_side_drag", {"noSlip": False}),
                      ("second_order_bc", {"secondOrderBC": True})):
         st, g = prepared(**kw)
         cells = st.settings.nx * st.settings.ny
+        tile = tune_evp(g, exact=False, n_substeps=100)
         out[name] = per_substep(lambda: K.evp_solver(g, nsub), cells)
+        out[name]["tile"] = list(_native.evp_tile_shapes()[tile])
     st, g = prepared()
     cells = st.settings.nx * st.settings.ny
     out["residual_norms"] = per_substep(lambda: K.evp_solver_residuals(g, nsub), cells)
@@ -771,7 +777,7 @@ def main(argv=None) -> int:
     seaice_step(g, q0, sw, n_substeps=nsub)
     if args.tile > 0:
         tile = args.tile
-        K.EVP_TILE[(tuple(g.variables.iceMask.shape), g.variables.iceMask.dtype)] = tile
+        K.EVP_TILE[K.evp_tile_key(g)] = tile
     else:
         tile = tune_evp(g, exact=args.exact, n_substeps=100)
     ri, rj, kk = _native.evp_tile_shapes()[tile]

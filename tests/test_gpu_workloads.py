@@ -94,7 +94,7 @@ def test_full_step_1deg(cuda):
     names = ("uIce", "vIce", "sigma1", "sigma2", "sigma12", "OceanStressU", "OceanStressV",
              "hIceMean", "hSnowMean", "Area", "TSurf", "Qnet", "Qsw", "EmPmR",
              "forc_salt_surface_ice", "SeaIceLoad", "IcePenetSW", "recip_hIceMean")
-    key = (tuple(start.variables.iceMask.shape), start.variables.iceMask.dtype)
+    key = K.evp_tile_key(start)
     try:
         for tile in [0] + sorted(_tiles()):
             k = _tiles()[tile][2] if tile else 1

@@ -71,14 +71,6 @@ def _forward_fields(state, tile, pstar, h, drags):
     return c, h, pre, pstar_field
 
 
-def _default_settings(s) -> bool:
-    """The settings under which the subcycle takes none of its optional
-    branches (those of the library's default-settings kernels)."""
-    return (not s.useAdaptiveEVP and bool(s.noSlip) and not s.secondOrderBC
-            and float(s.sinWat) == 0.0 and float(s.basalDragK2) == 0.0
-            and float(s.tensileStrFac) == 0.0)
-
-
 def pair_tile(state, *, pStar=None, hIceMean=None) -> int:
     """A two-substep tile shape for :func:`evp_subcycle` (``tile=``) when
     its forward and recompute may run two substeps per launch on these
@@ -87,8 +79,7 @@ def pair_tile(state, *, pStar=None, hIceMean=None) -> int:
     (:func:`veris_amd.kernels.overlaps_periodic`, evaluated on the pre-EVP
     fields these parameters give).  Waits for the device."""
     vs, s = state.variables, state.settings
-    if not _default_settings(s) or any(
-            vs.get(k) is not None and not K._all_zero(vs.get(k)) for k in K._KTERMS):
+    if not K.default_settings(state):
         return 0
     pstar = _as_tensor(s.pStar if pStar is None else pStar)
     h = vs.hIceMean if hIceMean is None else hIceMean

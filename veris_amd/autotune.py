@@ -5,7 +5,7 @@ about one wave per SIMD; ensembles of large members are bandwidth bound), so
 it is measured rather than guessed: each candidate shape runs a short
 subcycle on the caller's own state, timed with HIP events on the current
 stream, and the fastest one is remembered per (shape, dtype) in
-:data:`veris_amd.kernels.EVP_TILE`.
+:data:`veris_amd.kernels.EVP_TILE` (per grid, dtype and settings class).
 
 Exact shapes (one substep per launch) reproduce the reference on every cell
 and are the default candidates; multi-substep shapes (``exact=False``) also
@@ -60,9 +60,7 @@ def tune_evp(state, exact: bool = True, n_substeps: int = 100, verbose: bool = F
         if verbose:
             print(f"tile {t} {N.evp_tile_shapes()[t]}: {times[t] * 1e6:.2f} us/substep")
     best = min(times, key=times.get)
-    ref = state.variables.iceMask
-    shape = tuple(ref.shape)
-    K.EVP_TILE[(shape, ref.dtype)] = best
+    K.EVP_TILE[K.evp_tile_key(state)] = best
     return best
 
 

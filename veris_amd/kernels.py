@@ -223,9 +223,31 @@ def pre_evp(state, pStar: torch.Tensor | None = None):
     return tuple(out)
 
 
-# EVP tile shape index used when a call does not name one (0 = library default);
-# set by veris_amd.autotune.
+# EVP tile shape index used when a call does not name one (0 = library default),
+# per evp_tile_key; set by veris_amd.autotune.
 EVP_TILE = {}
+
+
+def default_settings(state) -> bool:
+    """Whether ``state``'s settings take none of the EVP subcycle's optional
+    branches (no adaptive EVP, no-slip without the second-order term, no
+    turning angle, basal drag or metric terms, tensileStrFac = 0): those the
+    library runs on its default-settings kernels."""
+    vs, s = state.variables, state.settings
+    if any(vs.get(k) is not None and not _all_zero(vs.get(k)) for k in _KTERMS):
+        return False
+    return (not s.useAdaptiveEVP and bool(s.noSlip) and not s.secondOrderBC
+            and float(s.sinWat) == 0.0 and float(s.basalDragK2) == 0.0
+            and float(s.tensileStrFac) == 0.0)
+
+
+def evp_tile_key(state) -> tuple:
+    """Key of :data:`EVP_TILE`: the padded grid, the dtype and whether the
+    settings are the default ones (the best shape of the default-settings
+    kernels says nothing about the general ones, whose multi-substep shapes
+    hold more registers)."""
+    t = state.variables.iceMask
+    return (tuple(t.shape), t.dtype, default_settings(state))
 
 
 def overlaps_periodic(state) -> bool:
@@ -255,7 +277,7 @@ def evp_solver(state, n_substeps: int | None = None, tile: int | None = None):
     ``tile`` selects the workgroup shape (see :func:`veris_amd.autotune.tune_evp`)."""
     c = Ctx.of(state)
     if tile is None:
-        tile = EVP_TILE.get((c.shape, c.dtype), 0)
+        tile = EVP_TILE.get(evp_tile_key(state), 0)
         if tile and not single_substep_ok(state.settings, tile):
             tile = 0  # a tuned multi-substep shape cannot run the second-order boundary term
     c.dom.tile = int(tile)
