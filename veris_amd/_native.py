@@ -201,8 +201,14 @@ def last_error() -> str:
     return buf.value.decode(errors="replace")
 
 
+_fns: dict = {}
+
+
 def call(name: str, *args) -> None:
-    rc = getattr(lib(), name)(*args)
+    fn = _fns.get(name)
+    if fn is None:
+        fn = _fns[name] = getattr(lib(), name)
+    rc = fn(*args)
     if rc != 0:
         raise VerisHipError(name, rc, last_error())
 

@@ -6,24 +6,35 @@ from __future__ import annotations
 import ctypes
 
 from . import _native as N
-from .kernels import Ctx, _p
+from .kernels import Ctx
+
+
+def _gravity(c: Ctx, sett) -> float:
+    """The host model's ``settings.grav`` (Veros), or the plugin's
+    ``gravity`` when the settings carry no ``grav``; kept on the call
+    context, which is rebuilt whenever a setting is assigned."""
+    g = getattr(c, "_grav", None)
+    if g is None:
+        g = c._grav = float(sett["grav"]) if "grav" in sett.keys() else float(sett.gravity)
+    return g
 
 
 def bulkf_formula_lanl(state, uw, vw, ta, qa, tsf, ocn_mask):
     """veris/heat_flux_MITgcm.py:5-123 ->
     (flwupa, flha, fsha, df0dT, ust, vst, evp, ssq, dEvdT).
 
-    Inputs are device tensors of the state's padded shape and dtype; gravity
-    is the host model's ``settings.grav`` (Veros), or the plugin's
-    ``gravity`` when the settings carry no ``grav``."""
+    Inputs are device tensors of the state's padded shape and dtype."""
     c = Ctx.of(state)
-    for name, t in (("uw", uw), ("vw", vw), ("ta", ta), ("qa", qa), ("tsf", tsf),
-                    ("ocn_mask", ocn_mask)):
-        c._check(name, t)
-    sett = state.settings
-    grav = float(sett["grav"]) if "grav" in sett.keys() else float(sett.gravity)
-    out = c.empty(len(N.BULK_FLUX_NAMES))
-    o = N.BulkFluxOut(*[t.data_ptr() for t in out])
-    N.call("vh_bulkf_formula_lanl", ctypes.byref(c.dom), ctypes.byref(c.sett), grav, _p(uw),
-           _p(vw), _p(ta), _p(qa), _p(tsf), _p(ocn_mask), ctypes.byref(o), c.stream)
-    return tuple(out)
+    chk = c._check
+    chk("uw", uw)
+    chk("vw", vw)
+    chk("ta", ta)
+    chk("qa", qa)
+    chk("tsf", tsf)
+    chk("ocn_mask", ocn_mask)
+    out, ptrs = c.empty_addressed(9)
+    N.call("vh_bulkf_formula_lanl", ctypes.byref(c.dom), ctypes.byref(c.sett),
+           _gravity(c, state.settings), uw.data_ptr(), vw.data_ptr(), ta.data_ptr(),
+           qa.data_ptr(), tsf.data_ptr(), ocn_mask.data_ptr(), ctypes.byref(N.BulkFluxOut(*ptrs)),
+           c.stream)
+    return out

@@ -20,6 +20,15 @@ from .variables import VARIABLE_NAMES
 
 _KTERMS = ("k1AtC", "k2AtC", "k1AtZ", "k2AtZ")
 _ws_cache: dict = {}
+_raw_stream = getattr(torch._C, "_cuda_getCurrentRawStream", None)
+
+
+def current_stream(index: int) -> int:
+    """The handle of torch's current HIP stream on device ``index`` (the
+    cheap raw query where this torch build has it)."""
+    if _raw_stream is not None:
+        return _raw_stream(index)
+    return torch.cuda.current_stream(index).cuda_stream
 
 
 def _dtype_code(t: torch.Tensor) -> int:
@@ -57,8 +66,10 @@ class Ctx:
         if not isinstance(ref, torch.Tensor) or ref.device.type != "cuda":
             raise TypeError("state variables must be torch tensors on a GPU device")
         self.device = ref.device
+        self.index = ref.get_device()
         self.dtype = ref.dtype
         self.shape = (int(sett.nx) + 2 * int(sett.olx), int(sett.ny) + 2 * int(sett.oly))
+        self.field_bytes = self.shape[0] * self.shape[1] * ref.element_size()
         self.dom = N.domain_struct(sett, _dtype_code(ref))
         self.sett = N.settings_struct(sett)
         self.fields = N.Fields()
@@ -73,7 +84,7 @@ class Ctx:
             self.field_names.append(name)
             self._src[name] = weakref.ref(t)
         self._kterms(vs)
-        self.stream = torch.cuda.current_stream(self.device).cuda_stream
+        self.stream = current_stream(self.index)
 
     def _kterms(self, vs):
         """Metric-term fields that are absent or all zero are passed as NULL
@@ -117,11 +128,13 @@ class Ctx:
         vs, sett = state.variables, state.settings
         t = vs.get("iceMask") if ref is None else ref
         vver, sver = getattr(vs, "_version", None), getattr(sett, "_version", None)
-        if not isinstance(t, torch.Tensor) or t.device.type != "cuda" or vver is None or sver is None:
+        if not isinstance(t, torch.Tensor) or not t.is_cuda or vver is None or sver is None:
             return cls(state, ref)
-        kv = tuple(None if x is None else (id(x), x._version) for x in map(vs.get, _KTERMS))
-        stream = torch.cuda.current_stream(t.device).cuda_stream
-        slot = (t.dtype, t.device, stream)
+        get = vs.get
+        kv = tuple([None if x is None else (id(x), x._version) for x in
+                    (get("k1AtC"), get("k2AtC"), get("k1AtZ"), get("k2AtZ"))])
+        index = t.get_device()
+        slot = (t.dtype, index, current_stream(index))
         cache = vs.__dict__.setdefault("_ctx_cache", {})
         hit = cache.get(slot)
         if hit is not None and hit[1] is sett and hit[0][1] == sver:
@@ -136,20 +149,32 @@ class Ctx:
         return c
 
     def _check(self, name, t):
+        if (isinstance(t, torch.Tensor) and t.dtype is self.dtype and t.shape == self.shape
+                and t.get_device() == self.index and t.is_contiguous()):
+            return
+        self._check_failed(name, t)
+
+    def _check_failed(self, name, t):
         if not isinstance(t, torch.Tensor):
             raise TypeError(f"variable {name} is not a torch tensor")
         if t.device != self.device or t.dtype != self.dtype:
             raise TypeError(f"variable {name}: {t.dtype} on {t.device}, expected "
                             f"{self.dtype} on {self.device}")
-        if tuple(t.shape) != self.shape or not t.is_contiguous():
-            raise ValueError(f"variable {name}: shape {tuple(t.shape)} (contiguous="
-                             f"{t.is_contiguous()}), expected contiguous {self.shape}")
+        raise ValueError(f"variable {name}: shape {tuple(t.shape)} (contiguous="
+                         f"{t.is_contiguous()}), expected contiguous {self.shape}")
 
     def empty(self, n=1):
         """One output field, or a list of n (views of one allocation)."""
         if n == 1:
             return torch.empty(self.shape, dtype=self.dtype, device=self.device)
         return list(torch.empty((n,) + self.shape, dtype=self.dtype, device=self.device).unbind(0))
+
+    def empty_addressed(self, n):
+        """n output fields (views of one allocation) and their device
+        addresses."""
+        block = torch.empty((n,) + self.shape, dtype=self.dtype, device=self.device)
+        p0, nb = block.data_ptr(), self.field_bytes
+        return block.unbind(0), range(p0, p0 + n * nb, nb)
 
     def workspace(self, query: str) -> torch.Tensor:
         nbytes = N.workspace_bytes(query, self.dom)
