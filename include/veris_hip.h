@@ -677,6 +677,11 @@ int vh_growth(const vh_domain* d, const vh_settings* s, const vh_fields* f,
 /* fill_overlap (veris/fill_overlap.py:6-18) of one field, in place. */
 int vh_fill_overlap(const vh_domain* d, void* field, void* stream);
 
+/* fill_overlap of n distinct fields of one grid in place, in one launch
+   (n <= 8): the halo fills that close a model step
+   (veris/setup/seaice_global_4deg/seaice_global_4deg.py:928-932 fills them one call each). */
+int vh_fill_overlaps(const vh_domain* d, int32_t n, void* const* fields, void* stream);
+
 /* Workspace bytes vh_global_sum_f64 needs for this domain. */
 int vh_global_sum_workspace_bytes(const vh_domain* d, size_t* bytes);
 

@@ -4,7 +4,7 @@ seeded inputs.
 Bar: the FP64 kernels that use only +, -, *, /, sqrt and comparisons (mass,
 area, wind forcing, the EVP subcycle, ocean stress, advection, clean-up,
 ridging, fill) are compared BIT FOR BIT on every padded cell, halos included.
-Kernels that call exp/pow/log (strength, basal drag, growth) are compared with
+Kernels that call exp/exp10/log (strength, basal drag, growth) are compared with
 the relative tolerance stated in each test (max|diff| / max|oracle|).
 """
 
@@ -16,6 +16,7 @@ import torch
 
 from gpu_helpers import assert_close, assert_exact, evp_shapes_from_source, g2n, pre_evp
 from oracle import veris_np as O
+from veris_amd import _native as N
 from veris_amd import kernels as K
 from veris_amd import synth
 from veris_amd.step import seaice_step
@@ -150,7 +151,7 @@ def test_evp_config4_quarter_degree(cuda):
 def test_full_step_config4_quarter_degree(cuda):
     """One whole step on the full 0.25-degree grid (two EVP substeps to keep
     the oracle quick); tolerance 1e-9 of each field's maximum, as for the
-    4-degree steps (device exp/pow in strength and growth)."""
+    4-degree steps (device exp/exp10 in strength and growth)."""
     st = synth.config4()
     q0 = st.variables.Qnet.copy()
     sw = st.variables.SWdown.copy()
@@ -476,10 +477,8 @@ def test_clean_up_and_ridging(cuda):
     assert_exact("ridging", g2n(K.ridging(g)), O.ridging(st))
 
 
-def test_growth(cuda):
-    """pow/exp of the device math library: tolerance 1e-12 relative to the
-    largest value of each output."""
-    st = synth.config0()
+def _growth_inputs(**settings):
+    st = synth.config0(**settings)
     vs = st.variables
     r = np.random.default_rng(2)
     vs.TSurf = 250.0 + 25.0 * r.random(st.shape)
@@ -487,6 +486,13 @@ def test_growth(cuda):
     vs.precip = np.abs(r.normal(0, 1e-8, st.shape))
     vs.evap = r.normal(0, 1e-8, st.shape)
     vs.os_hIceMean = np.abs(r.normal(0, 1e-3, st.shape))
+    return st
+
+
+def test_growth(cuda):
+    """exp/exp10 of the device math library: tolerance 1e-12 relative to the
+    largest value of each output."""
+    st = _growth_inputs()
     g = st.to(cuda)
     want = O.Growth(st)
     got = K.Growth(g)
@@ -494,6 +500,36 @@ def test_growth(cuda):
              "Qnet", "SeaIceLoad", "IcePenetSW", "recip_hIceMean")
     for name, a, b in zip(names, got, want):
         assert_close(f"growth {name}", g2n(a), b, 1e-12)
+
+
+@pytest.mark.parametrize("nitc", [1, 12])
+def test_growth_category_counts(cuda, nitc):
+    """Other numbers of thickness categories against the oracle."""
+    st = _growth_inputs(nITC=nitc)
+    want = O.Growth(st)
+    got = K.Growth(st.to(cuda))
+    names = ("hIceMean", "hSnowMean", "Area", "TSurf", "EmPmR", "forc_salt_surface", "Qsw",
+             "Qnet", "SeaIceLoad", "IcePenetSW", "recip_hIceMean")
+    for name, a, b in zip(names, got, want):
+        assert_close(f"growth nITC={nitc} {name}", g2n(a), b, 1e-12)
+
+
+def test_fill_overlaps_equals_single_fills(cuda):
+    """One launch filling several fields in place equals fill_overlap of
+    each; more than eight fields or a repeated field is refused."""
+    for st in (synth.config0(), synth.config0(veros_fill=False)):
+        g = st.to(cuda)
+        r = np.random.default_rng(5)
+        fields = [torch.tensor(r.normal(size=st.shape), device=cuda) for _ in range(5)]
+        want = [K.fill_overlap(g, a) for a in fields]
+        K.fill_overlaps_(g, fields)
+        for a, b in zip(fields, want):
+            assert torch.equal(a, b)
+    extra = [torch.zeros(st.shape, dtype=torch.float64, device=cuda) for _ in range(9)]
+    with pytest.raises(N.VerisHipError, match="at most 8"):
+        K.fill_overlaps_(g, extra)
+    with pytest.raises(N.VerisHipError, match="repeats"):
+        K.fill_overlaps_(g, [extra[0], extra[1], extra[0]])
 
 
 def test_fill_and_global_sum(cuda):

@@ -169,6 +169,7 @@ SIGNATURES = {
     "vh_ridging": [_DP, _SP, _FP, _P, _P],
     "vh_growth": [_DP, _SP, _FP, ctypes.POINTER(GrowthOut), _P],
     "vh_fill_overlap": [_DP, _P, _P],
+    "vh_fill_overlaps": [_DP, _I32, ctypes.POINTER(_P), _P],
     "vh_global_sum_workspace_bytes": [_DP, ctypes.POINTER(_SZ)],
     "vh_global_sum_f64": [_DP, _P, _P, _P, _P, _SZ, _P],
 }

@@ -1,9 +1,11 @@
 // Thermodynamic growth (veris/growth.py:8-397) with the surface energy
 // balance solve4temp (veris/solve4temp.py:5-174): pointwise, one thread per
 // padded cell, all nITC thickness categories and their Newton iterations in
-// registers.  Built with -ffp-contract=off; expressions keep the reference's
-// association order (exp/pow come from the device math library, so results
-// agree with an IEEE host evaluation to a few ulp, not bit for bit).
+// registers (the kernel is bound by the arithmetic of the flux evaluations;
+// a variant with one wave per category was no faster).  Built
+// with -ffp-contract=off; expressions keep the reference's association order
+// (exp/pow come from the device math library, so results agree with an IEEE
+// host evaluation to a few ulp, not bit for bit).
 #include <cmath>
 
 #include "vh_common.h"
@@ -29,7 +31,7 @@ struct GrowthScalars {
   double dryIceAlb, drySnowAlb, wetIceAlb, wetSnowAlb, dryIceAlb_south, drySnowAlb_south,
       wetIceAlb_south, wetSnowAlb_south, hCut, shortwave, kIS /*iceConduct*snowConduct*/,
       snowConduct, iceConduct, minTIce_K, lhSublim;
-  double aa1, aa2, bb1, bb1m /*1-bb1*/, ppa, cc1, cc2;
+  double aa1, aa2, p10aa2 /*10^aa2*/, bb1, bb1m /*1-bb1*/, ppa, cc1, cc2;
 };
 
 template <typename T>
@@ -51,9 +53,13 @@ __device__ inline Fl<T> fluxes(const GrowthScalars& k, T t1, bool ice, T kEff, T
   const T t2 = t1 * t1;
   const T t3 = t2 * t1;
   const T t4 = t2 * t2;
-  const T svp = pow((T)10, (T)(-k.aa1) / t1 + (T)k.aa2);
+  // c3 = 10^(aa1/t1) (veris/solve4temp.py:120) and the saturation vapour
+  // pressure 10^(-aa1/t1 + aa2) (:114) as 10^aa2 / c3: one power of ten per
+  // evaluation instead of two (the powers were almost all of the kernel's
+  // arithmetic); within a few ulp of the reference's two powers
+  const T c3 = exp10((T)k.aa1 / t1);
+  const T svp = (T)k.p10aa2 / c3;
   const T qsurf = ice ? ((T)k.bb1 * svp) / ((T)k.ppa - (T)k.bb1m * svp) : (T)0;
-  const T c3 = pow((T)10, (T)k.aa1 / t1);
   const T cd = (T)k.cc2 - c3 * (T)k.ppa;
   const T dq = ice ? ((T)k.cc1 * c3) / ((cd * cd) * t2) : (T)0;
   r.Fc = ice ? kEff * (Tfrz - t1) : (T)0;
@@ -117,11 +123,20 @@ __device__ inline T solve4temp(const GrowthScalars& k, T hI, T hS, T Tin, T Tfrz
 
 // Storage type T, arithmetic in C = double for both storage types: the
 // solve4temp flux derivative (cc1 c3 / (cd^2 T^2)) exceeds the FP32 range at
-// the minTIce clamp, and the kernel is a negligible part of a step.
-template <typename T, typename C = double>
-__global__ __launch_bounds__(BLK) void growth_kernel(GrowthScalars k, GrowthIO<T> io, size_t n) {
-  const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= n) return;
+// the minTIce clamp.
+//
+// A cell's work splits into the inputs of its categories (cell_in), the
+// surface energy balance of each category (category: six Newton iterations,
+// two powers of ten per flux evaluation, almost all of a cell's arithmetic)
+// and the budgets that combine them (tail).
+template <typename C>
+struct CellIn {
+  C hP, hsP, AP, rregA, rhIA, hIA, hSA, sal, Tfrz, Ts0, fc, LWd, ATm, wsp, aqh, SWd;
+};
+
+template <typename T, typename C>
+__device__ inline CellIn<C> cell_in(const GrowthScalars& k, const GrowthIO<T>& io, size_t c) {
+  CellIn<C> x;
   const C h0 = io.h[c], hs0 = io.hs[c], A0 = io.A[c];
   const bool noIce0 = (h0 == (C)0) || (A0 == (C)0);
   const C hP = h0 * (C)(!noIce0);
@@ -136,28 +151,47 @@ __global__ __launch_bounds__(BLK) void growth_kernel(GrowthScalars k, GrowthIO<T
   hIA = fmax(hIA, (C)0.05);
   const C sal = io.ocSalt[c];
   const C Tfrz = ((C)k.tempFrz + (C)k.dtempFrz_dS * sal) + (C)k.celsius2K;
-  const C Ts0 = io.Ts[c];
-  const C fc = io.fC[c];
-  const C LWd = io.LWdown[c], ATm = io.ATemp[c], wsp = io.wSpeed[c], aqh = io.aqh[c],
-          SWd = io.SWdown[c];
-  C Tsum = 0, Fio_s = 0, Fia_s = 0, pen_s = 0;
-  for (int l = 0; l < k.nITC; ++l) {
-    const C p = (C)(2 * (l + 1) - 1) * (C)k.recip_nITC;
-    C Fio, Fia, pen;
-    const C Tl = solve4temp(k, hIA * p, hSA * p, Ts0, Tfrz, fc, LWd, ATm, wsp, aqh, SWd, &Fio, &Fia,
-                            &pen);
-    if (l == 0) {
-      Tsum = Tl;
-      Fio_s = Fio;
-      Fia_s = Fia;
-      pen_s = pen;
-    } else {
-      Tsum = Tsum + Tl;
-      Fio_s = Fio_s + Fio;
-      Fia_s = Fia_s + Fia;
-      pen_s = pen_s + pen;
-    }
-  }
+  x.hP = hP;
+  x.hsP = hsP;
+  x.AP = AP;
+  x.rregA = rregA;
+  x.rhIA = rhIA;
+  x.hIA = hIA;
+  x.hSA = hSA;
+  x.sal = sal;
+  x.Tfrz = Tfrz;
+  x.Ts0 = io.Ts[c];
+  x.fc = io.fC[c];
+  x.LWd = io.LWdown[c];
+  x.ATm = io.ATemp[c];
+  x.wsp = io.wSpeed[c];
+  x.aqh = io.aqh[c];
+  x.SWd = io.SWdown[c];
+  return x;
+}
+
+template <typename C>
+struct CatOut {
+  C T, Fio, Fia, pen;
+};
+
+// category l of nITC (veris/growth.py:63-91)
+template <typename C>
+__device__ inline CatOut<C> category(const GrowthScalars& k, const CellIn<C>& x, int l) {
+  const C p = (C)(2 * (l + 1) - 1) * (C)k.recip_nITC;
+  CatOut<C> r;
+  r.T = solve4temp(k, x.hIA * p, x.hSA * p, x.Ts0, x.Tfrz, x.fc, x.LWd, x.ATm, x.wsp, x.aqh, x.SWd,
+                   &r.Fio, &r.Fia, &r.pen);
+  return r;
+}
+
+// category means onwards (veris/growth.py:94-397), from the category sums
+// accumulated in category order
+template <typename T, typename C>
+__device__ inline void tail(const GrowthScalars& k, const GrowthIO<T>& io, size_t c,
+                            const CellIn<C>& x, C Tsum, C Fio_s, C Fia_s, C pen_s) {
+  const C hP = x.hP, hsP = x.hsP, AP = x.AP, rregA = x.rregA, rhIA = x.rhIA, sal = x.sal,
+          Tfrz = x.Tfrz, fc = x.fc;
   const C TSurf = Tsum * (C)k.recip_nITC;
   const C Fio = (Fio_s * (C)k.recip_nITC) * AP;
   C Fia = (Fia_s * (C)k.recip_nITC) * AP;
@@ -237,6 +271,30 @@ __global__ __launch_bounds__(BLK) void growth_kernel(GrowthScalars k, GrowthIO<T
   io.oload[c] = (T)(h * (C)k.rhoIce + hs * (C)k.rhoSnow);
   io.open[c] = (T)pen;
   io.orh[c] = (T)rh;
+}
+
+// one thread per padded cell, its categories in turn
+template <typename T, typename C = double>
+__global__ __launch_bounds__(BLK) void growth_kernel(GrowthScalars k, GrowthIO<T> io, size_t n) {
+  const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  const CellIn<C> x = cell_in<T, C>(k, io, c);
+  C Tsum = 0, Fio_s = 0, Fia_s = 0, pen_s = 0;
+  for (int l = 0; l < k.nITC; ++l) {
+    const CatOut<C> o = category(k, x, l);
+    if (l == 0) {
+      Tsum = o.T;
+      Fio_s = o.Fio;
+      Fia_s = o.Fia;
+      pen_s = o.pen;
+    } else {
+      Tsum = Tsum + o.T;
+      Fio_s = Fio_s + o.Fio;
+      Fia_s = Fia_s + o.Fia;
+      pen_s = pen_s + o.pen;
+    }
+  }
+  tail<T, C>(k, io, c, x, Tsum, Fio_s, Fia_s, pen_s);
 }
 
 template <typename T>
@@ -347,6 +405,7 @@ int growth(const Geom& g, const vh_settings* s, const vh_fields* f, const vh_gro
   k.bb1m = 1 - k.bb1;
   k.ppa = 100000;
   const double cc0 = std::pow(10.0, k.aa2);
+  k.p10aa2 = cc0;
   k.cc1 = cc0 * k.aa1 * k.bb1 * k.ppa * std::log(10.0);
   k.cc2 = cc0 * (1 - k.bb1);
   const size_t n = field_elems(g);

@@ -259,6 +259,33 @@ __global__ __launch_bounds__(BLK) void fill_kernel(Geom g, T* a) {
   a[idx] = a[AT(si, sj)];
 }
 
+// several fields of one grid in one launch (the step's closing fills):
+// every overlap cell reads only interior cells, so one pass per field equals
+// the x-then-y fill
+constexpr int kFillMax = 8;
+template <typename T>
+struct FillList {
+  T* f[kFillMax];
+  int n;
+};
+
+template <typename T>
+__global__ __launch_bounds__(BLK) void fill_many_kernel(Geom g, FillList<T> fl) {
+  CELL_PROLOGUE
+  if (is_computed(g, i, j)) return;
+  int si = i, sj = j;
+  if (g.fill_x) {
+    if (i < g.ol) si = i + g.nx;
+    else if (i >= g.nx + g.ol) si = i - g.nx;
+  }
+  if (g.fill_y) {
+    if (j < g.ol) sj = j + g.ny;
+    else if (j >= g.ny + g.ol) sj = j - g.ny;
+  }
+  const size_t src = AT(si, sj);
+  for (int q = 0; q < fl.n; ++q) fl.f[q][idx] = fl.f[q][src];
+}
+
 template <typename T>
 __global__ __launch_bounds__(BLK) void sum_partial_kernel(Geom g, const T* x, const T* w,
                                                           double* part) {
@@ -456,6 +483,25 @@ int fill(const Geom& g, void* a, hipStream_t st) {
   VH_REQUIRE(a != nullptr, VH_EINVAL, "fill_overlap: null field");
   if (!g.fill_x && !g.fill_y) return VH_OK;
   hipLaunchKernelGGL(fill_kernel<T>, cell_grid(g, BLK), dim3(BLK), 0, st, g, (T*)a);
+  VH_TRY_HIP(hipGetLastError());
+  return VH_OK;
+}
+
+template <typename T>
+int fill_many(const Geom& g, int n, void* const* fields, hipStream_t st) {
+  VH_REQUIRE(n >= 0 && n <= kFillMax, VH_EINVAL, "fill_overlaps: %d fields (at most %d)", n,
+             kFillMax);
+  VH_REQUIRE(n == 0 || fields != nullptr, VH_EINVAL, "fill_overlaps: null field list");
+  FillList<T> fl{};
+  fl.n = n;
+  for (int q = 0; q < n; ++q) {
+    VH_REQUIRE(fields[q] != nullptr, VH_EINVAL, "fill_overlaps: null field %d", q);
+    for (int r = 0; r < q; ++r)
+      VH_REQUIRE(fields[q] != fields[r], VH_EINVAL, "fill_overlaps: field %d repeats field %d", q, r);
+    fl.f[q] = (T*)fields[q];
+  }
+  if (n == 0 || (!g.fill_x && !g.fill_y)) return VH_OK;
+  hipLaunchKernelGGL(fill_many_kernel<T>, cell_grid(g, BLK), dim3(BLK), 0, st, g, fl);
   VH_TRY_HIP(hipGetLastError());
   return VH_OK;
 }
@@ -905,6 +951,15 @@ extern "C" int vh_fill_overlap(const vh_domain* d, void* a, void* stream) {
   if (rc) return rc;
   hipStream_t st = (hipStream_t)stream;
   return VH_DISPATCH(d, vh::fill<double>(g, a, st), vh::fill<float>(g, a, st));
+}
+
+extern "C" int vh_fill_overlaps(const vh_domain* d, int32_t n, void* const* fields, void* stream) {
+  vh::Geom g;
+  int rc = vh::make_geom(d, &g);
+  if (rc) return rc;
+  hipStream_t st = (hipStream_t)stream;
+  return VH_DISPATCH(d, vh::fill_many<double>(g, n, fields, st),
+                     vh::fill_many<float>(g, n, fields, st));
 }
 
 extern "C" int vh_global_sum_workspace_bytes(const vh_domain* d, size_t* bytes) {

@@ -407,6 +407,22 @@ def fill_overlap(state, a: torch.Tensor) -> torch.Tensor:
     return out
 
 
+def fill_overlaps_(state, fields) -> None:
+    """fill_overlap (veris/fill_overlap.py:6-18) of up to 8 distinct fields
+    of the state's grid and dtype, each in place, in one launch: the fills
+    that close a step, which the reference makes one call each
+    (veris/setup/seaice_global_4deg/seaice_global_4deg.py:928-932), on
+    fields the step has just produced and owns."""
+    fields = list(fields)
+    if not fields:
+        return
+    c = Ctx.of(state, ref=fields[0])
+    for q, a in enumerate(fields):
+        c._check(f"field {q}", a)
+    ptrs = (ctypes.c_void_p * len(fields))(*[a.data_ptr() for a in fields])
+    N.call("vh_fill_overlaps", ctypes.byref(c.dom), len(fields), ptrs, c.stream)
+
+
 def fill_overlap_uv(state, u, v):
     """veris/fill_overlap.py:21-23."""
     return fill_overlap(state, u), fill_overlap(state, v)

@@ -35,6 +35,7 @@ def seaice_step(state, qnet_forcing=None, swdown=None, n_substeps=None):
     vs.Area = K.ridging(state)
     (vs.hIceMean, vs.hSnowMean, vs.Area, vs.TSurf, vs.EmPmR, vs.forc_salt_surface_ice,
      vs.Qsw, vs.Qnet, vs.SeaIceLoad, vs.IcePenetSW, vs.recip_hIceMean) = K.Growth(state)
-    for name in STEP_FILLS:
-        vs[name] = K.fill_overlap(state, vs[name])
+    # the reference's fill_overlap of each (filled copies); the fields are the
+    # fresh outputs of Growth above, so they are filled in place, in one launch
+    K.fill_overlaps_(state, [vs[name] for name in STEP_FILLS])
     return state
