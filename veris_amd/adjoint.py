@@ -48,7 +48,7 @@ def _forward_fields(state, tile, pstar, h, drags):
     """ABI arguments of the subcycle with the given parameters: a fresh Ctx of
     ``state`` whose hIceMean, drag settings and pre-EVP fields (computed here
     on the device) are the parameters', with the tensors behind them."""
-    c = K.Ctx(state)
+    c = K.Ctx.of(state).clone()
     c.dom.tile = int(tile)
     dt, shape = c.dtype, c.shape
     for name, val in zip(_DRAGS, _scalars(drags)):

@@ -148,6 +148,18 @@ class Ctx:
         cache[slot] = ((vver, sver, kv), sett, c)
         return c
 
+    def clone(self) -> "Ctx":
+        """A copy with ABI structs of its own, which the caller may modify
+        (the cached Ctx of :meth:`of` must stay as it is)."""
+        c = object.__new__(Ctx)
+        c.__dict__.update(self.__dict__)
+        c.dom = type(self.dom).from_buffer_copy(self.dom)
+        c.sett = type(self.sett).from_buffer_copy(self.sett)
+        c.fields = type(self.fields).from_buffer_copy(self.fields)
+        c.field_names = list(self.field_names)
+        c._src = dict(self._src)
+        return c
+
     def _check(self, name, t):
         if (isinstance(t, torch.Tensor) and t.dtype is self.dtype and t.shape == self.shape
                 and t.get_device() == self.index and t.is_contiguous()):
