@@ -248,8 +248,8 @@ def bench_batched(dev, nsub: int, fp64_evp_rate: float, fp64_step_rate: float) -
     tile64, _ = tune_batched(e64, exact=False)
     out32 = e32.evp(nsub, tile=tile32)
     out64 = e64.evp(nsub, tile=tile64)
-    t32 = _sync_time(lambda: e32.evp(nsub, tile=tile32, out=out32))
-    t64 = _sync_time(lambda: e64.evp(nsub, tile=tile64, out=out64))
+    t32 = _sync_time(lambda: e32.evp(nsub, tile=tile32, out=out32), 7)
+    t64 = _sync_time(lambda: e64.evp(nsub, tile=tile64, out=out64), 7)
     masks = e64.unpack("maskInC")
     err = {}
     for q, name in enumerate(EVP_OUT):
@@ -430,9 +430,9 @@ def bench_adjoint(dev, ckpt: int = 10) -> dict:
         tune_evp(st, exact=False, n_substeps=n)
         for fn in (plain_forward, ckpt_forward, fwd_bwd):
             fn()
-        t_plain = _sync_time(plain_forward, 7)
-        t_fwd = _sync_time(ckpt_forward, 7)
-        t_fb = _sync_time(fwd_bwd, 7)
+        t_plain = _sync_time(plain_forward, 15)
+        t_fwd = _sync_time(ckpt_forward, 15)
+        t_fb = _sync_time(fwd_bwd, 15)
         r = {"plain_forward_ms": t_plain * 1e3, "checkpointed_forward_ms": t_fwd * 1e3,
              "forward_backward_ms": t_fb * 1e3, "forward_tile": tile,
              "plain_forward_tile": K.EVP_TILE.get(K.evp_tile_key(st), 0),
