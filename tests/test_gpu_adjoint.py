@@ -169,6 +169,28 @@ def test_checkpointed_forward_is_the_drop_in(cuda, kind):
             assert_exact(f"{kind} {NAMES[q]} n={n} ckpt={ckpt}", g2n(outs[q]), g2n(want[q]))
 
 
+def test_adjoint_leaves_the_drop_in_call_context_alone(cuda):
+    """The adjoint's forward starts from a copy of the drop-ins' cached call
+    context and overrides drags, thickness, forcing and launch shape in the
+    copy only: drop-in calls after it give the bits they gave before it."""
+    st = _state("global")
+    gst = st.to(cuda)
+    vs = gst.variables
+    vs.SeaIceMassC, vs.SeaIceMassU, vs.SeaIceMassV = K.SeaIceMass(gst)
+    vs.AreaW, vs.AreaS = K.AreaWS(gst)
+    vs.WindForcingX, vs.WindForcingY = K.WindForcingXY(gst)
+    vs.SeaIceStrength = K.SeaIceStrength(gst)
+    before = [t.clone() for t in K.evp_solver(gst, n_substeps=7)]
+    s = st.settings
+    outs = A.evp_subcycle(gst, n_substeps=7, ckpt_every=3, waterIceDrag=2.0 * s.waterIceDrag,
+                          airIceDrag=0.5 * s.airIceDrag, pStar=torch.tensor(1.5 * s.pStar),
+                          hIceMean=vs.hIceMean * 1.1)
+    assert not torch.equal(outs[0], before[0])
+    after = K.evp_solver(gst, n_substeps=7)
+    for q in range(5):
+        assert torch.equal(after[q], before[q]), NAMES[q]
+
+
 @pytest.mark.parametrize("kind", ["global", "doubly_periodic", "regional", "turning_metric",
                                   "unpacked", "side_drag", "coastline", "basal", "adaptive",
                                   "second_order", "sea_ice_load"])
