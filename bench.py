@@ -159,6 +159,22 @@ def _sync_time(fn, reps: int = 5) -> float:
     return ts[len(ts) // 2]
 
 
+def _sync_times_interleaved(fns, reps: int) -> list:
+    """Median wall seconds of each of ``fns`` between device
+    synchronisations, the functions called in turn for ``reps`` rounds."""
+    import torch
+
+    ts = [[] for _ in fns]
+    for _ in range(reps):
+        for q, fn in enumerate(fns):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            fn()
+            torch.cuda.synchronize()
+            ts[q].append(time.perf_counter() - t0)
+    return [sorted(t)[len(t) // 2] for t in ts]
+
+
 def _rel_l2(pairs):
     """sqrt(sum |got - ref|^2 / sum |ref|^2) over (got, ref, mask) triples of
     padded fields, interior cells with mask > 0."""
@@ -430,9 +446,9 @@ def bench_adjoint(dev, ckpt: int = 10) -> dict:
         tune_evp(st, exact=False, n_substeps=n)
         for fn in (plain_forward, ckpt_forward, fwd_bwd):
             fn()
-        t_plain = _sync_time(plain_forward, 15)
-        t_fwd = _sync_time(ckpt_forward, 15)
-        t_fb = _sync_time(fwd_bwd, 15)
+        # the three measured in turn, round after round (a drift of the
+        # device's clock then touches all three alike), medians of 15
+        t_plain, t_fwd, t_fb = _sync_times_interleaved((plain_forward, ckpt_forward, fwd_bwd), 15)
         r = {"plain_forward_ms": t_plain * 1e3, "checkpointed_forward_ms": t_fwd * 1e3,
              "forward_backward_ms": t_fb * 1e3, "forward_tile": tile,
              "plain_forward_tile": K.EVP_TILE.get(K.evp_tile_key(st), 0),

@@ -121,6 +121,8 @@ __device__ inline T* fwd_loc(const BlockPlan<T>& p, int k, int q) {
 // substep the next checkpoint (slot nck holds a copy of the final state),
 // which the reverse sweep reads and never recomputes;
 // mode 2: two-substep forward blocks (block m: state 2m -> state 2m+2);
+// mode 4: forward blocks of four-substep launches before the last segment
+// and two-substep launches in it (block m: launch m of that sequence);
 // mode 3: two-substep recompute blocks, at every even substep k of a segment
 // short of its last: state k -> state k+2 with state k+1 kept (mid), all in
 // the segment's slots (state k+2 goes to the spare slot len when it is the
@@ -148,6 +150,30 @@ __global__ void adj_write_blocks_kernel(EvpArgBlock<T>* blocks, size_t stride, E
       b.in[q] = fwd_loc(p, 2 * k, q);
       b.mid[q] = keep_mid ? fwd_loc(p, 2 * k + 1, q) : nullptr;
       b.out[q] = fwd_loc(p, 2 * k + 2, q);
+    }
+  } else if (mode == 4) {
+    // forward launches with four substeps: each segment before the last runs
+    // ce/4 four-substep launches (and one two-substep launch when ce/2 is
+    // odd) from its checkpoint to the next through the two ping-pong slots;
+    // the last segment runs two-substep launches that keep every state
+    const int q4 = p.ce / 4, lps = q4 + (p.ce % 4) / 2, head = (p.nck - 1) * lps;
+    if (k < head) {
+      const int sgm = k / lps, j = k % lps;
+      for (int q = 0; q < 5; ++q) {
+        b.in[q] = j == 0 ? (sgm == 0 ? p.in0[q] : slot(p, p.off_ckpt, 5 * sgm + q))
+                         : slot(p, p.off_pp, 5 * ((j - 1) % 2) + q);
+        b.mid[q] = nullptr;
+        b.out[q] = j == lps - 1 ? slot(p, p.off_ckpt, 5 * (sgm + 1) + q)
+                                : slot(p, p.off_pp, 5 * (j % 2) + q);
+      }
+    } else {
+      const int k0 = (p.nck - 1) * p.ce + 2 * (k - head);
+      if (k0 >= p.n) return;
+      for (int q = 0; q < 5; ++q) {
+        b.in[q] = fwd_loc(p, k0, q);
+        b.mid[q] = fwd_loc(p, k0 + 1, q);
+        b.out[q] = fwd_loc(p, k0 + 2, q);
+      }
     }
   } else if (mode == 3) {
     const int jj = k % p.ce, sgm = k / p.ce;
@@ -1091,6 +1117,13 @@ bool adj_overlap() {
   return v == 1;
 }
 
+// Whether the checkpointed forward runs four substeps per launch before the
+// last segment where it runs two (environment VERIS_ADJ_QUAD=0 keeps two).
+bool adj_quad() {
+  const char* e = getenv("VERIS_ADJ_QUAD");
+  return !(e && atoi(e) == 0);
+}
+
 // A second stream and two events per device, for the fork/join of the
 // reverse sweep (created once, default priority, kept for the process).
 struct SideStream {
@@ -1271,18 +1304,29 @@ int forward_ckpt(const Geom& g, const vh_domain* d, const vh_settings* s, const 
              "and even n_substeps (%d) and ckpt_every (%d)", d->tile, n, ce);
   BlockPlan<T> plan = block_plan<T>(L, f, outs, ws);
   if (pcfg) plan.kstep = 2;
+  // four substeps per launch before the last segment, where the pairs run
+  // and the domain is small enough that the four-substep shape is the faster
+  const int qcfg = (pcfg && L.nck >= 2 && ce >= 4 && adj_quad() &&
+                    field_elems(g) * (size_t)nm < 400000)
+                       ? evp_quad_cfg()
+                       : 0;
+  const int q4 = ce / 4, lps = q4 + (ce % 4) / 2;
   hipLaunchKernelGGL(adj_write_blocks_kernel<T>, dim3((n + 63) / 64), dim3(64), 0, st, fblk, L.blk,
-                     proto, plan, pcfg ? 2 : 0, (AdjArgs<T>*)nullptr, unused, pack_status);
+                     proto, plan, qcfg ? 4 : pcfg ? 2 : 0, (AdjArgs<T>*)nullptr, unused,
+                     pack_status);
   VH_TRY_HIP(hipGetLastError());
   T* final_slot = wsf<T>(ws, L.off_ckpt, L.fb, 5 * L.nck);
   const int n_all = (int)(field_elems(g) * (size_t)nm);
-  int p_tiles = 0, p_tiles_j = 0;
+  int p_tiles = 0, p_tiles_j = 0, q_tiles = 0, q_tiles_j = 0;
   if (pcfg) tiles_of(pcfg, g, &p_tiles, &p_tiles_j);
-  const int n_launch = pcfg ? n / 2 : n;
+  if (qcfg) tiles_of(qcfg, g, &q_tiles, &q_tiles_j);
+  const int head = qcfg ? (L.nck - 1) * lps : 0;
+  const int n_launch = qcfg ? head + (n - (L.nck - 1) * ce) / 2 : pcfg ? n / 2 : n;
   auto record = [&](hipStream_t cs) {
     for (int kk = 0; kk < n_launch; ++kk) {
       const EvpArgBlock<T>* b = (const EvpArgBlock<T>*)((char*)fblk + (size_t)kk * L.blk);
-      if (pcfg) evp_launch_pair<T>(pcfg, b, p_tiles, p_tiles_j, cs, nm);
+      if (kk < head && kk % lps < q4) evp_launch_pair<T>(qcfg, b, q_tiles, q_tiles_j, cs, nm);
+      else if (pcfg) evp_launch_pair<T>(pcfg, b, p_tiles, p_tiles_j, cs, nm);
       else evp_launch_single<T, double>(cfg, b, n_tiles, tiles_j, cs, nm, so, plain);
     }
     hipLaunchKernelGGL(adj_keep_final_kernel<T>, dim3((unsigned)((n_all + BLK - 1) / BLK)),
@@ -1290,8 +1334,9 @@ int forward_ckpt(const Geom& g, const vh_domain* d, const vh_settings* s, const 
                        (const EvpArgBlock<T>*)((char*)fblk + (size_t)(n_launch - 1) * L.blk),
                        n_all, final_slot, L.fb);
   };
-  return run_recorded<T>("adjf", g, L, ((cfg * 2 + (so ? 1 : 0)) * 2 + (plain ? 1 : 0)) * 32 + pcfg,
-                         nm, ws, record, st);
+  return run_recorded<T>(
+      "adjf", g, L, (((cfg * 2 + (so ? 1 : 0)) * 2 + (plain ? 1 : 0)) * 32 + pcfg) * 32 + qcfg, nm, ws,
+      record, st);
 }
 
 template <typename T>

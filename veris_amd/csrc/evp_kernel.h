@@ -1218,21 +1218,29 @@ inline void evp_launch_cfg(int cfg, const EvpArgBlock<T, S>* ab, int which, int 
 }
 
 // Single-domain launch of the default-settings instantiation of shape `cfg`
-// (a shape with k = 2; anything else launches nothing) reading its arguments
-// from `ab` (nlaunch = 1: in -> out, the state after the first substep to
-// ab->mid when set), for n_members equal members packed back to back.
+// (a shape with k = 2, or the 32 x 32 shape with k = 4; anything else
+// launches nothing) reading its arguments from `ab` (nlaunch = 1: in -> out;
+// with k = 2 the state after the first substep to ab->mid when set), for
+// n_members equal members packed back to back.
 template <typename T>
 inline void evp_launch_pair(int cfg, const EvpArgBlock<T>* ab, int n_tiles, int tiles_j,
                             hipStream_t st, int n_members = 1) {
   const int per_xcd = (n_tiles + 7) / 8;
   evp_with_shape(cfg, [&](auto sh) {
     using Sh = decltype(sh);
-    if constexpr (Sh::k == 2)
+    if constexpr (Sh::k == 2 || (Sh::k == 4 && Sh::ri == 32 && Sh::rj == 32))
       hipLaunchKernelGGL(
-          (evp_substep_kernel<T, double, Sh::ri, Sh::rj, 2, false, T, false, false, T, true>),
+          (evp_substep_kernel<T, double, Sh::ri, Sh::rj, Sh::k, false, T, false, false, T, true>),
           dim3(per_xcd * 8, n_members), dim3(Sh::ri * Sh::rj), 0, st, ab, 0, 0, n_tiles, tiles_j,
           per_xcd);
   });
+}
+
+// The 32 x 32 four-substep shape.
+inline int evp_quad_cfg() {
+  for (int c = 1; c < kEvpNumTiles; ++c)
+    if (kEvpTiles[c].ri == 32 && kEvpTiles[c].rj == 32 && kEvpTiles[c].k == 4) return c;
+  return 0;
 }
 
 // Single-domain, single-substep launch of shape `cfg` (a shape with k = 1)
