@@ -291,6 +291,31 @@ __device__ inline A evp_sqrt(A x) { return sqrt(x); }
 template <>
 __device__ inline float evp_sqrt<float>(float x) { return __builtin_amdgcn_sqrtf(x); }
 
+// Square root and quotient of the FP64 island (Delta, zeta, the replacement
+// pressure) of the FP32-arithmetic kernels, whose results are rounded to
+// FP32: the hardware FP64 reciprocal (square-root) estimate refined by one
+// Goldschmidt / Newton-Raphson step and one residual correction, within an
+// ulp or two of the correctly rounded FP64 value in the ranges they see
+// (x >= 0; d >= deltaMin > 0), in half the dependent instructions.  The
+// FP64-arithmetic kernels keep the correctly rounded sqrt and /.
+__device__ inline double island_sqrt(double x) {
+  const double y = __builtin_amdgcn_rsq(x);  // 1/sqrt(x)
+  double s = x * y, h = 0.5 * y;
+  const double r = fma(-s, h, 0.5);
+  s = fma(s, r, s);
+  h = fma(h, r, h);
+  s = fma(fma(-s, s, x), h, s);
+  // +0 (Delta of a motionless or masked cell) and values whose root is far
+  // below deltaMin, where the estimate's range ends: 0
+  return x > 1e-200 ? s : 0.0;
+}
+__device__ inline double island_div(double n, double d) {
+  double r = __builtin_amdgcn_rcp(d);
+  r = fma(r, fma(-d, r, 1.0), r);
+  const double q = n * r;
+  return fma(fma(-d, q, n), r, q);
+}
+
 __device__ inline unsigned long long bits_of(double x) { return (unsigned long long)__double_as_longlong(x); }
 __device__ inline unsigned long long bits_of(float x) { return __float_as_uint(x); }
 template <typename T>
@@ -814,12 +839,16 @@ __global__ __launch_bounds__(RI* RJ) void evp_substep_kernel(const EvpArgBlock<T
         }
         const A ep = e11 + e22, em = e11 - e22;
         const TD dsq = (TD)ep * (TD)ep + (TD)sc.rpd * ((TD)em * (TD)em + (TD)4.0 * (TD)e12c);
-        const TD dC = sqrt(dsq);
+        TD dC;
+        if constexpr (sizeof(A) == 4 && sizeof(TD) == 8) dC = island_sqrt(dsq);
+        else dC = sqrt(dsq);
         const TD dreg = dC + (TD)sc.deltaMin;
         // with t = 0 the products and the quotient by 1 + t = 1 are exact
         // identities, so skipping them keeps every bit
         const TD Pt = f_tens0 ? (TD)P : (TD)P * (TD)sc.tens1p;
-        const TD zeta = ((TD)0.5 * Pt) / dreg;
+        TD zeta;
+        if constexpr (sizeof(A) == 4 && sizeof(TD) == 8) zeta = island_div((TD)0.5 * Pt, dreg);
+        else zeta = ((TD)0.5 * Pt) / dreg;
         sZ[t] = (A)zeta;
         // relaxation parameter at c (veris/evp_solver.py:65-73 when adaptive)
         A am1 = (A)sc.alphaM1, den1 = (A)sc.den1;
