@@ -541,8 +541,15 @@ __global__ __launch_bounds__(RI* RJ) void adj_substep_kernel(const AdjArgs<T>* _
       ep = e11 + e22;
       em = e11 - e22;
       const TD dsq = (TD)ep * (TD)ep + (TD)sc.rpd * ((TD)em * (TD)em + (TD)4.0 * (TD)e12c);
-      const TD dCd = sqrt(dsq);
-      const TD zd = ((TD)0.5 * ((TD)P * (TD)sc.tens1p)) / (dCd + (TD)sc.deltaMin);
+      // the forward kernel's FP64 island: its square root and quotient forms
+      TD dCd, zd;
+      if constexpr (sizeof(T) == 4 && sizeof(TD) == 8) {
+        dCd = island_sqrt(dsq);
+        zd = island_div((TD)0.5 * ((TD)P * (TD)sc.tens1p), dCd + (TD)sc.deltaMin);
+      } else {
+        dCd = sqrt(dsq);
+        zd = ((TD)0.5 * ((TD)P * (TD)sc.tens1p)) / (dCd + (TD)sc.deltaMin);
+      }
       zeta = (T)zd;
       dC = (T)dCd;
       sZ[t] = zeta;
