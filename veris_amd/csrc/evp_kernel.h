@@ -306,8 +306,8 @@ __device__ inline double island_sqrt(double x) {
   h = fma(h, r, h);
   s = fma(fma(-s, s, x), h, s);
   // +0 (Delta of a motionless or masked cell) and values whose root is far
-  // below deltaMin, where the estimate's range ends: 0
-  return x > 1e-200 ? s : 0.0;
+  // below deltaMin, where the estimate's range ends: 0; a NaN stays NaN
+  return (x >= 0.0 && x <= 1e-200) ? 0.0 : s;
 }
 __device__ inline double island_div(double n, double d) {
   double r = __builtin_amdgcn_rcp(d);
