@@ -278,22 +278,10 @@ __global__ void evp_write_args_kernel(EvpArgBlock<T, S>* dst, EvpArgBlock<T, S> 
 constexpr int kEvpRowFields = 12;  // rdyU rdxV rAz rdxU rdyV rrA dyV dxU dxV dyU rrAu rrAv
 constexpr int kEvpMaskFields = 6;  // iceMask iceMaskU iceMaskV maskInC maskInU maskInV
 
-// Division and square root of the momentum update and the water drag.  FP64
-// arithmetic keeps the correctly rounded forms (the drop-in's bit-for-bit
-// parity); FP32 arithmetic takes the hardware reciprocal and square root
-// (about 1 ulp each, a third of the instructions of the rounded sequences).
-template <typename A>
-__device__ inline A evp_div(A a, A b) { return a / b; }
-template <>
-__device__ inline float evp_div<float>(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
-template <typename A>
-__device__ inline A evp_sqrt(A x) { return sqrt(x); }
-template <>
-__device__ inline float evp_sqrt<float>(float x) { return __builtin_amdgcn_sqrtf(x); }
 
 // Square root and quotient of the FP64 island (Delta, zeta, the replacement
-// pressure) of the FP32-arithmetic kernels, whose results are rounded to
-// FP32: the hardware FP64 reciprocal (square-root) estimate refined by one
+// pressure) of the kernels over FP32 step constants, whose results are not
+// held to the FP64 drop-in's bits: the hardware FP64 reciprocal (square-root) estimate refined by one
 // Goldschmidt / Newton-Raphson step and one residual correction, within an
 // ulp or two of the correctly rounded FP64 value in the ranges they see
 // (x >= 0; d >= deltaMin > 0), in half the dependent instructions.  The
@@ -314,6 +302,25 @@ __device__ inline double island_div(double n, double d) {
   r = fma(r, fma(-d, r, 1.0), r);
   const double q = n * r;
   return fma(fma(-d, q, n), r, q);
+}
+
+// Division and square root of the momentum update and the water drag.  FP64
+// arithmetic on FP64 step constants keeps the correctly rounded forms (the
+// drop-in's bit-for-bit parity); FP32 arithmetic takes the hardware
+// reciprocal and square root (about 1 ulp each, a third of the instructions
+// of the rounded sequences); FP64 arithmetic over FP32 step constants (the
+// ensemble's mixed modes) takes the refined FP64 forms above.
+template <typename A, typename T>
+__device__ inline A evp_div(A a, A b) {
+  if constexpr (sizeof(A) == 4) return a * __builtin_amdgcn_rcpf(b);
+  else if constexpr (sizeof(T) == 4) return island_div(a, b);
+  else return a / b;
+}
+template <typename A, typename T>
+__device__ inline A evp_sqrt(A x) {
+  if constexpr (sizeof(A) == 4) return __builtin_amdgcn_sqrtf(x);
+  else if constexpr (sizeof(T) == 4) return island_sqrt(x);
+  else return sqrt(x);
 }
 
 __device__ inline unsigned long long bits_of(double x) { return (unsigned long long)__double_as_longlong(x); }
@@ -840,14 +847,14 @@ __global__ __launch_bounds__(RI* RJ) void evp_substep_kernel(const EvpArgBlock<T
         const A ep = e11 + e22, em = e11 - e22;
         const TD dsq = (TD)ep * (TD)ep + (TD)sc.rpd * ((TD)em * (TD)em + (TD)4.0 * (TD)e12c);
         TD dC;
-        if constexpr (sizeof(A) == 4 && sizeof(TD) == 8) dC = island_sqrt(dsq);
+        if constexpr (sizeof(T) == 4 && sizeof(TD) == 8) dC = island_sqrt(dsq);
         else dC = sqrt(dsq);
         const TD dreg = dC + (TD)sc.deltaMin;
         // with t = 0 the products and the quotient by 1 + t = 1 are exact
         // identities, so skipping them keeps every bit
         const TD Pt = f_tens0 ? (TD)P : (TD)P * (TD)sc.tens1p;
         TD zeta;
-        if constexpr (sizeof(A) == 4 && sizeof(TD) == 8) zeta = island_div((TD)0.5 * Pt, dreg);
+        if constexpr (sizeof(T) == 4 && sizeof(TD) == 8) zeta = island_div((TD)0.5 * Pt, dreg);
         else zeta = ((TD)0.5 * Pt) / dreg;
         sZ[t] = (A)zeta;
         // relaxation parameter at c (veris/evp_solver.py:65-73 when adaptive)
@@ -875,7 +882,7 @@ __global__ __launch_bounds__(RI* RJ) void evp_substep_kernel(const EvpArgBlock<T
           const A D = (A)(fc < (A)0 ? sc.cwS : sc.cwN) * (A)sc.rhoSea;
           const A su = sDU[t] + sDU[t + RJ], sv = sDV[t] + sDV[t + 1];
           const A w = (A)0.25 * (su * su + sv * sv);
-          const A cd = ((D * D) * w > (A)sc.cDragMin2) ? D * evp_sqrt(w) : (A)sc.cDragMin;
+          const A cd = ((D * D) * w > (A)sc.cDragMin2) ? D * evp_sqrt<A, T>(w) : (A)sc.cDragMin;
           sCD[t] = cd * Mc;
           if (f_basal) {
             const int cip = next_axis<K>(gi, g.NX, g.nx, g.fill_x) * g.NY + gj;
@@ -981,9 +988,9 @@ __global__ __launch_bounds__(RI* RJ) void evp_substep_kernel(const EvpArgBlock<T
           denV = denV + ((mV * (A)sc.sdCoeff) * sdmV) / (spV + (A)sc.sdU0);
         }
         const A un =
-            evp_div(MUc * (((bU * mU) * u + (mU * (A)sc.rdt) * uNm1 + fx) + divx), denU);
+            evp_div<A, T>(MUc * (((bU * mU) * u + (mU * (A)sc.rdt) * uNm1 + fx) + divx), denU);
         const A vn =
-            evp_div(MVc * (((bV * mV) * v + (mV * (A)sc.rdt) * vNm1 + fy) + divy), denV);
+            evp_div<A, T>(MVc * (((bV * mV) * v + (mV * (A)sc.rdt) * vNm1 + fy) + divy), denV);
         u = un;  // other threads read sU/sV, never these registers
         v = vn;
       }
